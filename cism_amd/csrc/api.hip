@@ -1,0 +1,719 @@
+// api.hip -- context management, host<->device layout conversion and the
+// C-ABI entry points of libcism_gpu.so (declared in include/cism_gpu.h).
+#include "internal.h"
+
+#include <chrono>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static thread_local char g_err[512] = "";
+
+void cism_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *cism_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// layout conversion: Fortran (27, nn) block <-> 27 planes of npad
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_block_to_planes(long long nn, long long np,
+                                                         const double *__restrict__ src,
+                                                         double *__restrict__ dst) {
+    // 27 x 256 tile through LDS: reads and writes are both unit-stride
+    __shared__ double t[27 * 256 + 1];
+    const long long q0 = (long long)blockIdx.x * 256;
+    const long long cnt = (q0 + 256 <= nn ? 256 : nn - q0) * 27;
+    for (int s = threadIdx.x; s < cnt; s += 256) t[s] = src[q0 * 27 + s];
+    __syncthreads();
+    const long long q = q0 + threadIdx.x;
+    if (q < nn)
+        for (int m = 0; m < 27; ++m) dst[m * np + q] = t[threadIdx.x * 27 + m];
+}
+
+__global__ __launch_bounds__(256) void k_planes_to_block(long long nn, long long np,
+                                                         const double *__restrict__ src,
+                                                         double *__restrict__ dst) {
+    __shared__ double t[27 * 256 + 1];
+    const long long q0 = (long long)blockIdx.x * 256;
+    const long long q = q0 + threadIdx.x;
+    if (q < nn)
+        for (int m = 0; m < 27; ++m) t[threadIdx.x * 27 + m] = src[m * np + q];
+    __syncthreads();
+    const long long cnt = (q0 + 256 <= nn ? 256 : nn - q0) * 27;
+    for (int s = threadIdx.x; s < cnt; s += 256) dst[q0 * 27 + s] = t[s];
+}
+
+int ensure_stage(cism_ctx *c) {
+    if (!c->stage) HIP_TRY(hipMalloc(&c->stage, sizeof(double) * 27 * (size_t)c->g.nn));
+    return CISM_OK;
+}
+
+int ensure_ke(cism_ctx *c) {
+    if (!c->Ke) {
+        hipError_t e = hipMalloc(&c->Ke, sizeof(double) * CISM_KE_ENTRIES * (size_t)c->g.nepad);
+        if (e != hipSuccess) {
+            cism_set_error("element scratch allocation failed: %s", hipGetErrorString(e));
+            return CISM_ERR_NOMEM;
+        }
+    }
+    return CISM_OK;
+}
+
+int upload_matrix(cism_ctx *c, const double *Auu, const double *Auv, const double *Avu,
+                  const double *Avv) {
+    CISM_TRY(ensure_stage(c));
+    const double *h[4] = {Auu, Auv, Avu, Avv};
+    for (int b = 0; b < 4; ++b) {
+        HIP_TRY(hipMemcpyAsync(c->stage, h[b], sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
+                               c->stream));
+        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
+                           c->g.nn, c->g.npad, c->stage, c->A + (size_t)b * 27 * c->g.npad);
+        HIP_TRY(hipGetLastError());
+    }
+    return CISM_OK;
+}
+
+int download_matrix(cism_ctx *c, double *Auu, double *Auv, double *Avu, double *Avv) {
+    CISM_TRY(ensure_stage(c));
+    double *h[4] = {Auu, Auv, Avu, Avv};
+    for (int b = 0; b < 4; ++b) {
+        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
+                           c->g.nn, c->g.npad, c->A + (size_t)b * 27 * c->g.npad, c->stage);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(h[b], c->stage, sizeof(double) * 27 * c->g.nn, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+int upload_vec(cism_ctx *c, double *dst, const double *hu, const double *hv) {
+    HIP_TRY(hipMemcpyAsync(dst, hu, sizeof(double) * c->g.nn, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dst + c->g.npad, hv, sizeof(double) * c->g.nn, hipMemcpyHostToDevice,
+                           c->stream));
+    return CISM_OK;
+}
+
+int download_vec(cism_ctx *c, const double *src, double *hu, double *hv) {
+    HIP_TRY(hipMemcpyAsync(hu, src, sizeof(double) * c->g.nn, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hv, src + c->g.npad, sizeof(double) * c->g.nn, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+template <typename T>
+static int up(cism_ctx *c, T *dst, const T *src, size_t n) {
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+    return CISM_OK;
+}
+template <typename T>
+static int down(cism_ctx *c, T *dst, const T *src, size_t n) {
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static int check_indxA(const int32_t *indxA) {
+    if (!indxA) return CISM_OK;
+    for (int kA = -1; kA <= 1; ++kA)
+        for (int jA = -1; jA <= 1; ++jA)
+            for (int iA = -1; iA <= 1; ++iA)
+                if (indxA[(iA + 1) + 3 * (jA + 1) + 9 * (kA + 1)] != slot_of(iA, jA, kA) + 1) {
+                    cism_set_error("indxA is not the standard 27-point map");
+                    return CISM_ERR_ARG;
+                }
+    return CISM_OK;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n) {
+    hipError_t e = hipMalloc(p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) {
+        cism_set_error("device allocation of %zu bytes failed: %s", sizeof(T) * n, hipGetErrorString(e));
+        return CISM_ERR_NOMEM;
+    }
+    HIP_TRY(hipMemset(*p, 0, sizeof(T) * (n ? n : 1)));
+    return CISM_OK;
+}
+
+extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, int device,
+                               cism_ctx **out) {
+    if (!out) return CISM_ERR_ARG;
+    *out = nullptr;
+    if (nhalo != CISM_NHALO || nx < 7 || ny < 7 || nz < 2) {
+        cism_set_error("invalid grid nx=%d ny=%d nz=%d nhalo=%d (need nhalo=2, nx,ny>=7, nz>=2)", nx,
+                       ny, nz, nhalo);
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(device));
+    cism_ctx *c = (cism_ctx *)calloc(1, sizeof(cism_ctx));
+    if (!c) return CISM_ERR_NOMEM;
+    GridDev &g = c->g;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.sx = nx - 1; g.sy = ny - 1;
+    g.periodic = periodic ? 1 : 0;
+    // parallel_slap.F90:661-673, 0-based
+    g.ilo = (periodic ? 3 : 2) - 1;
+    g.jlo = (periodic ? 3 : 2) - 1;
+    g.ihi = (nx - 2) - 1;
+    g.jhi = (ny - 2) - 1;
+    g.nn = (long long)nz * g.sx * g.sy;
+    g.npad = (g.nn + 255) / 256 * 256;
+    g.ne = (long long)(nz - 1) * nx * ny;
+    g.nepad = (g.ne + 255) / 256 * 256;
+    c->device = device;
+    c->maxiters = 200;
+    c->tolerance = 1.0e-8;
+    int st = CISM_OK;
+    auto fail = [&](int s) { cism_ctx_destroy(c); return s; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(CISM_ERR_HIP);
+    const size_t v2 = 2 * (size_t)g.npad, ncol = (size_t)g.sx * g.sy;
+    if ((st = dalloc(&c->A, 108 * (size_t)g.npad))) return fail(st);
+    double **vecs[] = {&c->b, &c->x, &c->r, &c->d, &c->z, &c->q, &c->Az, &c->t, &c->load, &c->usav};
+    for (double **v : vecs)
+        if ((st = dalloc(v, v2))) return fail(st);
+    if ((st = dalloc(&c->av, ncol)) || (st = dalloc(&c->ac, (size_t)nx * ny)) ||
+        (st = dalloc(&c->umask, (size_t)g.nn)) || (st = dalloc(&c->vmask, (size_t)g.nn)))
+        return fail(st);
+    double **geo[] = {&c->xV, &c->yV, &c->susrf, &c->sthck, &c->beta, &c->dsdx, &c->dsdy};
+    for (double **v : geo)
+        if ((st = dalloc(v, ncol))) return fail(st);
+    if ((st = dalloc(&c->sigma, (size_t)nz)) || (st = dalloc(&c->wsig, (size_t)nz + 1)) ||
+        (st = dalloc(&c->flwafact, (size_t)g.nepad)) || (st = dalloc(&c->efvs, (size_t)g.nepad)))
+        return fail(st);
+    if ((st = dalloc(&c->partials, (size_t)CISM_RED_BLOCKS * 8))) return fail(st);
+    if ((st = dalloc(&c->sc, 1))) return fail(st);
+    if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
+        return fail(CISM_ERR_HIP);
+    *out = c;
+    return CISM_OK;
+}
+
+extern "C" int cism_ctx_destroy(cism_ctx *c) {
+    if (!c) return CISM_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav,
+                    c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->av, c->ac, c->umask, c->vmask,
+                    c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
+                    c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    free(c);
+    return CISM_OK;
+}
+
+extern "C" int cism_ctx_set_linear_solver(cism_ctx *c, int maxiters, double tolerance) {
+    if (!c || maxiters < 1 || !(tolerance > 0)) return CISM_ERR_ARG;
+    c->maxiters = maxiters;
+    c->tolerance = tolerance;
+    return CISM_OK;
+}
+
+#define CTX_BEGIN(c)                                \
+    do {                                            \
+        if (!(c)) {                                 \
+            cism_set_error("null context");         \
+            return CISM_ERR_ARG;                    \
+        }                                           \
+        HIP_TRY(hipSetDevice((c)->device));         \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// stateless entry points (host arrays in and out)
+// ---------------------------------------------------------------------------
+extern "C" int cism_matvec_multiply_structured_3d(cism_ctx *c, const int32_t *indxA,
+                                                  const int32_t *active_vertex,
+                                                  const double *Auu, const double *Auv,
+                                                  const double *Avu, const double *Avv,
+                                                  const double *xu, const double *xv,
+                                                  double *yu, double *yv) {
+    CTX_BEGIN(c);
+    CISM_TRY(check_indxA(indxA));
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(upload_vec(c, c->x, xu, xv));
+    CISM_TRY(launch_matvec64(c, c->A, c->x, c->q));
+    return download_vec(c, c->q, yu, yv);
+}
+
+static int pcg_entry(cism_ctx *c, int chrongear, const int32_t *indxA, const int32_t *active_vertex,
+                     const double *Auu, const double *Auv, const double *Avu, const double *Avv,
+                     const double *bu, const double *bv, double *xu, double *xv, int precond,
+                     double *err, int *niters) {
+    CTX_BEGIN(c);
+    CISM_TRY(check_indxA(indxA));
+    if (precond < 0 || precond > 2) {
+        cism_set_error("precond must be 0, 1 or 2 (got %d)", precond);
+        return CISM_ERR_ARG;
+    }
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(upload_vec(c, c->b, bu, bv));
+    CISM_TRY(upload_vec(c, c->x, xu, xv));
+    double e = 0.0;
+    int n = 0;
+    int st = solver_pcg64(c, chrongear, precond, c->maxiters, c->tolerance, &e, &n);
+    if (st) return st;
+    if (err) *err = e;
+    if (niters) *niters = n;
+    return download_vec(c, c->x, xu, xv);
+}
+
+extern "C" int cism_pcg_solver_standard_3d(cism_ctx *c, const int32_t *indxA,
+                                           const int32_t *active_vertex, const double *Auu,
+                                           const double *Auv, const double *Avu, const double *Avv,
+                                           const double *bu, const double *bv, double *xu, double *xv,
+                                           int precond, double *err, int *niters) {
+    return pcg_entry(c, 0, indxA, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, err,
+                     niters);
+}
+
+extern "C" int cism_pcg_solver_chrongear_3d(cism_ctx *c, const int32_t *indxA,
+                                            const int32_t *active_vertex, const double *Auu,
+                                            const double *Auv, const double *Avu, const double *Avv,
+                                            const double *bu, const double *bv, double *xu,
+                                            double *xv, int precond, double *err, int *niters) {
+    return pcg_entry(c, 1, indxA, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, err,
+                     niters);
+}
+
+static int check_options(int whichapprox, int whichefvs) {
+    if (whichapprox < 0 || whichapprox > 2 || whichefvs < 0 || whichefvs > 2) {
+        cism_set_error("unsupported whichapprox=%d / whichefvs=%d for the 3D assembly", whichapprox,
+                       whichefvs);
+        return CISM_ERR_UNSUPPORTED;
+    }
+    return CISM_OK;
+}
+
+static int upload_geometry(cism_ctx *c, const double *sigma, const int32_t *active_cell,
+                           const double *xVertex, const double *yVertex, const double *stagusrf,
+                           const double *stagthck, const double *flwafact) {
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(up(c, c->sigma, sigma, (size_t)g.nz));
+    CISM_TRY(up(c, c->ac, active_cell, (size_t)g.nx * g.ny));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(up(c, c->susrf, stagusrf, ncol));
+    CISM_TRY(up(c, c->sthck, stagthck, ncol));
+    CISM_TRY(up(c, c->flwafact, flwafact, (size_t)g.ne));
+    return CISM_OK;
+}
+
+static int check_flags(cism_ctx *c) {
+    CISM_TRY(sync_scalars(c));
+    if (c->sc_host->singular_flag) {
+        cism_set_error("Jacobian matrix is singular");
+        return CISM_ERR_SINGULAR;
+    }
+    return CISM_OK;
+}
+
+extern "C" int cism_assemble_stiffness_matrix_3d(cism_ctx *c, const double *sigma,
+                                                 const int32_t *active_cell, const double *xVertex,
+                                                 const double *yVertex, const double *uvel,
+                                                 const double *vvel, const double *stagusrf,
+                                                 const double *stagthck, const double *flwafact,
+                                                 int whichapprox, double *efvs, int whichefvs,
+                                                 double efvs_constant, double *Auu, double *Auv,
+                                                 double *Avu, double *Avv) {
+    CTX_BEGIN(c);
+    CISM_TRY(check_options(whichapprox, whichefvs));
+    CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    CISM_TRY(assemble_device(c, whichapprox, whichefvs, efvs_constant, 0, 0.0, 0.0));
+    CISM_TRY(check_flags(c));
+    CISM_TRY(down(c, efvs, c->efvs, (size_t)c->g.ne));
+    return download_matrix(c, Auu, Auv, Avu, Avv);
+}
+
+__global__ void k_basal_local(GridDev g, const int32_t *__restrict__ av, const double *__restrict__ beta,
+                              double scale, double *__restrict__ A) {
+    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long long)g.sx * g.sy || !av[col]) return;
+    const long long q = (g.nz - 1) + (long long)g.nz * col;
+    A[(0 * 27 + CISM_DIAG) * g.npad + q] = A[(0 * 27 + CISM_DIAG) * g.npad + q] + scale * beta[col];
+    A[(3 * 27 + CISM_DIAG) * g.npad + q] = A[(3 * 27 + CISM_DIAG) * g.npad + q] + scale * beta[col];
+}
+
+extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
+                                        const int32_t *active_vertex, const double *beta,
+                                        int whichassemble_beta, double *Auu, double *Avv) {
+    CTX_BEGIN(c);
+    if (whichassemble_beta != 1) {
+        cism_set_error("only the local basal assembly (which_ho_assemble_beta = 1) is implemented");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const size_t ncol = (size_t)c->g.sx * c->g.sy;
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta, ncol));
+    // only the two diagonal blocks are needed; stage them through Auu/Avv planes
+    CISM_TRY(ensure_stage(c));
+    const double *h[2] = {Auu, Avv};
+    const int blk[2] = {0, 3};
+    for (int s = 0; s < 2; ++s) {
+        HIP_TRY(hipMemcpyAsync(c->stage, h[s], sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
+                               c->stream));
+        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
+                           c->g.nn, c->g.npad, c->stage, c->A + (size_t)blk[s] * 27 * c->g.npad);
+    }
+    hipLaunchKernelGGL(k_basal_local, dim3(nblocks_for((long long)ncol)), dim3(256), 0, c->stream,
+                       c->g, c->av, c->beta, dx * dy / CISM_VOL0, c->A);
+    HIP_TRY(hipGetLastError());
+    double *ho[2] = {Auu, Avv};
+    for (int s = 0; s < 2; ++s) {
+        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
+                           c->g.nn, c->g.npad, c->A + (size_t)blk[s] * 27 * c->g.npad, c->stage);
+        HIP_TRY(hipMemcpyAsync(ho[s], c->stage, sizeof(double) * 27 * c->g.nn, hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return CISM_OK;
+}
+
+extern "C" int cism_dirichlet_boundary_conditions_3d(cism_ctx *c, const int32_t *active_vertex,
+                                                     const int32_t *umask_dirichlet,
+                                                     const int32_t *vmask_dirichlet,
+                                                     const double *uvel, const double *vvel,
+                                                     double *Auu, double *Auv, double *Avu,
+                                                     double *Avv, double *bu, double *bv) {
+    CTX_BEGIN(c);
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)c->g.nn));
+    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)c->g.nn));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    CISM_TRY(upload_vec(c, c->b, bu, bv));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(dirichlet_device(c));
+    CISM_TRY(download_matrix(c, Auu, Auv, Avu, Avv));
+    return download_vec(c, c->b, bu, bv);
+}
+
+extern "C" int cism_check_symmetry_assembled_matrix_3d(cism_ctx *c, const int32_t *active_vertex,
+                                                       double *Auu, double *Auv, double *Avu,
+                                                       double *Avv, int64_t *nwarn) {
+    CTX_BEGIN(c);
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(symmetry_device(c));
+    CISM_TRY(sync_scalars(c));
+    if (nwarn) *nwarn = (int64_t)c->sc_host->sums[0];
+    if (c->sc_host->sym_count > (unsigned)c->sym_cap) {
+        cism_set_error("too many asymmetric pairs to repair (%u)", c->sc_host->sym_count);
+        return CISM_ERR_UNSUPPORTED;
+    }
+    return download_matrix(c, Auu, Auv, Avu, Avv);
+}
+
+extern "C" int cism_count_nonzeros_3d(cism_ctx *c, const double *Auu, const double *Auv,
+                                      const double *Avu, const double *Avv,
+                                      const int32_t *active_vertex, int64_t *nNonzeros) {
+    CTX_BEGIN(c);
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(count_nonzeros_device(c));
+    CISM_TRY(sync_scalars(c));
+    if (nNonzeros) *nNonzeros = (int64_t)llround(c->sc_host->sums[0]);
+    return CISM_OK;
+}
+
+extern "C" int cism_compute_residual_vector_3d(cism_ctx *c, const int32_t *active_vertex,
+                                               const double *Auu, const double *Auv,
+                                               const double *Avu, const double *Avv,
+                                               const double *bu, const double *bv,
+                                               const double *uvel, const double *vvel,
+                                               double *resid_u, double *resid_v, double *L2_norm,
+                                               double *L2_norm_relative) {
+    CTX_BEGIN(c);
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    CISM_TRY(upload_vec(c, c->b, bu, bv));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    CISM_TRY(residual_vector_device(c, c->t, nullptr));
+    CISM_TRY(sync_scalars(c));
+    const double L2 = sqrt(c->sc_host->sums[0]), rhs = sqrt(c->sc_host->sums[1]);
+    if (L2_norm) *L2_norm = L2;
+    if (L2_norm_relative) *L2_norm_relative = rhs > 0.0 ? L2 / rhs : 0.0;
+    return download_vec(c, c->t, resid_u, resid_v);
+}
+
+extern "C" int cism_compute_residual_velocity_3d(cism_ctx *c, int whichresid, const double *uvel,
+                                                 const double *vvel, const double *usav,
+                                                 const double *vsav, double *resid_velo) {
+    CTX_BEGIN(c);
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    CISM_TRY(upload_vec(c, c->usav, usav, vsav));
+    CISM_TRY(residual_velocity_device(c, whichresid));
+    CISM_TRY(sync_scalars(c));
+    double r = c->sc_host->sums[0];
+    if (whichresid == 2) r = c->sc_host->sums[1] > 0 ? r / c->sc_host->sums[1] : 0.0;
+    if (resid_velo) *resid_velo = r;
+    return CISM_OK;
+}
+
+extern "C" int cism_load_vector_gravity(cism_ctx *c, const double *sigma, const double *stagwbndsigma,
+                                        double dx, double dy, const int32_t *active_cell,
+                                        const int32_t *active_vertex, const double *xVertex,
+                                        const double *yVertex, const double *stagusrf,
+                                        const double *stagthck, const double *dusrf_dx,
+                                        const double *dusrf_dy, int whichassemble_taud,
+                                        double *loadu, double *loadv) {
+    CTX_BEGIN(c);
+    const size_t ncol = (size_t)c->g.sx * c->g.sy;
+    CISM_TRY(up(c, c->sigma, sigma, (size_t)c->g.nz));
+    CISM_TRY(up(c, c->wsig, stagwbndsigma, (size_t)c->g.nz + 1));
+    CISM_TRY(up(c, c->ac, active_cell, (size_t)c->g.nx * c->g.ny));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(up(c, c->susrf, stagusrf, ncol));
+    CISM_TRY(up(c, c->sthck, stagthck, ncol));
+    CISM_TRY(up(c, c->dsdx, dusrf_dx, ncol));
+    CISM_TRY(up(c, c->dsdy, dusrf_dy, ncol));
+    CISM_TRY(upload_vec(c, c->load, loadu, loadv));
+    CISM_TRY(load_gravity_device(c, whichassemble_taud, dx, dy));
+    return download_vec(c, c->load, loadu, loadv);
+}
+
+extern "C" int cism_staggered_parallel_halo(cism_ctx *c, int lead, double *a) {
+    CTX_BEGIN(c);
+    const GridDev &g = c->g;
+    if (lead == g.nz) {
+        // a single (nz, nx-1, ny-1) array: use the u plane of the temp vector
+        CISM_TRY(up(c, c->t, a, (size_t)g.nn));
+        CISM_TRY(launch_halo_vec(c, c->t));   // the v plane is ignored
+        CISM_TRY(down(c, a, c->t, (size_t)g.nn));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return CISM_OK;
+    }
+    if (lead == 27 * g.nz) {
+        CISM_TRY(ensure_stage(c));
+        HIP_TRY(hipMemcpyAsync(c->stage, a, sizeof(double) * 27 * g.nn, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g.nn,
+                           g.npad, c->stage, c->A);
+        CISM_TRY(launch_halo_n(c, c->A, 27));
+        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g.nn,
+                           g.npad, c->A, c->stage);
+        HIP_TRY(hipMemcpyAsync(a, c->stage, sizeof(double) * 27 * g.nn, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return CISM_OK;
+    }
+    cism_set_error("halo: lead must be nz or 27*nz (got %d)", lead);
+    return CISM_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident Picard loop (glissade_velo_higher.F90:2003-2016, 2206-3727)
+// ---------------------------------------------------------------------------
+extern "C" void cism_solve_opts_default(cism_solve_opts *o) {
+    memset(o, 0, sizeof *o);
+    o->whichbabc = 4;
+    o->whichefvs = 2;
+    o->whichresid = 3;
+    o->whichsparse = 3;
+    o->whichprecond = 2;
+    o->whichapprox = 2;
+    o->maxiter_nonlinear = 100;
+    o->precision = 0;
+    o->efvs_constant = 2336041.42829;
+    o->dx = o->dy = 1000.0;
+    o->inner_rtol = 1.0e-3;
+    o->adaptive_gamma = 0.05;
+}
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, const double *sigma,
+                                         const int32_t *active_cell, const int32_t *active_vertex,
+                                         const double *xVertex, const double *yVertex,
+                                         const double *stagusrf, const double *stagthck,
+                                         const double *flwafact, const double *beta_internal,
+                                         const int32_t *umask_dirichlet,
+                                         const int32_t *vmask_dirichlet, const double *loadu,
+                                         const double *loadv, double *uvel, double *vvel,
+                                         double *efvs, double *bu, double *bv, double *resid_u,
+                                         double *resid_v, cism_solve_stats *stats) {
+    CTX_BEGIN(c);
+    if (!o) return CISM_ERR_ARG;
+    CISM_TRY(check_options(o->whichapprox, o->whichefvs));
+    if (o->whichsparse != 2 && o->whichsparse != 3) {
+        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const double t0 = now_s();
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta_internal, ncol));
+    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)g.nn));
+    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)g.nn));
+    CISM_TRY(upload_vec(c, c->load, loadu, loadv));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+
+    double crit = 1.0e10, target = 1.0e-12;
+    double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
+    int counter = 0, lin_total = 0, st = CISM_OK;
+    double t_asm = 0, t_lin = 0;
+    bool zero_exit = false;
+    cism_solve_stats S;
+    memset(&S, 0, sizeof S);
+    while (crit >= target && counter < o->maxiter_nonlinear) {
+        counter++;
+        HIP_TRY(hipMemcpyAsync(c->usav, c->x, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
+                               c->stream));
+        double ta = now_s();
+        CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                 o->whichbabc != 6, o->dx, o->dy));
+        HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
+                               c->stream));
+        CISM_TRY(dirichlet_device(c));
+        CISM_TRY(launch_halo_mat(c, c->A));
+        CISM_TRY(launch_halo_vec(c, c->b));
+        CISM_TRY(symmetry_device(c));
+        CISM_TRY(count_nonzeros_device(c));
+        CISM_TRY(sync_scalars(c));
+        if (c->sc_host->singular_flag) {
+            cism_set_error("Jacobian matrix is singular");
+            return CISM_ERR_SINGULAR;
+        }
+        const double nnz = c->sc_host->sums[0];
+        t_asm += now_s() - ta;
+        if (nnz == 0.0) {
+            zero_exit = true;
+            L2 = 0.0;
+            break;
+        }
+        CISM_TRY(residual_vector_device(c, c->t, nullptr));
+        CISM_TRY(sync_scalars(c));
+        L2 = sqrt(c->sc_host->sums[0]);
+        const double rhs = sqrt(c->sc_host->sums[1]);
+        L2rel = rhs > 0.0 ? L2 / rhs : 0.0;
+        double tl = now_s(), err = 0.0;
+        int niters = 0;
+        double lin_tol = c->tolerance;
+        if (o->adaptive_linear) {
+            // inexact Picard: no tighter than gamma times the current relative residual
+            const double gam = o->adaptive_gamma > 0 ? o->adaptive_gamma : 0.05;
+            lin_tol = fmin(1.0e-2, fmax(c->tolerance, gam * L2rel));
+        }
+        if (o->precision == 1) {
+            int n64 = 0, n32 = 0;
+            CISM_TRY(convert_a32(c));
+            st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
+                              &n64, &n32);
+            S.fp64_matvecs += n64;
+            S.fp32_matvecs += n32;
+            S.refinement_steps += n64;
+        } else {
+            st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err,
+                              &niters);
+        }
+        if (st) return st;
+        t_lin += now_s() - tl;
+        lin_total += niters;
+        CISM_TRY(launch_halo_vec(c, c->x));
+        CISM_TRY(residual_velocity_device(c, o->whichresid));
+        CISM_TRY(sync_scalars(c));
+        resid_velo = c->sc_host->sums[0];
+        if (o->whichresid == 2) resid_velo = c->sc_host->sums[1] > 0 ? resid_velo / c->sc_host->sums[1] : 0.0;
+        if (o->whichresid == 3) { crit = L2; target = 1.0e-4; }
+        else if (o->whichresid == 4) { crit = L2rel; target = 1.0e-7; }
+        else { crit = resid_velo; target = 1.0e-4; }
+        if (o->target_override > 0) target = o->target_override;
+        if (o->stop_rel_err > 0 && c->xref) {
+            double e = 0.0;
+            CISM_TRY(rel_err_device(c, &e));
+            S.final_rel_err = e;
+            if (e <= o->stop_rel_err) break;
+        }
+    }
+    if (zero_exit) {
+        HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
+        HIP_TRY(hipMemsetAsync(c->b, 0, sizeof(double) * 2 * g.npad, c->stream));
+        HIP_TRY(hipMemsetAsync(c->t, 0, sizeof(double) * 2 * g.npad, c->stream));
+    }
+    CISM_TRY(download_vec(c, c->x, uvel, vvel));
+    if (efvs) CISM_TRY(down(c, efvs, c->efvs, (size_t)g.ne));
+    if (bu && bv) CISM_TRY(download_vec(c, c->b, bu, bv));
+    if (resid_u && resid_v) CISM_TRY(download_vec(c, c->t, resid_u, resid_v));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->xref) {
+        double e = 0.0;
+        CISM_TRY(rel_err_device(c, &e));
+        S.final_rel_err = e;
+    } else {
+        S.final_rel_err = -1.0;
+    }
+    S.picard_iters = counter;
+    S.linear_iters = lin_total;
+    S.final_L2_norm = L2;
+    S.final_resid_velo = resid_velo;
+    S.seconds_assembly = t_asm;
+    S.seconds_linear = t_lin;
+    S.seconds_total = now_s() - t0;
+    S.status = st;
+    if (stats) *stats = S;
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reference velocity and kernel timing (measurement support)
+// ---------------------------------------------------------------------------
+extern "C" int cism_set_reference_velocity(cism_ctx *c, const double *uref, const double *vref) {
+    CTX_BEGIN(c);
+    if (!uref || !vref) {
+        if (c->xref) HIP_TRY(hipFree(c->xref));
+        c->xref = nullptr;
+        return CISM_OK;
+    }
+    if (!c->xref) CISM_TRY(dalloc(&c->xref, 2 * (size_t)c->g.npad));
+    CISM_TRY(upload_vec(c, c->xref, uref, vref));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per_launch) {
+    CTX_BEGIN(c);
+    if (reps < 1 || which < 0 || which > 2) return CISM_ERR_ARG;
+    if (which == 1 && !c->A32) {
+        cism_set_error("no FP32 matrix yet: run a mixed-precision solve first");
+        return CISM_ERR_ARG;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    // one untimed launch to settle clocks and caches
+    if (which == 2) CISM_TRY(assemble_device(c, 2, 2, 0.0, 0, 0.0, 0.0));
+    else CISM_TRY(launch_cg_matvec(c, which));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (which == 2) CISM_TRY(assemble_device(c, 2, 2, 0.0, 0, 0.0, 0.0));
+        else CISM_TRY(launch_cg_matvec(c, which));
+    }
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    return CISM_OK;
+}

@@ -1,0 +1,670 @@
+// assemble.hip -- finite-element assembly of the Blatter-Pattyn blocks and the
+// operations applied to the assembled system before the linear solve
+// (glissade_velo_higher.F90).
+//
+// Assembly is split in two atomic-free kernels:
+//  1. k_element: one workgroup = 32 hexahedral elements x 8 quadrature points.
+//     Each lane evaluates one quadrature point (Jacobian, inverse, basis
+//     derivatives, Glen-law viscosity) into LDS; then all lanes form the 136
+//     distinct entries of the element's 8x8 blocks (Kuu, Kvv symmetric; Kvu =
+//     Kuv^T, both exactly, since every product is formed from the same factors)
+//     and store them element-fastest to a scratch array.
+//  2. k_gather: one lane per node sums, slot by slot, the entries of the (up to
+//     8) elements that share the node, in the reference's element loop order
+//     (j, i, k ascending), so every coefficient is the reference's sum in the
+//     reference's order; the local basal friction term is added last.
+#include "internal.h"
+
+#include <math.h>
+
+// node n of a hexahedron: horizontal offsets (ix, jy) in {0,1} from the SW
+// vertex, layer kl = 0 upper (level k) / 1 lower (level k+1)   (velo_higher:453-486)
+__constant__ int c_node_ix[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int c_node_jy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__constant__ int c_node_kl[8] = {1, 1, 1, 1, 0, 0, 0, 0};
+
+__host__ __device__ constexpr int node_id(int ix, int jy, int kl) {
+    return kl == 1 ? (jy == 0 ? (ix == 0 ? 0 : 1) : (ix == 0 ? 3 : 2))
+                   : (jy == 0 ? (ix == 0 ? 4 : 5) : (ix == 0 ? 7 : 6));
+}
+// index of (a, b), a <= b, in the packed upper triangle of an 8x8 matrix
+__host__ __device__ constexpr int tri_index(int a, int b) {
+    return a <= b ? (a * (15 - a)) / 2 + b : (b * (15 - b)) / 2 + a;
+}
+
+#define EL_PER_WG 32
+
+struct QPLds {
+    double dx[EL_PER_WG][8][8], dy[EL_PER_WG][8][8], dz[EL_PER_WG][8][8];   // [el][p][n]
+    double f[EL_PER_WG][8], ev[EL_PER_WG][8];
+    double nodes[EL_PER_WG][8][5];                                          // x y z u v
+};
+
+__global__ __launch_bounds__(256) void k_element(GridDev g, const double *__restrict__ sigma,
+                                                 const int32_t *__restrict__ ac,
+                                                 const double *__restrict__ xV,
+                                                 const double *__restrict__ yV,
+                                                 const double *__restrict__ uvel,
+                                                 const double *__restrict__ susrf,
+                                                 const double *__restrict__ sthck,
+                                                 const double *__restrict__ flwafact,
+                                                 int whichapprox, int whichefvs,
+                                                 double efvs_constant, double *__restrict__ Ke,
+                                                 double *__restrict__ efvs,
+                                                 DevScal *__restrict__ sc) {
+    __shared__ QPLds L;
+    __shared__ int valid[EL_PER_WG];
+    const int el = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const long long e = (long long)blockIdx.x * EL_PER_WG + el;
+    const int nzm = g.nz - 1;
+    int k = 0, ic = 0, jc = 0;
+    bool ok = e < g.ne;
+    if (ok) {
+        k = (int)(e % nzm) + 1;                 // 1-based layer
+        const long long cell = e / nzm;
+        ic = (int)(cell % g.nx) + 1;            // 1-based cell
+        jc = (int)(cell / g.nx) + 1;
+        ok = ic >= CISM_NHALO + 1 && ic <= g.nx - CISM_NHALO + 1 && jc >= CISM_NHALO + 1 &&
+             jc <= g.ny - CISM_NHALO + 1 && ac[(ic - 1) + (long long)g.nx * (jc - 1)] != 0;
+    }
+    if (sub == 0) valid[el] = ok;
+    // phase 0: lane (el, n) gathers node n (velo_higher:5100-5127)
+    if (ok) {
+        const int n = sub;
+        const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
+        const long long v = (in - 1) + (long long)g.sx * (jn - 1);
+        const long long qn = (kn - 1) + (long long)g.nz * v;
+        L.nodes[el][n][0] = xV[v];
+        L.nodes[el][n][1] = yV[v];
+        L.nodes[el][n][2] = susrf[v] - sigma[kn - 1] * sthck[v];
+        L.nodes[el][n][3] = uvel[qn];
+        L.nodes[el][n][4] = uvel[g.npad + qn];
+    }
+    __syncthreads();
+    // phase 1: lane (el, p) evaluates quadrature point p (velo_higher:6360-6579, 7199-7378)
+    if (ok) {
+        const int p = sub;
+        const double r = 1.0 / sqrt(3.0);
+        const double xq = (p == 0 || p == 3 || p == 4 || p == 7) ? -r : r;
+        const double yq = (p < 2 || p == 4 || p == 5) ? -r : r;
+        const double zq = p < 4 ? -r : r;
+        double a[8], b[8], cz[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
+            const double sz = c_node_kl[n] ? -1.0 : 1.0;
+            const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
+            const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
+            const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
+            a[n] = sx * (ay * az / 8.0);
+            b[n] = sy * (ax * az / 8.0);
+            cz[n] = sz * (ax * ay / 8.0);
+        }
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const double X = L.nodes[el][n][0], Y = L.nodes[el][n][1], Z = L.nodes[el][n][2];
+            J[0][0] = J[0][0] + a[n] * X;  J[0][1] = J[0][1] + a[n] * Y;  J[0][2] = J[0][2] + a[n] * Z;
+            J[1][0] = J[1][0] + b[n] * X;  J[1][1] = J[1][1] + b[n] * Y;  J[1][2] = J[1][2] + b[n] * Z;
+            J[2][0] = J[2][0] + cz[n] * X; J[2][1] = J[2][1] + cz[n] * Y; J[2][2] = J[2][2] + cz[n] * Z;
+        }
+        double cof[3][3];
+        cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        cof[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+        cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        cof[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+        cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        cof[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+        cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        cof[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+        cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
+        double Ji[3][3];
+        if (fabs(det) > 0.0) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = cof[cc][rr] / det;
+        } else {
+            atomicOr(&sc->singular_flag, 1);
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = 0.0;
+        }
+        double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const double ddx = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * cz[n];
+            const double ddy = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * cz[n];
+            const double ddz = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * cz[n];
+            L.dx[el][p][n] = ddx;
+            L.dy[el][p][n] = ddy;
+            L.dz[el][p][n] = ddz;
+            const double u = L.nodes[el][n][3], v = L.nodes[el][n][4];
+            ux = ux + ddx * u; uy = uy + ddy * u; uz = uz + ddz * u;
+            vx = vx + ddx * v; vy = vy + ddy * v; vz = vz + ddz * v;
+        }
+        const double ff = flwafact[e];
+        double ev;
+        if (whichefvs == 0) {
+            ev = efvs_constant;
+        } else if (whichefvs == 1) {
+            const double effstrain = (500.0 / CISM_SCYR) / 200.0e3 * CISM_SCYR;
+            ev = ff * pow(effstrain, (1.0 - 3.0) / 3.0);
+        } else {
+            const double emin = 1.0e-8;
+            double sq;
+            if (whichapprox == 0)
+                sq = emin * emin + 0.25 * (uz * uz + vz * vz);
+            else if (whichapprox == 1)
+                sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+            else
+                sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)))
+                   + 0.25 * (uz * uz + vz * vz);
+            ev = ff * pow(sq, ((1.0 - 3.0) / 3.0) / 2);
+        }
+        L.ev[el][p] = ev;
+        L.f[el][p] = ev * 1.0 * det / CISM_VOL0;
+    }
+    __syncthreads();
+    // element efvs = mean over quadrature points (velo_higher:5167-5172)
+    if (threadIdx.x < EL_PER_WG) {
+        const long long ee = (long long)blockIdx.x * EL_PER_WG + threadIdx.x;
+        if (ee < g.ne) {
+            double s = 0.0;
+            if (valid[threadIdx.x])
+                for (int p = 0; p < 8; ++p) s = s + L.ev[threadIdx.x][p];
+            efvs[ee] = valid[threadIdx.x] ? s / 8 : 0.0;
+        }
+    }
+    // phase 2: entries, element-fastest (compute_element_matrix, velo_higher:7960-7982)
+    for (int t = threadIdx.x; t < CISM_KE_ENTRIES * EL_PER_WG; t += blockDim.x) {
+        const int ent = t / EL_PER_WG, le = t % EL_PER_WG;
+        const long long ee = (long long)blockIdx.x * EL_PER_WG + le;
+        if (ee >= g.ne || !valid[le]) continue;
+        double acc = 0.0;
+        if (ent < 72) {
+            // packed upper triangle of Kuu (ent < 36) or Kvv
+            const bool isv = ent >= 36;
+            int rem = isv ? ent - 36 : ent, a = 0;
+            while (rem >= 8 - a) { rem -= 8 - a; ++a; }
+            const int bb = a + rem;
+            for (int p = 0; p < 8; ++p) {
+                const double f = L.f[le][p];
+                const double *X = L.dx[le][p], *Y = L.dy[le][p], *Z = L.dz[le][p];
+                double term;
+                if (whichapprox == 0) term = Z[a] * Z[bb];
+                else if (whichapprox == 1)
+                    term = isv ? 4.0 * Y[a] * Y[bb] + X[a] * X[bb] : 4.0 * X[a] * X[bb] + Y[a] * Y[bb];
+                else
+                    term = isv ? 4.0 * Y[a] * Y[bb] + X[a] * X[bb] + Z[a] * Z[bb]
+                               : 4.0 * X[a] * X[bb] + Y[a] * Y[bb] + Z[a] * Z[bb];
+                acc = acc + f * term;
+            }
+        } else {
+            const int idx = ent - 72, nr = idx & 7, nc = idx >> 3;   // Kuv(nr, nc)
+            if (whichapprox != 0)
+                for (int p = 0; p < 8; ++p) {
+                    const double *X = L.dx[le][p], *Y = L.dy[le][p];
+                    acc = acc + L.f[le][p] * (2.0 * X[nr] * Y[nc] + Y[nr] * X[nc]);
+                }
+        }
+        Ke[(long long)ent * g.nepad + ee] = acc;
+    }
+}
+
+// value of K_block(nr, nc) of element e from the packed scratch
+__device__ inline double ke_val(const double *__restrict__ Ke, long long nepad, int blk, int nr,
+                                int nc, long long e) {
+    int ent;
+    if (blk == 0) ent = tri_index(nr, nc);
+    else if (blk == 3) ent = 36 + tri_index(nr, nc);
+    else if (blk == 1) ent = 72 + nr + 8 * nc;   // Kuv(nr, nc)
+    else ent = 72 + nc + 8 * nr;                 // Kvu(nr, nc) = Kuv(nc, nr)
+    return Ke[(long long)ent * nepad + e];
+}
+
+// One lane per node (k, i, j): A(m, node) = sum over elements in (j, i, k) order
+// (element_to_global_matrix_3d, velo_higher:7990-8054), then the local basal
+// friction term (basal_sliding_bc LOCAL, velo_higher:8216-8232).
+__global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__restrict__ ac,
+                                                const int32_t *__restrict__ av,
+                                                const double *__restrict__ Ke,
+                                                const double *__restrict__ beta, int add_basal,
+                                                double basal_scale, double *__restrict__ A) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int k = (int)(q % g.nz) + 1;                // 1-based
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
+    // candidate elements: cell (i+dI, j+dJ), layer k-1+dK
+    long long eid[2][2][2];
+    bool ev[2][2][2];
+#pragma unroll
+    for (int dJ = 0; dJ < 2; ++dJ)
+#pragma unroll
+        for (int dI = 0; dI < 2; ++dI)
+#pragma unroll
+            for (int dK = 0; dK < 2; ++dK) {
+                const int icl = i + dI, jcl = j + dJ, ke = k - 1 + dK;
+                const bool okc = icl >= CISM_NHALO + 1 && icl <= g.nx - CISM_NHALO + 1 &&
+                                 jcl >= CISM_NHALO + 1 && jcl <= g.ny - CISM_NHALO + 1 && ke >= 1 &&
+                                 ke <= g.nz - 1;
+                ev[dJ][dI][dK] = okc && ac[(icl - 1) + (long long)g.nx * (jcl - 1)] != 0;
+                eid[dJ][dI][dK] = okc ? (ke - 1) + (long long)(g.nz - 1) * ((icl - 1) + (long long)g.nx * (jcl - 1)) : 0;
+            }
+    const long long np = g.npad;
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk) {
+#pragma unroll
+        for (int m = 0; m < 27; ++m) {
+            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+            double acc = 0.0;
+#pragma unroll
+            for (int dJ = 0; dJ < 2; ++dJ)
+#pragma unroll
+                for (int dI = 0; dI < 2; ++dI)
+#pragma unroll
+                    for (int dK = 0; dK < 2; ++dK) {
+                        const int ix = 1 - dI, jy = 1 - dJ, kl = 1 - dK;
+                        const int cx = ix + iA, cy = jy + jA, cl = kl + kA;
+                        if (cx < 0 || cx > 1 || cy < 0 || cy > 1 || cl < 0 || cl > 1) continue;
+                        if (!ev[dJ][dI][dK]) continue;
+                        acc = acc + ke_val(Ke, g.nepad, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
+                                           eid[dJ][dI][dK]);
+                    }
+            if (add_basal && m == CISM_DIAG && (blk == 0 || blk == 3) && k == g.nz && av[col])
+                acc = acc + basal_scale * beta[col];
+            A[(blk * 27 + m) * np + q] = acc;
+        }
+    }
+}
+
+int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
+                    int add_basal, double dx, double dy) {
+    CISM_TRY(ensure_ke(c));
+    const GridDev &g = c->g;
+    const int nb = (int)((g.ne + EL_PER_WG - 1) / EL_PER_WG);
+    hipLaunchKernelGGL(k_element, dim3(nb), dim3(256), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV,
+                       c->x, c->susrf, c->sthck, c->flwafact, whichapprox, whichefvs, efvs_constant,
+                       c->Ke, c->efvs, c->sc);
+    HIP_TRY(hipGetLastError());
+    const double basal_scale = dx * dy / CISM_VOL0;
+    hipLaunchKernelGGL(k_gather, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, c->ac, c->av,
+                       c->Ke, c->beta, add_basal, basal_scale, c->A);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dirichlet_boundary_conditions_3d (velo_higher:8385-8578) in gather form: one
+// lane per row N applies (a) N's own row replacement when N is a prescribed
+// node, and (b) the column eliminations of every prescribed neighbour D, in the
+// reference's order of D (j, i, k ascending, u before v), so the right-hand
+// side accumulates its corrections in the reference's order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__restrict__ av,
+                                                   const int32_t *__restrict__ um,
+                                                   const int32_t *__restrict__ vm,
+                                                   const double *__restrict__ vel,
+                                                   double *__restrict__ A, double *__restrict__ b) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const long long np = g.npad;
+    const int k = (int)(q % g.nz) + 1;
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
+    // prescribed-node loop range: i, j in nhalo .. n-nhalo+1 and active (velo_higher:8441-8443)
+    auto in_range = [&](int ii, int jj) {
+        return ii >= CISM_NHALO && ii <= g.nx - CISM_NHALO + 1 && jj >= CISM_NHALO &&
+               jj <= g.ny - CISM_NHALO + 1 && av[(ii - 1) + (long long)g.sx * (jj - 1)] != 0;
+    };
+    const bool Nu = um[q] == 1, Nv = vm[q] == 1;
+    const bool Nrange = in_range(i, j);
+    // (a) own row
+    if (Nrange && (Nu || Nv)) {
+        for (int m = 0; m < 27; ++m) {
+            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+            if (k + kA < 1 || k + kA > g.nz || i + iA < 1 || i + iA > g.sx || j + jA < 1 || j + jA > g.sy)
+                continue;
+            if (m == CISM_DIAG) continue;
+            if (Nu) { A[(0 * 27 + m) * np + q] = 0.0; A[(1 * 27 + m) * np + q] = 0.0; }
+            if (Nv) { A[(2 * 27 + m) * np + q] = 0.0; A[(3 * 27 + m) * np + q] = 0.0; }
+        }
+        if (Nu) {
+            b[q] = vel[q];
+            A[(0 * 27 + CISM_DIAG) * np + q] = 1.0;
+            A[(1 * 27 + CISM_DIAG) * np + q] = 0.0;
+            A[(2 * 27 + CISM_DIAG) * np + q] = 0.0;
+        }
+        if (Nv) {
+            b[np + q] = vel[np + q];
+            A[(1 * 27 + CISM_DIAG) * np + q] = 0.0;
+            A[(2 * 27 + CISM_DIAG) * np + q] = 0.0;
+            A[(3 * 27 + CISM_DIAG) * np + q] = 1.0;
+        }
+    }
+    if (Nu && Nv) return;   // no column elimination touches a doubly prescribed row
+    // (b) column eliminations from prescribed neighbours D = N + o
+    double bu = b[q], bv = b[np + q];
+    bool touched = false;
+    for (int oj = -1; oj <= 1; ++oj)
+        for (int oi = -1; oi <= 1; ++oi)
+            for (int ok = -1; ok <= 1; ++ok) {
+                if (oi == 0 && oj == 0 && ok == 0) continue;
+                const int di = i + oi, dj = j + oj, dk = k + ok;
+                if (dk < 1 || dk > g.nz || di < 1 || di > g.sx || dj < 1 || dj > g.sy) continue;
+                const long long qd = (dk - 1) + (long long)g.nz * ((di - 1) + (long long)g.sx * (dj - 1));
+                const bool Du = um[qd] == 1, Dv = vm[qd] == 1;
+                if (!(Du || Dv) || !in_range(di, dj)) continue;
+                const int m = slot_of(oi, oj, ok);
+                if (Du) {
+                    const double ud = vel[qd];
+                    if (!Nu) { bu = bu - A[(0 * 27 + m) * np + q] * ud; A[(0 * 27 + m) * np + q] = 0.0; }
+                    if (!Nv) { bv = bv - A[(2 * 27 + m) * np + q] * ud; A[(2 * 27 + m) * np + q] = 0.0; }
+                }
+                if (Dv) {
+                    const double vd = vel[np + qd];
+                    if (!Nu) { bu = bu - A[(1 * 27 + m) * np + q] * vd; A[(1 * 27 + m) * np + q] = 0.0; }
+                    if (!Nv) { bv = bv - A[(3 * 27 + m) * np + q] * vd; A[(3 * 27 + m) * np + q] = 0.0; }
+                }
+                touched = true;
+            }
+    if (touched) {
+        if (!Nu) b[q] = bu;
+        if (!Nv) b[np + q] = bv;
+    }
+}
+
+int dirichlet_device(cism_ctx *c) {
+    hipLaunchKernelGGL(k_dirichlet, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream, c->g, c->av,
+                       c->umask, c->vmask, c->x, c->A, c->b);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// check_symmetry_assembled_matrix_3d (velo_higher:9533-9741).  Each pair
+// (A(m,row), A'(mm,col)) is disjoint from every other pair, and the reference
+// repairs a pair iff |v1-v2| < 1e-8 |diag| from the side that visits it
+// (owned active rows), replacing both by their mean.  Phase 1 lists the pairs
+// to repair (read-only), phase 2 writes them.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sym_find(GridDev g, const int32_t *__restrict__ av,
+                                                  const double *__restrict__ A,
+                                                  long long *__restrict__ list,
+                                                  double *__restrict__ vals, long long cap,
+                                                  DevScal *__restrict__ sc,
+                                                  double *__restrict__ partials) {
+    double acc[1] = {0.0};   // number of pairs that stay asymmetric (warnings)
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long np = g.npad;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+        const int k = (int)(q % g.nz);
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        if (!is_owned(g, i, j) || !av[col]) continue;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int bd = pass == 0 ? 0 : 3, bo = pass == 0 ? 1 : 2, bt = pass == 0 ? 2 : 1;
+            const double diag = A[(bd * 27 + CISM_DIAG) * np + q];
+            for (int m = 0; m < 27; ++m) {
+                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+                if (k + kA < 0 || k + kA >= g.nz) continue;
+                const int mm = 26 - m;
+                const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+                for (int h = 0; h < 2; ++h) {
+                    const long long e1 = ((h == 0 ? bd : bo) * 27 + m) * np + q;
+                    const long long e2 = ((h == 0 ? bd : bt) * 27 + mm) * np + qc;
+                    const double v1 = A[e1], v2 = A[e2];
+                    if (v1 == v2) continue;
+                    if (fabs(v2 - v1) < 1.0e-8 * fabs(diag)) {
+                        const unsigned int s = atomicAdd(&sc->sym_count, 1u);
+                        if ((long long)s < cap) {
+                            list[2 * s] = e1;
+                            list[2 * s + 1] = e2;
+                            vals[s] = 0.5 * (v1 + v2);
+                        }
+                    } else {
+                        acc[0] += 1.0;
+                    }
+                }
+            }
+        }
+    }
+    block_reduce_store<1>(acc, partials);
+}
+
+__global__ void k_sym_apply(const long long *__restrict__ list, const double *__restrict__ vals,
+                            long long cap, const DevScal *__restrict__ sc, double *__restrict__ A) {
+    long long n = sc->sym_count;
+    if (n > cap) n = cap;
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+         s += (long long)gridDim.x * blockDim.x) {
+        A[list[2 * s]] = vals[s];
+        A[list[2 * s + 1]] = vals[s];
+    }
+}
+
+int symmetry_device(cism_ctx *c) {
+    if (!c->sym_list) {
+        c->sym_cap = 1 << 20;
+        HIP_TRY(hipMalloc(&c->sym_list, sizeof(long long) * 2 * c->sym_cap));
+        HIP_TRY(hipMalloc(&c->sym_vals, sizeof(double) * c->sym_cap));
+    }
+    HIP_TRY(hipMemsetAsync(&c->sc->sym_count, 0, sizeof(unsigned int), c->stream));
+    const int nb = red_blocks_for(c->g.nn);
+    hipLaunchKernelGGL(k_sym_find, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->sym_list,
+                       c->sym_vals, c->sym_cap, c->sc, c->partials);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_sym_apply, dim3(256), dim3(256), 0, c->stream, c->sym_list, c->sym_vals,
+                       c->sym_cap, c->sc, c->A);
+    HIP_TRY(hipGetLastError());
+    CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
+    return CISM_OK;
+}
+
+// count_nonzeros_3d (velo_higher:9355-9410) -> sums[0]
+__global__ __launch_bounds__(256) void k_nnz(GridDev g, const int32_t *__restrict__ av,
+                                             const double *__restrict__ A, double *__restrict__ partials) {
+    double acc[1] = {0.0};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        if (!is_owned(g, i, j) || !av[col]) continue;
+        int n = 0;
+        for (int pl = 0; pl < 108; ++pl) n += A[pl * g.npad + q] != 0.0;
+        acc[0] += n;
+    }
+    block_reduce_store<1>(acc, partials);
+}
+
+int count_nonzeros_device(cism_ctx *c) {
+    const int nb = red_blocks_for(c->g.nn);
+    hipLaunchKernelGGL(k_nnz, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->partials);
+    HIP_TRY(hipGetLastError());
+    return launch_reduce(c, nb, 1, RED_SUMS);
+}
+
+// compute_residual_vector_3d (velo_higher:8768-8927): r = A x - b on owned
+// active rows (0 elsewhere); sums[0] = |r|^2, sums[1] = |b|^2 over the same rows
+__global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__restrict__ av,
+                                                const double *__restrict__ A, const double *__restrict__ x,
+                                                const double *__restrict__ b, double *__restrict__ r,
+                                                double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long np = g.npad, cst = g.nz, rst = (long long)g.nz * g.sx;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+        const int k = (int)(q % g.nz);
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        double ru = 0.0, rv = 0.0;
+        if (is_owned(g, i, j) && av[col]) {
+            double su = 0.0, sv = 0.0;
+            for (int m = 0; m < 27; ++m) {
+                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+                if (k + kA < 0 || k + kA >= g.nz) continue;
+                const long long qn = q + kA + iA * cst + jA * rst;
+                const double xu = x[qn], xv = x[np + qn];
+                su = su + A[(0 * 27 + m) * np + q] * xu + A[(1 * 27 + m) * np + q] * xv;
+                sv = sv + A[(2 * 27 + m) * np + q] * xu + A[(3 * 27 + m) * np + q] * xv;
+            }
+            ru = su - b[q];
+            rv = sv - b[np + q];
+            acc[0] += ru * ru + rv * rv;
+            acc[1] += b[q] * b[q] + b[np + q] * b[np + q];
+        }
+        r[q] = ru;
+        r[np + q] = rv;
+    }
+    block_reduce_store<2>(acc, partials);
+}
+
+int residual_vector_device(cism_ctx *c, double *ru, double *rv) {
+    (void)rv;
+    const int nb = red_blocks_for(c->g.nn);
+    hipLaunchKernelGGL(k_resvec, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->x, c->b, ru,
+                       c->partials);
+    HIP_TRY(hipGetLastError());
+    return launch_reduce(c, nb, 2, RED_SUMS);
+}
+
+// compute_residual_velocity_3d (velo_higher:9111-9234) -> sums[0] (and the
+// count in sums[1] for the mean variant)
+__global__ __launch_bounds__(256) void k_resvel(GridDev g, int whichresid, const double *__restrict__ x,
+                                                const double *__restrict__ xs,
+                                                double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long np = g.npad;
+    const int kmax = (whichresid == 1 || whichresid == 2) ? g.nz - 1 : g.nz;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+        const int k = (int)(q % g.nz);
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        if (!is_owned(g, i, j) || k >= kmax) continue;
+        const double u = x[q], v = x[np + q];
+        const double speed = sqrt(u * u + v * v);
+        if (speed != 0.0) {
+            const double uo = xs[q], vo = xs[np + q];
+            const double old = sqrt(uo * uo + vo * vo);
+            const double dd = fabs((old - speed) / speed);
+            if (whichresid == 2) { acc[0] += dd; acc[1] += 1.0; }
+            else acc[0] = fmax(acc[0], dd);
+        }
+    }
+    if (whichresid == 2) block_reduce_store<2>(acc, partials);
+    else block_max_store<2>(acc, partials);
+}
+
+int residual_velocity_device(cism_ctx *c, int whichresid) {
+    const int nb = red_blocks_for(c->g.nn);
+    hipLaunchKernelGGL(k_resvel, dim3(nb), dim3(256), 0, c->stream, c->g, whichresid, c->x, c->usav,
+                       c->partials);
+    HIP_TRY(hipGetLastError());
+    return launch_reduce(c, nb, 2, whichresid == 2 ? RED_SUMS : RED_MAX);
+}
+
+// load_vector_gravity (velo_higher:4317-4532).  LOCAL: one lane per node.
+// STANDARD: one lane per node gathers, element by element in (j, i, k) order and
+// quadrature point by quadrature point, the terms the reference scatters.
+__global__ __launch_bounds__(256) void k_load_local(GridDev g, const double *__restrict__ wsig,
+                                                    double dx, double dy, const int32_t *__restrict__ av,
+                                                    const double *__restrict__ sthck,
+                                                    const double *__restrict__ dsdx,
+                                                    const double *__restrict__ dsdy,
+                                                    double *__restrict__ load) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int k = (int)(q % g.nz) + 1;
+    const long long col = q / g.nz;
+    if (!av[col]) return;
+    const double dz = sthck[col] * (wsig[k] - wsig[k - 1]);
+    const double f = CISM_RHOI * CISM_GRAV * dx * dy * dz / CISM_VOL0;
+    load[q] = load[q] - f * dsdx[col];
+    load[g.npad + q] = load[g.npad + q] - f * dsdy[col];
+}
+
+__global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *__restrict__ sigma,
+                                                       const int32_t *__restrict__ ac,
+                                                       const double *__restrict__ xV,
+                                                       const double *__restrict__ yV,
+                                                       const double *__restrict__ susrf,
+                                                       const double *__restrict__ sthck,
+                                                       const double *__restrict__ dsdx,
+                                                       const double *__restrict__ dsdy,
+                                                       double *__restrict__ load) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int k = (int)(q % g.nz) + 1;
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
+    double lu = load[q], lv = load[g.npad + q];
+    const double r = 1.0 / sqrt(3.0);
+    for (int dJ = 0; dJ < 2; ++dJ)
+        for (int dI = 0; dI < 2; ++dI)
+            for (int dK = 0; dK < 2; ++dK) {
+                const int icl = i + dI, jcl = j + dJ, ke = k - 1 + dK;
+                if (icl < CISM_NHALO + 1 || icl > g.nx - CISM_NHALO + 1 || jcl < CISM_NHALO + 1 ||
+                    jcl > g.ny - CISM_NHALO + 1 || ke < 1 || ke > g.nz - 1)
+                    continue;
+                if (!ac[(icl - 1) + (long long)g.nx * (jcl - 1)]) continue;
+                const int me = node_id(1 - dI, 1 - dJ, 1 - dK);
+                double X[8], Y[8], Z[8], SX[8], SY[8];
+                for (int n = 0; n < 8; ++n) {
+                    const int in = icl - 1 + c_node_ix[n], jn = jcl - 1 + c_node_jy[n], kn = ke + c_node_kl[n];
+                    const long long v = (in - 1) + (long long)g.sx * (jn - 1);
+                    X[n] = xV[v]; Y[n] = yV[v];
+                    Z[n] = susrf[v] - sigma[kn - 1] * sthck[v];
+                    SX[n] = dsdx[v]; SY[n] = dsdy[v];
+                }
+                for (int p = 0; p < 8; ++p) {
+                    const double xq = (p == 0 || p == 3 || p == 4 || p == 7) ? -r : r;
+                    const double yq = (p < 2 || p == 4 || p == 5) ? -r : r;
+                    const double zq = p < 4 ? -r : r;
+                    double a[8], b[8], cz[8], ph[8];
+                    for (int n = 0; n < 8; ++n) {
+                        const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
+                        const double sz = c_node_kl[n] ? -1.0 : 1.0;
+                        const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
+                        const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
+                        const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
+                        ph[n] = ax * ay * az / 8.0;
+                        a[n] = sx * (ay * az / 8.0);
+                        b[n] = sy * (ax * az / 8.0);
+                        cz[n] = sz * (ax * ay / 8.0);
+                    }
+                    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    for (int n = 0; n < 8; ++n) {
+                        J[0][0] = J[0][0] + a[n] * X[n];  J[0][1] = J[0][1] + a[n] * Y[n];  J[0][2] = J[0][2] + a[n] * Z[n];
+                        J[1][0] = J[1][0] + b[n] * X[n];  J[1][1] = J[1][1] + b[n] * Y[n];  J[1][2] = J[1][2] + b[n] * Z[n];
+                        J[2][0] = J[2][0] + cz[n] * X[n]; J[2][1] = J[2][1] + cz[n] * Y[n]; J[2][2] = J[2][2] + cz[n] * Z[n];
+                    }
+                    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+                    const double c01 = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+                    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+                    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+                    double qx = 0.0, qy = 0.0;
+                    for (int n = 0; n < 8; ++n) { qx = qx + ph[n] * SX[n]; qy = qy + ph[n] * SY[n]; }
+                    lu = lu - CISM_RHOI * CISM_GRAV * 1.0 * det / CISM_VOL0 * qx * ph[me];
+                    lv = lv - CISM_RHOI * CISM_GRAV * 1.0 * det / CISM_VOL0 * qy * ph[me];
+                }
+            }
+    load[q] = lu;
+    load[g.npad + q] = lv;
+}
+
+int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy) {
+    const GridDev &g = c->g;
+    if (whichassemble_taud == 1)
+        hipLaunchKernelGGL(k_load_local, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, c->wsig,
+                           dx, dy, c->av, c->sthck, c->dsdx, c->dsdy, c->load);
+    else
+        hipLaunchKernelGGL(k_load_standard, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g,
+                           c->sigma, c->ac, c->xV, c->yV, c->susrf, c->sthck, c->dsdx, c->dsdy,
+                           c->load);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}

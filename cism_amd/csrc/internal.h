@@ -1,0 +1,224 @@
+// internal.h -- shared definitions of libcism_gpu.so (gfx950 / CDNA4).
+//
+// Device data layout ("node-major planes"):
+//   node q (0-based) = k + nz*(i + sx*j), sx = nx-1, sy = ny-1, k fastest --
+//   the reference's vector layout (nz, nx-1, ny-1) without change.
+//   Matrix block b (0 Auu, 1 Auv, 2 Avu, 3 Avv), slot m (0..26, m+1 = indxA):
+//   A[(b*27 + m)*npad + q], i.e. 108 planes of npad (nz*sx*sy rounded up to a
+//   multiple of 256) values.  For a fixed slot, neighbouring lanes read
+//   neighbouring coefficients, so every matrix load is a unit-stride stream.
+//   Vectors hold u then v: x[c*npad + q], c = 0 (u), 1 (v).
+//   Element scratch: Ke[entry*nepad + e], e = (k-1) + (nz-1)*((i-1) + nx*(j-1)),
+//   the reference's efvs layout (nz-1, nx, ny).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/cism_gpu.h"
+
+#define CISM_NHALO 2
+#define CISM_SLOTS 27
+#define CISM_DIAG 13           // slot of indxA(0,0,0) = 14 (0-based 13)
+#define CISM_KE_ENTRIES 136    // 36 (Kuu upper) + 36 (Kvv upper) + 64 (Kuv)
+#define CISM_RED_BLOCKS 1024   // max partial-sum blocks of one reduction
+#define CISM_BLOCK 256
+
+// physical constants (glimmer_physcon.F90:64-75) and the matrix volume scale
+#define CISM_RHOI 910.0
+#define CISM_GRAV 9.81
+#define CISM_SCYR 31536000.0
+#define CISM_VOL0 1.0e9
+
+struct GridDev {
+    int nx, ny, nz, sx, sy;
+    int ilo, ihi, jlo, jhi;   // owned staggered vertices, 0-based inclusive
+    int periodic;
+    long long nn, npad;       // nodes (nz*sx*sy), padded plane length
+    long long ne, nepad;      // elements (nz-1)*nx*ny, padded
+};
+
+// device-resident scalars of the CG iterations (one per context)
+struct DevScal {
+    double rho_old, alpha, beta, sigma, eta0, eta1, eta2;
+    double L2_rhs, err;
+    double sums[8];
+    int nan_flag, singular_flag;
+    unsigned int sym_count;
+    int pad;
+};
+
+struct cism_ctx {
+    GridDev g;
+    int device;
+    int maxiters;
+    double tolerance;
+    hipStream_t stream;
+    // matrix, FP64 and (lazily) FP32
+    double *A;
+    float *A32;
+    // vectors (2*npad each): b x r d z q Az t(temp) load usav
+    double *b, *x, *r, *d, *z, *q, *Az, *t, *load, *usav;
+    // FP32 vectors for the inner solves (2*npad each)
+    float *x32, *b32, *r32, *d32, *z32, *q32, *Az32;
+    // masks and geometry
+    int32_t *av, *ac, *umask, *vmask;
+    double *xV, *yV, *susrf, *sthck, *beta, *sigma, *wsig, *flwafact, *efvs;
+    double *dsdx, *dsdy;
+    // FP64 reference velocity (lazily) for error reports
+    double *xref;
+    // element scratch (lazily)
+    double *Ke;
+    // reductions
+    double *partials;
+    DevScal *sc;
+    DevScal *sc_host;   // pinned mirror
+    // staging for layout conversion (27*nn doubles, lazily)
+    double *stage;
+    // symmetry-repair list (lazily)
+    long long *sym_list;
+    double *sym_vals;
+    long long sym_cap;
+};
+
+// ---------------------------------------------------------------------------
+// index helpers
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int slot_of(int iA, int jA, int kA) {
+    return (iA + 1) + 3 * (jA + 1) + 9 * (kA + 1);
+}
+
+// periodic / outflow image of a staggered index (parallel_slap.F90:2807-2925):
+// returns the owned index whose value the halo entry takes, or -1 for zero
+__device__ inline int halo_image(int i, int s, int periodic) {
+    if (periodic) {
+        if (i < 2) return i + s - 3;
+        if (i == s - 1) return 2;
+        return i;
+    }
+    if (i == 0 || i == s - 1) return -1;
+    return i;
+}
+
+__device__ inline bool is_owned(const GridDev &g, int i, int j) {
+    return i >= g.ilo && i <= g.ihi && j >= g.jlo && j <= g.jhi;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction (fixed lane/wave order)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], double *partials) {
+    __shared__ double red[CISM_BLOCK / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        double s = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][threadIdx.x];
+        partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+    }
+}
+
+template <int NV>
+__device__ inline void block_max_store(double (&v)[NV], double *partials) {
+    __shared__ double red[CISM_BLOCK / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        double s = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_down(s, off, 64));
+        if (lane == 0) red[wave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = red[0][threadIdx.x];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w][threadIdx.x]);
+        partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+    }
+}
+
+// grid-stride helpers
+inline int nblocks_for(long long n, int block = CISM_BLOCK) {
+    long long b = (n + block - 1) / block;
+    return (int)(b < 1 ? 1 : b);
+}
+inline int red_blocks_for(long long n) {
+    long long b = (n + CISM_BLOCK - 1) / CISM_BLOCK;
+    if (b > CISM_RED_BLOCKS) b = CISM_RED_BLOCKS;
+    return (int)(b < 1 ? 1 : b);
+}
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+void cism_set_error(const char *fmt, ...);
+#define HIP_TRY(call)                                                             \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            cism_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                           hipGetErrorString(e_));                                \
+            return CISM_ERR_HIP;                                                  \
+        }                                                                         \
+    } while (0)
+#define CISM_TRY(...)                  \
+    do {                               \
+        int s_ = (__VA_ARGS__);        \
+        if (s_ != CISM_OK) return s_;  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// launchers shared between translation units (all enqueue on ctx->stream)
+// ---------------------------------------------------------------------------
+// reduce partials[nb][nv] -> sc->sums[0..nv), then apply a scalar update (mode)
+enum ReduceMode {
+    RED_SUMS = 0,       // just the sums
+    RED_BB,             // L2_rhs = sqrt(s0)
+    RED_STD_ETA1,       // eta1 = s0; NaN check; beta = eta1/eta0
+    RED_STD_ETA2,       // eta2 = s0; eta0 = eta1; alpha = eta1/eta2; NaN check
+    RED_CG_FIRST,       // rho_old = s0; sigma = s1; alpha = rho_old/sigma
+    RED_CG_ITER,        // Chron-Gear scalars (pcg:1501-1512)
+    RED_ERR,            // err = sqrt(s0)/L2_rhs
+    RED_MAX             // max instead of sum (sums[c] = max)
+};
+int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
+
+int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
+int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
+int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
+int launch_halo_n(cism_ctx *c, double *v, int nplanes);
+int convert_a32(cism_ctx *c);
+int launch_cg_matvec(cism_ctx *c, int fp32);
+int rel_err_device(cism_ctx *c, double *rel);
+int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol,
+                 double *err, int *niters);
+int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
+                 double *err, int *niters, int *n64, int *n32);
+
+int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
+                    int add_basal, double dx, double dy);
+int dirichlet_device(cism_ctx *c);
+int symmetry_device(cism_ctx *c);
+int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
+int residual_vector_device(cism_ctx *c, double *ru, double *rv);  // sums[0]=|r|^2, [1]=|b|^2
+int residual_velocity_device(cism_ctx *c, int whichresid);       // sums[0]
+int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy);
+
+int ensure_stage(cism_ctx *c);
+int ensure_ke(cism_ctx *c);
+int ensure_a32(cism_ctx *c);
+int upload_matrix(cism_ctx *c, const double *Auu, const double *Auv, const double *Avu,
+                  const double *Avv);
+int download_matrix(cism_ctx *c, double *Auu, double *Auv, double *Avu, double *Avv);
+int upload_vec(cism_ctx *c, double *dst, const double *hu, const double *hv);
+int download_vec(cism_ctx *c, const double *src, double *hu, double *hv);
+int sync_scalars(cism_ctx *c);   // copies *sc to *sc_host, synchronizes

@@ -1,0 +1,296 @@
+"""ctypes binding of libcism_gpu.so (the C ABI declared in include/cism_gpu.h).
+
+The Python names and argument order follow the reference's Fortran
+subroutines (glissade_velo_higher_pcg.F90, glissade_velo_higher.F90), with the
+context first.  Arrays are numpy arrays in the reference's Fortran layouts;
+arguments the reference declares intent(inout)/intent(out) are modified in
+place, as in the reference.
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcism_gpu.so")
+
+CISM_OK = 0
+STATUS_NAMES = {1: "CISM_ERR_HIP", 2: "CISM_ERR_SINGULAR", 3: "CISM_ERR_ASYMMETRIC",
+                4: "CISM_ERR_NAN", 5: "CISM_ERR_ARG", 6: "CISM_ERR_NOMEM", 7: "CISM_ERR_UNSUPPORTED"}
+
+EXPORTED = [
+    "cism_ctx_create", "cism_ctx_destroy", "cism_last_error", "cism_ctx_set_linear_solver",
+    "cism_matvec_multiply_structured_3d", "cism_pcg_solver_standard_3d",
+    "cism_pcg_solver_chrongear_3d", "cism_assemble_stiffness_matrix_3d", "cism_basal_sliding_bc_3d",
+    "cism_dirichlet_boundary_conditions_3d", "cism_check_symmetry_assembled_matrix_3d",
+    "cism_count_nonzeros_3d", "cism_compute_residual_vector_3d", "cism_compute_residual_velocity_3d",
+    "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
+    "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
+]
+
+
+class CismError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("whichbabc", C.c_int), ("whichefvs", C.c_int), ("whichresid", C.c_int),
+                ("whichsparse", C.c_int), ("whichprecond", C.c_int), ("whichapprox", C.c_int),
+                ("maxiter_nonlinear", C.c_int), ("precision", C.c_int),
+                ("efvs_constant", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("target_override", C.c_double), ("inner_rtol", C.c_double),
+                ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
+                ("stop_rel_err", C.c_double), ("reserved", C.c_int * 5)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("picard_iters", C.c_int), ("linear_iters", C.c_int), ("fp64_matvecs", C.c_int),
+                ("fp32_matvecs", C.c_int), ("final_L2_norm", C.c_double),
+                ("final_resid_velo", C.c_double), ("seconds_assembly", C.c_double),
+                ("seconds_linear", C.c_double), ("seconds_total", C.c_double),
+                ("status", C.c_int), ("refinement_steps", C.c_int),
+                ("final_rel_err", C.c_double), ("reserved", C.c_int * 4)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+_lib = None
+D = C.POINTER(C.c_double)
+I32 = C.POINTER(C.c_int32)
+I64 = C.POINTER(C.c_int64)
+VP = C.c_void_p
+
+
+def lib():
+    """Loads libcism_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cism_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name in EXPORTED:
+            getattr(L, name)   # every declared symbol must be exported
+        L.cism_last_error.restype = C.c_char_p
+        L.cism_ctx_create.argtypes = [C.c_int] * 6 + [C.POINTER(VP)]
+        L.cism_ctx_destroy.argtypes = [VP]
+        L.cism_ctx_set_linear_solver.argtypes = [VP, C.c_int, C.c_double]
+        L.cism_matvec_multiply_structured_3d.argtypes = [VP, I32, I32, D, D, D, D, D, D, D, D]
+        pcg = [VP, I32, I32, D, D, D, D, D, D, D, D, C.c_int, D, C.POINTER(C.c_int)]
+        L.cism_pcg_solver_standard_3d.argtypes = pcg
+        L.cism_pcg_solver_chrongear_3d.argtypes = pcg
+        L.cism_assemble_stiffness_matrix_3d.argtypes = [VP, D, I32, D, D, D, D, D, D, D, C.c_int, D,
+                                                        C.c_int, C.c_double, D, D, D, D]
+        L.cism_basal_sliding_bc_3d.argtypes = [VP, C.c_double, C.c_double, I32, D, C.c_int, D, D]
+        L.cism_dirichlet_boundary_conditions_3d.argtypes = [VP, I32, I32, I32, D, D, D, D, D, D,
+                                                            D, D]
+        L.cism_check_symmetry_assembled_matrix_3d.argtypes = [VP, I32, D, D, D, D, I64]
+        L.cism_count_nonzeros_3d.argtypes = [VP, D, D, D, D, I32, I64]
+        L.cism_compute_residual_vector_3d.argtypes = [VP, I32, D, D, D, D, D, D, D, D, D, D, D, D]
+        L.cism_compute_residual_velocity_3d.argtypes = [VP, C.c_int, D, D, D, D, D]
+        L.cism_load_vector_gravity.argtypes = [VP, D, D, C.c_double, C.c_double, I32, I32, D, D,
+                                               D, D, D, D, C.c_int, D, D]
+        L.cism_staggered_parallel_halo.argtypes = [VP, C.c_int, D]
+        L.cism_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
+        L.cism_solve_opts_default.restype = None
+        L.cism_velo_higher_solve_3d.argtypes = [VP, C.POINTER(SolveOpts), D, I32, I32, D, D, D, D,
+                                                D, D, I32, I32, D, D, D, D, D, D, D, D, D,
+                                                C.POINTER(SolveStats)]
+        L.cism_set_reference_velocity.argtypes = [VP, D, D]
+        L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != CISM_OK:
+        raise CismError(st, lib().cism_last_error().decode())
+
+
+def dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim <= 1), "need F-ordered float64"
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and (a.flags.f_contiguous or a.ndim <= 1), "need F-ordered int32"
+    return a.ctypes.data_as(I32)
+
+
+def standard_indxA():
+    ind = np.zeros((3, 3, 3), dtype=np.int32, order="F")
+    m = 0
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                m += 1
+                ind[i, j, k] = m
+    return ind
+
+
+class Context:
+    """Owns one device context (grid + device buffers), like glissade_velo_higher_init."""
+
+    def __init__(self, nx, ny, nz, periodic=True, device=0, nhalo=2):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        h = VP()
+        _check(lib().cism_ctx_create(nx, ny, nz, nhalo, 1 if periodic else 0, device, C.byref(h)))
+        self.h = h
+        self.indxA = standard_indxA()
+
+    @classmethod
+    def for_problem(cls, p, device=0):
+        return cls(p.nx, p.ny, p.nz, p.periodic, device)
+
+    def close(self):
+        if self.h:
+            lib().cism_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_linear_solver(self, maxiters=200, tolerance=1e-8):
+        _check(lib().cism_ctx_set_linear_solver(self.h, maxiters, tolerance))
+
+    # --- glissade_velo_higher_pcg ------------------------------------------------
+    def matvec_multiply_structured_3d(self, active_vertex, Auu, Auv, Avu, Avv, xu, xv, yu, yv):
+        _check(lib().cism_matvec_multiply_structured_3d(
+            self.h, ip(self.indxA), ip(active_vertex), dp(Auu), dp(Auv), dp(Avu), dp(Avv),
+            dp(xu), dp(xv), dp(yu), dp(yv)))
+
+    def _pcg(self, fn, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond):
+        err, nit = C.c_double(), C.c_int()
+        _check(fn(self.h, ip(self.indxA), ip(active_vertex), dp(Auu), dp(Auv), dp(Avu), dp(Avv),
+                  dp(bu), dp(bv), dp(xu), dp(xv), precond, C.byref(err), C.byref(nit)))
+        return err.value, nit.value
+
+    def pcg_solver_standard_3d(self, *a):
+        """(active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond) -> (err, niters)"""
+        return self._pcg(lib().cism_pcg_solver_standard_3d, *a)
+
+    def pcg_solver_chrongear_3d(self, *a):
+        return self._pcg(lib().cism_pcg_solver_chrongear_3d, *a)
+
+    # --- glissade_velo_higher --------------------------------------------------
+    def assemble_stiffness_matrix_3d(self, sigma, active_cell, xVertex, yVertex, uvel, vvel,
+                                     stagusrf, stagthck, flwafact, whichapprox, efvs, whichefvs,
+                                     efvs_constant, Auu, Auv, Avu, Avv):
+        _check(lib().cism_assemble_stiffness_matrix_3d(
+            self.h, dp(sigma), ip(active_cell), dp(xVertex), dp(yVertex), dp(uvel), dp(vvel),
+            dp(stagusrf), dp(stagthck), dp(flwafact), whichapprox, dp(efvs), whichefvs,
+            efvs_constant, dp(Auu), dp(Auv), dp(Avu), dp(Avv)))
+
+    def basal_sliding_bc_3d(self, dx, dy, active_vertex, beta, whichassemble_beta, Auu, Avv):
+        _check(lib().cism_basal_sliding_bc_3d(self.h, dx, dy, ip(active_vertex), dp(beta),
+                                              whichassemble_beta, dp(Auu), dp(Avv)))
+
+    def dirichlet_boundary_conditions_3d(self, active_vertex, umask, vmask, uvel, vvel,
+                                         Auu, Auv, Avu, Avv, bu, bv):
+        _check(lib().cism_dirichlet_boundary_conditions_3d(
+            self.h, ip(active_vertex), ip(umask), ip(vmask), dp(uvel), dp(vvel), dp(Auu), dp(Auv),
+            dp(Avu), dp(Avv), dp(bu), dp(bv)))
+
+    def check_symmetry_assembled_matrix_3d(self, active_vertex, Auu, Auv, Avu, Avv):
+        nw = C.c_int64()
+        _check(lib().cism_check_symmetry_assembled_matrix_3d(
+            self.h, ip(active_vertex), dp(Auu), dp(Auv), dp(Avu), dp(Avv), C.byref(nw)))
+        return nw.value
+
+    def count_nonzeros_3d(self, Auu, Auv, Avu, Avv, active_vertex):
+        n = C.c_int64()
+        _check(lib().cism_count_nonzeros_3d(self.h, dp(Auu), dp(Auv), dp(Avu), dp(Avv),
+                                            ip(active_vertex), C.byref(n)))
+        return n.value
+
+    def compute_residual_vector_3d(self, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
+                                   resid_u, resid_v):
+        L2, L2r = C.c_double(), C.c_double()
+        _check(lib().cism_compute_residual_vector_3d(
+            self.h, ip(active_vertex), dp(Auu), dp(Auv), dp(Avu), dp(Avv), dp(bu), dp(bv),
+            dp(uvel), dp(vvel), dp(resid_u), dp(resid_v), C.byref(L2), C.byref(L2r)))
+        return L2.value, L2r.value
+
+    def compute_residual_velocity_3d(self, whichresid, uvel, vvel, usav, vsav):
+        r = C.c_double()
+        _check(lib().cism_compute_residual_velocity_3d(self.h, whichresid, dp(uvel), dp(vvel),
+                                                       dp(usav), dp(vsav), C.byref(r)))
+        return r.value
+
+    def load_vector_gravity(self, sigma, stagwbndsigma, dx, dy, active_cell, active_vertex,
+                            xVertex, yVertex, stagusrf, stagthck, dusrf_dx, dusrf_dy,
+                            whichassemble_taud, loadu, loadv):
+        _check(lib().cism_load_vector_gravity(
+            self.h, dp(sigma), dp(stagwbndsigma), dx, dy, ip(active_cell), ip(active_vertex),
+            dp(xVertex), dp(yVertex), dp(stagusrf), dp(stagthck), dp(dusrf_dx), dp(dusrf_dy),
+            whichassemble_taud, dp(loadu), dp(loadv)))
+
+    def set_reference_velocity(self, uref, vref):
+        _check(lib().cism_set_reference_velocity(self.h, dp(uref), dp(vref)))
+
+    def time_kernel(self, which, reps=20):
+        """Mean ms per launch (HIP events on the context stream): 0 FP64 matvec,
+        1 FP32 matvec, 2 assembly."""
+        ms = C.c_double()
+        _check(lib().cism_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def staggered_parallel_halo(self, a):
+        lead = int(np.prod(a.shape[:-2]))
+        _check(lib().cism_staggered_parallel_halo(self.h, lead, dp(a)))
+
+    # --- device-resident Picard solve ------------------------------------------
+    def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
+                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
+                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
+                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0):
+        o = SolveOpts()
+        lib().cism_solve_opts_default(C.byref(o))
+        o.whichbabc = p.whichbabc
+        o.whichefvs = p.whichefvs
+        o.whichresid = p.whichresid if whichresid is None else whichresid
+        o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+        o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+        o.whichapprox = p.whichapprox
+        o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter_nonlinear is None else maxiter_nonlinear
+        o.precision = precision
+        o.efvs_constant = p.efvs_constant
+        o.dx, o.dy = p.dx, p.dy
+        o.target_override = target_override
+        o.inner_rtol = inner_rtol
+        o.adaptive_linear = adaptive_linear
+        o.adaptive_gamma = adaptive_gamma
+        o.stop_rel_err = stop_rel_err
+        u = (p.uvel if uvel is None else uvel).copy(order="F")
+        v = (p.vvel if vvel is None else vvel).copy(order="F")
+        shape = (p.nz, p.nx - 1, p.ny - 1)
+        efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+        bu, bv, ru, rv = (np.zeros(shape, order="F") for _ in range(4))
+        st = SolveStats()
+        _check(lib().cism_velo_higher_solve_3d(
+            self.h, C.byref(o), dp(p.sigma), ip(p.active_cell), ip(p.active_vertex),
+            dp(p.xVertex), dp(p.yVertex), dp(p.stagusrf), dp(p.stagthck), dp(p.flwafact),
+            dp(p.beta_internal), ip(p.umask_dirichlet), ip(p.vmask_dirichlet), dp(loadu),
+            dp(loadv), dp(u), dp(v), dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv), C.byref(st)))
+        return dict(uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
+                    **st.as_dict())

@@ -1,0 +1,227 @@
+/*
+ * cism_gpu.h -- C ABI of libcism_gpu.so, the MI355X (gfx950) implementation of
+ * CISM Glissade's higher-order (Blatter-Pattyn) velocity solve.
+ *
+ * Every entry point is callable from Fortran through ISO_C_BINDING (see
+ * cism_amd/fortran/) and from C, C++ or Python (ctypes).  Arrays are plain
+ * pointers to HOST buffers in the reference's Fortran (column-major) layouts:
+ *
+ *   matrix blocks  (27, nz, nx-1, ny-1)   Auu, Auv, Avu, Avv     double
+ *   node vectors   (nz, nx-1, ny-1)       u, v, b, residuals      double
+ *   node masks     (nz, nx-1, ny-1)       Dirichlet masks         int32
+ *   vertex fields  (nx-1, ny-1)           coordinates, beta, ...  double
+ *   vertex masks   (nx-1, ny-1)           active_vertex           int32 (0/1)
+ *   cell masks     (nx, ny)               active_cell             int32 (0/1)
+ *   layer fields   (nz-1, nx, ny)         flwafact, efvs          double
+ *   indxA          (-1:1,-1:1,-1:1)       27 int32, must be the standard map
+ *
+ * nx, ny are the scalar-grid sizes including two halo cells on each side
+ * (nx = ewn + 4), as in the reference.  Fortran LOGICAL arrays are passed as
+ * int32 by the Fortran shim (LOGICAL's kind and .true. encoding are compiler
+ * specific).  Units are the solver's internal ones: m, m/yr, Pa, yr.
+ *
+ * The library owns all device memory, inside an opaque context.  Every call is
+ * synchronous (results are valid on return), runs on the context's single HIP
+ * stream, and is not re-entrant for one context.  Non-zero return values are
+ * the status codes below; cism_last_error() gives the message.
+ */
+#ifndef CISM_GPU_H
+#define CISM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CISM_OK               0
+#define CISM_ERR_HIP          1  /* HIP runtime error                                        */
+#define CISM_ERR_SINGULAR     2  /* "Jacobian matrix is singular" (velo_higher:6458-6473)    */
+#define CISM_ERR_ASYMMETRIC   3  /* element matrix asymmetric (velo_higher:9497-9527)        */
+#define CISM_ERR_NAN          4  /* "PCG solver has failed, alpha = NaN" (pcg:460-463)        */
+#define CISM_ERR_ARG          5  /* invalid argument (sizes, indxA, options)                  */
+#define CISM_ERR_NOMEM        6  /* device allocation failed                                  */
+#define CISM_ERR_UNSUPPORTED  7  /* option outside the implemented set                        */
+
+typedef struct cism_ctx cism_ctx;
+
+/* Context: the analogue of glissade_velo_higher_init plus the grid of
+ * parallel_slap's distributed_grid (parallel_slap.F90:563-686).  periodic = 1
+ * for periodic global boundaries (the serial default), 0 for outflow.  nhalo
+ * must be 2 (the higher-order dycores require it, parallel_slap.F90:33-38).
+ * device = HIP device ordinal. */
+int  cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, int device, cism_ctx **out);
+int  cism_ctx_destroy(cism_ctx *ctx);
+const char *cism_last_error(void);
+
+/* Linear-solver constants maxiters and tolerance (pcg:69-75); defaults 200, 1e-8. */
+int  cism_ctx_set_linear_solver(cism_ctx *ctx, int maxiters, double tolerance);
+
+/* matvec_multiply_structured_3d (glissade_velo_higher_pcg.F90:2533-2647) */
+int  cism_matvec_multiply_structured_3d(cism_ctx *ctx, const int32_t *indxA,
+                                        const int32_t *active_vertex,
+                                        const double *Auu, const double *Auv,
+                                        const double *Avu, const double *Avv,
+                                        const double *xu, const double *xv,
+                                        double *yu, double *yv);
+
+/* pcg_solver_standard_3d (glissade_velo_higher_pcg.F90:84-549).
+ * xu, xv: initial guess in, solution out.  precond: 0 none, 1 diagonal,
+ * 2 column-tridiagonal (SIA).  err, niters out. */
+int  cism_pcg_solver_standard_3d(cism_ctx *ctx, const int32_t *indxA,
+                                 const int32_t *active_vertex,
+                                 const double *Auu, const double *Auv,
+                                 const double *Avu, const double *Avv,
+                                 const double *bu, const double *bv,
+                                 double *xu, double *xv, int precond,
+                                 double *err, int *niters);
+
+/* pcg_solver_chrongear_3d (glissade_velo_higher_pcg.F90:965-1610) */
+int  cism_pcg_solver_chrongear_3d(cism_ctx *ctx, const int32_t *indxA,
+                                  const int32_t *active_vertex,
+                                  const double *Auu, const double *Auv,
+                                  const double *Avu, const double *Avv,
+                                  const double *bu, const double *bv,
+                                  double *xu, double *xv, int precond,
+                                  double *err, int *niters);
+
+/* assemble_stiffness_matrix_3d (glissade_velo_higher.F90:4954-5204), with
+ * get_basis_function_derivatives_3d (:6360-6579), compute_effective_viscosity
+ * (:7199-7378), compute_element_matrix (:7868-7986), the element symmetry
+ * check (:9468-9529) and element_to_global_matrix_3d (:7990-8054).
+ * Outputs efvs and the four blocks (all entries written). */
+int  cism_assemble_stiffness_matrix_3d(cism_ctx *ctx, const double *sigma,
+                                       const int32_t *active_cell,
+                                       const double *xVertex, const double *yVertex,
+                                       const double *uvel, const double *vvel,
+                                       const double *stagusrf, const double *stagthck,
+                                       const double *flwafact, int whichapprox,
+                                       double *efvs, int whichefvs, double efvs_constant,
+                                       double *Auu, double *Auv, double *Avu, double *Avv);
+
+/* basal_sliding_bc (glissade_velo_higher.F90:8125-8381) on the basal layer
+ * k = nz of the full 3D blocks; whichassemble_beta must be 1 (local). */
+int  cism_basal_sliding_bc_3d(cism_ctx *ctx, double dx, double dy,
+                              const int32_t *active_vertex, const double *beta,
+                              int whichassemble_beta, double *Auu, double *Avv);
+
+/* dirichlet_boundary_conditions_3d (glissade_velo_higher.F90:8385-8578) */
+int  cism_dirichlet_boundary_conditions_3d(cism_ctx *ctx, const int32_t *active_vertex,
+                                           const int32_t *umask_dirichlet,
+                                           const int32_t *vmask_dirichlet,
+                                           const double *uvel, const double *vvel,
+                                           double *Auu, double *Auv, double *Avu, double *Avv,
+                                           double *bu, double *bv);
+
+/* check_symmetry_assembled_matrix_3d (glissade_velo_higher.F90:9533-9741).
+ * nwarn (optional) = number of pairs too asymmetric to repair. */
+int  cism_check_symmetry_assembled_matrix_3d(cism_ctx *ctx, const int32_t *active_vertex,
+                                             double *Auu, double *Auv, double *Avu, double *Avv,
+                                             int64_t *nwarn);
+
+/* count_nonzeros_3d (glissade_velo_higher.F90:9355-9410) */
+int  cism_count_nonzeros_3d(cism_ctx *ctx, const double *Auu, const double *Auv,
+                            const double *Avu, const double *Avv,
+                            const int32_t *active_vertex, int64_t *nNonzeros);
+
+/* compute_residual_vector_3d (glissade_velo_higher.F90:8768-8927) */
+int  cism_compute_residual_vector_3d(cism_ctx *ctx, const int32_t *active_vertex,
+                                     const double *Auu, const double *Auv,
+                                     const double *Avu, const double *Avv,
+                                     const double *bu, const double *bv,
+                                     const double *uvel, const double *vvel,
+                                     double *resid_u, double *resid_v,
+                                     double *L2_norm, double *L2_norm_relative);
+
+/* compute_residual_velocity_3d (glissade_velo_higher.F90:9111-9234) */
+int  cism_compute_residual_velocity_3d(cism_ctx *ctx, int whichresid,
+                                       const double *uvel, const double *vvel,
+                                       const double *usav, const double *vsav,
+                                       double *resid_velo);
+
+/* load_vector_gravity (glissade_velo_higher.F90:4317-4532); adds into loadu/v.
+ * whichassemble_taud: 0 standard FE quadrature, 1 local. */
+int  cism_load_vector_gravity(cism_ctx *ctx, const double *sigma, const double *stagwbndsigma,
+                              double dx, double dy, const int32_t *active_cell,
+                              const int32_t *active_vertex,
+                              const double *xVertex, const double *yVertex,
+                              const double *stagusrf, const double *stagthck,
+                              const double *dusrf_dx, const double *dusrf_dy,
+                              int whichassemble_taud, double *loadu, double *loadv);
+
+/* staggered_parallel_halo_real8_3d/_4d (parallel_slap.F90:2807-2925) for an
+ * array of shape (lead, nx-1, ny-1): lead = nz (vectors) or 27*nz (blocks). */
+int  cism_staggered_parallel_halo(cism_ctx *ctx, int lead, double *a);
+
+/* Device-resident Picard loop of glissade_velo_higher_solve, 3D BP branch
+ * (glissade_velo_higher.F90:2003-2016, 2206-3727): the inputs are the arrays the
+ * reference prepares before its loop; uvel/vvel are the initial guess in and
+ * the solution out.  Host memory is touched only at entry and exit. */
+typedef struct {
+    int    whichbabc;          /* 4 beta large, 5 external, 6 no-slip, others: beta as given */
+    int    whichefvs;          /* 0 constant, 1 flow factor, 2 nonlinear                    */
+    int    whichresid;         /* 0..4, default 3 (L2 norm)                                  */
+    int    whichsparse;        /* 2 standard PCG, 3 Chron-Gear                               */
+    int    whichprecond;       /* 0 none, 1 diagonal, 2 SIA column tridiagonal               */
+    int    whichapprox;        /* 0 SIA, 1 SSA, 2 BP (element terms of the 3D assembly)     */
+    int    maxiter_nonlinear;  /* glissade_maxiter, default 100                              */
+    int    precision;          /* 0 FP64 (reference), 1 FP32 matrix + FP64 refinement        */
+    double efvs_constant;      /* Pa yr                                                      */
+    double dx, dy;             /* m                                                          */
+    double target_override;    /* > 0: replace the Picard target (1e-4 for L2)               */
+    double inner_rtol;         /* precision 1: relative tolerance of each FP32 inner solve   */
+    int    adaptive_linear;    /* 1: linear tolerance max(tol, gamma * |Ax-b|/|b|) per Picard
+                                  iteration (inexact Picard); 0: the reference's fixed tol */
+    double adaptive_gamma;     /* forcing factor of adaptive_linear (default 0.05)           */
+    double stop_rel_err;       /* > 0: also stop once |x - x_ref| / |x_ref| <= stop_rel_err,
+                                  x_ref set by cism_set_reference_velocity                   */
+    int    reserved[5];
+} cism_solve_opts;
+
+typedef struct {
+    int    picard_iters;
+    int    linear_iters;       /* total PCG iterations (FP64 + FP32)                          */
+    int    fp64_matvecs;
+    int    fp32_matvecs;
+    double final_L2_norm;      /* residual |Ax-b| of the last Picard iteration (old x)      */
+    double final_resid_velo;
+    double seconds_assembly;
+    double seconds_linear;
+    double seconds_total;
+    int    status;
+    int    refinement_steps;   /* FP64 residual evaluations of the mixed-precision solves    */
+    double final_rel_err;      /* |x - x_ref| / |x_ref| at exit (-1 without a reference)     */
+    int    reserved[4];
+} cism_solve_stats;
+
+void cism_solve_opts_default(cism_solve_opts *o);
+
+int  cism_velo_higher_solve_3d(cism_ctx *ctx, const cism_solve_opts *opts,
+                               const double *sigma, const int32_t *active_cell,
+                               const int32_t *active_vertex,
+                               const double *xVertex, const double *yVertex,
+                               const double *stagusrf, const double *stagthck,
+                               const double *flwafact, const double *beta_internal,
+                               const int32_t *umask_dirichlet, const int32_t *vmask_dirichlet,
+                               const double *loadu, const double *loadv,
+                               double *uvel, double *vvel, double *efvs,
+                               double *bu, double *bv, double *resid_u, double *resid_v,
+                               cism_solve_stats *stats);
+
+/* FP64 reference velocity for cism_solve_opts.stop_rel_err and the error report
+ * (host arrays (nz, nx-1, ny-1)); NULL pointers clear it.  No reference
+ * counterpart: the error of every fast solve is judged against an FP64 solve. */
+int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double *vref);
+
+/* Kernel timing on the context's stream with HIP events, for roofline reports
+ * (no reference counterpart).  Each times `reps` launches on the matrix and
+ * vectors left by the last solve and returns the mean duration in ms.
+ *   which = 0 FP64 matvec (with the fused Chron-Gear dot products)
+ *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
+ *           2 assembly (element kernel + node gather) of the current velocity */
+int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

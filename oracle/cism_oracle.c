@@ -1,0 +1,982 @@
+/*
+ * cism_oracle.c -- CPU oracle (test infrastructure only; see cism_oracle.h).
+ *
+ * A plain-C restatement of the reference's 3D Blatter-Pattyn path.  Each
+ * function cites the reference routine it follows (paths relative to the
+ * CISM source tree).  Loop orders, index ranges and the order of floating
+ * point operations follow the reference so that the oracle reproduces its
+ * rounding, not only its mathematics.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   orc_tables                 literal ishift/jshift/kshift/indxA tables of the
+ *                              reference; partition of unity; sum of derivatives 0
+ *   orc_basis_derivatives      box element: J = diag(dx/2,dy/2,dz/2), detJ closed form;
+ *                              sheared column: analytic J; affine field reproduced exactly
+ *   orc_efvs_qp                linear velocity field -> analytic strain rates -> closed form
+ *   orc_element_matrix         box element, constant efvs: Kronecker sums of 1D linear
+ *                              stiffness / mass matrices; Kvu == Kuv^T bitwise
+ *   orc_assemble_3d            brute-force scipy CSR assembly from element matrices;
+ *                              rigid translation u=const in the interior; symmetry
+ *   orc_dirichlet_3d           explicit column-elimination formula on a dense matrix
+ *   orc_halo                   explicit periodic index maps
+ *   orc_matvec_3d              brute-force CSR product
+ *   orc_tridiag                scipy.linalg.solve_banded
+ *   orc_pcg_*                  scipy.sparse.linalg.spsolve; standard vs Chron-Gear
+ *                              iterates agree; niters equal
+ *   orc_picard_solve           slab analytic first-order solution (tests/slab in the
+ *                              reference); converged-residual invariant
+ */
+#include "cism_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* glimmer_physcon.F90:64-75 (non-CESM values) */
+static const double RHOI = 910.0, GRAV = 9.81, SCYR = 31536000.0;
+static const int GN = 3;
+/* glissade_velo_higher.F90:1111 */
+static const double VOL0 = 1.0e9;
+/* glimmer_paramets.F90:146-147 */
+static const double LEN_SCALE = 200.0e3;
+#define VEL_SCALE (500.0 / SCYR)
+
+#define IDX_A(iA, jA, kA) (1 + ((iA) + 1) + 3 * ((jA) + 1) + 9 * ((kA) + 1))
+
+/* Fortran-layout accessors, 1-based logical indices, local nx, ny, nz in scope */
+#define V3(a, k, i, j) (a)[((k)-1) + (size_t)nz * (((i)-1) + (size_t)(nx - 1) * ((j)-1))]
+#define A4(a, m, k, i, j) \
+    (a)[((m)-1) + (size_t)27 * (((k)-1) + (size_t)nz * (((i)-1) + (size_t)(nx - 1) * ((j)-1)))]
+#define S2(a, i, j) (a)[((i)-1) + (size_t)(nx - 1) * ((j)-1)]
+#define C2(a, i, j) (a)[((i)-1) + (size_t)nx * ((j)-1)]
+#define F3(a, k, i, j) (a)[((k)-1) + (size_t)(nz - 1) * (((i)-1) + (size_t)nx * ((j)-1))]
+
+/* ------------------------------------------------------------------------ */
+/* tables: glissade_velo_higher_init (velo_higher:270-650)                  */
+/* ------------------------------------------------------------------------ */
+
+/* node n (0-based) position within the element: x/y offset 0|1, layer 0 = upper (k), 1 = lower (k+1) */
+static const int NODE_IX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int NODE_JY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int NODE_KL[8] = {1, 1, 1, 1, 0, 0, 0, 0};
+
+static double T_phi[8][8], T_dxr[8][8], T_dyr[8][8], T_dzr[8][8]; /* [p][n] */
+static int T_ready = 0;
+
+static void build_tables(void)
+{
+    if (T_ready) return;
+    const double r = 1.0 / sqrt(3.0);
+    /* quadrature point p has reference coordinates (sx*r, sy*r, sz*r), velo_higher:306-328 */
+    static const int QX[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    static const int QY[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    static const int QZ[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int p = 0; p < 8; ++p) {
+        double xq = QX[p] * r, yq = QY[p] * r, zq = QZ[p] * r;
+        for (int n = 0; n < 8; ++n) {
+            /* reference-element node signs: x,y from the horizontal position,
+               z = -1 on the lower face (nodes 1-4), +1 on the upper face */
+            double sx = NODE_IX[n] ? 1.0 : -1.0;
+            double sy = NODE_JY[n] ? 1.0 : -1.0;
+            double sz = NODE_KL[n] ? -1.0 : 1.0;
+            double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
+            double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
+            double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
+            T_phi[p][n] = ax * ay * az / 8.0;
+            T_dxr[p][n] = sx * (ay * az / 8.0);
+            T_dyr[p][n] = sy * (ax * az / 8.0);
+            T_dzr[p][n] = sz * (ax * ay / 8.0);
+        }
+    }
+    T_ready = 1;
+}
+
+void orc_tables(double *phi, double *dxr, double *dyr, double *dzr,
+                int *ishift, int *jshift, int *kshift, int *indxA)
+{
+    build_tables();
+    /* Fortran layout (n, p) for the basis tables, (nr, nc) for the shifts */
+    for (int p = 0; p < 8; ++p)
+        for (int n = 0; n < 8; ++n) {
+            if (phi) phi[n + 8 * p] = T_phi[p][n];
+            if (dxr) dxr[n + 8 * p] = T_dxr[p][n];
+            if (dyr) dyr[n + 8 * p] = T_dyr[p][n];
+            if (dzr) dzr[n + 8 * p] = T_dzr[p][n];
+        }
+    for (int nc = 0; nc < 8; ++nc)
+        for (int nr = 0; nr < 8; ++nr) {
+            if (ishift) ishift[nr + 8 * nc] = NODE_IX[nc] - NODE_IX[nr];
+            if (jshift) jshift[nr + 8 * nc] = NODE_JY[nc] - NODE_JY[nr];
+            if (kshift) kshift[nr + 8 * nc] = NODE_KL[nc] - NODE_KL[nr];
+        }
+    if (indxA)
+        for (int kA = -1; kA <= 1; ++kA)
+            for (int jA = -1; jA <= 1; ++jA)
+                for (int iA = -1; iA <= 1; ++iA)
+                    indxA[(iA + 1) + 3 * (jA + 1) + 9 * (kA + 1)] = IDX_A(iA, jA, kA);
+}
+
+/* parallel_slap.F90:563-686 with nhalo = 2: staggered_lhalo = 2, uhalo = 1 */
+void orc_grid_init(orc_grid *g, int nx, int ny, int nz, int periodic)
+{
+    g->nx = nx; g->ny = ny; g->nz = nz; g->nhalo = 2;
+    g->periodic = periodic;
+    g->ilo = periodic ? 3 : 2;
+    g->jlo = periodic ? 3 : 2;
+    g->ihi = nx - 1 - 1;
+    g->jhi = ny - 1 - 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* get_basis_function_derivatives_3d (velo_higher:6360-6579)               */
+/* ------------------------------------------------------------------------ */
+int orc_basis_derivatives(const double *x, const double *y, const double *z, int p,
+                          double *dx, double *dy, double *dz, double *detJ)
+{
+    build_tables();
+    const double *a = T_dxr[p], *b = T_dyr[p], *c = T_dzr[p];
+    double J[3][3] = {{0}};
+    for (int n = 0; n < 8; ++n) {
+        J[0][0] = J[0][0] + a[n] * x[n];
+        J[0][1] = J[0][1] + a[n] * y[n];
+        J[0][2] = J[0][2] + a[n] * z[n];
+        J[1][0] = J[1][0] + b[n] * x[n];
+        J[1][1] = J[1][1] + b[n] * y[n];
+        J[1][2] = J[1][2] + b[n] * z[n];
+        J[2][0] = J[2][0] + c[n] * x[n];
+        J[2][1] = J[2][1] + c[n] * y[n];
+        J[2][2] = J[2][2] + c[n] * z[n];
+    }
+    double cof[3][3];
+    cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    cof[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    cof[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    cof[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    cof[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    double det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
+    *detJ = det;
+    if (!(fabs(det) > 0.0)) return ORC_ERR_SINGULAR;
+    double Ji[3][3];
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) Ji[row][col] = cof[col][row] / det;
+    for (int n = 0; n < 8; ++n) {
+        dx[n] = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * c[n];
+        dy[n] = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * c[n];
+        dz[n] = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * c[n];
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_effective_viscosity (velo_higher:7199-7378)                      */
+/* ------------------------------------------------------------------------ */
+double orc_efvs_qp(int whichefvs, int whichapprox, double efvs_constant,
+                   const double *dphx, const double *dphy, const double *dphz,
+                   const double *u, const double *v, double flwafact)
+{
+    const double p_effstr = (1.0 - (double)GN) / (double)GN;
+    const double p2_effstr = p_effstr / 2;
+    const double emin = 1.0e-8;
+    if (whichefvs == 0) return efvs_constant;
+    if (whichefvs == 1) {
+        double effstrain = VEL_SCALE / LEN_SCALE * SCYR;
+        return flwafact * pow(effstrain, p_effstr);
+    }
+    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0, sq;
+    if (whichapprox == 0) {            /* SIA */
+        for (int n = 0; n < 8; ++n) { uz = uz + dphz[n] * u[n]; vz = vz + dphz[n] * v[n]; }
+        sq = emin * emin + 0.25 * (uz * uz + vz * vz);
+    } else if (whichapprox == 1) {     /* SSA */
+        for (int n = 0; n < 8; ++n) {
+            ux = ux + dphx[n] * u[n]; uy = uy + dphy[n] * u[n];
+            vx = vx + dphx[n] * v[n]; vy = vy + dphy[n] * v[n];
+        }
+        sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+    } else {                           /* Blatter-Pattyn */
+        for (int n = 0; n < 8; ++n) {
+            ux = ux + dphx[n] * u[n]; uy = uy + dphy[n] * u[n]; uz = uz + dphz[n] * u[n];
+            vx = vx + dphx[n] * v[n]; vy = vy + dphy[n] * v[n]; vz = vz + dphz[n] * v[n];
+        }
+        sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)))
+           + 0.25 * (uz * uz + vz * vz);
+    }
+    return flwafact * pow(sq, p2_effstr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* One element: node gather (velo_higher:5100-5127), quadrature loop with   */
+/* compute_element_matrix (velo_higher:7868-7986), element efvs mean        */
+/* (velo_higher:5167-5172).  K is Fortran (nr, nc) layout, 8x8.             */
+/* ------------------------------------------------------------------------ */
+int orc_element_matrix(const orc_grid *g, int i, int j, int k,
+                       const double *sigma, const double *xV, const double *yV,
+                       const double *uvel, const double *vvel,
+                       const double *stagusrf, const double *stagthck,
+                       const double *flwafact, int whichapprox, int whichefvs,
+                       double efvs_constant,
+                       double *Kuu, double *Kuv, double *Kvu, double *Kvv,
+                       double *efvs_elem)
+{
+    const int nx = g->nx, nz = g->nz;
+    double x[8], y[8], z[8], u[8], v[8];
+    for (int n = 0; n < 8; ++n) {
+        int in = i - 1 + NODE_IX[n], jn = j - 1 + NODE_JY[n], kn = k + NODE_KL[n];
+        x[n] = S2(xV, in, jn);
+        y[n] = S2(yV, in, jn);
+        z[n] = S2(stagusrf, in, jn) - sigma[kn - 1] * S2(stagthck, in, jn);
+        u[n] = V3(uvel, kn, in, jn);
+        v[n] = V3(vvel, kn, in, jn);
+    }
+    for (int e = 0; e < 64; ++e) Kuu[e] = Kuv[e] = Kvu[e] = Kvv[e] = 0.0;
+    double efvs_qp[8];
+    const double ff = F3(flwafact, k, i, j);
+    for (int p = 0; p < 8; ++p) {
+        double dx[8], dy[8], dz[8], detJ;
+        int st = orc_basis_derivatives(x, y, z, p, dx, dy, dz, &detJ);
+        if (st) return st;
+        double efvs = orc_efvs_qp(whichefvs, whichapprox, efvs_constant, dx, dy, dz, u, v, ff);
+        efvs_qp[p] = efvs;
+        double f = efvs * 1.0 * detJ / VOL0;   /* wqp = 1 */
+        for (int nc = 0; nc < 8; ++nc)
+            for (int nr = 0; nr < 8; ++nr) {
+                int e = nr + 8 * nc;
+                if (whichapprox == 0) {
+                    Kuu[e] = Kuu[e] + f * (dz[nr] * dz[nc]);
+                    Kvv[e] = Kvv[e] + f * (dz[nr] * dz[nc]);
+                } else if (whichapprox == 1) {
+                    Kuu[e] = Kuu[e] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc]);
+                    Kuv[e] = Kuv[e] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                    Kvu[e] = Kvu[e] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                    Kvv[e] = Kvv[e] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc]);
+                } else {
+                    Kuu[e] = Kuu[e] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc] + dz[nr] * dz[nc]);
+                    Kuv[e] = Kuv[e] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                    Kvu[e] = Kvu[e] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                    Kvv[e] = Kvv[e] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc] + dz[nr] * dz[nc]);
+                }
+            }
+    }
+    double s = 0.0;
+    for (int p = 0; p < 8; ++p) s = s + efvs_qp[p];
+    *efvs_elem = s / 8;
+    /* check_symmetry_element_matrix (velo_higher:9468-9529) */
+    const double eps10 = 1.0e-10;
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) {
+            if (fabs(Kuu[a + 8 * b] - Kuu[b + 8 * a]) > eps10) return ORC_ERR_ASYMMETRIC;
+            if (fabs(Kvv[a + 8 * b] - Kvv[b + 8 * a]) > eps10) return ORC_ERR_ASYMMETRIC;
+            if (fabs(Kuv[a + 8 * b] - Kvu[b + 8 * a]) > eps10) return ORC_ERR_ASYMMETRIC;
+        }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* assemble_stiffness_matrix_3d (velo_higher:4954-5204) with                */
+/* element_to_global_matrix_3d (velo_higher:7990-8054)                      */
+/* ------------------------------------------------------------------------ */
+int orc_assemble_3d(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                    const double *xV, const double *yV,
+                    const double *uvel, const double *vvel,
+                    const double *stagusrf, const double *stagthck,
+                    const double *flwafact, int whichapprox,
+                    double *efvs, int whichefvs, double efvs_constant,
+                    double *Auu, double *Auv, double *Avu, double *Avv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nh = g->nhalo;
+    size_t na = (size_t)27 * nz * (nx - 1) * (ny - 1);
+    memset(Auu, 0, na * sizeof(double));
+    memset(Auv, 0, na * sizeof(double));
+    memset(Avu, 0, na * sizeof(double));
+    memset(Avv, 0, na * sizeof(double));
+    memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    double Kuu[64], Kuv[64], Kvu[64], Kvv[64];
+    for (int j = nh + 1; j <= ny - nh + 1; ++j)
+        for (int i = nh + 1; i <= nx - nh + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                double ev;
+                int st = orc_element_matrix(g, i, j, k, sigma, xV, yV, uvel, vvel, stagusrf,
+                                            stagthck, flwafact, whichapprox, whichefvs,
+                                            efvs_constant, Kuu, Kuv, Kvu, Kvv, &ev);
+                if (st) return st;
+                F3(efvs, k, i, j) = ev;
+                for (int nr = 0; nr < 8; ++nr) {
+                    int ir = i - 1 + NODE_IX[nr], jr = j - 1 + NODE_JY[nr], kr = k + NODE_KL[nr];
+                    for (int nc = 0; nc < 8; ++nc) {
+                        int m = IDX_A(NODE_IX[nc] - NODE_IX[nr], NODE_JY[nc] - NODE_JY[nr],
+                                      NODE_KL[nc] - NODE_KL[nr]);
+                        int e = nr + 8 * nc;
+                        A4(Auu, m, kr, ir, jr) += Kuu[e];
+                        A4(Auv, m, kr, ir, jr) += Kuv[e];
+                        A4(Avu, m, kr, ir, jr) += Kvu[e];
+                        A4(Avv, m, kr, ir, jr) += Kvv[e];
+                    }
+                }
+            }
+        }
+    return ORC_OK;
+}
+
+/* basal_sliding_bc, HO_ASSEMBLE_BETA_LOCAL (velo_higher:8216-8232), basal layer k = nz */
+void orc_basal_sliding_local(const orc_grid *g, double dx, double dy,
+                             const int32_t *active_vertex, const double *beta,
+                             double *Auu, double *Avv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int m = IDX_A(0, 0, 0);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i)
+            if (S2(active_vertex, i, j)) {
+                A4(Auu, m, nz, i, j) = A4(Auu, m, nz, i, j) + dx * dy / VOL0 * S2(beta, i, j);
+                A4(Avv, m, nz, i, j) = A4(Avv, m, nz, i, j) + dx * dy / VOL0 * S2(beta, i, j);
+            }
+}
+
+/* dirichlet_boundary_conditions_3d (velo_higher:8385-8578) */
+void orc_dirichlet_3d(const orc_grid *g, const int32_t *active_vertex,
+                      const int32_t *umask, const int32_t *vmask,
+                      const double *uvel, const double *vvel,
+                      double *Auu, double *Auv, double *Avu, double *Avv,
+                      double *bu, double *bv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nh = g->nhalo;
+    const int m0 = IDX_A(0, 0, 0);
+    for (int j = nh; j <= ny - nh + 1; ++j)
+        for (int i = nh; i <= nx - nh + 1; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k) {
+                if (V3(umask, k, i, j) == 1) {
+                    V3(bu, k, i, j) = V3(uvel, k, i, j);
+                    for (int kA = -1; kA <= 1; ++kA)
+                        for (int jA = -1; jA <= 1; ++jA)
+                            for (int iA = -1; iA <= 1; ++iA) {
+                                int kk = k + kA, ii = i + iA, jj = j + jA;
+                                if (kk < 1 || kk > nz || ii < 1 || ii > nx - 1 || jj < 1 || jj > ny - 1)
+                                    continue;
+                                if (iA == 0 && jA == 0 && kA == 0) {
+                                    A4(Auu, m0, k, i, j) = 1.0;
+                                    A4(Auv, m0, k, i, j) = 0.0;
+                                    A4(Avu, m0, k, i, j) = 0.0;
+                                } else {
+                                    int m = IDX_A(iA, jA, kA);
+                                    A4(Auu, m, k, i, j) = 0.0;
+                                    A4(Auv, m, k, i, j) = 0.0;
+                                    int mm = IDX_A(-iA, -jA, -kA);
+                                    if (V3(umask, kk, ii, jj) != 1) {
+                                        V3(bu, kk, ii, jj) = V3(bu, kk, ii, jj) - A4(Auu, mm, kk, ii, jj) * V3(uvel, k, i, j);
+                                        A4(Auu, mm, kk, ii, jj) = 0.0;
+                                    }
+                                    if (V3(vmask, kk, ii, jj) != 1) {
+                                        V3(bv, kk, ii, jj) = V3(bv, kk, ii, jj) - A4(Avu, mm, kk, ii, jj) * V3(uvel, k, i, j);
+                                        A4(Avu, mm, kk, ii, jj) = 0.0;
+                                    }
+                                }
+                            }
+                }
+                if (V3(vmask, k, i, j) == 1) {
+                    V3(bv, k, i, j) = V3(vvel, k, i, j);
+                    for (int kA = -1; kA <= 1; ++kA)
+                        for (int jA = -1; jA <= 1; ++jA)
+                            for (int iA = -1; iA <= 1; ++iA) {
+                                int kk = k + kA, ii = i + iA, jj = j + jA;
+                                if (kk < 1 || kk > nz || ii < 1 || ii > nx - 1 || jj < 1 || jj > ny - 1)
+                                    continue;
+                                if (iA == 0 && jA == 0 && kA == 0) {
+                                    A4(Auv, m0, k, i, j) = 0.0;
+                                    A4(Avu, m0, k, i, j) = 0.0;
+                                    A4(Avv, m0, k, i, j) = 1.0;
+                                } else {
+                                    int m = IDX_A(iA, jA, kA);
+                                    A4(Avu, m, k, i, j) = 0.0;
+                                    A4(Avv, m, k, i, j) = 0.0;
+                                    int mm = IDX_A(-iA, -jA, -kA);
+                                    if (V3(umask, kk, ii, jj) != 1) {
+                                        V3(bu, kk, ii, jj) = V3(bu, kk, ii, jj) - A4(Auv, mm, kk, ii, jj) * V3(vvel, k, i, j);
+                                        A4(Auv, mm, kk, ii, jj) = 0.0;
+                                    }
+                                    if (V3(vmask, kk, ii, jj) != 1) {
+                                        V3(bv, kk, ii, jj) = V3(bv, kk, ii, jj) - A4(Avv, mm, kk, ii, jj) * V3(vvel, k, i, j);
+                                        A4(Avv, mm, kk, ii, jj) = 0.0;
+                                    }
+                                }
+                            }
+                }
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* staggered_parallel_halo_real8_3d/_4d (parallel_slap.F90:2807-2925).      */
+/* The array has shape (lead, nx-1, ny-1); lead = nz or 27*nz.              */
+/* ------------------------------------------------------------------------ */
+void orc_halo(const orc_grid *g, int lead, double *a)
+{
+    const int sx = g->nx - 1, sy = g->ny - 1;   /* staggered extents */
+    const int lh = 2, uh = 1;
+#define H3(i, j) (a + (size_t)lead * (((i)-1) + (size_t)sx * ((j)-1)))
+    if (g->periodic) {
+        /* west/east strips over interior rows j = lh+1 .. sy-uh (copies taken before writes) */
+        for (int j = lh + 1; j <= sy - uh; ++j) {
+            double tmpw[1], *dummy = tmpw; (void)dummy;
+            /* east halo (i = sx) <- owned column lh+1; west halo (1..lh) <- sx-uh-lh+1 .. sx-uh */
+            for (int q = 0; q < uh; ++q)
+                memcpy(H3(sx - uh + 1 + q, j), H3(lh + 1 + q, j), sizeof(double) * lead);
+            for (int q = 0; q < lh; ++q)
+                memcpy(H3(1 + q, j), H3(sx - uh - lh + 1 + q, j), sizeof(double) * lead);
+        }
+        /* north/south full rows */
+        for (int q = 0; q < uh; ++q)
+            memcpy(H3(1, sy - uh + 1 + q), H3(1, lh + 1 + q), sizeof(double) * lead * sx);
+        for (int q = 0; q < lh; ++q)
+            memcpy(H3(1, 1 + q), H3(1, sy - uh - lh + 1 + q), sizeof(double) * lead * sx);
+    } else {
+        for (int j = 1; j <= sy; ++j) {
+            for (int i = 1; i <= lh - 1; ++i) memset(H3(i, j), 0, sizeof(double) * lead);
+            for (int i = sx - uh + 1; i <= sx; ++i) memset(H3(i, j), 0, sizeof(double) * lead);
+        }
+        for (int j = 1; j <= lh - 1; ++j) memset(H3(1, j), 0, sizeof(double) * lead * sx);
+        for (int j = sy - uh + 1; j <= sy; ++j) memset(H3(1, j), 0, sizeof(double) * lead * sx);
+    }
+#undef H3
+}
+
+/* check_symmetry_assembled_matrix_3d (velo_higher:9533-9741) */
+int orc_check_symmetry_3d(const orc_grid *g, const int32_t *active_vertex,
+                          double *Auu, double *Auv, double *Avu, double *Avv,
+                          double *maxdiff_out)
+{
+    const int nx = g->nx, nz = g->nz;
+    const double eps08 = 1.0e-8;
+    const int m0 = IDX_A(0, 0, 0);
+    double maxdiff = 0.0;
+    int nwarn = 0;
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k) {
+                for (int pass = 0; pass < 2; ++pass) {
+                    /* pass 0: Auu/Auv rows (diag from Auu); pass 1: Avv/Avu rows (diag from Avv) */
+                    double *Ad = pass == 0 ? Auu : Avv;
+                    double *Ao = pass == 0 ? Auv : Avu;   /* off-diagonal block of this row      */
+                    double *At = pass == 0 ? Avu : Auv;   /* its transpose partner               */
+                    double diag = A4(Ad, m0, k, i, j);
+                    for (int jA = -1; jA <= 1; ++jA)
+                        for (int iA = -1; iA <= 1; ++iA)
+                            for (int kA = -1; kA <= 1; ++kA) {
+                                if (k + kA < 1 || k + kA > nz) continue;
+                                int m = IDX_A(iA, jA, kA), mm = IDX_A(-iA, -jA, -kA);
+                                double v1 = A4(Ad, m, k, i, j), v2 = A4(Ad, mm, k + kA, i + iA, j + jA);
+                                if (v2 != v1) {
+                                    if (fabs(v2 - v1) > maxdiff) maxdiff = fabs(v2 - v1);
+                                    if (fabs(v2 - v1) < eps08 * fabs(diag)) {
+                                        double avg = 0.5 * (v1 + v2);
+                                        A4(Ad, m, k, i, j) = avg;
+                                        A4(Ad, mm, k + kA, i + iA, j + jA) = avg;
+                                    } else nwarn++;
+                                }
+                                v1 = A4(Ao, m, k, i, j); v2 = A4(At, mm, k + kA, i + iA, j + jA);
+                                if (pass == 1 && fabs(v2 - v1) > maxdiff) maxdiff = fabs(v2 - v1);
+                                if (v2 != v1) {
+                                    if (pass == 0 && fabs(v2 - v1) > maxdiff) maxdiff = fabs(v2 - v1);
+                                    if (fabs(v2 - v1) < eps08 * fabs(diag)) {
+                                        double avg = 0.5 * (v1 + v2);
+                                        A4(Ao, m, k, i, j) = avg;
+                                        A4(At, mm, k + kA, i + iA, j + jA) = avg;
+                                    } else nwarn++;
+                                }
+                            }
+                }
+            }
+        }
+    if (maxdiff_out) *maxdiff_out = maxdiff;
+    return nwarn;
+}
+
+/* count_nonzeros_3d (velo_higher:9355-9410) */
+long orc_count_nonzeros_3d(const orc_grid *g, const int32_t *active_vertex,
+                           const double *Auu, const double *Auv,
+                           const double *Avu, const double *Avv)
+{
+    const int nx = g->nx, nz = g->nz;
+    long n = 0;
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k)
+                for (int m = 1; m <= 27; ++m) {
+                    n += A4(Auu, m, k, i, j) != 0.0;
+                    n += A4(Auv, m, k, i, j) != 0.0;
+                    n += A4(Avu, m, k, i, j) != 0.0;
+                    n += A4(Avv, m, k, i, j) != 0.0;
+                }
+        }
+    return n;
+}
+
+/* the 27-point row product shared by matvec and the residual (pcg:2607-2645) */
+static inline void row_product(int nx, int ny, int nz, int k, int i, int j,
+                               const double *Auu, const double *Auv,
+                               const double *Avu, const double *Avv,
+                               const double *xu, const double *xv, double *yu, double *yv)
+{
+    double su = *yu, sv = *yv;
+    for (int kA = -1; kA <= 1; ++kA)
+        for (int jA = -1; jA <= 1; ++jA)
+            for (int iA = -1; iA <= 1; ++iA) {
+                int kk = k + kA, ii = i + iA, jj = j + jA;
+                if (kk < 1 || kk > nz || ii < 1 || ii > nx - 1 || jj < 1 || jj > ny - 1) continue;
+                int m = IDX_A(iA, jA, kA);
+                su = su + A4(Auu, m, k, i, j) * V3(xu, kk, ii, jj) + A4(Auv, m, k, i, j) * V3(xv, kk, ii, jj);
+                sv = sv + A4(Avu, m, k, i, j) * V3(xu, kk, ii, jj) + A4(Avv, m, k, i, j) * V3(xv, kk, ii, jj);
+            }
+    *yu = su; *yv = sv;
+}
+
+/* compute_residual_vector_3d (velo_higher:8768-8927) */
+void orc_residual_vector_3d(const orc_grid *g, const int32_t *active_vertex,
+                            const double *Auu, const double *Auv,
+                            const double *Avu, const double *Avv,
+                            const double *bu, const double *bv,
+                            const double *uvel, const double *vvel,
+                            double *ru, double *rv,
+                            double *L2_norm, double *L2_norm_relative)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    size_t nv = (size_t)nz * (nx - 1) * (ny - 1);
+    memset(ru, 0, nv * sizeof(double));
+    memset(rv, 0, nv * sizeof(double));
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k)
+                row_product(nx, ny, nz, k, i, j, Auu, Auv, Avu, Avv, uvel, vvel,
+                            &V3(ru, k, i, j), &V3(rv, k, i, j));
+        }
+    double L2 = 0.0;
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k) {
+                V3(ru, k, i, j) = V3(ru, k, i, j) - V3(bu, k, i, j);
+                V3(rv, k, i, j) = V3(rv, k, i, j) - V3(bv, k, i, j);
+                L2 = L2 + V3(ru, k, i, j) * V3(ru, k, i, j) + V3(rv, k, i, j) * V3(rv, k, i, j);
+            }
+        }
+    L2 = sqrt(L2);
+    *L2_norm = L2;
+    if (L2_norm_relative) {
+        double rhs = 0.0;
+        for (int j = g->jlo; j <= g->jhi; ++j)
+            for (int i = g->ilo; i <= g->ihi; ++i) {
+                if (!S2(active_vertex, i, j)) continue;
+                for (int k = 1; k <= nz; ++k)
+                    rhs = rhs + V3(bu, k, i, j) * V3(bu, k, i, j) + V3(bv, k, i, j) * V3(bv, k, i, j);
+            }
+        rhs = sqrt(rhs);
+        *L2_norm_relative = rhs > 0.0 ? L2 / rhs : 0.0;
+    }
+}
+
+/* compute_residual_velocity_3d (velo_higher:9111-9234); MEANU (2) is not parallel-safe
+ * in the reference and is restated the same way */
+double orc_residual_velocity_3d(const orc_grid *g, int whichresid,
+                                const double *uvel, const double *vvel,
+                                const double *usav, const double *vsav)
+{
+    const int nx = g->nx, nz = g->nz;
+    double r = 0.0;
+    long count = 0;
+    int kmax = (whichresid == 1 || whichresid == 2) ? nz - 1 : nz;
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i)
+            for (int k = 1; k <= kmax; ++k) {
+                double u = V3(uvel, k, i, j), v = V3(vvel, k, i, j);
+                double speed = sqrt(u * u + v * v);
+                if (speed != 0.0) {
+                    double uo = V3(usav, k, i, j), vo = V3(vsav, k, i, j);
+                    double old = sqrt(uo * uo + vo * vo);
+                    double d = fabs((old - speed) / speed);
+                    if (whichresid == 2) { count++; r = r + d; }
+                    else if (d > r) r = d;
+                }
+            }
+    if (whichresid == 2 && count > 0) r = r / count;
+    return r;
+}
+
+/* load_vector_gravity, HO_ASSEMBLE_TAUD_LOCAL (velo_higher:4397-4428) */
+void orc_load_gravity_local(const orc_grid *g, const double *stagwbndsigma,
+                            double dx, double dy, const int32_t *active_vertex,
+                            const double *stagthck, const double *dusrf_dx,
+                            const double *dusrf_dy, double *loadu, double *loadv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k) {
+                double dz = S2(stagthck, i, j) * (stagwbndsigma[k] - stagwbndsigma[k - 1]);
+                V3(loadu, k, i, j) = V3(loadu, k, i, j) - RHOI * GRAV * dx * dy * dz / VOL0 * S2(dusrf_dx, i, j);
+                V3(loadv, k, i, j) = V3(loadv, k, i, j) - RHOI * GRAV * dx * dy * dz / VOL0 * S2(dusrf_dy, i, j);
+            }
+        }
+}
+
+/* load_vector_gravity, standard FE assembly (velo_higher:4436-4528) */
+void orc_load_gravity_standard(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                               const double *xV, const double *yV,
+                               const double *stagusrf, const double *stagthck,
+                               const double *dusrf_dx, const double *dusrf_dy,
+                               double *loadu, double *loadv)
+{
+    build_tables();
+    const int nx = g->nx, ny = g->ny, nz = g->nz, nh = g->nhalo;
+    for (int j = nh + 1; j <= ny - nh + 1; ++j)
+        for (int i = nh + 1; i <= nx - nh + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                double x[8], y[8], z[8], sx[8], sy[8];
+                for (int n = 0; n < 8; ++n) {
+                    int in = i - 1 + NODE_IX[n], jn = j - 1 + NODE_JY[n], kn = k + NODE_KL[n];
+                    x[n] = S2(xV, in, jn); y[n] = S2(yV, in, jn);
+                    z[n] = S2(stagusrf, in, jn) - sigma[kn - 1] * S2(stagthck, in, jn);
+                    sx[n] = S2(dusrf_dx, in, jn); sy[n] = S2(dusrf_dy, in, jn);
+                }
+                for (int p = 0; p < 8; ++p) {
+                    double ddx[8], ddy[8], ddz[8], detJ;
+                    orc_basis_derivatives(x, y, z, p, ddx, ddy, ddz, &detJ);
+                    double qx = 0.0, qy = 0.0;
+                    for (int n = 0; n < 8; ++n) { qx = qx + T_phi[p][n] * sx[n]; qy = qy + T_phi[p][n] * sy[n]; }
+                    for (int n = 0; n < 8; ++n) {
+                        int in = i - 1 + NODE_IX[n], jn = j - 1 + NODE_JY[n], kn = k + NODE_KL[n];
+                        V3(loadu, kn, in, jn) = V3(loadu, kn, in, jn) - RHOI * GRAV * 1.0 * detJ / VOL0 * qx * T_phi[p][n];
+                        V3(loadv, kn, in, jn) = V3(loadv, kn, in, jn) - RHOI * GRAV * 1.0 * detJ / VOL0 * qy * T_phi[p][n];
+                    }
+                }
+            }
+        }
+}
+
+/* matvec_multiply_structured_3d (pcg:2533-2647) */
+void orc_matvec_3d(const orc_grid *g, const int32_t *active_vertex,
+                   const double *Auu, const double *Auv,
+                   const double *Avu, const double *Avv,
+                   const double *xu, const double *xv, double *yu, double *yv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    size_t nv = (size_t)nz * (nx - 1) * (ny - 1);
+    memset(yu, 0, nv * sizeof(double));
+    memset(yv, 0, nv * sizeof(double));
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            for (int k = 1; k <= nz; ++k)
+                row_product(nx, ny, nz, k, i, j, Auu, Auv, Avu, Avv, xu, xv,
+                            &V3(yu, k, i, j), &V3(yv, k, i, j));
+        }
+}
+
+/* global_sum_staggered_3d_real8 (pcg:2333-2379): owned vertices, all k, w1 + w2 */
+static double gsum2(const orc_grid *g, const double *w1, const double *w2)
+{
+    const int nx = g->nx, nz = g->nz;
+    double s = 0.0;
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i)
+            for (int k = 1; k <= nz; ++k) s = s + V3(w1, k, i, j) + V3(w2, k, i, j);
+    return s;
+}
+
+/* tridiag_solver (pcg:2852-2904), Thomas algorithm without pivoting */
+void orc_tridiag(int n, const double *sb, const double *d, const double *sp,
+                 const double *rhs, double *soln)
+{
+    double gam[64];
+    double beta = d[0];
+    soln[0] = rhs[0] / beta;
+    for (int k = 1; k < n; ++k) {
+        gam[k] = sp[k - 1] / beta;
+        beta = d[k] - sb[k] * gam[k];
+        soln[k] = (rhs[k] - sb[k] * soln[k - 1]) / beta;
+    }
+    for (int k = n - 2; k >= 0; --k) soln[k] = soln[k] - gam[k + 1] * soln[k + 1];
+}
+
+/* setup_preconditioner_3d (pcg:2150-2265) + the precond branches of the PCG loops
+ * (pcg:300-365) + easy_sia_solver (pcg:2759-2848).  M arrays are (-1:1, nz, nx-1, ny-1). */
+typedef struct { double *Adu, *Adv, *Muu, *Mvv; } orc_pc;
+
+static void pc_setup(const orc_grid *g, int precond, const double *Auu, const double *Avv, orc_pc *pc)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    size_t nv = (size_t)nz * (nx - 1) * (ny - 1);
+    pc->Adu = calloc(nv, sizeof(double)); pc->Adv = calloc(nv, sizeof(double));
+    pc->Muu = calloc(3 * nv, sizeof(double)); pc->Mvv = calloc(3 * nv, sizeof(double));
+    for (size_t q = 0; q < nv; ++q) {
+        if (precond == 1) {
+            pc->Adu[q] = Auu[27 * q + IDX_A(0, 0, 0) - 1];
+            pc->Adv[q] = Avv[27 * q + IDX_A(0, 0, 0) - 1];
+        } else if (precond == 2) {
+            for (int o = -1; o <= 1; ++o) {
+                pc->Muu[3 * q + o + 1] = Auu[27 * q + IDX_A(0, 0, o) - 1];
+                pc->Mvv[3 * q + o + 1] = Avv[27 * q + IDX_A(0, 0, o) - 1];
+            }
+        }
+    }
+}
+
+static void pc_free(orc_pc *pc) { free(pc->Adu); free(pc->Adv); free(pc->Muu); free(pc->Mvv); }
+
+static void sia_solve(const orc_grid *g, const int32_t *active_vertex, const double *M,
+                      const double *r, double *z)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    double sb[64], d[64], sp[64], rhs[64], soln[64];
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            size_t base = (size_t)nz * ((i - 1) + (size_t)(nx - 1) * (j - 1));
+            for (int k = 0; k < nz; ++k) {
+                rhs[k] = r[base + k];
+                sb[k] = (k == 0) ? 0.0 : M[3 * (base + k) + 0];
+                d[k] = M[3 * (base + k) + 1];
+                sp[k] = (k == nz - 1) ? 0.0 : M[3 * (base + k) + 2];
+            }
+            orc_tridiag(nz, sb, d, sp, rhs, soln);
+            for (int k = 0; k < nz; ++k) z[base + k] = soln[k];
+        }
+}
+
+static void pc_apply(const orc_grid *g, int precond, const int32_t *active_vertex, const orc_pc *pc,
+                     const double *ru, const double *rv, double *zu, double *zv)
+{
+    size_t nv = (size_t)g->nz * (g->nx - 1) * (g->ny - 1);
+    if (precond == 0) {
+        memcpy(zu, ru, nv * sizeof(double));
+        memcpy(zv, rv, nv * sizeof(double));
+    } else if (precond == 1) {
+        for (size_t q = 0; q < nv; ++q) {
+            zu[q] = pc->Adu[q] != 0.0 ? ru[q] / pc->Adu[q] : 0.0;
+            zv[q] = pc->Adv[q] != 0.0 ? rv[q] / pc->Adv[q] : 0.0;
+        }
+    } else if (precond == 2) {
+        sia_solve(g, active_vertex, pc->Muu, ru, zu);
+        sia_solve(g, active_vertex, pc->Mvv, rv, zv);
+    }
+}
+
+/* pcg_solver_standard_3d (pcg:84-549) */
+int orc_pcg_standard_3d(const orc_grid *g, const int32_t *av,
+                        const double *Auu, const double *Auv,
+                        const double *Avu, const double *Avv,
+                        const double *bu, const double *bv,
+                        double *xu, double *xv, int precond,
+                        int maxiters, double tolerance,
+                        double *err_out, int *niters_out)
+{
+    size_t nv = (size_t)g->nz * (g->nx - 1) * (g->ny - 1);
+    double *ru = calloc(nv, 8), *rv = calloc(nv, 8), *du = calloc(nv, 8), *dv = calloc(nv, 8);
+    double *qu = calloc(nv, 8), *qv = calloc(nv, 8), *zu = calloc(nv, 8), *zv = calloc(nv, 8);
+    double *wu = calloc(nv, 8), *wv = calloc(nv, 8);
+    orc_pc pc;
+    pc_setup(g, precond, Auu, Avv, &pc);
+    int status = ORC_OK;
+    double err = 0.0;
+    orc_halo(g, g->nz, xu); orc_halo(g, g->nz, xv);
+    orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
+    for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
+    int niters = maxiters;
+    double eta0 = 1.0, eta1, eta2, alpha, beta;
+    memset(du, 0, nv * 8); memset(dv, 0, nv * 8);
+    memset(zu, 0, nv * 8); memset(zv, 0, nv * 8);
+    for (size_t q = 0; q < nv; ++q) { wu[q] = bu[q] * bu[q]; wv[q] = bv[q] * bv[q]; }
+    double L2_rhs = sqrt(gsum2(g, wu, wv));
+    for (int n = 1; n <= maxiters; ++n) {
+        pc_apply(g, precond, av, &pc, ru, rv, zu, zv);
+        for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * zu[q]; wv[q] = rv[q] * zv[q]; }
+        eta1 = gsum2(g, wu, wv);
+        if (eta1 != eta1) { status = ORC_ERR_NAN; break; }
+        beta = eta1 / eta0;
+        for (size_t q = 0; q < nv; ++q) { du[q] = zu[q] + beta * du[q]; dv[q] = zv[q] + beta * dv[q]; }
+        orc_halo(g, g->nz, du); orc_halo(g, g->nz, dv);
+        orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, du, dv, qu, qv);
+        eta0 = eta1;
+        for (size_t q = 0; q < nv; ++q) { wu[q] = du[q] * qu[q]; wv[q] = dv[q] * qv[q]; }
+        eta2 = gsum2(g, wu, wv);
+        alpha = eta1 / eta2;
+        if (alpha != alpha) { status = ORC_ERR_NAN; break; }
+        for (size_t q = 0; q < nv; ++q) {
+            xu[q] = xu[q] + alpha * du[q]; xv[q] = xv[q] + alpha * dv[q];
+            ru[q] = ru[q] - alpha * qu[q]; rv[q] = rv[q] - alpha * qv[q];
+        }
+        if (n % 5 == 0) {
+            orc_halo(g, g->nz, xu); orc_halo(g, g->nz, xv);
+            orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
+            for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
+            for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * ru[q]; wv[q] = rv[q] * rv[q]; }
+            err = sqrt(gsum2(g, wu, wv)) / L2_rhs;
+            if (err < tolerance) { niters = n; break; }
+        }
+    }
+    *err_out = err;
+    *niters_out = niters;
+    pc_free(&pc);
+    free(ru); free(rv); free(du); free(dv); free(qu); free(qv); free(zu); free(zv); free(wu); free(wv);
+    return status;
+}
+
+/* pcg_solver_chrongear_3d (pcg:965-1610) */
+int orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *av,
+                         const double *Auu, const double *Auv,
+                         const double *Avu, const double *Avv,
+                         const double *bu, const double *bv,
+                         double *xu, double *xv, int precond,
+                         int maxiters, double tolerance,
+                         double *err_out, int *niters_out)
+{
+    const int nz = g->nz;
+    size_t nv = (size_t)nz * (g->nx - 1) * (g->ny - 1);
+    double *ru = calloc(nv, 8), *rv = calloc(nv, 8), *du = calloc(nv, 8), *dv = calloc(nv, 8);
+    double *zu = calloc(nv, 8), *zv = calloc(nv, 8), *qu = calloc(nv, 8), *qv = calloc(nv, 8);
+    double *Azu = calloc(nv, 8), *Azv = calloc(nv, 8), *wu = calloc(nv, 8), *wv = calloc(nv, 8);
+    double *w2u = calloc(nv, 8), *w2v = calloc(nv, 8);
+    orc_pc pc;
+    pc_setup(g, precond, Auu, Avv, &pc);
+    int status = ORC_OK, niters = maxiters;
+    double err = 0.0;
+    for (size_t q = 0; q < nv; ++q) { wu[q] = bu[q] * bu[q]; wv[q] = bv[q] * bv[q]; }
+    double L2_rhs = sqrt(gsum2(g, wu, wv));
+    orc_halo(g, nz, xu); orc_halo(g, nz, xv);
+    orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
+    for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
+    orc_halo(g, nz, ru); orc_halo(g, nz, rv);
+    pc_apply(g, precond, av, &pc, ru, rv, zu, zv);
+    for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * zu[q]; wv[q] = rv[q] * zv[q]; }
+    memcpy(du, zu, nv * 8); memcpy(dv, zv, nv * 8);
+    orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, du, dv, qu, qv);
+    for (size_t q = 0; q < nv; ++q) { w2u[q] = du[q] * qu[q]; w2v[q] = dv[q] * qv[q]; }
+    double rho_old = gsum2(g, wu, wv);
+    double sigma = gsum2(g, w2u, w2v);
+    orc_halo(g, nz, qu); orc_halo(g, nz, qv);
+    double alpha = rho_old / sigma;
+    if (alpha != alpha) status = ORC_ERR_NAN;
+    if (!status) {
+        for (size_t q = 0; q < nv; ++q) {
+            xu[q] = xu[q] + alpha * du[q]; xv[q] = xv[q] + alpha * dv[q];
+            ru[q] = ru[q] - alpha * qu[q]; rv[q] = rv[q] - alpha * qv[q];
+        }
+        for (int n = 2; n <= maxiters; ++n) {
+            pc_apply(g, precond, av, &pc, ru, rv, zu, zv);
+            orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, zu, zv, Azu, Azv);
+            for (size_t q = 0; q < nv; ++q) {
+                wu[q] = ru[q] * zu[q]; wv[q] = rv[q] * zv[q];
+                w2u[q] = Azu[q] * zu[q]; w2v[q] = Azv[q] * zv[q];
+            }
+            double rho = gsum2(g, wu, wv), delta = gsum2(g, w2u, w2v);
+            orc_halo(g, nz, Azu); orc_halo(g, nz, Azv);
+            double beta = rho / rho_old;
+            sigma = delta - beta * rho / alpha;
+            alpha = rho / sigma;
+            rho_old = rho;
+            if (alpha != alpha) { status = ORC_ERR_NAN; break; }
+            for (size_t q = 0; q < nv; ++q) {
+                du[q] = zu[q] + beta * du[q]; dv[q] = zv[q] + beta * dv[q];
+                qu[q] = Azu[q] + beta * qu[q]; qv[q] = Azv[q] + beta * qv[q];
+                xu[q] = xu[q] + alpha * du[q]; xv[q] = xv[q] + alpha * dv[q];
+                ru[q] = ru[q] - alpha * qu[q]; rv[q] = rv[q] - alpha * qv[q];
+            }
+            if (n % 5 == 0) {
+                orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
+                for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
+                for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * ru[q]; wv[q] = rv[q] * rv[q]; }
+                err = sqrt(gsum2(g, wu, wv)) / L2_rhs;
+                if (err < tolerance) { niters = n; break; }
+                orc_halo(g, nz, ru); orc_halo(g, nz, rv);
+            }
+        }
+    }
+    *err_out = err;
+    *niters_out = niters;
+    pc_free(&pc);
+    free(ru); free(rv); free(du); free(dv); free(zu); free(zv); free(qu); free(qv);
+    free(Azu); free(Azv); free(wu); free(wv); free(w2u); free(w2v);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Picard loop, 3D BP branch of glissade_velo_higher_solve                  */
+/* (velo_higher:2003-2016, 2206-3727)                                       */
+/* ------------------------------------------------------------------------ */
+int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
+                     const double *sigma, const int32_t *active_cell,
+                     const int32_t *active_vertex,
+                     const double *xV, const double *yV,
+                     const double *stagusrf, const double *stagthck,
+                     const double *flwafact, const double *beta_internal,
+                     const int32_t *umask, const int32_t *vmask,
+                     const double *loadu, const double *loadv,
+                     double *uvel, double *vvel, double *efvs,
+                     double *bu, double *bv, double *resid_u, double *resid_v)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    size_t nv = (size_t)nz * (nx - 1) * (ny - 1), na = 27 * nv;
+    double *Auu = malloc(na * 8), *Auv = malloc(na * 8), *Avu = malloc(na * 8), *Avv = malloc(na * 8);
+    double *usav = malloc(nv * 8), *vsav = malloc(nv * 8);
+    const double L2_target = 1.0e-4, L2_target_rel = 1.0e-7, resid_target = 1.0e-4;
+    double crit = 1.0e10, target = 1.0e-12;
+    double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
+    int counter = 0, status = ORC_OK;
+    o->total_linear_iters = 0;
+    while (crit >= target && counter < o->maxiter_nonlinear) {
+        counter++;
+        memcpy(usav, uvel, nv * 8); memcpy(vsav, vvel, nv * 8);
+        status = orc_assemble_3d(g, sigma, active_cell, xV, yV, uvel, vvel, stagusrf, stagthck,
+                                 flwafact, o->whichapprox, efvs, o->whichefvs, o->efvs_constant,
+                                 Auu, Auv, Avu, Avv);
+        if (status) break;
+        if (o->whichbabc != 6)   /* HO_BABC_NO_SLIP */
+            orc_basal_sliding_local(g, o->dx, o->dy, active_vertex, beta_internal, Auu, Avv);
+        memcpy(bu, loadu, nv * 8); memcpy(bv, loadv, nv * 8);
+        orc_dirichlet_3d(g, active_vertex, umask, vmask, uvel, vvel, Auu, Auv, Avu, Avv, bu, bv);
+        orc_halo(g, 27 * nz, Auu); orc_halo(g, 27 * nz, Auv);
+        orc_halo(g, 27 * nz, Avu); orc_halo(g, 27 * nz, Avv);
+        orc_halo(g, nz, bu); orc_halo(g, nz, bv);
+        orc_check_symmetry_3d(g, active_vertex, Auu, Auv, Avu, Avv, NULL);
+        long nnz = orc_count_nonzeros_3d(g, active_vertex, Auu, Auv, Avu, Avv);
+        if (nnz == 0) {
+            memset(resid_u, 0, nv * 8); memset(resid_v, 0, nv * 8);
+            memset(bu, 0, nv * 8); memset(bv, 0, nv * 8);
+            memset(uvel, 0, nv * 8); memset(vvel, 0, nv * 8);
+            L2 = 0.0;
+            break;
+        }
+        orc_residual_vector_3d(g, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
+                               resid_u, resid_v, &L2, &L2rel);
+        double err; int niters;
+        if (o->whichsparse == 3)
+            status = orc_pcg_chrongear_3d(g, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
+                                          o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                          &err, &niters);
+        else
+            status = orc_pcg_standard_3d(g, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
+                                         o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                         &err, &niters);
+        if (status) break;
+        o->total_linear_iters += niters;
+        orc_halo(g, nz, uvel); orc_halo(g, nz, vvel);
+        resid_velo = orc_residual_velocity_3d(g, o->whichresid, uvel, vvel, usav, vsav);
+        if (o->whichresid == 3) { crit = L2; target = L2_target; }
+        else if (o->whichresid == 4) { crit = L2rel; target = L2_target_rel; }
+        else { crit = resid_velo; target = resid_target; }
+    }
+    o->picard_iters = counter;
+    o->final_L2_norm = L2;
+    o->final_resid_velo = resid_velo;
+    o->status = status;
+    free(Auu); free(Auv); free(Avu); free(Avv); free(usav); free(vsav);
+    return status;
+}

@@ -1,0 +1,167 @@
+/*
+ * cism_oracle.h -- CPU oracle for the Glissade Blatter-Pattyn velocity solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm, used by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg as the checker and the timed CPU baseline.  The product
+ * (libcism_gpu.so) never links, loads or calls it.
+ *
+ * All arrays use the reference's Fortran (column-major) layouts with 1-based
+ * logical indices:
+ *   matrix blocks  A(27, nz, nx-1, ny-1)     -> Auu, Auv, Avu, Avv
+ *   node vectors   v(nz, nx-1, ny-1)
+ *   vertex fields  s(nx-1, ny-1)
+ *   cell fields    c(nx, ny)
+ *   layer fields   f(nz-1, nx, ny)           (flwafact, efvs)
+ * Fortran LOGICAL arrays are passed as int32 (0 / 1).
+ *
+ * Parity pins (which check fixes which function) are listed in the header of
+ * cism_oracle.c.
+ */
+#ifndef CISM_ORACLE_H
+#define CISM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nz, nhalo;   /* scalar grid (incl. halo), velocity levels, halo width */
+    int ilo, ihi, jlo, jhi;  /* owned staggered vertices, 1-based inclusive          */
+    int periodic;            /* 1: periodic global BC, 0: outflow                      */
+} orc_grid;
+
+/* status codes shared with the GPU library's C-ABI */
+#define ORC_OK              0
+#define ORC_ERR_SINGULAR    2   /* Jacobian determinant is zero                 */
+#define ORC_ERR_ASYMMETRIC  3   /* element matrix asymmetric beyond 1e-10       */
+#define ORC_ERR_NAN         4   /* PCG alpha is NaN                             */
+#define ORC_ERR_ARG         5   /* invalid argument                             */
+
+/* parallel_slap.F90:563-686 bounds for nhalo=2, staggered halo (2,1) */
+void orc_grid_init(orc_grid *g, int nx, int ny, int nz, int periodic);
+
+/* glissade_velo_higher_init tables (velo_higher:270-650) */
+void orc_tables(double *phi, double *dxr, double *dyr, double *dzr,
+                int *ishift, int *jshift, int *kshift, int *indxA);
+
+int  orc_basis_derivatives(const double *x, const double *y, const double *z, int p,
+                           double *dx, double *dy, double *dz, double *detJ);
+
+double orc_efvs_qp(int whichefvs, int whichapprox, double efvs_constant,
+                   const double *dphx, const double *dphy, const double *dphz,
+                   const double *u, const double *v, double flwafact);
+
+int  orc_element_matrix(const orc_grid *g, int i, int j, int k,
+                        const double *sigma, const double *xV, const double *yV,
+                        const double *uvel, const double *vvel,
+                        const double *stagusrf, const double *stagthck,
+                        const double *flwafact, int whichapprox, int whichefvs,
+                        double efvs_constant,
+                        double *Kuu, double *Kuv, double *Kvu, double *Kvv,
+                        double *efvs_elem);
+
+int  orc_assemble_3d(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                     const double *xV, const double *yV,
+                     const double *uvel, const double *vvel,
+                     const double *stagusrf, const double *stagthck,
+                     const double *flwafact, int whichapprox,
+                     double *efvs, int whichefvs, double efvs_constant,
+                     double *Auu, double *Auv, double *Avu, double *Avv);
+
+void orc_basal_sliding_local(const orc_grid *g, double dx, double dy,
+                             const int32_t *active_vertex, const double *beta,
+                             double *Auu, double *Avv);
+
+void orc_dirichlet_3d(const orc_grid *g, const int32_t *active_vertex,
+                      const int32_t *umask, const int32_t *vmask,
+                      const double *uvel, const double *vvel,
+                      double *Auu, double *Auv, double *Avu, double *Avv,
+                      double *bu, double *bv);
+
+/* staggered_parallel_halo for arrays of shape (lead, nx-1, ny-1) */
+void orc_halo(const orc_grid *g, int lead, double *a);
+
+int  orc_check_symmetry_3d(const orc_grid *g, const int32_t *active_vertex,
+                           double *Auu, double *Auv, double *Avu, double *Avv,
+                           double *maxdiff);
+
+long orc_count_nonzeros_3d(const orc_grid *g, const int32_t *active_vertex,
+                           const double *Auu, const double *Auv,
+                           const double *Avu, const double *Avv);
+
+void orc_residual_vector_3d(const orc_grid *g, const int32_t *active_vertex,
+                            const double *Auu, const double *Auv,
+                            const double *Avu, const double *Avv,
+                            const double *bu, const double *bv,
+                            const double *uvel, const double *vvel,
+                            double *resid_u, double *resid_v,
+                            double *L2_norm, double *L2_norm_relative);
+
+double orc_residual_velocity_3d(const orc_grid *g, int whichresid,
+                                const double *uvel, const double *vvel,
+                                const double *usav, const double *vsav);
+
+void orc_load_gravity_local(const orc_grid *g, const double *stagwbndsigma,
+                            double dx, double dy, const int32_t *active_vertex,
+                            const double *stagthck, const double *dusrf_dx,
+                            const double *dusrf_dy, double *loadu, double *loadv);
+
+void orc_load_gravity_standard(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                               const double *xV, const double *yV,
+                               const double *stagusrf, const double *stagthck,
+                               const double *dusrf_dx, const double *dusrf_dy,
+                               double *loadu, double *loadv);
+
+void orc_matvec_3d(const orc_grid *g, const int32_t *active_vertex,
+                   const double *Auu, const double *Auv,
+                   const double *Avu, const double *Avv,
+                   const double *xu, const double *xv, double *yu, double *yv);
+
+void orc_tridiag(int n, const double *sb, const double *d, const double *sp,
+                 const double *rhs, double *soln);
+
+int  orc_pcg_standard_3d(const orc_grid *g, const int32_t *active_vertex,
+                         const double *Auu, const double *Auv,
+                         const double *Avu, const double *Avv,
+                         const double *bu, const double *bv,
+                         double *xu, double *xv, int precond,
+                         int maxiters, double tolerance,
+                         double *err, int *niters);
+
+int  orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *active_vertex,
+                          const double *Auu, const double *Auv,
+                          const double *Avu, const double *Avv,
+                          const double *bu, const double *bv,
+                          double *xu, double *xv, int precond,
+                          int maxiters, double tolerance,
+                          double *err, int *niters);
+
+/* Picard loop of glissade_velo_higher_solve, 3D BP branch (velo_higher:2206-3727).
+ * Inputs are the arrays the reference computes before the loop. */
+typedef struct {
+    int whichbabc, whichefvs, whichresid, whichsparse, whichprecond;
+    int whichapprox, maxiter_nonlinear, linear_maxiters;
+    double efvs_constant, linear_tolerance, dx, dy;
+    /* outputs */
+    int picard_iters, total_linear_iters, status;
+    double final_L2_norm, final_resid_velo;
+} orc_picard_opts;
+
+int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
+                     const double *sigma, const int32_t *active_cell,
+                     const int32_t *active_vertex,
+                     const double *xV, const double *yV,
+                     const double *stagusrf, const double *stagthck,
+                     const double *flwafact, const double *beta_internal,
+                     const int32_t *umask_dirichlet, const int32_t *vmask_dirichlet,
+                     const double *loadu, const double *loadv,
+                     double *uvel, double *vvel, double *efvs,
+                     double *bu, double *bv, double *resid_u, double *resid_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

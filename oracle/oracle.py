@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  Arrays are numpy
+arrays in the reference's Fortran layouts (see cism_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+class Grid(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nx", "ny", "nz", "nhalo", "ilo", "ihi", "jlo", "jhi",
+                                       "periodic")]
+
+
+class PicardOpts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("whichbabc", "whichefvs", "whichresid", "whichsparse",
+                                       "whichprecond", "whichapprox", "maxiter_nonlinear",
+                                       "linear_maxiters")] + \
+               [(n, C.c_double) for n in ("efvs_constant", "linear_tolerance", "dx", "dy")] + \
+               [(n, C.c_int) for n in ("picard_iters", "total_linear_iters", "status")] + \
+               [(n, C.c_double) for n in ("final_L2_norm", "final_resid_velo")]
+
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int32)
+G = C.POINTER(Grid)
+
+
+def _declare(L):
+    L.orc_grid_init.argtypes = [G, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_tables.argtypes = [D, D, D, D, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.orc_basis_derivatives.argtypes = [D, D, D, C.c_int, D, D, D, D]
+    L.orc_efvs_qp.restype = C.c_double
+    L.orc_efvs_qp.argtypes = [C.c_int, C.c_int, C.c_double, D, D, D, D, D, C.c_double]
+    L.orc_element_matrix.argtypes = [G, C.c_int, C.c_int, C.c_int, D, D, D, D, D, D, D, D,
+                                     C.c_int, C.c_int, C.c_double, D, D, D, D, D]
+    L.orc_assemble_3d.argtypes = [G, D, I, D, D, D, D, D, D, D, C.c_int, D, C.c_int,
+                                  C.c_double, D, D, D, D]
+    L.orc_basal_sliding_local.argtypes = [G, C.c_double, C.c_double, I, D, D, D]
+    L.orc_dirichlet_3d.argtypes = [G, I, I, I, D, D, D, D, D, D, D, D]
+    L.orc_halo.argtypes = [G, C.c_int, D]
+    L.orc_check_symmetry_3d.argtypes = [G, I, D, D, D, D, D]
+    L.orc_count_nonzeros_3d.restype = C.c_long
+    L.orc_count_nonzeros_3d.argtypes = [G, I, D, D, D, D]
+    L.orc_residual_vector_3d.argtypes = [G, I, D, D, D, D, D, D, D, D, D, D, D, D]
+    L.orc_residual_velocity_3d.restype = C.c_double
+    L.orc_residual_velocity_3d.argtypes = [G, C.c_int, D, D, D, D]
+    L.orc_load_gravity_local.argtypes = [G, D, C.c_double, C.c_double, I, D, D, D, D, D]
+    L.orc_load_gravity_standard.argtypes = [G, D, I, D, D, D, D, D, D, D, D]
+    L.orc_matvec_3d.argtypes = [G, I, D, D, D, D, D, D, D, D]
+    L.orc_tridiag.argtypes = [C.c_int, D, D, D, D, D]
+    pcg = [G, I, D, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, D, C.POINTER(C.c_int)]
+    L.orc_pcg_standard_3d.argtypes = pcg
+    L.orc_pcg_chrongear_3d.argtypes = pcg
+    L.orc_picard_solve.argtypes = [G, C.POINTER(PicardOpts), D, I, I, D, D, D, D, D, D, I, I,
+                                   D, D, D, D, D, D, D, D, D]
+
+
+def dp(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim <= 1), a.flags
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    assert a.dtype == np.int32 and (a.flags.f_contiguous or a.ndim <= 1)
+    return a.ctypes.data_as(I)
+
+
+def grid(nx, ny, nz, periodic=True):
+    g = Grid()
+    lib().orc_grid_init(C.byref(g), nx, ny, nz, 1 if periodic else 0)
+    return g
+
+
+def grid_of(p):
+    return grid(p.nx, p.ny, p.nz, p.periodic)
+
+
+def tables():
+    phi, dxr, dyr, dzr = (np.zeros((8, 8), order="F") for _ in range(4))
+    ish, jsh, ksh = (np.zeros((8, 8), dtype=np.int32, order="F") for _ in range(3))
+    ind = np.zeros((3, 3, 3), dtype=np.int32, order="F")
+    P = C.POINTER(C.c_int)
+    lib().orc_tables(dp(phi), dp(dxr), dp(dyr), dp(dzr), ish.ctypes.data_as(P),
+                     jsh.ctypes.data_as(P), ksh.ctypes.data_as(P), ind.ctypes.data_as(P))
+    return dict(phi=phi, dxr=dxr, dyr=dyr, dzr=dzr, ishift=ish, jshift=jsh, kshift=ksh, indxA=ind)
+
+
+def basis_derivatives(x, y, z, p):
+    dx, dy, dz = np.zeros(8), np.zeros(8), np.zeros(8)
+    det = C.c_double()
+    st = lib().orc_basis_derivatives(dp(np.ascontiguousarray(x, float)), dp(np.ascontiguousarray(y, float)),
+                                     dp(np.ascontiguousarray(z, float)), p, dp(dx), dp(dy), dp(dz),
+                                     C.byref(det))
+    return st, dx, dy, dz, det.value
+
+
+def efvs_qp(whichefvs, whichapprox, efvs_constant, dphx, dphy, dphz, u, v, flwafact):
+    a = [np.ascontiguousarray(t, float) for t in (dphx, dphy, dphz, u, v)]
+    return lib().orc_efvs_qp(whichefvs, whichapprox, efvs_constant, *[dp(t) for t in a], flwafact)
+
+
+def mat_zeros(p):
+    return [np.zeros((27, p.nz, p.nx - 1, p.ny - 1), order="F") for _ in range(4)]
+
+
+def vec_zeros(p):
+    return np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
+
+
+def element_matrix(p, i, j, k, uvel=None, vvel=None):
+    uvel = p.uvel if uvel is None else uvel
+    vvel = p.vvel if vvel is None else vvel
+    K = [np.zeros((8, 8), order="F") for _ in range(4)]
+    ev = C.c_double()
+    g = grid_of(p)
+    st = lib().orc_element_matrix(C.byref(g), i, j, k, dp(p.sigma), dp(p.xVertex), dp(p.yVertex),
+                                  dp(uvel), dp(vvel), dp(p.stagusrf), dp(p.stagthck),
+                                  dp(p.flwafact), p.whichapprox, p.whichefvs, p.efvs_constant,
+                                  *[dp(k_) for k_ in K], C.byref(ev))
+    return st, K, ev.value
+
+
+def assemble(p, uvel=None, vvel=None):
+    uvel = p.uvel if uvel is None else uvel
+    vvel = p.vvel if vvel is None else vvel
+    A = mat_zeros(p)
+    efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+    g = grid_of(p)
+    st = lib().orc_assemble_3d(C.byref(g), dp(p.sigma), ip(p.active_cell), dp(p.xVertex),
+                               dp(p.yVertex), dp(uvel), dp(vvel), dp(p.stagusrf), dp(p.stagthck),
+                               dp(p.flwafact), p.whichapprox, dp(efvs), p.whichefvs,
+                               p.efvs_constant, *[dp(a) for a in A])
+    return st, A, efvs
+
+
+def basal_local(p, A):
+    g = grid_of(p)
+    lib().orc_basal_sliding_local(C.byref(g), p.dx, p.dy, ip(p.active_vertex), dp(p.beta_internal),
+                                  dp(A[0]), dp(A[3]))
+
+
+def dirichlet(p, A, bu, bv, uvel=None, vvel=None):
+    uvel = p.uvel if uvel is None else uvel
+    vvel = p.vvel if vvel is None else vvel
+    g = grid_of(p)
+    lib().orc_dirichlet_3d(C.byref(g), ip(p.active_vertex), ip(p.umask_dirichlet),
+                           ip(p.vmask_dirichlet), dp(uvel), dp(vvel), *[dp(a) for a in A],
+                           dp(bu), dp(bv))
+
+
+def halo(p, a):
+    g = grid_of(p)
+    lead = int(np.prod(a.shape[:-2])) if a.ndim > 2 else 1
+    lib().orc_halo(C.byref(g), lead, dp(a))
+
+
+def check_symmetry(p, A):
+    g = grid_of(p)
+    md = C.c_double()
+    nw = lib().orc_check_symmetry_3d(C.byref(g), ip(p.active_vertex), *[dp(a) for a in A],
+                                     C.byref(md))
+    return nw, md.value
+
+
+def count_nonzeros(p, A):
+    g = grid_of(p)
+    return lib().orc_count_nonzeros_3d(C.byref(g), ip(p.active_vertex), *[dp(a) for a in A])
+
+
+def residual_vector(p, A, bu, bv, uvel, vvel):
+    g = grid_of(p)
+    ru, rv = vec_zeros(p), vec_zeros(p)
+    L2, L2r = C.c_double(), C.c_double()
+    lib().orc_residual_vector_3d(C.byref(g), ip(p.active_vertex), *[dp(a) for a in A], dp(bu),
+                                 dp(bv), dp(uvel), dp(vvel), dp(ru), dp(rv), C.byref(L2),
+                                 C.byref(L2r))
+    return ru, rv, L2.value, L2r.value
+
+
+def residual_velocity(p, whichresid, u, v, us, vs):
+    g = grid_of(p)
+    return lib().orc_residual_velocity_3d(C.byref(g), whichresid, dp(u), dp(v), dp(us), dp(vs))
+
+
+def load_gravity(p, standard=False):
+    g = grid_of(p)
+    lu, lv = vec_zeros(p), vec_zeros(p)
+    if standard:
+        lib().orc_load_gravity_standard(C.byref(g), dp(p.sigma), ip(p.active_cell), dp(p.xVertex),
+                                        dp(p.yVertex), dp(p.stagusrf), dp(p.stagthck),
+                                        dp(p.dusrf_dx), dp(p.dusrf_dy), dp(lu), dp(lv))
+    else:
+        lib().orc_load_gravity_local(C.byref(g), dp(p.stagwbndsigma), p.dx, p.dy,
+                                     ip(p.active_vertex), dp(p.stagthck), dp(p.dusrf_dx),
+                                     dp(p.dusrf_dy), dp(lu), dp(lv))
+    return lu, lv
+
+
+def matvec(p, A, xu, xv, active_vertex=None):
+    g = grid_of(p)
+    av = p.active_vertex if active_vertex is None else active_vertex
+    yu, yv = vec_zeros(p), vec_zeros(p)
+    lib().orc_matvec_3d(C.byref(g), ip(av), *[dp(a) for a in A], dp(xu), dp(xv), dp(yu), dp(yv))
+    return yu, yv
+
+
+def tridiag(sb, d, sp, rhs):
+    n = len(d)
+    out = np.zeros(n)
+    lib().orc_tridiag(n, *[dp(np.ascontiguousarray(t, float)) for t in (sb, d, sp, rhs)], dp(out))
+    return out
+
+
+def pcg(p, A, bu, bv, xu, xv, precond, chrongear=True, maxiters=200, tolerance=1e-8):
+    """Runs the oracle PCG in place on xu, xv; returns (status, err, niters)."""
+    g = grid_of(p)
+    err, nit = C.c_double(), C.c_int()
+    f = lib().orc_pcg_chrongear_3d if chrongear else lib().orc_pcg_standard_3d
+    st = f(C.byref(g), ip(p.active_vertex), *[dp(a) for a in A], dp(bu), dp(bv), dp(xu), dp(xv),
+           precond, maxiters, tolerance, C.byref(err), C.byref(nit))
+    return st, err.value, nit.value
+
+
+def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
+           whichsparse=None, whichprecond=None, whichresid=None):
+    """Runs the oracle Picard loop from p.uvel/p.vvel (copied). Returns a result dict."""
+    g = grid_of(p)
+    o = PicardOpts()
+    o.whichbabc = p.whichbabc
+    o.whichefvs = p.whichefvs
+    o.whichresid = p.whichresid if whichresid is None else whichresid
+    o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+    o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+    o.whichapprox = p.whichapprox
+    o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter is None else maxiter
+    o.linear_maxiters = linear_maxiters
+    o.efvs_constant = p.efvs_constant
+    o.linear_tolerance = linear_tolerance
+    o.dx, o.dy = p.dx, p.dy
+    u, v = p.uvel.copy(order="F"), p.vvel.copy(order="F")
+    efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+    bu, bv, ru, rv = vec_zeros(p), vec_zeros(p), vec_zeros(p), vec_zeros(p)
+    st = lib().orc_picard_solve(C.byref(g), C.byref(o), dp(p.sigma), ip(p.active_cell),
+                                ip(p.active_vertex), dp(p.xVertex), dp(p.yVertex),
+                                dp(p.stagusrf), dp(p.stagthck), dp(p.flwafact),
+                                dp(p.beta_internal), ip(p.umask_dirichlet),
+                                ip(p.vmask_dirichlet), dp(loadu), dp(loadv), dp(u), dp(v),
+                                dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv))
+    return dict(status=st, uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
+                picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
+                L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)

@@ -1,0 +1,227 @@
+"""Parity of the HIP library (through its C ABI) with the CPU oracle.
+
+Bars: integer/index results bit-exact; floating point within the stated
+tolerances.  The HIP kernels are compiled with -ffp-contract=off and follow the
+oracle's operation order, so assembly, matvec and the Dirichlet update agree to
+the last bits except where pow() rounding differs (glibc vs device libm), which
+bounds the assembled coefficients at ~1e-15 relative.  Solver iterates differ
+only by the order of the global sums (tree vs sequential), which stays far
+below the 1e-8 relative L2 velocity bar.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from cism_amd import configs
+from helpers import assembled_system, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL_ASM = 1e-13      # assembled coefficients, max-norm relative to the block's max
+TOL_VEC = 1e-12      # matvec / residual vectors
+TOL_SOLVE = 1e-10    # PCG iterate after the same number of iterations
+TOL_PICARD = 1e-8    # north-star parity bar for FP64 velocities (relative L2)
+
+
+def problems():
+    return {
+        "dome": configs.dome(),
+        "ismip": configs.ismip_hom_a(ewn=24, nsn=24, nz=11, L=12000.0),
+        "grf": configs.grf_sheet(ewn=24, nsn=20, nz=7, dx=2000.0, corr=8000.0, seed=3),
+    }
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from cism_amd import gpu as G
+    G.lib()
+    return G
+
+
+@pytest.fixture(scope="module", params=["dome", "ismip", "grf"])
+def prob(request):
+    p = problems()[request.param]
+    print(f"\n[case] {p.name} ewn={p.ewn} nsn={p.nsn} nz={p.nz}", flush=True)
+    return p
+
+
+def test_library_exports(gpu):
+    L = gpu.lib()
+    for name in gpu.EXPORTED:
+        assert hasattr(L, name)
+
+
+def test_assembly(gpu, prob):
+    p = prob
+    rng = np.random.default_rng(7)
+    u = np.asfortranarray(50.0 * rng.standard_normal(p.uvel.shape))
+    v = np.asfortranarray(50.0 * rng.standard_normal(p.vvel.shape))
+    st, Ao, efo = O.assemble(p, u, v)
+    assert st == 0
+    Ag = O.mat_zeros(p)
+    efg = np.zeros_like(efo)
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.assemble_stiffness_matrix_3d(p.sigma, p.active_cell, p.xVertex, p.yVertex, u, v,
+                                         p.stagusrf, p.stagthck, p.flwafact, p.whichapprox, efg,
+                                         p.whichefvs, p.efvs_constant, *Ag)
+    for a, b in zip(Ag, Ao):
+        assert rel_max(a, b) < TOL_ASM
+    assert rel_max(efg, efo) < TOL_ASM
+    # identical sparsity: structural zeros stay exactly zero
+    for a, b in zip(Ag, Ao):
+        assert np.array_equal(a == 0.0, b == 0.0)
+
+
+def test_dirichlet_basal_symmetry_counts(gpu, prob):
+    p = prob
+    rng = np.random.default_rng(11)
+    u = np.asfortranarray(20.0 * rng.standard_normal(p.uvel.shape))
+    v = np.asfortranarray(20.0 * rng.standard_normal(p.vvel.shape))
+    st, A, _ = O.assemble(p, u, v)
+    lu, lv = O.load_gravity(p)
+    with gpu.Context.for_problem(p) as ctx:
+        # basal friction
+        Ab = [a.copy(order="F") for a in A]
+        O.basal_local(p, Ab)
+        Ag = [a.copy(order="F") for a in A]
+        ctx.basal_sliding_bc_3d(p.dx, p.dy, p.active_vertex, p.beta_internal, 1, Ag[0], Ag[3])
+        assert np.array_equal(Ag[0], Ab[0]) and np.array_equal(Ag[3], Ab[3])
+        # Dirichlet (exact: the same products and sums in the same order)
+        Ao = [a.copy(order="F") for a in Ab]
+        bo_u, bo_v = lu.copy(order="F"), lv.copy(order="F")
+        O.dirichlet(p, Ao, bo_u, bo_v, u, v)
+        Ag = [a.copy(order="F") for a in Ab]
+        bg_u, bg_v = lu.copy(order="F"), lv.copy(order="F")
+        ctx.dirichlet_boundary_conditions_3d(p.active_vertex, p.umask_dirichlet,
+                                             p.vmask_dirichlet, u, v, *Ag, bg_u, bg_v)
+        for a, b in zip(Ag, Ao):
+            assert np.array_equal(a, b)
+        assert np.array_equal(bg_u, bo_u) and np.array_equal(bg_v, bo_v)
+        # halo (periodic copies)
+        for a in Ao:
+            O.halo(p, a)
+        for a in Ag:
+            ctx.staggered_parallel_halo(a)
+        for a, b in zip(Ag, Ao):
+            assert np.array_equal(a, b)
+        # symmetry check and nonzero count
+        nw_o, _ = O.check_symmetry(p, Ao)
+        nw_g = ctx.check_symmetry_assembled_matrix_3d(p.active_vertex, *Ag)
+        for a, b in zip(Ag, Ao):
+            assert np.array_equal(a, b)
+        assert ctx.count_nonzeros_3d(*Ag, p.active_vertex) == O.count_nonzeros(p, Ao)
+
+
+def test_matvec_and_residual(gpu, prob):
+    p = prob
+    A, bu, bv, u, v, _ = assembled_system(p, seed=5)
+    rng = np.random.default_rng(3)
+    xu = np.asfortranarray(rng.standard_normal(u.shape))
+    xv = np.asfortranarray(rng.standard_normal(u.shape))
+    yo_u, yo_v = O.matvec(p, A, xu, xv)
+    yg_u, yg_v = O.vec_zeros(p), O.vec_zeros(p)
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.matvec_multiply_structured_3d(p.active_vertex, *A, xu, xv, yg_u, yg_v)
+        # same products summed in the same order
+        assert np.array_equal(yg_u, yo_u) and np.array_equal(yg_v, yo_v)
+        ru_o, rv_o, L2o, L2ro = O.residual_vector(p, A, bu, bv, xu, xv)
+        ru_g, rv_g = O.vec_zeros(p), O.vec_zeros(p)
+        L2g, L2rg = ctx.compute_residual_vector_3d(p.active_vertex, *A, bu, bv, xu, xv, ru_g, rv_g)
+        assert np.array_equal(ru_g, ru_o) and np.array_equal(rv_g, rv_o)
+        assert abs(L2g - L2o) <= 1e-13 * L2o and abs(L2rg - L2ro) <= 1e-13 * L2ro
+        us, vs = xu + 0.01 * rng.standard_normal(u.shape), xv.copy(order="F")
+        us = np.asfortranarray(us)
+        for wr in (0, 1, 3):
+            assert ctx.compute_residual_velocity_3d(wr, xu, xv, us, vs) == pytest.approx(
+                O.residual_velocity(p, wr, xu, xv, us, vs), rel=1e-15, abs=0)
+
+
+@pytest.mark.parametrize("chrongear", [False, True])
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_pcg_short(gpu, prob, chrongear, precond):
+    """10 iterations from the same start: the iterates agree to rounding (the global sums
+    are formed in a different order, nothing else differs)."""
+    p = prob
+    A, bu, bv, u, v, _ = assembled_system(p, seed=5)
+    xo_u, xo_v = u.copy(order="F"), v.copy(order="F")
+    st, err_o, nit_o = O.pcg(p, A, bu, bv, xo_u, xo_v, precond, chrongear=chrongear, maxiters=10)
+    assert st == 0
+    xg_u, xg_v = u.copy(order="F"), v.copy(order="F")
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.set_linear_solver(10, 1e-8)
+        f = ctx.pcg_solver_chrongear_3d if chrongear else ctx.pcg_solver_standard_3d
+        err_g, nit_g = f(p.active_vertex, *A, bu, bv, xg_u, xg_v, precond)
+    assert nit_g == nit_o == 10
+    x_o = np.concatenate([xo_u.ravel(), xo_v.ravel()])
+    x_g = np.concatenate([xg_u.ravel(), xg_v.ravel()])
+    print(f"  10 its: rel L2 diff {rel_l2(x_g, x_o):.2e}")
+    assert rel_l2(x_g, x_o) < 1e-11
+    assert err_g == pytest.approx(err_o, rel=1e-9)
+
+
+@pytest.mark.parametrize("chrongear", [False, True])
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_pcg(gpu, prob, chrongear, precond):
+    """Full reference solve (maxiters 200, tolerance 1e-8, checks every 5 iterations)."""
+    p = prob
+    A, bu, bv, u, v, _ = assembled_system(p, seed=5)
+    xo_u, xo_v = u.copy(order="F"), v.copy(order="F")
+    st, err_o, nit_o = O.pcg(p, A, bu, bv, xo_u, xo_v, precond, chrongear=chrongear)
+    assert st == 0
+    xg_u, xg_v = u.copy(order="F"), v.copy(order="F")
+    with gpu.Context.for_problem(p) as ctx:
+        f = ctx.pcg_solver_chrongear_3d if chrongear else ctx.pcg_solver_standard_3d
+        err_g, nit_g = f(p.active_vertex, *A, bu, bv, xg_u, xg_v, precond)
+    x_o = np.concatenate([xo_u.ravel(), xo_v.ravel()])
+    x_g = np.concatenate([xg_u.ravel(), xg_v.ravel()])
+    d = rel_l2(x_g, x_o)
+    print(f"  precond={precond} chrongear={chrongear} oracle niters={nit_o} err={err_o:.3e} "
+          f"gpu niters={nit_g} err={err_g:.3e} rel L2 diff {d:.2e}")
+    # the stopping test is checked every 5 iterations: same count unless the residual
+    # lands within rounding of the tolerance
+    assert abs(nit_g - nit_o) <= 5
+    if nit_o < 200:
+        assert err_g < 1e-8 and err_o < 1e-8
+        assert d < 1e-8
+    else:
+        # not converged in 200 iterations: rounding differences grow with the iteration
+        # count; they must stay well below the iterate's own distance to the solution
+        xs_u, xs_v = u.copy(order="F"), v.copy(order="F")
+        O.pcg(p, A, bu, bv, xs_u, xs_v, 2, chrongear=True, maxiters=2000, tolerance=1e-14)
+        x_s = np.concatenate([xs_u.ravel(), xs_v.ravel()])
+        e_o = rel_l2(x_o, x_s)
+        print(f"  unconverged: oracle distance to solution {e_o:.2e}")
+        assert d < 0.1 * e_o
+
+
+def test_load_gravity(gpu, prob):
+    p = prob
+    for standard in (False, True):
+        lo_u, lo_v = O.load_gravity(p, standard=standard)
+        lg_u, lg_v = O.vec_zeros(p), O.vec_zeros(p)
+        with gpu.Context.for_problem(p) as ctx:
+            ctx.load_vector_gravity(p.sigma, p.stagwbndsigma, p.dx, p.dy, p.active_cell,
+                                    p.active_vertex, p.xVertex, p.yVertex, p.stagusrf, p.stagthck,
+                                    p.dusrf_dx, p.dusrf_dy, 0 if standard else 1, lg_u, lg_v)
+        if standard:
+            assert rel_max(lg_u, lo_u) < 1e-13 and rel_max(lg_v, lo_v) < 1e-13
+        else:
+            assert np.array_equal(lg_u, lo_u) and np.array_equal(lg_v, lo_v)
+
+
+def test_picard_fp64(gpu, prob):
+    p = prob
+    lu, lv = O.load_gravity(p)
+    ro = O.picard(p, lu, lv)
+    assert ro["status"] == 0
+    with gpu.Context.for_problem(p) as ctx:
+        rg = ctx.velo_higher_solve_3d(p, lu, lv)
+    print(f"  oracle picard={ro['picard_iters']} lin={ro['linear_iters']}  "
+          f"gpu picard={rg['picard_iters']} lin={rg['linear_iters']} t={rg['seconds_total']:.3f}s")
+    assert rg["picard_iters"] == ro["picard_iters"]
+    xo = np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()])
+    xg = np.concatenate([rg["uvel"].ravel(), rg["vvel"].ravel()])
+    e = rel_l2(xg, xo)
+    print(f"  relative L2 velocity difference {e:.3e}")
+    assert e < TOL_PICARD
+    assert rel_max(rg["efvs"], ro["efvs"]) < 1e-6
