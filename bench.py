@@ -69,7 +69,7 @@ def cpu_baseline(p, lu, lv, fp64_stats, budget_s=20.0):
     O.dirichlet(p, A, bu, bv, p.uvel, p.vvel)
     for a in A:
         O.halo(p, a)
-    n_it = 10
+    n_it = 40
     xu, xv = p.uvel.copy(order="F"), p.vvel.copy(order="F")
     t0 = time.perf_counter()
     O.pcg(p, A, bu, bv, xu, xv, p.whichprecond, chrongear=True, maxiters=n_it, tolerance=1e-30)
@@ -80,6 +80,30 @@ def cpu_baseline(p, lu, lv, fp64_stats, budget_s=20.0):
             "sample": (f"oracle (C, -O3, 1 thread): 1 assembly {t_asm:.2f} s + {n_it} Chron-Gear "
                        f"iterations {t_pcg * 1e3:.1f} ms/it, extrapolated to the GPU FP64 run's "
                        f"{n_pic} Picard / {n_lin} PCG iterations")}
+
+
+def reference_solution(ctx, p, lu, lv, chunk=10, max_picard=300, velo_tol=1e-10):
+    """FP64 reference: plain FP64 Picard with linear tolerance 1e-11, run in chunks of
+    `chunk` iterations (Picard depends only on the current velocity, so chunks restart
+    exactly) until the largest relative velocity change is below velo_tol."""
+    t0 = time.perf_counter()
+    ctx.set_linear_solver(2000, 1e-11)
+    u, v = p.uvel, p.vvel
+    done, lin = 0, 0
+    while done < max_picard:
+        r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u, vvel=v, maxiter_nonlinear=chunk,
+                                     whichresid=0, target_override=velo_tol)
+        u, v = r["uvel"], r["vvel"]
+        done += r["picard_iters"]
+        lin += r["linear_iters"]
+        print(f"[bench] reference: {done} Picard, {lin} PCG its, velocity change "
+              f"{r['final_resid_velo']:.2e}, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        if r["final_resid_velo"] < velo_tol:
+            break
+    ctx.set_linear_solver(200, 1e-8)
+    r["picard_iters"], r["linear_iters"] = done, lin
+    return r
 
 
 def main(argv=None):
@@ -107,16 +131,8 @@ def main(argv=None):
                             p.xVertex, p.yVertex, p.stagusrf, p.stagthck, p.dusrf_dx, p.dusrf_dy,
                             p.whichassemble_taud, lu, lv)
 
-    # FP64 reference solution (untimed)
-    t0 = time.perf_counter()
-    ctx.set_linear_solver(2000, 1e-11)
-    ref = ctx.velo_higher_solve_3d(p, lu, lv, maxiter_nonlinear=400, whichresid=0,
-                                   target_override=1e-11)
-    ctx.set_linear_solver(200, 1e-8)
+    ref = reference_solution(ctx, p, lu, lv)
     ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
-    print(f"[bench] FP64 reference: {ref['picard_iters']} Picard, {ref['linear_iters']} PCG its, "
-          f"velo change {ref['final_resid_velo']:.2e}, {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
 
     def fast():
         return ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1,

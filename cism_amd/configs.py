@@ -306,21 +306,24 @@ def ismip_hom_a(ewn=160, nsn=160, nz=21, L=80000.0, A=1.0e-16, thklim=100.0,
                 whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA, **kw) -> Problem:
     """ISMIP-HOM experiment A: s = -x tan(0.5 deg), b = s - 1000 + 500 sin sin, frozen bed.
 
-    Cell centres x = (i + 1/2) dx with i the 0-based global index (runISMIP_HOM.py:287-344);
-    the surface continues linearly into the halo, as periodic_offset_ew makes it.
+    Cell centres x = (i + 1/2) dx with i the 0-based global index (runISMIP_HOM.py:287-344).
+    Halo cells are exact copies of their periodic images (parallel_halo), the surface
+    shifted by the east-west offset.
     """
     dx = L / ewn
-    nx, ny = ewn + 2 * NHALO, nsn + 2 * NHALO
-    ig = np.arange(nx) - NHALO            # 0-based global index incl. halo
-    jg = np.arange(ny) - NHALO
-    x = (ig + 0.5) * dx
-    y = (jg + 0.5) * dx
+    x = (np.arange(ewn) + 0.5) * dx
+    y = (np.arange(nsn) + 0.5) * dx
     om = 2.0 * math.pi / L
     alpha = 0.5 * math.pi / 180.0
     H = 1000.0 - 500.0 * np.sin(om * x)[:, None] * np.sin(om * y)[None, :]
-    s = np.broadcast_to(-x[:, None] * math.tan(alpha), (nx, ny))
-    thck = np.asfortranarray(H)
-    usrf = np.asfortranarray(s.copy())
+    s = np.broadcast_to(-x[:, None] * math.tan(alpha), (ewn, nsn))
+    thck = pad_periodic(H)
+    # halo copies of the surface carry the east-west offset L tan(alpha)
+    # (periodic_offset_ew, runISMIP_HOM.py:220-227), so it continues linearly
+    usrf = pad_periodic(s)
+    off = ewn * dx * math.tan(alpha)
+    usrf[:NHALO, :] += off
+    usrf[-NHALO:, :] -= off
     topg = np.asfortranarray(usrf - thck)
     return _finish("ismip_hom_a", ewn, nsn, nz, dx, dx, thck, usrf, topg, thklim, A,
                    HO_BABC_NO_SLIP, whichsparse=whichsparse, whichprecond=whichprecond, **kw)

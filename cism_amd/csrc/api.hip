@@ -21,33 +21,37 @@ void cism_set_error(const char *fmt, ...) {
 extern "C" const char *cism_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------------------
-// layout conversion: Fortran (27, nn) block <-> 27 planes of npad
+// layout conversion: Fortran (27, nn) block <-> tiled matrix; (nn) u, (nn) v
+// <-> interleaved (u, v) pairs.  One workgroup per 64-node tile.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_block_to_planes(long long nn, long long np,
-                                                         const double *__restrict__ src,
-                                                         double *__restrict__ dst) {
-    // 27 x 256 tile through LDS: reads and writes are both unit-stride
-    __shared__ double t[27 * 256 + 1];
-    const long long q0 = (long long)blockIdx.x * 256;
-    const long long cnt = (q0 + 256 <= nn ? 256 : nn - q0) * 27;
-    for (int s = threadIdx.x; s < cnt; s += 256) t[s] = src[q0 * 27 + s];
-    __syncthreads();
-    const long long q = q0 + threadIdx.x;
-    if (q < nn)
-        for (int m = 0; m < 27; ++m) dst[m * np + q] = t[threadIdx.x * 27 + m];
+__global__ __launch_bounds__(256) void k_fortran_to_tiled(long long nn, int b,
+                                                          const double *__restrict__ src,
+                                                          double *__restrict__ A) {
+    const long long q0 = (long long)blockIdx.x * 64;
+    const int cnt = (int)((q0 + 64 <= nn ? 64 : nn - q0) * 27);
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x)
+        A[aidx(b, t % 27, q0 + t / 27)] = src[q0 * 27 + t];
 }
 
-__global__ __launch_bounds__(256) void k_planes_to_block(long long nn, long long np,
-                                                         const double *__restrict__ src,
-                                                         double *__restrict__ dst) {
-    __shared__ double t[27 * 256 + 1];
-    const long long q0 = (long long)blockIdx.x * 256;
-    const long long q = q0 + threadIdx.x;
-    if (q < nn)
-        for (int m = 0; m < 27; ++m) t[threadIdx.x * 27 + m] = src[m * np + q];
-    __syncthreads();
-    const long long cnt = (q0 + 256 <= nn ? 256 : nn - q0) * 27;
-    for (int s = threadIdx.x; s < cnt; s += 256) dst[q0 * 27 + s] = t[s];
+__global__ __launch_bounds__(256) void k_tiled_to_fortran(long long nn, int b,
+                                                          const double *__restrict__ A,
+                                                          double *__restrict__ dst) {
+    const long long q0 = (long long)blockIdx.x * 64;
+    const int cnt = (int)((q0 + 64 <= nn ? 64 : nn - q0) * 27);
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x)
+        dst[q0 * 27 + t] = A[aidx(b, t % 27, q0 + t / 27)];
+}
+
+__global__ __launch_bounds__(256) void k_interleave(long long nn, const double *__restrict__ s,
+                                                    double *__restrict__ x) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2 * nn) x[t] = s[(t & 1) * nn + (t >> 1)];
+}
+
+__global__ __launch_bounds__(256) void k_deinterleave(long long nn, const double *__restrict__ x,
+                                                      double *__restrict__ s) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2 * nn) s[(t & 1) * nn + (t >> 1)] = x[t];
 }
 
 int ensure_stage(cism_ctx *c) {
@@ -66,16 +70,32 @@ int ensure_ke(cism_ctx *c) {
     return CISM_OK;
 }
 
+static int upload_block(cism_ctx *c, int b, const double *h) {
+    HIP_TRY(hipMemcpyAsync(c->stage, h, sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
+                           c->stream));
+    hipLaunchKernelGGL(k_fortran_to_tiled, dim3((unsigned)((c->g.nn + 63) / 64)), dim3(256), 0,
+                       c->stream, c->g.nn, b, c->stage, c->A);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+static int download_block(cism_ctx *c, int b, double *h) {
+    hipLaunchKernelGGL(k_tiled_to_fortran, dim3((unsigned)((c->g.nn + 63) / 64)), dim3(256), 0,
+                       c->stream, c->g.nn, b, c->A, c->stage);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h, c->stage, sizeof(double) * 27 * c->g.nn, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));   // the staging buffer is reused by the next block
+    return CISM_OK;
+}
+
 int upload_matrix(cism_ctx *c, const double *Auu, const double *Auv, const double *Avu,
                   const double *Avv) {
     CISM_TRY(ensure_stage(c));
     const double *h[4] = {Auu, Auv, Avu, Avv};
     for (int b = 0; b < 4; ++b) {
-        HIP_TRY(hipMemcpyAsync(c->stage, h[b], sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
-                               c->stream));
-        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
-                           c->g.nn, c->g.npad, c->stage, c->A + (size_t)b * 27 * c->g.npad);
-        HIP_TRY(hipGetLastError());
+        CISM_TRY(upload_block(c, b, h[b]));
+        HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return CISM_OK;
 }
@@ -83,28 +103,28 @@ int upload_matrix(cism_ctx *c, const double *Auu, const double *Auv, const doubl
 int download_matrix(cism_ctx *c, double *Auu, double *Auv, double *Avu, double *Avv) {
     CISM_TRY(ensure_stage(c));
     double *h[4] = {Auu, Auv, Avu, Avv};
-    for (int b = 0; b < 4; ++b) {
-        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
-                           c->g.nn, c->g.npad, c->A + (size_t)b * 27 * c->g.npad, c->stage);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(h[b], c->stage, sizeof(double) * 27 * c->g.nn, hipMemcpyDeviceToHost,
-                               c->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < 4; ++b) CISM_TRY(download_block(c, b, h[b]));
     return CISM_OK;
 }
 
 int upload_vec(cism_ctx *c, double *dst, const double *hu, const double *hv) {
-    HIP_TRY(hipMemcpyAsync(dst, hu, sizeof(double) * c->g.nn, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(dst + c->g.npad, hv, sizeof(double) * c->g.nn, hipMemcpyHostToDevice,
-                           c->stream));
+    const long long nn = c->g.nn;
+    HIP_TRY(hipMemcpyAsync(c->vstage, hu, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->vstage + nn, hv, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_interleave, dim3(nblocks_for(2 * nn)), dim3(256), 0, c->stream, nn, c->vstage,
+                       dst);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));   // vstage is reused by the next transfer
     return CISM_OK;
 }
 
 int download_vec(cism_ctx *c, const double *src, double *hu, double *hv) {
-    HIP_TRY(hipMemcpyAsync(hu, src, sizeof(double) * c->g.nn, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(hv, src + c->g.npad, sizeof(double) * c->g.nn, hipMemcpyDeviceToHost,
-                           c->stream));
+    const long long nn = c->g.nn;
+    hipLaunchKernelGGL(k_deinterleave, dim3(nblocks_for(2 * nn)), dim3(256), 0, c->stream, nn, src,
+                       c->vstage);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(hu, c->vstage, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hv, c->vstage + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CISM_OK;
 }
@@ -178,9 +198,11 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(CISM_ERR_HIP);
     const size_t v2 = 2 * (size_t)g.npad, ncol = (size_t)g.sx * g.sy;
     if ((st = dalloc(&c->A, 108 * (size_t)g.npad))) return fail(st);
-    double **vecs[] = {&c->b, &c->x, &c->r, &c->d, &c->z, &c->q, &c->Az, &c->t, &c->load, &c->usav};
+    double **vecs[] = {&c->b, &c->x, &c->r, &c->d, &c->z, &c->q, &c->Az, &c->t, &c->load, &c->usav,
+                       &c->vstage};
     for (double **v : vecs)
         if ((st = dalloc(v, v2))) return fail(st);
+    if ((st = dalloc(&c->F64, 6 * (size_t)g.npad))) return fail(st);
     if ((st = dalloc(&c->av, ncol)) || (st = dalloc(&c->ac, (size_t)nx * ny)) ||
         (st = dalloc(&c->umask, (size_t)g.nn)) || (st = dalloc(&c->vmask, (size_t)g.nn)))
         return fail(st);
@@ -190,7 +212,8 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     if ((st = dalloc(&c->sigma, (size_t)nz)) || (st = dalloc(&c->wsig, (size_t)nz + 1)) ||
         (st = dalloc(&c->flwafact, (size_t)g.nepad)) || (st = dalloc(&c->efvs, (size_t)g.nepad)))
         return fail(st);
-    if ((st = dalloc(&c->partials, (size_t)CISM_RED_BLOCKS * 8))) return fail(st);
+    const size_t nparts = (size_t)(nblocks_for(g.nn) > CISM_RED_BLOCKS ? nblocks_for(g.nn) : CISM_RED_BLOCKS);
+    if ((st = dalloc(&c->partials, nparts * 8))) return fail(st);
     if ((st = dalloc(&c->sc, 1))) return fail(st);
     if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
         return fail(CISM_ERR_HIP);
@@ -202,8 +225,8 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     if (!c) return CISM_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav,
-                    c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->av, c->ac, c->umask, c->vmask,
+    void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
+                    c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals};
     for (void *p : ptrs)
@@ -346,8 +369,8 @@ __global__ void k_basal_local(GridDev g, const int32_t *__restrict__ av, const d
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (long long)g.sx * g.sy || !av[col]) return;
     const long long q = (g.nz - 1) + (long long)g.nz * col;
-    A[(0 * 27 + CISM_DIAG) * g.npad + q] = A[(0 * 27 + CISM_DIAG) * g.npad + q] + scale * beta[col];
-    A[(3 * 27 + CISM_DIAG) * g.npad + q] = A[(3 * 27 + CISM_DIAG) * g.npad + q] + scale * beta[col];
+    A[aidx(0, CISM_DIAG, q)] = A[aidx(0, CISM_DIAG, q)] + scale * beta[col];
+    A[aidx(3, CISM_DIAG, q)] = A[aidx(3, CISM_DIAG, q)] + scale * beta[col];
 }
 
 extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
@@ -361,27 +384,15 @@ extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
     const size_t ncol = (size_t)c->g.sx * c->g.sy;
     CISM_TRY(up(c, c->av, active_vertex, ncol));
     CISM_TRY(up(c, c->beta, beta, ncol));
-    // only the two diagonal blocks are needed; stage them through Auu/Avv planes
     CISM_TRY(ensure_stage(c));
-    const double *h[2] = {Auu, Avv};
-    const int blk[2] = {0, 3};
-    for (int s = 0; s < 2; ++s) {
-        HIP_TRY(hipMemcpyAsync(c->stage, h[s], sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
-                               c->stream));
-        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
-                           c->g.nn, c->g.npad, c->stage, c->A + (size_t)blk[s] * 27 * c->g.npad);
-    }
+    CISM_TRY(upload_block(c, 0, Auu));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    CISM_TRY(upload_block(c, 3, Avv));
     hipLaunchKernelGGL(k_basal_local, dim3(nblocks_for((long long)ncol)), dim3(256), 0, c->stream,
                        c->g, c->av, c->beta, dx * dy / CISM_VOL0, c->A);
     HIP_TRY(hipGetLastError());
-    double *ho[2] = {Auu, Avv};
-    for (int s = 0; s < 2; ++s) {
-        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream,
-                           c->g.nn, c->g.npad, c->A + (size_t)blk[s] * 27 * c->g.npad, c->stage);
-        HIP_TRY(hipMemcpyAsync(ho[s], c->stage, sizeof(double) * 27 * c->g.nn, hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    CISM_TRY(download_block(c, 0, Auu));
+    CISM_TRY(download_block(c, 3, Avv));
     return CISM_OK;
 }
 
@@ -493,24 +504,20 @@ extern "C" int cism_staggered_parallel_halo(cism_ctx *c, int lead, double *a) {
     CTX_BEGIN(c);
     const GridDev &g = c->g;
     if (lead == g.nz) {
-        // a single (nz, nx-1, ny-1) array: use the u plane of the temp vector
-        CISM_TRY(up(c, c->t, a, (size_t)g.nn));
-        CISM_TRY(launch_halo_vec(c, c->t));   // the v plane is ignored
-        CISM_TRY(down(c, a, c->t, (size_t)g.nn));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        return CISM_OK;
+        // one (nz, nx-1, ny-1) array: carried as the u component of the temp vector
+        CISM_TRY(upload_vec(c, c->t, a, a));
+        CISM_TRY(launch_halo_vec(c, c->t));
+        double *tmp = (double *)malloc(sizeof(double) * g.nn);
+        if (!tmp) return CISM_ERR_NOMEM;
+        int st = download_vec(c, c->t, a, tmp);
+        free(tmp);
+        return st;
     }
     if (lead == 27 * g.nz) {
         CISM_TRY(ensure_stage(c));
-        HIP_TRY(hipMemcpyAsync(c->stage, a, sizeof(double) * 27 * g.nn, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_block_to_planes, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g.nn,
-                           g.npad, c->stage, c->A);
-        CISM_TRY(launch_halo_n(c, c->A, 27));
-        hipLaunchKernelGGL(k_planes_to_block, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g.nn,
-                           g.npad, c->A, c->stage);
-        HIP_TRY(hipMemcpyAsync(a, c->stage, sizeof(double) * 27 * g.nn, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        return CISM_OK;
+        CISM_TRY(upload_block(c, 0, a));
+        CISM_TRY(launch_halo_mat(c, c->A));
+        return download_block(c, 0, a);
     }
     cism_set_error("halo: lead must be nz or 27*nz (got %d)", lead);
     return CISM_ERR_ARG;

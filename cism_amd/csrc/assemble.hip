@@ -34,11 +34,26 @@ __host__ __device__ constexpr int tri_index(int a, int b) {
 
 #define EL_PER_WG 32
 
+// element scratch: tiles of 64 elements, entry-major inside a tile, so a wave's
+// reads and writes of one entry are one contiguous 512-byte run
+__host__ __device__ inline long long keidx(int ent, long long e) {
+    return (((e >> 6) * CISM_KE_ENTRIES + ent) << 6) + (e & 63);
+}
+
+// Per-element LDS records are padded to an odd number of doubles so that the
+// 32 elements a wave reads side by side fall in 32 different bank pairs.
+#define QP_STRIDE (8 * 8 + 1)
 struct QPLds {
-    double dx[EL_PER_WG][8][8], dy[EL_PER_WG][8][8], dz[EL_PER_WG][8][8];   // [el][p][n]
-    double f[EL_PER_WG][8], ev[EL_PER_WG][8];
-    double nodes[EL_PER_WG][8][5];                                          // x y z u v
+    double dx[EL_PER_WG][QP_STRIDE], dy[EL_PER_WG][QP_STRIDE], dz[EL_PER_WG][QP_STRIDE];  // [el][p*8+n]
+    double f[EL_PER_WG][9], ev[EL_PER_WG][9];
+    double nodes[EL_PER_WG][8 * 5 + 1];                                                  // x y z u v
 };
+
+// packed upper-triangle entry t (0..35) -> (a, b), a <= b
+__constant__ unsigned char c_tri_a[36] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2,
+                                          2, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 6, 6, 7};
+__constant__ unsigned char c_tri_b[36] = {0, 1, 2, 3, 4, 5, 6, 7, 1, 2, 3, 4, 5, 6, 7, 2, 3, 4,
+                                          5, 6, 7, 3, 4, 5, 6, 7, 4, 5, 6, 7, 5, 6, 7, 6, 7, 7};
 
 __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__restrict__ sigma,
                                                  const int32_t *__restrict__ ac,
@@ -74,11 +89,11 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
         const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
         const long long v = (in - 1) + (long long)g.sx * (jn - 1);
         const long long qn = (kn - 1) + (long long)g.nz * v;
-        L.nodes[el][n][0] = xV[v];
-        L.nodes[el][n][1] = yV[v];
-        L.nodes[el][n][2] = susrf[v] - sigma[kn - 1] * sthck[v];
-        L.nodes[el][n][3] = uvel[qn];
-        L.nodes[el][n][4] = uvel[g.npad + qn];
+        L.nodes[el][n * 5 + 0] = xV[v];
+        L.nodes[el][n * 5 + 1] = yV[v];
+        L.nodes[el][n * 5 + 2] = susrf[v] - sigma[kn - 1] * sthck[v];
+        L.nodes[el][n * 5 + 3] = uvel[2 * qn];
+        L.nodes[el][n * 5 + 4] = uvel[2 * qn + 1];
     }
     __syncthreads();
     // phase 1: lane (el, p) evaluates quadrature point p (velo_higher:6360-6579, 7199-7378)
@@ -103,7 +118,7 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
         double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
-            const double X = L.nodes[el][n][0], Y = L.nodes[el][n][1], Z = L.nodes[el][n][2];
+            const double X = L.nodes[el][n * 5 + 0], Y = L.nodes[el][n * 5 + 1], Z = L.nodes[el][n * 5 + 2];
             J[0][0] = J[0][0] + a[n] * X;  J[0][1] = J[0][1] + a[n] * Y;  J[0][2] = J[0][2] + a[n] * Z;
             J[1][0] = J[1][0] + b[n] * X;  J[1][1] = J[1][1] + b[n] * Y;  J[1][2] = J[1][2] + b[n] * Z;
             J[2][0] = J[2][0] + cz[n] * X; J[2][1] = J[2][1] + cz[n] * Y; J[2][2] = J[2][2] + cz[n] * Z;
@@ -138,10 +153,10 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
             const double ddx = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * cz[n];
             const double ddy = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * cz[n];
             const double ddz = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * cz[n];
-            L.dx[el][p][n] = ddx;
-            L.dy[el][p][n] = ddy;
-            L.dz[el][p][n] = ddz;
-            const double u = L.nodes[el][n][3], v = L.nodes[el][n][4];
+            L.dx[el][p * 8 + n] = ddx;
+            L.dy[el][p * 8 + n] = ddy;
+            L.dz[el][p * 8 + n] = ddz;
+            const double u = L.nodes[el][n * 5 + 3], v = L.nodes[el][n * 5 + 4];
             ux = ux + ddx * u; uy = uy + ddy * u; uz = uz + ddz * u;
             vx = vx + ddx * v; vy = vy + ddy * v; vz = vz + ddz * v;
         }
@@ -187,12 +202,11 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
         if (ent < 72) {
             // packed upper triangle of Kuu (ent < 36) or Kvv
             const bool isv = ent >= 36;
-            int rem = isv ? ent - 36 : ent, a = 0;
-            while (rem >= 8 - a) { rem -= 8 - a; ++a; }
-            const int bb = a + rem;
+            const int tt = isv ? ent - 36 : ent;
+            const int a = c_tri_a[tt], bb = c_tri_b[tt];
             for (int p = 0; p < 8; ++p) {
                 const double f = L.f[le][p];
-                const double *X = L.dx[le][p], *Y = L.dy[le][p], *Z = L.dz[le][p];
+                const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8], *Z = &L.dz[le][p * 8];
                 double term;
                 if (whichapprox == 0) term = Z[a] * Z[bb];
                 else if (whichapprox == 1)
@@ -206,23 +220,23 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
             const int idx = ent - 72, nr = idx & 7, nc = idx >> 3;   // Kuv(nr, nc)
             if (whichapprox != 0)
                 for (int p = 0; p < 8; ++p) {
-                    const double *X = L.dx[le][p], *Y = L.dy[le][p];
+                    const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8];
                     acc = acc + L.f[le][p] * (2.0 * X[nr] * Y[nc] + Y[nr] * X[nc]);
                 }
         }
-        Ke[(long long)ent * g.nepad + ee] = acc;
+        Ke[keidx(ent, ee)] = acc;
     }
 }
 
 // value of K_block(nr, nc) of element e from the packed scratch
-__device__ inline double ke_val(const double *__restrict__ Ke, long long nepad, int blk, int nr,
-                                int nc, long long e) {
+__device__ inline double ke_val(const double *__restrict__ Ke, int blk, int nr, int nc,
+                                long long e) {
     int ent;
     if (blk == 0) ent = tri_index(nr, nc);
     else if (blk == 3) ent = 36 + tri_index(nr, nc);
     else if (blk == 1) ent = 72 + nr + 8 * nc;   // Kuv(nr, nc)
     else ent = 72 + nc + 8 * nr;                 // Kvu(nr, nc) = Kuv(nc, nr)
-    return Ke[(long long)ent * nepad + e];
+    return Ke[keidx(ent, e)];
 }
 
 // One lane per node (k, i, j): A(m, node) = sum over elements in (j, i, k) order
@@ -254,7 +268,6 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                 ev[dJ][dI][dK] = okc && ac[(icl - 1) + (long long)g.nx * (jcl - 1)] != 0;
                 eid[dJ][dI][dK] = okc ? (ke - 1) + (long long)(g.nz - 1) * ((icl - 1) + (long long)g.nx * (jcl - 1)) : 0;
             }
-    const long long np = g.npad;
 #pragma unroll
     for (int blk = 0; blk < 4; ++blk) {
 #pragma unroll
@@ -271,12 +284,12 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                         const int cx = ix + iA, cy = jy + jA, cl = kl + kA;
                         if (cx < 0 || cx > 1 || cy < 0 || cy > 1 || cl < 0 || cl > 1) continue;
                         if (!ev[dJ][dI][dK]) continue;
-                        acc = acc + ke_val(Ke, g.nepad, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
+                        acc = acc + ke_val(Ke, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
                                            eid[dJ][dI][dK]);
                     }
             if (add_basal && m == CISM_DIAG && (blk == 0 || blk == 3) && k == g.nz && av[col])
                 acc = acc + basal_scale * beta[col];
-            A[(blk * 27 + m) * np + q] = acc;
+            A[aidx(blk, m, q)] = acc;
         }
     }
 }
@@ -311,7 +324,6 @@ __global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__r
                                                    double *__restrict__ A, double *__restrict__ b) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nn) return;
-    const long long np = g.npad;
     const int k = (int)(q % g.nz) + 1;
     const long long col = q / g.nz;
     const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
@@ -329,25 +341,25 @@ __global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__r
             if (k + kA < 1 || k + kA > g.nz || i + iA < 1 || i + iA > g.sx || j + jA < 1 || j + jA > g.sy)
                 continue;
             if (m == CISM_DIAG) continue;
-            if (Nu) { A[(0 * 27 + m) * np + q] = 0.0; A[(1 * 27 + m) * np + q] = 0.0; }
-            if (Nv) { A[(2 * 27 + m) * np + q] = 0.0; A[(3 * 27 + m) * np + q] = 0.0; }
+            if (Nu) { A[aidx(0, m, q)] = 0.0; A[aidx(1, m, q)] = 0.0; }
+            if (Nv) { A[aidx(2, m, q)] = 0.0; A[aidx(3, m, q)] = 0.0; }
         }
         if (Nu) {
-            b[q] = vel[q];
-            A[(0 * 27 + CISM_DIAG) * np + q] = 1.0;
-            A[(1 * 27 + CISM_DIAG) * np + q] = 0.0;
-            A[(2 * 27 + CISM_DIAG) * np + q] = 0.0;
+            b[2 * q] = vel[2 * q];
+            A[aidx(0, CISM_DIAG, q)] = 1.0;
+            A[aidx(1, CISM_DIAG, q)] = 0.0;
+            A[aidx(2, CISM_DIAG, q)] = 0.0;
         }
         if (Nv) {
-            b[np + q] = vel[np + q];
-            A[(1 * 27 + CISM_DIAG) * np + q] = 0.0;
-            A[(2 * 27 + CISM_DIAG) * np + q] = 0.0;
-            A[(3 * 27 + CISM_DIAG) * np + q] = 1.0;
+            b[2 * q + 1] = vel[2 * q + 1];
+            A[aidx(1, CISM_DIAG, q)] = 0.0;
+            A[aidx(2, CISM_DIAG, q)] = 0.0;
+            A[aidx(3, CISM_DIAG, q)] = 1.0;
         }
     }
     if (Nu && Nv) return;   // no column elimination touches a doubly prescribed row
     // (b) column eliminations from prescribed neighbours D = N + o
-    double bu = b[q], bv = b[np + q];
+    double bu = b[2 * q], bv = b[2 * q + 1];
     bool touched = false;
     for (int oj = -1; oj <= 1; ++oj)
         for (int oi = -1; oi <= 1; ++oi)
@@ -360,20 +372,20 @@ __global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__r
                 if (!(Du || Dv) || !in_range(di, dj)) continue;
                 const int m = slot_of(oi, oj, ok);
                 if (Du) {
-                    const double ud = vel[qd];
-                    if (!Nu) { bu = bu - A[(0 * 27 + m) * np + q] * ud; A[(0 * 27 + m) * np + q] = 0.0; }
-                    if (!Nv) { bv = bv - A[(2 * 27 + m) * np + q] * ud; A[(2 * 27 + m) * np + q] = 0.0; }
+                    const double ud = vel[2 * qd];
+                    if (!Nu) { bu = bu - A[aidx(0, m, q)] * ud; A[aidx(0, m, q)] = 0.0; }
+                    if (!Nv) { bv = bv - A[aidx(2, m, q)] * ud; A[aidx(2, m, q)] = 0.0; }
                 }
                 if (Dv) {
-                    const double vd = vel[np + qd];
-                    if (!Nu) { bu = bu - A[(1 * 27 + m) * np + q] * vd; A[(1 * 27 + m) * np + q] = 0.0; }
-                    if (!Nv) { bv = bv - A[(3 * 27 + m) * np + q] * vd; A[(3 * 27 + m) * np + q] = 0.0; }
+                    const double vd = vel[2 * qd + 1];
+                    if (!Nu) { bu = bu - A[aidx(1, m, q)] * vd; A[aidx(1, m, q)] = 0.0; }
+                    if (!Nv) { bv = bv - A[aidx(3, m, q)] * vd; A[aidx(3, m, q)] = 0.0; }
                 }
                 touched = true;
             }
     if (touched) {
-        if (!Nu) b[q] = bu;
-        if (!Nv) b[np + q] = bv;
+        if (!Nu) b[2 * q] = bu;
+        if (!Nv) b[2 * q + 1] = bv;
     }
 }
 
@@ -399,7 +411,6 @@ __global__ __launch_bounds__(256) void k_sym_find(GridDev g, const int32_t *__re
                                                   double *__restrict__ partials) {
     double acc[1] = {0.0};   // number of pairs that stay asymmetric (warnings)
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long np = g.npad;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
@@ -407,15 +418,15 @@ __global__ __launch_bounds__(256) void k_sym_find(GridDev g, const int32_t *__re
         if (!is_owned(g, i, j) || !av[col]) continue;
         for (int pass = 0; pass < 2; ++pass) {
             const int bd = pass == 0 ? 0 : 3, bo = pass == 0 ? 1 : 2, bt = pass == 0 ? 2 : 1;
-            const double diag = A[(bd * 27 + CISM_DIAG) * np + q];
+            const double diag = A[aidx(bd, CISM_DIAG, q)];
             for (int m = 0; m < 27; ++m) {
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
                 if (k + kA < 0 || k + kA >= g.nz) continue;
                 const int mm = 26 - m;
                 const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
                 for (int h = 0; h < 2; ++h) {
-                    const long long e1 = ((h == 0 ? bd : bo) * 27 + m) * np + q;
-                    const long long e2 = ((h == 0 ? bd : bt) * 27 + mm) * np + qc;
+                    const long long e1 = aidx(h == 0 ? bd : bo, m, q);
+                    const long long e2 = aidx(h == 0 ? bd : bt, mm, qc);
                     const double v1 = A[e1], v2 = A[e2];
                     if (v1 == v2) continue;
                     if (fabs(v2 - v1) < 1.0e-8 * fabs(diag)) {
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(256) void k_nnz(GridDev g, const int32_t *__restric
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         if (!is_owned(g, i, j) || !av[col]) continue;
         int n = 0;
-        for (int pl = 0; pl < 108; ++pl) n += A[pl * g.npad + q] != 0.0;
+        for (int pl = 0; pl < 108; ++pl) n += A[aidx(pl / 27, pl % 27, q)] != 0.0;
         acc[0] += n;
     }
     block_reduce_store<1>(acc, partials);
@@ -495,7 +506,7 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
                                                 double *__restrict__ partials) {
     double acc[2] = {0.0, 0.0};
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long np = g.npad, cst = g.nz, rst = (long long)g.nz * g.sx;
+    const long long cst = g.nz, rst = (long long)g.nz * g.sx;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
@@ -507,17 +518,17 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
                 if (k + kA < 0 || k + kA >= g.nz) continue;
                 const long long qn = q + kA + iA * cst + jA * rst;
-                const double xu = x[qn], xv = x[np + qn];
-                su = su + A[(0 * 27 + m) * np + q] * xu + A[(1 * 27 + m) * np + q] * xv;
-                sv = sv + A[(2 * 27 + m) * np + q] * xu + A[(3 * 27 + m) * np + q] * xv;
+                const double xu = x[2 * qn], xv = x[2 * qn + 1];
+                su = su + A[aidx(0, m, q)] * xu + A[aidx(1, m, q)] * xv;
+                sv = sv + A[aidx(2, m, q)] * xu + A[aidx(3, m, q)] * xv;
             }
-            ru = su - b[q];
-            rv = sv - b[np + q];
+            ru = su - b[2 * q];
+            rv = sv - b[2 * q + 1];
             acc[0] += ru * ru + rv * rv;
-            acc[1] += b[q] * b[q] + b[np + q] * b[np + q];
+            acc[1] += b[2 * q] * b[2 * q] + b[2 * q + 1] * b[2 * q + 1];
         }
-        r[q] = ru;
-        r[np + q] = rv;
+        r[2 * q] = ru;
+        r[2 * q + 1] = rv;
     }
     block_reduce_store<2>(acc, partials);
 }
@@ -538,17 +549,16 @@ __global__ __launch_bounds__(256) void k_resvel(GridDev g, int whichresid, const
                                                 double *__restrict__ partials) {
     double acc[2] = {0.0, 0.0};
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long np = g.npad;
     const int kmax = (whichresid == 1 || whichresid == 2) ? g.nz - 1 : g.nz;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         if (!is_owned(g, i, j) || k >= kmax) continue;
-        const double u = x[q], v = x[np + q];
+        const double u = x[2 * q], v = x[2 * q + 1];
         const double speed = sqrt(u * u + v * v);
         if (speed != 0.0) {
-            const double uo = xs[q], vo = xs[np + q];
+            const double uo = xs[2 * q], vo = xs[2 * q + 1];
             const double old = sqrt(uo * uo + vo * vo);
             const double dd = fabs((old - speed) / speed);
             if (whichresid == 2) { acc[0] += dd; acc[1] += 1.0; }
@@ -583,8 +593,8 @@ __global__ __launch_bounds__(256) void k_load_local(GridDev g, const double *__r
     if (!av[col]) return;
     const double dz = sthck[col] * (wsig[k] - wsig[k - 1]);
     const double f = CISM_RHOI * CISM_GRAV * dx * dy * dz / CISM_VOL0;
-    load[q] = load[q] - f * dsdx[col];
-    load[g.npad + q] = load[g.npad + q] - f * dsdy[col];
+    load[2 * q] = load[2 * q] - f * dsdx[col];
+    load[2 * q + 1] = load[2 * q + 1] - f * dsdy[col];
 }
 
 __global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *__restrict__ sigma,
@@ -601,7 +611,7 @@ __global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *
     const int k = (int)(q % g.nz) + 1;
     const long long col = q / g.nz;
     const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
-    double lu = load[q], lv = load[g.npad + q];
+    double lu = load[2 * q], lv = load[2 * q + 1];
     const double r = 1.0 / sqrt(3.0);
     for (int dJ = 0; dJ < 2; ++dJ)
         for (int dI = 0; dI < 2; ++dI)
@@ -652,8 +662,8 @@ __global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *
                     lv = lv - CISM_RHOI * CISM_GRAV * 1.0 * det / CISM_VOL0 * qy * ph[me];
                 }
             }
-    load[q] = lu;
-    load[g.npad + q] = lv;
+    load[2 * q] = lu;
+    load[2 * q + 1] = lv;
 }
 
 int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy) {

@@ -1,13 +1,16 @@
 // internal.h -- shared definitions of libcism_gpu.so (gfx950 / CDNA4).
 //
-// Device data layout ("node-major planes"):
+// Device data layout:
 //   node q (0-based) = k + nz*(i + sx*j), sx = nx-1, sy = ny-1, k fastest --
-//   the reference's vector layout (nz, nx-1, ny-1) without change.
-//   Matrix block b (0 Auu, 1 Auv, 2 Avu, 3 Avv), slot m (0..26, m+1 = indxA):
-//   A[(b*27 + m)*npad + q], i.e. 108 planes of npad (nz*sx*sy rounded up to a
-//   multiple of 256) values.  For a fixed slot, neighbouring lanes read
-//   neighbouring coefficients, so every matrix load is a unit-stride stream.
-//   Vectors hold u then v: x[c*npad + q], c = 0 (u), 1 (v).
+//   the reference's vector order (nz, nx-1, ny-1).
+//   Matrix: tiles of 64 nodes (one wavefront); inside a tile, slot-major, then
+//   lane, then block b (0 Auu, 1 Auv, 2 Avu, 3 Avv):
+//       A[aidx(b, m, q)],  aidx = ((q/64)*27 + m)*256 + (q%64)*4 + b
+//   so one wave's coefficients are one contiguous 27*64*4-value region (every
+//   load instruction covers 1 KB (FP32, 4 blocks) or 2 x 1 KB (FP64, 2 blocks)),
+//   and a lane gets all four blocks of a slot with one 16-byte (FP32) or two
+//   16-byte (FP64) loads.  npad (nz*sx*sy rounded up to 256) keeps tiles whole.
+//   Vectors interleave u and v: x[2*q + c], c = 0 (u), 1 (v).
 //   Element scratch: Ke[entry*nepad + e], e = (k-1) + (nz-1)*((i-1) + nx*(j-1)),
 //   the reference's efvs layout (nz-1, nx, ny).
 #pragma once
@@ -60,8 +63,12 @@ struct cism_ctx {
     float *A32;
     // vectors (2*npad each): b x r d z q Az t(temp) load usav
     double *b, *x, *r, *d, *z, *q, *Az, *t, *load, *usav;
+    double *vstage;   // 2*npad staging for (de)interleaving host vectors
     // FP32 vectors for the inner solves (2*npad each)
     float *x32, *b32, *r32, *d32, *z32, *q32, *Az32;
+    // column factors of the SIA preconditioner (6*npad each)
+    double *F64;
+    float *F32;
     // masks and geometry
     int32_t *av, *ac, *umask, *vmask;
     double *xV, *yV, *susrf, *sthck, *beta, *sigma, *wsig, *flwafact, *efvs;
@@ -87,6 +94,11 @@ struct cism_ctx {
 // ---------------------------------------------------------------------------
 __host__ __device__ inline int slot_of(int iA, int jA, int kA) {
     return (iA + 1) + 3 * (jA + 1) + 9 * (kA + 1);
+}
+
+// position of coefficient (block b, slot m) of node q in the tiled matrix layout
+__host__ __device__ inline long long aidx(int b, int m, long long q) {
+    return (((q >> 6) * 27 + m) << 8) + ((q & 63) << 2) + b;
 }
 
 // periodic / outflow image of a staggered index (parallel_slap.F90:2807-2925):
@@ -195,7 +207,6 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
-int launch_halo_n(cism_ctx *c, double *v, int nplanes);
 int convert_a32(cism_ctx *c);
 int launch_cg_matvec(cism_ctx *c, int fp32);
 int rel_err_device(cism_ctx *c, double *rel);

@@ -90,12 +90,29 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode) {
 // 27-point row product (pcg:2607-2645): slots in indxA order (kA, jA, iA
 // nested, iA fastest); per slot yu += Auu*xu + Auv*xv, yv += Avu*xu + Avv*xv.
 // Only k+kA needs a bounds test: owned rows never touch i, j out of range.
+// The four blocks of a slot come with one 16-byte load (FP32) or two (FP64);
+// x arrives as (u, v) pairs.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA>
-__device__ inline void row_product(const GridDev &g, const TA *__restrict__ A,
-                                   const T *__restrict__ x, long long q, int k,
-                                   T &yu, T &yv) {
-    const long long np = g.npad;
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using t = float2; };
+template <> struct Vec2<double> { using t = double2; };
+
+__device__ inline void load_blocks(const float *__restrict__ A, long long q, int m, float &uu,
+                                   float &uv, float &vu, float &vv) {
+    const float4 a = *reinterpret_cast<const float4 *>(A + aidx(0, m, q));
+    uu = a.x; uv = a.y; vu = a.z; vv = a.w;
+}
+__device__ inline void load_blocks(const double *__restrict__ A, long long q, int m, double &uu,
+                                   double &uv, double &vu, double &vv) {
+    const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, m, q));
+    const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
+    uu = a.x; uv = a.y; vu = b.x; vv = b.y;
+}
+
+template <typename T>
+__device__ inline void row_product(const GridDev &g, const T *__restrict__ A,
+                                   const T *__restrict__ x, long long q, int k, T &yu, T &yv) {
+    using V2 = typename Vec2<T>::t;
     const long long cst = g.nz, rst = (long long)g.nz * g.sx;
     T su = yu, sv = yv;
 #pragma unroll
@@ -107,18 +124,29 @@ __device__ inline void row_product(const GridDev &g, const TA *__restrict__ A,
             for (int iA = -1; iA <= 1; ++iA) {
                 const int m = slot_of(iA, jA, kA);
                 const long long qn = q + kA + iA * cst + jA * rst;
-                const T xu = x[qn], xv = x[np + qn];
-                su = su + (T)A[(0 * 27 + m) * np + q] * xu + (T)A[(1 * 27 + m) * np + q] * xv;
-                sv = sv + (T)A[(2 * 27 + m) * np + q] * xu + (T)A[(3 * 27 + m) * np + q] * xv;
+                const V2 xx = *reinterpret_cast<const V2 *>(x + 2 * qn);
+                T uu, uv, vu, vv;
+                load_blocks(A, q, m, uu, uv, vu, vv);
+                su = su + uu * xx.x + uv * xx.y;
+                sv = sv + vu * xx.x + vv * xx.y;
             }
     }
     yu = su;
     yv = sv;
 }
 
+template <typename T>
+__device__ inline void store2(T *__restrict__ y, long long q, T a, T b) {
+    using V2 = typename Vec2<T>::t;
+    V2 v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<V2 *>(y + 2 * q) = v;
+}
+
 // y = A x on owned active rows, 0 elsewhere (pcg:2533-2647)
-template <typename T, typename TA>
-__global__ __launch_bounds__(CISM_BLOCK) void k_matvec(GridDev g, const TA *__restrict__ A,
+template <typename T>
+__global__ __launch_bounds__(CISM_BLOCK) void k_matvec(GridDev g, const T *__restrict__ A,
                                                        const int32_t *__restrict__ av,
                                                        const T *__restrict__ x, T *__restrict__ y) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,47 +155,49 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec(GridDev g, const TA *__re
     const long long col = q / g.nz;
     const int i = (int)(col % g.sx), j = (int)(col / g.sx);
     T yu = 0, yv = 0;
-    if (is_owned(g, i, j) && av[col]) row_product<T, TA>(g, A, x, q, k, yu, yv);
-    y[q] = yu;
-    y[g.npad + q] = yv;
+    if (is_owned(g, i, j) && av[col]) row_product<T>(g, A, x, q, k, yu, yv);
+    store2<T>(y, q, yu, yv);
 }
 
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y) {
-    hipLaunchKernelGGL((k_matvec<double, double>), dim3(nblocks_for(c->g.nn)), dim3(CISM_BLOCK), 0,
+    hipLaunchKernelGGL((k_matvec<double>), dim3(nblocks_for(c->g.nn)), dim3(CISM_BLOCK), 0,
                        c->stream, c->g, A, c->av, x, y);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
 
 // y = A x plus per-block partial dot products over owned nodes:
-//   NDOT = 1: s0 = (x, y)            [standard PCG (d, Ad); Chron-Gear first pass]
+//   NDOT = 1: s0 = (x, y)               [standard PCG (d, Ad)]
 //   NDOT = 2: s0 = (r, x), s1 = (y, x)  [Chron-Gear (r, z), (Az, z)]
-template <typename T, typename TA, int NDOT>
-__global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const TA *__restrict__ A,
+// One lane per node, one pass over the grid; block partials go to a second
+// deterministic pass (k_reduce).
+template <typename T, int NDOT>
+__global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const T *__restrict__ A,
                                                             const int32_t *__restrict__ av,
                                                             const T *__restrict__ x, T *__restrict__ y,
                                                             const T *__restrict__ r,
                                                             double *__restrict__ partials) {
+    using V2 = typename Vec2<T>::t;
     double acc[NDOT];
 #pragma unroll
     for (int c = 0; c < NDOT; ++c) acc[c] = 0.0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.nn) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         T yu = 0, yv = 0;
         const bool own = is_owned(g, i, j);
-        if (own && av[col]) row_product<T, TA>(g, A, x, q, k, yu, yv);
-        y[q] = yu;
-        y[g.npad + q] = yv;
+        if (own && av[col]) row_product<T>(g, A, x, q, k, yu, yv);
+        store2<T>(y, q, yu, yv);
         if (own) {
-            const double xu = x[q], xv = x[g.npad + q];
+            const V2 xx = *reinterpret_cast<const V2 *>(x + 2 * q);
             if (NDOT == 1) {
-                acc[0] += xu * (double)yu + xv * (double)yv;
+                acc[0] = (double)xx.x * (double)yu + (double)xx.y * (double)yv;
             } else {
-                acc[0] += (double)r[q] * xu + (double)r[g.npad + q] * xv;
-                acc[NDOT - 1] += (double)yu * xu + (double)yv * xv;
+                const V2 rr = *reinterpret_cast<const V2 *>(r + 2 * q);
+                acc[0] = (double)rr.x * (double)xx.x + (double)rr.y * (double)xx.y;
+                acc[NDOT - 1] = (double)yu * (double)xx.x + (double)yv * (double)xx.y;
             }
         }
     }
@@ -177,15 +207,15 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const TA 
 // r = b - A x on the full grid (A x = 0 off owned active rows), s0 = (r, r) over
 // owned nodes.  halo_r: the halo takes the periodic image of r afterwards
 // (Chron-Gear, pcg:1592-1595), computed here from the image row directly.
-template <typename T, typename TA>
-__global__ __launch_bounds__(CISM_BLOCK) void k_residual(GridDev g, const TA *__restrict__ A,
+template <typename T>
+__global__ __launch_bounds__(CISM_BLOCK) void k_residual(GridDev g, const T *__restrict__ A,
                                                          const int32_t *__restrict__ av,
                                                          const T *__restrict__ x,
                                                          const T *__restrict__ b, T *__restrict__ r,
                                                          int halo_r, double *__restrict__ partials) {
     double acc[1] = {0.0};
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.nn) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         int i = (int)(col % g.sx), j = (int)(col / g.sx);
@@ -201,13 +231,12 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_residual(GridDev g, const TA *__
         if (!zero) {
             T yu = 0, yv = 0;
             const long long cs = qs / g.nz;
-            if (is_owned(g, i, j) && av[cs]) row_product<T, TA>(g, A, x, qs, k, yu, yv);
-            ru = b[qs] - yu;
-            rv = b[g.npad + qs] - yv;
+            if (is_owned(g, i, j) && av[cs]) row_product<T>(g, A, x, qs, k, yu, yv);
+            ru = b[2 * qs] - yu;
+            rv = b[2 * qs + 1] - yv;
         }
-        r[q] = ru;
-        r[g.npad + q] = rv;
-        if (own) acc[0] += (double)ru * ru + (double)rv * rv;
+        store2<T>(r, q, ru, rv);
+        if (own) acc[0] = (double)ru * ru + (double)rv * rv;
     }
     block_reduce_store<1>(acc, partials);
 }
@@ -223,7 +252,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_dot(GridDev g, const T *__restri
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         if (is_owned(g, i, j))
-            acc[0] += (double)a[q] * (double)b[q] + (double)a[g.npad + q] * (double)b[g.npad + q];
+            acc[0] += (double)a[2 * q] * (double)b[2 * q] + (double)a[2 * q + 1] * (double)b[2 * q + 1];
     }
     block_reduce_store<1>(acc, partials);
 }
@@ -233,95 +262,125 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_dot(GridDev g, const T *__restri
 // branches pcg:300-365 / 1416-1454; easy_sia_solver + tridiag_solver
 // pcg:2759-2904).  Coefficients are read straight from the matrix slots.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA>
-__global__ __launch_bounds__(CISM_BLOCK) void k_precond_diag(GridDev g, const TA *__restrict__ A,
+template <typename T>
+__global__ __launch_bounds__(CISM_BLOCK) void k_precond_diag(GridDev g, const T *__restrict__ A,
                                                              const T *__restrict__ r,
                                                              T *__restrict__ z) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nn) return;
-    const long long np = g.npad;
-    const T du = (T)A[(0 * 27 + CISM_DIAG) * np + q], dv = (T)A[(3 * 27 + CISM_DIAG) * np + q];
-    z[q] = du != (T)0 ? r[q] / du : (T)0;
-    z[np + q] = dv != (T)0 ? r[np + q] / dv : (T)0;
+    const T du = A[aidx(0, CISM_DIAG, q)], dv = A[aidx(3, CISM_DIAG, q)];
+    store2<T>(z, q, du != (T)0 ? r[2 * q] / du : (T)0, dv != (T)0 ? r[2 * q + 1] / dv : (T)0);
 }
 
 template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_copy2(long long nn, long long np,
-                                                      const T *__restrict__ a, T *__restrict__ b) {
+__global__ __launch_bounds__(CISM_BLOCK) void k_copy(long long n, const T *__restrict__ a,
+                                                     T *__restrict__ b) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nn) return;
-    b[q] = a[q];
-    b[np + q] = a[np + q];
+    if (q < n) b[q] = a[q];
 }
 
-// One lane per (column, component): Thomas algorithm without pivoting on the
-// vertical couplings (slots indxA(0,0,-1|0|+1)) of Auu (c=0) or Avv (c=1),
-// for every active vertex including the halo; inactive columns keep z.
-// The column of r is staged through LDS with unit-stride loads.
-#define SIA_COLS 64
-template <typename T, typename TA, int NZMAX>
-__global__ __launch_bounds__(SIA_COLS) void k_precond_sia(GridDev g, const TA *__restrict__ A,
+// Column-tridiagonal (SIA) preconditioner.  Setup, once per linear solve (the
+// role of setup_preconditioner_3d): each (column, component) lane runs the
+// forward part of the Thomas algorithm of tridiag_solver on the vertical
+// couplings (slots indxA(0,0,-1|0|+1) of Auu or Avv) and stores, for every
+// level k, the sub-diagonal sb(k), the pivot beta(k) and gamma(k) =
+// super(k-1)/beta(k-1).  The factors are stored level-major across columns,
+// F[((f*2 + comp)*nz + k)*ncol + col], so lanes of neighbouring columns read
+// neighbouring values.  Apply, every iteration: the same forward/back
+// substitution as tridiag_solver, with identical operations, on r staged
+// through LDS.  Inactive columns keep z, as easy_sia_solver leaves them.
+template <typename T>
+__global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const T *__restrict__ A,
                                                           const int32_t *__restrict__ av,
-                                                          const T *__restrict__ r,
-                                                          T *__restrict__ z) {
+                                                          T *__restrict__ F) {
+    const long long ncol = (long long)g.sx * g.sy;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * ncol) return;
+    const int comp = (int)(t / ncol);
+    const long long col = t - comp * ncol;
+    if (!av[col]) return;
+    const int nz = g.nz, blk = comp == 0 ? 0 : 3;
+    const long long qb = col * nz;
+    auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + col; };
+    T beta = A[aidx(blk, CISM_DIAG, qb)];
+    F[Fi(0, 0)] = (T)0;
+    F[Fi(1, 0)] = beta;
+    F[Fi(2, 0)] = (T)0;
+    for (int k = 1; k < nz; ++k) {
+        const T gam = A[aidx(blk, CISM_DIAG + 9, qb + k - 1)] / beta;   // super(k-1) / beta(k-1)
+        const T sb = A[aidx(blk, CISM_DIAG - 9, qb + k)];
+        beta = A[aidx(blk, CISM_DIAG, qb + k)] - sb * gam;
+        F[Fi(0, k)] = sb;
+        F[Fi(1, k)] = beta;
+        F[Fi(2, k)] = gam;
+    }
+}
+
+template <typename T, int NZMAX>
+__global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restrict__ F,
+                                                   const int32_t *__restrict__ av,
+                                                   const T *__restrict__ r, T *__restrict__ z) {
     const int nz = g.nz;
     const long long ncol = (long long)g.sx * g.sy;
-    const int comp = blockIdx.y;
-    const long long col0 = (long long)blockIdx.x * SIA_COLS;
-    const long long np = g.npad;
-    const TA *Am = A + (long long)(comp == 0 ? 0 : 3) * 27 * np;
-    const T *rc = r + comp * np;
-    T *zc = z + comp * np;
-    __shared__ T sr[SIA_COLS * NZMAX];
-    const long long base = col0 * nz;
-    const long long span = (col0 + SIA_COLS <= ncol ? SIA_COLS : ncol - col0) * nz;
-    for (int t = threadIdx.x; t < span; t += SIA_COLS) sr[t] = rc[base + t];
+    const long long col0 = (long long)blockIdx.x * 64;
+    __shared__ T sr[2 * 64 * NZMAX];
+    const int ncols = (int)(col0 + 64 <= ncol ? 64 : ncol - col0);
+    const long long Q0 = col0 * nz;
+    const int n2 = 2 * ncols * nz;
+    for (int t = threadIdx.x; t < n2; t += blockDim.x) sr[t] = r[2 * Q0 + t];
     __syncthreads();
-    const long long col = col0 + threadIdx.x;
-    const bool act = col < ncol && av[col];
-    if (act) {
-        T gam[NZMAX];
-        T *s = sr + threadIdx.x * nz;
-        const long long qb = col * nz;
-        // top layer: sub = 0; bottom: super = 0 (pcg:2803-2829)
-        T beta = (T)Am[CISM_DIAG * np + qb];
-        s[0] = s[0] / beta;
+    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+    const long long col = col0 + lc;
+    if (lc < ncols && av[col]) {
+        T *s = sr + 2 * lc * nz + comp;   // level k at s[2k]
+        const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
+        const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
+        const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
+        T prev = s[0] / be[0];
+        s[0] = prev;
         for (int k = 1; k < nz; ++k) {
-            const T sp = (T)Am[(CISM_DIAG + 9) * np + qb + k - 1];   // A(+1) of row k-1
-            const T sb = (T)Am[(CISM_DIAG - 9) * np + qb + k];       // A(-1) of row k
-            gam[k] = sp / beta;
-            beta = (T)Am[CISM_DIAG * np + qb + k] - sb * gam[k];
-            s[k] = (s[k] - sb * s[k - 1]) / beta;
+            prev = (s[2 * k] - sb[k * ncol] * prev) / be[k * ncol];
+            s[2 * k] = prev;
         }
-        for (int k = nz - 2; k >= 0; --k) s[k] = s[k] - gam[k + 1] * s[k + 1];
+        for (int k = nz - 2; k >= 0; --k) {
+            prev = s[2 * k] - ga[(k + 1) * ncol] * prev;
+            s[2 * k] = prev;
+        }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < span; t += SIA_COLS) {
-        const long long cc = col0 + t / nz;
-        if (av[cc]) zc[base + t] = sr[t];
-    }
+    for (int t = threadIdx.x; t < n2; t += blockDim.x)
+        if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
 }
 
-template <typename T, typename TA>
-static int launch_precond(cism_ctx *c, int precond, const TA *A, const T *r, T *z) {
+template <typename T>
+static int launch_sia_setup(cism_ctx *c, const T *A, T *F) {
+    const long long ncol = (long long)c->g.sx * c->g.sy;
+    hipLaunchKernelGGL(k_sia_setup<T>, dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, c->g,
+                       A, c->av, F);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+template <typename T>
+static int launch_precond(cism_ctx *c, int precond, const T *A, const T *F, const T *r, T *z) {
     const GridDev &g = c->g;
     if (precond == 0) {
-        hipLaunchKernelGGL(k_copy2<T>, dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
-                           g.nn, g.npad, r, z);
+        hipLaunchKernelGGL(k_copy<T>, dim3(nblocks_for(2 * g.nn)), dim3(CISM_BLOCK), 0, c->stream,
+                           2 * g.nn, r, z);
     } else if (precond == 1) {
-        hipLaunchKernelGGL((k_precond_diag<T, TA>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,
+        hipLaunchKernelGGL((k_precond_diag<T>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,
                            c->stream, g, A, r, z);
     } else {
         const long long ncol = (long long)g.sx * g.sy;
-        dim3 grid((unsigned)((ncol + SIA_COLS - 1) / SIA_COLS), 2);
-        if (g.nz <= 16)
-            hipLaunchKernelGGL((k_precond_sia<T, TA, 16>), grid, dim3(SIA_COLS), 0, c->stream, g, A,
-                               c->av, r, z);
-        else if (g.nz <= 32)
-            hipLaunchKernelGGL((k_precond_sia<T, TA, 32>), grid, dim3(SIA_COLS), 0, c->stream, g, A,
-                               c->av, r, z);
+        const dim3 grid((unsigned)((ncol + 63) / 64));
+        if (g.nz <= 12)
+            hipLaunchKernelGGL((k_sia_apply<T, 12>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
+        else if (g.nz <= 24)
+            hipLaunchKernelGGL((k_sia_apply<T, 24>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
+        else if (g.nz <= 48)
+            hipLaunchKernelGGL((k_sia_apply<T, 48>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
         else {
-            cism_set_error("SIA preconditioner supports nz <= 32 (nz = %d)", g.nz);
+            cism_set_error("SIA preconditioner supports nz <= 48 (nz = %d)", g.nz);
             return CISM_ERR_UNSUPPORTED;
         }
     }
@@ -350,65 +409,64 @@ __device__ inline bool halo_vertex_of(const GridDev &g, long long h, int &i, int
     return true;
 }
 
-template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_halo_planes(GridDev g, T *__restrict__ v,
-                                                            long long plane, int nplanes) {
+// node halo copies: NV values per node, stored per node with stride `vstride`
+// (vectors: NV = 2 interleaved components) or through aidx (matrix: 108)
+template <typename T, bool MAT>
+__global__ __launch_bounds__(CISM_BLOCK) void k_halo(GridDev g, T *__restrict__ v) {
     const long long nh = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz;
+    const int NV = MAT ? 108 : 2;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nh * nplanes) return;
-    const int p = (int)(t / nh);
+    if (t >= nh * NV) return;
+    const int p = (int)(t / nh);           // value index (block*27+slot, or component)
     const long long hv = t % nh;
     const int k = (int)(hv % g.nz);
     int i, j;
     if (!halo_vertex_of(g, hv / g.nz, i, j)) return;
     const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
-    T *vp = v + p * plane;
     const long long qd = k + (long long)g.nz * (i + (long long)g.sx * j);
-    if (ii < 0 || jj < 0) vp[qd] = (T)0;
-    else vp[qd] = vp[k + (long long)g.nz * (ii + (long long)g.sx * jj)];
+    const long long qs = k + (long long)g.nz * (ii + (long long)g.sx * jj);
+    const long long dst = MAT ? aidx(p / 27, p % 27, qd) : 2 * qd + p;
+    const long long src = MAT ? aidx(p / 27, p % 27, qs) : 2 * qs + p;
+    v[dst] = (ii < 0 || jj < 0) ? (T)0 : v[src];
 }
 
-template <typename T>
-static int launch_halo_planes(cism_ctx *c, T *v, int nplanes) {
+template <typename T, bool MAT>
+static int launch_halo_t(cism_ctx *c, T *v) {
     const GridDev &g = c->g;
-    const long long nh = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz * nplanes;
-    hipLaunchKernelGGL(k_halo_planes<T>, dim3(nblocks_for(nh)), dim3(CISM_BLOCK), 0, c->stream, g, v,
-                       g.npad, nplanes);
+    const long long nh = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz * (MAT ? 108 : 2);
+    hipLaunchKernelGGL((k_halo<T, MAT>), dim3(nblocks_for(nh)), dim3(CISM_BLOCK), 0, c->stream, g, v);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
 
-int launch_halo_vec(cism_ctx *c, double *v) { return launch_halo_planes<double>(c, v, 2); }
-int launch_halo_mat(cism_ctx *c, double *A) { return launch_halo_planes<double>(c, A, 108); }
+int launch_halo_vec(cism_ctx *c, double *v) { return launch_halo_t<double, false>(c, v); }
+int launch_halo_mat(cism_ctx *c, double *A) { return launch_halo_t<double, true>(c, A); }
 
 // ---------------------------------------------------------------------------
-// vector updates
+// vector updates (elementwise over the 2*nn interleaved entries)
 // ---------------------------------------------------------------------------
 // standard PCG: d = z + beta*d (pcg:383-393)
 template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_std_d(long long nn, long long np,
-                                                      const T *__restrict__ z, T *__restrict__ d,
+__global__ __launch_bounds__(CISM_BLOCK) void k_std_d(long long n2, const T *__restrict__ z,
+                                                      T *__restrict__ d,
                                                       const DevScal *__restrict__ sc) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nn) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
     const T beta = (T)sc->beta;
-    d[q] = z[q] + beta * d[q];
-    d[np + q] = z[np + q] + beta * d[np + q];
+    d[t] = z[t] + beta * d[t];
 }
 
 // standard PCG: x += alpha*d, r -= alpha*q (pcg:466-472)
 template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long nn, long long np,
-                                                       const T *__restrict__ d, const T *__restrict__ qv,
-                                                       T *__restrict__ x, T *__restrict__ r,
+__global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long n2, const T *__restrict__ d,
+                                                       const T *__restrict__ qv, T *__restrict__ x,
+                                                       T *__restrict__ r,
                                                        const DevScal *__restrict__ sc) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nn) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
     const T alpha = (T)sc->alpha;
-    x[q] = x[q] + alpha * d[q];
-    x[np + q] = x[np + q] + alpha * d[np + q];
-    r[q] = r[q] - alpha * qv[q];
-    r[np + q] = r[np + q] - alpha * qv[np + q];
+    x[t] = x[t] + alpha * d[t];
+    r[t] = r[t] - alpha * qv[t];
 }
 
 // Chron-Gear (pcg:1517-1536) with the halo update of Az (pcg:1494-1497) folded
@@ -419,49 +477,37 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
                                                           T *__restrict__ d, T *__restrict__ qv,
                                                           T *__restrict__ x, T *__restrict__ r,
                                                           const DevScal *__restrict__ sc, int first) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nn) return;
-    const long long np = g.npad;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * g.nn) return;
+    const long long q = t >> 1;
+    const int c = (int)(t & 1);
     const T alpha = (T)sc->alpha, beta = (T)sc->beta;
-    T azu, azv;
+    T az;
     {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         if (is_owned(g, i, j)) {
-            azu = Az[q];
-            azv = Az[np + q];
+            az = Az[t];
         } else {
             const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
-            if (ii < 0 || jj < 0) {
-                azu = azv = (T)0;
-            } else {
-                const long long qs = k + (long long)g.nz * (ii + (long long)g.sx * jj);
-                azu = Az[qs];
-                azv = Az[np + qs];
-            }
+            az = (ii < 0 || jj < 0) ? (T)0
+                                    : Az[2 * (k + (long long)g.nz * (ii + (long long)g.sx * jj)) + c];
         }
     }
-    T du, dv, qu, qvv;
+    T dd, qq;
     if (first) {   // first pass: d = z, q = A d (already halo-mapped), pcg:1351-1405
-        du = z[q]; dv = z[np + q];
-        qu = azu; qvv = azv;
+        dd = z[t];
+        qq = az;
     } else {
-        du = z[q] + beta * d[q];
-        dv = z[np + q] + beta * d[np + q];
-        qu = azu + beta * qv[q];
-        qvv = azv + beta * qv[np + q];
+        dd = z[t] + beta * d[t];
+        qq = az + beta * qv[t];
     }
-    d[q] = du; d[np + q] = dv;
-    qv[q] = qu; qv[np + q] = qvv;
-    x[q] = x[q] + alpha * du;
-    x[np + q] = x[np + q] + alpha * dv;
-    r[q] = r[q] - alpha * qu;
-    r[np + q] = r[np + q] - alpha * qvv;
+    d[t] = dd;
+    qv[t] = qq;
+    x[t] = x[t] + alpha * dd;
+    r[t] = r[t] - alpha * qq;
 }
-
-
-int launch_halo_n(cism_ctx *c, double *v, int nplanes) { return launch_halo_planes<double>(c, v, nplanes); }
 
 // ---------------------------------------------------------------------------
 // PCG drivers
@@ -492,6 +538,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_zero2(long long n2, T *__restric
 template <typename T>
 struct CGVecs {
     T *x, *b, *r, *d, *z, *q, *Az;
+    T *F;   // column factors of the SIA preconditioner (6*nn)
 };
 
 // The reference iterations on a generic precision.  check = 0: every fifth
@@ -499,13 +546,14 @@ struct CGVecs {
 // reference rule, pcg:478-537, 1542-1597).  check = 1 (inner solves of the
 // mixed-precision refinement): the recursive residual r is measured instead,
 // with no extra matvec, against tol * |r0|.
-template <typename T, typename TA>
-static int run_cg(cism_ctx *c, const TA *A, CGVecs<T> v, int chrongear, int precond, int maxiters,
+template <typename T>
+static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int precond, int maxiters,
                   double tol, int check, double *err_out, int *niters_out) {
     const GridDev &g = c->g;
-    const int nb = red_blocks_for(g.nn), nbe = nblocks_for(g.nn);
+    const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
     const long long np = g.npad;
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc);
+    if (precond == 2) CISM_TRY(launch_sia_setup<T>(c, A, v.F));
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
                        v.d, v.z, v.q);
     // (b, b) -> L2_rhs
@@ -515,13 +563,14 @@ static int run_cg(cism_ctx *c, const TA *A, CGVecs<T> v, int chrongear, int prec
     double err = 0.0;
     auto check_conv = [&](int n, bool &done) -> int {
         if (check == 0) {
-            hipLaunchKernelGGL((k_residual<T, TA>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, A,
+            hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);
+            CISM_TRY(launch_reduce(c, nbm, 1, RED_ERR));
         } else {
             hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.r, v.r,
                                c->partials);
+            CISM_TRY(launch_reduce(c, nb, 1, RED_ERR));
         }
-        CISM_TRY(launch_reduce(c, nb, 1, RED_ERR));
         CISM_TRY(sync_scalars(c));
         if (c->sc_host->nan_flag) {
             cism_set_error("PCG solver has failed, alpha = NaN");
@@ -535,44 +584,44 @@ static int run_cg(cism_ctx *c, const TA *A, CGVecs<T> v, int chrongear, int prec
     bool done = false;
     if (!chrongear) {
         // pcg:253-304: halo x, r = b - A x (no halo on r)
-        CISM_TRY(launch_halo_planes<T>(c, v.x, 2));
-        hipLaunchKernelGGL((k_residual<T, TA>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
+        CISM_TRY((launch_halo_t<T, false>(c, v.x)));
+        hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
                            v.x, v.b, v.r, 0, c->partials);
         for (int n = 1; n <= maxiters && !done; ++n) {
-            CISM_TRY(launch_precond<T, TA>(c, precond, A, v.r, v.z));
+            CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
             hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.r, v.z,
                                c->partials);
             CISM_TRY(launch_reduce(c, nb, 1, RED_STD_ETA1));
-            hipLaunchKernelGGL(k_std_d<T>, dim3(nbe), dim3(CISM_BLOCK), 0, c->stream, g.nn, np, v.z,
+            hipLaunchKernelGGL(k_std_d<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.z,
                                v.d, c->sc);
-            CISM_TRY(launch_halo_planes<T>(c, v.d, 2));
-            hipLaunchKernelGGL((k_matvec_dots<T, TA, 1>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
+            CISM_TRY((launch_halo_t<T, false>(c, v.d)));
+            hipLaunchKernelGGL((k_matvec_dots<T, 1>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g,
                                A, c->av, v.d, v.q, (const T *)nullptr, c->partials);
-            CISM_TRY(launch_reduce(c, nb, 1, RED_STD_ETA2));
-            hipLaunchKernelGGL(k_std_xr<T>, dim3(nbe), dim3(CISM_BLOCK), 0, c->stream, g.nn, np, v.d,
+            CISM_TRY(launch_reduce(c, nbm, 1, RED_STD_ETA2));
+            hipLaunchKernelGGL(k_std_xr<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.d,
                                v.q, v.x, v.r, c->sc);
             if (n % 5 == 0) {
-                if (check == 0) CISM_TRY(launch_halo_planes<T>(c, v.x, 2));
+                if (check == 0) CISM_TRY((launch_halo_t<T, false>(c, v.x)));
                 CISM_TRY(check_conv(n, done));
             }
         }
     } else {
         // pcg:1258-1405: halo x; r = b - A x; halo r; z = M^-1 r; d = z; q = A d
-        CISM_TRY(launch_halo_planes<T>(c, v.x, 2));
-        hipLaunchKernelGGL((k_residual<T, TA>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
+        CISM_TRY((launch_halo_t<T, false>(c, v.x)));
+        hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
                            v.x, v.b, v.r, 1, c->partials);
-        CISM_TRY(launch_precond<T, TA>(c, precond, A, v.r, v.z));
-        hipLaunchKernelGGL((k_matvec_dots<T, TA, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, A,
+        CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
+        hipLaunchKernelGGL((k_matvec_dots<T, 2>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                            c->av, v.z, v.Az, v.r, c->partials);
-        CISM_TRY(launch_reduce(c, nb, 2, RED_CG_FIRST));
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
+        CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_FIRST));
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
                            v.d, v.q, v.x, v.r, c->sc, 1);
         for (int n = 2; n <= maxiters && !done; ++n) {
-            CISM_TRY(launch_precond<T, TA>(c, precond, A, v.r, v.z));
-            hipLaunchKernelGGL((k_matvec_dots<T, TA, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream,
+            CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
+            hipLaunchKernelGGL((k_matvec_dots<T, 2>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
-            CISM_TRY(launch_reduce(c, nb, 2, RED_CG_ITER));
-            hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
+            CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_ITER));
+            hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
                                v.d, v.q, v.x, v.r, c->sc, 0);
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
@@ -590,8 +639,8 @@ static int run_cg(cism_ctx *c, const TA *A, CGVecs<T> v, int chrongear, int prec
 // Solves A x = b for c->x (initial guess in place) with the FP64 matrix c->A.
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol, double *err,
                  int *niters) {
-    CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az};
-    return run_cg<double, double>(c, c->A, v, chrongear, precond, maxiters, tol, 0, err, niters);
+    CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az, c->F64};
+    return run_cg<double>(c, c->A, v, chrongear, precond, maxiters, tol, 0, err, niters);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,6 +653,7 @@ int ensure_a32(cism_ctx *c) {
     float **vs[] = {&c->x32, &c->b32, &c->r32, &c->d32, &c->z32, &c->q32, &c->Az32};
     for (float **p : vs)
         if (e == hipSuccess) e = hipMalloc(p, sizeof(float) * v2);
+    if (e == hipSuccess) e = hipMalloc(&c->F32, sizeof(float) * 6 * (size_t)c->g.npad);
     if (e != hipSuccess) {
         cism_set_error("FP32 matrix allocation failed: %s", hipGetErrorString(e));
         return CISM_ERR_NOMEM;
@@ -662,12 +712,12 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
     const double bnorm = sqrt(c->sc_host->sums[0]);
     int total = 0, outer = 0;
     double err = 0.0;
-    CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32};
+    CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32};
     while (true) {
         CISM_TRY(launch_halo_vec(c, c->x));
-        hipLaunchKernelGGL((k_residual<double, double>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->A,
-                           c->av, c->x, c->b, c->r, 1, c->partials);
-        CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
+        hipLaunchKernelGGL((k_residual<double>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
+                           g, c->A, c->av, c->x, c->b, c->r, 1, c->partials);
+        CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
         hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc);
         CISM_TRY(sync_scalars(c));
         ++*n64;
@@ -678,7 +728,7 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         double e_in = 0.0;
         int it = 0;
         // the inner reductions use sums[0..1]; the scale |r| stays in sums[2]
-        CISM_TRY(run_cg<float, float>(c, c->A32, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it));
+        CISM_TRY(run_cg<float>(c, c->A32, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it));
         hipLaunchKernelGGL(k_refine, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x32, c->sc,
                            c->x);
         HIP_TRY(hipGetLastError());
@@ -697,12 +747,12 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
 // one launch of the solver's matvec exactly as the Chron-Gear iteration issues it
 int launch_cg_matvec(cism_ctx *c, int fp32) {
     const GridDev &g = c->g;
-    const int nb = red_blocks_for(g.nn);
+    const int nb = nblocks_for(g.nn);
     if (fp32)
-        hipLaunchKernelGGL((k_matvec_dots<float, float, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
+        hipLaunchKernelGGL((k_matvec_dots<float, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
                            c->A32, c->av, c->z32, c->Az32, c->r32, c->partials);
     else
-        hipLaunchKernelGGL((k_matvec_dots<double, double, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream,
+        hipLaunchKernelGGL((k_matvec_dots<double, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream,
                            g, c->A, c->av, c->z, c->Az, c->r, c->partials);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
@@ -718,9 +768,9 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_err_sums(GridDev g, const double
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         if (!is_owned(g, i, j)) continue;
-        const double du = x[q] - xr[q], dv = x[g.npad + q] - xr[g.npad + q];
+        const double du = x[2 * q] - xr[2 * q], dv = x[2 * q + 1] - xr[2 * q + 1];
         acc[0] += du * du + dv * dv;
-        acc[1] += xr[q] * xr[q] + xr[g.npad + q] * xr[g.npad + q];
+        acc[1] += xr[2 * q] * xr[2 * q] + xr[2 * q + 1] * xr[2 * q + 1];
     }
     block_reduce_store<2>(acc, partials);
 }
