@@ -96,6 +96,17 @@ def build_fortran():
     return FLIB
 
 
+def build_fortran_driver():
+    """Test driver of the Fortran module (tests/fortran); built beside its source."""
+    src = os.path.join(ROOT, "tests", "fortran", "fortran_driver.F90")
+    if not os.path.exists(src) or not os.path.exists(FLIB):
+        return None
+    exe = os.path.join(ROOT, "tests", "fortran", "fortran_driver")
+    _run([FLANG, "-O2", src, "-o", exe, "-I", os.path.join(OBJ, "mod"), "-L" + PKG,
+          "-lcism_fortran", "-lcism_gpu", "-Wl,-rpath,$ORIGIN/../../cism_amd"])
+    return exe
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="HIP library only")
@@ -104,6 +115,7 @@ def main(argv=None):
     build_lib(force=not a.incremental)
     if not a.lib:
         build_fortran()
+        build_fortran_driver()
 
 
 if __name__ == "__main__":

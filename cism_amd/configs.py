@@ -225,6 +225,10 @@ class Problem:
     uvel: np.ndarray
     vvel: np.ndarray
     whichbabc: int
+    ice_mask: np.ndarray = None
+    floating_mask: np.ndarray = None
+    ocean_mask: np.ndarray = None
+    land_mask: np.ndarray = None
     whichefvs: int = HO_EFVS_NONLINEAR
     whichapprox: int = HO_APPROX_BP
     whichresid: int = HO_RESID_L2NORM
@@ -253,7 +257,7 @@ class Problem:
 
 
 def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whichbabc,
-            beta_vertex=None, periodic=True, flwa=None, **kw) -> Problem:
+            beta_vertex=None, periodic=True, flwa=None, f_ground=None, **kw) -> Problem:
     nx, ny = ewn + 2 * NHALO, nsn + 2 * NHALO
     sigma = glide_sigma(nz)
     eus = 0.0
@@ -271,6 +275,8 @@ def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whic
         beta[:] = 1.0e10
     elif whichbabc == HO_BABC_BETA_EXTERNAL and beta_vertex is not None:
         beta[:] = beta_vertex
+    if f_ground is not None:            # calcbeta weights beta by the grounded fraction
+        beta[:] = beta * f_ground
     um = np.zeros((nz, nx - 1, ny - 1), dtype=np.int32, order="F")
     vm = np.zeros((nz, nx - 1, ny - 1), dtype=np.int32, order="F")
     if whichbabc == HO_BABC_NO_SLIP:
@@ -282,7 +288,8 @@ def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whic
                 stagusrf=stagusrf, stagthck=stagthck, dusrf_dx=dsdx, dusrf_dy=dsdy,
                 flwa=flwa, flwafact=ff, beta_internal=beta, umask_dirichlet=um,
                 vmask_dirichlet=vm, uvel=fzeros((nz, nx - 1, ny - 1)),
-                vvel=fzeros((nz, nx - 1, ny - 1)), whichbabc=whichbabc, **kw)
+                vvel=fzeros((nz, nx - 1, ny - 1)), whichbabc=whichbabc, ice_mask=ice,
+                floating_mask=floating, ocean_mask=ocean, land_mask=land, **kw)
     return p
 
 
@@ -375,6 +382,59 @@ def slab(ewn=40, nsn=40, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, efvs=2336041
                    HO_BABC_NO_SLIP, whichefvs=HO_EFVS_CONSTANT, efvs_constant=efvs, **kw)
 
 
+def grounded_fraction_vertex(ice, floating):
+    """Fraction of the ice cells around each vertex that are grounded (0 where no ice); a
+    flotation-criterion stand-in for glissade_grounded_fraction without a grounding-line
+    parameterisation."""
+    ice = ice.astype(np.float64)
+    gr = ice * (1 - floating)
+    n_ice = ice[:-1, :-1] + ice[1:, :-1] + ice[:-1, 1:] + ice[1:, 1:]
+    n_gr = gr[:-1, :-1] + gr[1:, :-1] + gr[:-1, 1:] + gr[1:, 1:]
+    return np.asfortranarray(np.where(n_ice > 0, n_gr / np.maximum(n_ice, 1), 0.0))
+
+
+def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
+              beta_median=1.0e3, beta_sigma=1.0, A=1.0e-16, thklim=100.0,
+              whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA, **kw) -> Problem:
+    """Large single-GPU ice sheet: multiscale random thickness in [0, H_max] under an
+    elliptical envelope, random bed relief (sd bed_sd) on a bed that deepens toward the
+    domain edge, ice afloat where rho_i H < rho_o (-b) (a floating fringe), ice-free ocean
+    beyond; lognormal beta on grounded vertices weighted by the grounded fraction; lateral
+    ice-ocean pressure at the margin.  Floating ice not connected to grounded ice is
+    removed (an unsupported floating patch has no unique velocity)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    Lx, Ly = ewn * dx, nsn * dx
+    x = (np.arange(ewn) + 0.5) * dx - Lx / 2
+    y = (np.arange(nsn) + 0.5) * dx - Ly / 2
+    rr = np.sqrt((x[:, None] / (0.42 * Lx)) ** 2 + (y[None, :] / (0.42 * Ly)) ** 2)
+    env = np.sqrt(np.clip(1.0 - rr ** 2, 0.0, None))
+    g = (0.6 * gaussian_random_field(ewn, nsn, dx, 150e3, rng)
+         + 0.3 * gaussian_random_field(ewn, nsn, dx, 40e3, rng)
+         + 0.1 * gaussian_random_field(ewn, nsn, dx, 10e3, rng))
+    H = np.clip(H_max * env * (0.75 + 0.25 * g), 0.0, H_max)
+    bed = 400.0 - 1400.0 * rr ** 2 + bed_sd * gaussian_random_field(ewn, nsn, dx, 50e3, rng)
+    fl = RHOI * H < RHOO * (-bed)
+    grounded = (H > thklim) & ~fl
+    lab, _ = ndimage.label(H > thklim)
+    keep = np.unique(lab[grounded])
+    keep = keep[keep > 0]
+    H = np.where(np.isin(lab, keep), H, 0.0)
+    s = np.where(RHOI * H < RHOO * (-bed), H * (1.0 - RHOI / RHOO), bed + H)
+    thck = pad_periodic(H)
+    topg = pad_periodic(bed)
+    usrf = pad_periodic(s)
+    ice, floating, _, _, _ = get_masks(thck, topg, 0.0, thklim)
+    fg = grounded_fraction_vertex(ice, floating)
+    bo = np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn)))
+    bv = np.zeros((ewn + 3, nsn + 3))
+    bv[2:ewn + 2, 2:nsn + 2] = bo
+    bv = stag_halo(np.asfortranarray(bv), True)
+    return _finish("ice_sheet", ewn, nsn, nz, dx, dx, thck, usrf, topg, thklim, A,
+                   HO_BABC_BETA_EXTERNAL, beta_vertex=bv, f_ground=fg,
+                   whichsparse=whichsparse, whichprecond=whichprecond, **kw)
+
+
 def build(name: str, **kw) -> Problem:
     if name in ("dome", "c1"):
         return dome(**kw)
@@ -384,4 +444,6 @@ def build(name: str, **kw) -> Problem:
         return grf_sheet(**kw)
     if name == "slab":
         return slab(**kw)
+    if name in ("ice_sheet", "c5"):
+        return ice_sheet(**kw)
     raise ValueError(f"unknown problem {name}")

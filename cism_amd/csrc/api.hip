@@ -500,6 +500,32 @@ extern "C" int cism_load_vector_gravity(cism_ctx *c, const double *sigma, const 
     return download_vec(c, c->load, loadu, loadv);
 }
 
+extern "C" int cism_load_vector_lateral_bc(cism_ctx *c, const double *sigma, const int32_t *land_mask,
+                                           const int32_t *ocean_mask,
+                                           const int32_t *calving_front_mask,
+                                           const int32_t *active_cell, const double *xVertex,
+                                           const double *yVertex, const double *stagusrf,
+                                           const double *stagthck, double *loadu, double *loadv) {
+    CTX_BEGIN(c);
+    (void)land_mask;
+    const size_t ncol = (size_t)c->g.sx * c->g.sy, ncell = (size_t)c->g.nx * c->g.ny;
+    CISM_TRY(up(c, c->sigma, sigma, (size_t)c->g.nz));
+    CISM_TRY(up(c, c->ac, active_cell, ncell));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(up(c, c->susrf, stagusrf, ncol));
+    CISM_TRY(up(c, c->sthck, stagthck, ncol));
+    int32_t *masks = nullptr;
+    HIP_TRY(hipMalloc(&masks, sizeof(int32_t) * 2 * ncell));
+    int st = up(c, masks, ocean_mask, ncell);
+    if (!st) st = up(c, masks + ncell, calving_front_mask, ncell);
+    if (!st) st = upload_vec(c, c->load, loadu, loadv);
+    if (!st) st = load_lateral_device(c, masks, masks + ncell);
+    if (!st) st = download_vec(c, c->load, loadu, loadv);
+    (void)hipFree(masks);
+    return st;
+}
+
 extern "C" int cism_staggered_parallel_halo(cism_ctx *c, int lead, double *a) {
     CTX_BEGIN(c);
     const GridDev &g = c->g;

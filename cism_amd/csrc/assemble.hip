@@ -678,3 +678,110 @@ int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double d
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// load_vector_lateral_bc + lateral_shelf_bc (velo_higher:4536-4950): pressure of
+// ice against ocean on vertical faces of active cells that border an ocean cell
+// (or an inactive calving-front cell).  One lane per node gathers, in the
+// reference's order (cells j, i; faces W, E, S, N; layers; Gauss points), the
+// terms the reference scatters onto the node.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_load_lateral(GridDev g, const double *__restrict__ sigma,
+                                                      const int32_t *__restrict__ ocean,
+                                                      const int32_t *__restrict__ cfront,
+                                                      const int32_t *__restrict__ ac,
+                                                      const double *__restrict__ xV,
+                                                      const double *__restrict__ yV,
+                                                      const double *__restrict__ susrf,
+                                                      const double *__restrict__ sthck,
+                                                      double *__restrict__ load) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int k = (int)(q % g.nz) + 1;
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
+    auto C2i = [&](const int32_t *a, int ii, int jj) { return a[(ii - 1) + (long long)g.nx * (jj - 1)]; };
+    auto V2 = [&](const double *a, int ii, int jj) { return a[(ii - 1) + (long long)g.sx * (jj - 1)]; };
+    double lu = load[2 * q], lv = load[2 * q + 1];
+    bool any = false;
+    const double r = 1.0 / sqrt(3.0);
+    for (int dJ = 0; dJ < 2; ++dJ)
+        for (int dI = 0; dI < 2; ++dI) {
+            const int ic = i + dI, jc = j + dJ;
+            if (ic < CISM_NHALO + 1 || ic > g.nx - CISM_NHALO + 1 || jc < CISM_NHALO + 1 ||
+                jc > g.ny - CISM_NHALO + 1 || !C2i(ac, ic, jc))
+                continue;
+            const int ni[4] = {ic - 1, ic + 1, ic, ic}, nj[4] = {jc, jc, jc - 1, jc + 1};
+            for (int f = 0; f < 4; ++f) {
+                if (!(C2i(ocean, ni[f], nj[f]) == 1 ||
+                      (C2i(cfront, ni[f], nj[f]) == 1 && !C2i(ac, ni[f], nj[f]))))
+                    continue;
+                int in0, jn0, in1, jn1;
+                if (f == 0) { in0 = ic - 1; jn0 = jc; in1 = ic - 1; jn1 = jc - 1; }
+                else if (f == 1) { in0 = ic; jn0 = jc - 1; in1 = ic; jn1 = jc; }
+                else if (f == 2) { in0 = ic - 1; jn0 = jc - 1; in1 = ic; jn1 = jc - 1; }
+                else { in0 = ic; jn0 = jc; in1 = ic - 1; jn1 = jc; }
+                int n0;
+                if (in0 == i && jn0 == j) n0 = 0;
+                else if (in1 == i && jn1 == j) n0 = 1;
+                else continue;
+                double x[4], s[4], h[4];
+                x[0] = f <= 1 ? V2(yV, in0, jn0) : V2(xV, in0, jn0);
+                x[1] = f <= 1 ? V2(yV, in1, jn1) : V2(xV, in1, jn1);
+                x[2] = x[1]; x[3] = x[0];
+                s[0] = V2(susrf, in0, jn0); s[1] = V2(susrf, in1, jn1); s[2] = s[1]; s[3] = s[0];
+                h[0] = V2(sthck, in0, jn0); h[1] = V2(sthck, in1, jn1); h[2] = h[1]; h[3] = h[0];
+                for (int kf = k - 1; kf <= k; ++kf) {
+                    if (kf < 1 || kf > g.nz - 1) continue;
+                    const int n = kf == k - 1 ? n0 : 3 - n0;   // lower pair 0,1; upper 3 (above 0), 2 (above 1)
+                    double y[4];
+                    y[0] = s[0] - sigma[kf] * h[0];
+                    y[1] = s[1] - sigma[kf] * h[1];
+                    y[2] = s[2] - sigma[kf - 1] * h[2];
+                    y[3] = s[3] - sigma[kf - 1] * h[3];
+                    for (int p = 0; p < 4; ++p) {
+                        const double xq = (p == 0 || p == 3) ? -r : r, yq = p < 2 ? -r : r;
+                        double phi[4], dxr[4], dyr[4];
+                        for (int m = 0; m < 4; ++m) {
+                            const double sx = (m == 0 || m == 3) ? -1.0 : 1.0, sy = m < 2 ? -1.0 : 1.0;
+                            const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
+                            const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
+                            phi[m] = ax * ay / 4.0;
+                            dxr[m] = sx * (ay / 4.0);
+                            dyr[m] = sy * (ax / 4.0);
+                        }
+                        double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+                        for (int m = 0; m < 4; ++m) {
+                            J11 = J11 + dxr[m] * x[m];
+                            J12 = J12 + dxr[m] * y[m];
+                            J21 = J21 + dyr[m] * x[m];
+                            J22 = J22 + dyr[m] * y[m];
+                        }
+                        const double detJ = fabs(J11 * J22 - J12 * J21);
+                        double hq = 0.0, sq = 0.0;
+                        for (int m = 0; m < 4; ++m) { hq = hq + phi[m] * h[m]; sq = sq + phi[m] * s[m]; }
+                        const double ratio = sq / hq < 1.0 ? sq / hq : 1.0;
+                        const double p_av = 0.5 * CISM_RHOI * CISM_GRAV * hq -
+                                            0.5 * CISM_RHOO * CISM_GRAV * hq * ((1.0 - ratio) * (1.0 - ratio));
+                        const double t = p_av * 1.0 * detJ / CISM_VOL0 * phi[n];
+                        if (f == 0) lu = lu - t;
+                        else if (f == 1) lu = lu + t;
+                        else if (f == 2) lv = lv - t;
+                        else lv = lv + t;
+                        any = true;
+                    }
+                }
+            }
+        }
+    if (any) {
+        load[2 * q] = lu;
+        load[2 * q + 1] = lv;
+    }
+}
+
+int load_lateral_device(cism_ctx *c, const int32_t *ocean, const int32_t *cfront) {
+    hipLaunchKernelGGL(k_load_lateral, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream, c->g, c->sigma,
+                       ocean, cfront, c->ac, c->xV, c->yV, c->susrf, c->sthck, c->load);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}

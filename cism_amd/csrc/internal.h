@@ -30,6 +30,7 @@
 
 // physical constants (glimmer_physcon.F90:64-75) and the matrix volume scale
 #define CISM_RHOI 910.0
+#define CISM_RHOO 1028.0
 #define CISM_GRAV 9.81
 #define CISM_SCYR 31536000.0
 #define CISM_VOL0 1.0e9
@@ -223,6 +224,7 @@ int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
 int residual_vector_device(cism_ctx *c, double *ru, double *rv);  // sums[0]=|r|^2, [1]=|b|^2
 int residual_velocity_device(cism_ctx *c, int whichresid);       // sums[0]
 int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy);
+int load_lateral_device(cism_ctx *c, const int32_t *ocean, const int32_t *cfront);
 
 int ensure_stage(cism_ctx *c);
 int ensure_ke(cism_ctx *c);

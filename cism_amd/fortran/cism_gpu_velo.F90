@@ -1,0 +1,404 @@
+!-----------------------------------------------------------------------------
+! cism_gpu_velo.F90 -- Fortran drop-in for the native structured solver path of
+! Glissade's higher-order (Blatter-Pattyn) velocity solve, on one MI355X.
+!
+! The public procedures keep the names and dummy-argument lists of the
+! reference routines they replace:
+!   pcg_solver_standard_3d         glissade_velo_higher_pcg.F90:84-194
+!   pcg_solver_chrongear_3d        glissade_velo_higher_pcg.F90:965-1126
+!   matvec_multiply_structured_3d  glissade_velo_higher_pcg.F90:2533-2585
+!   assemble_stiffness_matrix_3d   glissade_velo_higher.F90:4954-5016
+! and add
+!   glissade_velo_higher_solve_3d_gpu  the Picard loop of glissade_velo_higher_solve
+!                                      (3D branch), device-resident, with explicit
+!                                      arrays instead of the model derived type
+!   cism_gpu_velo_init / _finalize     context lifetime (glissade_velo_higher_init)
+!
+! Each call converts LOGICAL masks to int32 (LOGICAL's kind and .true. value
+! are compiler specific) and calls libcism_gpu.so through bind(C).  A non-zero
+! status is fatal, as write_log(msg, GM_FATAL) is in the reference.
+!-----------------------------------------------------------------------------
+module cism_gpu_velo
+
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter :: dp = c_double
+
+  public :: pcg_solver_standard_3d, pcg_solver_chrongear_3d, matvec_multiply_structured_3d
+  public :: assemble_stiffness_matrix_3d, glissade_velo_higher_solve_3d_gpu
+  public :: cism_gpu_velo_init, cism_gpu_velo_finalize, cism_gpu_set_linear_solver
+
+  type, bind(C) :: cism_solve_opts
+     integer(c_int) :: whichbabc, whichefvs, whichresid, whichsparse, whichprecond
+     integer(c_int) :: whichapprox, maxiter_nonlinear, precision
+     real(c_double) :: efvs_constant, dx, dy, target_override, inner_rtol
+     integer(c_int) :: adaptive_linear
+     real(c_double) :: adaptive_gamma, stop_rel_err
+     integer(c_int) :: reserved(5)
+  end type cism_solve_opts
+
+  type, bind(C) :: cism_solve_stats
+     integer(c_int) :: picard_iters, linear_iters, fp64_matvecs, fp32_matvecs
+     real(c_double) :: final_L2_norm, final_resid_velo, seconds_assembly, seconds_linear
+     real(c_double) :: seconds_total
+     integer(c_int) :: status, refinement_steps
+     real(c_double) :: final_rel_err
+     integer(c_int) :: reserved(4)
+  end type cism_solve_stats
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: ctx_nx = -1, ctx_ny = -1, ctx_nz = -1
+  logical, save :: ctx_periodic = .true.
+
+  interface
+     integer(c_int) function c_ctx_create(nx, ny, nz, nhalo, periodic, device, out) &
+          bind(C, name='cism_ctx_create')
+       import :: c_int, c_ptr
+       integer(c_int), value :: nx, ny, nz, nhalo, periodic, device
+       type(c_ptr) :: out
+     end function c_ctx_create
+
+     integer(c_int) function c_ctx_destroy(h) bind(C, name='cism_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function c_ctx_destroy
+
+     type(c_ptr) function c_last_error() bind(C, name='cism_last_error')
+       import :: c_ptr
+     end function c_last_error
+
+     integer(c_int) function c_set_linear_solver(h, maxiters, tolerance) &
+          bind(C, name='cism_ctx_set_linear_solver')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: maxiters
+       real(c_double), value :: tolerance
+     end function c_set_linear_solver
+
+     integer(c_int) function c_matvec(h, indxA, av, Auu, Auv, Avu, Avv, xu, xv, yu, yv) &
+          bind(C, name='cism_matvec_multiply_structured_3d')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t) :: indxA(*), av(*)
+       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), xu(*), xv(*), yu(*), yv(*)
+     end function c_matvec
+
+     integer(c_int) function c_pcg_standard(h, indxA, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+          precond, err, niters) bind(C, name='cism_pcg_solver_standard_3d')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t) :: indxA(*), av(*)
+       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*), xu(*), xv(*)
+       integer(c_int), value :: precond
+       real(c_double) :: err
+       integer(c_int) :: niters
+     end function c_pcg_standard
+
+     integer(c_int) function c_pcg_chrongear(h, indxA, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+          precond, err, niters) bind(C, name='cism_pcg_solver_chrongear_3d')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t) :: indxA(*), av(*)
+       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*), xu(*), xv(*)
+       integer(c_int), value :: precond
+       real(c_double) :: err
+       integer(c_int) :: niters
+     end function c_pcg_chrongear
+
+     integer(c_int) function c_assemble(h, sigma, ac, xV, yV, uvel, vvel, susrf, sthck, flwafact, &
+          whichapprox, efvs, whichefvs, efvs_constant, Auu, Auv, Avu, Avv) &
+          bind(C, name='cism_assemble_stiffness_matrix_3d')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: sigma(*), xV(*), yV(*), uvel(*), vvel(*), susrf(*), sthck(*), flwafact(*)
+       integer(c_int32_t) :: ac(*)
+       integer(c_int), value :: whichapprox, whichefvs
+       real(c_double) :: efvs(*)
+       real(c_double), value :: efvs_constant
+       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*)
+     end function c_assemble
+
+     subroutine c_solve_opts_default(o) bind(C, name='cism_solve_opts_default')
+       import :: cism_solve_opts
+       type(cism_solve_opts) :: o
+     end subroutine c_solve_opts_default
+
+     integer(c_int) function c_solve(h, o, sigma, ac, av, xV, yV, susrf, sthck, flwafact, beta, &
+          umask, vmask, loadu, loadv, uvel, vvel, efvs, bu, bv, ru, rv, stats) &
+          bind(C, name='cism_velo_higher_solve_3d')
+       import :: c_int, c_int32_t, c_ptr, c_double, cism_solve_opts, cism_solve_stats
+       type(c_ptr), value :: h
+       type(cism_solve_opts) :: o
+       real(c_double) :: sigma(*), xV(*), yV(*), susrf(*), sthck(*), flwafact(*), beta(*)
+       integer(c_int32_t) :: ac(*), av(*), umask(*), vmask(*)
+       real(c_double) :: loadu(*), loadv(*), uvel(*), vvel(*), efvs(*), bu(*), bv(*), ru(*), rv(*)
+       type(cism_solve_stats) :: stats
+     end function c_solve
+  end interface
+
+contains
+
+  !> Creates the device context (the analogue of glissade_velo_higher_init).
+  !> periodic: the reference's periodic_bc of module parallel (default .true.).
+  subroutine cism_gpu_velo_init(nx, ny, nz, periodic)
+    integer, intent(in) :: nx, ny, nz
+    logical, intent(in), optional :: periodic
+    integer(c_int) :: st, iper
+    if (present(periodic)) ctx_periodic = periodic
+    if (c_associated(ctx)) then
+       if (nx == ctx_nx .and. ny == ctx_ny .and. nz == ctx_nz) return
+       call cism_gpu_velo_finalize()
+    endif
+    iper = merge(1, 0, ctx_periodic)
+    st = c_ctx_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), 2_c_int, iper, 0_c_int, ctx)
+    call check(st, 'cism_gpu_velo_init')
+    ctx_nx = nx
+    ctx_ny = ny
+    ctx_nz = nz
+  end subroutine cism_gpu_velo_init
+
+  subroutine cism_gpu_velo_finalize()
+    integer(c_int) :: st
+    if (c_associated(ctx)) st = c_ctx_destroy(ctx)
+    ctx = c_null_ptr
+    ctx_nx = -1
+  end subroutine cism_gpu_velo_finalize
+
+  !> maxiters and tolerance of the linear solvers (module constants of the reference)
+  subroutine cism_gpu_set_linear_solver(nx, ny, nz, maxiters, tolerance)
+    integer, intent(in) :: nx, ny, nz, maxiters
+    real(dp), intent(in) :: tolerance
+    call cism_gpu_velo_init(nx, ny, nz)
+    call check(c_set_linear_solver(ctx, int(maxiters, c_int), tolerance), 'set_linear_solver')
+  end subroutine cism_gpu_set_linear_solver
+
+  subroutine check(st, where)
+    integer(c_int), intent(in) :: st
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    character(len=512) :: text
+    if (st == 0) return
+    call c_f_pointer(c_last_error(), msg, [512])
+    text = ' '
+    do n = 1, 512
+       if (msg(n) == c_null_char) exit
+       text(n:n) = msg(n)
+    enddo
+    write(*,'(a,a,a,i0,a,a)') 'FATAL (', where, '): status ', st, ': ', trim(text)
+    error stop 1
+  end subroutine check
+
+  subroutine to_int_mask(l, m)
+    logical, intent(in) :: l(:,:)
+    integer(c_int32_t), intent(out) :: m(:,:)
+    m = merge(1_c_int32_t, 0_c_int32_t, l)
+  end subroutine to_int_mask
+
+  !---------------------------------------------------------------------------
+  subroutine matvec_multiply_structured_3d(nx,        ny,            &
+                                           nz,        nhalo,         &
+                                           indxA,     active_vertex, &
+                                           Auu,       Auv,           &
+                                           Avu,       Avv,           &
+                                           xu,        xv,            &
+                                           yu,        yv)
+    integer, intent(in) :: nx, ny, nz, nhalo
+    integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(27,nz,nx-1,ny-1), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: xu, xv
+    real(dp), dimension(nz,nx-1,ny-1), intent(out) :: yu, yv
+    integer(c_int32_t) :: av(nx-1,ny-1), ind(27)
+    real(dp) :: a1(27,nz,nx-1,ny-1), a2(27,nz,nx-1,ny-1), a3(27,nz,nx-1,ny-1), a4(27,nz,nx-1,ny-1)
+    real(dp) :: x1(nz,nx-1,ny-1), x2(nz,nx-1,ny-1)
+    if (nhalo /= 2) call check(5_c_int, 'matvec: nhalo must be 2')
+    call cism_gpu_velo_init(nx, ny, nz)
+    call to_int_mask(active_vertex, av)
+    ind = int(reshape(indxA, [27]), c_int32_t)
+    a1 = Auu; a2 = Auv; a3 = Avu; a4 = Avv; x1 = xu; x2 = xv
+    call check(c_matvec(ctx, ind, av, a1, a2, a3, a4, x1, x2, yu, yv), &
+               'matvec_multiply_structured_3d')
+  end subroutine matvec_multiply_structured_3d
+
+  !---------------------------------------------------------------------------
+  subroutine pcg_solver_standard_3d(nx,        ny,            &
+                                    nz,        nhalo,         &
+                                    indxA,     active_vertex, &
+                                    Auu,       Auv,           &
+                                    Avu,       Avv,           &
+                                    bu,        bv,            &
+                                    xu,        xv,            &
+                                    precond,   err,           &
+                                    niters,                   &
+                                    itest_in,  jtest_in,  rtest_in,   &
+                                    verbose)
+    integer, intent(in) :: nx, ny, nz, nhalo
+    integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(27,nz,nx-1,ny-1), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: bu, bv
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: xu, xv
+    integer, intent(in) :: precond
+    real(dp), intent(out) :: err
+    integer, intent(out) :: niters
+    integer, intent(in), optional :: itest_in, jtest_in, rtest_in
+    logical, intent(in), optional :: verbose
+    call pcg_common(0, nx, ny, nz, nhalo, indxA, active_vertex, Auu, Auv, Avu, Avv, &
+                    bu, bv, xu, xv, precond, err, niters)
+  end subroutine pcg_solver_standard_3d
+
+  !---------------------------------------------------------------------------
+  subroutine pcg_solver_chrongear_3d(nx,        ny,            &
+                                     nz,        nhalo,         &
+                                     indxA,     active_vertex, &
+                                     Auu,       Auv,           &
+                                     Avu,       Avv,           &
+                                     bu,        bv,            &
+                                     xu,        xv,            &
+                                     precond,   err,           &
+                                     niters,                   &
+                                     itest_in,  jtest_in,  rtest_in,   &
+                                     verbose)
+    integer, intent(in) :: nx, ny, nz, nhalo
+    integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(27,nz,nx-1,ny-1), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: bu, bv
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: xu, xv
+    integer, intent(in) :: precond
+    real(dp), intent(out) :: err
+    integer, intent(out) :: niters
+    integer, intent(in), optional :: itest_in, jtest_in, rtest_in
+    logical, intent(in), optional :: verbose
+    call pcg_common(1, nx, ny, nz, nhalo, indxA, active_vertex, Auu, Auv, Avu, Avv, &
+                    bu, bv, xu, xv, precond, err, niters)
+  end subroutine pcg_solver_chrongear_3d
+
+  subroutine pcg_common(kind_cg, nx, ny, nz, nhalo, indxA, active_vertex, Auu, Auv, Avu, Avv, &
+                        bu, bv, xu, xv, precond, err, niters)
+    integer, intent(in) :: kind_cg, nx, ny, nz, nhalo
+    integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(27,nz,nx-1,ny-1), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: bu, bv
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: xu, xv
+    integer, intent(in) :: precond
+    real(dp), intent(out) :: err
+    integer, intent(out) :: niters
+    integer(c_int32_t) :: av(nx-1,ny-1), ind(27)
+    real(c_double) :: e
+    integer(c_int) :: n, st
+    real(dp), allocatable :: a1(:,:,:,:), a2(:,:,:,:), a3(:,:,:,:), a4(:,:,:,:), b1(:,:,:), b2(:,:,:)
+    if (nhalo /= 2) call check(5_c_int, 'pcg: nhalo must be 2')
+    call cism_gpu_velo_init(nx, ny, nz)
+    call to_int_mask(active_vertex, av)
+    ind = int(reshape(indxA, [27]), c_int32_t)
+    ! intent(in) arrays are passed through copies: the C side takes non-const pointers
+    allocate(a1, source=Auu); allocate(a2, source=Auv); allocate(a3, source=Avu)
+    allocate(a4, source=Avv); allocate(b1, source=bu); allocate(b2, source=bv)
+    if (kind_cg == 0) then
+       st = c_pcg_standard(ctx, ind, av, a1, a2, a3, a4, b1, b2, xu, xv, int(precond, c_int), e, n)
+       call check(st, 'pcg_solver_standard_3d')
+    else
+       st = c_pcg_chrongear(ctx, ind, av, a1, a2, a3, a4, b1, b2, xu, xv, int(precond, c_int), e, n)
+       call check(st, 'pcg_solver_chrongear_3d')
+    endif
+    err = e
+    niters = n
+  end subroutine pcg_common
+
+  !---------------------------------------------------------------------------
+  subroutine assemble_stiffness_matrix_3d(nx,               ny,              &
+                                          nz,               sigma,           &
+                                          nhalo,                             &
+                                          active_cell,                       &
+                                          xVertex,          yVertex,         &
+                                          uvel,             vvel,            &
+                                          stagusrf,         stagthck,        &
+                                          flwafact,         whichapprox,     &
+                                          efvs,             whichefvs,       &
+                                          efvs_constant,                     &
+                                          Auu,              Auv,             &
+                                          Avu,              Avv)
+    integer, intent(in) :: nx, ny, nz, nhalo
+    real(dp), dimension(nz), intent(in) :: sigma
+    logical, dimension(nx,ny), intent(in) :: active_cell
+    real(dp), dimension(nx-1,ny-1), intent(in) :: xVertex, yVertex
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: uvel, vvel
+    real(dp), dimension(nx-1,ny-1), intent(in) :: stagusrf, stagthck
+    real(dp), dimension(nz-1,nx,ny), intent(in) :: flwafact
+    integer, intent(in) :: whichapprox, whichefvs
+    real(dp), intent(in) :: efvs_constant
+    real(dp), dimension(nz-1,nx,ny), intent(out) :: efvs
+    real(dp), dimension(27,nz,nx-1,ny-1), intent(out) :: Auu, Auv, Avu, Avv
+    integer(c_int32_t) :: ac(nx,ny)
+    real(dp) :: s(nz), xv_(nx-1,ny-1), yv_(nx-1,ny-1), u(nz,nx-1,ny-1), v(nz,nx-1,ny-1)
+    real(dp) :: su(nx-1,ny-1), st_(nx-1,ny-1), ff(nz-1,nx,ny)
+    if (nhalo /= 2) call check(5_c_int, 'assemble: nhalo must be 2')
+    call cism_gpu_velo_init(nx, ny, nz)
+    ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
+    s = sigma; xv_ = xVertex; yv_ = yVertex; u = uvel; v = vvel
+    su = stagusrf; st_ = stagthck; ff = flwafact
+    call check(c_assemble(ctx, s, ac, xv_, yv_, u, v, su, st_, ff, int(whichapprox, c_int), efvs, &
+                          int(whichefvs, c_int), efvs_constant, Auu, Auv, Avu, Avv), &
+               'assemble_stiffness_matrix_3d')
+  end subroutine assemble_stiffness_matrix_3d
+
+  !---------------------------------------------------------------------------
+  !> The Picard loop of glissade_velo_higher_solve (3D branch, velo_higher:2003-3727)
+  !> from the arrays the reference prepares before the loop; uvel, vvel in/out.
+  subroutine glissade_velo_higher_solve_3d_gpu(nx, ny, nz, dx, dy, sigma, active_cell,      &
+                                               active_vertex, xVertex, yVertex,            &
+                                               stagusrf, stagthck, flwafact,               &
+                                               beta_internal, umask_dirichlet,             &
+                                               vmask_dirichlet, loadu, loadv,              &
+                                               whichbabc, whichefvs, whichresid,           &
+                                               whichsparse, whichprecond, whichapprox,     &
+                                               maxiter_nonlinear, efvs_constant,           &
+                                               uvel, vvel, efvs, bu, bv, resid_u, resid_v, &
+                                               picard_iters, linear_iters)
+    integer, intent(in) :: nx, ny, nz
+    real(dp), intent(in) :: dx, dy
+    real(dp), dimension(nz), intent(in) :: sigma
+    logical, dimension(nx,ny), intent(in) :: active_cell
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(nx-1,ny-1), intent(in) :: xVertex, yVertex, stagusrf, stagthck, beta_internal
+    real(dp), dimension(nz-1,nx,ny), intent(in) :: flwafact
+    integer, dimension(nz,nx-1,ny-1), intent(in) :: umask_dirichlet, vmask_dirichlet
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: loadu, loadv
+    integer, intent(in) :: whichbabc, whichefvs, whichresid, whichsparse, whichprecond
+    integer, intent(in) :: whichapprox, maxiter_nonlinear
+    real(dp), intent(in) :: efvs_constant
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: uvel, vvel
+    real(dp), dimension(nz-1,nx,ny), intent(out) :: efvs
+    real(dp), dimension(nz,nx-1,ny-1), intent(out) :: bu, bv, resid_u, resid_v
+    integer, intent(out) :: picard_iters, linear_iters
+    type(cism_solve_opts) :: o
+    type(cism_solve_stats) :: stats
+    integer(c_int32_t) :: ac(nx,ny), av(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1)
+    real(dp), allocatable :: s(:), xv_(:,:), yv_(:,:), su(:,:), st_(:,:), ff(:,:,:), be(:,:)
+    real(dp), allocatable :: lu(:,:,:), lv(:,:,:)
+    call cism_gpu_velo_init(nx, ny, nz)
+    call c_solve_opts_default(o)
+    o%whichbabc = whichbabc; o%whichefvs = whichefvs; o%whichresid = whichresid
+    o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
+    o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
+    o%dx = dx; o%dy = dy
+    ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
+    av = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
+    um = int(umask_dirichlet, c_int32_t)
+    vm = int(vmask_dirichlet, c_int32_t)
+    allocate(s, source=sigma); allocate(xv_, source=xVertex); allocate(yv_, source=yVertex)
+    allocate(su, source=stagusrf); allocate(st_, source=stagthck); allocate(ff, source=flwafact)
+    allocate(be, source=beta_internal); allocate(lu, source=loadu); allocate(lv, source=loadv)
+    call check(c_solve(ctx, o, s, ac, av, xv_, yv_, su, st_, ff, be, um, vm, lu, lv, &
+                       uvel, vvel, efvs, bu, bv, resid_u, resid_v, stats), &
+               'glissade_velo_higher_solve_3d_gpu')
+    picard_iters = stats%picard_iters
+    linear_iters = stats%linear_iters
+  end subroutine glissade_velo_higher_solve_3d_gpu
+
+end module cism_gpu_velo

@@ -30,6 +30,7 @@ EXPORTED = [
     "cism_count_nonzeros_3d", "cism_compute_residual_vector_3d", "cism_compute_residual_velocity_3d",
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
     "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
+    "cism_load_vector_lateral_bc",
 ]
 
 
@@ -98,6 +99,7 @@ def lib():
         L.cism_load_vector_gravity.argtypes = [VP, D, D, C.c_double, C.c_double, I32, I32, D, D,
                                                D, D, D, D, C.c_int, D, D]
         L.cism_staggered_parallel_halo.argtypes = [VP, C.c_int, D]
+        L.cism_load_vector_lateral_bc.argtypes = [VP, D, I32, I32, I32, I32, D, D, D, D, D, D]
         L.cism_solve_opts_default.argtypes = [C.POINTER(SolveOpts)]
         L.cism_solve_opts_default.restype = None
         L.cism_velo_higher_solve_3d.argtypes = [VP, C.POINTER(SolveOpts), D, I32, I32, D, D, D, D,
@@ -254,6 +256,26 @@ class Context:
         ms = C.c_double()
         _check(lib().cism_time_kernel(self.h, which, reps, C.byref(ms)))
         return ms.value
+
+    def load_vector_lateral_bc(self, sigma, land_mask, ocean_mask, calving_front_mask, active_cell,
+                               xVertex, yVertex, stagusrf, stagthck, loadu, loadv):
+        _check(lib().cism_load_vector_lateral_bc(
+            self.h, dp(sigma), ip(land_mask), ip(ocean_mask), ip(calving_front_mask),
+            ip(active_cell), dp(xVertex), dp(yVertex), dp(stagusrf), dp(stagthck), dp(loadu),
+            dp(loadv)))
+
+    def loads(self, p):
+        """Gravity plus lateral ice-ocean load, computed on the device (velo_higher:2033-2111)."""
+        shape = (p.nz, p.nx - 1, p.ny - 1)
+        lu, lv = np.zeros(shape, order="F"), np.zeros(shape, order="F")
+        self.load_vector_gravity(p.sigma, p.stagwbndsigma, p.dx, p.dy, p.active_cell,
+                                 p.active_vertex, p.xVertex, p.yVertex, p.stagusrf, p.stagthck,
+                                 p.dusrf_dx, p.dusrf_dy, p.whichassemble_taud, lu, lv)
+        if p.ocean_mask.any():
+            cf = np.zeros_like(p.ocean_mask)
+            self.load_vector_lateral_bc(p.sigma, p.land_mask, p.ocean_mask, cf, p.active_cell,
+                                        p.xVertex, p.yVertex, p.stagusrf, p.stagthck, lu, lv)
+        return lu, lv
 
     def staggered_parallel_halo(self, a):
         lead = int(np.prod(a.shape[:-2]))

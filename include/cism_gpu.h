@@ -149,6 +149,17 @@ int  cism_load_vector_gravity(cism_ctx *ctx, const double *sigma, const double *
                               const double *dusrf_dx, const double *dusrf_dy,
                               int whichassemble_taud, double *loadu, double *loadv);
 
+/* load_vector_lateral_bc (glissade_velo_higher.F90:4536-4950, with
+ * lateral_shelf_bc): adds the net ice-ocean pressure on faces of active cells
+ * that border an ocean cell (or an inactive calving-front cell) to loadu/v.
+ * land_mask is accepted for the reference's argument list and not used. */
+int  cism_load_vector_lateral_bc(cism_ctx *ctx, const double *sigma, const int32_t *land_mask,
+                                 const int32_t *ocean_mask, const int32_t *calving_front_mask,
+                                 const int32_t *active_cell,
+                                 const double *xVertex, const double *yVertex,
+                                 const double *stagusrf, const double *stagthck,
+                                 double *loadu, double *loadv);
+
 /* staggered_parallel_halo_real8_3d/_4d (parallel_slap.F90:2807-2925) for an
  * array of shape (lead, nx-1, ny-1): lead = nz (vectors) or 27*nz (blocks). */
 int  cism_staggered_parallel_halo(cism_ctx *ctx, int lead, double *a);

@@ -19,6 +19,7 @@
  *                              rigid translation u=const in the interior; symmetry
  *   orc_dirichlet_3d           explicit column-elimination formula on a dense matrix
  *   orc_halo                   explicit periodic index maps
+ *   orc_load_lateral_bc        closed form: uniform face -> p_av * H * face length
  *   orc_matvec_3d              brute-force CSR product
  *   orc_tridiag                scipy.linalg.solve_banded
  *   orc_pcg_*                  scipy.sparse.linalg.spsolve; standard vs Chron-Gear
@@ -33,7 +34,7 @@
 #include <string.h>
 
 /* glimmer_physcon.F90:64-75 (non-CESM values) */
-static const double RHOI = 910.0, GRAV = 9.81, SCYR = 31536000.0;
+static const double RHOI = 910.0, RHOO = 1028.0, GRAV = 9.81, SCYR = 31536000.0;
 static const int GN = 3;
 /* glissade_velo_higher.F90:1111 */
 static const double VOL0 = 1.0e9;
@@ -658,6 +659,88 @@ void orc_load_gravity_standard(const orc_grid *g, const double *sigma, const int
                     }
                 }
             }
+        }
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* load_vector_lateral_bc + lateral_shelf_bc (velo_higher:4536-4950): net    */
+/* hydrostatic pressure on vertical ice faces that border ice-free ocean,    */
+/* integrated with 2x2 Gauss points over each layer of the face.            */
+/* ------------------------------------------------------------------------ */
+static void shelf_face(const orc_grid *g, const double *sigma, int face, int ic, int jc,
+                       const double *susrf, const double *sthck, const double *xV, const double *yV,
+                       double *loadu, double *loadv)
+{
+    const int nx = g->nx, nz = g->nz;
+    int in[4], jn[4];
+    double x[4], y[4], s[4], h[4];
+    /* face 0 west, 1 east, 2 south, 3 north (nodes 1,2 along the face, 3,4 above them) */
+    if (face == 0) { in[0] = ic - 1; jn[0] = jc;     in[1] = ic - 1; jn[1] = jc - 1; }
+    else if (face == 1) { in[0] = ic; jn[0] = jc - 1; in[1] = ic;     jn[1] = jc; }
+    else if (face == 2) { in[0] = ic - 1; jn[0] = jc - 1; in[1] = ic; jn[1] = jc - 1; }
+    else { in[0] = ic; jn[0] = jc; in[1] = ic - 1; jn[1] = jc; }
+    for (int n = 0; n < 2; ++n)
+        x[n] = face <= 1 ? S2(yV, in[n], jn[n]) : S2(xV, in[n], jn[n]);
+    in[2] = in[1]; jn[2] = jn[1]; in[3] = in[0]; jn[3] = jn[0];
+    x[2] = x[1]; x[3] = x[0];
+    s[0] = S2(susrf, in[0], jn[0]); s[1] = S2(susrf, in[1], jn[1]); s[2] = s[1]; s[3] = s[0];
+    h[0] = S2(sthck, in[0], jn[0]); h[1] = S2(sthck, in[1], jn[1]); h[2] = h[1]; h[3] = h[0];
+    const double r = 1.0 / sqrt(3.0);
+    static const int SX[4] = {-1, 1, 1, -1}, SY[4] = {-1, -1, 1, 1};
+    for (int k = 1; k <= nz - 1; ++k) {
+        y[0] = s[0] - sigma[k] * h[0];
+        y[1] = s[1] - sigma[k] * h[1];
+        y[2] = s[2] - sigma[k - 1] * h[2];
+        y[3] = s[3] - sigma[k - 1] * h[3];
+        const int kn[4] = {k + 1, k + 1, k, k};
+        for (int p = 0; p < 4; ++p) {
+            const double xq = SX[p] * r, yq = SY[p] * r;
+            double phi[4], dxr[4], dyr[4];
+            for (int n = 0; n < 4; ++n) {
+                const double ax = SX[n] > 0 ? 1.0 + xq : 1.0 - xq;
+                const double ay = SY[n] > 0 ? 1.0 + yq : 1.0 - yq;
+                phi[n] = ax * ay / 4.0;
+                dxr[n] = SX[n] * (ay / 4.0);
+                dyr[n] = SY[n] * (ax / 4.0);
+            }
+            double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+            for (int n = 0; n < 4; ++n) {
+                J11 = J11 + dxr[n] * x[n];
+                J12 = J12 + dxr[n] * y[n];
+                J21 = J21 + dyr[n] * x[n];
+                J22 = J22 + dyr[n] * y[n];
+            }
+            const double detJ = fabs(J11 * J22 - J12 * J21);
+            double hq = 0.0, sq = 0.0;
+            for (int n = 0; n < 4; ++n) { hq = hq + phi[n] * h[n]; sq = sq + phi[n] * s[n]; }
+            const double ratio = sq / hq < 1.0 ? sq / hq : 1.0;
+            const double p_av = 0.5 * RHOI * GRAV * hq - 0.5 * RHOO * GRAV * hq * ((1.0 - ratio) * (1.0 - ratio));
+            for (int n = 0; n < 4; ++n) {
+                const double t = p_av * 1.0 * detJ / VOL0 * phi[n];
+                if (face == 0) V3(loadu, kn[n], in[n], jn[n]) = V3(loadu, kn[n], in[n], jn[n]) - t;
+                else if (face == 1) V3(loadu, kn[n], in[n], jn[n]) = V3(loadu, kn[n], in[n], jn[n]) + t;
+                else if (face == 2) V3(loadv, kn[n], in[n], jn[n]) = V3(loadv, kn[n], in[n], jn[n]) - t;
+                else V3(loadv, kn[n], in[n], jn[n]) = V3(loadv, kn[n], in[n], jn[n]) + t;
+            }
+        }
+    }
+}
+
+void orc_load_lateral_bc(const orc_grid *g, const double *sigma, const int32_t *ocean_mask,
+                         const int32_t *calving_front_mask, const int32_t *active_cell,
+                         const double *xV, const double *yV, const double *stagusrf,
+                         const double *stagthck, double *loadu, double *loadv)
+{
+    const int nx = g->nx, ny = g->ny, nh = g->nhalo;
+    for (int j = nh + 1; j <= ny - nh + 1; ++j)
+        for (int i = nh + 1; i <= nx - nh + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            const int ni[4] = {i - 1, i + 1, i, i}, nj[4] = {j, j, j - 1, j + 1};
+            for (int f = 0; f < 4; ++f)
+                if (C2(ocean_mask, ni[f], nj[f]) == 1 ||
+                    (C2(calving_front_mask, ni[f], nj[f]) == 1 && !C2(active_cell, ni[f], nj[f])))
+                    shelf_face(g, sigma, f, i, j, stagusrf, stagthck, xV, yV, loadu, loadv);
         }
 }
 

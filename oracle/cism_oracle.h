@@ -115,6 +115,11 @@ void orc_load_gravity_standard(const orc_grid *g, const double *sigma, const int
                                const double *dusrf_dx, const double *dusrf_dy,
                                double *loadu, double *loadv);
 
+void orc_load_lateral_bc(const orc_grid *g, const double *sigma, const int32_t *ocean_mask,
+                         const int32_t *calving_front_mask, const int32_t *active_cell,
+                         const double *xV, const double *yV, const double *stagusrf,
+                         const double *stagthck, double *loadu, double *loadv);
+
 void orc_matvec_3d(const orc_grid *g, const int32_t *active_vertex,
                    const double *Auu, const double *Auv,
                    const double *Avu, const double *Avv,

@@ -74,6 +74,7 @@ def _declare(L):
     L.orc_load_gravity_local.argtypes = [G, D, C.c_double, C.c_double, I, D, D, D, D, D]
     L.orc_load_gravity_standard.argtypes = [G, D, I, D, D, D, D, D, D, D, D]
     L.orc_matvec_3d.argtypes = [G, I, D, D, D, D, D, D, D, D]
+    L.orc_load_lateral_bc.argtypes = [G, D, I, I, I, D, D, D, D, D, D]
     L.orc_tridiag.argtypes = [C.c_int, D, D, D, D, D]
     pcg = [G, I, D, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, D, C.POINTER(C.c_int)]
     L.orc_pcg_standard_3d.argtypes = pcg
@@ -220,6 +221,23 @@ def load_gravity(p, standard=False):
         lib().orc_load_gravity_local(C.byref(g), dp(p.stagwbndsigma), p.dx, p.dy,
                                      ip(p.active_vertex), dp(p.stagthck), dp(p.dusrf_dx),
                                      dp(p.dusrf_dy), dp(lu), dp(lv))
+    return lu, lv
+
+
+def load_lateral(p, loadu, loadv):
+    """Adds the ice-ocean face pressure to loadu, loadv in place."""
+    g = grid_of(p)
+    cf = np.zeros((p.nx, p.ny), dtype=np.int32, order="F")
+    lib().orc_load_lateral_bc(C.byref(g), dp(p.sigma), ip(p.ocean_mask), ip(cf), ip(p.active_cell),
+                              dp(p.xVertex), dp(p.yVertex), dp(p.stagusrf), dp(p.stagthck),
+                              dp(loadu), dp(loadv))
+
+
+def loads(p):
+    """Gravity (per p.whichassemble_taud) plus the lateral ice-ocean load, as the reference
+    assembles them before the Picard loop (velo_higher:2033-2111)."""
+    lu, lv = load_gravity(p, standard=(p.whichassemble_taud == 0))
+    load_lateral(p, lu, lv)
     return lu, lv
 
 

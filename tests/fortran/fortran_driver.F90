@@ -1,0 +1,90 @@
+! Test driver for the Fortran drop-in module cism_gpu_velo: reads one case from a
+! binary stream file, calls the module's public procedures exactly as Fortran
+! code in the ice-sheet model would, writes the results to a second file.
+!   usage: fortran_driver <in.bin> <out.bin>
+! Input: int32 header (mode, nx, ny, nz, precond, whichapprox, whichefvs, pad),
+! then the arrays of the mode, in Fortran layout.
+program fortran_driver
+  use cism_gpu_velo
+  implicit none
+  integer, parameter :: dp = kind(1.0d0)
+  character(len=512) :: fin, fout
+  integer :: hdr(8), mode, nx, ny, nz, precond, wapprox, wefvs, niters, n1, n2, k, i, j
+  integer :: indxA(-1:1,-1:1,-1:1)
+  integer, allocatable :: iav(:,:), iac(:,:), um(:,:,:), vm(:,:,:)
+  logical, allocatable :: av(:,:), ac(:,:)
+  real(dp), allocatable :: Auu(:,:,:,:), Auv(:,:,:,:), Avu(:,:,:,:), Avv(:,:,:,:)
+  real(dp), allocatable :: xu(:,:,:), xv(:,:,:), yu(:,:,:), yv(:,:,:), bu(:,:,:), bv(:,:,:)
+  real(dp), allocatable :: sigma(:), xVx(:,:), yVx(:,:), susrf(:,:), sthck(:,:), ff(:,:,:), efvs(:,:,:)
+  real(dp), allocatable :: beta(:,:), lu(:,:,:), lv(:,:,:), ru(:,:,:), rv(:,:,:)
+  real(dp) :: err, scal(4)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) hdr
+  mode = hdr(1); nx = hdr(2); ny = hdr(3); nz = hdr(4); precond = hdr(5)
+  wapprox = hdr(6); wefvs = hdr(7)
+  do k = -1, 1
+     do j = -1, 1
+        do i = -1, 1
+           indxA(i,j,k) = 1 + (i+1) + 3*(j+1) + 9*(k+1)
+        enddo
+     enddo
+  enddo
+  allocate(iav(nx-1,ny-1), av(nx-1,ny-1))
+  allocate(Auu(27,nz,nx-1,ny-1), Auv(27,nz,nx-1,ny-1), Avu(27,nz,nx-1,ny-1), Avv(27,nz,nx-1,ny-1))
+  allocate(xu(nz,nx-1,ny-1), xv(nz,nx-1,ny-1), yu(nz,nx-1,ny-1), yv(nz,nx-1,ny-1))
+  allocate(bu(nz,nx-1,ny-1), bv(nz,nx-1,ny-1))
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  call cism_gpu_velo_init(nx, ny, nz, periodic=.true.)
+
+  select case (mode)
+  case (1)   ! matvec
+     read(10) iav, Auu, Auv, Avu, Avv, xu, xv
+     av = iav /= 0
+     call matvec_multiply_structured_3d(nx, ny, nz, 2, indxA, av, Auu, Auv, Avu, Avv, &
+                                        xu, xv, yu, yv)
+     write(11) yu, yv
+  case (2, 3)   ! pcg standard / chrongear
+     read(10) iav, Auu, Auv, Avu, Avv, bu, bv, xu, xv
+     av = iav /= 0
+     if (mode == 2) then
+        call pcg_solver_standard_3d(nx, ny, nz, 2, indxA, av, Auu, Auv, Avu, Avv, &
+                                    bu, bv, xu, xv, precond, err, niters)
+     else
+        call pcg_solver_chrongear_3d(nx, ny, nz, 2, indxA, av, Auu, Auv, Avu, Avv, &
+                                     bu, bv, xu, xv, precond, err, niters)
+     endif
+     scal = [err, real(niters, dp), 0.0_dp, 0.0_dp]
+     write(11) xu, xv, scal
+  case (4)   ! assembly
+     allocate(sigma(nz), iac(nx,ny), ac(nx,ny), xVx(nx-1,ny-1), yVx(nx-1,ny-1))
+     allocate(susrf(nx-1,ny-1), sthck(nx-1,ny-1), ff(nz-1,nx,ny), efvs(nz-1,nx,ny))
+     read(10) sigma, iac, xVx, yVx, xu, xv, susrf, sthck, ff, scal
+     ac = iac /= 0
+     call assemble_stiffness_matrix_3d(nx, ny, nz, sigma, 2, ac, xVx, yVx, xu, xv, susrf, sthck, &
+                                       ff, wapprox, efvs, wefvs, scal(1), Auu, Auv, Avu, Avv)
+     write(11) efvs, Auu, Auv, Avu, Avv
+  case (5)   ! Picard solve
+     allocate(sigma(nz), iac(nx,ny), ac(nx,ny), xVx(nx-1,ny-1), yVx(nx-1,ny-1))
+     allocate(susrf(nx-1,ny-1), sthck(nx-1,ny-1), ff(nz-1,nx,ny), efvs(nz-1,nx,ny))
+     allocate(beta(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1), lu(nz,nx-1,ny-1), lv(nz,nx-1,ny-1))
+     allocate(ru(nz,nx-1,ny-1), rv(nz,nx-1,ny-1))
+     read(10) sigma, iac, iav, xVx, yVx, susrf, sthck, ff, beta, um, vm, lu, lv, xu, xv, scal
+     ac = iac /= 0
+     av = iav /= 0
+     ! scal: dx, dy, efvs_constant, whichbabc; hdr: precond, whichapprox, whichefvs, sparse
+     call glissade_velo_higher_solve_3d_gpu(nx, ny, nz, scal(1), scal(2), sigma, ac, av, xVx, yVx, &
+          susrf, sthck, ff, beta, um, vm, lu, lv, nint(scal(4)), wefvs, 3, hdr(8), precond, &
+          wapprox, 100, scal(3), xu, xv, efvs, bu, bv, ru, rv, n1, n2)
+     scal = [real(n1, dp), real(n2, dp), 0.0_dp, 0.0_dp]
+     write(11) xu, xv, scal
+  case default
+     write(*,*) 'unknown mode', mode
+     error stop 2
+  end select
+  close(10)
+  close(11)
+  call cism_gpu_velo_finalize()
+end program fortran_driver

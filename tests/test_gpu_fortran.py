@@ -1,0 +1,99 @@
+"""The Fortran drop-in module (cism_amd/fortran/cism_gpu_velo.F90) called from a Fortran
+program with the reference's subroutine signatures, on the GPU, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cism_amd import configs
+from helpers import assembled_system, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "fortran", "fortran_driver")
+
+
+def run_driver(tmp_path, header, arrays, out_shapes):
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(np.asarray(header, dtype=np.int32).tobytes())
+        for a in arrays:
+            f.write(np.asfortranarray(a).tobytes(order="F"))
+    print(f"[fortran] mode {header[0]} grid {header[1:4]}", flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = np.fromfile(fout, dtype=np.float64)
+    out, o = [], 0
+    for shp in out_shapes:
+        n = int(np.prod(shp))
+        out.append(data[o:o + n].reshape(shp, order="F"))
+        o += n
+    assert o == data.size
+    return out
+
+
+@pytest.fixture(scope="module")
+def prob():
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    return configs.dome()
+
+
+def test_fortran_matvec_and_pcg(prob, tmp_path):
+    p = prob
+    A, bu, bv, u, v, _ = assembled_system(p, seed=5)
+    vshape = (p.nz, p.nx - 1, p.ny - 1)
+    rng = np.random.default_rng(1)
+    xu, xv = rng.standard_normal(vshape), rng.standard_normal(vshape)
+    xu, xv = np.asfortranarray(xu), np.asfortranarray(xv)
+    yu, yv = run_driver(tmp_path, [1, p.nx, p.ny, p.nz, 0, 2, 2, 0],
+                        [p.active_vertex.astype(np.int32), *A, xu, xv], [vshape, vshape])
+    yo_u, yo_v = O.matvec(p, A, xu, xv)
+    assert np.array_equal(yu, yo_u) and np.array_equal(yv, yo_v)
+    for mode, cg in ((2, False), (3, True)):
+        gu, gv, sc = run_driver(tmp_path, [mode, p.nx, p.ny, p.nz, 2, 2, 2, 0],
+                                [p.active_vertex.astype(np.int32), *A, bu, bv, u, v],
+                                [vshape, vshape, (4,)])
+        ou, ov = u.copy(order="F"), v.copy(order="F")
+        st, err, nit = O.pcg(p, A, bu, bv, ou, ov, 2, chrongear=cg)
+        assert int(sc[1]) == nit
+        assert rel_l2(np.concatenate([gu.ravel(), gv.ravel()]),
+                      np.concatenate([ou.ravel(), ov.ravel()])) < 1e-10
+
+
+def test_fortran_assembly(prob, tmp_path):
+    p = prob
+    rng = np.random.default_rng(2)
+    u = np.asfortranarray(30 * rng.standard_normal(p.uvel.shape))
+    v = np.asfortranarray(30 * rng.standard_normal(p.uvel.shape))
+    ashape = (27, p.nz, p.nx - 1, p.ny - 1)
+    outs = run_driver(tmp_path, [4, p.nx, p.ny, p.nz, 0, p.whichapprox, p.whichefvs, 0],
+                      [p.sigma, p.active_cell.astype(np.int32), p.xVertex, p.yVertex, u, v,
+                       p.stagusrf, p.stagthck, p.flwafact, np.array([p.efvs_constant, 0, 0, 0])],
+                      [(p.nz - 1, p.nx, p.ny)] + [ashape] * 4)
+    st, Ao, efo = O.assemble(p, u, v)
+    assert rel_max(outs[0], efo) < 1e-13
+    for a, b in zip(outs[1:], Ao):
+        assert rel_max(a, b) < 1e-13
+
+
+def test_fortran_picard(prob, tmp_path):
+    p = prob
+    lu, lv = O.load_gravity(p)
+    vshape = (p.nz, p.nx - 1, p.ny - 1)
+    gu, gv, sc = run_driver(
+        tmp_path, [5, p.nx, p.ny, p.nz, p.whichprecond, p.whichapprox, p.whichefvs, p.whichsparse],
+        [p.sigma, p.active_cell.astype(np.int32), p.active_vertex.astype(np.int32), p.xVertex,
+         p.yVertex, p.stagusrf, p.stagthck, p.flwafact, p.beta_internal, p.umask_dirichlet,
+         p.vmask_dirichlet, lu, lv, p.uvel, p.vvel,
+         np.array([p.dx, p.dy, p.efvs_constant, float(p.whichbabc)])],
+        [vshape, vshape, (4,)])
+    ro = O.picard(p, lu, lv)
+    assert int(sc[0]) == ro["picard_iters"]
+    e = rel_l2(np.concatenate([gu.ravel(), gv.ravel()]),
+               np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()]))
+    print(f"  Fortran-driven Picard: {int(sc[0])} iterations, rel L2 diff {e:.2e}")
+    assert e < 1e-8

@@ -28,6 +28,7 @@ def problems():
         "dome": configs.dome(),
         "ismip": configs.ismip_hom_a(ewn=24, nsn=24, nz=11, L=12000.0),
         "grf": configs.grf_sheet(ewn=24, nsn=20, nz=7, dx=2000.0, corr=8000.0, seed=3),
+        "sheet": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7),
     }
 
 
@@ -38,7 +39,7 @@ def gpu():
     return G
 
 
-@pytest.fixture(scope="module", params=["dome", "ismip", "grf"])
+@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet"])
 def prob(request):
     p = problems()[request.param]
     print(f"\n[case] {p.name} ewn={p.ewn} nsn={p.nsn} nz={p.nz}", flush=True)
@@ -194,6 +195,14 @@ def test_pcg(gpu, prob, chrongear, precond):
         assert d < 0.1 * e_o
 
 
+def test_load_lateral(gpu, prob):
+    p = prob
+    lo_u, lo_v = O.loads(p)
+    with gpu.Context.for_problem(p) as ctx:
+        lg_u, lg_v = ctx.loads(p)
+    assert np.array_equal(lg_u, lo_u) and np.array_equal(lg_v, lo_v)
+
+
 def test_load_gravity(gpu, prob):
     p = prob
     for standard in (False, True):
@@ -211,7 +220,7 @@ def test_load_gravity(gpu, prob):
 
 def test_picard_fp64(gpu, prob):
     p = prob
-    lu, lv = O.load_gravity(p)
+    lu, lv = O.loads(p)
     ro = O.picard(p, lu, lv)
     assert ro["status"] == 0
     with gpu.Context.for_problem(p) as ctx:

@@ -256,3 +256,29 @@ def test_picard_slab_analytic():
     assert ua[0] == pytest.approx(16.67, rel=1e-3)
     assert np.allclose(u, ua, rtol=2e-3, atol=2e-3 * ua.max())
     assert np.abs(r["vvel"]).max() < 1e-6 * ua.max()
+
+
+def test_lateral_bc_closed_form():
+    """Uniform ice faces against the ocean: each face carries p_av * H * dy / vol0 with
+    p_av = rho_i g H / 2 - rho_o g H (1 - min(s/H, 1))^2 / 2, outward (+x on east faces,
+    -x on west faces), so the summed u load is (east - west faces) p_av H dy / vol0."""
+    H, dy = 800.0, 33333.0
+    s = H * (1 - 910.0 / 1028.0)          # afloat: the ocean pushes back on (rho_i/rho_o) H
+    p_av = 0.5 * 910 * 9.81 * H - 0.5 * 1028 * 9.81 * H * (1 - min(s / H, 1.0)) ** 2
+    q = configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=dy, seed=7)
+    q.stagthck[:] = H
+    q.stagusrf[:] = s
+    lu, lv = O.vec_zeros(q), O.vec_zeros(q)
+    O.load_lateral(q, lu, lv)
+    east = west = north = south = 0
+    for j in range(3, q.ny):
+        for i in range(3, q.nx):
+            if q.active_cell[i - 1, j - 1]:
+                east += int(q.ocean_mask[i, j - 1] == 1)
+                west += int(q.ocean_mask[i - 2, j - 1] == 1)
+                north += int(q.ocean_mask[i - 1, j] == 1)
+                south += int(q.ocean_mask[i - 1, j - 2] == 1)
+    assert east + west > 0 and north + south > 0
+    assert lu.sum() == pytest.approx((east - west) * p_av * H * dy / 1e9, rel=1e-10, abs=1e-6)
+    assert lv.sum() == pytest.approx((north - south) * p_av * H * dy / 1e9, rel=1e-10, abs=1e-6)
+    assert np.abs(lu).sum() == pytest.approx((east + west) * p_av * H * dy / 1e9, rel=1e-10)
