@@ -82,30 +82,6 @@ def cpu_baseline(p, lu, lv, fp64_stats, budget_s=20.0):
                        f"{n_pic} Picard / {n_lin} PCG iterations")}
 
 
-def reference_solution(ctx, p, lu, lv, chunk=10, max_picard=300, velo_tol=1e-10):
-    """FP64 reference: plain FP64 Picard with linear tolerance 1e-11, run in chunks of
-    `chunk` iterations (Picard depends only on the current velocity, so chunks restart
-    exactly) until the largest relative velocity change is below velo_tol."""
-    t0 = time.perf_counter()
-    ctx.set_linear_solver(2000, 1e-11)
-    u, v = p.uvel, p.vvel
-    done, lin = 0, 0
-    while done < max_picard:
-        r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u, vvel=v, maxiter_nonlinear=chunk,
-                                     whichresid=0, target_override=velo_tol)
-        u, v = r["uvel"], r["vvel"]
-        done += r["picard_iters"]
-        lin += r["linear_iters"]
-        print(f"[bench] reference: {done} Picard, {lin} PCG its, velocity change "
-              f"{r['final_resid_velo']:.2e}, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-              flush=True)
-        if r["final_resid_velo"] < velo_tol:
-            break
-    ctx.set_linear_solver(200, 1e-8)
-    r["picard_iters"], r["linear_iters"] = done, lin
-    return r
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +98,7 @@ def main(argv=None):
         return 2
 
     from cism_amd import gpu as G
+    from cism_amd.fp64_reference import reference_solution
     p, desc = workload(a.workload)
     print(f"[bench] workload {a.workload}: {desc}", file=sys.stderr, flush=True)
     ctx = G.Context.for_problem(p)
@@ -131,7 +108,8 @@ def main(argv=None):
                             p.xVertex, p.yVertex, p.stagusrf, p.stagthck, p.dusrf_dx, p.dusrf_dy,
                             p.whichassemble_taud, lu, lv)
 
-    ref = reference_solution(ctx, p, lu, lv)
+    ref = reference_solution(ctx, p, lu, lv, log=lambda m: print(f"[bench] {m}", file=sys.stderr,
+                                                               flush=True))
     ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
 
     def fast():

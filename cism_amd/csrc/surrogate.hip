@@ -1,0 +1,804 @@
+// surrogate.hip -- learned initial guess for the velocity solve (no reference
+// counterpart): a fully convolutional 4-level encoder-decoder (~1.0M
+// parameters) mapping 2D ice-sheet fields on the vertex grid to u, v at every
+// level plus a per-column preconditioner scale, with hand-written forward and
+// backward kernels and Adam.
+//
+// Precision: FP32 master weights and activations; every convolution runs as an
+// implicit GEMM on MFMA (v_mfma_f32_16x16x16_bf16) with BF16 operands and FP32
+// accumulation.  Activations are NHWC (channels fastest); 3x3 convolutions use
+// periodic (circular) padding, matching the periodic grids of the solver.
+//
+//   enc0  3x3 16->32, 32->32          (H x W; 5 input channels zero-padded to 16)
+//   enc1  pool, 3x3 32->64, 64->64    (H/2)
+//   enc2  pool, 3x3 64->128, 128->128 (H/4)
+//   mid   pool, 3x3 128->128 x2       (H/8)
+//   dec2  up, cat enc2, 3x3 256->128
+//   dec1  up, cat enc1, 3x3 192->64
+//   dec0  up, cat enc0, 3x3 96->32
+//   head  1x1 32->32 (24 used: u, v at nz<=11 levels and two log-scales)
+// ReLU after every convolution except the head.
+#include "internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline short f2bf(float f) {
+    // round to nearest even
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+#define SG_IN_RAW 5
+#define SG_IN 16
+#define SG_OUT 24
+#define SG_OUT_PAD 32
+
+// ---------------------------------------------------------------------------
+// implicit-GEMM convolution: out[p][co] = act(bias[co] + sum_k W[co][k] * X(p, k))
+// k = tap*Ci + ci, tap = ky*3 + kx (ks = 3) or 0 (ks = 1); X(p, k) = in at pixel p
+// shifted by (ky-1, kx-1) with periodic wrap.  Requires Ci % 16 == 0.
+// Block tile: 64 output channels x 64 pixels, 4 waves of 2x2 16x16 MFMA tiles.
+// ---------------------------------------------------------------------------
+struct ConvDims {
+    int B, H, W, Ci, Co, ks, relu;
+};
+
+__device__ inline long long shifted_pixel(long long p, int tap, int ks, int H, int W) {
+    if (ks == 1) return p;
+    const int HW = H * W;
+    const long long b = p / HW;
+    const int r = (int)(p - b * HW);
+    int y = r / W, x = r - (r / W) * W;
+    y += tap / 3 - 1;
+    x += tap % 3 - 1;
+    y = y < 0 ? y + H : (y >= H ? y - H : y);
+    x = x < 0 ? x + W : (x >= W ? x - W : x);
+    return b * HW + (long long)y * W + x;
+}
+
+__global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__restrict__ in,
+                                                   const short *__restrict__ Wb,
+                                                   const float *__restrict__ bias,
+                                                   float *__restrict__ out) {
+    __shared__ short As[64][16 + 4];
+    __shared__ short Bs[64][16 + 4];
+    const int K = d.ks * d.ks * d.Ci;
+    const long long P = (long long)d.B * d.H * d.W;
+    const int co0 = blockIdx.y * 64;
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wc = (wave >> 1) * 32, wp = (wave & 1) * 32;
+    f32x4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int lr = t >> 2, lk = (t & 3) * 4;      // loader: row / pixel lr, 4 k values from lk
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        // A: weights rows co0+lr, k0+lk..+3
+        {
+            const int co = co0 + lr;
+            for (int j = 0; j < 4; ++j) As[lr][lk + j] = co < d.Co ? Wb[(long long)co * K + k0 + lk + j] : (short)0;
+        }
+        // B: pixel p0+lr, channels of tap k0/Ci
+        {
+            const long long p = p0 + lr;
+            const int tap = k0 / d.Ci, ci = k0 % d.Ci + lk;
+            if (p < P) {
+                const long long src = shifted_pixel(p, tap, d.ks, d.H, d.W);
+                const float4 v = *reinterpret_cast<const float4 *>(in + src * d.Ci + ci);
+                Bs[lr][lk + 0] = f2bf(v.x);
+                Bs[lr][lk + 1] = f2bf(v.y);
+                Bs[lr][lk + 2] = f2bf(v.z);
+                Bs[lr][lk + 3] = f2bf(v.w);
+            } else {
+                for (int j = 0; j < 4; ++j) Bs[lr][lk + j] = 0;
+            }
+        }
+        __syncthreads();
+        bf16x4 af[2], bfm[2];
+        for (int a = 0; a < 2; ++a) {
+            const int row = wc + a * 16 + (lane & 15);
+            for (int j = 0; j < 4; ++j) af[a][j] = As[row][4 * (lane >> 4) + j];
+        }
+        for (int b = 0; b < 2; ++b) {
+            const int col = wp + b * 16 + (lane & 15);
+            for (int j = 0; j < 4; ++j) bfm[b][j] = Bs[col][4 * (lane >> 4) + j];
+        }
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 4; ++i) {
+                const int co = co0 + wc + a * 16 + 4 * (lane >> 4) + i;
+                const long long p = p0 + wp + b * 16 + (lane & 15);
+                if (co < d.Co && p < P) {
+                    float v = acc[a][b][i] + (bias ? bias[co] : 0.f);
+                    if (d.relu) v = v > 0.f ? v : 0.f;
+                    out[p * d.Co + co] = v;
+                }
+            }
+}
+
+// weight gradient: dW[co][k] = sum_p dY[p][co] * X(p, k); pixel range split over
+// gridDim.z, partial tiles reduced by k_wgrad_reduce (fixed order).
+__global__ __launch_bounds__(256) void k_wgrad_mfma(ConvDims d, const float *__restrict__ dY,
+                                                    const float *__restrict__ X,
+                                                    float *__restrict__ part, long long pchunk) {
+    __shared__ short As[64][16 + 4];   // [co][pix]
+    __shared__ short Bs[64][16 + 4];   // [k][pix]
+    const int K = d.ks * d.ks * d.Ci;
+    const long long P = (long long)d.B * d.H * d.W;
+    const int co0 = blockIdx.y * 64, kq0 = blockIdx.x * 64;
+    const long long pbeg = (long long)blockIdx.z * pchunk;
+    const long long pend = pbeg + pchunk < P ? pbeg + pchunk : P;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wc = (wave >> 1) * 32, wk = (wave & 1) * 32;
+    f32x4 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int lp = t >> 4, lc = (t & 15) * 4;     // loader: pixel lp (0..15), 4 values from lc
+    for (long long q0 = pbeg; q0 < pend; q0 += 16) {
+        const long long p = q0 + lp;
+        const bool okp = p < pend;
+        {   // A: dY[p][co0+lc..+3] -> As[co][pix]
+            float4 v = {0.f, 0.f, 0.f, 0.f};
+            if (okp && co0 + lc < d.Co) v = *reinterpret_cast<const float4 *>(dY + p * d.Co + co0 + lc);
+            As[lc + 0][lp] = f2bf(v.x);
+            As[lc + 1][lp] = f2bf(v.y);
+            As[lc + 2][lp] = f2bf(v.z);
+            As[lc + 3][lp] = f2bf(v.w);
+        }
+        {   // B: X(p, kq0+lc..+3) -> Bs[k][pix]
+            float4 v = {0.f, 0.f, 0.f, 0.f};
+            const int k = kq0 + lc;
+            if (okp && k < K) {
+                const int tap = k / d.Ci, ci = k % d.Ci;
+                const long long src = shifted_pixel(p, tap, d.ks, d.H, d.W);
+                v = *reinterpret_cast<const float4 *>(X + src * d.Ci + ci);
+            }
+            Bs[lc + 0][lp] = f2bf(v.x);
+            Bs[lc + 1][lp] = f2bf(v.y);
+            Bs[lc + 2][lp] = f2bf(v.z);
+            Bs[lc + 3][lp] = f2bf(v.w);
+        }
+        __syncthreads();
+        bf16x4 af[2], bfm[2];
+        for (int a = 0; a < 2; ++a) {
+            const int row = wc + a * 16 + (lane & 15);
+            for (int j = 0; j < 4; ++j) af[a][j] = As[row][4 * (lane >> 4) + j];
+        }
+        for (int b = 0; b < 2; ++b) {
+            const int col = wk + b * 16 + (lane & 15);
+            for (int j = 0; j < 4; ++j) bfm[b][j] = Bs[col][4 * (lane >> 4) + j];
+        }
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    float *pz = part + (long long)blockIdx.z * d.Co * K;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 4; ++i) {
+                const int co = co0 + wc + a * 16 + 4 * (lane >> 4) + i;
+                const int k = kq0 + wk + b * 16 + (lane & 15);
+                if (co < d.Co && k < K) pz[(long long)co * K + k] = acc[a][b][i];
+            }
+}
+
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nz, long long n, float *__restrict__ dW) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += part[(long long)z * n + t];
+    dW[t] = s;
+}
+
+// bias gradient: db[co] = sum_p dY[p][co] (one block per channel, fixed order)
+__global__ __launch_bounds__(256) void k_bgrad(const float *__restrict__ dY, long long P, int Co,
+                                               float *__restrict__ db) {
+    __shared__ float red[256];
+    const int co = blockIdx.x;
+    float s = 0.f;
+    for (long long p = threadIdx.x; p < P; p += 256) s += dY[p * Co + co];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = red[0];
+}
+
+__global__ void k_relu_bwd(long long n, const float *__restrict__ y, float *__restrict__ dy) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && !(y[t] > 0.f)) dy[t] = 0.f;
+}
+
+// 2x2 average pool and nearest 2x upsample, NHWC, with their adjoints
+__global__ void k_pool(int B, int H, int W, int C, const float *__restrict__ in, float *__restrict__ out) {
+    const long long n = (long long)B * (H / 2) * (W / 2) * C;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long long r = t / C;
+    const int x = (int)(r % (W / 2));
+    r /= (W / 2);
+    const int y = (int)(r % (H / 2));
+    const long long b = r / (H / 2);
+    auto at = [&](int yy, int xx) { return in[((b * H + yy) * W + xx) * C + c]; };
+    out[t] = 0.25f * (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1) + at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+}
+
+__global__ void k_pool_bwd(int B, int H, int W, int C, const float *__restrict__ dout, float *__restrict__ din) {
+    const long long n = (long long)B * H * W * C;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = (int)(t % C);
+    long long r = t / C;
+    const int x = (int)(r % W);
+    r /= W;
+    const int y = (int)(r % H);
+    const long long b = r / H;
+    din[t] = 0.25f * dout[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// upsample a (H/2 x W/2, Ca) to H x W into channels [0, Ca) of out (Ctot) and copy
+// skip (H x W, Cs) into channels [Ca, Ca + Cs): the decoder's up + concat
+__global__ void k_up_cat(int B, int H, int W, int Ca, int Cs, const float *__restrict__ a,
+                         const float *__restrict__ skip, float *__restrict__ out) {
+    const int Ct = Ca + Cs;
+    const long long n = (long long)B * H * W * Ct;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = (int)(t % Ct);
+    const long long p = t / Ct;
+    if (c < Ca) {
+        const int x = (int)(p % W);
+        const long long r = p / W;
+        const int y = (int)(r % H);
+        const long long b = r / H;
+        out[t] = a[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c];
+    } else {
+        out[t] = skip[p * Cs + (c - Ca)];
+    }
+}
+
+// adjoint of k_up_cat: da (H/2 x W/2, Ca) = sum of the 4 children; dskip += tail
+__global__ void k_up_cat_bwd_a(int B, int H, int W, int Ca, int Cs, const float *__restrict__ dout,
+                               float *__restrict__ da) {
+    const int Ct = Ca + Cs, h = H / 2, w = W / 2;
+    const long long n = (long long)B * h * w * Ca;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = (int)(t % Ca);
+    long long r = t / Ca;
+    const int x = (int)(r % w);
+    r /= w;
+    const int y = (int)(r % h);
+    const long long b = r / h;
+    float s = 0.f;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) s += dout[((b * H + 2 * y + dy) * W + 2 * x + dx) * Ct + c];
+    da[t] = s;
+}
+
+__global__ void k_up_cat_bwd_skip(long long P, int Ca, int Cs, const float *__restrict__ dout,
+                                  float *__restrict__ dskip) {
+    const long long n = P * Cs;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long p = t / Cs;
+    const int c = (int)(t % Cs);
+    dskip[t] += dout[p * (Ca + Cs) + Ca + c];
+}
+
+// masked mean squared error over the SG_OUT used channels; out has SG_OUT_PAD channels
+__global__ __launch_bounds__(256) void k_loss(long long P, int nch, const float *__restrict__ out,
+                                              const float *__restrict__ tgt,
+                                              const float *__restrict__ mask,
+                                              const float *__restrict__ chw,
+                                              double *__restrict__ partials) {
+    __shared__ double red[256];
+    double s = 0.0, m = 0.0;
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < P; p += (long long)gridDim.x * 256) {
+        const float mk = mask[p];
+        if (mk == 0.f) continue;
+        m += mk;
+        for (int c = 0; c < nch; ++c) {
+            const double e = (double)out[p * SG_OUT_PAD + c] - tgt[p * SG_OUT + c];
+            s += mk * chw[c] * e * e;
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        red[threadIdx.x] = pass == 0 ? s : m;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[2 * blockIdx.x + pass] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_loss_final(const double *__restrict__ partials, int nb, double *__restrict__ res) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, m = 0.0;
+    for (int b = 0; b < nb; ++b) { s += partials[2 * b]; m += partials[2 * b + 1]; }
+    res[0] = m > 0 ? s / m : 0.0;
+    res[1] = m;
+}
+
+__global__ void k_loss_grad(long long P, int nch, const float *__restrict__ out,
+                            const float *__restrict__ tgt, const float *__restrict__ mask,
+                            const float *__restrict__ chw, const double *__restrict__ res,
+                            float *__restrict__ dout) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * SG_OUT_PAD) return;
+    const long long p = t / SG_OUT_PAD;
+    const int c = (int)(t % SG_OUT_PAD);
+    const double m = res[1];
+    float g = 0.f;
+    if (c < nch && m > 0) g = (float)(2.0 * mask[p] * chw[c] * ((double)out[t] - tgt[p * SG_OUT + c]) / m);
+    dout[t] = g;
+}
+
+// Adam on FP32 masters
+__global__ void k_adam(long long n, float *__restrict__ w, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gg = g[t];
+    const float mm = b1 * m[t] + (1.f - b1) * gg;
+    const float vv = b2 * v[t] + (1.f - b2) * gg * gg;
+    m[t] = mm;
+    v[t] = vv;
+    w[t] = w[t] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// BF16 copy of the forward weights W[co][k] and of the data-gradient weights
+// W'[ci][k'] (k' = tap'*Co + co, tap' = 8 - tap): dX = conv(dY, W') with the
+// flipped kernel, periodic padding making it exact
+__global__ void k_prep_weights(int Co, int Ci, int ks, const float *__restrict__ W,
+                               short *__restrict__ Wb, short *__restrict__ Wt) {
+    const int K = ks * ks * Ci;
+    const long long n = (long long)Co * K;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int co = (int)(t / K), k = (int)(t % K);
+    const int tap = k / Ci, ci = k % Ci;
+    const short b = f2bf(W[t]);
+    Wb[t] = b;
+    const int tapf = ks == 3 ? 8 - tap : 0;
+    Wt[(long long)ci * (ks * ks * Co) + (long long)tapf * Co + co] = b;
+}
+
+// ---------------------------------------------------------------------------
+// network container
+// ---------------------------------------------------------------------------
+struct ConvLayer {
+    int ks, Ci, Co, relu;
+    long long woff, boff;   // offsets into the parameter vector
+    short *Wb, *Wt;         // BF16 forward / data-gradient weights
+};
+
+struct Act {
+    int H, W, C;
+    float *y, *dy;
+    long long n(int B) const { return (long long)B * H * W * C; }
+};
+
+struct cism_surrogate {
+    int B, H, W;
+    std::vector<ConvLayer> L;
+    long long nparams;
+    float *w, *g, *m, *v;   // params, grads, Adam moments
+    long long step;
+    // activations: 0 in; conv outputs and pool/cat buffers
+    std::vector<Act> A;
+    float *raw, *tgt, *mask, *chw, *wpart;
+    long long wpart_n;
+    double *lpart, *lres;
+    hipStream_t stream;
+};
+
+// activation indices
+enum {
+    A_IN, A_E0a, A_E0b, A_P0, A_E1a, A_E1b, A_P1, A_E2a, A_E2b, A_P2, A_M0, A_M1, A_C2, A_D2,
+    A_C1, A_D1, A_C0, A_D0, A_OUT, A_N
+};
+// layer indices
+enum { L_E0a, L_E0b, L_E1a, L_E1b, L_E2a, L_E2b, L_M0, L_M1, L_D2, L_D1, L_D0, L_HEAD, L_N };
+
+static const int LAYER_IN[L_N] = {A_IN, A_E0a, A_P0, A_E1a, A_P1, A_E2a, A_P2, A_M0, A_C2, A_C1, A_C0, A_D0};
+static const int LAYER_OUT[L_N] = {A_E0a, A_E0b, A_E1a, A_E1b, A_E2a, A_E2b, A_M0, A_M1, A_D2, A_D1, A_D0, A_OUT};
+
+static int sg_alloc(float **p, long long n) {
+    hipError_t e = hipMalloc(p, sizeof(float) * (n > 0 ? n : 1));
+    if (e != hipSuccess) {
+        cism_set_error("surrogate allocation failed: %s", hipGetErrorString(e));
+        return CISM_ERR_NOMEM;
+    }
+    HIP_TRY(hipMemset(*p, 0, sizeof(float) * (n > 0 ? n : 1)));
+    return CISM_OK;
+}
+
+extern "C" int cism_surrogate_destroy(cism_surrogate *s);
+
+extern "C" int cism_surrogate_create(int H, int W, int batch, int device, cism_surrogate **out) {
+    if (!out || H % 8 || W % 8 || H < 8 || W < 8 || batch < 1) {
+        cism_set_error("surrogate: H and W must be positive multiples of 8 (got %d x %d)", H, W);
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(device));
+    cism_surrogate *s = new cism_surrogate();
+    s->B = batch; s->H = H; s->W = W;
+    const int spec[L_N][4] = {{3, SG_IN, 32, 1}, {3, 32, 32, 1}, {3, 32, 64, 1}, {3, 64, 64, 1},
+                              {3, 64, 128, 1}, {3, 128, 128, 1}, {3, 128, 128, 1}, {3, 128, 128, 1},
+                              {3, 256, 128, 1}, {3, 192, 64, 1}, {3, 96, 32, 1}, {1, 32, SG_OUT_PAD, 0}};
+    long long off = 0;
+    for (int l = 0; l < L_N; ++l) {
+        ConvLayer c;
+        c.ks = spec[l][0]; c.Ci = spec[l][1]; c.Co = spec[l][2]; c.relu = spec[l][3];
+        c.woff = off;
+        off += (long long)c.Co * c.ks * c.ks * c.Ci;
+        c.boff = off;
+        off += c.Co;
+        c.Wb = c.Wt = nullptr;
+        s->L.push_back(c);
+    }
+    s->nparams = off;
+    const int dims[A_N][3] = {{H, W, SG_IN}, {H, W, 32}, {H, W, 32}, {H / 2, W / 2, 32},
+                              {H / 2, W / 2, 64}, {H / 2, W / 2, 64}, {H / 4, W / 4, 64},
+                              {H / 4, W / 4, 128}, {H / 4, W / 4, 128}, {H / 8, W / 8, 128},
+                              {H / 8, W / 8, 128}, {H / 8, W / 8, 128}, {H / 4, W / 4, 256},
+                              {H / 4, W / 4, 128}, {H / 2, W / 2, 192}, {H / 2, W / 2, 64},
+                              {H, W, 96}, {H, W, 32}, {H, W, SG_OUT_PAD}};
+    int st = CISM_OK;
+    for (int a = 0; a < A_N; ++a) {
+        Act x;
+        x.H = dims[a][0]; x.W = dims[a][1]; x.C = dims[a][2];
+        x.y = x.dy = nullptr;
+        s->A.push_back(x);
+    }
+    for (int a = 0; a < A_N && !st; ++a) {
+        st = sg_alloc(&s->A[a].y, s->A[a].n(batch));
+        if (!st) st = sg_alloc(&s->A[a].dy, s->A[a].n(batch));
+    }
+    for (int l = 0; l < L_N && !st; ++l) {
+        const long long nw = (long long)s->L[l].Co * s->L[l].ks * s->L[l].ks * s->L[l].Ci;
+        if (hipMalloc(&s->L[l].Wb, sizeof(short) * nw) != hipSuccess ||
+            hipMalloc(&s->L[l].Wt, sizeof(short) * nw) != hipSuccess)
+            st = CISM_ERR_NOMEM;
+    }
+    if (!st) st = sg_alloc(&s->w, s->nparams);
+    if (!st) st = sg_alloc(&s->g, s->nparams);
+    if (!st) st = sg_alloc(&s->m, s->nparams);
+    if (!st) st = sg_alloc(&s->v, s->nparams);
+    const long long P = (long long)batch * H * W;
+    if (!st) st = sg_alloc(&s->raw, P * SG_IN_RAW);
+    if (!st) st = sg_alloc(&s->tgt, P * SG_OUT);
+    if (!st) st = sg_alloc(&s->mask, P);
+    if (!st) st = sg_alloc(&s->chw, SG_OUT);
+    // weight-gradient partials: the largest Co*K times the pixel split
+    s->wpart_n = 0;
+    for (int l = 0; l < L_N; ++l) {
+        const long long n = (long long)s->L[l].Co * s->L[l].ks * s->L[l].ks * s->L[l].Ci * 64;
+        if (n > s->wpart_n) s->wpart_n = n;
+    }
+    if (!st) st = sg_alloc(&s->wpart, s->wpart_n);
+    if (!st && hipMalloc(&s->lpart, sizeof(double) * 2 * 1024) != hipSuccess) st = CISM_ERR_NOMEM;
+    if (!st && hipMalloc(&s->lres, sizeof(double) * 2) != hipSuccess) st = CISM_ERR_NOMEM;
+    if (!st && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) st = CISM_ERR_HIP;
+    if (st) {
+        cism_surrogate_destroy(s);
+        return st;
+    }
+    // default initialisation: He-normal from a fixed LCG, zero biases, zero head
+    std::vector<float> h(s->nparams, 0.f);
+    unsigned long long rs = 0x9E3779B97F4A7C15ull;
+    auto unif = [&]() {
+        rs = rs * 6364136223846793005ull + 1442695040888963407ull;
+        return ((rs >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < L_N; ++l) {
+        const ConvLayer &c = s->L[l];
+        const long long nw = (long long)c.Co * c.ks * c.ks * c.Ci;
+        const double sd = l == L_HEAD ? 1e-3 : sqrt(2.0 / (c.ks * c.ks * c.Ci));
+        for (long long i = 0; i < nw; ++i) {
+            const double u1 = unif(), u2 = unif();
+            h[c.woff + i] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        }
+    }
+    float ones[SG_OUT];
+    for (int c = 0; c < SG_OUT; ++c) ones[c] = 1.f;
+    HIP_TRY(hipMemcpy(s->chw, ones, sizeof ones, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->w, h.data(), sizeof(float) * s->nparams, hipMemcpyHostToDevice));
+    s->step = 0;
+    *out = s;
+    return CISM_OK;
+}
+
+extern "C" int cism_surrogate_destroy(cism_surrogate *s) {
+    if (!s) return CISM_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &a : s->A) {
+        if (a.y) (void)hipFree(a.y);
+        if (a.dy) (void)hipFree(a.dy);
+    }
+    for (auto &l : s->L) {
+        if (l.Wb) (void)hipFree(l.Wb);
+        if (l.Wt) (void)hipFree(l.Wt);
+    }
+    void *ptrs[] = {s->w, s->g, s->m, s->v, s->raw, s->tgt, s->mask, s->chw, s->wpart, s->lpart, s->lres};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return CISM_OK;
+}
+
+extern "C" long long cism_surrogate_num_params(const cism_surrogate *s) { return s ? s->nparams : 0; }
+
+extern "C" int cism_surrogate_get_params(cism_surrogate *s, float *host) {
+    HIP_TRY(hipMemcpy(host, s->w, sizeof(float) * s->nparams, hipMemcpyDeviceToHost));
+    return CISM_OK;
+}
+
+extern "C" int cism_surrogate_set_params(cism_surrogate *s, const float *host) {
+    HIP_TRY(hipMemcpy(s->w, host, sizeof(float) * s->nparams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(s->m, 0, sizeof(float) * s->nparams));
+    HIP_TRY(hipMemset(s->v, 0, sizeof(float) * s->nparams));
+    s->step = 0;
+    return CISM_OK;
+}
+
+static int prep_weights(cism_surrogate *s) {
+    for (auto &c : s->L) {
+        const long long n = (long long)c.Co * c.ks * c.ks * c.Ci;
+        hipLaunchKernelGGL(k_prep_weights, dim3(nblocks_for(n)), dim3(256), 0, s->stream, c.Co, c.Ci, c.ks,
+                           s->w + c.woff, c.Wb, c.Wt);
+    }
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+static int conv_launch(cism_surrogate *s, const ConvDims &d, const float *in, const short *Wb,
+                       const float *bias, float *out) {
+    const long long P = (long long)d.B * d.H * d.W;
+    dim3 grid((unsigned)((P + 63) / 64), (unsigned)((d.Co + 63) / 64));
+    hipLaunchKernelGGL(k_conv_mfma, grid, dim3(256), 0, s->stream, d, in, Wb, bias, out);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+static int forward(cism_surrogate *s) {
+    const int B = s->B;
+    CISM_TRY(prep_weights(s));
+    auto conv = [&](int l) -> int {
+        const ConvLayer &c = s->L[l];
+        const Act &in = s->A[LAYER_IN[l]], &out = s->A[LAYER_OUT[l]];
+        ConvDims d = {B, in.H, in.W, c.Ci, c.Co, c.ks, c.relu};
+        return conv_launch(s, d, in.y, c.Wb, s->w + c.boff, out.y);
+    };
+    auto pool = [&](int a_in, int a_out) {
+        const Act &x = s->A[a_in];
+        hipLaunchKernelGGL(k_pool, dim3(nblocks_for(s->A[a_out].n(B))), dim3(256), 0, s->stream, B, x.H, x.W,
+                           x.C, x.y, s->A[a_out].y);
+    };
+    auto upcat = [&](int a_low, int a_skip, int a_out) {
+        const Act &o = s->A[a_out];
+        hipLaunchKernelGGL(k_up_cat, dim3(nblocks_for(o.n(B))), dim3(256), 0, s->stream, B, o.H, o.W,
+                           s->A[a_low].C, s->A[a_skip].C, s->A[a_low].y, s->A[a_skip].y, o.y);
+    };
+    CISM_TRY(conv(L_E0a)); CISM_TRY(conv(L_E0b)); pool(A_E0b, A_P0);
+    CISM_TRY(conv(L_E1a)); CISM_TRY(conv(L_E1b)); pool(A_E1b, A_P1);
+    CISM_TRY(conv(L_E2a)); CISM_TRY(conv(L_E2b)); pool(A_E2b, A_P2);
+    CISM_TRY(conv(L_M0)); CISM_TRY(conv(L_M1));
+    upcat(A_M1, A_E2b, A_C2); CISM_TRY(conv(L_D2));
+    upcat(A_D2, A_E1b, A_C1); CISM_TRY(conv(L_D1));
+    upcat(A_D1, A_E0b, A_C0); CISM_TRY(conv(L_D0));
+    CISM_TRY(conv(L_HEAD));
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// backward of layer l given dY of its output (already ReLU-masked); adds dX into the
+// input activation's gradient when want_dx
+static int conv_backward(cism_surrogate *s, int l, bool want_dx) {
+    const int B = s->B;
+    const ConvLayer &c = s->L[l];
+    const Act &in = s->A[LAYER_IN[l]], &out = s->A[LAYER_OUT[l]];
+    const long long P = (long long)B * in.H * in.W;
+    const int K = c.ks * c.ks * c.Ci;
+    // weight gradient
+    int nzs = (int)((P + 4095) / 4096);
+    if (nzs > 64) nzs = 64;
+    const long long pchunk = ((P + nzs - 1) / nzs + 15) / 16 * 16;
+    ConvDims d = {B, in.H, in.W, c.Ci, c.Co, c.ks, 0};
+    dim3 grid((unsigned)((K + 63) / 64), (unsigned)((c.Co + 63) / 64), (unsigned)nzs);
+    hipLaunchKernelGGL(k_wgrad_mfma, grid, dim3(256), 0, s->stream, d, out.dy, in.y, s->wpart, pchunk);
+    const long long nw = (long long)c.Co * K;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblocks_for(nw)), dim3(256), 0, s->stream, s->wpart, nzs, nw,
+                       s->g + c.woff);
+    hipLaunchKernelGGL(k_bgrad, dim3(c.Co), dim3(256), 0, s->stream, out.dy, P, c.Co, s->g + c.boff);
+    HIP_TRY(hipGetLastError());
+    if (want_dx) {
+        // dX = conv(dY, W') : channels Co -> Ci
+        ConvDims dd = {B, in.H, in.W, c.Co, c.Ci, c.ks, 0};
+        CISM_TRY(conv_launch(s, dd, out.dy, c.Wt, nullptr, in.dy));
+    }
+    return CISM_OK;
+}
+
+static int backward(cism_surrogate *s) {
+    const int B = s->B;
+    auto relu_bwd = [&](int a) {
+        const Act &x = s->A[a];
+        hipLaunchKernelGGL(k_relu_bwd, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream, x.n(B), x.y, x.dy);
+    };
+    auto upcat_bwd = [&](int a_low, int a_skip, int a_cat) {
+        // d(low) = sum of children of the first channels; d(skip) += tail channels
+        const Act &o = s->A[a_cat];
+        hipLaunchKernelGGL(k_up_cat_bwd_a, dim3(nblocks_for(s->A[a_low].n(B))), dim3(256), 0, s->stream, B,
+                           o.H, o.W, s->A[a_low].C, s->A[a_skip].C, o.dy, s->A[a_low].dy);
+        hipLaunchKernelGGL(k_up_cat_bwd_skip, dim3(nblocks_for(s->A[a_skip].n(B))), dim3(256), 0, s->stream,
+                           (long long)B * o.H * o.W, s->A[a_low].C, s->A[a_skip].C, o.dy, s->A[a_skip].dy);
+    };
+    // head: dOUT set by the loss
+    CISM_TRY(conv_backward(s, L_HEAD, true));          // -> dD0
+    relu_bwd(A_D0); CISM_TRY(conv_backward(s, L_D0, true));   // -> dC0
+    upcat_bwd(A_D1, A_E0b, A_C0);                       // dD1 = up^T, dE0b (skip) = tail (added below)
+    relu_bwd(A_D1); CISM_TRY(conv_backward(s, L_D1, true));   // -> dC1
+    upcat_bwd(A_D2, A_E1b, A_C1);
+    relu_bwd(A_D2); CISM_TRY(conv_backward(s, L_D2, true));   // -> dC2
+    upcat_bwd(A_M1, A_E2b, A_C2);
+    relu_bwd(A_M1); CISM_TRY(conv_backward(s, L_M1, true));   // -> dM0
+    relu_bwd(A_M0); CISM_TRY(conv_backward(s, L_M0, true));   // -> dP2
+    // dE2b = pool^T dP2 + skip part; the skip part was added by upcat_bwd into dE2b
+    // before pool_bwd would overwrite it, so accumulate via a temporary: pool_bwd
+    // writes into dE2a's buffer first (free at this point), then add
+    {
+        const Act &x = s->A[A_E2b];
+        float *tmp = s->A[A_E2a].dy;
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream, B, x.H, x.W, x.C,
+                           s->A[A_P2].dy, tmp);
+        hipLaunchKernelGGL(k_up_cat_bwd_skip, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream,
+                           (long long)B * x.H * x.W, 0, x.C, tmp, x.dy);
+    }
+    relu_bwd(A_E2b); CISM_TRY(conv_backward(s, L_E2b, true));   // -> dE2a
+    relu_bwd(A_E2a); CISM_TRY(conv_backward(s, L_E2a, true));   // -> dP1
+    {
+        const Act &x = s->A[A_E1b];
+        float *tmp = s->A[A_E1a].dy;
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream, B, x.H, x.W, x.C,
+                           s->A[A_P1].dy, tmp);
+        hipLaunchKernelGGL(k_up_cat_bwd_skip, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream,
+                           (long long)B * x.H * x.W, 0, x.C, tmp, x.dy);
+    }
+    relu_bwd(A_E1b); CISM_TRY(conv_backward(s, L_E1b, true));
+    relu_bwd(A_E1a); CISM_TRY(conv_backward(s, L_E1a, true));   // -> dP0
+    {
+        const Act &x = s->A[A_E0b];
+        float *tmp = s->A[A_E0a].dy;
+        hipLaunchKernelGGL(k_pool_bwd, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream, B, x.H, x.W, x.C,
+                           s->A[A_P0].dy, tmp);
+        hipLaunchKernelGGL(k_up_cat_bwd_skip, dim3(nblocks_for(x.n(B))), dim3(256), 0, s->stream,
+                           (long long)B * x.H * x.W, 0, x.C, tmp, x.dy);
+    }
+    relu_bwd(A_E0b); CISM_TRY(conv_backward(s, L_E0b, true));
+    relu_bwd(A_E0a); CISM_TRY(conv_backward(s, L_E0a, false));
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+static int zero_skip_grads(cism_surrogate *s) {
+    // the skip activations accumulate gradient from two paths
+    const int sk[3] = {A_E0b, A_E1b, A_E2b};
+    for (int a : sk) HIP_TRY(hipMemsetAsync(s->A[a].dy, 0, sizeof(float) * s->A[a].n(s->B), s->stream));
+    return CISM_OK;
+}
+
+// inputs arrive with SG_IN_RAW channels; the first layer reads SG_IN (zero padded)
+__global__ void k_widen(long long P, const float *__restrict__ raw, float *__restrict__ in) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * SG_IN) return;
+    const long long p = t / SG_IN;
+    const int c = (int)(t % SG_IN);
+    in[t] = c < SG_IN_RAW ? raw[p * SG_IN_RAW + c] : 0.f;
+}
+
+static int upload_batch(cism_surrogate *s, const float *in, const float *tgt, const float *mask) {
+    const long long P = (long long)s->B * s->H * s->W;
+    HIP_TRY(hipMemcpyAsync(s->raw, in, sizeof(float) * P * SG_IN_RAW, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_widen, dim3(nblocks_for(P * SG_IN)), dim3(256), 0, s->stream, P, s->raw,
+                       s->A[A_IN].y);
+    if (tgt) HIP_TRY(hipMemcpyAsync(s->tgt, tgt, sizeof(float) * P * SG_OUT, hipMemcpyHostToDevice, s->stream));
+    if (mask) HIP_TRY(hipMemcpyAsync(s->mask, mask, sizeof(float) * P, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CISM_OK;
+}
+
+static int loss_and_grad(cism_surrogate *s, int nch, double *loss) {
+    const long long P = (long long)s->B * s->H * s->W;
+    int nb = (int)((P + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s->stream, P, nch, s->A[A_OUT].y, s->tgt, s->mask,
+                       s->chw, s->lpart);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1), 0, s->stream, s->lpart, nb, s->lres);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblocks_for(P * SG_OUT_PAD)), dim3(256), 0, s->stream, P, nch,
+                       s->A[A_OUT].y, s->tgt, s->mask, s->chw, s->lres, s->A[A_OUT].dy);
+    HIP_TRY(hipGetLastError());
+    double res[2];
+    HIP_TRY(hipMemcpyAsync(res, s->lres, sizeof res, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (loss) *loss = res[0];
+    return CISM_OK;
+}
+
+__global__ void k_narrow(long long P, const float *__restrict__ o, float *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P * SG_OUT) return;
+    out[t] = o[(t / SG_OUT) * SG_OUT_PAD + t % SG_OUT];
+}
+
+// forward on host inputs (B x H x W x 5, NHWC) -> host outputs (B x H x W x 24)
+extern "C" int cism_surrogate_forward(cism_surrogate *s, const float *in, float *out) {
+    if (!s) return CISM_ERR_ARG;
+    CISM_TRY(upload_batch(s, in, nullptr, nullptr));
+    CISM_TRY(forward(s));
+    const long long P = (long long)s->B * s->H * s->W;
+    // narrow SG_OUT_PAD -> SG_OUT channels into the target buffer (free during inference)
+    hipLaunchKernelGGL(k_narrow, dim3(nblocks_for(P * SG_OUT)), dim3(256), 0, s->stream, P, s->A[A_OUT].y,
+                       s->tgt);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, s->tgt, sizeof(float) * P * SG_OUT, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CISM_OK;
+}
+
+// loss and gradient w.r.t. all parameters (grad may be NULL) without an update
+extern "C" int cism_surrogate_loss_grad(cism_surrogate *s, const float *in, const float *tgt,
+                                        const float *mask, int nch, double *loss, float *grad) {
+    if (!s || nch < 1 || nch > SG_OUT) return CISM_ERR_ARG;
+    CISM_TRY(upload_batch(s, in, tgt, mask));
+    CISM_TRY(forward(s));
+    CISM_TRY(loss_and_grad(s, nch, loss));
+    CISM_TRY(zero_skip_grads(s));
+    CISM_TRY(backward(s));
+    if (grad) {
+        HIP_TRY(hipMemcpyAsync(grad, s->g, sizeof(float) * s->nparams, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    return CISM_OK;
+}
+
+extern "C" int cism_surrogate_train_step(cism_surrogate *s, const float *in, const float *tgt,
+                                         const float *mask, int nch, float lr, double *loss) {
+    CISM_TRY(cism_surrogate_loss_grad(s, in, tgt, mask, nch, loss, nullptr));
+    s->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)s->step), c2 = 1.f - powf(b2, (float)s->step);
+    hipLaunchKernelGGL(k_adam, dim3(nblocks_for(s->nparams)), dim3(256), 0, s->stream, s->nparams, s->w, s->g,
+                       s->m, s->v, lr, b1, b2, 1e-8f, c1, c2);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_surrogate_set_channel_weights(cism_surrogate *s, const float *w24) {
+    HIP_TRY(hipMemcpy(s->chw, w24, sizeof(float) * SG_OUT, hipMemcpyHostToDevice));
+    return CISM_OK;
+}

@@ -81,30 +81,33 @@ module cism_gpu_velo
           bind(C, name='cism_matvec_multiply_structured_3d')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: h
-       integer(c_int32_t) :: indxA(*), av(*)
-       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), xu(*), xv(*), yu(*), yv(*)
+       integer(c_int32_t), intent(in) :: indxA(*), av(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), xu(*), xv(*)
+       real(c_double), intent(out) :: yu(*), yv(*)
      end function c_matvec
 
      integer(c_int) function c_pcg_standard(h, indxA, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
           precond, err, niters) bind(C, name='cism_pcg_solver_standard_3d')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: h
-       integer(c_int32_t) :: indxA(*), av(*)
-       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*), xu(*), xv(*)
+       integer(c_int32_t), intent(in) :: indxA(*), av(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*)
+       real(c_double), intent(inout) :: xu(*), xv(*)
        integer(c_int), value :: precond
-       real(c_double) :: err
-       integer(c_int) :: niters
+       real(c_double), intent(out) :: err
+       integer(c_int), intent(out) :: niters
      end function c_pcg_standard
 
      integer(c_int) function c_pcg_chrongear(h, indxA, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
           precond, err, niters) bind(C, name='cism_pcg_solver_chrongear_3d')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: h
-       integer(c_int32_t) :: indxA(*), av(*)
-       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*), xu(*), xv(*)
+       integer(c_int32_t), intent(in) :: indxA(*), av(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*)
+       real(c_double), intent(inout) :: xu(*), xv(*)
        integer(c_int), value :: precond
-       real(c_double) :: err
-       integer(c_int) :: niters
+       real(c_double), intent(out) :: err
+       integer(c_int), intent(out) :: niters
      end function c_pcg_chrongear
 
      integer(c_int) function c_assemble(h, sigma, ac, xV, yV, uvel, vvel, susrf, sthck, flwafact, &
@@ -112,12 +115,13 @@ module cism_gpu_velo
           bind(C, name='cism_assemble_stiffness_matrix_3d')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: h
-       real(c_double) :: sigma(*), xV(*), yV(*), uvel(*), vvel(*), susrf(*), sthck(*), flwafact(*)
-       integer(c_int32_t) :: ac(*)
+       real(c_double), intent(in) :: sigma(*), xV(*), yV(*), uvel(*), vvel(*), susrf(*), sthck(*)
+       real(c_double), intent(in) :: flwafact(*)
+       integer(c_int32_t), intent(in) :: ac(*)
        integer(c_int), value :: whichapprox, whichefvs
-       real(c_double) :: efvs(*)
+       real(c_double), intent(out) :: efvs(*)
        real(c_double), value :: efvs_constant
-       real(c_double) :: Auu(*), Auv(*), Avu(*), Avv(*)
+       real(c_double), intent(out) :: Auu(*), Auv(*), Avu(*), Avv(*)
      end function c_assemble
 
      subroutine c_solve_opts_default(o) bind(C, name='cism_solve_opts_default')
@@ -131,10 +135,13 @@ module cism_gpu_velo
        import :: c_int, c_int32_t, c_ptr, c_double, cism_solve_opts, cism_solve_stats
        type(c_ptr), value :: h
        type(cism_solve_opts) :: o
-       real(c_double) :: sigma(*), xV(*), yV(*), susrf(*), sthck(*), flwafact(*), beta(*)
-       integer(c_int32_t) :: ac(*), av(*), umask(*), vmask(*)
-       real(c_double) :: loadu(*), loadv(*), uvel(*), vvel(*), efvs(*), bu(*), bv(*), ru(*), rv(*)
-       type(cism_solve_stats) :: stats
+       real(c_double), intent(in) :: sigma(*), xV(*), yV(*), susrf(*), sthck(*), flwafact(*)
+       real(c_double), intent(in) :: beta(*)
+       integer(c_int32_t), intent(in) :: ac(*), av(*), umask(*), vmask(*)
+       real(c_double), intent(in) :: loadu(*), loadv(*)
+       real(c_double), intent(inout) :: uvel(*), vvel(*)
+       real(c_double), intent(out) :: efvs(*), bu(*), bv(*), ru(*), rv(*)
+       type(cism_solve_stats), intent(out) :: stats
      end function c_solve
   end interface
 
@@ -211,15 +218,14 @@ contains
     real(dp), dimension(27,nz,nx-1,ny-1), intent(in) :: Auu, Auv, Avu, Avv
     real(dp), dimension(nz,nx-1,ny-1), intent(in) :: xu, xv
     real(dp), dimension(nz,nx-1,ny-1), intent(out) :: yu, yv
-    integer(c_int32_t) :: av(nx-1,ny-1), ind(27)
-    real(dp) :: a1(27,nz,nx-1,ny-1), a2(27,nz,nx-1,ny-1), a3(27,nz,nx-1,ny-1), a4(27,nz,nx-1,ny-1)
-    real(dp) :: x1(nz,nx-1,ny-1), x2(nz,nx-1,ny-1)
+    integer(c_int32_t), allocatable :: av(:,:)
+    integer(c_int32_t) :: ind(27)
     if (nhalo /= 2) call check(5_c_int, 'matvec: nhalo must be 2')
     call cism_gpu_velo_init(nx, ny, nz)
+    allocate(av(nx-1,ny-1))
     call to_int_mask(active_vertex, av)
     ind = int(reshape(indxA, [27]), c_int32_t)
-    a1 = Auu; a2 = Auv; a3 = Avu; a4 = Avv; x1 = xu; x2 = xv
-    call check(c_matvec(ctx, ind, av, a1, a2, a3, a4, x1, x2, yu, yv), &
+    call check(c_matvec(ctx, ind, av, Auu, Auv, Avu, Avv, xu, xv, yu, yv), &
                'matvec_multiply_structured_3d')
   end subroutine matvec_multiply_structured_3d
 
@@ -288,22 +294,20 @@ contains
     integer, intent(in) :: precond
     real(dp), intent(out) :: err
     integer, intent(out) :: niters
-    integer(c_int32_t) :: av(nx-1,ny-1), ind(27)
+    integer(c_int32_t), allocatable :: av(:,:)
+    integer(c_int32_t) :: ind(27)
     real(c_double) :: e
     integer(c_int) :: n, st
-    real(dp), allocatable :: a1(:,:,:,:), a2(:,:,:,:), a3(:,:,:,:), a4(:,:,:,:), b1(:,:,:), b2(:,:,:)
     if (nhalo /= 2) call check(5_c_int, 'pcg: nhalo must be 2')
     call cism_gpu_velo_init(nx, ny, nz)
+    allocate(av(nx-1,ny-1))
     call to_int_mask(active_vertex, av)
     ind = int(reshape(indxA, [27]), c_int32_t)
-    ! intent(in) arrays are passed through copies: the C side takes non-const pointers
-    allocate(a1, source=Auu); allocate(a2, source=Auv); allocate(a3, source=Avu)
-    allocate(a4, source=Avv); allocate(b1, source=bu); allocate(b2, source=bv)
     if (kind_cg == 0) then
-       st = c_pcg_standard(ctx, ind, av, a1, a2, a3, a4, b1, b2, xu, xv, int(precond, c_int), e, n)
+       st = c_pcg_standard(ctx, ind, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, int(precond, c_int), e, n)
        call check(st, 'pcg_solver_standard_3d')
     else
-       st = c_pcg_chrongear(ctx, ind, av, a1, a2, a3, a4, b1, b2, xu, xv, int(precond, c_int), e, n)
+       st = c_pcg_chrongear(ctx, ind, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, int(precond, c_int), e, n)
        call check(st, 'pcg_solver_chrongear_3d')
     endif
     err = e
@@ -334,15 +338,13 @@ contains
     real(dp), intent(in) :: efvs_constant
     real(dp), dimension(nz-1,nx,ny), intent(out) :: efvs
     real(dp), dimension(27,nz,nx-1,ny-1), intent(out) :: Auu, Auv, Avu, Avv
-    integer(c_int32_t) :: ac(nx,ny)
-    real(dp) :: s(nz), xv_(nx-1,ny-1), yv_(nx-1,ny-1), u(nz,nx-1,ny-1), v(nz,nx-1,ny-1)
-    real(dp) :: su(nx-1,ny-1), st_(nx-1,ny-1), ff(nz-1,nx,ny)
+    integer(c_int32_t), allocatable :: ac(:,:)
     if (nhalo /= 2) call check(5_c_int, 'assemble: nhalo must be 2')
     call cism_gpu_velo_init(nx, ny, nz)
+    allocate(ac(nx,ny))
     ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
-    s = sigma; xv_ = xVertex; yv_ = yVertex; u = uvel; v = vvel
-    su = stagusrf; st_ = stagthck; ff = flwafact
-    call check(c_assemble(ctx, s, ac, xv_, yv_, u, v, su, st_, ff, int(whichapprox, c_int), efvs, &
+    call check(c_assemble(ctx, sigma, ac, xVertex, yVertex, uvel, vvel, stagusrf, stagthck, flwafact, &
+                          int(whichapprox, c_int), efvs, &
                           int(whichefvs, c_int), efvs_constant, Auu, Auv, Avu, Avv), &
                'assemble_stiffness_matrix_3d')
   end subroutine assemble_stiffness_matrix_3d
@@ -378,10 +380,9 @@ contains
     integer, intent(out) :: picard_iters, linear_iters
     type(cism_solve_opts) :: o
     type(cism_solve_stats) :: stats
-    integer(c_int32_t) :: ac(nx,ny), av(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1)
-    real(dp), allocatable :: s(:), xv_(:,:), yv_(:,:), su(:,:), st_(:,:), ff(:,:,:), be(:,:)
-    real(dp), allocatable :: lu(:,:,:), lv(:,:,:)
+    integer(c_int32_t), allocatable :: ac(:,:), av(:,:), um(:,:,:), vm(:,:,:)
     call cism_gpu_velo_init(nx, ny, nz)
+    allocate(ac(nx,ny), av(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1))
     call c_solve_opts_default(o)
     o%whichbabc = whichbabc; o%whichefvs = whichefvs; o%whichresid = whichresid
     o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
@@ -391,10 +392,8 @@ contains
     av = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     um = int(umask_dirichlet, c_int32_t)
     vm = int(vmask_dirichlet, c_int32_t)
-    allocate(s, source=sigma); allocate(xv_, source=xVertex); allocate(yv_, source=yVertex)
-    allocate(su, source=stagusrf); allocate(st_, source=stagthck); allocate(ff, source=flwafact)
-    allocate(be, source=beta_internal); allocate(lu, source=loadu); allocate(lv, source=loadv)
-    call check(c_solve(ctx, o, s, ac, av, xv_, yv_, su, st_, ff, be, um, vm, lu, lv, &
+    call check(c_solve(ctx, o, sigma, ac, av, xVertex, yVertex, stagusrf, stagthck, flwafact, &
+                       beta_internal, um, vm, loadu, loadv, &
                        uvel, vvel, efvs, bu, bv, resid_u, resid_v, stats), &
                'glissade_velo_higher_solve_3d_gpu')
     picard_iters = stats%picard_iters

@@ -31,6 +31,9 @@ EXPORTED = [
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
     "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_load_vector_lateral_bc",
+    "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
+    "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
+    "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
 ]
 
 

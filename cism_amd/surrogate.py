@@ -1,0 +1,237 @@
+"""Learned initial guess for the Blatter-Pattyn velocity solve (no reference counterpart).
+
+A fully convolutional 4-level encoder-decoder (about 1.0M parameters) maps five
+2D fields on the owned vertices of a periodic grid
+
+    0  stagthck / 1000 m
+    1  100 * d(usrf)/dx
+    2  100 * d(usrf)/dy
+    3  (ln(max(beta, 1e-3)) - 7) / 2        beta in Pa yr/m
+    4  log10(column-mean A) + 16           A = flwa in Pa^-3 yr^-1
+
+to u and v at each of nz <= 11 levels (channels 0..10 and 11..21, in units of
+`uscale` m/yr) and two per-column log preconditioner scales (channels 22, 23).
+The network runs in libcism_gpu.so (MFMA BF16 convolutions, FP32 accumulation,
+hand-written forward/backward and Adam); this module holds the host side:
+feature and target extraction, weight files and the prediction of a full
+velocity field in the reference's (nz, nx-1, ny-1) layout.
+
+Inactive vertices get zero features and zero predicted velocity; the solver's
+own Dirichlet and halo rules apply on top of the guess.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import numpy as np
+
+from . import configs
+from .gpu import lib, CismError
+
+N_IN = 5
+N_OUT = 24
+NZ_MAX = 11
+U_CH = 0
+V_CH = 11
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+
+_c = ctypes
+_P = _c.c_void_p
+_fp = _c.POINTER(_c.c_float)
+_dp = _c.POINTER(_c.c_double)
+
+
+def _setup(L):
+    if getattr(L, "_sg_ready", False):
+        return L
+    L.cism_surrogate_create.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_P)]
+    L.cism_surrogate_destroy.argtypes = [_P]
+    L.cism_surrogate_num_params.argtypes = [_P]
+    L.cism_surrogate_num_params.restype = _c.c_longlong
+    L.cism_surrogate_get_params.argtypes = [_P, _fp]
+    L.cism_surrogate_set_params.argtypes = [_P, _fp]
+    L.cism_surrogate_forward.argtypes = [_P, _fp, _fp]
+    L.cism_surrogate_loss_grad.argtypes = [_P, _fp, _fp, _fp, _c.c_int, _dp, _fp]
+    L.cism_surrogate_train_step.argtypes = [_P, _fp, _fp, _fp, _c.c_int, _c.c_float, _dp]
+    L.cism_surrogate_set_channel_weights.argtypes = [_P, _fp]
+    L._sg_ready = True
+    return L
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def _check(L, st, what):
+    if st != 0:
+        raise CismError(st, f"{what}: {L.cism_last_error().decode()}")
+
+
+class Surrogate:
+    """The network on one GPU, for batches of `batch` grids of H x W owned vertices."""
+
+    def __init__(self, H: int, W: int, batch: int = 1, device: int = 0):
+        self.L = _setup(lib())
+        self.H, self.W, self.B = H, W, batch
+        h = _P()
+        _check(self.L, self.L.cism_surrogate_create(H, W, batch, device, _c.byref(h)), "surrogate_create")
+        self.h = h
+        self.nparams = int(self.L.cism_surrogate_num_params(h))
+
+    def close(self):
+        if self.h:
+            self.L.cism_surrogate_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape_in(self, x):
+        x = np.asarray(x, dtype=np.float32)
+        if x.shape != (self.B, self.H, self.W, N_IN):
+            raise ValueError(f"input shape {x.shape} != {(self.B, self.H, self.W, N_IN)}")
+        return x
+
+    def get_params(self) -> np.ndarray:
+        w = np.empty(self.nparams, dtype=np.float32)
+        _check(self.L, self.L.cism_surrogate_get_params(self.h, w.ctypes.data_as(_fp)), "get_params")
+        return w
+
+    def set_params(self, w: np.ndarray):
+        w, pw = _f32(w)
+        if w.size != self.nparams:
+            raise ValueError(f"{w.size} parameters given, network has {self.nparams}")
+        _check(self.L, self.L.cism_surrogate_set_params(self.h, pw), "set_params")
+
+    def set_channel_weights(self, w):
+        w, pw = _f32(w)
+        assert w.size == N_OUT
+        _check(self.L, self.L.cism_surrogate_set_channel_weights(self.h, pw), "set_channel_weights")
+
+    def forward(self, x) -> np.ndarray:
+        x, px = _f32(self._shape_in(x))
+        out = np.empty((self.B, self.H, self.W, N_OUT), dtype=np.float32)
+        _check(self.L, self.L.cism_surrogate_forward(self.h, px, out.ctypes.data_as(_fp)), "forward")
+        return out
+
+    def loss_grad(self, x, tgt, mask, nch=N_OUT):
+        x, px = _f32(self._shape_in(x))
+        t, pt = _f32(tgt)
+        m, pm = _f32(mask)
+        assert t.shape == (self.B, self.H, self.W, N_OUT) and m.shape == (self.B, self.H, self.W)
+        loss = _c.c_double()
+        g = np.empty(self.nparams, dtype=np.float32)
+        _check(self.L, self.L.cism_surrogate_loss_grad(self.h, px, pt, pm, nch, _c.byref(loss),
+                                                       g.ctypes.data_as(_fp)), "loss_grad")
+        return loss.value, g
+
+    def train_step(self, x, tgt, mask, lr, nch=N_OUT) -> float:
+        x, px = _f32(self._shape_in(x))
+        t, pt = _f32(tgt)
+        m, pm = _f32(mask)
+        loss = _c.c_double()
+        _check(self.L, self.L.cism_surrogate_train_step(self.h, px, pt, pm, nch, lr, _c.byref(loss)),
+               "train_step")
+        return loss.value
+
+
+# ---------------------------------------------------------------------------
+# fields <-> network tensors (owned vertices of a periodic grid, pixel (y, x) is
+# vertex (i, j) = (x + 2, y + 2) in 0-based staggered indices)
+# ---------------------------------------------------------------------------
+def owned(a2d: np.ndarray, p: configs.Problem) -> np.ndarray:
+    """(nx-1, ny-1) vertex field -> (nsn, ewn) image."""
+    return np.asarray(a2d)[2:p.nx - 2, 2:p.ny - 2].T
+
+
+def features(p: configs.Problem) -> np.ndarray:
+    """(H, W, 5) float32 input of one problem."""
+    av = owned(p.active_vertex, p) != 0
+    f = np.zeros(av.shape + (N_IN,), dtype=np.float32)
+    flwa = np.asarray(p.flwa)                                  # (nz-1, nx, ny) cells
+    colA = flwa.mean(axis=0)
+    # vertex (i, j) (0-based) is the NE corner of cell (i, j): mean of cells i..i+1, j..j+1
+    vA = 0.25 * (colA[:-1, :-1] + colA[1:, :-1] + colA[:-1, 1:] + colA[1:, 1:])
+    f[..., 0] = owned(p.stagthck, p) / 1000.0
+    f[..., 1] = owned(p.dusrf_dx, p) * 100.0
+    f[..., 2] = owned(p.dusrf_dy, p) * 100.0
+    f[..., 3] = (np.log(np.maximum(owned(p.beta_internal, p), 1e-3)) - 7.0) / 2.0
+    f[..., 4] = np.log10(np.maximum(owned(vA, p), 1e-30)) + 16.0
+    f[~av] = 0.0
+    return f
+
+
+def targets(p: configs.Problem, uvel, vvel, uscale: float):
+    """(H, W, 24) float32 target and (H, W) mask from a solved velocity field."""
+    if p.nz > NZ_MAX:
+        raise ValueError(f"nz = {p.nz} > {NZ_MAX}")
+    av = owned(p.active_vertex, p) != 0
+    t = np.zeros(av.shape + (N_OUT,), dtype=np.float32)
+    u = np.asarray(uvel)[:, 2:p.nx - 2, 2:p.ny - 2].transpose(2, 1, 0)   # (H, W, nz)
+    v = np.asarray(vvel)[:, 2:p.nx - 2, 2:p.ny - 2].transpose(2, 1, 0)
+    t[..., U_CH:U_CH + p.nz] = u / uscale
+    t[..., V_CH:V_CH + p.nz] = v / uscale
+    t[~av] = 0.0
+    return t, av.astype(np.float32)
+
+
+def velocity_from_output(p: configs.Problem, out: np.ndarray, uscale: float):
+    """Network output (H, W, 24) -> uvel, vvel (nz, nx-1, ny-1), halos by periodic copy."""
+    av = owned(p.active_vertex, p) != 0
+    u = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
+    v = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
+    o = np.where(av[..., None], out, 0.0)
+    u[:, 2:p.nx - 2, 2:p.ny - 2] = (uscale * o[..., U_CH:U_CH + p.nz]).transpose(2, 1, 0)
+    v[:, 2:p.nx - 2, 2:p.ny - 2] = (uscale * o[..., V_CH:V_CH + p.nz]).transpose(2, 1, 0)
+    for a in (u, v):
+        for k in range(p.nz):
+            a[k] = configs.stag_halo(np.asfortranarray(a[k]), p.periodic)
+    return u, v
+
+
+# ---------------------------------------------------------------------------
+# weight files: float32 parameter vector + JSON metadata, in one .npz (no pickle)
+# ---------------------------------------------------------------------------
+def save_weights(path: str, params: np.ndarray, meta: dict):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=np.asarray(params, dtype=np.float32), meta=np.array(json.dumps(meta)))
+
+
+def load_weights(path: str):
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32), json.loads(str(z["meta"]))
+
+
+def default_weights(family: str) -> str:
+    return os.path.join(WEIGHTS_DIR, f"surrogate_{family}.npz")
+
+
+class Predictor:
+    """Holds a trained network for one grid shape and predicts initial guesses."""
+
+    def __init__(self, p: configs.Problem, weights: str, device: int = 0):
+        if not p.periodic:
+            raise ValueError("the surrogate runs on periodic grids (circular padding)")
+        params, self.meta = load_weights(weights)
+        self.uscale = float(self.meta["uscale"])
+        self.net = Surrogate(p.nsn, p.ewn, 1, device)
+        self.net.set_params(params)
+
+    def predict(self, p: configs.Problem):
+        out = self.net.forward(features(p)[None])[0]
+        return velocity_from_output(p, out, self.uscale)
+
+    def close(self):
+        self.net.close()

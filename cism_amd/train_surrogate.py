@@ -1,0 +1,194 @@
+"""Trains the learned initial guess on velocity fields solved by the plain FP64 GPU path.
+
+    python -m cism_amd.train_surrogate --family grf --n 256 --samples 160 --epochs 40 \
+        --out cism_amd/weights/surrogate_grf.npz
+
+Data: `samples` seeded problems of the chosen family (grf: Gaussian-random-field
+thickness on a flat bed with lognormal beta, seeds 1000..; sheet: the ice-sheet
+generator with a floating fringe), each solved on the GPU by the plain FP64
+Picard + Chron-Gear path (the reference algorithm at its default tolerances).
+Training: Adam on the masked MSE of u, v at every level (in units of uscale
+m/yr), with the eight exact symmetries of the square grid as augmentation
+(x-flip negates u and d(usrf)/dx, y-flip negates v and d(usrf)/dy, the
+transpose swaps them).  Validation: relative L2 velocity error of the
+prediction on held-out seeds.  Everything runs on the GPU through
+libcism_gpu.so; the host only moves batches.
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import configs
+from . import gpu as G
+from . import surrogate as SG
+
+
+def make_problem(family: str, ewn: int, nsn: int, seed: int):
+    if family == "grf":
+        return configs.grf_sheet(ewn=ewn, nsn=nsn, nz=11, dx=1000.0, seed=seed)
+    if family == "sheet":
+        return configs.ice_sheet(ewn=ewn, nsn=nsn, nz=11, dx=1000.0, seed=seed)
+    raise SystemExit(f"unknown family {family}")
+
+
+def solve_plain(ctx, p, log, chunk=10):
+    """The plain FP64 path at the reference's default tolerances, run in chunks of Picard
+    iterations (a Picard iteration depends only on the current velocity, so a chunk restarts
+    exactly) so that long solves report progress."""
+    lu, lv = ctx.loads(p)
+    u, v, done = p.uvel, p.vvel, 0
+    while True:
+        r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u, vvel=v, precision=0, maxiter_nonlinear=chunk)
+        u, v = r["uvel"], r["vvel"]
+        done += r["picard_iters"]
+        if r["picard_iters"] < chunk or done >= p.maxiter_nonlinear:
+            r["picard_iters"] = done
+            return r
+        log(f"  {done} Picard, L2 {r['final_L2_norm']:.2e}")
+
+
+def crop(x, t, m, c, margin, rng):
+    """Random c x c window; the mask drops `margin` pixels at its edges, where the window's
+    circular padding does not match the field outside it."""
+    H, W = m.shape
+    if c >= H and c >= W:
+        return x, t, m
+    y0, x0 = int(rng.integers(0, H - c + 1)), int(rng.integers(0, W - c + 1))
+    sl = (slice(y0, y0 + c), slice(x0, x0 + c))
+    mm = m[sl].copy()
+    mm[:margin] = mm[-margin:] = 0
+    mm[:, :margin] = mm[:, -margin:] = 0
+    return x[sl], t[sl], mm
+
+
+def augment(x, t, m, k):
+    """Symmetry k in 0..7 of (H, W, C) inputs / targets and the (H, W) mask: bit 0 x-flip,
+    bit 1 y-flip, bit 2 transpose."""
+    x, t, m = x.copy(), t.copy(), m.copy()
+    nz = SG.V_CH - SG.U_CH
+    if k & 1:
+        x, t, m = x[:, ::-1].copy(), t[:, ::-1].copy(), m[:, ::-1]
+        x[..., 1] *= -1
+        t[..., SG.U_CH:SG.U_CH + nz] *= -1
+    if k & 2:
+        x, t, m = x[::-1].copy(), t[::-1].copy(), m[::-1]
+        x[..., 2] *= -1
+        t[..., SG.V_CH:SG.V_CH + nz] *= -1
+    if k & 4:
+        x, t, m = x.transpose(1, 0, 2).copy(), t.transpose(1, 0, 2).copy(), m.T
+        x[..., [1, 2]] = x[..., [2, 1]]
+        u = t[..., SG.U_CH:SG.U_CH + nz].copy()
+        t[..., SG.U_CH:SG.U_CH + nz] = t[..., SG.V_CH:SG.V_CH + nz]
+        t[..., SG.V_CH:SG.V_CH + nz] = u
+    return np.ascontiguousarray(x), np.ascontiguousarray(t), np.ascontiguousarray(m)
+
+
+def rel_err(pred, tgt, mask, nch):
+    e = (pred[..., :nch] - tgt[..., :nch]) * mask[..., None]
+    return float(np.linalg.norm(e) / max(np.linalg.norm(tgt[..., :nch] * mask[..., None]), 1e-300))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--family", default="grf")
+    ap.add_argument("--n", type=int, default=256, help="square grid size")
+    ap.add_argument("--ewn", type=int, default=0, help="grid size in x (overrides --n)")
+    ap.add_argument("--nsn", type=int, default=0, help="grid size in y (overrides --n)")
+    ap.add_argument("--crop", type=int, default=0, help="train on random crops of this size")
+    ap.add_argument("--margin", type=int, default=16)
+    ap.add_argument("--samples", type=int, default=160)
+    ap.add_argument("--val", type=int, default=8)
+    ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--val-seed0", type=int, default=3000)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--uscale", type=float, default=100.0)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--init", default=None, help="start from these weights")
+    a = ap.parse_args(argv)
+    t_start = time.perf_counter()
+    nch = 2 * 11
+    ewn, nsn = a.ewn or a.n, a.nsn or a.n
+    c = a.crop if a.crop else 0
+    if c and (c > min(ewn, nsn) or c % 8):
+        raise SystemExit("--crop must be a multiple of 8 no larger than the grid")
+    th, tw = (c, c) if c else (nsn, ewn)
+
+    def gen(seeds, tag):
+        X, T, M = [], [], []
+        ctx = None
+        for i, s in enumerate(seeds):
+            p = make_problem(a.family, ewn, nsn, s)
+            if ctx is None:
+                ctx = G.Context.for_problem(p)
+            t0 = time.perf_counter()
+            r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True))
+            x = SG.features(p)
+            t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale)
+            X.append(x), T.append(t), M.append(m)
+            umax = float(np.abs(r["uvel"]).max())
+            print(f"[train] {tag} {i + 1}/{len(seeds)} seed {s}: {r['picard_iters']} Picard, "
+                  f"{time.perf_counter() - t0:.2f} s, max|u| {umax:.0f} m/yr", flush=True)
+        ctx.close()
+        return np.stack(X), np.stack(T), np.stack(M)
+
+    Xtr, Ttr, Mtr = gen(range(a.seed0, a.seed0 + a.samples), "data")
+    Xva, Tva, Mva = gen(range(a.val_seed0, a.val_seed0 + a.val), "val")
+    print(f"[train] data ready in {time.perf_counter() - t_start:.0f} s", flush=True)
+
+    net = SG.Surrogate(th, tw, a.batch)
+    if a.init:
+        w0, _ = SG.load_weights(a.init)
+        net.set_params(w0)
+    evalnet = SG.Surrogate(nsn, ewn, 1)
+
+    def validate():
+        evalnet.set_params(net.get_params())
+        errs = [rel_err(evalnet.forward(Xva[i:i + 1])[0], Tva[i], Mva[i], nch) for i in range(len(Xva))]
+        return float(np.mean(errs))
+
+    rng = np.random.default_rng(0)
+    n = len(Xtr)
+    # one epoch sees each sample once (whole grids) or as many crops as would tile it
+    per_sample = max(1, (nsn // th) * (ewn // tw))
+    steps_per_epoch = max(1, n * per_sample // a.batch)
+    total = a.epochs * steps_per_epoch
+    step = 0
+    hist = []
+    e0 = validate()
+    print(f"[train] initial validation rel L2 {e0:.3f}", flush=True)
+    for ep in range(a.epochs):
+        perm = rng.permutation(n)
+        losses = []
+        t0 = time.perf_counter()
+        for b in range(steps_per_epoch):
+            idx = [perm[(b * a.batch + q) % n] for q in range(a.batch)]
+            xs, ts, ms = zip(*(augment(*crop(Xtr[i], Ttr[i], Mtr[i], c, a.margin, rng),
+                                       int(rng.integers(0, 8))) for i in idx))
+            lr = a.lr * (0.03 + 0.97 * 0.5 * (1 + math.cos(math.pi * step / total)))
+            losses.append(net.train_step(np.stack(xs), np.stack(ts), np.stack(ms), lr, nch=nch))
+            step += 1
+        ve = validate()
+        hist.append((ep, float(np.mean(losses)), ve))
+        print(f"[train] epoch {ep + 1}/{a.epochs}: loss {np.mean(losses):.4f}, val rel L2 {ve:.4f}, "
+              f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
+    meta = {"family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
+            "seeds": [a.seed0, a.seed0 + a.samples - 1], "val_seeds": [a.val_seed0, a.val_seed0 + a.val - 1],
+            "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
+            "history": hist, "seconds": time.perf_counter() - t_start}
+    SG.save_weights(a.out, net.get_params(), meta)
+    print(f"[train] saved {a.out}: val rel L2 {hist[-1][2]:.4f} after {a.epochs} epochs, "
+          f"{time.perf_counter() - t_start:.0f} s", flush=True)
+    net.close()
+    evalnet.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

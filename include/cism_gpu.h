@@ -232,6 +232,37 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *           2 assembly (element kernel + node gather) of the current velocity */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
+/* ---------------------------------------------------------------------------
+ * Learned initial guess (no counterpart in the reference).  A fully
+ * convolutional 4-level encoder-decoder (about 1.0M parameters, FP32 master
+ * weights, BF16 MFMA convolutions with FP32 accumulation, periodic padding)
+ * on batches of H x W images (H, W multiples of 8):
+ *   input  (B, H, W, 5)  float32, NHWC: thickness, surface slopes, log beta,
+ *                        log column-mean flow factor at owned vertices
+ *   output (B, H, W, 24) float32: u at levels 0..10, v at levels 0..10, and
+ *                        two per-column log preconditioner scales
+ * Parameters are one float32 vector: per layer W[co][tap*Ci + ci] then b[co],
+ * layers in forward order.  Loss: sum_p mask_p sum_{c<nch} w_c (out-tgt)^2 /
+ * sum_p mask_p (w_c = 1 unless set).  One stream per network; calls are
+ * synchronous and not re-entrant per handle.
+ * ------------------------------------------------------------------------- */
+typedef struct cism_surrogate cism_surrogate;
+
+int  cism_surrogate_create(int H, int W, int batch, int device, cism_surrogate **out);
+int  cism_surrogate_destroy(cism_surrogate *s);
+long long cism_surrogate_num_params(const cism_surrogate *s);
+int  cism_surrogate_get_params(cism_surrogate *s, float *params);
+/* also resets the Adam moments and step count */
+int  cism_surrogate_set_params(cism_surrogate *s, const float *params);
+int  cism_surrogate_set_channel_weights(cism_surrogate *s, const float *w24);
+int  cism_surrogate_forward(cism_surrogate *s, const float *in, float *out);
+/* loss and its gradient w.r.t. every parameter (grad may be NULL), no update */
+int  cism_surrogate_loss_grad(cism_surrogate *s, const float *in, const float *target,
+                              const float *mask, int nch, double *loss, float *grad);
+/* loss_grad followed by one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) */
+int  cism_surrogate_train_step(cism_surrogate *s, const float *in, const float *target,
+                               const float *mask, int nch, float lr, double *loss);
+
 #ifdef __cplusplus
 }
 #endif
