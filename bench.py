@@ -40,15 +40,21 @@ STOP_REL_ERR = 1.0e-6
 
 
 def workload(name):
+    """(problem, description, surrogate weight family or None)"""
     from cism_amd import configs
     if name == "c1":
-        return configs.dome(), "dome 31x31x10, dx 2 km, no-slip, diag precond"
+        return configs.dome(), "dome 31x31x10, dx 2 km, no-slip, diag precond", None
     if name == "c2":
         return (configs.ismip_hom_a(ewn=160, nsn=160, nz=21, L=80000.0),
-                "ISMIP-HOM A 160x160x21 periodic, L=80 km, frozen bed, Chron-Gear + SIA precond")
+                "ISMIP-HOM A 160x160x21 periodic, L=80 km, frozen bed, Chron-Gear + SIA precond",
+                None)
     if name == "c3":
         return (configs.grf_sheet(ewn=512, nsn=512, nz=11, dx=1000.0, seed=1),
-                "GRF sheet 512x512x11, dx 1 km, lognormal beta, Chron-Gear + SIA precond")
+                "GRF sheet 512x512x11, dx 1 km, lognormal beta, Chron-Gear + SIA precond", "grf")
+    if name == "c5":
+        return (configs.ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5),
+                "ice sheet 1200x1400x11, dx 1 km, multiscale thickness, bed relief, floating "
+                "fringe with lateral ocean pressure, lognormal beta, Chron-Gear + SIA precond", "sheet")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -69,7 +75,7 @@ def cpu_baseline(p, lu, lv, fp64_stats, budget_s=20.0):
     O.dirichlet(p, A, bu, bv, p.uvel, p.vvel)
     for a in A:
         O.halo(p, a)
-    n_it = 40
+    n_it = 40 if p.nnodes < 2_000_000 else 8     # keeps the sample near 10-30 s of CPU work
     xu, xv = p.uvel.copy(order="F"), p.vvel.copy(order="F")
     t0 = time.perf_counter()
     O.pcg(p, A, bu, bv, xu, xv, p.whichprecond, chrongear=True, maxiters=n_it, tolerance=1e-30)
@@ -88,6 +94,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
+    ap.add_argument("--guess", default="learned", choices=["learned", "none"])
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--baseline-steps", type=int, default=1)
     a = ap.parse_args(argv)
@@ -98,51 +107,81 @@ def main(argv=None):
         return 2
 
     from cism_amd import gpu as G
+    from cism_amd import surrogate as SG
     from cism_amd.fp64_reference import reference_solution
-    p, desc = workload(a.workload)
-    print(f"[bench] workload {a.workload}: {desc}", file=sys.stderr, flush=True)
+
+    def log(m):
+        print(f"[bench] {m}", file=sys.stderr, flush=True)
+
+    p, desc, family = workload(a.workload)
+    log(f"workload {a.workload}: {desc}")
     ctx = G.Context.for_problem(p)
-    shape = (p.nz, p.nx - 1, p.ny - 1)
-    lu, lv = np.zeros(shape, order="F"), np.zeros(shape, order="F")
-    ctx.load_vector_gravity(p.sigma, p.stagwbndsigma, p.dx, p.dy, p.active_cell, p.active_vertex,
-                            p.xVertex, p.yVertex, p.stagusrf, p.stagthck, p.dusrf_dx, p.dusrf_dy,
-                            p.whichassemble_taud, lu, lv)
+    lu, lv = ctx.loads(p)
+    nodes = owned_active_nodes(p)
 
-    ref = reference_solution(ctx, p, lu, lv, log=lambda m: print(f"[bench] {m}", file=sys.stderr,
-                                                               flush=True))
-    ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
-
-    def fast():
-        return ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1,
+    def fast(u0=None, v0=None):
+        return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
                                         stop_rel_err=STOP_REL_ERR, inner_rtol=1e-3)
 
-    def plain():
-        return ctx.velo_higher_solve_3d(p, lu, lv, precision=0, stop_rel_err=STOP_REL_ERR)
+    def plain(u0=None, v0=None, stop=STOP_REL_ERR):
+        return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=0, stop_rel_err=stop)
+
+    if a.kernels_only:
+        # one fast solve leaves the FP32 matrix; then only the kernels under study run
+        fast()
+        ms32 = ctx.time_kernel(1, a.steps)
+        msc = ctx.time_kernel(3, a.steps)
+        print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
+                          "fp32_matvec_ms": ms32, "copy_1GiB_ms": msc, "launches": a.steps}))
+        ctx.close()
+        return 0
+
+    ref = reference_solution(ctx, p, lu, lv, log=log)
+    ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
+
+    weights = SG.default_weights(family) if family else None
+    use_guess = a.guess == "learned" and weights is not None and os.path.exists(weights)
+    pred = SG.Predictor(p, weights) if use_guess else None
+
+    def headline():
+        """the timed step: (learned guess +) mixed-precision solve"""
+        t0 = time.perf_counter()
+        u0 = v0 = None
+        if pred is not None:
+            u0, v0 = pred.predict(p)
+        t_inf = time.perf_counter() - t0
+        r = fast(u0, v0)
+        r["seconds"] = time.perf_counter() - t0
+        r["inference_s"] = t_inf
+        return r
 
     for _ in range(a.warmup):
-        fast()
-    times, stats = [], None
+        headline()
+    runs = []
     for s in range(a.steps):
-        t0 = time.perf_counter()
-        stats = fast()
-        times.append(time.perf_counter() - t0)
-        print(f"[bench] step {s}: {times[-1]:.3f} s, {stats['picard_iters']} Picard, "
-              f"{stats['fp32_matvecs']} FP32 + {stats['fp64_matvecs']} FP64 matvecs, "
-              f"err {stats['final_rel_err']:.2e}", file=sys.stderr, flush=True)
-    fast_s = float(np.mean(times))
+        r = headline()
+        runs.append(r)
+        log(f"step {s}: {r['seconds']:.3f} s (inference {r['inference_s'] * 1e3:.1f} ms), "
+            f"{r['picard_iters']} Picard, {r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 "
+            f"matvecs, err {r['final_rel_err']:.2e}")
+    fast_s = float(np.mean([r["seconds"] for r in runs]))
+    stats = runs[-1]
 
-    base_times, base = [], None
-    for _ in range(max(1, a.baseline_steps)):
+    def timed(fn, *args):
         t0 = time.perf_counter()
-        base = plain()
-        base_times.append(time.perf_counter() - t0)
-    plain_s = float(np.mean(base_times))
-    print(f"[bench] plain FP64: {plain_s:.3f} s, {base['picard_iters']} Picard, "
-          f"{base['linear_iters']} PCG its, err {base['final_rel_err']:.2e}", file=sys.stderr,
-          flush=True)
+        r = fn(*args)
+        r["seconds"] = time.perf_counter() - t0
+        return r
+
+    nog = timed(fast) if pred is not None else None
+    if nog is not None:
+        log(f"mixed precision, zero guess: {nog['seconds']:.3f} s, {nog['picard_iters']} Picard, "
+            f"err {nog['final_rel_err']:.2e}")
+    base = timed(plain)
+    log(f"plain FP64: {base['seconds']:.3f} s, {base['picard_iters']} Picard, "
+        f"{base['linear_iters']} PCG its, err {base['final_rel_err']:.2e}")
 
     # kernels: the matrices left by the last solves are current
-    nodes = owned_active_nodes(p)
     fast()   # leaves the FP32 matrix of a fast solve
     ms32 = ctx.time_kernel(1, 50)
     ms64 = ctx.time_kernel(0, 50)
@@ -152,12 +191,15 @@ def main(argv=None):
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     n_el = int(p.active_cell.sum()) * (p.nz - 1)
-    asm_flop = n_el * FLOP_PER_ELEMENT
-    asm_tflops = asm_flop / (ms_asm * 1e-3) / 1e12
+    asm_tflops = n_el * FLOP_PER_ELEMENT / (ms_asm * 1e-3) / 1e12
+    traffic = measured_traffic(a.workload)
 
     cpu = None
     if not a.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread, bounded sample)")
         cpu = cpu_baseline(p, lu, lv, base)
+    if pred is not None:
+        pred.close()
 
     line = {
         "metric": "s/velocity solve to 1e-6 err: learned+FP32-PCG vs FP64 PCG; matvec HBM GB/s",
@@ -170,30 +212,53 @@ def main(argv=None):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 inner / fp64 refinement",
+        "dtype": "fp32 inner / fp64 refinement" + (" / bf16 surrogate" if pred is not None else ""),
         "data": "synthetic",
         "config": {"workload": desc, "grid": [p.ewn, p.nsn, p.nz],
                    "owned_active_nodes": nodes, "stop_rel_err": STOP_REL_ERR,
+                   "learned_guess": weights if pred is not None else None,
                    "parallelism": "single GPU, whole domain resident"},
         "fast_path": {"seconds": round(fast_s, 4), "picard_iters": stats["picard_iters"],
                       "fp32_matvecs": stats["fp32_matvecs"], "fp64_matvecs": stats["fp64_matvecs"],
-                      "rel_err": stats["final_rel_err"], "learned_guess": False},
-        "plain_fp64": {"seconds": round(plain_s, 4), "picard_iters": base["picard_iters"],
+                      "rel_err": stats["final_rel_err"], "learned_guess": pred is not None,
+                      "inference_s": round(stats["inference_s"], 4)},
+        "fast_zero_guess": None if nog is None else {
+            "seconds": round(nog["seconds"], 4), "picard_iters": nog["picard_iters"],
+            "fp32_matvecs": nog["fp32_matvecs"], "fp64_matvecs": nog["fp64_matvecs"],
+            "rel_err": nog["final_rel_err"]},
+        "plain_fp64": {"seconds": round(base["seconds"], 4), "picard_iters": base["picard_iters"],
                        "pcg_iters": base["linear_iters"], "rel_err": base["final_rel_err"]},
-        "speedup_vs_plain_fp64": round(plain_s / fast_s, 3),
+        "speedup_vs_plain_fp64": round(base["seconds"] / fast_s, 3),
+        "reference_fp64": {"picard_iters": ref["picard_iters"], "pcg_iters": ref["linear_iters"]},
         "matvec": {"fp32_ms": round(ms32, 4), "fp32_GBps": round(gbs32, 1),
                    "fp64_ms": round(ms64, 4), "fp64_GBps": round(gbs64, 1),
                    "bytes_per_node_fp32": 448, "bytes_per_node_fp64": 896},
         "assembly": {"ms": round(ms_asm, 3), "TFLOPs": round(asm_tflops, 3),
-                     "flop_per_element": FLOP_PER_ELEMENT, "elements": n_el},
+                     "flop_per_element": FLOP_PER_ELEMENT, "elements": n_el,
+                     "frac_fp64_peak": round(asm_tflops / FP64_VALU_PEAK_TFLOPS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_matvec_dots<float,float,2>"},
+                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_matvec_dots<float,2>",
+                     "algorithmic_bytes_per_launch": bytes32},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
+
+
+def measured_traffic(workload_name):
+    """HBM bytes per launch of the FP32 matvec from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or
+    None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_k_matvec_dots_fp32_{workload_name}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
 
 
 # FP64 operations of one hexahedral element in the assembly kernels, counted from the code.

@@ -723,30 +723,61 @@ extern "C" int cism_set_reference_velocity(cism_ctx *c, const double *uref, cons
     return CISM_OK;
 }
 
+// plain copy of n doubles: the calibration kernel for hardware byte counters
+__global__ __launch_bounds__(256) void k_copy_calib(long long n, const double2 *__restrict__ a,
+                                                    double2 *__restrict__ b) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n / 2) b[t] = a[t];
+}
+
 extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per_launch) {
     CTX_BEGIN(c);
-    if (reps < 1 || which < 0 || which > 2) return CISM_ERR_ARG;
+    if (reps < 1 || which < 0 || which > 3) return CISM_ERR_ARG;
     if (which == 1 && !c->A32) {
         cism_set_error("no FP32 matrix yet: run a mixed-precision solve first");
         return CISM_ERR_ARG;
     }
+    // which 3: 1 GiB read + 1 GiB write copy, buffers owned by this call
+    const long long ncopy = 1LL << 27;
+    double *ca = nullptr, *cb = nullptr;
+    if (which == 3) {
+        HIP_TRY(hipMalloc(&ca, sizeof(double) * ncopy));
+        if (hipMalloc(&cb, sizeof(double) * ncopy) != hipSuccess) {
+            (void)hipFree(ca);
+            cism_set_error("copy calibration: allocation failed");
+            return CISM_ERR_NOMEM;
+        }
+        HIP_TRY(hipMemsetAsync(ca, 0, sizeof(double) * ncopy, c->stream));
+    }
+    auto launch = [&]() -> int {
+        if (which == 2) return assemble_device(c, 2, 2, 0.0, 0, 0.0, 0.0);
+        if (which == 3) {
+            hipLaunchKernelGGL(k_copy_calib, dim3(nblocks_for(ncopy / 2)), dim3(256), 0, c->stream, ncopy,
+                               (const double2 *)ca, (double2 *)cb);
+            HIP_TRY(hipGetLastError());
+            return CISM_OK;
+        }
+        return launch_cg_matvec(c, which);
+    };
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     // one untimed launch to settle clocks and caches
-    if (which == 2) CISM_TRY(assemble_device(c, 2, 2, 0.0, 0, 0.0, 0.0));
-    else CISM_TRY(launch_cg_matvec(c, which));
-    HIP_TRY(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < reps; ++r) {
-        if (which == 2) CISM_TRY(assemble_device(c, 2, 2, 0.0, 0, 0.0, 0.0));
-        else CISM_TRY(launch_cg_matvec(c, which));
-    }
-    HIP_TRY(hipEventRecord(e1, c->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    int st = launch();
+    if (!st) st = hipEventRecord(e0, c->stream) == hipSuccess ? CISM_OK : CISM_ERR_HIP;
+    for (int r = 0; r < reps && !st; ++r) st = launch();
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (!st && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = CISM_ERR_HIP;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (ca) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(ca);
+        (void)hipFree(cb);
+    }
+    if (st) return st;
     if (ms_per_launch) *ms_per_launch = (double)ms / reps;
     return CISM_OK;
 }

@@ -198,6 +198,21 @@ contains
     error stop 1
   end subroutine check
 
+  !> indxA(-1:1,-1:1,-1:1) -> int32(27) in array element order.  Explicit loops:
+  !> amdflang -O2 miscompiles RESHAPE of an array with lower bounds of -1.
+  subroutine pack_indxA(indxA, ind)
+    integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA
+    integer(c_int32_t), intent(out) :: ind(27)
+    integer :: i, j, k
+    do k = -1, 1
+       do j = -1, 1
+          do i = -1, 1
+             ind(1 + (i+1) + 3*(j+1) + 9*(k+1)) = int(indxA(i,j,k), c_int32_t)
+          enddo
+       enddo
+    enddo
+  end subroutine pack_indxA
+
   subroutine to_int_mask(l, m)
     logical, intent(in) :: l(:,:)
     integer(c_int32_t), intent(out) :: m(:,:)
@@ -224,7 +239,7 @@ contains
     call cism_gpu_velo_init(nx, ny, nz)
     allocate(av(nx-1,ny-1))
     call to_int_mask(active_vertex, av)
-    ind = int(reshape(indxA, [27]), c_int32_t)
+    call pack_indxA(indxA, ind)
     call check(c_matvec(ctx, ind, av, Auu, Auv, Avu, Avv, xu, xv, yu, yv), &
                'matvec_multiply_structured_3d')
   end subroutine matvec_multiply_structured_3d
@@ -302,7 +317,7 @@ contains
     call cism_gpu_velo_init(nx, ny, nz)
     allocate(av(nx-1,ny-1))
     call to_int_mask(active_vertex, av)
-    ind = int(reshape(indxA, [27]), c_int32_t)
+    call pack_indxA(indxA, ind)
     if (kind_cg == 0) then
        st = c_pcg_standard(ctx, ind, av, Auu, Auv, Avu, Avv, bu, bv, xu, xv, int(precond, c_int), e, n)
        call check(st, 'pcg_solver_standard_3d')

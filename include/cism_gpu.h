@@ -229,7 +229,9 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  * vectors left by the last solve and returns the mean duration in ms.
  *   which = 0 FP64 matvec (with the fused Chron-Gear dot products)
  *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
- *           2 assembly (element kernel + node gather) of the current velocity */
+ *           2 assembly (element kernel + node gather) of the current velocity
+ *           3 plain copy of 1 GiB (k_copy_calib: 1 GiB read + 1 GiB written), the
+ *             calibration of hardware byte counters */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
 /* ---------------------------------------------------------------------------
