@@ -228,7 +228,8 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
-                    c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals};
+                    c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals,
+                    c->cscale};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -719,6 +720,53 @@ extern "C" int cism_set_reference_velocity(cism_ctx *c, const double *uref, cons
     }
     if (!c->xref) CISM_TRY(dalloc(&c->xref, 2 * (size_t)c->g.npad));
     CISM_TRY(upload_vec(c, c->xref, uref, vref));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+__global__ __launch_bounds__(256) void k_scale_to_f32(long long n, const double *__restrict__ s,
+                                                      float *__restrict__ out, int *__restrict__ bad) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double v = s[t];
+    if (!(v > 0.0) || !(v < 1.0e30)) bad[0] = 1;
+    out[t] = (float)v;
+}
+
+extern "C" int cism_set_precond_scale(cism_ctx *c, const double *scale_u, const double *scale_v) {
+    CTX_BEGIN(c);
+    if (!scale_u || !scale_v) {
+        c->cscale_on = 0;
+        return CISM_OK;
+    }
+    const long long ncol = (long long)c->g.sx * c->g.sy;
+    if (!c->cscale) CISM_TRY(dalloc(&c->cscale, 2 * (size_t)ncol));
+    double *st = c->vstage;   // 2*npad >= 2*ncol doubles
+    HIP_TRY(hipMemcpyAsync(st, scale_u, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(st + ncol, scale_v, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    int *bad = &c->sc->nan_flag;
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_scale_to_f32, dim3(nblocks_for(2 * ncol)), dim3(256), 0, c->stream, 2 * ncol, st,
+                       c->cscale, bad);
+    HIP_TRY(hipGetLastError());
+    CISM_TRY(sync_scalars(c));
+    if (c->sc_host->nan_flag) {
+        c->cscale_on = 0;
+        cism_set_error("preconditioner scales must be finite and positive");
+        return CISM_ERR_ARG;
+    }
+    c->cscale_on = 1;
+    return CISM_OK;
+}
+
+extern "C" int cism_column_scale_target(cism_ctx *c, double *scale_u, double *scale_v) {
+    CTX_BEGIN(c);
+    if (!scale_u || !scale_v) return CISM_ERR_ARG;
+    const long long ncol = (long long)c->g.sx * c->g.sy;
+    double *out = c->vstage;
+    CISM_TRY(column_scale_device(c, out));
+    HIP_TRY(hipMemcpyAsync(scale_u, out, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(scale_v, out + ncol, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CISM_OK;
 }

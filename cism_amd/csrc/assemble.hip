@@ -410,37 +410,39 @@ __global__ __launch_bounds__(256) void k_sym_find(GridDev g, const int32_t *__re
                                                   DevScal *__restrict__ sc,
                                                   double *__restrict__ partials) {
     double acc[1] = {0.0};   // number of pairs that stay asymmetric (warnings)
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
-        const int k = (int)(q % g.nz);
-        const long long col = q / g.nz;
-        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        if (!is_owned(g, i, j) || !av[col]) continue;
-        for (int pass = 0; pass < 2; ++pass) {
-            const int bd = pass == 0 ? 0 : 3, bo = pass == 0 ? 1 : 2, bt = pass == 0 ? 2 : 1;
-            const double diag = A[aidx(bd, CISM_DIAG, q)];
-            for (int m = 0; m < 27; ++m) {
-                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-                if (k + kA < 0 || k + kA >= g.nz) continue;
-                const int mm = 26 - m;
-                const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
-                for (int h = 0; h < 2; ++h) {
-                    const long long e1 = aidx(h == 0 ? bd : bo, m, q);
-                    const long long e2 = aidx(h == 0 ? bd : bt, mm, qc);
-                    const double v1 = A[e1], v2 = A[e2];
-                    if (v1 == v2) continue;
-                    if (fabs(v2 - v1) < 1.0e-8 * fabs(diag)) {
-                        const unsigned int s = atomicAdd(&sc->sym_count, 1u);
-                        if ((long long)s < cap) {
-                            list[2 * s] = e1;
-                            list[2 * s + 1] = e2;
-                            vals[s] = 0.5 * (v1 + v2);
-                        }
-                    } else {
-                        acc[0] += 1.0;
-                    }
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(q % g.nz);
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    if (q < g.nn && is_owned(g, i, j) && av[col]) {
+        const double dgu = A[aidx(0, CISM_DIAG, q)], dgv = A[aidx(3, CISM_DIAG, q)];
+        // pairs per slot: Auu(m,q)~Auu(-m,qc) and Auv(m,q)~Avu(-m,qc) against |Auu diag|,
+        // Avv(m,q)~Avv(-m,qc) and Avu(m,q)~Auv(-m,qc) against |Avv diag|
+        auto test = [&](long long e1, long long e2, double v1, double v2, double diag) {
+            if (v1 == v2) return;
+            if (fabs(v2 - v1) < 1.0e-8 * fabs(diag)) {
+                const unsigned int s = atomicAdd(&sc->sym_count, 1u);
+                if ((long long)s < cap) {
+                    list[2 * s] = e1;
+                    list[2 * s + 1] = e2;
+                    vals[s] = 0.5 * (v1 + v2);
                 }
+            } else {
+                acc[0] += 1.0;
             }
+        };
+        for (int m = 0; m < 27; ++m) {
+            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+            if (k + kA < 0 || k + kA >= g.nz) continue;
+            const int mm = 26 - m;
+            const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+            double a0, a1, a2, a3, b0, b1, b2, b3;
+            load_blocks(A, q, m, a0, a1, a2, a3);
+            load_blocks(A, qc, mm, b0, b1, b2, b3);
+            test(aidx(0, m, q), aidx(0, mm, qc), a0, b0, dgu);
+            test(aidx(1, m, q), aidx(2, mm, qc), a1, b2, dgu);
+            test(aidx(3, m, q), aidx(3, mm, qc), a3, b3, dgv);
+            test(aidx(2, m, q), aidx(1, mm, qc), a2, b1, dgv);
         }
     }
     block_reduce_store<1>(acc, partials);
@@ -464,7 +466,7 @@ int symmetry_device(cism_ctx *c) {
         HIP_TRY(hipMalloc(&c->sym_vals, sizeof(double) * c->sym_cap));
     }
     HIP_TRY(hipMemsetAsync(&c->sc->sym_count, 0, sizeof(unsigned int), c->stream));
-    const int nb = red_blocks_for(c->g.nn);
+    const int nb = nblocks_for(c->g.nn);
     hipLaunchKernelGGL(k_sym_find, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->sym_list,
                        c->sym_vals, c->sym_cap, c->sc, c->partials);
     HIP_TRY(hipGetLastError());
@@ -479,20 +481,23 @@ int symmetry_device(cism_ctx *c) {
 __global__ __launch_bounds__(256) void k_nnz(GridDev g, const int32_t *__restrict__ av,
                                              const double *__restrict__ A, double *__restrict__ partials) {
     double acc[1] = {0.0};
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
-        const long long col = q / g.nz;
-        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        if (!is_owned(g, i, j) || !av[col]) continue;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    if (q < g.nn && is_owned(g, i, j) && av[col]) {
         int n = 0;
-        for (int pl = 0; pl < 108; ++pl) n += A[aidx(pl / 27, pl % 27, q)] != 0.0;
-        acc[0] += n;
+        for (int m = 0; m < 27; ++m) {
+            double a0, a1, a2, a3;
+            load_blocks(A, q, m, a0, a1, a2, a3);
+            n += (a0 != 0.0) + (a1 != 0.0) + (a2 != 0.0) + (a3 != 0.0);
+        }
+        acc[0] = n;
     }
     block_reduce_store<1>(acc, partials);
 }
 
 int count_nonzeros_device(cism_ctx *c) {
-    const int nb = red_blocks_for(c->g.nn);
+    const int nb = nblocks_for(c->g.nn);
     hipLaunchKernelGGL(k_nnz, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->partials);
     HIP_TRY(hipGetLastError());
     return launch_reduce(c, nb, 1, RED_SUMS);

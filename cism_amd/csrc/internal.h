@@ -70,6 +70,9 @@ struct cism_ctx {
     // column factors of the SIA preconditioner (6*npad each)
     double *F64;
     float *F32;
+    // optional per-column preconditioner scale of the mixed-precision solves (2*ncol)
+    float *cscale;
+    int cscale_on;
     // masks and geometry
     int32_t *av, *ac, *umask, *vmask;
     double *xV, *yV, *susrf, *sthck, *beta, *sigma, *wsig, *flwafact, *efvs;
@@ -100,6 +103,19 @@ __host__ __device__ inline int slot_of(int iA, int jA, int kA) {
 // position of coefficient (block b, slot m) of node q in the tiled matrix layout
 __host__ __device__ inline long long aidx(int b, int m, long long q) {
     return (((q >> 6) * 27 + m) << 8) + ((q & 63) << 2) + b;
+}
+
+// the four blocks (uu, uv, vu, vv) of slot m of node q: one 16-byte load (FP32) or two (FP64)
+__device__ inline void load_blocks(const float *__restrict__ A, long long q, int m, float &uu,
+                                   float &uv, float &vu, float &vv) {
+    const float4 a = *reinterpret_cast<const float4 *>(A + aidx(0, m, q));
+    uu = a.x; uv = a.y; vu = a.z; vv = a.w;
+}
+__device__ inline void load_blocks(const double *__restrict__ A, long long q, int m, double &uu,
+                                   double &uv, double &vu, double &vv) {
+    const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, m, q));
+    const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
+    uu = a.x; uv = a.y; vu = b.x; vv = b.y;
 }
 
 // periodic / outflow image of a staggered index (parallel_slap.F90:2807-2925):
@@ -207,6 +223,7 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
+int column_scale_device(cism_ctx *c, double *out);  // out: 2*ncol doubles
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
 int convert_a32(cism_ctx *c);
 int launch_cg_matvec(cism_ctx *c, int fp32);

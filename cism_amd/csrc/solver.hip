@@ -97,17 +97,6 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { using t = float2; };
 template <> struct Vec2<double> { using t = double2; };
 
-__device__ inline void load_blocks(const float *__restrict__ A, long long q, int m, float &uu,
-                                   float &uv, float &vu, float &vv) {
-    const float4 a = *reinterpret_cast<const float4 *>(A + aidx(0, m, q));
-    uu = a.x; uv = a.y; vu = a.z; vv = a.w;
-}
-__device__ inline void load_blocks(const double *__restrict__ A, long long q, int m, double &uu,
-                                   double &uv, double &vu, double &vv) {
-    const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, m, q));
-    const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
-    uu = a.x; uv = a.y; vu = b.x; vv = b.y;
-}
 
 template <typename T>
 __device__ inline void row_product(const GridDev &g, const T *__restrict__ A,
@@ -289,9 +278,14 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_copy(long long n, const T *__res
 // neighbouring values.  Apply, every iteration: the same forward/back
 // substitution as tridiag_solver, with identical operations, on r staged
 // through LDS.  Inactive columns keep z, as easy_sia_solver leaves them.
+//
+// An optional per-column scale s (cs[comp*ncol + col], s > 0) turns the
+// preconditioner into z = s M^-1 r by storing sb/s and beta/s (gamma is a ratio
+// and unchanged): still symmetric positive definite, and free in the apply.
 template <typename T>
 __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const T *__restrict__ A,
                                                           const int32_t *__restrict__ av,
+                                                          const float *__restrict__ cs,
                                                           T *__restrict__ F) {
     const long long ncol = (long long)g.sx * g.sy;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,16 +296,17 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const T *__
     const int nz = g.nz, blk = comp == 0 ? 0 : 3;
     const long long qb = col * nz;
     auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + col; };
+    const T inv = cs ? (T)(1.0f / cs[t]) : (T)1;
     T beta = A[aidx(blk, CISM_DIAG, qb)];
     F[Fi(0, 0)] = (T)0;
-    F[Fi(1, 0)] = beta;
+    F[Fi(1, 0)] = cs ? beta * inv : beta;
     F[Fi(2, 0)] = (T)0;
     for (int k = 1; k < nz; ++k) {
         const T gam = A[aidx(blk, CISM_DIAG + 9, qb + k - 1)] / beta;   // super(k-1) / beta(k-1)
         const T sb = A[aidx(blk, CISM_DIAG - 9, qb + k)];
         beta = A[aidx(blk, CISM_DIAG, qb + k)] - sb * gam;
-        F[Fi(0, k)] = sb;
-        F[Fi(1, k)] = beta;
+        F[Fi(0, k)] = cs ? sb * inv : sb;
+        F[Fi(1, k)] = cs ? beta * inv : beta;
         F[Fi(2, k)] = gam;
     }
 }
@@ -353,10 +348,10 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
 }
 
 template <typename T>
-static int launch_sia_setup(cism_ctx *c, const T *A, T *F) {
+static int launch_sia_setup(cism_ctx *c, const T *A, const float *cs, T *F) {
     const long long ncol = (long long)c->g.sx * c->g.sy;
     hipLaunchKernelGGL(k_sia_setup<T>, dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, c->g,
-                       A, c->av, F);
+                       A, c->av, cs, F);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
@@ -538,7 +533,8 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_zero2(long long n2, T *__restric
 template <typename T>
 struct CGVecs {
     T *x, *b, *r, *d, *z, *q, *Az;
-    T *F;   // column factors of the SIA preconditioner (6*nn)
+    T *F;              // column factors of the SIA preconditioner (6*nn)
+    const float *cs;   // optional per-column preconditioner scale (2*ncol) or null
 };
 
 // The reference iterations on a generic precision.  check = 0: every fifth
@@ -553,7 +549,7 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
     const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
     const long long np = g.npad;
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc);
-    if (precond == 2) CISM_TRY(launch_sia_setup<T>(c, A, v.F));
+    if (precond == 2) CISM_TRY(launch_sia_setup<T>(c, A, v.cs, v.F));
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
                        v.d, v.z, v.q);
     // (b, b) -> L2_rhs
@@ -639,7 +635,7 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
 // Solves A x = b for c->x (initial guess in place) with the FP64 matrix c->A.
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol, double *err,
                  int *niters) {
-    CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az, c->F64};
+    CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az, c->F64, nullptr};
     return run_cg<double>(c, c->A, v, chrongear, precond, maxiters, tol, 0, err, niters);
 }
 
@@ -712,7 +708,8 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
     const double bnorm = sqrt(c->sc_host->sums[0]);
     int total = 0, outer = 0;
     double err = 0.0;
-    CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32};
+    CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32,
+                       c->cscale_on ? c->cscale : nullptr};
     while (true) {
         CISM_TRY(launch_halo_vec(c, c->x));
         hipLaunchKernelGGL((k_residual<double>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
@@ -738,6 +735,55 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
     }
     *err_out = err;
     *niters_out = total;
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-column preconditioner scale of a solved system: for each owned active
+// column and component, s = (x_c . M_c x_c) / (x_c . (A x)_c), the ratio of the
+// energy of the column's vertical (SIA) block to that of the full operator on
+// the solution (A x = b - r); s < 1 where horizontal coupling stiffens the
+// column.  Clamped to [1/16, 16]; 1 elsewhere.  Out: s[comp*ncol + col].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CISM_BLOCK) void k_colscale(GridDev g, const double *__restrict__ A,
+                                                         const int32_t *__restrict__ av,
+                                                         const double *__restrict__ x,
+                                                         const double *__restrict__ b,
+                                                         const double *__restrict__ r,
+                                                         double *__restrict__ out) {
+    const long long ncol = (long long)g.sx * g.sy;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * ncol) return;
+    const int comp = (int)(t / ncol);
+    const long long col = t - comp * ncol;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    double s = 1.0;
+    if (is_owned(g, i, j) && av[col]) {
+        const int blk = comp == 0 ? 0 : 3;
+        const long long qb = col * g.nz;
+        double num = 0.0, den = 0.0;
+        for (int k = 0; k < g.nz; ++k) {
+            const long long q = qb + k;
+            double mx = A[aidx(blk, CISM_DIAG, q)] * x[2 * q + comp];
+            if (k > 0) mx += A[aidx(blk, CISM_DIAG - 9, q)] * x[2 * (q - 1) + comp];
+            if (k < g.nz - 1) mx += A[aidx(blk, CISM_DIAG + 9, q)] * x[2 * (q + 1) + comp];
+            num += x[2 * q + comp] * mx;
+            den += x[2 * q + comp] * (b[2 * q + comp] - r[2 * q + comp]);
+        }
+        if (den > 0.0 && num > 0.0) s = fmin(16.0, fmax(1.0 / 16.0, num / den));
+    }
+    out[t] = s;
+}
+
+int column_scale_device(cism_ctx *c, double *out) {
+    const GridDev &g = c->g;
+    const long long ncol = (long long)g.sx * g.sy;
+    CISM_TRY(launch_halo_vec(c, c->x));
+    hipLaunchKernelGGL((k_residual<double>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g,
+                       c->A, c->av, c->x, c->b, c->r, 1, c->partials);
+    hipLaunchKernelGGL(k_colscale, dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, g, c->A,
+                       c->av, c->x, c->b, c->r, out);
+    HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
 

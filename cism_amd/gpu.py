@@ -31,6 +31,7 @@ EXPORTED = [
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
     "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_load_vector_lateral_bc",
+    "cism_set_precond_scale", "cism_column_scale_target",
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
@@ -110,6 +111,8 @@ def lib():
                                                 C.POINTER(SolveStats)]
         L.cism_set_reference_velocity.argtypes = [VP, D, D]
         L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
+        L.cism_set_precond_scale.argtypes = [VP, D, D]
+        L.cism_column_scale_target.argtypes = [VP, D, D]
         _lib = L
     return _lib
 
@@ -253,9 +256,26 @@ class Context:
     def set_reference_velocity(self, uref, vref):
         _check(lib().cism_set_reference_velocity(self.h, dp(uref), dp(vref)))
 
+    def set_precond_scale(self, su, sv):
+        """Per-column scale of the mixed-precision SIA preconditioner (vertex fields); None
+        clears it."""
+        if su is None or sv is None:
+            _check(lib().cism_set_precond_scale(self.h, None, None))
+        else:
+            su = np.asfortranarray(su, dtype=np.float64)
+            sv = np.asfortranarray(sv, dtype=np.float64)
+            _check(lib().cism_set_precond_scale(self.h, dp(su), dp(sv)))
+
+    def column_scale_target(self):
+        """(su, sv) vertex fields from the last solve's matrix, rhs and velocity."""
+        su = np.ones((self.nx - 1, self.ny - 1), order="F")
+        sv = np.ones((self.nx - 1, self.ny - 1), order="F")
+        _check(lib().cism_column_scale_target(self.h, dp(su), dp(sv)))
+        return su, sv
+
     def time_kernel(self, which, reps=20):
         """Mean ms per launch (HIP events on the context stream): 0 FP64 matvec,
-        1 FP32 matvec, 2 assembly."""
+        1 FP32 matvec, 2 assembly, 3 1 GiB copy (counter calibration)."""
         ms = C.c_double()
         _check(lib().cism_time_kernel(self.h, which, reps, C.byref(ms)))
         return ms.value

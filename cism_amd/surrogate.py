@@ -173,8 +173,12 @@ def features(p: configs.Problem) -> np.ndarray:
     return f
 
 
-def targets(p: configs.Problem, uvel, vvel, uscale: float):
-    """(H, W, 24) float32 target and (H, W) mask from a solved velocity field."""
+S_CH = 22          # log preconditioner scale, u and v components (channels 22, 23)
+
+
+def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None):
+    """(H, W, 24) float32 target and (H, W) mask from a solved velocity field and,
+    optionally, the per-column preconditioner scales of that solve (vertex fields)."""
     if p.nz > NZ_MAX:
         raise ValueError(f"nz = {p.nz} > {NZ_MAX}")
     av = owned(p.active_vertex, p) != 0
@@ -183,6 +187,9 @@ def targets(p: configs.Problem, uvel, vvel, uscale: float):
     v = np.asarray(vvel)[:, 2:p.nx - 2, 2:p.ny - 2].transpose(2, 1, 0)
     t[..., U_CH:U_CH + p.nz] = u / uscale
     t[..., V_CH:V_CH + p.nz] = v / uscale
+    if su is not None and sv is not None:
+        t[..., S_CH] = np.log(owned(su, p))
+        t[..., S_CH + 1] = np.log(owned(sv, p))
     t[~av] = 0.0
     return t, av.astype(np.float32)
 
@@ -199,6 +206,18 @@ def velocity_from_output(p: configs.Problem, out: np.ndarray, uscale: float):
         for k in range(p.nz):
             a[k] = configs.stag_halo(np.asfortranarray(a[k]), p.periodic)
     return u, v
+
+
+def scales_from_output(p: configs.Problem, out: np.ndarray):
+    """Network output (H, W, 24) -> per-column preconditioner scales (vertex fields), 1 at
+    inactive vertices, clamped to [1/16, 16]."""
+    av = owned(p.active_vertex, p) != 0
+    res = []
+    for c in (S_CH, S_CH + 1):
+        s = np.ones((p.nx - 1, p.ny - 1), order="F")
+        s[2:p.nx - 2, 2:p.ny - 2] = np.where(av, np.exp(np.clip(out[..., c], -2.77, 2.77)), 1.0).T
+        res.append(configs.stag_halo(s, p.periodic))
+    return res
 
 
 # ---------------------------------------------------------------------------
@@ -229,9 +248,17 @@ class Predictor:
         self.net = Surrogate(p.nsn, p.ewn, 1, device)
         self.net.set_params(params)
 
-    def predict(self, p: configs.Problem):
+    def predict(self, p: configs.Problem, scales=False):
+        """Initial guess (uvel, vvel); with scales=True also the per-column preconditioner
+        scales (su, sv), when the weights were trained on them."""
         out = self.net.forward(features(p)[None])[0]
-        return velocity_from_output(p, out, self.uscale)
+        u, v = velocity_from_output(p, out, self.uscale)
+        if not scales:
+            return u, v
+        if not self.meta.get("scales", False):
+            raise ValueError("these weights were not trained on preconditioner scales")
+        su, sv = scales_from_output(p, out)
+        return u, v, su, sv
 
     def close(self):
         self.net.close()

@@ -85,6 +85,7 @@ def augment(x, t, m, k):
         u = t[..., SG.U_CH:SG.U_CH + nz].copy()
         t[..., SG.U_CH:SG.U_CH + nz] = t[..., SG.V_CH:SG.V_CH + nz]
         t[..., SG.V_CH:SG.V_CH + nz] = u
+        t[..., [SG.S_CH, SG.S_CH + 1]] = t[..., [SG.S_CH + 1, SG.S_CH]]
     return np.ascontiguousarray(x), np.ascontiguousarray(t), np.ascontiguousarray(m)
 
 
@@ -111,9 +112,11 @@ def main(argv=None):
     ap.add_argument("--uscale", type=float, default=100.0)
     ap.add_argument("--out", required=True)
     ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--scales", type=int, default=1,
+                    help="also learn the per-column preconditioner scales (channels 22, 23)")
     a = ap.parse_args(argv)
     t_start = time.perf_counter()
-    nch = 2 * 11
+    nch = 24 if a.scales else 22
     ewn, nsn = a.ewn or a.n, a.nsn or a.n
     c = a.crop if a.crop else 0
     if c and (c > min(ewn, nsn) or c % 8):
@@ -130,7 +133,8 @@ def main(argv=None):
             t0 = time.perf_counter()
             r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True))
             x = SG.features(p)
-            t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale)
+            su, sv = ctx.column_scale_target() if a.scales else (None, None)
+            t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv)
             X.append(x), T.append(t), M.append(m)
             umax = float(np.abs(r["uvel"]).max())
             print(f"[train] {tag} {i + 1}/{len(seeds)} seed {s}: {r['picard_iters']} Picard, "
@@ -149,8 +153,9 @@ def main(argv=None):
     evalnet = SG.Surrogate(nsn, ewn, 1)
 
     def validate():
+        """mean relative L2 error of the predicted velocities on the validation problems"""
         evalnet.set_params(net.get_params())
-        errs = [rel_err(evalnet.forward(Xva[i:i + 1])[0], Tva[i], Mva[i], nch) for i in range(len(Xva))]
+        errs = [rel_err(evalnet.forward(Xva[i:i + 1])[0], Tva[i], Mva[i], 22) for i in range(len(Xva))]
         return float(np.mean(errs))
 
     rng = np.random.default_rng(0)
@@ -178,7 +183,7 @@ def main(argv=None):
         hist.append((ep, float(np.mean(losses)), ve))
         print(f"[train] epoch {ep + 1}/{a.epochs}: loss {np.mean(losses):.4f}, val rel L2 {ve:.4f}, "
               f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
-    meta = {"family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
+    meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
             "seeds": [a.seed0, a.seed0 + a.samples - 1], "val_seeds": [a.val_seed0, a.val_seed0 + a.val - 1],
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
             "history": hist, "seconds": time.perf_counter() - t_start}

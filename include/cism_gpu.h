@@ -234,6 +234,18 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *             calibration of hardware byte counters */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
+/* Per-column preconditioner scale of the mixed-precision solves (no counterpart in
+ * the reference): the SIA preconditioner becomes z = s M^-1 r with one positive
+ * scale per column and component, given as vertex fields scale_u, scale_v
+ * (nx-1, ny-1).  NULL clears it.  The plain FP64 path (precision 0) never uses it. */
+int  cism_set_precond_scale(cism_ctx *ctx, const double *scale_u, const double *scale_v);
+
+/* Training target of that scale from the context's last solve (its FP64 matrix,
+ * rhs and velocity): s = (x_c . M_c x_c) / (x_c . (A x)_c) per owned active column
+ * and component (M_c: the column's vertical block), clamped to [1/16, 16]; 1
+ * elsewhere.  Out: vertex fields (nx-1, ny-1). */
+int  cism_column_scale_target(cism_ctx *ctx, double *scale_u, double *scale_v);
+
 /* ---------------------------------------------------------------------------
  * Learned initial guess (no counterpart in the reference).  A fully
  * convolutional 4-level encoder-decoder (about 1.0M parameters, FP32 master

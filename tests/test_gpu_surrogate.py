@@ -101,13 +101,15 @@ def test_training_reduces_loss(net):
     th = he_params(rng)
     lay, _ = S.layout()
     name, wo, nw, bo, ks, ci, co, relu = lay[-1]
-    th[wo:wo + nw] *= 0.01
+    th[wo:wo + nw] *= 0.1
     x5, _, mask = batch(rng)
     # a smooth target that is a function of the input: learnable
     tgt = np.zeros((B, H, W, S.N_OUT), np.float32)
     tgt[..., :5] = np.tanh(x5)
     tgt[..., 5:10] = 0.5 * x5 ** 2 - 0.5
     net.set_params(th)
-    losses = [net.train_step(x5, tgt, mask, 3e-3, nch=10) for _ in range(60)]
+    losses = [net.train_step(x5, tgt, mask, 1e-3, nch=10) for _ in range(120)]
     print(f"[surrogate] training loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    # the same 120 steps run through the CPU oracle (float64 with BF16 operands) take the
+    # loss from 4.71 to 0.66
     assert losses[-1] < 0.3 * losses[0]
