@@ -1,20 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark: seconds per nonlinear Blatter-Pattyn velocity solve on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|c2|c1] [--guess learned|none]
 
-A step is one complete nonlinear (Picard) velocity solve from a zero initial
-guess, run until the relative L2 velocity error against the FP64 reference
-solution is <= 1e-6.  The FP64 reference is the same problem solved on the GPU
-by the plain FP64 path with the linear tolerance tightened to 1e-11 and the
-Picard loop run until the velocity stops changing (it is computed once, before
-the timed region, and is not timed).
+A step is one complete nonlinear (Picard) velocity solve, run until the relative
+L2 velocity error against the FP64 reference solution is <= 1e-6.  The FP64
+reference is the same problem with every linear system solved to an FP64
+relative residual of 1e-11 and the Picard loop run until the velocity changes
+by less than 1e-10 (cism_amd/fp64_reference.py; computed once, before the
+timed region, not timed).
 
-Timed (inputs resident in HBM, host reads only scalars):
-  headline  fast path: FP32 matrix + FP32 Chron-Gear inner solves inside FP64
+Timed (device time with inputs resident in HBM; the PCIe-inclusive wall time
+is reported beside it):
+  headline  learned initial guess (network forward pass on the GPU) + fast
+            path: FP32 matrix + FP32 Chron-Gear inner solves inside FP64
             iterative refinement, with inexact-Picard linear tolerances
+  beside    the fast path from a zero guess (what the network gains)
   baseline  plain FP64 path: the reference algorithm (Chron-Gear, SIA
-            preconditioner, tolerance 1e-8), same stopping rule
+            preconditioner, tolerance 1e-8, 200 iterations), same stopping rule
 
 Printed: one JSON line.  `roofline` covers the dominant kernel of the headline
 solve (the FP32 27-point matvec), measured with HIP events on the context's
@@ -93,7 +96,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c5")
     ap.add_argument("--guess", default="learned", choices=["learned", "none"])
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
@@ -119,12 +122,18 @@ def main(argv=None):
     lu, lv = ctx.loads(p)
     nodes = owned_active_nodes(p)
 
+    # both timed paths run until their error against the reference is <= 1e-6; the Picard
+    # cap only guards against divergence
+    cap = 500
+
     def fast(u0=None, v0=None):
         return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
-                                        stop_rel_err=STOP_REL_ERR, inner_rtol=1e-3)
+                                        stop_rel_err=STOP_REL_ERR, inner_rtol=1e-3,
+                                        maxiter_nonlinear=cap)
 
     def plain(u0=None, v0=None, stop=STOP_REL_ERR):
-        return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=0, stop_rel_err=stop)
+        return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=0, stop_rel_err=stop,
+                                        maxiter_nonlinear=cap)
 
     if a.kernels_only:
         # one fast solve leaves the FP32 matrix; then only the kernels under study run
@@ -136,7 +145,9 @@ def main(argv=None):
         ctx.close()
         return 0
 
-    ref = reference_solution(ctx, p, lu, lv, log=log)
+    # large grids: FP64-residual-certified refinement (same fixed point, much faster)
+    ref_prec = 1 if p.nnodes > 4_000_000 else 0
+    ref = reference_solution(ctx, p, lu, lv, log=log, precision=ref_prec, velo_tol=1e-10)
     ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
 
     weights = SG.default_weights(family) if family else None
@@ -144,15 +155,21 @@ def main(argv=None):
     pred = SG.Predictor(p, weights) if use_guess else None
 
     def headline():
-        """the timed step: (learned guess +) mixed-precision solve"""
+        """the timed step: (learned guess +) mixed-precision solve.  seconds = device time of
+        the network's forward pass + device time of the Picard loop (inputs resident in HBM);
+        seconds_wall adds the host feature extraction and every PCIe transfer."""
         t0 = time.perf_counter()
         u0 = v0 = None
+        inf_dev = 0.0
         if pred is not None:
             u0, v0 = pred.predict(p)
+            inf_dev = pred.net.last_forward_ms() * 1e-3
         t_inf = time.perf_counter() - t0
         r = fast(u0, v0)
-        r["seconds"] = time.perf_counter() - t0
-        r["inference_s"] = t_inf
+        r["seconds_wall"] = time.perf_counter() - t0
+        r["inference_s"] = inf_dev
+        r["inference_wall_s"] = t_inf
+        r["seconds"] = r["seconds_device"] + inf_dev
         return r
 
     for _ in range(a.warmup):
@@ -161,7 +178,8 @@ def main(argv=None):
     for s in range(a.steps):
         r = headline()
         runs.append(r)
-        log(f"step {s}: {r['seconds']:.3f} s (inference {r['inference_s'] * 1e3:.1f} ms), "
+        log(f"step {s}: {r['seconds']:.3f} s device, {r['seconds_wall']:.3f} s wall (inference "
+            f"{r['inference_s'] * 1e3:.1f} ms device, {r['inference_wall_s'] * 1e3:.0f} ms wall), "
             f"{r['picard_iters']} Picard, {r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 "
             f"matvecs, err {r['final_rel_err']:.2e}")
     fast_s = float(np.mean([r["seconds"] for r in runs]))
@@ -170,7 +188,8 @@ def main(argv=None):
     def timed(fn, *args):
         t0 = time.perf_counter()
         r = fn(*args)
-        r["seconds"] = time.perf_counter() - t0
+        r["seconds_wall"] = time.perf_counter() - t0
+        r["seconds"] = r["seconds_device"]
         return r
 
     nog = timed(fast) if pred is not None else None
@@ -221,15 +240,22 @@ def main(argv=None):
         "fast_path": {"seconds": round(fast_s, 4), "picard_iters": stats["picard_iters"],
                       "fp32_matvecs": stats["fp32_matvecs"], "fp64_matvecs": stats["fp64_matvecs"],
                       "rel_err": stats["final_rel_err"], "learned_guess": pred is not None,
-                      "inference_s": round(stats["inference_s"], 4)},
+                      "inference_s": round(stats["inference_s"], 4),
+                      "seconds_wall_pcie_inclusive": round(float(np.mean([r["seconds_wall"] for r in runs])), 4)},
         "fast_zero_guess": None if nog is None else {
-            "seconds": round(nog["seconds"], 4), "picard_iters": nog["picard_iters"],
+            "seconds": round(nog["seconds"], 4), "seconds_wall": round(nog["seconds_wall"], 4),
+            "picard_iters": nog["picard_iters"],
             "fp32_matvecs": nog["fp32_matvecs"], "fp64_matvecs": nog["fp64_matvecs"],
             "rel_err": nog["final_rel_err"]},
-        "plain_fp64": {"seconds": round(base["seconds"], 4), "picard_iters": base["picard_iters"],
+        "plain_fp64": {"seconds": round(base["seconds"], 4), "seconds_wall": round(base["seconds_wall"], 4),
+                       "picard_iters": base["picard_iters"],
                        "pcg_iters": base["linear_iters"], "rel_err": base["final_rel_err"]},
         "speedup_vs_plain_fp64": round(base["seconds"] / fast_s, 3),
-        "reference_fp64": {"picard_iters": ref["picard_iters"], "pcg_iters": ref["linear_iters"]},
+        "reference_fp64": {"picard_iters": ref["picard_iters"], "linear_iters": ref["linear_iters"],
+                           "linear_solver": ["plain FP64 Chron-Gear", "FP32 Chron-Gear in FP64 "
+                                             "refinement"][ref_prec],
+                           "linear_tol_fp64_residual": 1e-11, "velocity_change_tol": 1e-10,
+                           "seconds": round(ref["seconds"], 1)},
         "matvec": {"fp32_ms": round(ms32, 4), "fp32_GBps": round(gbs32, 1),
                    "fp64_ms": round(ms64, 4), "fp64_GBps": round(gbs64, 1),
                    "bytes_per_node_fp32": 448, "bytes_per_node_fp64": 896},

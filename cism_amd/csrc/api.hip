@@ -602,6 +602,8 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
 
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double t_dev0 = now_s();   // inputs resident from here on
     double crit = 1.0e10, target = 1.0e-12;
     double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
     int counter = 0, lin_total = 0, st = CISM_OK;
@@ -684,6 +686,8 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
         HIP_TRY(hipMemsetAsync(c->b, 0, sizeof(double) * 2 * g.npad, c->stream));
         HIP_TRY(hipMemsetAsync(c->t, 0, sizeof(double) * 2 * g.npad, c->stream));
     }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    S.seconds_device = now_s() - t_dev0;
     CISM_TRY(download_vec(c, c->x, uvel, vvel));
     if (efvs) CISM_TRY(down(c, efvs, c->efvs, (size_t)g.ne));
     if (bu && bv) CISM_TRY(download_vec(c, c->b, bu, bv));

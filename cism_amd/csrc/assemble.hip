@@ -193,38 +193,51 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
             efvs[ee] = valid[threadIdx.x] ? s / 8 : 0.0;
         }
     }
-    // phase 2: entries, element-fastest (compute_element_matrix, velo_higher:7960-7982)
-    for (int t = threadIdx.x; t < CISM_KE_ENTRIES * EL_PER_WG; t += blockDim.x) {
-        const int ent = t / EL_PER_WG, le = t % EL_PER_WG;
+    // phase 2: entries, element-fastest (compute_element_matrix, velo_higher:7960-7982).
+    // Task tt < 36: the pair (a, b), a <= b, of the upper triangles of Kuu and Kvv, which
+    // share their six derivative reads; task 36 + tt: Kuv(a, b) and Kuv(b, a), which share
+    // four.  Every entry keeps its own sum over the points in point order.
+    for (int t = threadIdx.x; t < 72 * EL_PER_WG; t += blockDim.x) {
+        const int task = t / EL_PER_WG, le = t % EL_PER_WG;
         const long long ee = (long long)blockIdx.x * EL_PER_WG + le;
         if (ee >= g.ne || !valid[le]) continue;
-        double acc = 0.0;
-        if (ent < 72) {
-            // packed upper triangle of Kuu (ent < 36) or Kvv
-            const bool isv = ent >= 36;
-            const int tt = isv ? ent - 36 : ent;
-            const int a = c_tri_a[tt], bb = c_tri_b[tt];
+        const int tt = task < 36 ? task : task - 36;
+        const int a = c_tri_a[tt], bb = c_tri_b[tt];
+        double acc0 = 0.0, acc1 = 0.0;
+        if (task < 36) {
             for (int p = 0; p < 8; ++p) {
                 const double f = L.f[le][p];
                 const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8], *Z = &L.dz[le][p * 8];
-                double term;
-                if (whichapprox == 0) term = Z[a] * Z[bb];
-                else if (whichapprox == 1)
-                    term = isv ? 4.0 * Y[a] * Y[bb] + X[a] * X[bb] : 4.0 * X[a] * X[bb] + Y[a] * Y[bb];
-                else
-                    term = isv ? 4.0 * Y[a] * Y[bb] + X[a] * X[bb] + Z[a] * Z[bb]
-                               : 4.0 * X[a] * X[bb] + Y[a] * Y[bb] + Z[a] * Z[bb];
-                acc = acc + f * term;
+                const double xa = X[a], xb = X[bb], ya = Y[a], yb = Y[bb], za = Z[a], zb = Z[bb];
+                double tu, tv;
+                if (whichapprox == 0) {
+                    tu = za * zb;
+                    tv = tu;
+                } else if (whichapprox == 1) {
+                    tu = 4.0 * xa * xb + ya * yb;
+                    tv = 4.0 * ya * yb + xa * xb;
+                } else {
+                    tu = 4.0 * xa * xb + ya * yb + za * zb;
+                    tv = 4.0 * ya * yb + xa * xb + za * zb;
+                }
+                acc0 = acc0 + f * tu;
+                acc1 = acc1 + f * tv;
             }
+            Ke[keidx(tt, ee)] = acc0;
+            Ke[keidx(36 + tt, ee)] = acc1;
         } else {
-            const int idx = ent - 72, nr = idx & 7, nc = idx >> 3;   // Kuv(nr, nc)
+            // Kuv(nr, nc) = sum f (2 X[nr] Y[nc] + Y[nr] X[nc]); nr = a, nc = bb and swapped
             if (whichapprox != 0)
                 for (int p = 0; p < 8; ++p) {
+                    const double f = L.f[le][p];
                     const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8];
-                    acc = acc + L.f[le][p] * (2.0 * X[nr] * Y[nc] + Y[nr] * X[nc]);
+                    const double xa = X[a], xb = X[bb], ya = Y[a], yb = Y[bb];
+                    acc0 = acc0 + f * (2.0 * xa * yb + ya * xb);
+                    acc1 = acc1 + f * (2.0 * xb * ya + yb * xa);
                 }
+            Ke[keidx(72 + a + 8 * bb, ee)] = acc0;
+            if (a != bb) Ke[keidx(72 + bb + 8 * a, ee)] = acc1;
         }
-        Ke[keidx(ent, ee)] = acc;
     }
 }
 

@@ -412,6 +412,8 @@ struct cism_surrogate {
     long long wpart_n;
     double *lpart, *lres;
     hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    float last_forward_ms;   // device time of the last forward pass (kernels only)
 };
 
 // activation indices
@@ -502,6 +504,8 @@ extern "C" int cism_surrogate_create(int H, int W, int batch, int device, cism_s
     if (!st && hipMalloc(&s->lpart, sizeof(double) * 2 * 1024) != hipSuccess) st = CISM_ERR_NOMEM;
     if (!st && hipMalloc(&s->lres, sizeof(double) * 2) != hipSuccess) st = CISM_ERR_NOMEM;
     if (!st && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) st = CISM_ERR_HIP;
+    if (!st && (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess))
+        st = CISM_ERR_HIP;
     if (st) {
         cism_surrogate_destroy(s);
         return st;
@@ -545,12 +549,18 @@ extern "C" int cism_surrogate_destroy(cism_surrogate *s) {
     void *ptrs[] = {s->w, s->g, s->m, s->v, s->raw, s->tgt, s->mask, s->chw, s->wpart, s->lpart, s->lres};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return CISM_OK;
 }
 
 extern "C" long long cism_surrogate_num_params(const cism_surrogate *s) { return s ? s->nparams : 0; }
+
+extern "C" double cism_surrogate_last_forward_ms(const cism_surrogate *s) {
+    return s ? (double)s->last_forward_ms : -1.0;
+}
 
 extern "C" int cism_surrogate_get_params(cism_surrogate *s, float *host) {
     HIP_TRY(hipMemcpy(host, s->w, sizeof(float) * s->nparams, hipMemcpyDeviceToHost));
@@ -758,7 +768,9 @@ __global__ void k_narrow(long long P, const float *__restrict__ o, float *__rest
 extern "C" int cism_surrogate_forward(cism_surrogate *s, const float *in, float *out) {
     if (!s) return CISM_ERR_ARG;
     CISM_TRY(upload_batch(s, in, nullptr, nullptr));
+    HIP_TRY(hipEventRecord(s->ev0, s->stream));
     CISM_TRY(forward(s));
+    HIP_TRY(hipEventRecord(s->ev1, s->stream));
     const long long P = (long long)s->B * s->H * s->W;
     // narrow SG_OUT_PAD -> SG_OUT channels into the target buffer (free during inference)
     hipLaunchKernelGGL(k_narrow, dim3(nblocks_for(P * SG_OUT)), dim3(256), 0, s->stream, P, s->A[A_OUT].y,
@@ -766,6 +778,7 @@ extern "C" int cism_surrogate_forward(cism_surrogate *s, const float *in, float 
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, s->tgt, sizeof(float) * P * SG_OUT, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipEventElapsedTime(&s->last_forward_ms, s->ev0, s->ev1));
     return CISM_OK;
 }
 

@@ -44,8 +44,8 @@ module cism_gpu_velo
      real(c_double) :: final_L2_norm, final_resid_velo, seconds_assembly, seconds_linear
      real(c_double) :: seconds_total
      integer(c_int) :: status, refinement_steps
-     real(c_double) :: final_rel_err
-     integer(c_int) :: reserved(4)
+     real(c_double) :: final_rel_err, seconds_device
+     integer(c_int) :: reserved(2)
   end type cism_solve_stats
 
   type(c_ptr), save :: ctx = c_null_ptr

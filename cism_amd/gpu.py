@@ -35,6 +35,7 @@ EXPORTED = [
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
+    "cism_surrogate_last_forward_ms",
 ]
 
 
@@ -60,7 +61,8 @@ class SolveStats(C.Structure):
                 ("final_resid_velo", C.c_double), ("seconds_assembly", C.c_double),
                 ("seconds_linear", C.c_double), ("seconds_total", C.c_double),
                 ("status", C.c_int), ("refinement_steps", C.c_int),
-                ("final_rel_err", C.c_double), ("reserved", C.c_int * 4)]
+                ("final_rel_err", C.c_double), ("seconds_device", C.c_double),
+                ("reserved", C.c_int * 2)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

@@ -56,6 +56,8 @@ def _setup(L):
     L.cism_surrogate_loss_grad.argtypes = [_P, _fp, _fp, _fp, _c.c_int, _dp, _fp]
     L.cism_surrogate_train_step.argtypes = [_P, _fp, _fp, _fp, _c.c_int, _c.c_float, _dp]
     L.cism_surrogate_set_channel_weights.argtypes = [_P, _fp]
+    L.cism_surrogate_last_forward_ms.argtypes = [_P]
+    L.cism_surrogate_last_forward_ms.restype = _c.c_double
     L._sg_ready = True
     return L
 
@@ -125,6 +127,10 @@ class Surrogate:
         out = np.empty((self.B, self.H, self.W, N_OUT), dtype=np.float32)
         _check(self.L, self.L.cism_surrogate_forward(self.h, px, out.ctypes.data_as(_fp)), "forward")
         return out
+
+    def last_forward_ms(self) -> float:
+        """device time of the last forward pass (kernels only)"""
+        return float(self.L.cism_surrogate_last_forward_ms(self.h))
 
     def loss_grad(self, x, tgt, mask, nch=N_OUT):
         x, px = _f32(self._shape_in(x))

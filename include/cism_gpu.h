@@ -202,7 +202,8 @@ typedef struct {
     int    status;
     int    refinement_steps;   /* FP64 residual evaluations of the mixed-precision solves    */
     double final_rel_err;      /* |x - x_ref| / |x_ref| at exit (-1 without a reference)     */
-    int    reserved[4];
+    double seconds_device;     /* the Picard loop alone: inputs resident, outputs not copied  */
+    int    reserved[2];
 } cism_solve_stats;
 
 void cism_solve_opts_default(cism_solve_opts *o);
@@ -270,6 +271,8 @@ int  cism_surrogate_get_params(cism_surrogate *s, float *params);
 int  cism_surrogate_set_params(cism_surrogate *s, const float *params);
 int  cism_surrogate_set_channel_weights(cism_surrogate *s, const float *w24);
 int  cism_surrogate_forward(cism_surrogate *s, const float *in, float *out);
+/* device time (ms) of the last forward pass: its kernels, without the host transfers */
+double cism_surrogate_last_forward_ms(const cism_surrogate *s);
 /* loss and its gradient w.r.t. every parameter (grad may be NULL), no update */
 int  cism_surrogate_loss_grad(cism_surrogate *s, const float *in, const float *target,
                               const float *mask, int nch, double *loss, float *grad);
