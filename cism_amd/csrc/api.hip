@@ -229,7 +229,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals,
-                    c->cscale};
+                    c->cscale, c->aa, c->aa_sums};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -567,6 +567,7 @@ extern "C" void cism_solve_opts_default(cism_solve_opts *o) {
     o->dx = o->dy = 1000.0;
     o->inner_rtol = 1.0e-3;
     o->adaptive_gamma = 0.05;
+    o->anderson_m = 0;
 }
 
 static double now_s() {
@@ -602,6 +603,11 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
 
+    if (o->anderson_m < 0 || o->anderson_m > AA_MAX) {
+        cism_set_error("anderson_m must be 0..%d", AA_MAX);
+        return CISM_ERR_ARG;
+    }
+    anderson_reset(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double t_dev0 = now_s();   // inputs resident from here on
     double crit = 1.0e10, target = 1.0e-12;
@@ -666,6 +672,11 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
         t_lin += now_s() - tl;
         lin_total += niters;
         CISM_TRY(launch_halo_vec(c, c->x));
+        if (o->precision == 1 && o->anderson_m > 0) {
+            double fn = 0.0;
+            CISM_TRY(anderson_step(c, o->anderson_m, &fn));
+            CISM_TRY(launch_halo_vec(c, c->x));
+        }
         CISM_TRY(residual_velocity_device(c, o->whichresid));
         CISM_TRY(sync_scalars(c));
         resid_velo = c->sc_host->sums[0];

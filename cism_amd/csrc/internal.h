@@ -73,6 +73,10 @@ struct cism_ctx {
     // optional per-column preconditioner scale of the mixed-precision solves (2*ncol)
     float *cscale;
     int cscale_on;
+    // Anderson acceleration of the Picard map: m columns of dX and dF, previous x and f
+    double *aa, *aa_sums;
+    int aa_m, aa_count, aa_head;
+    double aa_fnorm_prev;
     // masks and geometry
     int32_t *av, *ac, *umask, *vmask;
     double *xV, *yV, *susrf, *sthck, *beta, *sigma, *wsig, *flwafact, *efvs;
@@ -224,6 +228,11 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
 int column_scale_device(cism_ctx *c, double *out);  // out: 2*ncol doubles
+// Anderson-accelerated update of c->x (the Picard map's output) from c->usav (its
+// input); m <= AA_MAX columns; restarts the history when the fixed-point residual grows.
+#define AA_MAX 4
+int anderson_reset(cism_ctx *c);
+int anderson_step(cism_ctx *c, int m, double *fnorm);
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
 int convert_a32(cism_ctx *c);
 int launch_cg_matvec(cism_ctx *c, int fp32);

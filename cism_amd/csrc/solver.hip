@@ -788,6 +788,199 @@ int column_scale_device(cism_ctx *c, double *out) {
 }
 
 // ---------------------------------------------------------------------------
+// Anderson acceleration (type II, Walker & Ni 2011) of the Picard fixed-point
+// map x -> G(x), G(x) = the solution of A(x) y = b(x).  With f_n = G(x_n) - x_n
+// and the last k <= m differences dX_i = x_{i+1} - x_i, dF_i = f_{i+1} - f_i:
+//   gamma = argmin |f_n - dF gamma|_2,   x_{n+1} = G(x_n) - (dX + dF) gamma.
+// The fixed point is the Picard fixed point; only the path to it changes.
+// Buffers (each 2*npad doubles): dX[0..m), dF[0..m), xprev, fprev.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_form(long long n2, const double *__restrict__ x,
+                                                        const double *__restrict__ xn,
+                                                        double *__restrict__ dX, double *__restrict__ dF,
+                                                        double *__restrict__ xprev,
+                                                        double *__restrict__ fprev, int add) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
+    const double f = x[t] - xn[t];
+    if (add) {
+        dF[t] = f - fprev[t];
+        dX[t] = xn[t] - xprev[t];
+    }
+    fprev[t] = f;
+    xprev[t] = xn[t];
+}
+
+// Gram matrix of the k columns of dF (upper triangle, row-major), then dF_i . f,
+// then f . f; owned nodes only.  NV = 4*5/2 + 4 + 1 = 15 slots, unused ones 0.
+#define AA_NV 15
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, int k, const double *__restrict__ dF,
+                                                        long long stride, const double *__restrict__ f,
+                                                        double *__restrict__ partials) {
+    double acc[AA_NV];
+#pragma unroll
+    for (int c = 0; c < AA_NV; ++c) acc[c] = 0.0;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.nn) {
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        if (is_owned(g, i, j)) {
+            for (int h = 0; h < 2; ++h) {
+                const long long t = 2 * q + h;
+                double v[AA_MAX];
+                for (int a = 0; a < AA_MAX; ++a) v[a] = a < k ? dF[a * stride + t] : 0.0;
+                const double fv = f[t];
+                int s = 0;
+                for (int a = 0; a < AA_MAX; ++a)
+                    for (int b = a; b < AA_MAX; ++b) acc[s++] += v[a] * v[b];
+                for (int a = 0; a < AA_MAX; ++a) acc[s++] += v[a] * fv;
+                acc[s] += fv * fv;
+            }
+        }
+    }
+    block_reduce_store<AA_NV>(acc, partials);
+}
+
+// fixed-order reduction of the per-block partials: thread t sums blocks t, t+256, ...;
+// then a tree over the 256 threads per value
+__global__ __launch_bounds__(256) void k_aa_reduce(const double *__restrict__ partials, int nb,
+                                                   double *__restrict__ out) {
+    __shared__ double red[256];
+    double acc[AA_NV];
+    for (int c = 0; c < AA_NV; ++c) acc[c] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int c = 0; c < AA_NV; ++c) acc[c] += partials[(long long)b * AA_NV + c];
+    for (int c = 0; c < AA_NV; ++c) {
+        red[threadIdx.x] = acc[c];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_update(long long n2, int k, const double *__restrict__ dX,
+                                                          const double *__restrict__ dF, long long stride,
+                                                          double g0, double g1, double g2, double g3,
+                                                          double *__restrict__ x) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
+    const double gm[AA_MAX] = {g0, g1, g2, g3};
+    double v = x[t];
+    for (int a = 0; a < k; ++a) v -= gm[a] * (dX[a * stride + t] + dF[a * stride + t]);
+    x[t] = v;
+}
+
+int anderson_reset(cism_ctx *c) {
+    c->aa_count = 0;
+    c->aa_head = 0;
+    c->aa_fnorm_prev = 0.0;
+    return CISM_OK;
+}
+
+// Solves the k x k system G gamma = h by Cholesky with a relative Tikhonov term.
+static bool small_lsq(int k, const double *Gu, const double *h, double *gamma) {
+    double G[AA_MAX][AA_MAX], L[AA_MAX][AA_MAX] = {};
+    int s = 0;
+    double tr = 0.0;
+    for (int a = 0; a < AA_MAX; ++a)
+        for (int b = a; b < AA_MAX; ++b, ++s) G[a][b] = G[b][a] = Gu[s];
+    for (int a = 0; a < k; ++a) tr += G[a][a];
+    if (!(tr > 0.0)) return false;
+    for (int a = 0; a < k; ++a) G[a][a] += 1.0e-10 * tr;
+    for (int a = 0; a < k; ++a) {
+        for (int b = 0; b <= a; ++b) {
+            double v = G[a][b];
+            for (int q = 0; q < b; ++q) v -= L[a][q] * L[b][q];
+            if (a == b) {
+                if (!(v > 0.0)) return false;
+                L[a][a] = sqrt(v);
+            } else {
+                L[a][b] = v / L[b][b];
+            }
+        }
+    }
+    double y[AA_MAX];
+    for (int a = 0; a < k; ++a) {
+        double v = h[a];
+        for (int q = 0; q < a; ++q) v -= L[a][q] * y[q];
+        y[a] = v / L[a][a];
+    }
+    for (int a = k - 1; a >= 0; --a) {
+        double v = y[a];
+        for (int q = a + 1; q < k; ++q) v -= L[q][a] * gamma[q];
+        gamma[a] = v / L[a][a];
+    }
+    return true;
+}
+
+int anderson_step(cism_ctx *c, int m, double *fnorm) {
+    const GridDev &g = c->g;
+    const long long n2 = 2 * g.npad;
+    if (m < 1 || m > AA_MAX) {
+        cism_set_error("Anderson depth must be 1..%d", AA_MAX);
+        return CISM_ERR_ARG;
+    }
+    if (!c->aa || c->aa_m != m) {
+        if (c->aa) HIP_TRY(hipFree(c->aa));
+        c->aa = nullptr;
+        hipError_t e = hipMalloc(&c->aa, sizeof(double) * (size_t)(2 * m + 2) * n2);
+        if (e != hipSuccess) {
+            cism_set_error("Anderson history allocation failed: %s", hipGetErrorString(e));
+            return CISM_ERR_NOMEM;
+        }
+        if (!c->aa_sums) HIP_TRY(hipMalloc(&c->aa_sums, sizeof(double) * 32));
+        c->aa_m = m;
+        anderson_reset(c);
+    }
+    double *dX = c->aa, *dF = c->aa + (size_t)m * n2, *xprev = c->aa + (size_t)2 * m * n2,
+           *fprev = xprev + n2;
+    const int add = c->aa_count > 0;
+    const int slot = c->aa_head;
+    hipLaunchKernelGGL(k_aa_form, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x, c->usav,
+                       dX + (size_t)slot * n2, dF + (size_t)slot * n2, xprev, fprev, add);
+    const int k = add ? (c->aa_count < m ? c->aa_count : m) : 0;
+    if (add) c->aa_head = (c->aa_head + 1) % m;
+    c->aa_count++;
+    // dots: Gram of the k stored columns, their products with f (= fprev now), |f|^2
+    const int nb = nblocks_for(g.nn);
+    hipLaunchKernelGGL(k_aa_dots, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, k, dF, n2, fprev,
+                       c->partials);
+    hipLaunchKernelGGL(k_aa_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->aa_sums);
+    HIP_TRY(hipGetLastError());
+    double hs[AA_NV];
+    HIP_TRY(hipMemcpyAsync(hs, c->aa_sums, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double fn = sqrt(hs[AA_NV - 1]);
+    *fnorm = fn;
+    // safeguard: a growing fixed-point residual restarts the history (plain Picard step)
+    if (k == 0 || (c->aa_fnorm_prev > 0.0 && fn > 2.0 * c->aa_fnorm_prev)) {
+        if (k > 0) {
+            anderson_reset(c);
+            c->aa_count = 1;   // the current (x, f) stays as the base of the next difference
+        }
+        c->aa_fnorm_prev = fn;
+        return CISM_OK;
+    }
+    c->aa_fnorm_prev = fn;
+    // columns are stored in ring order; the least-squares problem does not depend on it
+    double gam[AA_MAX] = {0.0, 0.0, 0.0, 0.0};
+    double Gu[AA_NV], h[AA_MAX];
+    int s = 0;
+    for (int a = 0; a < AA_MAX; ++a)
+        for (int b = a; b < AA_MAX; ++b, ++s) Gu[s] = hs[s];
+    for (int a = 0; a < AA_MAX; ++a) h[a] = hs[10 + a];
+    if (!small_lsq(k, Gu, h, gam)) return CISM_OK;   // degenerate history: plain step
+    hipLaunchKernelGGL(k_aa_update, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, k, dX, dF, n2,
+                       gam[0], gam[1], gam[2], gam[3], c->x);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
 // measurement helpers
 // ---------------------------------------------------------------------------
 // one launch of the solver's matvec exactly as the Chron-Gear iteration issues it

@@ -36,7 +36,7 @@ module cism_gpu_velo
      real(c_double) :: efvs_constant, dx, dy, target_override, inner_rtol
      integer(c_int) :: adaptive_linear
      real(c_double) :: adaptive_gamma, stop_rel_err
-     integer(c_int) :: reserved(5)
+     integer(c_int) :: anderson_m, reserved(4)
   end type cism_solve_opts
 
   type, bind(C) :: cism_solve_stats

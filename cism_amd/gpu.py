@@ -52,7 +52,7 @@ class SolveOpts(C.Structure):
                 ("efvs_constant", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
-                ("stop_rel_err", C.c_double), ("reserved", C.c_int * 5)]
+                ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class SolveStats(C.Structure):
@@ -310,7 +310,8 @@ class Context:
     def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
                              maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
                              whichresid=None, target_override=0.0, inner_rtol=1e-3,
-                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0):
+                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0,
+                             anderson_m=0):
         o = SolveOpts()
         lib().cism_solve_opts_default(C.byref(o))
         o.whichbabc = p.whichbabc
@@ -328,6 +329,7 @@ class Context:
         o.adaptive_linear = adaptive_linear
         o.adaptive_gamma = adaptive_gamma
         o.stop_rel_err = stop_rel_err
+        o.anderson_m = anderson_m
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
         shape = (p.nz, p.nx - 1, p.ny - 1)

@@ -186,7 +186,9 @@ typedef struct {
     double adaptive_gamma;     /* forcing factor of adaptive_linear (default 0.05)           */
     double stop_rel_err;       /* > 0: also stop once |x - x_ref| / |x_ref| <= stop_rel_err,
                                   x_ref set by cism_set_reference_velocity                   */
-    int    reserved[5];
+    int    anderson_m;         /* precision 1: Anderson acceleration of the Picard map with
+                                  this many history columns (0 off, at most 4)              */
+    int    reserved[4];
 } cism_solve_opts;
 
 typedef struct {
