@@ -213,7 +213,8 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
         (st = dalloc(&c->flwafact, (size_t)g.nepad)) || (st = dalloc(&c->efvs, (size_t)g.nepad)))
         return fail(st);
     const size_t nparts = (size_t)(nblocks_for(g.nn) > CISM_RED_BLOCKS ? nblocks_for(g.nn) : CISM_RED_BLOCKS);
-    if ((st = dalloc(&c->partials, nparts * 8))) return fail(st);
+    // per-block partial sums: up to CISM_MAX_NV values per block (the widest reduction)
+    if ((st = dalloc(&c->partials, nparts * CISM_MAX_NV))) return fail(st);
     if ((st = dalloc(&c->sc, 1))) return fail(st);
     if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
         return fail(CISM_ERR_HIP);
@@ -783,6 +784,105 @@ extern "C" int cism_column_scale_target(cism_ctx *c, double *scale_u, double *sc
     HIP_TRY(hipMemcpyAsync(scale_u, out, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(scale_v, out + ncol, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// post-solve diagnostics
+// ---------------------------------------------------------------------------
+static int download_tau(cism_ctx *c, const double *tau, double *txz, double *tyz, double *txx,
+                        double *tyy, double *txy, double *teff) {
+    const long long ne = c->g.ne;
+    double *outs[6] = {txz, tyz, txx, tyy, txy, teff};
+    for (int q = 0; q < 6; ++q)
+        if (outs[q]) CISM_TRY(down(c, outs[q], tau + q * ne, (size_t)ne));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_compute_internal_stress(cism_ctx *c, const double *sigma, const int32_t *active_cell,
+                                            const double *xVertex, const double *yVertex,
+                                            const double *stagusrf, const double *stagthck,
+                                            const double *flwafact, const double *efvs, int whichefvs,
+                                            double efvs_constant, int whichapprox, const double *uvel,
+                                            const double *vvel, double *tau_xz, double *tau_yz,
+                                            double *tau_xx, double *tau_yy, double *tau_xy,
+                                            double *tau_eff) {
+    CTX_BEGIN(c);
+    (void)efvs;   // used by the reference for L1L2 only, which this path does not run
+    CISM_TRY(check_options(whichapprox, whichefvs));
+    CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    CISM_TRY(ensure_ke(c));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    CISM_TRY(internal_stress_device(c, whichapprox, whichefvs, efvs_constant, c->Ke));
+    CISM_TRY(check_flags(c));
+    return download_tau(c, c->Ke, tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff);
+}
+
+extern "C" int cism_compute_basal_friction_heatflx(cism_ctx *c, const int32_t *active_cell,
+                                                   const int32_t *active_vertex, const double *xVertex,
+                                                   const double *yVertex, const double *ubas,
+                                                   const double *vbas, const double *beta,
+                                                   int whichassemble_bfric, double *bfricflx) {
+    CTX_BEGIN(c);
+    (void)xVertex, (void)yVertex;   // the reference's quadrature weights are 1: no geometry enters
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(up(c, c->ac, active_cell, (size_t)g.nx * g.ny));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta, ncol));
+    // interleave the basal velocity (u, v per vertex) into the staging buffer
+    double *xb = c->vstage + 2 * ncol;   // vstage holds 2*npad >= 4*ncol doubles
+    HIP_TRY(hipMemcpyAsync(c->vstage, ubas, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->vstage + ncol, vbas, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_interleave, dim3(nblocks_for(2 * (long long)ncol)), dim3(256), 0, c->stream,
+                       (long long)ncol, c->vstage, xb);
+    HIP_TRY(hipGetLastError());
+    CISM_TRY(ensure_ke(c));
+    CISM_TRY(basal_diagnostics_device(c, whichassemble_bfric, c->Ke, nullptr, xb, 2));
+    CISM_TRY(down(c, bfricflx, c->Ke, (size_t)g.nx * g.ny));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_compute_basal_traction(cism_ctx *c, const double *beta, const double *uvel,
+                                           const double *vvel, double *btractx, double *btracty) {
+    CTX_BEGIN(c);
+    const size_t ncol = (size_t)c->g.sx * c->g.sy;
+    CISM_TRY(up(c, c->beta, beta, ncol));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    CISM_TRY(ensure_ke(c));
+    CISM_TRY(basal_diagnostics_device(c, 1, nullptr, c->Ke));
+    CISM_TRY(down(c, btractx, c->Ke, ncol));
+    CISM_TRY(down(c, btracty, c->Ke + ncol, ncol));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_velo_diagnostics(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
+                                     int whichassemble_bfric, double *tau_xz, double *tau_yz,
+                                     double *tau_xx, double *tau_yy, double *tau_xy, double *tau_eff,
+                                     double *bfricflx, double *btractx, double *btracty) {
+    CTX_BEGIN(c);
+    const GridDev &g = c->g;
+    CISM_TRY(ensure_ke(c));
+    if (tau_xz || tau_yz || tau_xx || tau_yy || tau_xy || tau_eff) {
+        CISM_TRY(check_options(whichapprox, whichefvs));
+        HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+        CISM_TRY(internal_stress_device(c, whichapprox, whichefvs, efvs_constant, c->Ke));
+        CISM_TRY(check_flags(c));
+        CISM_TRY(download_tau(c, c->Ke, tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff));
+    }
+    if (bfricflx || btractx || btracty) {
+        const size_t ncol = (size_t)g.sx * g.sy;
+        double *bf = c->Ke, *bt = c->Ke + (size_t)g.nx * g.ny;
+        CISM_TRY(basal_diagnostics_device(c, whichassemble_bfric, bf, bt));
+        if (bfricflx) CISM_TRY(down(c, bfricflx, bf, (size_t)g.nx * g.ny));
+        if (btractx) CISM_TRY(down(c, btractx, bt, ncol));
+        if (btracty) CISM_TRY(down(c, btracty, bt + ncol, ncol));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     return CISM_OK;
 }
 

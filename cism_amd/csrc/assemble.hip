@@ -55,6 +55,98 @@ __constant__ unsigned char c_tri_a[36] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,
 __constant__ unsigned char c_tri_b[36] = {0, 1, 2, 3, 4, 5, 6, 7, 1, 2, 3, 4, 5, 6, 7, 2, 3, 4,
                                           5, 6, 7, 3, 4, 5, 6, 7, 4, 5, 6, 7, 5, 6, 7, 6, 7, 7};
 
+// One quadrature point p of a hexahedral element whose 8 nodes are given as
+// (x, y, z, u, v) records: basis-function derivatives (get_basis_function_
+// derivatives_3d, velo_higher:6360-6579), the velocity gradient (ux uy uz vx vy vz)
+// and the Glen-law viscosity (compute_effective_viscosity, velo_higher:7199-7378).
+// A zero Jacobian determinant gives zero derivatives and sets `singular`.
+__device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes, int whichapprox,
+                                        int whichefvs, double efvs_constant, double ff,
+                                        double *dxo, double *dyo, double *dzo, double &det,
+                                        double &ev, double (&strain)[6], bool &singular) {
+    const double r = 1.0 / sqrt(3.0);
+    const double xq = (p == 0 || p == 3 || p == 4 || p == 7) ? -r : r;
+    const double yq = (p < 2 || p == 4 || p == 5) ? -r : r;
+    const double zq = p < 4 ? -r : r;
+    double a[8], b[8], cz[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
+        const double sz = c_node_kl[n] ? -1.0 : 1.0;
+        const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
+        const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
+        const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
+        a[n] = sx * (ay * az / 8.0);
+        b[n] = sy * (ax * az / 8.0);
+        cz[n] = sz * (ax * ay / 8.0);
+    }
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double X = nodes[n * 5 + 0], Y = nodes[n * 5 + 1], Z = nodes[n * 5 + 2];
+        J[0][0] = J[0][0] + a[n] * X;  J[0][1] = J[0][1] + a[n] * Y;  J[0][2] = J[0][2] + a[n] * Z;
+        J[1][0] = J[1][0] + b[n] * X;  J[1][1] = J[1][1] + b[n] * Y;  J[1][2] = J[1][2] + b[n] * Z;
+        J[2][0] = J[2][0] + cz[n] * X; J[2][1] = J[2][1] + cz[n] * Y; J[2][2] = J[2][2] + cz[n] * Z;
+    }
+    double cof[3][3];
+    cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    cof[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    cof[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    cof[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    cof[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
+    double Ji[3][3];
+    singular = !(fabs(det) > 0.0);
+    if (!singular) {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = cof[cc][rr] / det;
+    } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = 0.0;
+    }
+    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double ddx = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * cz[n];
+        const double ddy = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * cz[n];
+        const double ddz = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * cz[n];
+        if (dxo) {
+            dxo[n] = ddx;
+            dyo[n] = ddy;
+            dzo[n] = ddz;
+        }
+        const double u = nodes[n * 5 + 3], v = nodes[n * 5 + 4];
+        ux = ux + ddx * u; uy = uy + ddy * u; uz = uz + ddz * u;
+        vx = vx + ddx * v; vy = vy + ddy * v; vz = vz + ddz * v;
+    }
+    strain[0] = ux; strain[1] = uy; strain[2] = uz; strain[3] = vx; strain[4] = vy; strain[5] = vz;
+    if (whichefvs == 0) {
+        ev = efvs_constant;
+    } else if (whichefvs == 1) {
+        const double effstrain = (500.0 / CISM_SCYR) / 200.0e3 * CISM_SCYR;
+        ev = ff * pow(effstrain, (1.0 - 3.0) / 3.0);
+    } else {
+        const double emin = 1.0e-8;
+        double sq;
+        if (whichapprox == 0)
+            sq = emin * emin + 0.25 * (uz * uz + vz * vz);
+        else if (whichapprox == 1)
+            sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+        else
+            sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)))
+               + 0.25 * (uz * uz + vz * vz);
+        ev = ff * pow(sq, ((1.0 - 3.0) / 3.0) / 2);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__restrict__ sigma,
                                                  const int32_t *__restrict__ ac,
                                                  const double *__restrict__ xV,
@@ -99,86 +191,11 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
     // phase 1: lane (el, p) evaluates quadrature point p (velo_higher:6360-6579, 7199-7378)
     if (ok) {
         const int p = sub;
-        const double r = 1.0 / sqrt(3.0);
-        const double xq = (p == 0 || p == 3 || p == 4 || p == 7) ? -r : r;
-        const double yq = (p < 2 || p == 4 || p == 5) ? -r : r;
-        const double zq = p < 4 ? -r : r;
-        double a[8], b[8], cz[8];
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
-            const double sz = c_node_kl[n] ? -1.0 : 1.0;
-            const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
-            const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
-            const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
-            a[n] = sx * (ay * az / 8.0);
-            b[n] = sy * (ax * az / 8.0);
-            cz[n] = sz * (ax * ay / 8.0);
-        }
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const double X = L.nodes[el][n * 5 + 0], Y = L.nodes[el][n * 5 + 1], Z = L.nodes[el][n * 5 + 2];
-            J[0][0] = J[0][0] + a[n] * X;  J[0][1] = J[0][1] + a[n] * Y;  J[0][2] = J[0][2] + a[n] * Z;
-            J[1][0] = J[1][0] + b[n] * X;  J[1][1] = J[1][1] + b[n] * Y;  J[1][2] = J[1][2] + b[n] * Z;
-            J[2][0] = J[2][0] + cz[n] * X; J[2][1] = J[2][1] + cz[n] * Y; J[2][2] = J[2][2] + cz[n] * Z;
-        }
-        double cof[3][3];
-        cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        cof[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
-        cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        cof[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
-        cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-        cof[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
-        cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-        cof[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
-        cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        const double det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
-        double Ji[3][3];
-        if (fabs(det) > 0.0) {
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = cof[cc][rr] / det;
-        } else {
-            atomicOr(&sc->singular_flag, 1);
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = 0.0;
-        }
-        double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const double ddx = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * cz[n];
-            const double ddy = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * cz[n];
-            const double ddz = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * cz[n];
-            L.dx[el][p * 8 + n] = ddx;
-            L.dy[el][p * 8 + n] = ddy;
-            L.dz[el][p * 8 + n] = ddz;
-            const double u = L.nodes[el][n * 5 + 3], v = L.nodes[el][n * 5 + 4];
-            ux = ux + ddx * u; uy = uy + ddy * u; uz = uz + ddz * u;
-            vx = vx + ddx * v; vy = vy + ddy * v; vz = vz + ddz * v;
-        }
-        const double ff = flwafact[e];
-        double ev;
-        if (whichefvs == 0) {
-            ev = efvs_constant;
-        } else if (whichefvs == 1) {
-            const double effstrain = (500.0 / CISM_SCYR) / 200.0e3 * CISM_SCYR;
-            ev = ff * pow(effstrain, (1.0 - 3.0) / 3.0);
-        } else {
-            const double emin = 1.0e-8;
-            double sq;
-            if (whichapprox == 0)
-                sq = emin * emin + 0.25 * (uz * uz + vz * vz);
-            else if (whichapprox == 1)
-                sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
-            else
-                sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)))
-                   + 0.25 * (uz * uz + vz * vz);
-            ev = ff * pow(sq, ((1.0 - 3.0) / 3.0) / 2);
-        }
+        double det, ev, st[6];
+        bool singular;
+        eval_qp(p, L.nodes[el], whichapprox, whichefvs, efvs_constant, flwafact[e], &L.dx[el][p * 8],
+                &L.dy[el][p * 8], &L.dz[el][p * 8], det, ev, st, singular);
+        if (singular) atomicOr(&sc->singular_flag, 1);
         L.ev[el][p] = ev;
         L.f[el][p] = ev * 1.0 * det / CISM_VOL0;
     }
@@ -239,6 +256,192 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
             if (a != bb) Ke[keidx(72 + bb + 8 * a, ee)] = acc1;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// compute_internal_stress (velo_higher:6968-7195): the element mean over its 8
+// quadrature points of 2 efvs times the strain-rate components, and the
+// effective stress.  Same element ranges and quadrature as the assembly; SIA
+// uses only the vertical shear, SSA only the horizontal gradients.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stress(GridDev g, const double *__restrict__ sigma,
+                                                const int32_t *__restrict__ ac,
+                                                const double *__restrict__ xV,
+                                                const double *__restrict__ yV,
+                                                const double *__restrict__ uvel,
+                                                const double *__restrict__ susrf,
+                                                const double *__restrict__ sthck,
+                                                const double *__restrict__ flwafact, int whichapprox,
+                                                int whichefvs, double efvs_constant,
+                                                double *__restrict__ tau, DevScal *__restrict__ sc) {
+    __shared__ double nodes[EL_PER_WG][8 * 5 + 1];
+    __shared__ double prod[EL_PER_WG][8][5];
+    const int el = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const long long e = (long long)blockIdx.x * EL_PER_WG + el;
+    const int nzm = g.nz - 1;
+    int k = 0, ic = 0, jc = 0;
+    bool ok = e < g.ne;
+    if (ok) {
+        k = (int)(e % nzm) + 1;
+        const long long cell = e / nzm;
+        ic = (int)(cell % g.nx) + 1;
+        jc = (int)(cell / g.nx) + 1;
+        ok = ic >= CISM_NHALO + 1 && ic <= g.nx - CISM_NHALO + 1 && jc >= CISM_NHALO + 1 &&
+             jc <= g.ny - CISM_NHALO + 1 && ac[(ic - 1) + (long long)g.nx * (jc - 1)] != 0;
+    }
+    if (ok) {
+        const int n = sub;
+        const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
+        const long long v = (in - 1) + (long long)g.sx * (jn - 1);
+        const long long qn = (kn - 1) + (long long)g.nz * v;
+        nodes[el][n * 5 + 0] = xV[v];
+        nodes[el][n * 5 + 1] = yV[v];
+        nodes[el][n * 5 + 2] = susrf[v] - sigma[kn - 1] * sthck[v];
+        nodes[el][n * 5 + 3] = uvel[2 * qn];
+        nodes[el][n * 5 + 4] = uvel[2 * qn + 1];
+    }
+    __syncthreads();
+    if (ok) {
+        const int p = sub;
+        double det, ev, st[6];
+        bool singular;
+        eval_qp(p, nodes[el], whichapprox, whichefvs, efvs_constant, flwafact[e], nullptr, nullptr,
+                nullptr, det, ev, st, singular);
+        if (singular) atomicOr(&sc->singular_flag, 1);
+        double ux = st[0], uy = st[1], uz = st[2], vx = st[3], vy = st[4], vz = st[5];
+        if (whichapprox == 0) ux = uy = vx = vy = 0.0;
+        else if (whichapprox == 1) uz = vz = 0.0;
+        prod[el][p][0] = ev * uz;
+        prod[el][p][1] = ev * vz;
+        prod[el][p][2] = 2.0 * ev * ux;
+        prod[el][p][3] = 2.0 * ev * vy;
+        prod[el][p][4] = ev * (vx + uy);
+    }
+    __syncthreads();
+    if (sub == 0 && e < g.ne) {
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (ok) {
+            for (int p = 0; p < 8; ++p)
+                for (int c = 0; c < 5; ++c) t[c] = t[c] + prod[el][p][c];
+            for (int c = 0; c < 5; ++c) t[c] = t[c] / 8;
+        }
+        const double xz = t[0], yz = t[1], xx = t[2], yy = t[3], xy = t[4];
+        const long long ne = g.ne;
+        tau[e] = xz;
+        tau[ne + e] = yz;
+        tau[2 * ne + e] = xx;
+        tau[3 * ne + e] = yy;
+        tau[4 * ne + e] = xy;
+        tau[5 * ne + e] = ok ? sqrt(xx * xx + yy * yy + xx * yy + xy * xy + xz * xz + yz * yz) : 0.0;
+    }
+}
+
+int internal_stress_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant, double *tau) {
+    const GridDev &g = c->g;
+    const int nb = (int)((g.ne + EL_PER_WG - 1) / EL_PER_WG);
+    hipLaunchKernelGGL(k_stress, dim3(nb), dim3(256), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV, c->x,
+                       c->susrf, c->sthck, c->flwafact, whichapprox, whichefvs, efvs_constant, tau, c->sc);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// cell (1-based i) -> the owned cell whose value a periodic halo cell takes
+// (parallel_halo, parallel_slap.F90:1794-1860), or -1 for an outflow halo cell
+__device__ inline int cell_image(int i, int n, int periodic) {
+    if (i > CISM_NHALO && i <= n - CISM_NHALO) return i;
+    if (!periodic) return -1;
+    return i <= CISM_NHALO ? i + (n - 2 * CISM_NHALO) : i - (n - 2 * CISM_NHALO);
+}
+
+// compute_basal_friction_heatflx (velo_higher:6755-6964) on the basal layer of the
+// velocity: one lane per cell, halo cells by their periodic image
+// xb: basal velocity of vertex column col at xb[cs*col] (u) and xb[cs*col + 1] (v)
+__global__ __launch_bounds__(256) void k_bfric(GridDev g, const int32_t *__restrict__ ac,
+                                               const int32_t *__restrict__ av,
+                                               const double *__restrict__ xb, long long cs,
+                                               const double *__restrict__ beta, int standard,
+                                               double *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny) return;
+    const int i0 = (int)(t % g.nx) + 1, j0 = (int)(t / g.nx) + 1;
+    const int i = cell_image(i0, g.nx, g.periodic), j = cell_image(j0, g.ny, g.periodic);
+    double val = 0.0;
+    if (i > 0 && j > 0) {
+        // basal velocity and beta at vertex (iv, jv), 1-based
+        auto ub = [&](int iv, int jv) { return xb[cs * ((iv - 1) + (long long)g.sx * (jv - 1))]; };
+        auto vb = [&](int iv, int jv) { return xb[cs * ((iv - 1) + (long long)g.sx * (jv - 1)) + 1]; };
+        auto bt = [&](int iv, int jv) { return beta[(iv - 1) + (long long)g.sx * (jv - 1)]; };
+        if (standard) {
+            if (ac[(i - 1) + (long long)g.nx * (j - 1)]) {
+                const double r = 1.0 / sqrt(3.0);
+                const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r};
+                const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+                double u[4], v[4], b[4];
+                for (int n = 0; n < 4; ++n) {
+                    u[n] = ub(i + di[n], j + dj[n]);
+                    v[n] = vb(i + di[n], j + dj[n]);
+                    b[n] = bt(i + di[n], j + dj[n]);
+                }
+                double acc = 0.0, sw = 0.0;
+                for (int p = 0; p < 4; ++p) {
+                    double phi[4];
+                    phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+                    phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+                    phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+                    phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+                    double uq = 0.0, vq = 0.0, bq = 0.0;
+                    for (int n = 0; n < 4; ++n) {
+                        uq = uq + phi[n] * u[n];
+                        vq = vq + phi[n] * v[n];
+                        bq = bq + phi[n] * b[n];
+                    }
+                    acc = acc + 1.0 * bq * (uq * uq + vq * vq);
+                    sw = sw + 1.0;
+                }
+                val = acc / (sw * CISM_SCYR);
+            }
+        } else {
+            auto stv = [&](int iv, int jv) {
+                if (!av[(iv - 1) + (long long)g.sx * (jv - 1)]) return 0.0;
+                const double q = bt(iv, jv) * (ub(iv, jv) * ub(iv, jv) + vb(iv, jv) * vb(iv, jv));
+                return q / CISM_SCYR;
+            };
+            val = (stv(i, j) + stv(i - 1, j) + stv(i, j - 1) + stv(i - 1, j - 1)) / 4.0;
+        }
+    }
+    out[t] = val;
+}
+
+// basal traction (velo_higher:3893-3894): beta times the basal velocity, every vertex
+__global__ __launch_bounds__(256) void k_btract(GridDev g, const double *__restrict__ x,
+                                                const double *__restrict__ beta,
+                                                double *__restrict__ out) {
+    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncol = (long long)g.sx * g.sy;
+    if (col >= ncol) return;
+    const long long q = (g.nz - 1) + (long long)g.nz * col;
+    out[col] = beta[col] * x[2 * q];
+    out[ncol + col] = beta[col] * x[2 * q + 1];
+}
+
+int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric, double *btract,
+                             const double *xb, long long cs) {
+    const GridDev &g = c->g;
+    if (!xb) {
+        xb = c->x + 2 * (g.nz - 1);
+        cs = 2LL * g.nz;
+    }
+    if (bfric) {
+        hipLaunchKernelGGL(k_bfric, dim3(nblocks_for((long long)g.nx * g.ny)), dim3(256), 0, c->stream, g,
+                           c->ac, c->av, xb, cs, c->beta, whichassemble_bfric == 0, bfric);
+        HIP_TRY(hipGetLastError());
+    }
+    if (btract) {
+        hipLaunchKernelGGL(k_btract, dim3(nblocks_for((long long)g.sx * g.sy)), dim3(256), 0, c->stream, g,
+                           c->x, c->beta, btract);
+        HIP_TRY(hipGetLastError());
+    }
+    return CISM_OK;
 }
 
 // value of K_block(nr, nc) of element e from the packed scratch

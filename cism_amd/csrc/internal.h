@@ -27,6 +27,7 @@
 #define CISM_KE_ENTRIES 136    // 36 (Kuu upper) + 36 (Kvv upper) + 64 (Kuv)
 #define CISM_RED_BLOCKS 1024   // max partial-sum blocks of one reduction
 #define CISM_BLOCK 256
+#define CISM_MAX_NV 16         // widest per-block partial record of any reduction
 
 // physical constants (glimmer_physcon.F90:64-75) and the matrix volume scale
 #define CISM_RHOI 910.0
@@ -143,6 +144,7 @@ __device__ inline bool is_owned(const GridDev &g, int i, int j) {
 // ---------------------------------------------------------------------------
 template <int NV>
 __device__ inline void block_reduce_store(double (&v)[NV], double *partials) {
+    static_assert(NV <= CISM_MAX_NV, "partials hold CISM_MAX_NV values per block");
     __shared__ double red[CISM_BLOCK / 64][NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -162,6 +164,7 @@ __device__ inline void block_reduce_store(double (&v)[NV], double *partials) {
 
 template <int NV>
 __device__ inline void block_max_store(double (&v)[NV], double *partials) {
+    static_assert(NV <= CISM_MAX_NV, "partials hold CISM_MAX_NV values per block");
     __shared__ double red[CISM_BLOCK / 64][NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -228,6 +231,13 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
 int column_scale_device(cism_ctx *c, double *out);  // out: 2*ncol doubles
+// post-solve diagnostics from the context's geometry and velocity c->x
+int internal_stress_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
+                           double *tau);   // tau: 6 * ne (xz yz xx yy xy eff)
+// basal velocity from xb (u at xb[cs*col], v at xb[cs*col+1]); xb null: the bed layer of c->x
+int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric /* nx*ny */,
+                             double *btract /* 2*ncol */, const double *xb = nullptr,
+                             long long cs = 0);
 // Anderson-accelerated update of c->x (the Picard map's output) from c->usav (its
 // input); m <= AA_MAX columns; restarts the history when the fixed-point residual grows.
 #define AA_MAX 4

@@ -32,6 +32,8 @@ EXPORTED = [
     "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_load_vector_lateral_bc",
     "cism_set_precond_scale", "cism_column_scale_target",
+    "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
+    "cism_compute_basal_traction", "cism_velo_diagnostics",
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
@@ -114,6 +116,11 @@ def lib():
         L.cism_set_reference_velocity.argtypes = [VP, D, D]
         L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
         L.cism_set_precond_scale.argtypes = [VP, D, D]
+        L.cism_compute_internal_stress.argtypes = [VP, D, I32, D, D, D, D, D, D, C.c_int, C.c_double,
+                                                   C.c_int, D, D, D, D, D, D, D, D]
+        L.cism_compute_basal_friction_heatflx.argtypes = [VP, I32, I32, D, D, D, D, D, C.c_int, D]
+        L.cism_compute_basal_traction.argtypes = [VP, D, D, D, D, D]
+        L.cism_velo_diagnostics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_int] + [D] * 9
         L.cism_column_scale_target.argtypes = [VP, D, D]
         _lib = L
     return _lib
@@ -257,6 +264,45 @@ class Context:
 
     def set_reference_velocity(self, uref, vref):
         _check(lib().cism_set_reference_velocity(self.h, dp(uref), dp(vref)))
+
+    # --- post-solve diagnostics (velo_higher:3842-3894, 6755-7195) ----------------
+    def compute_internal_stress(self, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck,
+                                flwafact, efvs, whichefvs, efvs_constant, whichapprox, uvel, vvel):
+        """(tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff), each (nz-1, nx, ny)"""
+        outs = [np.zeros((self.nz - 1, self.nx, self.ny), order="F") for _ in range(6)]
+        _check(lib().cism_compute_internal_stress(
+            self.h, dp(sigma), ip(active_cell), dp(xVertex), dp(yVertex), dp(stagusrf),
+            dp(stagthck), dp(flwafact), dp(efvs), whichefvs, efvs_constant, whichapprox, dp(uvel),
+            dp(vvel), *[dp(o) for o in outs]))
+        return tuple(outs)
+
+    def compute_basal_friction_heatflx(self, active_cell, active_vertex, xVertex, yVertex, ubas,
+                                       vbas, beta, whichassemble_bfric=1):
+        out = np.zeros((self.nx, self.ny), order="F")
+        _check(lib().cism_compute_basal_friction_heatflx(
+            self.h, ip(active_cell), ip(active_vertex), dp(xVertex), dp(yVertex),
+            dp(np.asfortranarray(ubas)), dp(np.asfortranarray(vbas)), dp(beta), whichassemble_bfric,
+            dp(out)))
+        return out
+
+    def compute_basal_traction(self, beta, uvel, vvel):
+        bx = np.zeros((self.nx - 1, self.ny - 1), order="F")
+        by = np.zeros((self.nx - 1, self.ny - 1), order="F")
+        _check(lib().cism_compute_basal_traction(self.h, dp(beta), dp(uvel), dp(vvel), dp(bx), dp(by)))
+        return bx, by
+
+    def velo_diagnostics(self, whichapprox=2, whichefvs=2, efvs_constant=2336041.42829,
+                         whichassemble_bfric=1):
+        """stress components, basal friction heat flux and basal traction of the last solve"""
+        tau = [np.zeros((self.nz - 1, self.nx, self.ny), order="F") for _ in range(6)]
+        bf = np.zeros((self.nx, self.ny), order="F")
+        bx = np.zeros((self.nx - 1, self.ny - 1), order="F")
+        by = np.zeros((self.nx - 1, self.ny - 1), order="F")
+        _check(lib().cism_velo_diagnostics(self.h, whichapprox, whichefvs, efvs_constant,
+                                           whichassemble_bfric, *[dp(t) for t in tau], dp(bf),
+                                           dp(bx), dp(by)))
+        return dict(tau_xz=tau[0], tau_yz=tau[1], tau_xx=tau[2], tau_yy=tau[3], tau_xy=tau[4],
+                    tau_eff=tau[5], bfricflx=bf, btractx=bx, btracty=by)
 
     def set_precond_scale(self, su, sv):
         """Per-column scale of the mixed-precision SIA preconditioner (vertex fields); None

@@ -56,7 +56,7 @@ def crop(x, t, m, c, margin, rng):
     """Random c x c window; the mask drops `margin` pixels at its edges, where the window's
     circular padding does not match the field outside it."""
     H, W = m.shape
-    if c >= H and c >= W:
+    if not c or (c >= H and c >= W):
         return x, t, m
     y0, x0 = int(rng.integers(0, H - c + 1)), int(rng.integers(0, W - c + 1))
     sl = (slice(y0, y0 + c), slice(x0, x0 + c))
@@ -112,7 +112,9 @@ def main(argv=None):
     ap.add_argument("--uscale", type=float, default=100.0)
     ap.add_argument("--out", required=True)
     ap.add_argument("--init", default=None, help="start from these weights")
-    ap.add_argument("--scales", type=int, default=1,
+    ap.add_argument("--candidates", type=int, default=3,
+                    help="try up to this many seeds per wanted sample (skipping unconverged ones)")
+    ap.add_argument("--scales", type=int, default=0,
                     help="also learn the per-column preconditioner scales (channels 22, 23)")
     a = ap.parse_args(argv)
     t_start = time.perf_counter()
@@ -123,27 +125,36 @@ def main(argv=None):
         raise SystemExit("--crop must be a multiple of 8 no larger than the grid")
     th, tw = (c, c) if c else (nsn, ewn)
 
-    def gen(seeds, tag):
-        X, T, M = [], [], []
+    def gen(seed0, want, tag):
+        """solves seeds seed0, seed0+1, ... and keeps the first `want` whose Picard loop
+        converged (a solve that reaches the iteration cap is not a solution to learn)"""
+        X, T, M, used = [], [], [], []
         ctx = None
-        for i, s in enumerate(seeds):
+        s = seed0
+        while len(X) < want and s < seed0 + want * a.candidates:
             p = make_problem(a.family, ewn, nsn, s)
             if ctx is None:
                 ctx = G.Context.for_problem(p)
             t0 = time.perf_counter()
             r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True))
-            x = SG.features(p)
-            su, sv = ctx.column_scale_target() if a.scales else (None, None)
-            t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv)
-            X.append(x), T.append(t), M.append(m)
             umax = float(np.abs(r["uvel"]).max())
-            print(f"[train] {tag} {i + 1}/{len(seeds)} seed {s}: {r['picard_iters']} Picard, "
-                  f"{time.perf_counter() - t0:.2f} s, max|u| {umax:.0f} m/yr", flush=True)
+            ok = r["picard_iters"] < p.maxiter_nonlinear
+            print(f"[train] {tag} {len(X) + ok}/{want} seed {s}: {r['picard_iters']} Picard, "
+                  f"{time.perf_counter() - t0:.2f} s, max|u| {umax:.0f} m/yr"
+                  + ("" if ok else " -- not converged, skipped"), flush=True)
+            if ok:
+                x = SG.features(p)
+                su, sv = ctx.column_scale_target() if a.scales else (None, None)
+                t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv)
+                X.append(x), T.append(t), M.append(m), used.append(s)
+            s += 1
         ctx.close()
-        return np.stack(X), np.stack(T), np.stack(M)
+        if not X:
+            raise SystemExit(f"no converged {tag} problem among the candidate seeds")
+        return np.stack(X), np.stack(T), np.stack(M), used
 
-    Xtr, Ttr, Mtr = gen(range(a.seed0, a.seed0 + a.samples), "data")
-    Xva, Tva, Mva = gen(range(a.val_seed0, a.val_seed0 + a.val), "val")
+    Xtr, Ttr, Mtr, seeds_tr = gen(a.seed0, a.samples, "data")
+    Xva, Tva, Mva, seeds_va = gen(a.val_seed0, a.val, "val")
     print(f"[train] data ready in {time.perf_counter() - t_start:.0f} s", flush=True)
 
     net = SG.Surrogate(th, tw, a.batch)
@@ -184,7 +195,7 @@ def main(argv=None):
         print(f"[train] epoch {ep + 1}/{a.epochs}: loss {np.mean(losses):.4f}, val rel L2 {ve:.4f}, "
               f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
     meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
-            "seeds": [a.seed0, a.seed0 + a.samples - 1], "val_seeds": [a.val_seed0, a.val_seed0 + a.val - 1],
+            "seeds": seeds_tr, "val_seeds": seeds_va,
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
             "history": hist, "seconds": time.perf_counter() - t_start}
     SG.save_weights(a.out, net.get_params(), meta)

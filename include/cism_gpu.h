@@ -237,6 +237,41 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *             calibration of hardware byte counters */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
+/* ---------------------------------------------------------------------------
+ * Post-solve diagnostics.  Outputs are Fortran-layout host arrays; each entry
+ * point is synchronous.
+ * ------------------------------------------------------------------------- */
+/* compute_internal_stress (glissade_velo_higher.F90:6968-7195): element means of
+ * the stress components (Pa) at cell layers (nz-1, nx, ny).  whichapprox 0..2;
+ * efvs is accepted for the reference's dummy list (its L1L2 branch) and unused. */
+int  cism_compute_internal_stress(cism_ctx *ctx, const double *sigma, const int32_t *active_cell,
+                                  const double *xVertex, const double *yVertex,
+                                  const double *stagusrf, const double *stagthck,
+                                  const double *flwafact, const double *efvs, int whichefvs,
+                                  double efvs_constant, int whichapprox, const double *uvel,
+                                  const double *vvel, double *tau_xz, double *tau_yz,
+                                  double *tau_xx, double *tau_yy, double *tau_xy, double *tau_eff);
+
+/* compute_basal_friction_heatflx (glissade_velo_higher.F90:6755-6964): W/m^2 at
+ * cells (nx, ny), halo filled (parallel_halo).  ubas, vbas, beta: vertex fields.
+ * whichassemble_bfric 0 finite-element quadrature, 1 vertex values unstaggered. */
+int  cism_compute_basal_friction_heatflx(cism_ctx *ctx, const int32_t *active_cell,
+                                         const int32_t *active_vertex, const double *xVertex,
+                                         const double *yVertex, const double *ubas,
+                                         const double *vbas, const double *beta,
+                                         int whichassemble_bfric, double *bfricflx);
+
+/* basal traction (glissade_velo_higher.F90:3893-3894): beta * velocity at the bed */
+int  cism_compute_basal_traction(cism_ctx *ctx, const double *beta, const double *uvel,
+                                 const double *vvel, double *btractx, double *btracty);
+
+/* All three from the context's last solve (its geometry, beta and velocity stay
+ * on the device); any output pointer may be NULL. */
+int  cism_velo_diagnostics(cism_ctx *ctx, int whichapprox, int whichefvs, double efvs_constant,
+                           int whichassemble_bfric, double *tau_xz, double *tau_yz,
+                           double *tau_xx, double *tau_yy, double *tau_xy, double *tau_eff,
+                           double *bfricflx, double *btractx, double *btracty);
+
 /* Per-column preconditioner scale of the mixed-precision solves (no counterpart in
  * the reference): the SIA preconditioner becomes z = s M^-1 r with one positive
  * scale per column and component, given as vertex fields scale_u, scale_v

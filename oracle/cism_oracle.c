@@ -1063,3 +1063,179 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
     free(Auu); free(Auv); free(Avu); free(Avv); free(usav); free(vsav);
     return status;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Post-solve diagnostics                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* parallel_halo for a cell field (nx, ny), nhalo = 2 (parallel_slap.F90:1794-1860):
+ * periodic: west halo <- the last two owned columns, east halo <- the first two,
+ * then the same for full rows; outflow: halo cells zero. */
+void orc_cell_halo(const orc_grid *g, double *a)
+{
+    const int nx = g->nx, ny = g->ny, h = g->nhalo;
+    if (!g->periodic) {
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                if (i <= h || i > nx - h || j <= h || j > ny - h) C2(a, i, j) = 0.0;
+        return;
+    }
+    for (int j = h + 1; j <= ny - h; ++j)
+        for (int t = 1; t <= h; ++t) {
+            const double e = C2(a, nx - 2 * h + t, j), w = C2(a, h + t, j);
+            C2(a, t, j) = e;
+            C2(a, nx - h + t, j) = w;
+        }
+    for (int t = 1; t <= h; ++t)
+        for (int i = 1; i <= nx; ++i) {
+            const double n = C2(a, i, ny - 2 * h + t), s = C2(a, i, h + t);
+            C2(a, i, t) = n;
+            C2(a, i, ny - h + t) = s;
+        }
+}
+
+/* compute_internal_stress (velo_higher:6968-7195): element-mean stress components
+ * from the quadrature points; efvs is used for L1L2 only (whichapprox 3). */
+int orc_internal_stress(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                        const double *xV, const double *yV, const double *stagusrf,
+                        const double *stagthck, const double *flwafact, const double *efvs,
+                        int whichefvs, double efvs_constant, int whichapprox,
+                        const double *uvel, const double *vvel,
+                        double *tau_xz, double *tau_yz, double *tau_xx, double *tau_yy,
+                        double *tau_xy, double *tau_eff)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, h = g->nhalo;
+    const size_t ne = (size_t)(nz - 1) * nx * ny;
+    memset(tau_xz, 0, ne * sizeof(double));
+    memset(tau_yz, 0, ne * sizeof(double));
+    memset(tau_xx, 0, ne * sizeof(double));
+    memset(tau_yy, 0, ne * sizeof(double));
+    memset(tau_xy, 0, ne * sizeof(double));
+    memset(tau_eff, 0, ne * sizeof(double));
+    for (int j = h + 1; j <= ny - h + 1; ++j)
+        for (int i = h + 1; i <= nx - h + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                double x[8], y[8], z[8], u[8], v[8];
+                for (int n = 0; n < 8; ++n) {
+                    const int in = i - 1 + NODE_IX[n], jn = j - 1 + NODE_JY[n], kn = k + NODE_KL[n];
+                    x[n] = S2(xV, in, jn);
+                    y[n] = S2(yV, in, jn);
+                    z[n] = S2(stagusrf, in, jn) - sigma[kn - 1] * S2(stagthck, in, jn);
+                    u[n] = V3(uvel, kn, in, jn);
+                    v[n] = V3(vvel, kn, in, jn);
+                }
+                double sxz = 0, syz = 0, sxx = 0, syy = 0, sxy = 0;
+                for (int p = 0; p < 8; ++p) {
+                    double dx[8], dy[8], dz[8], detJ;
+                    const int st = orc_basis_derivatives(x, y, z, p, dx, dy, dz, &detJ);
+                    if (st) return st;
+                    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+                    if (whichapprox == 0) {
+                        for (int n = 0; n < 8; ++n) { uz = uz + dz[n] * u[n]; vz = vz + dz[n] * v[n]; }
+                    } else if (whichapprox == 1) {
+                        for (int n = 0; n < 8; ++n) {
+                            ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+                            vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+                        }
+                    } else {
+                        for (int n = 0; n < 8; ++n) {
+                            ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n]; uz = uz + dz[n] * u[n];
+                            vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n]; vz = vz + dz[n] * v[n];
+                        }
+                    }
+                    const double ev = whichapprox == 3
+                        ? F3(efvs, k, i, j)
+                        : orc_efvs_qp(whichefvs, whichapprox, efvs_constant, dx, dy, dz, u, v,
+                                      F3(flwafact, k, i, j));
+                    sxz = sxz + ev * uz;
+                    syz = syz + ev * vz;
+                    sxx = sxx + 2.0 * ev * ux;
+                    syy = syy + 2.0 * ev * vy;
+                    sxy = sxy + ev * (vx + uy);
+                }
+                sxz = sxz / 8; syz = syz / 8; sxx = sxx / 8; syy = syy / 8; sxy = sxy / 8;
+                F3(tau_xz, k, i, j) = sxz;
+                F3(tau_yz, k, i, j) = syz;
+                F3(tau_xx, k, i, j) = sxx;
+                F3(tau_yy, k, i, j) = syy;
+                F3(tau_xy, k, i, j) = sxy;
+                F3(tau_eff, k, i, j) = sqrt(sxx * sxx + syy * syy + sxx * syy + sxy * sxy
+                                            + sxz * sxz + syz * syz);
+            }
+        }
+    return ORC_OK;
+}
+
+/* compute_basal_friction_heatflx (velo_higher:6755-6964).  ubas, vbas, beta are
+ * vertex fields; out: cell field (W/m^2) with its halo filled.  whichassemble_bfric
+ * 0: 2x2-Gauss finite-element mean over each active cell; 1 (default): vertex
+ * values averaged to cell centres (glissade_unstagger, glissade_grid_operators.F90:152-231). */
+void orc_basal_friction_heatflx(const orc_grid *g, const int32_t *active_cell,
+                                const int32_t *active_vertex, const double *ubas,
+                                const double *vbas, const double *beta, int whichassemble_bfric,
+                                double *bfricflx)
+{
+    const int nx = g->nx, ny = g->ny, h = g->nhalo;
+    memset(bfricflx, 0, (size_t)nx * ny * sizeof(double));
+    if (whichassemble_bfric == 0) {
+        const double r = 1.0 / sqrt(3.0);
+        const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r};
+        /* nodes counter-clockwise from the SW vertex (i-1, j-1) */
+        const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+        for (int j = h + 1; j <= ny - h; ++j)
+            for (int i = h + 1; i <= nx - h; ++i) {
+                if (!C2(active_cell, i, j)) continue;
+                double u[4], v[4], b[4];
+                for (int n = 0; n < 4; ++n) {
+                    u[n] = S2(ubas, i + di[n], j + dj[n]);
+                    v[n] = S2(vbas, i + di[n], j + dj[n]);
+                    b[n] = S2(beta, i + di[n], j + dj[n]);
+                }
+                double acc = 0.0, sw = 0.0;
+                for (int p = 0; p < 4; ++p) {
+                    double phi[4];
+                    phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+                    phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+                    phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+                    phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+                    double uq = 0.0, vq = 0.0, bq = 0.0;
+                    for (int n = 0; n < 4; ++n) {
+                        uq = uq + phi[n] * u[n];
+                        vq = vq + phi[n] * v[n];
+                        bq = bq + phi[n] * b[n];
+                    }
+                    acc = acc + 1.0 * bq * (uq * uq + vq * vq);
+                    sw = sw + 1.0;
+                }
+                C2(bfricflx, i, j) = acc / (sw * SCYR);
+            }
+    } else {
+        double *st = (double *)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i)
+                if (S2(active_vertex, i, j)) {
+                    double q = S2(beta, i, j) * (S2(ubas, i, j) * S2(ubas, i, j)
+                                                 + S2(vbas, i, j) * S2(vbas, i, j));
+                    S2(st, i, j) = q / SCYR;
+                }
+        for (int j = 2; j <= ny - 1; ++j)
+            for (int i = 2; i <= nx - 1; ++i)
+                C2(bfricflx, i, j) = (S2(st, i, j) + S2(st, i - 1, j) + S2(st, i, j - 1)
+                                      + S2(st, i - 1, j - 1)) / 4.0;
+        free(st);
+    }
+    orc_cell_halo(g, bfricflx);
+}
+
+/* basal traction (velo_higher:3893-3894): beta * velocity at the bed, vertex fields */
+void orc_basal_traction(const orc_grid *g, const double *beta, const double *uvel,
+                        const double *vvel, double *btractx, double *btracty)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            S2(btractx, i, j) = S2(beta, i, j) * V3(uvel, nz, i, j);
+            S2(btracty, i, j) = S2(beta, i, j) * V3(vvel, nz, i, j);
+        }
+}

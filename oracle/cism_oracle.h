@@ -144,6 +144,22 @@ int  orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *active_vertex,
                           int maxiters, double tolerance,
                           double *err, int *niters);
 
+/* post-solve diagnostics (velo_higher:3842-3894, 6755-7195) */
+void orc_cell_halo(const orc_grid *g, double *a);
+int  orc_internal_stress(const orc_grid *g, const double *sigma, const int32_t *active_cell,
+                         const double *xV, const double *yV, const double *stagusrf,
+                         const double *stagthck, const double *flwafact, const double *efvs,
+                         int whichefvs, double efvs_constant, int whichapprox,
+                         const double *uvel, const double *vvel,
+                         double *tau_xz, double *tau_yz, double *tau_xx, double *tau_yy,
+                         double *tau_xy, double *tau_eff);
+void orc_basal_friction_heatflx(const orc_grid *g, const int32_t *active_cell,
+                                const int32_t *active_vertex, const double *ubas,
+                                const double *vbas, const double *beta, int whichassemble_bfric,
+                                double *bfricflx);
+void orc_basal_traction(const orc_grid *g, const double *beta, const double *uvel,
+                        const double *vvel, double *btractx, double *btracty);
+
 /* Picard loop of glissade_velo_higher_solve, 3D BP branch (velo_higher:2206-3727).
  * Inputs are the arrays the reference computes before the loop. */
 typedef struct {

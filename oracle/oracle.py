@@ -76,6 +76,11 @@ def _declare(L):
     L.orc_matvec_3d.argtypes = [G, I, D, D, D, D, D, D, D, D]
     L.orc_load_lateral_bc.argtypes = [G, D, I, I, I, D, D, D, D, D, D]
     L.orc_tridiag.argtypes = [C.c_int, D, D, D, D, D]
+    L.orc_cell_halo.argtypes = [G, D]
+    L.orc_internal_stress.argtypes = [G, D, I, D, D, D, D, D, D, C.c_int, C.c_double, C.c_int, D, D,
+                                      D, D, D, D, D, D]
+    L.orc_basal_friction_heatflx.argtypes = [G, I, I, D, D, D, C.c_int, D]
+    L.orc_basal_traction.argtypes = [G, D, D, D, D, D]
     pcg = [G, I, D, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, D, C.POINTER(C.c_int)]
     L.orc_pcg_standard_3d.argtypes = pcg
     L.orc_pcg_chrongear_3d.argtypes = pcg
@@ -294,3 +299,42 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
     return dict(status=st, uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
                 picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
                 L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)
+
+
+# --- post-solve diagnostics (velo_higher:3842-3894, 6755-7195) -----------------
+def cell_halo(p, a):
+    g = grid_of(p)
+    lib().orc_cell_halo(C.byref(g), dp(a))
+
+
+def internal_stress(p, uvel, vvel, efvs=None, whichapprox=None, whichefvs=None):
+    """(tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff), each (nz-1, nx, ny); status"""
+    g = grid_of(p)
+    efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F") if efvs is None else efvs
+    outs = [np.zeros((p.nz - 1, p.nx, p.ny), order="F") for _ in range(6)]
+    st = lib().orc_internal_stress(
+        C.byref(g), dp(p.sigma), ip(p.active_cell), dp(p.xVertex), dp(p.yVertex), dp(p.stagusrf),
+        dp(p.stagthck), dp(p.flwafact), dp(efvs),
+        p.whichefvs if whichefvs is None else whichefvs, p.efvs_constant,
+        p.whichapprox if whichapprox is None else whichapprox, dp(uvel), dp(vvel),
+        *[dp(o) for o in outs])
+    return st, outs
+
+
+def basal_friction_heatflx(p, ubas, vbas, beta=None, whichassemble_bfric=1):
+    g = grid_of(p)
+    out = np.zeros((p.nx, p.ny), order="F")
+    beta = p.beta_internal if beta is None else beta
+    lib().orc_basal_friction_heatflx(C.byref(g), ip(p.active_cell), ip(p.active_vertex),
+                                     dp(np.asfortranarray(ubas)), dp(np.asfortranarray(vbas)),
+                                     dp(beta), whichassemble_bfric, dp(out))
+    return out
+
+
+def basal_traction(p, uvel, vvel, beta=None):
+    g = grid_of(p)
+    beta = p.beta_internal if beta is None else beta
+    bx = np.zeros((p.nx - 1, p.ny - 1), order="F")
+    by = np.zeros((p.nx - 1, p.ny - 1), order="F")
+    lib().orc_basal_traction(C.byref(g), dp(beta), dp(uvel), dp(vvel), dp(bx), dp(by))
+    return bx, by
