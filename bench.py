@@ -136,8 +136,11 @@ def main(argv=None):
                                         maxiter_nonlinear=cap)
 
     if a.kernels_only:
-        # one fast solve leaves the FP32 matrix; then only the kernels under study run
-        fast()
+        # one short mixed-precision Picard iteration leaves an FP32 matrix; then only the
+        # kernels under study run
+        ctx.set_linear_solver(5, 1e-8)
+        ctx.velo_higher_solve_3d(p, lu, lv, precision=1, maxiter_nonlinear=1)
+        ctx.set_linear_solver(200, 1e-8)
         ms32 = ctx.time_kernel(1, a.steps)
         msc = ctx.time_kernel(3, a.steps)
         print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
