@@ -14,7 +14,8 @@ Timed (device time with inputs resident in HBM; the PCIe-inclusive wall time
 is reported beside it):
   headline  learned initial guess (network forward pass on the GPU) + fast
             path: FP32 matrix + FP32 Chron-Gear inner solves inside FP64
-            iterative refinement, with inexact-Picard linear tolerances
+            iterative refinement, inexact-Picard linear tolerances and
+            Anderson-accelerated Picard iterations
   beside    the fast path from a zero guess (what the network gains)
   baseline  plain FP64 path: the reference algorithm (Chron-Gear, SIA
             preconditioner, tolerance 1e-8, 200 iterations), same stopping rule
@@ -98,6 +99,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5")
     ap.add_argument("--guess", default="learned", choices=["learned", "none"])
+    ap.add_argument("--weights", default=None,
+                    help="network weights (default: the workload family's file, if trained)")
+    ap.add_argument("--anderson", type=int, default=4,
+                    help="Anderson acceleration depth of the fast path's Picard loop (0 off)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -129,7 +134,7 @@ def main(argv=None):
     def fast(u0=None, v0=None):
         return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
                                         stop_rel_err=STOP_REL_ERR, inner_rtol=1e-3,
-                                        maxiter_nonlinear=cap)
+                                        maxiter_nonlinear=cap, anderson_m=a.anderson)
 
     def plain(u0=None, v0=None, stop=STOP_REL_ERR):
         return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=0, stop_rel_err=stop,
@@ -153,7 +158,7 @@ def main(argv=None):
     ref = reference_solution(ctx, p, lu, lv, log=log, precision=ref_prec, velo_tol=1e-10)
     ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
 
-    weights = SG.default_weights(family) if family else None
+    weights = a.weights or (SG.default_weights(family) if family else None)
     use_guess = a.guess == "learned" and weights is not None and os.path.exists(weights)
     pred = SG.Predictor(p, weights) if use_guess else None
 
@@ -240,7 +245,8 @@ def main(argv=None):
                    "owned_active_nodes": nodes, "stop_rel_err": STOP_REL_ERR,
                    "learned_guess": weights if pred is not None else None,
                    "parallelism": "single GPU, whole domain resident"},
-        "fast_path": {"seconds": round(fast_s, 4), "picard_iters": stats["picard_iters"],
+        "fast_path": {"seconds": round(fast_s, 4), "anderson_m": a.anderson,
+                      "picard_iters": stats["picard_iters"],
                       "fp32_matvecs": stats["fp32_matvecs"], "fp64_matvecs": stats["fp64_matvecs"],
                       "rel_err": stats["final_rel_err"], "learned_guess": pred is not None,
                       "inference_s": round(stats["inference_s"], 4),

@@ -484,12 +484,14 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                 ev[dJ][dI][dK] = okc && ac[(icl - 1) + (long long)g.nx * (jcl - 1)] != 0;
                 eid[dJ][dI][dK] = okc ? (ke - 1) + (long long)(g.nz - 1) * ((icl - 1) + (long long)g.nx * (jcl - 1)) : 0;
             }
+    // slot by slot, the four blocks together: each slot's coefficients leave as two
+    // 16-byte stores (the tile keeps a node's four blocks of one slot adjacent)
 #pragma unroll
-    for (int blk = 0; blk < 4; ++blk) {
+    for (int m = 0; m < 27; ++m) {
+        const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int m = 0; m < 27; ++m) {
-            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-            double acc = 0.0;
+        for (int blk = 0; blk < 4; ++blk) {
 #pragma unroll
             for (int dJ = 0; dJ < 2; ++dJ)
 #pragma unroll
@@ -500,13 +502,14 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                         const int cx = ix + iA, cy = jy + jA, cl = kl + kA;
                         if (cx < 0 || cx > 1 || cy < 0 || cy > 1 || cl < 0 || cl > 1) continue;
                         if (!ev[dJ][dI][dK]) continue;
-                        acc = acc + ke_val(Ke, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
-                                           eid[dJ][dI][dK]);
+                        acc[blk] = acc[blk] + ke_val(Ke, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
+                                                     eid[dJ][dI][dK]);
                     }
             if (add_basal && m == CISM_DIAG && (blk == 0 || blk == 3) && k == g.nz && av[col])
-                acc = acc + basal_scale * beta[col];
-            A[aidx(blk, m, q)] = acc;
+                acc[blk] = acc[blk] + basal_scale * beta[col];
         }
+        *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = make_double2(acc[0], acc[1]);
+        *reinterpret_cast<double2 *>(A + aidx(2, m, q)) = make_double2(acc[2], acc[3]);
     }
 }
 
