@@ -394,26 +394,35 @@ def grounded_fraction_vertex(ice, floating):
 
 
 def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
-              beta_median=1.0e3, beta_sigma=1.0, A=1.0e-16, thklim=100.0,
-              whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA, **kw) -> Problem:
-    """Large single-GPU ice sheet: multiscale random thickness in [0, H_max] under an
-    elliptical envelope, random bed relief (sd bed_sd) on a bed that deepens toward the
-    domain edge, ice afloat where rho_i H < rho_o (-b) (a floating fringe), ice-free ocean
-    beyond; lognormal beta on grounded vertices weighted by the grounded fraction; lateral
-    ice-ocean pressure at the margin.  Floating ice not connected to grounded ice is
-    removed (an unsupported floating patch has no unique velocity)."""
+              s_max=3300.0, shelf=0.12, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-16,
+              thklim=100.0, whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA,
+              **kw) -> Problem:
+    """Large single-GPU ice sheet on an elliptical footprint (normalised radius r = 1 at
+    0.42 of the domain).  Grounded part: a Vialov-type surface s = s_max (1 - r^(4/3))^(3/8)
+    with a long-wavelength (150 km) random perturbation, over a bed that deepens toward
+    the edge (400 - 1400 r^2 m) and carries random relief at 50 km and 15 km (sd bed_sd),
+    so the thickness H = s - b (clipped to [0, H_max]) is multiscale and random while the
+    surface slopes stay glaciological.  Floating fringe: ice is afloat wherever
+    rho_i H < rho_o (-b), with surface H (1 - rho_i / rho_o); beyond the grounded margin a
+    shelf of width `shelf` (in r) thins linearly from 500 m to 150 m where the sea is deep
+    enough to float it.  Ice-free ocean beyond; lognormal beta on grounded vertices
+    weighted by the grounded fraction; lateral ice-ocean pressure at the margin.  Floating
+    ice not connected to grounded ice is removed (an unsupported floating patch has no
+    unique velocity)."""
     from scipy import ndimage
     rng = np.random.default_rng(seed)
     Lx, Ly = ewn * dx, nsn * dx
     x = (np.arange(ewn) + 0.5) * dx - Lx / 2
     y = (np.arange(nsn) + 0.5) * dx - Ly / 2
     rr = np.sqrt((x[:, None] / (0.42 * Lx)) ** 2 + (y[None, :] / (0.42 * Ly)) ** 2)
-    env = np.sqrt(np.clip(1.0 - rr ** 2, 0.0, None))
-    g = (0.6 * gaussian_random_field(ewn, nsn, dx, 150e3, rng)
-         + 0.3 * gaussian_random_field(ewn, nsn, dx, 40e3, rng)
-         + 0.1 * gaussian_random_field(ewn, nsn, dx, 10e3, rng))
-    H = np.clip(H_max * env * (0.75 + 0.25 * g), 0.0, H_max)
-    bed = 400.0 - 1400.0 * rr ** 2 + bed_sd * gaussian_random_field(ewn, nsn, dx, 50e3, rng)
+    prof = np.clip(1.0 - np.clip(rr, 0.0, 1.0) ** (4.0 / 3.0), 0.0, None) ** (3.0 / 8.0)
+    s = s_max * prof * (1.0 + 0.08 * gaussian_random_field(ewn, nsn, dx, 150e3, rng))
+    bed = (400.0 - 1400.0 * rr ** 2
+           + bed_sd * (0.8 * gaussian_random_field(ewn, nsn, dx, 50e3, rng)
+                       + 0.6 * gaussian_random_field(ewn, nsn, dx, 15e3, rng)))
+    H = np.where(rr < 1.0, np.clip(s - bed, 0.0, H_max), 0.0)
+    H_shelf = np.where((rr >= 1.0) & (rr < 1.0 + shelf), 500.0 - 350.0 * (rr - 1.0) / shelf, 0.0)
+    H = np.where((H <= thklim) & (RHOI * H_shelf < RHOO * (-bed)), H_shelf, H)
     fl = RHOI * H < RHOO * (-bed)
     grounded = (H > thklim) & ~fl
     lab, _ = ndimage.label(H > thklim)

@@ -33,7 +33,7 @@ EXPORTED = [
     "cism_load_vector_lateral_bc",
     "cism_set_precond_scale", "cism_column_scale_target",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
-    "cism_compute_basal_traction", "cism_velo_diagnostics",
+    "cism_compute_basal_traction", "cism_velo_diagnostics", "cism_glissade_velo_inputs",
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
@@ -121,6 +121,7 @@ def lib():
         L.cism_compute_basal_friction_heatflx.argtypes = [VP, I32, I32, D, D, D, D, D, C.c_int, D]
         L.cism_compute_basal_traction.argtypes = [VP, D, D, D, D, D]
         L.cism_velo_diagnostics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_int] + [D] * 9
+        L.cism_glissade_velo_inputs.argtypes = [VP] + [C.c_double] * 5 + [D] * 4 + [I32] * 6 + [D] * 7
         L.cism_column_scale_target.argtypes = [VP, D, D]
         _lib = L
     return _lib
@@ -264,6 +265,26 @@ class Context:
 
     def set_reference_velocity(self, uref, vref):
         _check(lib().cism_set_reference_velocity(self.h, dp(uref), dp(vref)))
+
+    # --- the solve's inputs from the geometry (velo_higher:1449-1939) -------------
+    def glissade_velo_inputs(self, dx, dy, thklim, eus, thck, usrf, topg, flwa, max_slope=1.0):
+        """dict of masks, vertex geometry, staggered fields, surface gradient, flwafact"""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        ci = lambda: np.zeros((nx, ny), dtype=np.int32, order="F")
+        vi = lambda: np.zeros((nx - 1, ny - 1), dtype=np.int32, order="F")
+        vd = lambda: np.zeros((nx - 1, ny - 1), order="F")
+        o = dict(ice_mask=ci(), floating_mask=ci(), ocean_mask=ci(), land_mask=ci(),
+                 active_cell=ci(), active_vertex=vi(), xVertex=vd(), yVertex=vd(), stagusrf=vd(),
+                 stagthck=vd(), dusrf_dx=vd(), dusrf_dy=vd(),
+                 flwafact=np.zeros((nz - 1, nx, ny), order="F"))
+        f = lambda a: np.asfortranarray(a, dtype=np.float64)
+        _check(lib().cism_glissade_velo_inputs(
+            self.h, dx, dy, thklim, eus, max_slope, dp(f(thck)), dp(f(usrf)), dp(f(topg)), dp(f(flwa)),
+            *[ip(o[k]) for k in ("ice_mask", "floating_mask", "ocean_mask", "land_mask",
+                                 "active_cell", "active_vertex")],
+            *[dp(o[k]) for k in ("xVertex", "yVertex", "stagusrf", "stagthck", "dusrf_dx",
+                                 "dusrf_dy", "flwafact")]))
+        return o
 
     # --- post-solve diagnostics (velo_higher:3842-3894, 6755-7195) ----------------
     def compute_internal_stress(self, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck,

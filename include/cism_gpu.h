@@ -238,6 +238,27 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
 /* ---------------------------------------------------------------------------
+ * The solve's inputs computed on the device from the ice geometry (the set-up of
+ * glissade_velo_higher_solve, glissade_velo_higher.F90:1449-1939):
+ *   masks            glissade_get_masks without a subgrid calving front
+ *                    (glissade_masks.F90:130-210)
+ *   stagthck/usrf    glissade_stagger, stagger_margin_in = 1, over ice-or-land cells
+ *                    (glissade_grid_operators.F90:118-146)
+ *   dusrf_dx/dy      glissade_surface_elevation_gradient, centered, hybrid margin,
+ *                    |slope| <= max_slope, staggered halo (glissade_grid_operators.F90:830-990)
+ *   vertex geometry  get_vertex_geometry (glissade_velo_higher.F90:4163-4313)
+ *   flwafact         0.5 A^(-1/3) (glissade_velo_higher.F90:1930-1939)
+ * Inputs: cell fields thck, usrf, topg (nx, ny), flwa (nz-1, nx, ny).  The results stay
+ * in the context; every output pointer may be NULL (not copied). */
+int  cism_glissade_velo_inputs(cism_ctx *ctx, double dx, double dy, double thklim, double eus,
+                               double max_slope, const double *thck, const double *usrf,
+                               const double *topg, const double *flwa, int32_t *ice_mask,
+                               int32_t *floating_mask, int32_t *ocean_mask, int32_t *land_mask,
+                               int32_t *active_cell, int32_t *active_vertex, double *xVertex,
+                               double *yVertex, double *stagusrf, double *stagthck,
+                               double *dusrf_dx, double *dusrf_dy, double *flwafact);
+
+/* ---------------------------------------------------------------------------
  * Post-solve diagnostics.  Outputs are Fortran-layout host arrays; each entry
  * point is synchronous.
  * ------------------------------------------------------------------------- */

@@ -281,4 +281,6 @@ def test_lateral_bc_closed_form():
     assert east + west > 0 and north + south > 0
     assert lu.sum() == pytest.approx((east - west) * p_av * H * dy / 1e9, rel=1e-10, abs=1e-6)
     assert lv.sum() == pytest.approx((north - south) * p_av * H * dy / 1e9, rel=1e-10, abs=1e-6)
-    assert np.abs(lu).sum() == pytest.approx((east + west) * p_av * H * dy / 1e9, rel=1e-10)
+    # faces on opposite sides of a one-cell-wide strip load the same vertices with
+    # opposite signs, so the absolute sum is bounded by (not equal to) the face count
+    assert 0 < np.abs(lu).sum() <= (east + west) * p_av * H * dy / 1e9 * (1 + 1e-12)
