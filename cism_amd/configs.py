@@ -394,15 +394,16 @@ def grounded_fraction_vertex(ice, floating):
 
 
 def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
-              s_max=3300.0, shelf=0.12, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-16,
+              s_max=3300.0, shelf=0.12, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-17,
               thklim=100.0, whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA,
               **kw) -> Problem:
     """Large single-GPU ice sheet on an elliptical footprint (normalised radius r = 1 at
     0.42 of the domain).  Grounded part: a Vialov-type surface s = s_max (1 - r^(4/3))^(3/8)
     with a long-wavelength (150 km) random perturbation, over a bed that deepens toward
     the edge (400 - 1400 r^2 m) and carries random relief at 50 km and 15 km (sd bed_sd),
-    so the thickness H = s - b (clipped to [0, H_max]) is multiscale and random while the
-    surface slopes stay glaciological.  Floating fringe: ice is afloat wherever
+    so the thickness H = s - b is multiscale and random in [0, H_max] (troughs deeper
+    than s - H_max are filled) while the surface stays smooth.  Flow factor A = 1e-17
+    Pa^-3 yr^-1 (cold ice), which keeps a 3 km thick sheet's speeds glaciological.  Floating fringe: ice is afloat wherever
     rho_i H < rho_o (-b), with surface H (1 - rho_i / rho_o); beyond the grounded margin a
     shelf of width `shelf` (in r) thins linearly from 500 m to 150 m where the sea is deep
     enough to float it.  Ice-free ocean beyond; lognormal beta on grounded vertices
@@ -420,10 +421,20 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd
     bed = (400.0 - 1400.0 * rr ** 2
            + bed_sd * (0.8 * gaussian_random_field(ewn, nsn, dx, 50e3, rng)
                        + 0.6 * gaussian_random_field(ewn, nsn, dx, 15e3, rng)))
-    H = np.where(rr < 1.0, np.clip(s - bed, 0.0, H_max), 0.0)
+    # thickness where the smooth surface meets the rough bed; where it would exceed H_max
+    # the bed is raised (troughs filled), so the surface stays smooth
+    bed = np.where(rr < 1.0, np.maximum(bed, s - H_max), bed)
+    H = np.where(rr < 1.0, np.clip(s - bed, 0.0, None), 0.0)
     H_shelf = np.where((rr >= 1.0) & (rr < 1.0 + shelf), 500.0 - 350.0 * (rr - 1.0) / shelf, 0.0)
     H = np.where((H <= thklim) & (RHOI * H_shelf < RHOO * (-bed)), H_shelf, H)
     fl = RHOI * H < RHOO * (-bed)
+    # trim floating filaments: a floating cell needs two ice neighbours (four-connected);
+    # a one-cell-wide floating finger has almost no lateral support
+    for _ in range(4):
+        ice = H > thklim
+        nb = np.zeros(ice.shape, int)
+        nb[1:] += ice[:-1]; nb[:-1] += ice[1:]; nb[:, 1:] += ice[:, :-1]; nb[:, :-1] += ice[:, 1:]
+        H = np.where(ice & fl & (nb < 2), 0.0, H)
     grounded = (H > thklim) & ~fl
     lab, _ = ndimage.label(H > thklim)
     keep = np.unique(lab[grounded])
