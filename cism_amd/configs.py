@@ -382,6 +382,23 @@ def slab(ewn=40, nsn=40, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, efvs=2336041
                    HO_BABC_NO_SLIP, whichefvs=HO_EFVS_CONSTANT, efvs_constant=efvs, **kw)
 
 
+def sliding_slab(ewn=24, nsn=20, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, beta=500.0,
+                 A=1.0e-16, **kw) -> Problem:
+    """Inclined slab of uniform thickness sliding over a uniform-beta bed (babc 5), periodic:
+    every horizontal gradient of the solution vanishes, so the SSA solution is the basal
+    balance u = rho_i g H tan(theta) / beta, v = 0."""
+    nx, ny = ewn + 2 * NHALO, nsn + 2 * NHALO
+    ig = np.arange(nx) - NHALO
+    x = (ig + 0.5) * dx
+    th = math.radians(theta_deg)
+    thck = np.full((nx, ny), H, order="F")
+    usrf = np.asfortranarray(np.broadcast_to(-x[:, None] * math.tan(th), (nx, ny)).copy())
+    topg = np.asfortranarray(usrf - thck)
+    bv = np.full((nx - 1, ny - 1), beta, order="F")
+    return _finish("sliding_slab", ewn, nsn, nz, dx, dx, thck, usrf, topg, 1.0, A,
+                   HO_BABC_BETA_EXTERNAL, beta_vertex=bv, **kw)
+
+
 def grounded_fraction_vertex(ice, floating):
     """Fraction of the ice cells around each vertex that are grounded (0 where no ice); a
     flotation-criterion stand-in for glissade_grounded_fraction without a grounding-line

@@ -9,6 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 static thread_local char g_err[512] = "";
 
 void cism_set_error(const char *fmt, ...) {
@@ -170,8 +172,8 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
                                cism_ctx **out) {
     if (!out) return CISM_ERR_ARG;
     *out = nullptr;
-    if (nhalo != CISM_NHALO || nx < 7 || ny < 7 || nz < 2) {
-        cism_set_error("invalid grid nx=%d ny=%d nz=%d nhalo=%d (need nhalo=2, nx,ny>=7, nz>=2)", nx,
+    if (nhalo != CISM_NHALO || nx < 7 || ny < 7 || nz < 1) {
+        cism_set_error("invalid grid nx=%d ny=%d nz=%d nhalo=%d (need nhalo=2, nx,ny>=7, nz>=1)", nx,
                        ny, nz, nhalo);
         return CISM_ERR_ARG;
     }
@@ -230,7 +232,8 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals,
-                    c->cscale, c->aa, c->aa_sums};
+                    c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
+                    c->ssa_flw3};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -356,6 +359,10 @@ extern "C" int cism_assemble_stiffness_matrix_3d(cism_ctx *c, const double *sigm
                                                  double efvs_constant, double *Auu, double *Auv,
                                                  double *Avu, double *Avv) {
     CTX_BEGIN(c);
+    if (c->g.nz < 2) {
+        cism_set_error("assemble_stiffness_matrix_3d needs a 3D context (nz >= 2)");
+        return CISM_ERR_ARG;
+    }
     CISM_TRY(check_options(whichapprox, whichefvs));
     CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
     CISM_TRY(upload_vec(c, c->x, uvel, vvel));
@@ -575,35 +582,12 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, const double *sigma,
-                                         const int32_t *active_cell, const int32_t *active_vertex,
-                                         const double *xVertex, const double *yVertex,
-                                         const double *stagusrf, const double *stagthck,
-                                         const double *flwafact, const double *beta_internal,
-                                         const int32_t *umask_dirichlet,
-                                         const int32_t *vmask_dirichlet, const double *loadu,
-                                         const double *loadv, double *uvel, double *vvel,
-                                         double *efvs, double *bu, double *bv, double *resid_u,
-                                         double *resid_v, cism_solve_stats *stats) {
-    CTX_BEGIN(c);
-    if (!o) return CISM_ERR_ARG;
-    CISM_TRY(check_options(o->whichapprox, o->whichefvs));
-    if (o->whichsparse != 2 && o->whichsparse != 3) {
-        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
-        return CISM_ERR_UNSUPPORTED;
-    }
-    const double t0 = now_s();
+// The Picard loop on the device (velo_higher:2206-3727), inputs already resident: the
+// 3D Blatter-Pattyn system, or with dim2 the SSA system of a one-level context.
+static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double t0, double *uvel,
+                         double *vvel, double *efvs, double *bu, double *bv, double *resid_u,
+                         double *resid_v, cism_solve_stats *stats) {
     const GridDev &g = c->g;
-    const size_t ncol = (size_t)g.sx * g.sy;
-    CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
-    CISM_TRY(up(c, c->av, active_vertex, ncol));
-    CISM_TRY(up(c, c->beta, beta_internal, ncol));
-    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)g.nn));
-    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)g.nn));
-    CISM_TRY(upload_vec(c, c->load, loadu, loadv));
-    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
-    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
-
     if (o->anderson_m < 0 || o->anderson_m > AA_MAX) {
         cism_set_error("anderson_m must be 0..%d", AA_MAX);
         return CISM_ERR_ARG;
@@ -623,8 +607,11 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
         HIP_TRY(hipMemcpyAsync(c->usav, c->x, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                c->stream));
         double ta = now_s();
-        CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
-                                 o->whichbabc != 6, o->dx, o->dy));
+        if (dim2)
+            CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, o->whichbabc != 6, o->dx, o->dy));
+        else
+            CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                     o->whichbabc != 6, o->dx, o->dy));
         HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                c->stream));
         CISM_TRY(dirichlet_device(c));
@@ -701,7 +688,12 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     HIP_TRY(hipStreamSynchronize(c->stream));
     S.seconds_device = now_s() - t_dev0;
     CISM_TRY(download_vec(c, c->x, uvel, vvel));
-    if (efvs) CISM_TRY(down(c, efvs, c->efvs, (size_t)g.ne));
+    if (efvs && dim2) {
+        CISM_TRY(ssa_efvs_layers_device(c, c->ssa_flw3));
+        CISM_TRY(down(c, efvs, c->ssa_flw3, (size_t)(c->ssa_nz3 - 1) * g.nx * g.ny));
+    } else if (efvs) {
+        CISM_TRY(down(c, efvs, c->efvs, (size_t)g.ne));
+    }
     if (bu && bv) CISM_TRY(download_vec(c, c->b, bu, bv));
     if (resid_u && resid_v) CISM_TRY(download_vec(c, c->t, resid_u, resid_v));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -721,6 +713,110 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     S.seconds_total = now_s() - t0;
     S.status = st;
     if (stats) *stats = S;
+    return CISM_OK;
+}
+
+extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, const double *sigma,
+                                         const int32_t *active_cell, const int32_t *active_vertex,
+                                         const double *xVertex, const double *yVertex,
+                                         const double *stagusrf, const double *stagthck,
+                                         const double *flwafact, const double *beta_internal,
+                                         const int32_t *umask_dirichlet,
+                                         const int32_t *vmask_dirichlet, const double *loadu,
+                                         const double *loadv, double *uvel, double *vvel,
+                                         double *efvs, double *bu, double *bv, double *resid_u,
+                                         double *resid_v, cism_solve_stats *stats) {
+    CTX_BEGIN(c);
+    if (!o) return CISM_ERR_ARG;
+    CISM_TRY(check_options(o->whichapprox, o->whichefvs));
+    if (o->whichsparse != 2 && o->whichsparse != 3) {
+        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const double t0 = now_s();
+    const GridDev &g = c->g;
+    if (g.nz < 2) {
+        cism_set_error("the 3D solve needs nz >= 2 (a one-level context runs cism_velo_higher_solve_2d)");
+        return CISM_ERR_ARG;
+    }
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta_internal, ncol));
+    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)g.nn));
+    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)g.nn));
+    CISM_TRY(upload_vec(c, c->load, loadu, loadv));
+    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    return picard_device(c, o, 0, t0, uvel, vvel, efvs, bu, bv, resid_u, resid_v, stats);
+}
+
+// The SSA branch of glissade_velo_higher_solve (solve_2d, velo_higher:2206-3767) on a
+// one-level context: the 2D velocity starts from the bed level of uvel/vvel, the 2D
+// load is the column sum of the 3D load (velo_higher:2123-2137), the Dirichlet masks are
+// the bed level's, and every level of uvel/vvel takes the 2D solution at the end.
+extern "C" int cism_velo_higher_solve_2d(cism_ctx *c, const cism_solve_opts *o, int nz3, const double *sigma,
+                                         const int32_t *active_cell, const int32_t *active_vertex,
+                                         const double *xVertex, const double *yVertex,
+                                         const double *stagusrf, const double *stagthck,
+                                         const double *flwafact, const double *beta_internal,
+                                         const int32_t *umask_dirichlet,
+                                         const int32_t *vmask_dirichlet, const double *loadu,
+                                         const double *loadv, double *uvel, double *vvel,
+                                         double *efvs, double *bu, double *bv, double *resid_u,
+                                         double *resid_v, cism_solve_stats *stats) {
+    CTX_BEGIN(c);
+    (void)stagusrf;
+    const GridDev &g = c->g;
+    if (!o || g.nz != 1 || nz3 < 2 || nz3 > 64) {
+        cism_set_error("velo_higher_solve_2d: needs a one-level context (nz = 1) and 2 <= nz3 <= 64");
+        return CISM_ERR_ARG;
+    }
+    if (o->whichapprox != 1 || o->whichefvs < 0 || o->whichefvs > 2) {
+        cism_set_error("velo_higher_solve_2d runs the SSA approximation (whichapprox = 1)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    if (o->whichsparse != 2 && o->whichsparse != 3) {
+        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const double t0 = now_s();
+    const size_t ncol = (size_t)g.sx * g.sy, ncell = (size_t)g.nx * g.ny;
+    // host marshalling between the 3D arrays and the one-level system
+    std::vector<double> u2(ncol), v2(ncol), lu2(ncol, 0.0), lv2(ncol, 0.0);
+    std::vector<int32_t> um2(ncol), vm2(ncol);
+    for (size_t col = 0; col < ncol; ++col) {
+        const size_t b = (size_t)(nz3 - 1) + (size_t)nz3 * col;
+        u2[col] = uvel[b];
+        v2[col] = vvel[b];
+        um2[col] = umask_dirichlet[b];
+        vm2[col] = vmask_dirichlet[b];
+        for (int k = 0; k < nz3; ++k) {
+            lu2[col] = lu2[col] + loadu[k + (size_t)nz3 * col];
+            lv2[col] = lv2[col] + loadv[k + (size_t)nz3 * col];
+        }
+    }
+    CISM_TRY(ensure_ssa(c, nz3));
+    CISM_TRY(up(c, c->ssa_sigma, sigma, (size_t)nz3));
+    CISM_TRY(up(c, c->ssa_flw3, flwafact, (size_t)(nz3 - 1) * ncell));
+    CISM_TRY(up(c, c->ac, active_cell, ncell));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(up(c, c->sthck, stagthck, ncol));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta_internal, ncol));
+    CISM_TRY(up(c, c->umask, um2.data(), ncol));
+    CISM_TRY(up(c, c->vmask, vm2.data(), ncol));
+    CISM_TRY(upload_vec(c, c->load, lu2.data(), lv2.data()));
+    CISM_TRY(upload_vec(c, c->x, u2.data(), v2.data()));
+    CISM_TRY(ssa_prepare_device(c));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    CISM_TRY(picard_device(c, o, 1, t0, u2.data(), v2.data(), efvs, bu, bv, resid_u, resid_v, stats));
+    for (size_t col = 0; col < ncol; ++col)
+        for (int k = 0; k < nz3; ++k) {
+            uvel[k + (size_t)nz3 * col] = u2[col];
+            vvel[k + (size_t)nz3 * col] = v2[col];
+        }
     return CISM_OK;
 }
 
@@ -809,6 +905,10 @@ extern "C" int cism_compute_internal_stress(cism_ctx *c, const double *sigma, co
                                             double *tau_xx, double *tau_yy, double *tau_xy,
                                             double *tau_eff) {
     CTX_BEGIN(c);
+    if (c->g.nz < 2) {
+        cism_set_error("compute_internal_stress needs a 3D context (nz >= 2)");
+        return CISM_ERR_ARG;
+    }
     (void)efvs;   // used by the reference for L1L2 only, which this path does not run
     CISM_TRY(check_options(whichapprox, whichefvs));
     CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
@@ -868,6 +968,10 @@ extern "C" int cism_velo_diagnostics(cism_ctx *c, int whichapprox, int whichefvs
     const GridDev &g = c->g;
     CISM_TRY(ensure_ke(c));
     if (tau_xz || tau_yz || tau_xx || tau_yy || tau_xy || tau_eff) {
+        if (g.nz < 2) {
+            cism_set_error("the stress components need a 3D context (nz >= 2)");
+            return CISM_ERR_ARG;
+        }
         CISM_TRY(check_options(whichapprox, whichefvs));
         HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
         CISM_TRY(internal_stress_device(c, whichapprox, whichefvs, efvs_constant, c->Ke));

@@ -74,6 +74,10 @@ struct cism_ctx {
     // optional per-column preconditioner scale of the mixed-precision solves (2*ncol)
     float *cscale;
     int cscale_on;
+    // one-level (SSA) contexts: element scratch, column-mean flow factor, cell efvs,
+    // sigma and the 3D flow factor of the caller's nz3 levels
+    double *ssa_Ke, *ssa_ff2, *ssa_efvs2, *ssa_sigma, *ssa_flw3;
+    int ssa_nz3;
     // Anderson acceleration of the Picard map: m columns of dX and dF, previous x and f
     double *aa, *aa_sums;
     int aa_m, aa_count, aa_head;
@@ -231,6 +235,11 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
 int column_scale_device(cism_ctx *c, double *out);  // out: 2*ncol doubles
+// SSA (one-level context, ssa.hip)
+int ensure_ssa(cism_ctx *c, int nz3);
+int ssa_prepare_device(cism_ctx *c);
+int assemble2d_device(cism_ctx *c, int whichefvs, double efvs_constant, int add_basal, double dx, double dy);
+int ssa_efvs_layers_device(cism_ctx *c, double *efvs3);
 // post-solve diagnostics from the context's geometry and velocity c->x
 int internal_stress_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                            double *tau);   // tau: 6 * ne (xz yz xx yy xy eff)

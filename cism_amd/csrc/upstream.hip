@@ -152,6 +152,10 @@ extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, doub
     }
     HIP_TRY(hipSetDevice(c->device));
     const GridDev &g = c->g;
+    if (g.nz < 2) {
+        cism_set_error("glissade_velo_inputs needs a 3D context (nz >= 2)");
+        return CISM_ERR_ARG;
+    }
     const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
     // device staging: cell fields in the element scratch (allocated by the first assembly
     // or here), bits in the node-mask buffer of u (nn >= ncell ints)

@@ -29,7 +29,7 @@ EXPORTED = [
     "cism_dirichlet_boundary_conditions_3d", "cism_check_symmetry_assembled_matrix_3d",
     "cism_count_nonzeros_3d", "cism_compute_residual_vector_3d", "cism_compute_residual_velocity_3d",
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
-    "cism_velo_higher_solve_3d", "cism_set_reference_velocity", "cism_time_kernel",
+    "cism_velo_higher_solve_3d", "cism_velo_higher_solve_2d", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_load_vector_lateral_bc",
     "cism_set_precond_scale", "cism_column_scale_target",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
@@ -113,6 +113,9 @@ def lib():
         L.cism_velo_higher_solve_3d.argtypes = [VP, C.POINTER(SolveOpts), D, I32, I32, D, D, D, D,
                                                 D, D, I32, I32, D, D, D, D, D, D, D, D, D,
                                                 C.POINTER(SolveStats)]
+        L.cism_velo_higher_solve_2d.argtypes = [VP, C.POINTER(SolveOpts), C.c_int, D, I32, I32, D, D,
+                                                D, D, D, D, I32, I32, D, D, D, D, D, D, D, D, D,
+                                                C.POINTER(SolveStats)]
         L.cism_set_reference_velocity.argtypes = [VP, D, D]
         L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
         L.cism_set_precond_scale.argtypes = [VP, D, D]
@@ -170,6 +173,11 @@ class Context:
     @classmethod
     def for_problem(cls, p, device=0):
         return cls(p.nx, p.ny, p.nz, p.periodic, device)
+
+    @classmethod
+    def for_problem_2d(cls, p, device=0):
+        """a one-level context for the SSA solve of problem p"""
+        return cls(p.nx, p.ny, 1, p.periodic, device)
 
     def close(self):
         if self.h:
@@ -405,6 +413,45 @@ class Context:
         st = SolveStats()
         _check(lib().cism_velo_higher_solve_3d(
             self.h, C.byref(o), dp(p.sigma), ip(p.active_cell), ip(p.active_vertex),
+            dp(p.xVertex), dp(p.yVertex), dp(p.stagusrf), dp(p.stagthck), dp(p.flwafact),
+            dp(p.beta_internal), ip(p.umask_dirichlet), ip(p.vmask_dirichlet), dp(loadu),
+            dp(loadv), dp(u), dp(v), dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv), C.byref(st)))
+        return dict(uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
+                    **st.as_dict())
+
+    def velo_higher_solve_2d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
+                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
+                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
+                             adaptive_linear=0, adaptive_gamma=0.05, anderson_m=0):
+        """The SSA branch (whichapprox 1) on a one-level context: 3D inputs of p in, the
+        2D solution broadcast to every level of uvel/vvel out; bu/bv/resid are 2D."""
+        if self.nz != 1:
+            raise ValueError("velo_higher_solve_2d needs a one-level context (Context.for_problem_2d)")
+        o = SolveOpts()
+        lib().cism_solve_opts_default(C.byref(o))
+        o.whichbabc = p.whichbabc
+        o.whichefvs = p.whichefvs
+        o.whichresid = p.whichresid if whichresid is None else whichresid
+        o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+        o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+        o.whichapprox = 1
+        o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter_nonlinear is None else maxiter_nonlinear
+        o.precision = precision
+        o.efvs_constant = p.efvs_constant
+        o.dx, o.dy = p.dx, p.dy
+        o.target_override = target_override
+        o.inner_rtol = inner_rtol
+        o.adaptive_linear = adaptive_linear
+        o.adaptive_gamma = adaptive_gamma
+        o.anderson_m = anderson_m
+        u = (p.uvel if uvel is None else uvel).copy(order="F")
+        v = (p.vvel if vvel is None else vvel).copy(order="F")
+        s2 = (p.nx - 1, p.ny - 1)
+        efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+        bu, bv, ru, rv = (np.zeros(s2, order="F") for _ in range(4))
+        st = SolveStats()
+        _check(lib().cism_velo_higher_solve_2d(
+            self.h, C.byref(o), p.nz, dp(p.sigma), ip(p.active_cell), ip(p.active_vertex),
             dp(p.xVertex), dp(p.yVertex), dp(p.stagusrf), dp(p.stagthck), dp(p.flwafact),
             dp(p.beta_internal), ip(p.umask_dirichlet), ip(p.vmask_dirichlet), dp(loadu),
             dp(loadv), dp(u), dp(v), dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv), C.byref(st)))

@@ -237,6 +237,25 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *             calibration of hardware byte counters */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
+/* The SSA branch of glissade_velo_higher_solve (solve_2d with which_ho_approx = 1,
+ * glissade_velo_higher.F90:2206-3767, assemble_stiffness_matrix_2d :5208-5595, and
+ * pcg_solver_*_2d, glissade_velo_higher_pcg.F90:553-961, 1614-2146) on a one-level
+ * context (cism_ctx_create with nz = 1).  The arrays are the 3D solve's, with nz3
+ * levels: the 2D velocity starts from the bed level of uvel/vvel and is written to
+ * every level; the 2D load is the column sum of loadu/loadv; the Dirichlet masks are
+ * the bed level's; flwafact (nz3-1, nx, ny) is averaged over the column; efvs
+ * (nz3-1, nx, ny) out.  bu, bv, resid_u, resid_v are one-level (nx-1, ny-1). */
+int  cism_velo_higher_solve_2d(cism_ctx *ctx, const cism_solve_opts *opts, int nz3,
+                               const double *sigma, const int32_t *active_cell,
+                               const int32_t *active_vertex, const double *xVertex,
+                               const double *yVertex, const double *stagusrf,
+                               const double *stagthck, const double *flwafact,
+                               const double *beta_internal, const int32_t *umask_dirichlet,
+                               const int32_t *vmask_dirichlet, const double *loadu,
+                               const double *loadv, double *uvel, double *vvel, double *efvs,
+                               double *bu, double *bv, double *resid_u, double *resid_v,
+                               cism_solve_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * The solve's inputs computed on the device from the ice geometry (the set-up of
  * glissade_velo_higher_solve, glissade_velo_higher.F90:1449-1939):

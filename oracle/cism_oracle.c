@@ -1239,3 +1239,244 @@ void orc_basal_traction(const orc_grid *g, const double *beta, const double *uve
             S2(btracty, i, j) = S2(beta, i, j) * V3(vvel, nz, i, j);
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* 2D (SSA) solve.  The 2D system lives in a one-level grid (nz = 1): its     */
+/* nine couplings (iA, jA) occupy the 27-slot layout's kA = 0 slots          */
+/* IDX_A(iA, jA, 0) (indxA_2d(iA,jA) + 9), so the 3D Dirichlet, halo,          */
+/* symmetry, nnz, residual and PCG routines above are exactly the 2D ones      */
+/* (pcg:553-961, 1614-2146, 2651-2755; velo_higher:8582-8764, 8931-9107).      */
+/* ------------------------------------------------------------------------ */
+
+/* glissade_vertical_average without a mask (glissade_grid_operators.F90:1172-1236) */
+void orc_vertical_average(int nx, int ny, int nz, const double *sigma, const double *var,
+                          double *var_2d)
+{
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            double s = 0.0;
+            for (int k = 1; k <= nz - 1; ++k)
+                s = s + F3(var, k, i, j) * (sigma[k] - sigma[k - 1]);
+            C2(var_2d, i, j) = s;
+        }
+}
+
+/* assemble_stiffness_matrix_2d, SSA branch (velo_higher:5208-5595) with
+ * get_basis_function_derivatives_2d (:6583-6751), compute_effective_viscosity on
+ * the four element nodes (:7199-7378), compute_element_matrix SSA (:7868-7986) and
+ * element_to_global_matrix_2d (:8058-8121).  g: the one-level grid; nz3, sigma,
+ * flwafact (nz3-1, nx, ny) and efvs (nz3-1, nx, ny, out) belong to the 3D grid. */
+int orc_assemble_2d_ssa(const orc_grid *g, int nz3, const double *sigma,
+                        const int32_t *active_cell, const double *xV, const double *yV,
+                        const double *uvel, const double *vvel, const double *stagthck,
+                        const double *flwafact, int whichefvs, double efvs_constant,
+                        double *efvs, double *Auu, double *Auv, double *Avu, double *Avv)
+{
+    const int nx = g->nx, ny = g->ny, h = g->nhalo;
+    const int nz = 1;   /* for the V3 / A4 accessors of the one-level system */
+    const size_t na = 27 * (size_t)(nx - 1) * (ny - 1);
+    memset(Auu, 0, na * 8); memset(Auv, 0, na * 8); memset(Avu, 0, na * 8); memset(Avv, 0, na * 8);
+    memset(efvs, 0, (size_t)(nz3 - 1) * nx * ny * 8);
+    double *ff2 = (double *)malloc((size_t)nx * ny * 8);
+    orc_vertical_average(nx, ny, nz3, sigma, flwafact, ff2);
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r};
+    /* lower-face nodes SW, SE, NE, NW relative to the element's NE vertex (ishift(3,n)) */
+    const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+    const double p_eff = (1.0 - (double)GN) / (double)GN, p2 = p_eff / 2, emin = 1.0e-8;
+    int status = ORC_OK;
+    for (int j = h + 1; j <= ny - h + 1 && !status; ++j)
+        for (int i = h + 1; i <= nx - h + 1 && !status; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            double x[4], y[4], u[4], v[4], hn[4];
+            for (int n = 0; n < 4; ++n) {
+                x[n] = S2(xV, i + di[n], j + dj[n]);
+                y[n] = S2(yV, i + di[n], j + dj[n]);
+                u[n] = S2(uvel, i + di[n], j + dj[n]);
+                v[n] = S2(vvel, i + di[n], j + dj[n]);
+                hn[n] = S2(stagthck, i + di[n], j + dj[n]);
+            }
+            double Kuu[16] = {0}, Kuv[16] = {0}, Kvu[16] = {0}, Kvv[16] = {0}, evq[4];
+            for (int p = 0; p < 4; ++p) {
+                double dxr[4], dyr[4], phi[4];
+                dxr[0] = -(1.0 - yq[p]) / 4.0; dxr[1] = (1.0 - yq[p]) / 4.0;
+                dxr[2] = (1.0 + yq[p]) / 4.0;  dxr[3] = -(1.0 + yq[p]) / 4.0;
+                dyr[0] = -(1.0 - xq[p]) / 4.0; dyr[1] = -(1.0 + xq[p]) / 4.0;
+                dyr[2] = (1.0 + xq[p]) / 4.0;  dyr[3] = (1.0 - xq[p]) / 4.0;
+                phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+                phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+                phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+                phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+                double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+                for (int n = 0; n < 4; ++n) {
+                    J11 = J11 + dxr[n] * x[n];
+                    J12 = J12 + dxr[n] * y[n];
+                    J21 = J21 + dyr[n] * x[n];
+                    J22 = J22 + dyr[n] * y[n];
+                }
+                const double det = J11 * J22 - J12 * J21;
+                if (!(fabs(det) > 0.0)) { status = ORC_ERR_SINGULAR; break; }
+                const double i11 = J22 / det, i12 = -J12 / det, i21 = -J21 / det, i22 = J11 / det;
+                double dx[4], dy[4];
+                for (int n = 0; n < 4; ++n) {
+                    dx[n] = i11 * dxr[n] + i12 * dyr[n];
+                    dy[n] = i21 * dxr[n] + i22 * dyr[n];
+                }
+                double ev;
+                const double ff = C2(ff2, i, j);
+                if (whichefvs == 0) {
+                    ev = efvs_constant;
+                } else if (whichefvs == 1) {
+                    ev = ff * pow(VEL_SCALE / LEN_SCALE * SCYR, p_eff);
+                } else {
+                    double ux = 0, uy = 0, vx = 0, vy = 0;
+                    for (int n = 0; n < 4; ++n) {
+                        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+                        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+                    }
+                    const double sq = emin * emin + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+                    ev = ff * pow(sq, p2);
+                }
+                evq[p] = ev;
+                double hq = 0.0;
+                for (int n = 0; n < 4; ++n) hq = hq + phi[n] * hn[n];
+                const double f = hq * ev * 1.0 * det / VOL0;
+                for (int nc = 0; nc < 4; ++nc)
+                    for (int nr = 0; nr < 4; ++nr) {
+                        const int e = nr + 4 * nc;
+                        Kuu[e] = Kuu[e] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc]);
+                        Kuv[e] = Kuv[e] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                        Kvu[e] = Kvu[e] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                        Kvv[e] = Kvv[e] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc]);
+                    }
+            }
+            if (status) break;
+            for (int k = 1; k <= nz3 - 1; ++k) {
+                double s = 0.0;
+                for (int p = 0; p < 4; ++p) s = s + evq[p];
+                efvs[(k - 1) + (size_t)(nz3 - 1) * ((i - 1) + (size_t)nx * (j - 1))] = s / 4;
+            }
+            for (int a = 0; a < 4 && !status; ++a)
+                for (int b = 0; b < 4; ++b)
+                    if (fabs(Kuu[a + 4 * b] - Kuu[b + 4 * a]) > 1.0e-10 ||
+                        fabs(Kvv[a + 4 * b] - Kvv[b + 4 * a]) > 1.0e-10 ||
+                        fabs(Kuv[a + 4 * b] - Kvu[b + 4 * a]) > 1.0e-10) {
+                        status = ORC_ERR_ASYMMETRIC;
+                        break;
+                    }
+            for (int nr = 0; nr < 4; ++nr) {
+                const int iv = i + di[nr], jv = j + dj[nr];
+                for (int nc = 0; nc < 4; ++nc) {
+                    const int iA = di[nc] - di[nr], jA = dj[nc] - dj[nr];
+                    const int m = IDX_A(iA, jA, 0);
+                    const int e = nr + 4 * nc;
+                    A4(Auu, m, 1, iv, jv) = A4(Auu, m, 1, iv, jv) + Kuu[e];
+                    A4(Auv, m, 1, iv, jv) = A4(Auv, m, 1, iv, jv) + Kuv[e];
+                    A4(Avu, m, 1, iv, jv) = A4(Avu, m, 1, iv, jv) + Kvu[e];
+                    A4(Avv, m, 1, iv, jv) = A4(Avv, m, 1, iv, jv) + Kvv[e];
+                }
+            }
+        }
+    free(ff2);
+    return status;
+}
+
+/* The SSA branch of the Picard loop (velo_higher:2206-3767 with solve_2d): the 2D
+ * velocity starts from the bed level, the 2D load is the column sum of the 3D load,
+ * the Dirichlet masks are the bed level's, and every level of the 3D velocity takes
+ * the 2D solution at the end.  g: the 3D grid; outputs bu, bv, resid_u, resid_v are
+ * one-level ((nx-1) x (ny-1)). */
+int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
+                        const double *sigma, const int32_t *active_cell,
+                        const int32_t *active_vertex,
+                        const double *xV, const double *yV,
+                        const double *stagusrf, const double *stagthck,
+                        const double *flwafact, const double *beta_internal,
+                        const int32_t *umask, const int32_t *vmask,
+                        const double *loadu, const double *loadv,
+                        double *uvel, double *vvel, double *efvs,
+                        double *bu, double *bv, double *resid_u, double *resid_v)
+{
+    (void)stagusrf;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    orc_grid g2;
+    orc_grid_init(&g2, nx, ny, 1, g->periodic);
+    const size_t nv = (size_t)(nx - 1) * (ny - 1), na = 27 * nv;
+    double *Auu = malloc(na * 8), *Auv = malloc(na * 8), *Avu = malloc(na * 8), *Avv = malloc(na * 8);
+    double *u2 = malloc(nv * 8), *v2 = malloc(nv * 8), *us = malloc(nv * 8), *vs = malloc(nv * 8);
+    double *l2u = calloc(nv, 8), *l2v = calloc(nv, 8);
+    int32_t *um2 = malloc(nv * 4), *vm2 = malloc(nv * 4);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const size_t c = (i - 1) + (size_t)(nx - 1) * (j - 1);
+            u2[c] = V3(uvel, nz, i, j);
+            v2[c] = V3(vvel, nz, i, j);
+            um2[c] = V3(umask, nz, i, j);
+            vm2[c] = V3(vmask, nz, i, j);
+            for (int k = 1; k <= nz; ++k) {
+                l2u[c] = l2u[c] + V3(loadu, k, i, j);
+                l2v[c] = l2v[c] + V3(loadv, k, i, j);
+            }
+        }
+    const double L2_target = 1.0e-4, L2_target_rel = 1.0e-7, resid_target = 1.0e-4;
+    double crit = 1.0e10, target = 1.0e-12;
+    double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
+    int counter = 0, status = ORC_OK;
+    o->total_linear_iters = 0;
+    while (crit >= target && counter < o->maxiter_nonlinear) {
+        counter++;
+        memcpy(us, u2, nv * 8); memcpy(vs, v2, nv * 8);
+        status = orc_assemble_2d_ssa(&g2, nz, sigma, active_cell, xV, yV, u2, v2, stagthck, flwafact,
+                                     o->whichefvs, o->efvs_constant, efvs, Auu, Auv, Avu, Avv);
+        if (status) break;
+        if (o->whichbabc != 6)
+            orc_basal_sliding_local(&g2, o->dx, o->dy, active_vertex, beta_internal, Auu, Avv);
+        memcpy(bu, l2u, nv * 8); memcpy(bv, l2v, nv * 8);
+        orc_dirichlet_3d(&g2, active_vertex, um2, vm2, u2, v2, Auu, Auv, Avu, Avv, bu, bv);
+        orc_halo(&g2, 27, Auu); orc_halo(&g2, 27, Auv); orc_halo(&g2, 27, Avu); orc_halo(&g2, 27, Avv);
+        orc_halo(&g2, 1, bu); orc_halo(&g2, 1, bv);
+        orc_check_symmetry_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, NULL);
+        long nnz = orc_count_nonzeros_3d(&g2, active_vertex, Auu, Auv, Avu, Avv);
+        if (nnz == 0) {
+            memset(resid_u, 0, nv * 8); memset(resid_v, 0, nv * 8);
+            memset(bu, 0, nv * 8); memset(bv, 0, nv * 8);
+            memset(u2, 0, nv * 8); memset(v2, 0, nv * 8);
+            L2 = 0.0;
+            break;
+        }
+        orc_residual_vector_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2, resid_u, resid_v,
+                               &L2, &L2rel);
+        double err; int niters;
+        if (o->whichsparse == 3)
+            status = orc_pcg_chrongear_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                          o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                          &err, &niters);
+        else
+            status = orc_pcg_standard_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                         o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                         &err, &niters);
+        if (status) break;
+        o->total_linear_iters += niters;
+        orc_halo(&g2, 1, u2); orc_halo(&g2, 1, v2);
+        /* compute_residual_velocity_2d: the 2D field has no separate bed level */
+        resid_velo = orc_residual_velocity_3d(&g2, o->whichresid == 1 ? 0 : o->whichresid, u2, v2, us, vs);
+        if (o->whichresid == 3) { crit = L2; target = L2_target; }
+        else if (o->whichresid == 4) { crit = L2rel; target = L2_target_rel; }
+        else { crit = resid_velo; target = resid_target; }
+    }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const size_t c = (i - 1) + (size_t)(nx - 1) * (j - 1);
+            for (int k = 1; k <= nz; ++k) {
+                V3(uvel, k, i, j) = u2[c];
+                V3(vvel, k, i, j) = v2[c];
+            }
+        }
+    o->picard_iters = counter;
+    o->final_L2_norm = L2;
+    o->final_resid_velo = resid_velo;
+    o->status = status;
+    free(Auu); free(Auv); free(Avu); free(Avv); free(u2); free(v2); free(us); free(vs);
+    free(l2u); free(l2v); free(um2); free(vm2);
+    return status;
+}

@@ -182,6 +182,26 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
                      double *uvel, double *vvel, double *efvs,
                      double *bu, double *bv, double *resid_u, double *resid_v);
 
+/* 2D (SSA) solve on a one-level grid: the nine 2D couplings sit in the 27-slot
+ * layout's kA = 0 slots, so the 3D routines above serve the 2D system unchanged */
+void orc_vertical_average(int nx, int ny, int nz, const double *sigma, const double *var,
+                          double *var_2d);
+int  orc_assemble_2d_ssa(const orc_grid *g, int nz3, const double *sigma,
+                         const int32_t *active_cell, const double *xV, const double *yV,
+                         const double *uvel, const double *vvel, const double *stagthck,
+                         const double *flwafact, int whichefvs, double efvs_constant,
+                         double *efvs, double *Auu, double *Auv, double *Avu, double *Avv);
+int  orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
+                         const double *sigma, const int32_t *active_cell,
+                         const int32_t *active_vertex,
+                         const double *xV, const double *yV,
+                         const double *stagusrf, const double *stagthck,
+                         const double *flwafact, const double *beta_internal,
+                         const int32_t *umask, const int32_t *vmask,
+                         const double *loadu, const double *loadv,
+                         double *uvel, double *vvel, double *efvs,
+                         double *bu, double *bv, double *resid_u, double *resid_v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,11 @@ def _declare(L):
     L.orc_load_lateral_bc.argtypes = [G, D, I, I, I, D, D, D, D, D, D]
     L.orc_tridiag.argtypes = [C.c_int, D, D, D, D, D]
     L.orc_cell_halo.argtypes = [G, D]
+    L.orc_vertical_average.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D]
+    L.orc_assemble_2d_ssa.argtypes = [G, C.c_int, D, I, D, D, D, D, D, D, C.c_int, C.c_double, D,
+                                      D, D, D, D]
+    L.orc_picard_solve_2d.argtypes = [G, C.POINTER(PicardOpts), D, I, I, D, D, D, D, D, D, I, I,
+                                      D, D, D, D, D, D, D, D, D]
     L.orc_internal_stress.argtypes = [G, D, I, D, D, D, D, D, D, C.c_int, C.c_double, C.c_int, D, D,
                                       D, D, D, D, D, D]
     L.orc_basal_friction_heatflx.argtypes = [G, I, I, D, D, D, C.c_int, D]
@@ -296,6 +301,59 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
                                 dp(p.beta_internal), ip(p.umask_dirichlet),
                                 ip(p.vmask_dirichlet), dp(loadu), dp(loadv), dp(u), dp(v),
                                 dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv))
+    return dict(status=st, uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
+                picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
+                L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)
+
+
+# --- 2D (SSA) solve on a one-level grid ------------------------------------------
+def grid2d_of(p):
+    return grid(p.nx, p.ny, 1, p.periodic)
+
+
+def mat2d_zeros(p):
+    return [np.zeros((27, 1, p.nx - 1, p.ny - 1), order="F") for _ in range(4)]
+
+
+def assemble_2d(p, u2, v2, whichefvs=None):
+    """SSA matrix on the one-level grid (27, 1, nx-1, ny-1) (kA = 0 slots) and efvs"""
+    g = grid2d_of(p)
+    A = mat2d_zeros(p)
+    efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+    st = lib().orc_assemble_2d_ssa(C.byref(g), p.nz, dp(p.sigma), ip(p.active_cell),
+                                   dp(p.xVertex), dp(p.yVertex), dp(np.asfortranarray(u2)),
+                                   dp(np.asfortranarray(v2)), dp(p.stagthck), dp(p.flwafact),
+                                   p.whichefvs if whichefvs is None else whichefvs,
+                                   p.efvs_constant, dp(efvs), *[dp(a) for a in A])
+    return st, A, efvs
+
+
+def picard_2d(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
+              whichsparse=None, whichprecond=None, whichresid=None):
+    """The SSA branch of the Picard loop from p.uvel/p.vvel (copied); 3D velocity out"""
+    g = grid_of(p)
+    o = PicardOpts()
+    o.whichbabc = p.whichbabc
+    o.whichefvs = p.whichefvs
+    o.whichresid = p.whichresid if whichresid is None else whichresid
+    o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+    o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+    o.whichapprox = 1
+    o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter is None else maxiter
+    o.linear_maxiters = linear_maxiters
+    o.efvs_constant = p.efvs_constant
+    o.linear_tolerance = linear_tolerance
+    o.dx, o.dy = p.dx, p.dy
+    u, v = p.uvel.copy(order="F"), p.vvel.copy(order="F")
+    efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+    s2 = (p.nx - 1, p.ny - 1)
+    bu, bv, ru, rv = (np.zeros(s2, order="F") for _ in range(4))
+    st = lib().orc_picard_solve_2d(C.byref(g), C.byref(o), dp(p.sigma), ip(p.active_cell),
+                                   ip(p.active_vertex), dp(p.xVertex), dp(p.yVertex),
+                                   dp(p.stagusrf), dp(p.stagthck), dp(p.flwafact),
+                                   dp(p.beta_internal), ip(p.umask_dirichlet),
+                                   ip(p.vmask_dirichlet), dp(loadu), dp(loadv), dp(u), dp(v),
+                                   dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv))
     return dict(status=st, uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
                 picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
                 L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)
