@@ -1,6 +1,6 @@
 """The 2D (SSA, whichapprox 1) solve on the GPU against the CPU oracle's SSA Picard loop
-(glissade_velo_higher.F90:2123-2137 set-up, 2880-3260 loop, assemble_stiffness_matrix_2d
-5400-5640): same Picard count, velocity within the FP64 parity bar, efvs and the final
+(glissade_velo_higher.F90:2206-3767 with solve_2d, assemble_stiffness_matrix_2d
+5208-5595): same Picard count, velocity within the FP64 parity bar, efvs and the final
 residual; the sliding slab against its closed form; mixed precision within 1e-6."""
 import math
 
