@@ -51,7 +51,7 @@ struct DevScal {
     double sums[8];
     int nan_flag, singular_flag;
     unsigned int sym_count;
-    int pad;
+    int zero_stop;   // inner solves: an exactly zero residual ends the iteration (alpha = 0)
 };
 
 struct cism_ctx {
@@ -115,6 +115,15 @@ __host__ __device__ inline long long aidx(int b, int m, long long q) {
 }
 
 // the four blocks (uu, uv, vu, vv) of slot m of node q: one 16-byte load (FP32) or two (FP64)
+// FP32 symmetric storage of the mixed-precision solve: 14 slots per node, the
+// diagonal block (slot CISM_DIAG) and the 13 couplings after it in indxA order,
+// in tiles of 64 nodes like aidx.  A coupling before the diagonal, slot m of node
+// q, is the transpose of slot 26 - m of the neighbour it points to.
+#define CISM_SYM_SLOTS 14
+__host__ __device__ inline long long sidx(int b, int s, long long q) {
+    return (((q >> 6) * CISM_SYM_SLOTS + s) << 8) + ((q & 63) << 2) + b;
+}
+
 __device__ inline void load_blocks(const float *__restrict__ A, long long q, int m, float &uu,
                                    float &uv, float &vu, float &vv) {
     const float4 a = *reinterpret_cast<const float4 *>(A + aidx(0, m, q));

@@ -59,11 +59,21 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_reduce(const double *__restrict_
     case RED_CG_FIRST:
         sc->rho_old = s0;
         sc->sigma = s1;
+        if (sc->zero_stop && s0 == 0.0) {
+            sc->alpha = 0.0;
+            break;
+        }
         sc->alpha = sc->rho_old / sc->sigma;
         if (sc->alpha != sc->alpha) sc->nan_flag = 1;
         break;
     case RED_CG_ITER: {
         const double rho = s0, delta = s1;
+        if (sc->zero_stop && (rho == 0.0 || sc->rho_old == 0.0)) {
+            sc->alpha = 0.0;
+            sc->beta = 0.0;
+            sc->rho_old = rho;
+            break;
+        }
         sc->beta = rho / sc->rho_old;
         sc->sigma = delta - sc->beta * rho / sc->alpha;
         sc->alpha = rho / sc->sigma;
@@ -98,9 +108,45 @@ template <> struct Vec2<float> { using t = float2; };
 template <> struct Vec2<double> { using t = double2; };
 
 
+// Matrix accessors.  blocks(q, m, qn): the four blocks of slot m of row q, whose
+// neighbour is qn; entry(g, b, m, q): one block of one slot.
 template <typename T>
-__device__ inline void row_product(const GridDev &g, const T *__restrict__ A,
-                                   const T *__restrict__ x, long long q, int k, T &yu, T &yv) {
+struct MatFull {   // all 27 slots stored (aidx)
+    const T *A;
+    __device__ inline void blocks(long long q, int m, long long qn, T &uu, T &uv, T &vu, T &vv) const {
+        (void)qn;
+        load_blocks(A, q, m, uu, uv, vu, vv);
+    }
+    __device__ inline T entry(const GridDev &g, int b, int m, long long q) const {
+        (void)g;
+        return A[aidx(b, m, q)];
+    }
+};
+
+struct MatSym32 {   // FP32 symmetric storage (sidx)
+    const float *A;
+    __device__ inline void blocks(long long q, int m, long long qn, float &uu, float &uv, float &vu,
+                                  float &vv) const {
+        if (m >= CISM_DIAG) {
+            const float4 a = *reinterpret_cast<const float4 *>(A + sidx(0, m - CISM_DIAG, q));
+            uu = a.x; uv = a.y; vu = a.z; vv = a.w;
+        } else {
+            const float4 a = *reinterpret_cast<const float4 *>(A + sidx(0, CISM_DIAG - m, qn));
+            uu = a.x; uv = a.z; vu = a.y; vv = a.w;
+        }
+    }
+    __device__ inline float entry(const GridDev &g, int b, int m, long long q) const {
+        if (m >= CISM_DIAG) return A[sidx(b, m - CISM_DIAG, q)];
+        const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+        const long long qn = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+        const int bt = b == 1 ? 2 : (b == 2 ? 1 : b);
+        return A[sidx(bt, CISM_DIAG - m, qn)];
+    }
+};
+
+template <typename T, typename M>
+__device__ inline void row_product(const GridDev &g, const M &A, const T *__restrict__ x, long long q,
+                                   int k, T &yu, T &yv) {
     using V2 = typename Vec2<T>::t;
     const long long cst = g.nz, rst = (long long)g.nz * g.sx;
     T su = yu, sv = yv;
@@ -115,7 +161,7 @@ __device__ inline void row_product(const GridDev &g, const T *__restrict__ A,
                 const long long qn = q + kA + iA * cst + jA * rst;
                 const V2 xx = *reinterpret_cast<const V2 *>(x + 2 * qn);
                 T uu, uv, vu, vv;
-                load_blocks(A, q, m, uu, uv, vu, vv);
+                A.blocks(q, m, qn, uu, uv, vu, vv);
                 su = su + uu * xx.x + uv * xx.y;
                 sv = sv + vu * xx.x + vv * xx.y;
             }
@@ -144,7 +190,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec(GridDev g, const T *__res
     const long long col = q / g.nz;
     const int i = (int)(col % g.sx), j = (int)(col / g.sx);
     T yu = 0, yv = 0;
-    if (is_owned(g, i, j) && av[col]) row_product<T>(g, A, x, q, k, yu, yv);
+    if (is_owned(g, i, j) && av[col]) row_product<T>(g, MatFull<T>{A}, x, q, k, yu, yv);
     store2<T>(y, q, yu, yv);
 }
 
@@ -160,8 +206,8 @@ int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y) {
 //   NDOT = 2: s0 = (r, x), s1 = (y, x)  [Chron-Gear (r, z), (Az, z)]
 // One lane per node, one pass over the grid; block partials go to a second
 // deterministic pass (k_reduce).
-template <typename T, int NDOT>
-__global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const T *__restrict__ A,
+template <typename T, int NDOT, typename M>
+__global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const M A,
                                                             const int32_t *__restrict__ av,
                                                             const T *__restrict__ x, T *__restrict__ y,
                                                             const T *__restrict__ r,
@@ -196,8 +242,8 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const T *
 // r = b - A x on the full grid (A x = 0 off owned active rows), s0 = (r, r) over
 // owned nodes.  halo_r: the halo takes the periodic image of r afterwards
 // (Chron-Gear, pcg:1592-1595), computed here from the image row directly.
-template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_residual(GridDev g, const T *__restrict__ A,
+template <typename T, typename M>
+__global__ __launch_bounds__(CISM_BLOCK) void k_residual(GridDev g, const M A,
                                                          const int32_t *__restrict__ av,
                                                          const T *__restrict__ x,
                                                          const T *__restrict__ b, T *__restrict__ r,
@@ -251,13 +297,13 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_dot(GridDev g, const T *__restri
 // branches pcg:300-365 / 1416-1454; easy_sia_solver + tridiag_solver
 // pcg:2759-2904).  Coefficients are read straight from the matrix slots.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_precond_diag(GridDev g, const T *__restrict__ A,
+template <typename T, typename M>
+__global__ __launch_bounds__(CISM_BLOCK) void k_precond_diag(GridDev g, const M A,
                                                              const T *__restrict__ r,
                                                              T *__restrict__ z) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nn) return;
-    const T du = A[aidx(0, CISM_DIAG, q)], dv = A[aidx(3, CISM_DIAG, q)];
+    const T du = A.entry(g, 0, CISM_DIAG, q), dv = A.entry(g, 3, CISM_DIAG, q);
     store2<T>(z, q, du != (T)0 ? r[2 * q] / du : (T)0, dv != (T)0 ? r[2 * q + 1] / dv : (T)0);
 }
 
@@ -282,8 +328,8 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_copy(long long n, const T *__res
 // An optional per-column scale s (cs[comp*ncol + col], s > 0) turns the
 // preconditioner into z = s M^-1 r by storing sb/s and beta/s (gamma is a ratio
 // and unchanged): still symmetric positive definite, and free in the apply.
-template <typename T>
-__global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const T *__restrict__ A,
+template <typename T, typename M>
+__global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
                                                           const int32_t *__restrict__ av,
                                                           const float *__restrict__ cs,
                                                           T *__restrict__ F) {
@@ -297,14 +343,14 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const T *__
     const long long qb = col * nz;
     auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + col; };
     const T inv = cs ? (T)(1.0f / cs[t]) : (T)1;
-    T beta = A[aidx(blk, CISM_DIAG, qb)];
+    T beta = A.entry(g, blk, CISM_DIAG, qb);
     F[Fi(0, 0)] = (T)0;
     F[Fi(1, 0)] = cs ? beta * inv : beta;
     F[Fi(2, 0)] = (T)0;
     for (int k = 1; k < nz; ++k) {
-        const T gam = A[aidx(blk, CISM_DIAG + 9, qb + k - 1)] / beta;   // super(k-1) / beta(k-1)
-        const T sb = A[aidx(blk, CISM_DIAG - 9, qb + k)];
-        beta = A[aidx(blk, CISM_DIAG, qb + k)] - sb * gam;
+        const T gam = A.entry(g, blk, CISM_DIAG + 9, qb + k - 1) / beta;   // super(k-1) / beta(k-1)
+        const T sb = A.entry(g, blk, CISM_DIAG - 9, qb + k);
+        beta = A.entry(g, blk, CISM_DIAG, qb + k) - sb * gam;
         F[Fi(0, k)] = cs ? sb * inv : sb;
         F[Fi(1, k)] = cs ? beta * inv : beta;
         F[Fi(2, k)] = gam;
@@ -347,23 +393,23 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
         if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
 }
 
-template <typename T>
-static int launch_sia_setup(cism_ctx *c, const T *A, const float *cs, T *F) {
+template <typename T, typename M>
+static int launch_sia_setup(cism_ctx *c, const M A, const float *cs, T *F) {
     const long long ncol = (long long)c->g.sx * c->g.sy;
-    hipLaunchKernelGGL(k_sia_setup<T>, dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, c->g,
+    hipLaunchKernelGGL((k_sia_setup<T, M>), dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, c->g,
                        A, c->av, cs, F);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
 
-template <typename T>
-static int launch_precond(cism_ctx *c, int precond, const T *A, const T *F, const T *r, T *z) {
+template <typename T, typename M>
+static int launch_precond(cism_ctx *c, int precond, const M A, const T *F, const T *r, T *z) {
     const GridDev &g = c->g;
     if (precond == 0) {
         hipLaunchKernelGGL(k_copy<T>, dim3(nblocks_for(2 * g.nn)), dim3(CISM_BLOCK), 0, c->stream,
                            2 * g.nn, r, z);
     } else if (precond == 1) {
-        hipLaunchKernelGGL((k_precond_diag<T>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,
+        hipLaunchKernelGGL((k_precond_diag<T, M>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,
                            c->stream, g, A, r, z);
     } else {
         const long long ncol = (long long)g.sx * g.sy;
@@ -467,22 +513,29 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long n2, const T *__
 // Chron-Gear (pcg:1517-1536) with the halo update of Az (pcg:1494-1497) folded
 // in: halo entries read Az at their periodic image (or 0, outflow).
 template <typename T>
+//
+// skip_av (diagonal or SIA preconditioner only): owned nodes of inactive columns are
+// left alone.  There z = 0 (zeroed at the start, never written by those
+// preconditioners), A z = 0 and d = q = 0, so their update would rewrite d = q = 0
+// and leave x and r unchanged: skipping it changes no value and saves the traffic.
 __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__restrict__ z,
                                                           const T *__restrict__ Az,
                                                           T *__restrict__ d, T *__restrict__ qv,
                                                           T *__restrict__ x, T *__restrict__ r,
-                                                          const DevScal *__restrict__ sc, int first) {
+                                                          const DevScal *__restrict__ sc, int first,
+                                                          const int32_t *__restrict__ skip_av) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * g.nn) return;
     const long long q = t >> 1;
     const int c = (int)(t & 1);
-    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
     T az;
     {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        if (is_owned(g, i, j)) {
+        const bool own = is_owned(g, i, j);
+        if (own && skip_av && !skip_av[col]) return;
+        if (own) {
             az = Az[t];
         } else {
             const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
@@ -490,6 +543,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
                                     : Az[2 * (k + (long long)g.nz * (ii + (long long)g.sx * jj)) + c];
         }
     }
+    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
     T dd, qq;
     if (first) {   // first pass: d = z, q = A d (already halo-mapped), pcg:1351-1405
         dd = z[t];
@@ -507,11 +561,12 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
 // ---------------------------------------------------------------------------
 // PCG drivers
 // ---------------------------------------------------------------------------
-__global__ void k_init_scalars(DevScal *sc) {
+__global__ void k_init_scalars(DevScal *sc, int zero_stop) {
     sc->rho_old = sc->alpha = sc->beta = sc->sigma = sc->eta1 = sc->eta2 = 0.0;
     sc->eta0 = 1.0;
     sc->err = 0.0;
     sc->nan_flag = 0;
+    sc->zero_stop = zero_stop;
 }
 
 int sync_scalars(cism_ctx *c) {
@@ -542,14 +597,16 @@ struct CGVecs {
 // reference rule, pcg:478-537, 1542-1597).  check = 1 (inner solves of the
 // mixed-precision refinement): the recursive residual r is measured instead,
 // with no extra matvec, against tol * |r0|.
-template <typename T>
-static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int precond, int maxiters,
+template <typename T, typename M>
+static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precond, int maxiters,
                   double tol, int check, double *err_out, int *niters_out) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
     const long long np = g.npad;
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc);
-    if (precond == 2) CISM_TRY(launch_sia_setup<T>(c, A, v.cs, v.F));
+    // an inner solve (check = 1) whose residual becomes exactly zero has converged; the
+    // reference recurrences would divide 0 by 0 there
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc, check);
+    if (precond == 2) CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
                        v.d, v.z, v.q);
     // (b, b) -> L2_rhs
@@ -559,7 +616,7 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
     double err = 0.0;
     auto check_conv = [&](int n, bool &done) -> int {
         if (check == 0) {
-            hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
+            hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);
             CISM_TRY(launch_reduce(c, nbm, 1, RED_ERR));
         } else {
@@ -581,17 +638,17 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
     if (!chrongear) {
         // pcg:253-304: halo x, r = b - A x (no halo on r)
         CISM_TRY((launch_halo_t<T, false>(c, v.x)));
-        hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
+        hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
                            v.x, v.b, v.r, 0, c->partials);
         for (int n = 1; n <= maxiters && !done; ++n) {
-            CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
+            CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
             hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.r, v.z,
                                c->partials);
             CISM_TRY(launch_reduce(c, nb, 1, RED_STD_ETA1));
             hipLaunchKernelGGL(k_std_d<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.z,
                                v.d, c->sc);
             CISM_TRY((launch_halo_t<T, false>(c, v.d)));
-            hipLaunchKernelGGL((k_matvec_dots<T, 1>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g,
+            hipLaunchKernelGGL((k_matvec_dots<T, 1, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g,
                                A, c->av, v.d, v.q, (const T *)nullptr, c->partials);
             CISM_TRY(launch_reduce(c, nbm, 1, RED_STD_ETA2));
             hipLaunchKernelGGL(k_std_xr<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.d,
@@ -604,21 +661,21 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
     } else {
         // pcg:1258-1405: halo x; r = b - A x; halo r; z = M^-1 r; d = z; q = A d
         CISM_TRY((launch_halo_t<T, false>(c, v.x)));
-        hipLaunchKernelGGL((k_residual<T>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
+        hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
                            v.x, v.b, v.r, 1, c->partials);
-        CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
-        hipLaunchKernelGGL((k_matvec_dots<T, 2>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
+        CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
+        hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                            c->av, v.z, v.Az, v.r, c->partials);
         CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_FIRST));
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
-                           v.d, v.q, v.x, v.r, c->sc, 1);
+                           v.d, v.q, v.x, v.r, c->sc, 1, precond != 0 ? c->av : nullptr);
         for (int n = 2; n <= maxiters && !done; ++n) {
-            CISM_TRY(launch_precond<T>(c, precond, A, v.F, v.r, v.z));
-            hipLaunchKernelGGL((k_matvec_dots<T, 2>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream,
+            CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
+            hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
             CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_ITER));
             hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
-                               v.d, v.q, v.x, v.r, c->sc, 0);
+                               v.d, v.q, v.x, v.r, c->sc, 0, precond != 0 ? c->av : nullptr);
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
     }
@@ -636,7 +693,7 @@ static int run_cg(cism_ctx *c, const T *A, CGVecs<T> v, int chrongear, int preco
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol, double *err,
                  int *niters) {
     CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az, c->F64, nullptr};
-    return run_cg<double>(c, c->A, v, chrongear, precond, maxiters, tol, 0, err, niters);
+    return run_cg<double>(c, MatFull<double>{c->A}, v, chrongear, precond, maxiters, tol, 0, err, niters);
 }
 
 // ---------------------------------------------------------------------------
@@ -645,7 +702,7 @@ int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double t
 int ensure_a32(cism_ctx *c) {
     if (c->A32) return CISM_OK;
     const size_t v2 = 2 * (size_t)c->g.npad;
-    hipError_t e = hipMalloc(&c->A32, sizeof(float) * 108 * (size_t)c->g.npad);
+    hipError_t e = hipMalloc(&c->A32, sizeof(float) * 4 * CISM_SYM_SLOTS * (size_t)c->g.npad);
     float **vs[] = {&c->x32, &c->b32, &c->r32, &c->d32, &c->z32, &c->q32, &c->Az32};
     for (float **p : vs)
         if (e == hipSuccess) e = hipMalloc(p, sizeof(float) * v2);
@@ -655,12 +712,6 @@ int ensure_a32(cism_ctx *c) {
         return CISM_ERR_NOMEM;
     }
     return CISM_OK;
-}
-
-__global__ __launch_bounds__(CISM_BLOCK) void k_to_f32(long long n, const double *__restrict__ a,
-                                                       float *__restrict__ b) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) b[q] = (float)a[q];
 }
 
 // r32 = r64 / s (all entries), x32 = 0
@@ -685,10 +736,26 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_refine(long long n2, const float
 
 __global__ void k_save_scale(DevScal *sc) { sc->sums[2] = sqrt(sc->sums[0]); }
 
+// FP32 symmetric storage from the FP64 matrix: slot s of node q takes the four
+// blocks of slot CISM_DIAG + s; one lane per (node, slot), in destination order
+__global__ __launch_bounds__(CISM_BLOCK) void k_to_sym32(long long n, const double *__restrict__ A,
+                                                         float *__restrict__ As) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long tile = t / (64 * CISM_SYM_SLOTS);
+    const int s = (int)((t >> 6) % CISM_SYM_SLOTS);
+    const long long q = tile * 64 + (t & 63);
+    const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, CISM_DIAG + s, q));
+    const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, CISM_DIAG + s, q));
+    float4 o;
+    o.x = (float)a.x; o.y = (float)a.y; o.z = (float)b.x; o.w = (float)b.y;
+    *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = o;
+}
+
 int convert_a32(cism_ctx *c) {
     CISM_TRY(ensure_a32(c));
-    const long long n = 108LL * c->g.npad;
-    hipLaunchKernelGGL(k_to_f32, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, n, c->A, c->A32);
+    const long long n = (long long)CISM_SYM_SLOTS * c->g.npad;
+    hipLaunchKernelGGL(k_to_sym32, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, n, c->A, c->A32);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
@@ -712,8 +779,8 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
                        c->cscale_on ? c->cscale : nullptr};
     while (true) {
         CISM_TRY(launch_halo_vec(c, c->x));
-        hipLaunchKernelGGL((k_residual<double>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
-                           g, c->A, c->av, c->x, c->b, c->r, 1, c->partials);
+        hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
+                           g, MatFull<double>{c->A}, c->av, c->x, c->b, c->r, 1, c->partials);
         CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
         hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc);
         CISM_TRY(sync_scalars(c));
@@ -725,7 +792,7 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         double e_in = 0.0;
         int it = 0;
         // the inner reductions use sums[0..1]; the scale |r| stays in sums[2]
-        CISM_TRY(run_cg<float>(c, c->A32, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it));
+        CISM_TRY(run_cg<float>(c, MatSym32{c->A32}, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it));
         hipLaunchKernelGGL(k_refine, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x32, c->sc,
                            c->x);
         HIP_TRY(hipGetLastError());
@@ -779,8 +846,8 @@ int column_scale_device(cism_ctx *c, double *out) {
     const GridDev &g = c->g;
     const long long ncol = (long long)g.sx * g.sy;
     CISM_TRY(launch_halo_vec(c, c->x));
-    hipLaunchKernelGGL((k_residual<double>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g,
-                       c->A, c->av, c->x, c->b, c->r, 1, c->partials);
+    hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,
+                       c->stream, g, MatFull<double>{c->A}, c->av, c->x, c->b, c->r, 1, c->partials);
     hipLaunchKernelGGL(k_colscale, dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, g, c->A,
                        c->av, c->x, c->b, c->r, out);
     HIP_TRY(hipGetLastError());
@@ -988,11 +1055,11 @@ int launch_cg_matvec(cism_ctx *c, int fp32) {
     const GridDev &g = c->g;
     const int nb = nblocks_for(g.nn);
     if (fp32)
-        hipLaunchKernelGGL((k_matvec_dots<float, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
-                           c->A32, c->av, c->z32, c->Az32, c->r32, c->partials);
+        hipLaunchKernelGGL((k_matvec_dots<float, 2, MatSym32>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
+                           MatSym32{c->A32}, c->av, c->z32, c->Az32, c->r32, c->partials);
     else
-        hipLaunchKernelGGL((k_matvec_dots<double, 2>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream,
-                           g, c->A, c->av, c->z, c->Az, c->r, c->partials);
+        hipLaunchKernelGGL((k_matvec_dots<double, 2, MatFull<double>>), dim3(nb), dim3(CISM_BLOCK), 0,
+                           c->stream, g, MatFull<double>{c->A}, c->av, c->z, c->Az, c->r, c->partials);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }

@@ -36,14 +36,18 @@ def make_problem(family: str, ewn: int, nsn: int, seed: int):
     raise SystemExit(f"unknown family {family}")
 
 
-def solve_plain(ctx, p, log, chunk=10):
-    """The plain FP64 path at the reference's default tolerances, run in chunks of Picard
-    iterations (a Picard iteration depends only on the current velocity, so a chunk restarts
-    exactly) so that long solves report progress."""
+def solve_plain(ctx, p, log, chunk=10, precision=0):
+    """The plain FP64 path at the reference's default tolerances (precision 0), or the
+    mixed-precision path with inexact linear solves and Anderson acceleration to the same
+    Picard stopping rule (precision 1), run in chunks of Picard iterations (a Picard
+    iteration depends only on the current velocity, so a plain chunk restarts exactly) so
+    that long solves report progress."""
     lu, lv = ctx.loads(p)
     u, v, done = p.uvel, p.vvel, 0
+    fast = dict(adaptive_linear=1, anderson_m=4) if precision else {}
     while True:
-        r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u, vvel=v, precision=0, maxiter_nonlinear=chunk)
+        r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u, vvel=v, precision=precision,
+                                     maxiter_nonlinear=chunk, **fast)
         u, v = r["uvel"], r["vvel"]
         done += r["picard_iters"]
         if r["picard_iters"] < chunk or done >= p.maxiter_nonlinear:
@@ -114,6 +118,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--candidates", type=int, default=3,
                     help="try up to this many seeds per wanted sample (skipping unconverged ones)")
+    ap.add_argument("--data-precision", type=int, default=0,
+                    help="0: training solves on the plain FP64 path; 1: on the mixed-precision path")
     ap.add_argument("--scales", type=int, default=0,
                     help="also learn the per-column preconditioner scales (channels 22, 23)")
     a = ap.parse_args(argv)
@@ -136,7 +142,8 @@ def main(argv=None):
             if ctx is None:
                 ctx = G.Context.for_problem(p)
             t0 = time.perf_counter()
-            r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True))
+            r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True),
+                            precision=a.data_precision)
             umax = float(np.abs(r["uvel"]).max())
             ok = r["picard_iters"] < p.maxiter_nonlinear
             print(f"[train] {tag} {len(X) + ok}/{want} seed {s}: {r['picard_iters']} Picard, "
@@ -195,7 +202,7 @@ def main(argv=None):
         print(f"[train] epoch {ep + 1}/{a.epochs}: loss {np.mean(losses):.4f}, val rel L2 {ve:.4f}, "
               f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
     meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
-            "seeds": seeds_tr, "val_seeds": seeds_va,
+            "seeds": seeds_tr, "val_seeds": seeds_va, "data_precision": a.data_precision,
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
             "history": hist, "seconds": time.perf_counter() - t_start}
     SG.save_weights(a.out, net.get_params(), meta)
