@@ -275,8 +275,9 @@ def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whic
         beta[:] = 1.0e10
     elif whichbabc == HO_BABC_BETA_EXTERNAL and beta_vertex is not None:
         beta[:] = beta_vertex
-    if f_ground is not None:            # calcbeta weights beta by the grounded fraction
-        beta[:] = beta * f_ground
+    if f_ground is not None:            # calcbeta: beta * f_ground, floored at
+        beta[:] = beta * f_ground       # beta_grounded_min = 1 Pa yr/m where f_ground > 0
+        beta[:] = np.where((f_ground > 0) & (beta < 1.0), 1.0, beta)
     um = np.zeros((nz, nx - 1, ny - 1), dtype=np.int32, order="F")
     vm = np.zeros((nz, nx - 1, ny - 1), dtype=np.int32, order="F")
     if whichbabc == HO_BABC_NO_SLIP:
@@ -399,15 +400,16 @@ def sliding_slab(ewn=24, nsn=20, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, beta
                    HO_BABC_BETA_EXTERNAL, beta_vertex=bv, **kw)
 
 
-def grounded_fraction_vertex(ice, floating):
-    """Fraction of the ice cells around each vertex that are grounded (0 where no ice); a
-    flotation-criterion stand-in for glissade_grounded_fraction without a grounding-line
-    parameterisation."""
-    ice = ice.astype(np.float64)
-    gr = ice * (1 - floating)
-    n_ice = ice[:-1, :-1] + ice[1:, :-1] + ice[:-1, 1:] + ice[1:, 1:]
-    n_gr = gr[:-1, :-1] + gr[1:, :-1] + gr[:-1, 1:] + gr[1:, 1:]
-    return np.asfortranarray(np.where(n_ice > 0, n_gr / np.maximum(n_ice, 1), 0.0))
+def grounded_fraction_vertex(ice, floating, land):
+    """glissade_grounded_fraction with HO_GROUND_NO_GLP (glissade_grounding_line.F90:55-178):
+    1 at a vertex next to land or grounded ice, else 0; the vertex mask tests
+    land(i+1, j+1) twice and never land(i+1, j), as the reference's condition does."""
+    ice, floating, land = (np.asarray(a) == 1 for a in (ice, floating, land))
+    vm = (ice[:-1, 1:] | ice[1:, 1:] | ice[:-1, :-1] | ice[1:, :-1]
+          | land[:-1, 1:] | land[1:, 1:] | land[:-1, :-1])
+    cg = (ice & ~floating) | land
+    gv = cg[:-1, 1:] | cg[1:, 1:] | cg[:-1, :-1] | cg[1:, :-1]
+    return np.asfortranarray(np.where(vm & gv, 1.0, 0.0))
 
 
 def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
@@ -472,8 +474,8 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd
     thck = pad_periodic(H)
     topg = pad_periodic(bed)
     usrf = pad_periodic(s)
-    ice, floating, _, _, _ = get_masks(thck, topg, 0.0, thklim)
-    fg = grounded_fraction_vertex(ice, floating)
+    ice, floating, _, land, _ = get_masks(thck, topg, 0.0, thklim)
+    fg = grounded_fraction_vertex(ice, floating, land)
     bo = np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn)))
     bv = np.zeros((ewn + 3, nsn + 3))
     bv[2:ewn + 2, 2:nsn + 2] = bo

@@ -233,7 +233,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals,
                     c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
-                    c->ssa_flw3};
+                    c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -606,6 +606,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         counter++;
         HIP_TRY(hipMemcpyAsync(c->usav, c->x, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                c->stream));
+        if (c->bp_on) CISM_TRY(calcbeta_device(c));   // velo_higher:2502-2517
         double ta = now_s();
         if (dim2)
             CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, o->whichbabc != 6, o->dx, o->dy));
@@ -623,6 +624,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         if (c->sc_host->singular_flag) {
             cism_set_error("Jacobian matrix is singular");
             return CISM_ERR_SINGULAR;
+        }
+        if (c->bp_on && c->sc_host->beta_bad) {
+            cism_set_error("Invalid beta value in calcbeta (negative or NaN)");
+            return CISM_ERR_NAN;
         }
         const double nnz = c->sc_host->sums[0];
         t_asm += now_s() - ta;

@@ -52,6 +52,7 @@ struct DevScal {
     int nan_flag, singular_flag;
     unsigned int sym_count;
     int zero_stop;   // inner solves: an exactly zero residual ends the iteration (alpha = 0)
+    int beta_bad;    // calcbeta met a negative or NaN beta
 };
 
 struct cism_ctx {
@@ -78,6 +79,11 @@ struct cism_ctx {
     // sigma and the 3D flow factor of the caller's nz3 levels
     double *ssa_Ke, *ssa_ff2, *ssa_efvs2, *ssa_sigma, *ssa_flw3;
     int ssa_nz3;
+    // basal physics of the Picard loop (cism_set_basal_physics): with bp_on, beta is
+    // recomputed from the bed velocity at the start of every Picard iteration
+    cism_basal_physics bp;
+    int bp_on;
+    double *bp_bext, *bp_N, *bp_fg;   // beta_external, effective pressure, f_ground (vertices)
     // Anderson acceleration of the Picard map: m columns of dX and dF, previous x and f
     double *aa, *aa_sums;
     int aa_m, aa_count, aa_head;
@@ -249,6 +255,7 @@ int ensure_ssa(cism_ctx *c, int nz3);
 int ssa_prepare_device(cism_ctx *c);
 int assemble2d_device(cism_ctx *c, int whichefvs, double efvs_constant, int add_basal, double dx, double dy);
 int ssa_efvs_layers_device(cism_ctx *c, double *efvs3);
+int calcbeta_device(cism_ctx *c);
 // post-solve diagnostics from the context's geometry and velocity c->x
 int internal_stress_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                            double *tau);   // tau: 6 * ne (xz yz xx yy xy eff)

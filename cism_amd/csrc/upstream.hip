@@ -198,3 +198,258 @@ extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, doub
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CISM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// grounded fraction and basal traction
+// ---------------------------------------------------------------------------
+// glissade_grounded_fraction (glissade_grounding_line.F90:55-178), whichground 0
+// (NO_GLP) or 2 (ALL).  The vertex mask tests land(i+1, j+1) twice and never
+// land(i+1, j), as the reference's condition does.
+__global__ __launch_bounds__(256) void k_grounded_fraction(GridDev g, const int32_t *__restrict__ ice,
+                                                           const int32_t *__restrict__ fl,
+                                                           const int32_t *__restrict__ land, int whichground,
+                                                           double *__restrict__ fg) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)g.sx * g.sy) return;
+    const int i = (int)(v % g.sx), j = (int)(v / g.sx);
+    const long long sw = i + (long long)g.nx * j, se = sw + 1, nw = sw + g.nx, ne = nw + 1;
+    const bool vm = ice[nw] == 1 || ice[ne] == 1 || ice[sw] == 1 || ice[se] == 1 || land[nw] == 1 ||
+                    land[ne] == 1 || land[sw] == 1 || land[ne] == 1;
+    double f = 0.0;
+    if (vm) {
+        if (whichground == 2) {
+            f = 1.0;
+        } else {
+            auto cg = [&](long long c) { return (ice[c] == 1 && fl[c] == 0) || land[c] == 1; };
+            f = (cg(nw) || cg(ne) || cg(sw) || cg(se)) ? 1.0 : 0.0;
+        }
+    }
+    fg[v] = f;
+}
+
+// calcbeta (glissade_basal_traction.F90:67-598) on every vertex: the basal speed
+// sqrt(u^2 + v^2 + 0.01^2) (capped by beta_powerlaw_umax when > 0), the law of
+// whichbabc, then beta * f_ground and the beta_grounded_min floor where f_ground > 0.
+// A negative or NaN beta raises *bad.  ub, vb: basal velocity with stride `st`
+// between vertices (st = 1 for plain 2D arrays, 2 nz for the bed level of the
+// interleaved device velocity).
+__global__ __launch_bounds__(256) void k_calcbeta(GridDev g, cism_basal_physics bp, const double *__restrict__ ub,
+                                                  const double *__restrict__ vb, long long st,
+                                                  const double *__restrict__ bext,
+                                                  const double *__restrict__ N, const double *__restrict__ fg,
+                                                  double *__restrict__ beta, int *__restrict__ bad) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)g.sx * g.sy) return;
+    const double u = ub[v * st], w = vb[v * st];
+    double speed = sqrt(u * u + w * w + 1.0e-2 * 1.0e-2);
+    if (bp.beta_powerlaw_umax > 0.0) speed = fmin(speed, bp.beta_powerlaw_umax);
+    double b = 0.0;
+    switch (bp.whichbabc) {
+    case 0: b = bp.ho_beta_const; break;
+    case 4: b = bp.ho_beta_large; break;
+    case 5: b = bext[v]; break;
+    case 9: b = bp.powerlaw_c * pow(speed, 1.0 / bp.powerlaw_m - 1.0); break;
+    case 11: {
+        const double m = bp.powerlaw_m, n = N[v];
+        const double num = bp.powerlaw_c * bp.coulomb_c * n;
+        const double den = pow(pow(bp.powerlaw_c, m) * speed + pow(bp.coulomb_c * n, m), 1.0 / m);
+        b = (num / den) * pow(speed, 1.0 / m - 1.0);
+        if (b > 1.0e8) b = 1.0e8;
+        break;
+    }
+    case 12: {
+        const double tp = bp.powerlaw_c * pow(speed, 1.0 / bp.powerlaw_m);
+        const double tc = bp.coulomb_c * N[v];
+        b = tc <= tp ? tc / speed : tp / speed;
+        break;
+    }
+    case 14: {
+        const int j1 = (int)(v / g.sx) + 1;   // 1-based row, strip ns = 5 .. nsn-5
+        b = (j1 >= 5 && j1 <= g.ny - 5) ? 100.0 : 1.0e4;
+        break;
+    }
+    default: break;
+    }
+    b = b * fg[v];
+    if (fg[v] > 0.0 && b < bp.beta_grounded_min) b = bp.beta_grounded_min;
+    if (!(b >= 0.0)) bad[0] = 1;
+    beta[v] = b;
+}
+
+// staggered_parallel_halo of one vertex field: a halo vertex takes its periodic image
+// (always an owned vertex, so no entry is read after it is written) or 0 (outflow)
+__global__ __launch_bounds__(256) void k_vertex_halo(GridDev g, double *__restrict__ a) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (long long)g.sx * g.sy) return;
+    const int i = (int)(v % g.sx), j = (int)(v / g.sx);
+    const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
+    if (ii == i && jj == j) return;
+    a[v] = (ii < 0 || jj < 0) ? 0.0 : a[ii + (long long)g.sx * jj];
+}
+
+static bool babc_supported(int w) {
+    return w == 0 || w == 4 || w == 5 || w == 9 || w == 11 || w == 12 || w == 14;
+}
+
+static int launch_calcbeta(cism_ctx *c, const cism_basal_physics &bp, const double *ub, const double *vb,
+                           long long st, const double *bext, const double *N, const double *fg, double *beta,
+                           int *bad) {
+    const GridDev &g = c->g;
+    const long long ncol = (long long)g.sx * g.sy;
+    hipLaunchKernelGGL(k_calcbeta, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, bp, ub, vb, st, bext, N,
+                       fg, beta, bad);
+    hipLaunchKernelGGL(k_vertex_halo, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, beta);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+// the Picard loop's beta (c->beta) from the current basal velocity, the bed level of
+// c->x; a bad value raises sc->beta_bad
+int calcbeta_device(cism_ctx *c) {
+    const GridDev &g = c->g;
+    const double *ub = c->x + 2 * (long long)(g.nz - 1);
+    return launch_calcbeta(c, c->bp, ub, ub + 1, 2LL * g.nz, c->bp_bext, c->bp_N, c->bp_fg, c->beta,
+                           &c->sc->beta_bad);
+}
+
+static int check_basal_fields(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
+                              const double *effecpress_stag, const double *f_ground) {
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    if (!babc_supported(bp->whichbabc)) {
+        cism_set_error("calcbeta: which_ho_babc %d is not supported (0, 4, 5, 9, 11, 12, 14)", bp->whichbabc);
+        return CISM_ERR_UNSUPPORTED;
+    }
+    if (!f_ground || (bp->whichbabc == 5 && !beta_external) ||
+        ((bp->whichbabc == 11 || bp->whichbabc == 12) && !effecpress_stag) ||
+        ((bp->whichbabc == 9 || bp->whichbabc == 11 || bp->whichbabc == 12) && !(bp->powerlaw_m > 0.0))) {
+        cism_set_error("calcbeta: missing input field or powerlaw_m <= 0");
+        return CISM_ERR_ARG;
+    }
+    if (bp->whichbabc == 5) {   // the reference's check that beta was read (maxval >= 0)
+        double mx = -1.0e300;
+        for (size_t t = 0; t < ncol; ++t) mx = fmax(mx, beta_external[t]);
+        if (mx < 0.0) {
+            cism_set_error("calcbeta: all external beta values are < 0");
+            return CISM_ERR_ARG;
+        }
+    }
+    return CISM_OK;
+}
+
+// f_ground, beta_external, N to the device (unused fields are zero-filled)
+static int upload_basal_fields(cism_ctx *c, const double *beta_external, const double *effecpress_stag,
+                               const double *f_ground, double *d_bext, double *d_N, double *d_fg) {
+    const size_t ncol = (size_t)c->g.sx * c->g.sy;
+    HIP_TRY(hipMemcpyAsync(d_fg, f_ground, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    if (beta_external)
+        HIP_TRY(hipMemcpyAsync(d_bext, beta_external, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    else
+        HIP_TRY(hipMemsetAsync(d_bext, 0, sizeof(double) * ncol, c->stream));
+    if (effecpress_stag)
+        HIP_TRY(hipMemcpyAsync(d_N, effecpress_stag, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    else
+        HIP_TRY(hipMemsetAsync(d_N, 0, sizeof(double) * ncol, c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_glissade_grounded_fraction(cism_ctx *c, const int32_t *ice_mask,
+                                               const int32_t *floating_mask, const int32_t *land_mask,
+                                               int whichground, double *f_ground) {
+    if (!c || !ice_mask || !floating_mask || !land_mask || !f_ground) {
+        cism_set_error("grounded_fraction: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    if (whichground != 0 && whichground != 2) {
+        cism_set_error("grounded_fraction: which_ho_ground %d not supported (0 or 2; the reference "
+                       "stops on the GLP option 1)", whichground);
+        return CISM_ERR_UNSUPPORTED;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const GridDev &g = c->g;
+    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
+    // masks staged in the node-mask buffers (nn >= ncell ints each) and the load vector
+    int32_t *d_ice = c->umask, *d_fl = c->vmask, *d_land = c->umask + ncell;
+    if (3 * ncell > 2 * g.nn) {
+        cism_set_error("grounded_fraction: grid too small for the staging buffers");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipMemcpyAsync(d_ice, ice_mask, sizeof(int32_t) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_fl, floating_mask, sizeof(int32_t) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_land, land_mask, sizeof(int32_t) * ncell, hipMemcpyHostToDevice, c->stream));
+    double *d_fg = c->vstage;
+    hipLaunchKernelGGL(k_grounded_fraction, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, d_ice, d_fl,
+                       d_land, whichground, d_fg);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(f_ground, d_fg, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" int cism_calcbeta(cism_ctx *c, const cism_basal_physics *bp, const double *ubas,
+                             const double *vbas, const double *beta_external, const double *effecpress_stag,
+                             const double *f_ground, double *beta) {
+    if (!c || !bp || !ubas || !vbas || !beta) {
+        cism_set_error("calcbeta: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(check_basal_fields(c, bp, beta_external, effecpress_stag, f_ground));
+    // six vertex fields of temporary device memory; the context's own fields (the
+    // Picard loop's beta and basal physics) are left as they are
+    double *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, sizeof(double) * 6 * ncol));
+    double *d_u = tmp, *d_v = tmp + ncol, *d_bext = tmp + 2 * ncol, *d_N = tmp + 3 * ncol,
+           *d_fg = tmp + 4 * ncol, *d_beta = tmp + 5 * ncol;
+    int st = upload_basal_fields(c, beta_external, effecpress_stag, f_ground, d_bext, d_N, d_fg);
+    if (st == CISM_OK) {
+        hipError_t e = hipMemcpyAsync(d_u, ubas, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_v, vbas, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(&c->sc->beta_bad, 0, sizeof(int), c->stream);
+        if (e != hipSuccess) {
+            cism_set_error("calcbeta: %s", hipGetErrorString(e));
+            st = CISM_ERR_HIP;
+        }
+    }
+    if (st == CISM_OK) st = launch_calcbeta(c, *bp, d_u, d_v, 1, d_bext, d_N, d_fg, d_beta, &c->sc->beta_bad);
+    if (st == CISM_OK && hipMemcpyAsync(beta, d_beta, sizeof(double) * ncol, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+        cism_set_error("calcbeta: copy of beta failed");
+        st = CISM_ERR_HIP;
+    }
+    if (st == CISM_OK) st = sync_scalars(c);
+    hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (st != CISM_OK) return st;
+    if (c->sc_host->beta_bad) {
+        cism_set_error("Invalid beta value in calcbeta (negative or NaN)");
+        return CISM_ERR_NAN;
+    }
+    return CISM_OK;
+}
+
+extern "C" int cism_set_basal_physics(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
+                                      const double *effecpress_stag, const double *f_ground) {
+    if (!c) {
+        cism_set_error("set_basal_physics: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    if (!bp) {
+        c->bp_on = 0;
+        return CISM_OK;
+    }
+    CISM_TRY(check_basal_fields(c, bp, beta_external, effecpress_stag, f_ground));
+    const size_t ncol = (size_t)c->g.sx * c->g.sy;
+    if (!c->bp_fg) {
+        HIP_TRY(hipMalloc(&c->bp_fg, sizeof(double) * ncol));
+        HIP_TRY(hipMalloc(&c->bp_bext, sizeof(double) * ncol));
+        HIP_TRY(hipMalloc(&c->bp_N, sizeof(double) * ncol));
+    }
+    CISM_TRY(upload_basal_fields(c, beta_external, effecpress_stag, f_ground, c->bp_bext, c->bp_N, c->bp_fg));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->bp = *bp;
+    c->bp_on = 1;
+    return CISM_OK;
+}

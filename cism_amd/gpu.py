@@ -34,6 +34,7 @@ EXPORTED = [
     "cism_set_precond_scale", "cism_column_scale_target",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
     "cism_compute_basal_traction", "cism_velo_diagnostics", "cism_glissade_velo_inputs",
+    "cism_glissade_grounded_fraction", "cism_calcbeta", "cism_set_basal_physics",
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
@@ -55,6 +56,20 @@ class SolveOpts(C.Structure):
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
                 ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("reserved", C.c_int * 4)]
+
+
+class BasalPhysics(C.Structure):
+    """cism_basal_physics: the basal_physics fields calcbeta reads, reference defaults"""
+    _fields_ = [("whichbabc", C.c_int), ("reserved", C.c_int)] + \
+               [(n, C.c_double) for n in ("ho_beta_const", "ho_beta_large", "powerlaw_c",
+                                          "powerlaw_m", "coulomb_c", "beta_grounded_min",
+                                          "beta_powerlaw_umax")]
+
+    def __init__(self, whichbabc=5, **kw):
+        d = dict(ho_beta_const=1000.0, ho_beta_large=1.0e10, powerlaw_c=1.0e4, powerlaw_m=3.0,
+                 coulomb_c=0.42, beta_grounded_min=1.0, beta_powerlaw_umax=0.0)
+        d.update(kw)
+        super().__init__(whichbabc=whichbabc, **d)
 
 
 class SolveStats(C.Structure):
@@ -126,6 +141,9 @@ def lib():
         L.cism_velo_diagnostics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_int] + [D] * 9
         L.cism_glissade_velo_inputs.argtypes = [VP] + [C.c_double] * 5 + [D] * 4 + [I32] * 6 + [D] * 7
         L.cism_column_scale_target.argtypes = [VP, D, D]
+        L.cism_glissade_grounded_fraction.argtypes = [VP, I32, I32, I32, C.c_int, D]
+        L.cism_calcbeta.argtypes = [VP, C.POINTER(BasalPhysics), D, D, D, D, D, D]
+        L.cism_set_basal_physics.argtypes = [VP, C.POINTER(BasalPhysics), D, D, D]
         _lib = L
     return _lib
 
@@ -273,6 +291,35 @@ class Context:
 
     def set_reference_velocity(self, uref, vref):
         _check(lib().cism_set_reference_velocity(self.h, dp(uref), dp(vref)))
+
+    # --- grounded fraction and basal traction ----------------------------------------
+    def grounded_fraction(self, ice_mask, floating_mask, land_mask, whichground=0):
+        fg = np.zeros((self.nx - 1, self.ny - 1), order="F")
+        ms = [np.asfortranarray(m, dtype=np.int32) for m in (ice_mask, floating_mask, land_mask)]
+        _check(lib().cism_glissade_grounded_fraction(self.h, *[ip(m) for m in ms], whichground, dp(fg)))
+        return fg
+
+    @staticmethod
+    def _opt(a):
+        return None if a is None else np.asfortranarray(a, dtype=np.float64)
+
+    def calcbeta(self, bp, ubas, vbas, beta_external=None, effecpress_stag=None, f_ground=None):
+        s2 = (self.nx - 1, self.ny - 1)
+        fg = np.ones(s2, order="F") if f_ground is None else f_ground
+        arrs = [self._opt(a) for a in (ubas, vbas, beta_external, effecpress_stag, fg)]
+        beta = np.zeros(s2, order="F")
+        _check(lib().cism_calcbeta(self.h, C.byref(bp), *[None if a is None else dp(a) for a in arrs],
+                                   dp(beta)))
+        return beta
+
+    def set_basal_physics(self, bp, beta_external=None, effecpress_stag=None, f_ground=None):
+        """bp None switches the in-loop calcbeta off"""
+        if bp is None:
+            _check(lib().cism_set_basal_physics(self.h, None, None, None, None))
+            return
+        arrs = [self._opt(a) for a in (beta_external, effecpress_stag, f_ground)]
+        _check(lib().cism_set_basal_physics(self.h, C.byref(bp),
+                                            *[None if a is None else dp(a) for a in arrs]))
 
     # --- the solve's inputs from the geometry (velo_higher:1449-1939) -------------
     def glissade_velo_inputs(self, dx, dy, thklim, eus, thck, usrf, topg, flwa, max_slope=1.0):
@@ -422,9 +469,12 @@ class Context:
     def velo_higher_solve_2d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
                              maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
                              whichresid=None, target_override=0.0, inner_rtol=1e-3,
-                             adaptive_linear=0, adaptive_gamma=0.05, anderson_m=0):
+                             adaptive_linear=0, adaptive_gamma=0.05, anderson_m=0,
+                             stop_rel_err=0.0):
         """The SSA branch (whichapprox 1) on a one-level context: 3D inputs of p in, the
-        2D solution broadcast to every level of uvel/vvel out; bu/bv/resid are 2D."""
+        2D solution broadcast to every level of uvel/vvel out; bu/bv/resid are 2D.
+        stop_rel_err > 0 stops at that relative error against the reference velocity
+        (set_reference_velocity with (1, nx-1, ny-1) arrays)."""
         if self.nz != 1:
             raise ValueError("velo_higher_solve_2d needs a one-level context (Context.for_problem_2d)")
         o = SolveOpts()
@@ -444,6 +494,7 @@ class Context:
         o.adaptive_linear = adaptive_linear
         o.adaptive_gamma = adaptive_gamma
         o.anderson_m = anderson_m
+        o.stop_rel_err = stop_rel_err
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
         s2 = (p.nx - 1, p.ny - 1)

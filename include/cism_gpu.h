@@ -256,6 +256,18 @@ int  cism_velo_higher_solve_2d(cism_ctx *ctx, const cism_solve_opts *opts, int n
                                double *bu, double *bv, double *resid_u, double *resid_v,
                                cism_solve_stats *stats);
 
+/* The basal_physics fields calcbeta reads (glide_types.F90:1308-1370; reference
+ * defaults ho_beta_const 1000, ho_beta_large 1e10, powerlaw_c 1e4, powerlaw_m 3,
+ * coulomb_c 0.42, beta_grounded_min 1, beta_powerlaw_umax 0). whichbabc is
+ * which_ho_babc: 0 constant, 4 large, 5 external, 9 power law, 11 Coulomb-power
+ * law (Schoof), 12 Coulomb-power law (Tsai), 14 simple. */
+typedef struct {
+    int whichbabc;
+    int reserved;
+    double ho_beta_const, ho_beta_large, powerlaw_c, powerlaw_m, coulomb_c;
+    double beta_grounded_min, beta_powerlaw_umax;
+} cism_basal_physics;
+
 /* ---------------------------------------------------------------------------
  * The solve's inputs computed on the device from the ice geometry (the set-up of
  * glissade_velo_higher_solve, glissade_velo_higher.F90:1449-1939):
@@ -276,6 +288,30 @@ int  cism_glissade_velo_inputs(cism_ctx *ctx, double dx, double dy, double thkli
                                int32_t *active_cell, int32_t *active_vertex, double *xVertex,
                                double *yVertex, double *stagusrf, double *stagthck,
                                double *dusrf_dx, double *dusrf_dy, double *flwafact);
+
+/* glissade_grounded_fraction (glissade_grounding_line.F90:55-178): f_ground at
+ * vertices (nx-1, ny-1) from the cell masks (nx, ny).  whichground 0 (HO_GROUND_NO_GLP)
+ * or 2 (HO_GROUND_ALL); 1 (GLP) is CISM_ERR_UNSUPPORTED (the reference stops on it). */
+int  cism_glissade_grounded_fraction(cism_ctx *ctx, const int32_t *ice_mask,
+                                     const int32_t *floating_mask, const int32_t *land_mask,
+                                     int whichground, double *f_ground);
+
+/* calcbeta (glissade_basal_traction.F90:67-598): beta at vertices from the basal
+ * velocity ubas, vbas (nx-1, ny-1), weighted by f_ground, floored at
+ * beta_grounded_min where f_ground > 0, staggered halo applied.  beta_external is read
+ * for whichbabc 5, effecpress_stag for 11 and 12 (else may be NULL).  A negative or NaN
+ * beta is CISM_ERR_NAN (the reference's GM_FATAL check). */
+int  cism_calcbeta(cism_ctx *ctx, const cism_basal_physics *bp, const double *ubas,
+                   const double *vbas, const double *beta_external,
+                   const double *effecpress_stag, const double *f_ground, double *beta);
+
+/* Basal physics of the device Picard loop: with bp non-NULL, every Picard iteration of
+ * cism_velo_higher_solve_3d/_2d first recomputes beta by calcbeta from the current
+ * basal velocity, as glissade_velo_higher_solve does (velo_higher:2502-2517), and the
+ * beta_internal argument of the solve is not read.  bp = NULL switches it off. */
+int  cism_set_basal_physics(cism_ctx *ctx, const cism_basal_physics *bp,
+                            const double *beta_external, const double *effecpress_stag,
+                            const double *f_ground);
 
 /* ---------------------------------------------------------------------------
  * Post-solve diagnostics.  Outputs are Fortran-layout host arrays; each entry

@@ -1013,10 +1013,24 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
     double crit = 1.0e10, target = 1.0e-12;
     double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
     int counter = 0, status = ORC_OK;
+    const size_t ncol = (size_t)(nx - 1) * (ny - 1);
+    double *beta_work = NULL, *ubas = NULL, *vbas = NULL;
+    if (o->bp) { beta_work = malloc(ncol * 8); ubas = malloc(ncol * 8); vbas = malloc(ncol * 8); }
     o->total_linear_iters = 0;
     while (crit >= target && counter < o->maxiter_nonlinear) {
         counter++;
         memcpy(usav, uvel, nv * 8); memcpy(vsav, vvel, nv * 8);
+        if (o->bp) {
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i) {
+                    S2(ubas, i, j) = V3(uvel, nz, i, j);
+                    S2(vbas, i, j) = V3(vvel, nz, i, j);
+                }
+            status = orc_calcbeta(g, o->bp, ubas, vbas, o->beta_external, o->effecpress_stag,
+                                  o->f_ground, beta_work);
+            if (status) break;
+            beta_internal = beta_work;
+        }
         status = orc_assemble_3d(g, sigma, active_cell, xV, yV, uvel, vvel, stagusrf, stagthck,
                                  flwafact, o->whichapprox, efvs, o->whichefvs, o->efvs_constant,
                                  Auu, Auv, Avu, Avv);
@@ -1061,6 +1075,7 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
     o->final_resid_velo = resid_velo;
     o->status = status;
     free(Auu); free(Auv); free(Avu); free(Avv); free(usav); free(vsav);
+    free(beta_work); free(ubas); free(vbas);
     return status;
 }
 
@@ -1238,6 +1253,94 @@ void orc_basal_traction(const orc_grid *g, const double *beta, const double *uve
             S2(btractx, i, j) = S2(beta, i, j) * V3(uvel, nz, i, j);
             S2(btracty, i, j) = S2(beta, i, j) * V3(vvel, nz, i, j);
         }
+}
+
+/* glissade_grounded_fraction (glissade_grounding_line.F90:55-178), options
+ * HO_GROUND_NO_GLP (0) and HO_GROUND_ALL (2); the GLP (1) is fatal in the
+ * reference and ORC_ERR_ARG here.  The vertex mask tests land_mask(i+1,j+1) twice
+ * and never land_mask(i+1,j), as the reference's condition does (:112-115). */
+int orc_grounded_fraction(int nx, int ny, const int32_t *ice, const int32_t *floating,
+                          const int32_t *land, int whichground, double *f_ground)
+{
+    if (whichground != 0 && whichground != 2) return ORC_ERR_ARG;
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const int vm = C2(ice, i, j + 1) == 1 || C2(ice, i + 1, j + 1) == 1 || C2(ice, i, j) == 1 ||
+                           C2(ice, i + 1, j) == 1 || C2(land, i, j + 1) == 1 || C2(land, i + 1, j + 1) == 1 ||
+                           C2(land, i, j) == 1 || C2(land, i + 1, j + 1) == 1;
+            double fg = 0.0;
+            if (vm) {
+                if (whichground == 2) {
+                    fg = 1.0;
+                } else {
+#define CG(ii, jj) ((C2(ice, ii, jj) == 1 && C2(floating, ii, jj) == 0) || C2(land, ii, jj) == 1)
+                    fg = (CG(i, j + 1) || CG(i + 1, j + 1) || CG(i, j) || CG(i + 1, j)) ? 1.0 : 0.0;
+#undef CG
+                }
+            }
+            S2(f_ground, i, j) = fg;
+        }
+    return ORC_OK;
+}
+
+/* calcbeta (glissade_basal_traction.F90:67-598) for the options that need no
+ * thermal or hydrology state: 0 constant, 4 large, 5 external, 9 power law,
+ * 11 Coulomb-power law (Schoof), 12 Coulomb-power law (Tsai), 14 simple; then
+ * beta *= f_ground, the beta_grounded_min floor where f_ground > 0, the check that
+ * every beta is >= 0 (ORC_ERR_NAN otherwise) and the staggered halo.  ubas, vbas:
+ * the basal velocity on vertices; effecpress_stag: N on vertices (options 11, 12);
+ * g: the solve's grid (nx, ny cells). */
+int orc_calcbeta(const orc_grid *g, const orc_basal_physics *bp, const double *ubas,
+                 const double *vbas, const double *beta_external, const double *effecpress_stag,
+                 const double *f_ground, double *beta)
+{
+    const int nx = g->nx, ny = g->ny;
+    const double smallnum = 1.0e-2;
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const double u = S2(ubas, i, j), v = S2(vbas, i, j);
+            double speed = sqrt(u * u + v * v + smallnum * smallnum);
+            if (bp->beta_powerlaw_umax > 0.0) speed = fmin(speed, bp->beta_powerlaw_umax);
+            double b;
+            switch (bp->whichbabc) {
+            case 0: b = bp->ho_beta_const; break;
+            case 4: b = bp->ho_beta_large; break;
+            case 5: b = S2(beta_external, i, j); break;
+            case 9: b = bp->powerlaw_c * pow(speed, 1.0 / bp->powerlaw_m - 1.0); break;
+            case 11: {
+                const double m = bp->powerlaw_m, N = S2(effecpress_stag, i, j);
+                const double num = bp->powerlaw_c * bp->coulomb_c * N;
+                const double den = pow(pow(bp->powerlaw_c, m) * speed + pow(bp->coulomb_c * N, m), 1.0 / m);
+                b = (num / den) * pow(speed, 1.0 / m - 1.0);
+                if (b > 1.0e8) b = 1.0e8;
+                break;
+            }
+            case 12: {
+                const double tp = bp->powerlaw_c * pow(speed, 1.0 / bp->powerlaw_m);
+                const double tc = bp->coulomb_c * S2(effecpress_stag, i, j);
+                b = tc <= tp ? tc / speed : tp / speed;
+                break;
+            }
+            case 14: b = (j >= 5 && j <= ny - 5) ? 100.0 : 1.0e4; break;
+            default: return ORC_ERR_ARG;
+            }
+            S2(beta, i, j) = b;
+        }
+    if (bp->whichbabc == 5) {
+        double mx = -1.0e300;
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i) mx = fmax(mx, S2(beta, i, j));
+        if (mx < 0.0) return ORC_ERR_ARG;
+    }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            double b = S2(beta, i, j) * S2(f_ground, i, j);
+            if (S2(f_ground, i, j) > 0.0 && b < bp->beta_grounded_min) b = bp->beta_grounded_min;
+            if (!(b >= 0.0)) return ORC_ERR_NAN;
+            S2(beta, i, j) = b;
+        }
+    orc_halo(g, 1, beta);
+    return ORC_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1422,10 +1525,17 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
     double crit = 1.0e10, target = 1.0e-12;
     double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
     int counter = 0, status = ORC_OK;
+    double *beta_work = o->bp ? malloc(nv * 8) : NULL;
     o->total_linear_iters = 0;
     while (crit >= target && counter < o->maxiter_nonlinear) {
         counter++;
         memcpy(us, u2, nv * 8); memcpy(vs, v2, nv * 8);
+        if (o->bp) {   /* the basal velocity of the 2D solve is the 2D velocity (:2241-2243) */
+            status = orc_calcbeta(g, o->bp, u2, v2, o->beta_external, o->effecpress_stag, o->f_ground,
+                                  beta_work);
+            if (status) break;
+            beta_internal = beta_work;
+        }
         status = orc_assemble_2d_ssa(&g2, nz, sigma, active_cell, xV, yV, u2, v2, stagthck, flwafact,
                                      o->whichefvs, o->efvs_constant, efvs, Auu, Auv, Avu, Avv);
         if (status) break;
@@ -1477,6 +1587,6 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
     o->final_resid_velo = resid_velo;
     o->status = status;
     free(Auu); free(Auv); free(Avu); free(Avv); free(u2); free(v2); free(us); free(vs);
-    free(l2u); free(l2v); free(um2); free(vm2);
+    free(l2u); free(l2v); free(um2); free(vm2); free(beta_work);
     return status;
 }

@@ -160,6 +160,19 @@ void orc_basal_friction_heatflx(const orc_grid *g, const int32_t *active_cell,
 void orc_basal_traction(const orc_grid *g, const double *beta, const double *uvel,
                         const double *vvel, double *btractx, double *btracty);
 
+/* the basal_physics fields calcbeta reads (glide_types.F90:1308-1370) */
+typedef struct {
+    int whichbabc, reserved;
+    double ho_beta_const, ho_beta_large, powerlaw_c, powerlaw_m, coulomb_c;
+    double beta_grounded_min, beta_powerlaw_umax;
+} orc_basal_physics;
+
+int orc_grounded_fraction(int nx, int ny, const int32_t *ice, const int32_t *floating,
+                          const int32_t *land, int whichground, double *f_ground);
+int orc_calcbeta(const orc_grid *g, const orc_basal_physics *bp, const double *ubas,
+                 const double *vbas, const double *beta_external, const double *effecpress_stag,
+                 const double *f_ground, double *beta);
+
 /* Picard loop of glissade_velo_higher_solve, 3D BP branch (velo_higher:2206-3727).
  * Inputs are the arrays the reference computes before the loop. */
 typedef struct {
@@ -169,6 +182,11 @@ typedef struct {
     /* outputs */
     int picard_iters, total_linear_iters, status;
     double final_L2_norm, final_resid_velo;
+    /* optional: with bp set, beta is recomputed by calcbeta from the current basal
+     * velocity at the start of every Picard iteration (velo_higher:2502-2517), and
+     * beta_internal is not read */
+    const orc_basal_physics *bp;
+    const double *beta_external, *effecpress_stag, *f_ground;
 } orc_picard_opts;
 
 int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,

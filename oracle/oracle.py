@@ -37,13 +37,40 @@ class Grid(C.Structure):
                                        "periodic")]
 
 
+class BasalPhysics(C.Structure):
+    """the basal_physics fields calcbeta reads (glide_types.F90:1308-1370), reference defaults"""
+    _fields_ = [("whichbabc", C.c_int), ("reserved", C.c_int)] + \
+               [(n, C.c_double) for n in ("ho_beta_const", "ho_beta_large", "powerlaw_c",
+                                          "powerlaw_m", "coulomb_c", "beta_grounded_min",
+                                          "beta_powerlaw_umax")]
+
+    def __init__(self, whichbabc=5, **kw):
+        d = dict(ho_beta_const=1000.0, ho_beta_large=1.0e10, powerlaw_c=1.0e4, powerlaw_m=3.0,
+                 coulomb_c=0.42, beta_grounded_min=1.0, beta_powerlaw_umax=0.0)
+        d.update(kw)
+        super().__init__(whichbabc=whichbabc, **d)
+
+
 class PicardOpts(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("whichbabc", "whichefvs", "whichresid", "whichsparse",
                                        "whichprecond", "whichapprox", "maxiter_nonlinear",
                                        "linear_maxiters")] + \
                [(n, C.c_double) for n in ("efvs_constant", "linear_tolerance", "dx", "dy")] + \
                [(n, C.c_int) for n in ("picard_iters", "total_linear_iters", "status")] + \
-               [(n, C.c_double) for n in ("final_L2_norm", "final_resid_velo")]
+               [(n, C.c_double) for n in ("final_L2_norm", "final_resid_velo")] + \
+               [("bp", C.POINTER(BasalPhysics))] + \
+               [(n, C.POINTER(C.c_double)) for n in ("beta_external", "effecpress_stag", "f_ground")]
+
+
+def _set_basal(o, basal, keep):
+    """basal = (BasalPhysics, beta_external, effecpress_stag, f_ground) or None"""
+    if basal is None:
+        return
+    bp, bext, N, fg = basal
+    arrs = [np.asfortranarray(a, dtype=np.float64) for a in (bext, N, fg)]
+    keep.extend([bp] + arrs)
+    o.bp = C.pointer(bp)
+    o.beta_external, o.effecpress_stag, o.f_ground = (dp(a) for a in arrs)
 
 
 D = C.POINTER(C.c_double)
@@ -86,6 +113,8 @@ def _declare(L):
                                       D, D, D, D, D, D]
     L.orc_basal_friction_heatflx.argtypes = [G, I, I, D, D, D, C.c_int, D]
     L.orc_basal_traction.argtypes = [G, D, D, D, D, D]
+    L.orc_grounded_fraction.argtypes = [C.c_int, C.c_int, I, I, I, C.c_int, D]
+    L.orc_calcbeta.argtypes = [G, C.POINTER(BasalPhysics), D, D, D, D, D, D]
     pcg = [G, I, D, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, D, C.POINTER(C.c_int)]
     L.orc_pcg_standard_3d.argtypes = pcg
     L.orc_pcg_chrongear_3d.argtypes = pcg
@@ -277,10 +306,14 @@ def pcg(p, A, bu, bv, xu, xv, precond, chrongear=True, maxiters=200, tolerance=1
 
 
 def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
-           whichsparse=None, whichprecond=None, whichresid=None):
-    """Runs the oracle Picard loop from p.uvel/p.vvel (copied). Returns a result dict."""
+           whichsparse=None, whichprecond=None, whichresid=None, basal=None):
+    """Runs the oracle Picard loop from p.uvel/p.vvel (copied). Returns a result dict.
+    basal: (BasalPhysics, beta_external, effecpress_stag, f_ground) to recompute beta with
+    calcbeta every Picard iteration, or None for the fixed p.beta_internal."""
     g = grid_of(p)
     o = PicardOpts()
+    keep = []
+    _set_basal(o, basal, keep)
     o.whichbabc = p.whichbabc
     o.whichefvs = p.whichefvs
     o.whichresid = p.whichresid if whichresid is None else whichresid
@@ -306,6 +339,27 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
                 L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)
 
 
+def grounded_fraction(ice, floating, land, whichground=0):
+    """glissade_grounded_fraction: f_ground (nx-1, ny-1) from cell masks (nx, ny)"""
+    nx, ny = ice.shape
+    fg = np.zeros((nx - 1, ny - 1), order="F")
+    ms = [np.asfortranarray(m, dtype=np.int32) for m in (ice, floating, land)]
+    st = lib().orc_grounded_fraction(nx, ny, *[ip(m) for m in ms], whichground, dp(fg))
+    return st, fg
+
+
+def calcbeta(p, bp, ubas, vbas, beta_external=None, effecpress_stag=None, f_ground=None):
+    """calcbeta on p's grid: beta (nx-1, ny-1) with its staggered halo"""
+    s2 = (p.nx - 1, p.ny - 1)
+    z = np.zeros(s2, order="F")
+    arrs = [np.asfortranarray(a if a is not None else z, dtype=np.float64)
+            for a in (ubas, vbas, beta_external, effecpress_stag,
+                      f_ground if f_ground is not None else np.ones(s2))]
+    beta = np.zeros(s2, order="F")
+    st = lib().orc_calcbeta(C.byref(grid_of(p)), C.byref(bp), *[dp(a) for a in arrs], dp(beta))
+    return st, beta
+
+
 # --- 2D (SSA) solve on a one-level grid ------------------------------------------
 def grid2d_of(p):
     return grid(p.nx, p.ny, 1, p.periodic)
@@ -329,10 +383,12 @@ def assemble_2d(p, u2, v2, whichefvs=None):
 
 
 def picard_2d(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
-              whichsparse=None, whichprecond=None, whichresid=None):
+              whichsparse=None, whichprecond=None, whichresid=None, basal=None):
     """The SSA branch of the Picard loop from p.uvel/p.vvel (copied); 3D velocity out"""
     g = grid_of(p)
     o = PicardOpts()
+    keep = []
+    _set_basal(o, basal, keep)
     o.whichbabc = p.whichbabc
     o.whichefvs = p.whichefvs
     o.whichresid = p.whichresid if whichresid is None else whichresid
