@@ -413,7 +413,7 @@ def grounded_fraction_vertex(ice, floating, land):
 
 
 def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
-              s_max=3300.0, shelf=0.02, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-17,
+              s_max=3300.0, shelf=0.0, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-17,
               thklim=100.0, whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA,
               **kw) -> Problem:
     """Large single-GPU ice sheet on an elliptical footprint (normalised radius r = 1 at
@@ -423,14 +423,15 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd
     so the thickness H = s - b is multiscale and random in [0, H_max] (troughs deeper
     than s - H_max are filled) while the surface stays smooth.  Flow factor A = 1e-17
     Pa^-3 yr^-1 (cold ice), which keeps a 3 km thick sheet's speeds glaciological.
-    Floating fringe: ice is afloat wherever rho_i H < rho_o (-b), with surface
-    H (1 - rho_i / rho_o); beyond the grounded margin a fringe of width `shelf` (in r;
-    0.02 is about 10 km on the 1200 x 1400 km grid) thins linearly from 500 m to 150 m
-    where the sea is deep enough to float it.  The fringe is kept narrow on purpose:
-    floating ice has no basal drag, so its depth-uniform (plug) modes are held only by
-    membrane stresses, and a wide shelf drives the Jacobi-scaled condition number of
-    the Blatter-Pattyn system from about 1e5 to 1e8 (a 60 km shelf), beyond what FP32
-    inner solves can refine.  Ice-free ocean beyond; lognormal beta on grounded vertices
+    Floating fringe: ice is afloat wherever rho_i H < rho_o (-b) (thin margin ice over
+    the deepening bed), with surface H (1 - rho_i / rho_o).  Optionally (shelf > 0) a
+    shelf of width `shelf` (in r) beyond the grounded margin thins linearly from 500 m
+    to 150 m where the sea is deep enough to float it.  It is off by default: floating
+    ice has no basal drag, so its depth-uniform (plug) modes are held only by membrane
+    stresses and the front corners by a single element; a 60 km shelf drives the
+    Jacobi-scaled condition number of the Blatter-Pattyn system from about 1e4 to 1e8,
+    and at 1 km the plain FP64 Picard loop (200 PCG iterations per solve) no longer
+    settles.  Ice-free ocean beyond; lognormal beta on grounded vertices
     weighted by the grounded fraction; lateral ice-ocean pressure at the margin.  Floating
     ice not connected to grounded ice is removed (an unsupported floating patch has no
     unique velocity)."""

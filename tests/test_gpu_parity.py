@@ -29,6 +29,8 @@ def problems():
         "ismip": configs.ismip_hom_a(ewn=24, nsn=24, nz=11, L=12000.0),
         "grf": configs.grf_sheet(ewn=24, nsn=20, nz=7, dx=2000.0, corr=8000.0, seed=3),
         "sheet": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7),
+        # a floating shelf fringe: lateral ice-ocean load at floating fronts, f_ground = 0
+        "shelf": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7, shelf=0.02),
     }
 
 
@@ -39,7 +41,7 @@ def gpu():
     return G
 
 
-@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet"])
+@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet", "shelf"])
 def prob(request):
     p = problems()[request.param]
     print(f"\n[case] {p.name} ewn={p.ewn} nsn={p.nsn} nz={p.nz}", flush=True)
