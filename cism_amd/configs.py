@@ -412,7 +412,7 @@ def grounded_fraction_vertex(ice, floating, land):
     return np.asfortranarray(np.where(vm & gv, 1.0, 0.0))
 
 
-def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd=600.0,
+def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, H_edge=300.0, bed_sd=600.0,
               s_max=3300.0, shelf=0.0, beta_median=1.0e3, beta_sigma=1.0, A=1.0e-17,
               thklim=100.0, whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA,
               **kw) -> Problem:
@@ -420,8 +420,9 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd
     0.42 of the domain).  Grounded part: a Vialov-type surface s = s_max (1 - r^(4/3))^(3/8)
     with a long-wavelength (150 km) random perturbation, over a bed that deepens toward
     the edge (400 - 1400 r^2 m) and carries random relief at 50 km and 15 km (sd bed_sd),
-    so the thickness H = s - b is multiscale and random in [0, H_max] (troughs deeper
-    than s - H_max are filled) while the surface stays smooth.  Flow factor A = 1e-17
+    so the thickness H = s - b is multiscale and random in [0, H_max] while the surface
+    stays smooth (troughs deeper than the cap H_edge + (H_max - H_edge) (1 - r^(4/3))^(3/8)
+    are filled, so the margin is at most H_edge = 300 m thick).  Flow factor A = 1e-17
     Pa^-3 yr^-1 (cold ice), which keeps a 3 km thick sheet's speeds glaciological.
     Floating fringe: ice is afloat wherever rho_i H < rho_o (-b) (thin margin ice over
     the deepening bed), with surface H (1 - rho_i / rho_o).  Optionally (shelf > 0) a
@@ -446,9 +447,12 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, bed_sd
     bed = (400.0 - 1400.0 * rr ** 2
            + bed_sd * (0.8 * gaussian_random_field(ewn, nsn, dx, 50e3, rng)
                        + 0.6 * gaussian_random_field(ewn, nsn, dx, 15e3, rng)))
-    # thickness where the smooth surface meets the rough bed; where it would exceed H_max
-    # the bed is raised (troughs filled), so the surface stays smooth
-    bed = np.where(rr < 1.0, np.maximum(bed, s - H_max), bed)
+    # thickness where the smooth surface meets the rough bed; where it would exceed the
+    # cap H_edge + (H_max - H_edge) * prof the bed is raised (troughs filled), so the
+    # surface stays smooth and the margin is at most H_edge thick: no kilometre-high
+    # cliffs at the ice edge, whose lateral ocean load on a single element per corner
+    # drives the solve to 1e5 m/yr corner speeds
+    bed = np.where(rr < 1.0, np.maximum(bed, s - (H_edge + (H_max - H_edge) * prof)), bed)
     H = np.where(rr < 1.0, np.clip(s - bed, 0.0, None), 0.0)
     H_shelf = np.where((rr >= 1.0) & (rr < 1.0 + shelf), 500.0 - 350.0 * (rr - 1.0) / max(shelf, 1e-12), 0.0)
     H = np.where((H <= thklim) & (RHOI * H_shelf < RHOO * (-bed)), H_shelf, H)
