@@ -231,7 +231,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
-                    c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage, c->sym_list, c->sym_vals,
+                    c->dsdx, c->dsdy, c->Ke, c->xref, c->partials, c->sc, c->stage,
                     c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
                     c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg};
     for (void *p : ptrs)
@@ -432,10 +432,6 @@ extern "C" int cism_check_symmetry_assembled_matrix_3d(cism_ctx *c, const int32_
     CISM_TRY(symmetry_device(c));
     CISM_TRY(sync_scalars(c));
     if (nwarn) *nwarn = (int64_t)c->sc_host->sums[0];
-    if (c->sc_host->sym_count > (unsigned)c->sym_cap) {
-        cism_set_error("too many asymmetric pairs to repair (%u)", c->sc_host->sym_count);
-        return CISM_ERR_UNSUPPORTED;
-    }
     return download_matrix(c, Auu, Auv, Avu, Avv);
 }
 

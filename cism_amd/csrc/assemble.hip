@@ -616,81 +616,88 @@ int dirichlet_device(cism_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// check_symmetry_assembled_matrix_3d (velo_higher:9533-9741).  Each pair
-// (A(m,row), A'(mm,col)) is disjoint from every other pair, and the reference
-// repairs a pair iff |v1-v2| < 1e-8 |diag| from the side that visits it
-// (owned active rows), replacing both by their mean.  Phase 1 lists the pairs
-// to repair (read-only), phase 2 writes them.
+// check_symmetry_assembled_matrix_3d (velo_higher:9533-9741).  Each pair of
+// coefficients (A(m, row), A'(26-m, col)) is disjoint from every other pair.  The
+// reference visits a pair from each of its two rows that is owned and active, and
+// replaces both by their mean as soon as one visit finds |v1 - v2| < 1e-8 |diag|
+// (the visiting row's Auu diagonal for the Auu and Auv tests, its Avv diagonal for
+// the Avv and Avu tests); a repaired pair compares equal at the other visit.  So a
+// pair is repaired iff it qualifies from at least one visiting side.
+//
+// Here one lane per node owns the pairs of its slots CISM_DIAG .. 26 (the diagonal
+// slot's Auv/Avu pair and the 13 couplings after it), tests both sides on the
+// unmodified values and writes both coefficients of a repaired pair.  No other lane
+// reads or writes those coefficients (a lane reads only its own slots from CISM_DIAG
+// on and, of other nodes, the lower partner slots of its own pairs and the diagonal
+// Auu/Avv, which no repair touches), so the pass is in place, race-free and
+// independent of the launch.  partials: per-block count of visits that find an
+// asymmetry above the threshold (the reference's warnings), on the unmodified matrix.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sym_find(GridDev g, const int32_t *__restrict__ av,
-                                                  const double *__restrict__ A,
-                                                  long long *__restrict__ list,
-                                                  double *__restrict__ vals, long long cap,
-                                                  DevScal *__restrict__ sc,
-                                                  double *__restrict__ partials) {
-    double acc[1] = {0.0};   // number of pairs that stay asymmetric (warnings)
+__global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__restrict__ av,
+                                                    double *__restrict__ A, double *__restrict__ partials) {
+    double acc[1] = {0.0};
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = (int)(q % g.nz);
-    const long long col = q / g.nz;
-    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-    if (q < g.nn && is_owned(g, i, j) && av[col]) {
+    if (q < g.nn) {
+        const int k = (int)(q % g.nz);
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        const bool vis_q = is_owned(g, i, j) && av[col];
         const double dgu = A[aidx(0, CISM_DIAG, q)], dgv = A[aidx(3, CISM_DIAG, q)];
-        // pairs per slot: Auu(m,q)~Auu(-m,qc) and Auv(m,q)~Avu(-m,qc) against |Auu diag|,
-        // Avv(m,q)~Avv(-m,qc) and Avu(m,q)~Auv(-m,qc) against |Avv diag|
-        auto test = [&](long long e1, long long e2, double v1, double v2, double diag) {
+        double warn = 0.0;
+        // one coefficient pair (e1, e2) with values v1, v2; qualification from each side
+        auto pair = [&](long long e1, long long e2, double v1, double v2, bool va, double da, bool vb,
+                        double db) {
             if (v1 == v2) return;
-            if (fabs(v2 - v1) < 1.0e-8 * fabs(diag)) {
-                const unsigned int s = atomicAdd(&sc->sym_count, 1u);
-                if ((long long)s < cap) {
-                    list[2 * s] = e1;
-                    list[2 * s + 1] = e2;
-                    vals[s] = 0.5 * (v1 + v2);
-                }
-            } else {
-                acc[0] += 1.0;
+            const double d = fabs(v2 - v1);
+            const bool qa = va && d < 1.0e-8 * fabs(da), qb = vb && d < 1.0e-8 * fabs(db);
+            warn += (va && !qa ? 1.0 : 0.0) + (vb && !qb ? 1.0 : 0.0);
+            if (qa || qb) {
+                const double avg = 0.5 * (v1 + v2);
+                A[e1] = avg;
+                A[e2] = avg;
             }
         };
-        for (int m = 0; m < 27; ++m) {
+        // the diagonal slot: Auv(q) ~ Avu(q), tested against both diagonals
+        {
+            const long long e1 = aidx(1, CISM_DIAG, q), e2 = aidx(2, CISM_DIAG, q);
+            const double v1 = A[e1], v2 = A[e2];
+            if (vis_q && v1 != v2) {
+                const double d = fabs(v2 - v1);
+                const bool qa = d < 1.0e-8 * fabs(dgu), qb = d < 1.0e-8 * fabs(dgv);
+                warn += (qa ? 0.0 : 1.0) + (qb ? 0.0 : 1.0);
+                if (qa || qb) {
+                    const double avg = 0.5 * (v1 + v2);
+                    A[e1] = avg;
+                    A[e2] = avg;
+                }
+            }
+        }
+        for (int m = CISM_DIAG + 1; m < 27; ++m) {
             const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-            if (k + kA < 0 || k + kA >= g.nz) continue;
+            if (k + kA < 0 || k + kA >= g.nz || i + iA < 0 || i + iA >= g.sx || j + jA < 0 || j + jA >= g.sy)
+                continue;
             const int mm = 26 - m;
             const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+            const long long cc = col + iA + (long long)g.sx * jA;
+            const bool vis_c = is_owned(g, i + iA, j + jA) && av[cc];
+            if (!vis_q && !vis_c) continue;
+            const double cgu = A[aidx(0, CISM_DIAG, qc)], cgv = A[aidx(3, CISM_DIAG, qc)];
             double a0, a1, a2, a3, b0, b1, b2, b3;
             load_blocks(A, q, m, a0, a1, a2, a3);
             load_blocks(A, qc, mm, b0, b1, b2, b3);
-            test(aidx(0, m, q), aidx(0, mm, qc), a0, b0, dgu);
-            test(aidx(1, m, q), aidx(2, mm, qc), a1, b2, dgu);
-            test(aidx(3, m, q), aidx(3, mm, qc), a3, b3, dgv);
-            test(aidx(2, m, q), aidx(1, mm, qc), a2, b1, dgv);
+            pair(aidx(0, m, q), aidx(0, mm, qc), a0, b0, vis_q, dgu, vis_c, cgu);   // Auu ~ Auu
+            pair(aidx(1, m, q), aidx(2, mm, qc), a1, b2, vis_q, dgu, vis_c, cgv);   // Auv ~ Avu
+            pair(aidx(2, m, q), aidx(1, mm, qc), a2, b1, vis_q, dgv, vis_c, cgu);   // Avu ~ Auv
+            pair(aidx(3, m, q), aidx(3, mm, qc), a3, b3, vis_q, dgv, vis_c, cgv);   // Avv ~ Avv
         }
+        acc[0] = warn;
     }
     block_reduce_store<1>(acc, partials);
 }
 
-__global__ void k_sym_apply(const long long *__restrict__ list, const double *__restrict__ vals,
-                            long long cap, const DevScal *__restrict__ sc, double *__restrict__ A) {
-    long long n = sc->sym_count;
-    if (n > cap) n = cap;
-    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n;
-         s += (long long)gridDim.x * blockDim.x) {
-        A[list[2 * s]] = vals[s];
-        A[list[2 * s + 1]] = vals[s];
-    }
-}
-
 int symmetry_device(cism_ctx *c) {
-    if (!c->sym_list) {
-        c->sym_cap = 1 << 20;
-        HIP_TRY(hipMalloc(&c->sym_list, sizeof(long long) * 2 * c->sym_cap));
-        HIP_TRY(hipMalloc(&c->sym_vals, sizeof(double) * c->sym_cap));
-    }
-    HIP_TRY(hipMemsetAsync(&c->sc->sym_count, 0, sizeof(unsigned int), c->stream));
     const int nb = nblocks_for(c->g.nn);
-    hipLaunchKernelGGL(k_sym_find, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->sym_list,
-                       c->sym_vals, c->sym_cap, c->sc, c->partials);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_sym_apply, dim3(256), dim3(256), 0, c->stream, c->sym_list, c->sym_vals,
-                       c->sym_cap, c->sc, c->A);
+    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->partials);
     HIP_TRY(hipGetLastError());
     CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
     return CISM_OK;

@@ -50,7 +50,7 @@ struct DevScal {
     double L2_rhs, err;
     double sums[8];
     int nan_flag, singular_flag;
-    unsigned int sym_count;
+    unsigned int unused;
     int zero_stop;   // inner solves: an exactly zero residual ends the iteration (alpha = 0)
     int beta_bad;    // calcbeta met a negative or NaN beta
 };
@@ -102,10 +102,6 @@ struct cism_ctx {
     DevScal *sc_host;   // pinned mirror
     // staging for layout conversion (27*nn doubles, lazily)
     double *stage;
-    // symmetry-repair list (lazily)
-    long long *sym_list;
-    double *sym_vals;
-    long long sym_cap;
 };
 
 // ---------------------------------------------------------------------------

@@ -182,7 +182,27 @@ def features(p: configs.Problem) -> np.ndarray:
 S_CH = 22          # log preconditioner scale, u and v components (channels 22, 23)
 
 
-def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None):
+def encode(vel, uscale: float, target: str = "linear"):
+    """velocity (m/yr) -> network units: vel / uscale ("linear") or
+    sign(vel) log(1 + |vel| / uscale) ("log1p", which keeps the few fast margin
+    vertices of an ice sheet from dominating the squared loss)"""
+    if target == "log1p":
+        return np.sign(vel) * np.log1p(np.abs(vel) / uscale)
+    if target == "linear":
+        return vel / uscale
+    raise ValueError(f"unknown target encoding {target!r}")
+
+
+def decode(out, uscale: float, target: str = "linear"):
+    """network units -> velocity (m/yr), the inverse of encode"""
+    if target == "log1p":
+        return np.sign(out) * uscale * np.expm1(np.abs(out))
+    if target == "linear":
+        return uscale * out
+    raise ValueError(f"unknown target encoding {target!r}")
+
+
+def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None, target: str = "linear"):
     """(H, W, 24) float32 target and (H, W) mask from a solved velocity field and,
     optionally, the per-column preconditioner scales of that solve (vertex fields)."""
     if p.nz > NZ_MAX:
@@ -191,8 +211,8 @@ def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None):
     t = np.zeros(av.shape + (N_OUT,), dtype=np.float32)
     u = np.asarray(uvel)[:, 2:p.nx - 2, 2:p.ny - 2].transpose(2, 1, 0)   # (H, W, nz)
     v = np.asarray(vvel)[:, 2:p.nx - 2, 2:p.ny - 2].transpose(2, 1, 0)
-    t[..., U_CH:U_CH + p.nz] = u / uscale
-    t[..., V_CH:V_CH + p.nz] = v / uscale
+    t[..., U_CH:U_CH + p.nz] = encode(u, uscale, target)
+    t[..., V_CH:V_CH + p.nz] = encode(v, uscale, target)
     if su is not None and sv is not None:
         t[..., S_CH] = np.log(owned(su, p))
         t[..., S_CH + 1] = np.log(owned(sv, p))
@@ -200,14 +220,14 @@ def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None):
     return t, av.astype(np.float32)
 
 
-def velocity_from_output(p: configs.Problem, out: np.ndarray, uscale: float):
+def velocity_from_output(p: configs.Problem, out: np.ndarray, uscale: float, target: str = "linear"):
     """Network output (H, W, 24) -> uvel, vvel (nz, nx-1, ny-1), halos by periodic copy."""
     av = owned(p.active_vertex, p) != 0
     u = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
     v = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
     o = np.where(av[..., None], out, 0.0)
-    u[:, 2:p.nx - 2, 2:p.ny - 2] = (uscale * o[..., U_CH:U_CH + p.nz]).transpose(2, 1, 0)
-    v[:, 2:p.nx - 2, 2:p.ny - 2] = (uscale * o[..., V_CH:V_CH + p.nz]).transpose(2, 1, 0)
+    u[:, 2:p.nx - 2, 2:p.ny - 2] = decode(o[..., U_CH:U_CH + p.nz], uscale, target).transpose(2, 1, 0)
+    v[:, 2:p.nx - 2, 2:p.ny - 2] = decode(o[..., V_CH:V_CH + p.nz], uscale, target).transpose(2, 1, 0)
     for a in (u, v):
         for k in range(p.nz):
             a[k] = configs.stag_halo(np.asfortranarray(a[k]), p.periodic)
@@ -251,6 +271,7 @@ class Predictor:
             raise ValueError("the surrogate runs on periodic grids (circular padding)")
         params, self.meta = load_weights(weights)
         self.uscale = float(self.meta["uscale"])
+        self.target = self.meta.get("target", "linear")
         self.net = Surrogate(p.nsn, p.ewn, 1, device)
         self.net.set_params(params)
 
@@ -258,7 +279,7 @@ class Predictor:
         """Initial guess (uvel, vvel); with scales=True also the per-column preconditioner
         scales (su, sv), when the weights were trained on them."""
         out = self.net.forward(features(p)[None])[0]
-        u, v = velocity_from_output(p, out, self.uscale)
+        u, v = velocity_from_output(p, out, self.uscale, self.target)
         if not scales:
             return u, v
         if not self.meta.get("scales", False):

@@ -118,6 +118,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--candidates", type=int, default=3,
                     help="try up to this many seeds per wanted sample (skipping unconverged ones)")
+    ap.add_argument("--target", default="linear", choices=["linear", "log1p"],
+                    help="velocity encoding of the network output (log1p compresses fast margins)")
     ap.add_argument("--data-precision", type=int, default=0,
                     help="0: training solves on the plain FP64 path; 1: on the mixed-precision path")
     ap.add_argument("--scales", type=int, default=0,
@@ -152,7 +154,7 @@ def main(argv=None):
             if ok:
                 x = SG.features(p)
                 su, sv = ctx.column_scale_target() if a.scales else (None, None)
-                t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv)
+                t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv, target=a.target)
                 X.append(x), T.append(t), M.append(m), used.append(s)
             s += 1
         ctx.close()
@@ -171,9 +173,12 @@ def main(argv=None):
     evalnet = SG.Surrogate(nsn, ewn, 1)
 
     def validate():
-        """mean relative L2 error of the predicted velocities on the validation problems"""
+        """mean relative L2 error of the predicted velocities (decoded to m/yr) on the
+        validation problems"""
         evalnet.set_params(net.get_params())
-        errs = [rel_err(evalnet.forward(Xva[i:i + 1])[0], Tva[i], Mva[i], 22) for i in range(len(Xva))]
+        dec = lambda t: SG.decode(t[..., :22], a.uscale, a.target)   # noqa: E731
+        errs = [rel_err(dec(evalnet.forward(Xva[i:i + 1])[0]), dec(Tva[i]), Mva[i], 22)
+                for i in range(len(Xva))]
         return float(np.mean(errs))
 
     rng = np.random.default_rng(0)
@@ -203,6 +208,7 @@ def main(argv=None):
               f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
     meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
             "seeds": seeds_tr, "val_seeds": seeds_va, "data_precision": a.data_precision,
+            "target": a.target,
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
             "history": hist, "seconds": time.perf_counter() - t_start}
     SG.save_weights(a.out, net.get_params(), meta)

@@ -236,3 +236,22 @@ def test_picard_fp64(gpu, prob):
     print(f"  relative L2 velocity difference {e:.3e}")
     assert e < TOL_PICARD
     assert rel_max(rg["efvs"], ro["efvs"]) < 1e-6
+
+
+def test_symmetry_repair_millions_of_pairs(gpu):
+    """every coefficient perturbed by ~1e-12 relative, so that (nearly) every pair of a
+    64 x 64 x 11 system needs the reference's averaging: more than 2^20 pairs, all repaired
+    exactly as the oracle repairs them"""
+    p = configs.grf_sheet(ewn=64, nsn=64, nz=11, dx=2000.0, corr=8000.0, seed=9)
+    A, bu, bv, u, v, _ = assembled_system(p, seed=2)
+    rng = np.random.default_rng(4)
+    Ap = [np.asfortranarray(a * (1.0 + 1e-12 * rng.standard_normal(a.shape))) for a in A]
+    Ao = [a.copy(order="F") for a in Ap]
+    O.check_symmetry(p, Ao)
+    changed = sum(int(np.count_nonzero(a != b)) for a, b in zip(Ao, Ap))
+    assert changed // 2 > (1 << 20)
+    Ag = [a.copy(order="F") for a in Ap]
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.check_symmetry_assembled_matrix_3d(p.active_vertex, *Ag)
+    for a, b in zip(Ag, Ao):
+        assert np.array_equal(a, b)
