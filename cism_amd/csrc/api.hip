@@ -214,7 +214,7 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     if ((st = dalloc(&c->sigma, (size_t)nz)) || (st = dalloc(&c->wsig, (size_t)nz + 1)) ||
         (st = dalloc(&c->flwafact, (size_t)g.nepad)) || (st = dalloc(&c->efvs, (size_t)g.nepad)))
         return fail(st);
-    const size_t nparts = (size_t)(nblocks_for(g.nn) > CISM_RED_BLOCKS ? nblocks_for(g.nn) : CISM_RED_BLOCKS);
+    const size_t nparts = (size_t)(xcd_blocks_for(g.nn) > CISM_RED_BLOCKS ? xcd_blocks_for(g.nn) : CISM_RED_BLOCKS);
     // per-block partial sums: up to CISM_MAX_NV values per block (the widest reduction)
     if ((st = dalloc(&c->partials, nparts * CISM_MAX_NV))) return fail(st);
     if ((st = dalloc(&c->sc, 1))) return fail(st);

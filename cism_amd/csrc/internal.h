@@ -202,6 +202,21 @@ inline int nblocks_for(long long n, int block = CISM_BLOCK) {
     long long b = (n + block - 1) / block;
     return (int)(b < 1 ? 1 : b);
 }
+// XCD-aware launch of the FP32 symmetric matvec: workgroups are dealt round-robin
+// to the 8 XCDs, so a grid of xcd_blocks_for(n) blocks (a multiple of 8 chunks) lets
+// XCD x take chunks x, x+8, x+16, ... of CISM_XCD_CHUNK consecutive node blocks; the
+// neighbours a node reads from the row before it were then read by the same XCD
+// moments earlier and sit in its L2
+#define CISM_XCD_CHUNK 512
+inline int xcd_blocks_for(long long n) {
+    const long long per = 8LL * CISM_XCD_CHUNK;
+    return (int)((nblocks_for(n) + per - 1) / per * per);
+}
+__device__ inline long long xcd_block() {
+    const unsigned b = blockIdx.x, x = b & 7u, k = b >> 3;
+    return ((long long)(k / CISM_XCD_CHUNK) * 8 + x) * CISM_XCD_CHUNK + (k % CISM_XCD_CHUNK);
+}
+
 inline int red_blocks_for(long long n) {
     long long b = (n + CISM_BLOCK - 1) / CISM_BLOCK;
     if (b > CISM_RED_BLOCKS) b = CISM_RED_BLOCKS;

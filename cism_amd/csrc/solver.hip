@@ -14,19 +14,35 @@
 // ---------------------------------------------------------------------------
 // final reduction + CG scalar updates (single block, fixed order)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CISM_BLOCK) void k_reduce(const double *__restrict__ partials,
-                                                       int nb, int nv, int mode,
-                                                       DevScal *__restrict__ sc) {
-    __shared__ double red[CISM_BLOCK];
+// 1024 lanes, each with 8 independent accumulators over a fixed set of partials (lane
+// t, chain u takes blocks t + 1024 (8 r + u)), combined in a fixed order: the partials
+// of a 70,000-block grid are read with 8 loads in flight per lane instead of one
+// dependent chain, and the result does not depend on timing.
+#define RED_THREADS 1024
+__global__ __launch_bounds__(RED_THREADS) void k_reduce(const double *__restrict__ partials,
+                                                        int nb, int nv, int mode,
+                                                        DevScal *__restrict__ sc) {
+    __shared__ double red[RED_THREADS];
+    const bool mx = mode == RED_MAX;
     for (int c = 0; c < nv; ++c) {
-        double s;
-        if (mode == RED_MAX) {
-            s = 0.0;
-            for (int b = threadIdx.x; b < nb; b += blockDim.x) s = fmax(s, partials[(size_t)b * nv + c]);
-        } else {
-            s = 0.0;
-            for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[(size_t)b * nv + c];
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = 0.0;
+        int b = threadIdx.x;
+        for (; b + 7 * RED_THREADS < nb; b += 8 * RED_THREADS) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[(size_t)(b + u * RED_THREADS) * nv + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = mx ? fmax(a[u], v[u]) : a[u] + v[u];
         }
+        for (int u = 0; b < nb; b += RED_THREADS, ++u) {   // fewer than 8 strides remain
+            const double v = partials[(size_t)b * nv + c];
+            a[u] = mx ? fmax(a[u], v) : a[u] + v;
+        }
+        double s = a[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = mx ? fmax(s, a[u]) : s + a[u];
         red[threadIdx.x] = s;
         __syncthreads();
         for (int off = blockDim.x / 2; off > 0; off >>= 1) {
@@ -90,7 +106,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_reduce(const double *__restrict_
 }
 
 int launch_reduce(cism_ctx *c, int nb, int nv, int mode) {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(CISM_BLOCK), 0, c->stream, c->partials, nb, nv,
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RED_THREADS), 0, c->stream, c->partials, nb, nv,
                        mode, c->sc);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
@@ -112,6 +128,7 @@ template <> struct Vec2<double> { using t = double2; };
 // neighbour is qn; entry(g, b, m, q): one block of one slot.
 template <typename T>
 struct MatFull {   // all 27 slots stored (aidx)
+    static constexpr bool xcd = false;
     const T *A;
     __device__ inline void blocks(long long q, int m, long long qn, T &uu, T &uv, T &vu, T &vv) const {
         (void)qn;
@@ -123,7 +140,8 @@ struct MatFull {   // all 27 slots stored (aidx)
     }
 };
 
-struct MatSym32 {   // FP32 symmetric storage (sidx)
+struct MatSym32 {   // FP32 symmetric storage (sidx); matvec launched XCD-aware
+    static constexpr bool xcd = true;
     const float *A;
     __device__ inline void blocks(long long q, int m, long long qn, float &uu, float &uv, float &vu,
                                   float &vv) const {
@@ -216,7 +234,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const M A
     double acc[NDOT];
 #pragma unroll
     for (int c = 0; c < NDOT; ++c) acc[c] = 0.0;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (M::xcd ? xcd_block() : (long long)blockIdx.x) * blockDim.x + threadIdx.x;
     if (q < g.nn) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
@@ -377,15 +395,42 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
         const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
         const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
         const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
-        T prev = s[0] / be[0];
-        s[0] = prev;
-        for (int k = 1; k < nz; ++k) {
-            prev = (s[2 * k] - sb[k * ncol] * prev) / be[k * ncol];
-            s[2 * k] = prev;
-        }
-        for (int k = nz - 2; k >= 0; --k) {
-            prev = s[2 * k] - ga[(k + 1) * ncol] * prev;
-            s[2 * k] = prev;
+        if constexpr (NZMAX <= 12) {
+            // every factor load is issued before the substitutions, which would otherwise
+            // wait on one global load per level
+            T fs[NZMAX], fb[NZMAX], fg[NZMAX];
+#pragma unroll
+            for (int k = 0; k < NZMAX; ++k)
+                if (k < nz) {
+                    fs[k] = sb[k * ncol];
+                    fb[k] = be[k * ncol];
+                    fg[k] = ga[k * ncol];
+                }
+            T prev = s[0] / fb[0];
+            s[0] = prev;
+#pragma unroll
+            for (int k = 1; k < NZMAX; ++k)
+                if (k < nz) {
+                    prev = (s[2 * k] - fs[k] * prev) / fb[k];
+                    s[2 * k] = prev;
+                }
+#pragma unroll
+            for (int k = NZMAX - 2; k >= 0; --k)
+                if (k <= nz - 2) {
+                    prev = s[2 * k] - fg[k + 1] * prev;
+                    s[2 * k] = prev;
+                }
+        } else {
+            T prev = s[0] / be[0];
+            s[0] = prev;
+            for (int k = 1; k < nz; ++k) {
+                prev = (s[2 * k] - sb[k * ncol] * prev) / be[k * ncol];
+                s[2 * k] = prev;
+            }
+            for (int k = nz - 2; k >= 0; --k) {
+                prev = s[2 * k] - ga[(k + 1) * ncol] * prev;
+                s[2 * k] = prev;
+            }
         }
     }
     __syncthreads();
@@ -524,11 +569,11 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
                                                           T *__restrict__ x, T *__restrict__ r,
                                                           const DevScal *__restrict__ sc, int first,
                                                           const int32_t *__restrict__ skip_av) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * g.nn) return;
-    const long long q = t >> 1;
-    const int c = (int)(t & 1);
-    T az;
+    // one lane per node: both components through one 8- or 16-byte access per vector
+    using V2 = typename Vec2<T>::t;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    V2 az;
     {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
@@ -536,26 +581,39 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
         const bool own = is_owned(g, i, j);
         if (own && skip_av && !skip_av[col]) return;
         if (own) {
-            az = Az[t];
+            az = reinterpret_cast<const V2 *>(Az)[q];
         } else {
             const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
-            az = (ii < 0 || jj < 0) ? (T)0
-                                    : Az[2 * (k + (long long)g.nz * (ii + (long long)g.sx * jj)) + c];
+            if (ii < 0 || jj < 0) {
+                az.x = (T)0;
+                az.y = (T)0;
+            } else {
+                az = reinterpret_cast<const V2 *>(Az)[k + (long long)g.nz * (ii + (long long)g.sx * jj)];
+            }
         }
     }
     const T alpha = (T)sc->alpha, beta = (T)sc->beta;
-    T dd, qq;
+    const V2 zz = reinterpret_cast<const V2 *>(z)[q];
+    V2 dd, qq;
     if (first) {   // first pass: d = z, q = A d (already halo-mapped), pcg:1351-1405
-        dd = z[t];
+        dd = zz;
         qq = az;
     } else {
-        dd = z[t] + beta * d[t];
-        qq = az + beta * qv[t];
+        const V2 d0 = reinterpret_cast<const V2 *>(d)[q], q0 = reinterpret_cast<const V2 *>(qv)[q];
+        dd.x = zz.x + beta * d0.x;
+        dd.y = zz.y + beta * d0.y;
+        qq.x = az.x + beta * q0.x;
+        qq.y = az.y + beta * q0.y;
     }
-    d[t] = dd;
-    qv[t] = qq;
-    x[t] = x[t] + alpha * dd;
-    r[t] = r[t] - alpha * qq;
+    reinterpret_cast<V2 *>(d)[q] = dd;
+    reinterpret_cast<V2 *>(qv)[q] = qq;
+    V2 xx = reinterpret_cast<const V2 *>(x)[q], rr = reinterpret_cast<const V2 *>(r)[q];
+    xx.x = xx.x + alpha * dd.x;
+    xx.y = xx.y + alpha * dd.y;
+    rr.x = rr.x - alpha * qq.x;
+    rr.y = rr.y - alpha * qq.y;
+    reinterpret_cast<V2 *>(x)[q] = xx;
+    reinterpret_cast<V2 *>(r)[q] = rr;
 }
 
 // ---------------------------------------------------------------------------
@@ -602,6 +660,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
                   double tol, int check, double *err_out, int *niters_out) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
+    const int nbv = M::xcd ? xcd_blocks_for(g.nn) : nbm;   // matvec grid (partials per block)
     const long long np = g.npad;
     // an inner solve (check = 1) whose residual becomes exactly zero has converged; the
     // reference recurrences would divide 0 by 0 there
@@ -648,9 +707,9 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             hipLaunchKernelGGL(k_std_d<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.z,
                                v.d, c->sc);
             CISM_TRY((launch_halo_t<T, false>(c, v.d)));
-            hipLaunchKernelGGL((k_matvec_dots<T, 1, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g,
+            hipLaunchKernelGGL((k_matvec_dots<T, 1, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream, g,
                                A, c->av, v.d, v.q, (const T *)nullptr, c->partials);
-            CISM_TRY(launch_reduce(c, nbm, 1, RED_STD_ETA2));
+            CISM_TRY(launch_reduce(c, nbv, 1, RED_STD_ETA2));
             hipLaunchKernelGGL(k_std_xr<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.d,
                                v.q, v.x, v.r, c->sc);
             if (n % 5 == 0) {
@@ -664,17 +723,17 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
         hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
                            v.x, v.b, v.r, 1, c->partials);
         CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
-        hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
+        hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream, g, A,
                            c->av, v.z, v.Az, v.r, c->partials);
-        CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_FIRST));
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
+        CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_FIRST));
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
                            v.d, v.q, v.x, v.r, c->sc, 1, precond != 0 ? c->av : nullptr);
         for (int n = 2; n <= maxiters && !done; ++n) {
             CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
-            hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream,
+            hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
-            CISM_TRY(launch_reduce(c, nbm, 2, RED_CG_ITER));
-            hipLaunchKernelGGL(k_cg_update<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
+            CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_ITER));
+            hipLaunchKernelGGL(k_cg_update<T>, dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
                                v.d, v.q, v.x, v.r, c->sc, 0, precond != 0 ? c->av : nullptr);
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
@@ -1055,7 +1114,8 @@ int launch_cg_matvec(cism_ctx *c, int fp32) {
     const GridDev &g = c->g;
     const int nb = nblocks_for(g.nn);
     if (fp32)
-        hipLaunchKernelGGL((k_matvec_dots<float, 2, MatSym32>), dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g,
+        hipLaunchKernelGGL((k_matvec_dots<float, 2, MatSym32>), dim3(xcd_blocks_for(g.nn)), dim3(CISM_BLOCK), 0,
+                           c->stream, g,
                            MatSym32{c->A32}, c->av, c->z32, c->Az32, c->r32, c->partials);
     else
         hipLaunchKernelGGL((k_matvec_dots<double, 2, MatFull<double>>), dim3(nb), dim3(CISM_BLOCK), 0,

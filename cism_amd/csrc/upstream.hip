@@ -419,8 +419,8 @@ extern "C" int cism_calcbeta(cism_ctx *c, const cism_basal_physics *bp, const do
         st = CISM_ERR_HIP;
     }
     if (st == CISM_OK) st = sync_scalars(c);
-    hipStreamSynchronize(c->stream);
-    hipFree(tmp);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
     if (st != CISM_OK) return st;
     if (c->sc_host->beta_bad) {
         cism_set_error("Invalid beta value in calcbeta (negative or NaN)");
