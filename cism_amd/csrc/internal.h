@@ -207,7 +207,7 @@ inline int nblocks_for(long long n, int block = CISM_BLOCK) {
 // XCD x take chunks x, x+8, x+16, ... of CISM_XCD_CHUNK consecutive node blocks; the
 // neighbours a node reads from the row before it were then read by the same XCD
 // moments earlier and sit in its L2
-#define CISM_XCD_CHUNK 512
+#define CISM_XCD_CHUNK 128
 inline int xcd_blocks_for(long long n) {
     const long long per = 8LL * CISM_XCD_CHUNK;
     return (int)((nblocks_for(n) + per - 1) / per * per);
