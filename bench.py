@@ -123,7 +123,7 @@ def main(argv=None):
         print(f"[bench] {m}", file=sys.stderr, flush=True)
 
     p, desc, family = workload(a.workload)
-    log(f"workload {a.workload}: {desc}")
+    log(f"workload {a.workload}: {desc}; inputs sha256 {inputs_hash(p)}")
     ctx = G.Context.for_problem(p)
     lu, lv = ctx.loads(p)
     nodes = owned_active_nodes(p)
@@ -284,6 +284,16 @@ def main(argv=None):
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
+
+
+def inputs_hash(p):
+    """short sha256 of the generated fields: the same value on every host means the same
+    problem, bit for bit"""
+    import hashlib
+    h = hashlib.sha256()
+    for a in (p.thck, p.usrf, p.topg, p.beta_internal, p.flwafact):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(workload_name):

@@ -257,7 +257,7 @@ class Problem:
 
 
 def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whichbabc,
-            beta_vertex=None, periodic=True, flwa=None, f_ground=None, **kw) -> Problem:
+            beta_vertex=None, periodic=True, flwa=None, f_ground=None, settled=False, **kw) -> Problem:
     nx, ny = ewn + 2 * NHALO, nsn + 2 * NHALO
     sigma = glide_sigma(nz)
     eus = 0.0
@@ -270,6 +270,8 @@ def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whic
     if flwa is None:
         flwa = np.full((nz - 1, nx, ny), flwa_val, order="F")
     ff = flwafact_of(flwa, nz, nx, ny)
+    if settled:                         # np.power, as settle() explains
+        ff = np.asfortranarray(settle(ff))
     beta = fzeros((nx - 1, ny - 1))
     if whichbabc == HO_BABC_BETA_LARGE:
         beta[:] = 1.0e10
@@ -337,6 +339,14 @@ def ismip_hom_a(ewn=160, nsn=160, nz=21, L=80000.0, A=1.0e-16, thklim=100.0,
                    HO_BABC_NO_SLIP, whichsparse=whichsparse, whichprecond=whichprecond, **kw)
 
 
+def settle(a, decimals=3):
+    """Rounds a generated field to a decimal grid (1 mm for 3 decimals).  numpy's exp, log
+    and power take different SIMD paths on different CPUs and can differ in the last bit;
+    rounding makes the synthetic inputs bit-identical from one host to the next, so a
+    benchmark's iteration counts do not depend on which machine generated its problem."""
+    return np.round(np.asarray(a, dtype=np.float64), decimals)
+
+
 def gaussian_random_field(n1, n2, dx, corr, rng):
     """Periodic Gaussian random field with Gaussian covariance exp(-r^2/(2 corr^2)), unit variance."""
     k1 = np.fft.fftfreq(n1, d=dx) * 2 * math.pi
@@ -355,18 +365,18 @@ def grf_sheet(ewn=512, nsn=512, nz=11, dx=1000.0, seed=1, mean=1500.0, sd=400.0,
               whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA, **kw) -> Problem:
     """Periodic ice sheet: GRF thickness on a flat bed, lognormal beta at vertices (babc 5)."""
     rng = np.random.default_rng(seed)
-    H = mean + sd * gaussian_random_field(ewn, nsn, dx, corr, rng)
+    H = settle(mean + sd * gaussian_random_field(ewn, nsn, dx, corr, rng))
     H = np.maximum(H, 10.0)
     thck = pad_periodic(H)
     topg = fzeros(thck.shape)
     usrf = np.asfortranarray(topg + thck)
-    bo = np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn)))
+    bo = settle(np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn))))
     bv = np.zeros((ewn + 3, nsn + 3))
     bv[2:ewn + 2, 2:nsn + 2] = bo          # owned vertices 3..nx-2 (1-based)
     bv = stag_halo(np.asfortranarray(bv), True)
     return _finish("grf_sheet", ewn, nsn, nz, dx, dx, thck, usrf, topg, thklim, A,
                    HO_BABC_BETA_EXTERNAL, beta_vertex=bv, whichsparse=whichsparse,
-                   whichprecond=whichprecond, **kw)
+                   whichprecond=whichprecond, settled=True, **kw)
 
 
 def slab(ewn=40, nsn=40, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, efvs=2336041.42829,
@@ -453,6 +463,7 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, H_edge
     # cliffs at the ice edge, whose lateral ocean load on a single element per corner
     # drives the solve to 1e5 m/yr corner speeds
     bed = np.where(rr < 1.0, np.maximum(bed, s - (H_edge + (H_max - H_edge) * prof)), bed)
+    s, bed = settle(s), settle(bed)
     H = np.where(rr < 1.0, np.clip(s - bed, 0.0, None), 0.0)
     H_shelf = np.where((rr >= 1.0) & (rr < 1.0 + shelf), 500.0 - 350.0 * (rr - 1.0) / max(shelf, 1e-12), 0.0)
     H = np.where((H <= thklim) & (RHOI * H_shelf < RHOO * (-bed)), H_shelf, H)
@@ -481,12 +492,12 @@ def ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5, H_max=3500.0, H_edge
     usrf = pad_periodic(s)
     ice, floating, _, land, _ = get_masks(thck, topg, 0.0, thklim)
     fg = grounded_fraction_vertex(ice, floating, land)
-    bo = np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn)))
+    bo = settle(np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn))))
     bv = np.zeros((ewn + 3, nsn + 3))
     bv[2:ewn + 2, 2:nsn + 2] = bo
     bv = stag_halo(np.asfortranarray(bv), True)
     return _finish("ice_sheet", ewn, nsn, nz, dx, dx, thck, usrf, topg, thklim, A,
-                   HO_BABC_BETA_EXTERNAL, beta_vertex=bv, f_ground=fg,
+                   HO_BABC_BETA_EXTERNAL, beta_vertex=bv, f_ground=fg, settled=True,
                    whichsparse=whichsparse, whichprecond=whichprecond, **kw)
 
 
