@@ -164,7 +164,11 @@ static int dalloc(T **p, size_t n) {
         cism_set_error("device allocation of %zu bytes failed: %s", sizeof(T) * n, hipGetErrorString(e));
         return CISM_ERR_NOMEM;
     }
+    // hipMemset runs on the null stream, which the context's non-blocking stream does
+    // not wait for: finish it here, or a later upload on the context stream can land
+    // first and be zeroed (the first call on a fresh context then sees wiped inputs)
     HIP_TRY(hipMemset(*p, 0, sizeof(T) * (n ? n : 1)));
+    HIP_TRY(hipDeviceSynchronize());
     return CISM_OK;
 }
 

@@ -433,7 +433,9 @@ static int sg_alloc(float **p, long long n) {
         cism_set_error("surrogate allocation failed: %s", hipGetErrorString(e));
         return CISM_ERR_NOMEM;
     }
+    // null-stream memset: complete it before anything runs on the non-blocking stream
     HIP_TRY(hipMemset(*p, 0, sizeof(float) * (n > 0 ? n : 1)));
+    HIP_TRY(hipDeviceSynchronize());
     return CISM_OK;
 }
 
@@ -528,8 +530,9 @@ extern "C" int cism_surrogate_create(int H, int W, int batch, int device, cism_s
     }
     float ones[SG_OUT];
     for (int c = 0; c < SG_OUT; ++c) ones[c] = 1.f;
-    HIP_TRY(hipMemcpy(s->chw, ones, sizeof ones, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->w, h.data(), sizeof(float) * s->nparams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(s->chw, ones, sizeof ones, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->w, h.data(), sizeof(float) * s->nparams, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
     *out = s;
     return CISM_OK;
@@ -562,15 +565,19 @@ extern "C" double cism_surrogate_last_forward_ms(const cism_surrogate *s) {
     return s ? (double)s->last_forward_ms : -1.0;
 }
 
+// Host transfers of the weights go through the network's own (non-blocking) stream and
+// wait for it, so they are ordered after every kernel launched before them.
 extern "C" int cism_surrogate_get_params(cism_surrogate *s, float *host) {
-    HIP_TRY(hipMemcpy(host, s->w, sizeof(float) * s->nparams, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(host, s->w, sizeof(float) * s->nparams, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return CISM_OK;
 }
 
 extern "C" int cism_surrogate_set_params(cism_surrogate *s, const float *host) {
-    HIP_TRY(hipMemcpy(s->w, host, sizeof(float) * s->nparams, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(s->m, 0, sizeof(float) * s->nparams));
-    HIP_TRY(hipMemset(s->v, 0, sizeof(float) * s->nparams));
+    HIP_TRY(hipMemcpyAsync(s->w, host, sizeof(float) * s->nparams, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemsetAsync(s->m, 0, sizeof(float) * s->nparams, s->stream));
+    HIP_TRY(hipMemsetAsync(s->v, 0, sizeof(float) * s->nparams, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
     return CISM_OK;
 }
@@ -812,6 +819,7 @@ extern "C" int cism_surrogate_train_step(cism_surrogate *s, const float *in, con
 }
 
 extern "C" int cism_surrogate_set_channel_weights(cism_surrogate *s, const float *w24) {
-    HIP_TRY(hipMemcpy(s->chw, w24, sizeof(float) * SG_OUT, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(s->chw, w24, sizeof(float) * SG_OUT, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return CISM_OK;
 }
