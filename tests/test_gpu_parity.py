@@ -255,3 +255,14 @@ def test_symmetry_repair_millions_of_pairs(gpu):
         ctx.check_symmetry_assembled_matrix_3d(p.active_vertex, *Ag)
     for a, b in zip(Ag, Ao):
         assert np.array_equal(a, b)
+
+
+def test_loads_on_a_fresh_full_size_context(gpu):
+    """the first call on a new 1200 x 1400 x 11 context sees its inputs intact: the zero-fill
+    of the new buffers is finished before the call's uploads run (at this size the fill
+    takes long enough to overtake them if it is not)"""
+    p = configs.ice_sheet(seed=5)
+    lo_u, lo_v = O.loads(p)
+    with gpu.Context.for_problem(p) as ctx:
+        lg_u, lg_v = ctx.loads(p)
+    assert np.array_equal(lg_u, lo_u) and np.array_equal(lg_v, lo_v)
