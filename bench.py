@@ -104,6 +104,8 @@ def main(argv=None):
                     help="network weights (default: the workload family's file, if trained)")
     ap.add_argument("--anderson", type=int, default=4,
                     help="Anderson acceleration depth of the fast path's Picard loop (0 off)")
+    ap.add_argument("--inner-rtol", type=float, default=1e-2,
+                    help="relative residual of each FP32 inner solve of the refinement")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -133,8 +135,11 @@ def main(argv=None):
     cap = 500
 
     def fast(u0=None, v0=None):
+        # inner FP32 solves to 1e-2 relative residual per refinement step: chosen on the
+        # held-out ice sheet of seed 7 (1e-3: 1.20 s, 1e-2: 0.87 s, 3e-2: 0.86 s), not on
+        # this workload
         return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
-                                        stop_rel_err=STOP_REL_ERR, inner_rtol=1e-3,
+                                        stop_rel_err=STOP_REL_ERR, inner_rtol=a.inner_rtol,
                                         maxiter_nonlinear=cap, anderson_m=a.anderson)
 
     def plain(u0=None, v0=None, stop=STOP_REL_ERR):
@@ -249,7 +254,7 @@ def main(argv=None):
                    "owned_active_nodes": nodes, "stop_rel_err": STOP_REL_ERR,
                    "learned_guess": weights if pred is not None else None,
                    "parallelism": "single GPU, whole domain resident"},
-        "fast_path": {"seconds": round(fast_s, 4), "anderson_m": a.anderson,
+        "fast_path": {"seconds": round(fast_s, 4), "anderson_m": a.anderson, "inner_rtol": a.inner_rtol,
                       "picard_iters": stats["picard_iters"],
                       "fp32_matvecs": stats["fp32_matvecs"], "fp64_matvecs": stats["fp64_matvecs"],
                       "rel_err": stats["final_rel_err"], "learned_guess": pred is not None,
