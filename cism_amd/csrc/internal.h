@@ -50,7 +50,6 @@ struct DevScal {
     double L2_rhs, err;
     double sums[8];
     int nan_flag, singular_flag;
-    unsigned int unused;
     int zero_stop;   // inner solves: an exactly zero residual ends the iteration (alpha = 0)
     int beta_bad;    // calcbeta met a negative or NaN beta
 };
