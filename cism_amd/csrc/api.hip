@@ -433,7 +433,7 @@ extern "C" int cism_check_symmetry_assembled_matrix_3d(cism_ctx *c, const int32_
     CTX_BEGIN(c);
     CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
     CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
-    CISM_TRY(symmetry_device(c));
+    CISM_TRY(symmetry_device(c, nullptr));
     CISM_TRY(sync_scalars(c));
     if (nwarn) *nwarn = (int64_t)c->sc_host->sums[0];
     return download_matrix(c, Auu, Auv, Avu, Avv);
@@ -618,8 +618,9 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         CISM_TRY(dirichlet_device(c));
         CISM_TRY(launch_halo_mat(c, c->A));
         CISM_TRY(launch_halo_vec(c, c->b));
-        CISM_TRY(symmetry_device(c));
-        CISM_TRY(count_nonzeros_device(c));
+        // the mixed-precision path gets its FP32 symmetric matrix from the same pass
+        if (o->precision == 1) CISM_TRY(ensure_a32(c));
+        CISM_TRY(symmetry_device(c, o->precision == 1 ? c->A32 : nullptr));
         CISM_TRY(sync_scalars(c));
         if (c->sc_host->singular_flag) {
             cism_set_error("Jacobian matrix is singular");
@@ -629,7 +630,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             cism_set_error("Invalid beta value in calcbeta (negative or NaN)");
             return CISM_ERR_NAN;
         }
-        const double nnz = c->sc_host->sums[0];
+        const double nnz = c->sc_host->sums[1];
         t_asm += now_s() - ta;
         if (nnz == 0.0) {
             zero_exit = true;
@@ -651,7 +652,6 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         }
         if (o->precision == 1) {
             int n64 = 0, n32 = 0;
-            CISM_TRY(convert_a32(c));
             st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
                               &n64, &n32);
             S.fp64_matvecs += n64;

@@ -630,12 +630,18 @@ int dirichlet_device(cism_ctx *c) {
 // reads or writes those coefficients (a lane reads only its own slots from CISM_DIAG
 // on and, of other nodes, the lower partner slots of its own pairs and the diagonal
 // Auu/Avv, which no repair touches), so the pass is in place, race-free and
-// independent of the launch.  partials: per-block count of visits that find an
-// asymmetry above the threshold (the reference's warnings), on the unmodified matrix.
+// independent of the launch.  partials, per block: [0] the visits that find an
+// asymmetry above the threshold (the reference's warnings, on the unmodified matrix),
+// [1] the nonzero coefficients of owned active rows after the repair
+// (count_nonzeros_3d, velo_higher:9355-9410; a row's coefficients before its diagonal
+// are counted by the lane that owns their pair).  As: when not null, the lane also
+// writes its node's FP32 symmetric storage (slots CISM_DIAG .. 26, sidx) from the
+// repaired values, which saves the mixed-precision path a separate conversion pass.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__restrict__ av,
-                                                    double *__restrict__ A, double *__restrict__ partials) {
-    double acc[1] = {0.0};
+                                                    double *__restrict__ A, float *__restrict__ As,
+                                                    double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < g.nn) {
         const int k = (int)(q % g.nz);
@@ -643,7 +649,18 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         const bool vis_q = is_owned(g, i, j) && av[col];
         const double dgu = A[aidx(0, CISM_DIAG, q)], dgv = A[aidx(3, CISM_DIAG, q)];
-        double warn = 0.0;
+        double warn = 0.0, nnz = 0.0;
+        auto count4 = [&](long long e0, long long e2) {   // four blocks: (b0, b1) at e0, (b2, b3) at e2
+            return (double)((A[e0] != 0.0) + (A[e0 + 1] != 0.0) + (A[e2] != 0.0) + (A[e2 + 1] != 0.0));
+        };
+        auto store32 = [&](int m) {
+            if (!As) return;
+            const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, m, q));
+            const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
+            float4 o;
+            o.x = (float)a.x; o.y = (float)a.y; o.z = (float)b.x; o.w = (float)b.y;
+            *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) = o;
+        };
         // one coefficient pair (e1, e2) with values v1, v2; qualification from each side
         auto pair = [&](long long e1, long long e2, double v1, double v2, bool va, double da, bool vb,
                         double db) {
@@ -671,16 +688,24 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
                     A[e2] = avg;
                 }
             }
+            if (vis_q) nnz += count4(aidx(0, CISM_DIAG, q), aidx(2, CISM_DIAG, q));
+            store32(CISM_DIAG);
         }
         for (int m = CISM_DIAG + 1; m < 27; ++m) {
             const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-            if (k + kA < 0 || k + kA >= g.nz || i + iA < 0 || i + iA >= g.sx || j + jA < 0 || j + jA >= g.sy)
+            if (k + kA < 0 || k + kA >= g.nz || i + iA < 0 || i + iA >= g.sx || j + jA < 0 || j + jA >= g.sy) {
+                if (vis_q) nnz += count4(aidx(0, m, q), aidx(2, m, q));
+                store32(m);
                 continue;
+            }
             const int mm = 26 - m;
             const long long qc = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
             const long long cc = col + iA + (long long)g.sx * jA;
             const bool vis_c = is_owned(g, i + iA, j + jA) && av[cc];
-            if (!vis_q && !vis_c) continue;
+            if (!vis_q && !vis_c) {
+                store32(m);
+                continue;
+            }
             const double cgu = A[aidx(0, CISM_DIAG, qc)], cgv = A[aidx(3, CISM_DIAG, qc)];
             double a0, a1, a2, a3, b0, b1, b2, b3;
             load_blocks(A, q, m, a0, a1, a2, a3);
@@ -689,17 +714,23 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
             pair(aidx(1, m, q), aidx(2, mm, qc), a1, b2, vis_q, dgu, vis_c, cgv);   // Auv ~ Avu
             pair(aidx(2, m, q), aidx(1, mm, qc), a2, b1, vis_q, dgv, vis_c, cgu);   // Avu ~ Auv
             pair(aidx(3, m, q), aidx(3, mm, qc), a3, b3, vis_q, dgv, vis_c, cgv);   // Avv ~ Avv
+            if (vis_q) nnz += count4(aidx(0, m, q), aidx(2, m, q));
+            if (vis_c) nnz += count4(aidx(0, mm, qc), aidx(2, mm, qc));
+            store32(m);
         }
         acc[0] = warn;
+        acc[1] = nnz;
     }
-    block_reduce_store<1>(acc, partials);
+    block_reduce_store<2>(acc, partials);
 }
 
-int symmetry_device(cism_ctx *c) {
+// symmetry repair; sums[0] = warnings, sums[1] = nonzeros of owned active rows.  As: the
+// FP32 symmetric storage to write as well, or null.
+int symmetry_device(cism_ctx *c, float *As) {
     const int nb = nblocks_for(c->g.nn);
-    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->partials);
+    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, As, c->partials);
     HIP_TRY(hipGetLastError());
-    CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
+    CISM_TRY(launch_reduce(c, nb, 2, RED_SUMS));
     return CISM_OK;
 }
 

@@ -279,7 +279,6 @@ int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric
 int anderson_reset(cism_ctx *c);
 int anderson_step(cism_ctx *c, int m, double *fnorm);
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
-int convert_a32(cism_ctx *c);
 int launch_cg_matvec(cism_ctx *c, int fp32);
 int rel_err_device(cism_ctx *c, double *rel);
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol,
@@ -290,7 +289,7 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);
 int dirichlet_device(cism_ctx *c);
-int symmetry_device(cism_ctx *c);
+int symmetry_device(cism_ctx *c, float *As);
 int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
 int residual_vector_device(cism_ctx *c, double *ru, double *rv);  // sums[0]=|r|^2, [1]=|b|^2
 int residual_velocity_device(cism_ctx *c, int whichresid);       // sums[0]

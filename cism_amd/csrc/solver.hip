@@ -795,33 +795,10 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_refine(long long n2, const float
 
 __global__ void k_save_scale(DevScal *sc) { sc->sums[2] = sqrt(sc->sums[0]); }
 
-// FP32 symmetric storage from the FP64 matrix: slot s of node q takes the four
-// blocks of slot CISM_DIAG + s; one lane per (node, slot), in destination order
-__global__ __launch_bounds__(CISM_BLOCK) void k_to_sym32(long long n, const double *__restrict__ A,
-                                                         float *__restrict__ As) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const long long tile = t / (64 * CISM_SYM_SLOTS);
-    const int s = (int)((t >> 6) % CISM_SYM_SLOTS);
-    const long long q = tile * 64 + (t & 63);
-    const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, CISM_DIAG + s, q));
-    const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, CISM_DIAG + s, q));
-    float4 o;
-    o.x = (float)a.x; o.y = (float)a.y; o.z = (float)b.x; o.w = (float)b.y;
-    *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = o;
-}
-
-int convert_a32(cism_ctx *c) {
-    CISM_TRY(ensure_a32(c));
-    const long long n = (long long)CISM_SYM_SLOTS * c->g.npad;
-    hipLaunchKernelGGL(k_to_sym32, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, n, c->A, c->A32);
-    HIP_TRY(hipGetLastError());
-    return CISM_OK;
-}
-
 // Iterative refinement: r = b - A64 x (FP64), solve A32 e = r/|r| with FP32
 // Chron-Gear to relative tolerance inner_rtol, x += |r| e, until the FP64
-// relative residual is below tol.  The FP32 matrix must be current (convert_a32).
+// relative residual is below tol.  The FP32 matrix must be current: the symmetry pass
+// of the Picard loop writes it (symmetry_device with c->A32).
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol, double *err_out,
                  int *niters_out, int *n64, int *n32) {
     const GridDev &g = c->g;
