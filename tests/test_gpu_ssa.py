@@ -58,10 +58,11 @@ def test_ssa_picard_fp64(case, sparse):
     assert rel_max(bg, bo) < 1e-6
 
 
-def test_ssa_slab_closed_form(case):
-    name, p, lu, lv, _ = case
-    if not name.startswith("slab"):
-        pytest.skip("closed form only for the sliding slab")
+@pytest.mark.parametrize("name", ["slab", "slab_efvs2"])
+def test_ssa_slab_closed_form(name):
+    """the sliding slab has the closed form u = rho g H tan(alpha) / beta"""
+    p = problems()[name]
+    lu, lv = O.loads(p)
     with G.Context.for_problem_2d(p) as ctx:
         rg = ctx.velo_higher_solve_2d(p, lu, lv)
     ua = 910.0 * 9.81 * 1000.0 * math.tan(math.radians(0.5)) / 500.0
