@@ -196,6 +196,12 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     g.npad = (g.nn + 255) / 256 * 256;
     g.ne = (long long)(nz - 1) * nx * ny;
     g.nepad = (g.ne + 255) / 256 * 256;
+    if (g.npad > (1LL << 30) || g.nepad > (1LL << 30)) {
+        // the assembly kernels index nodes and elements with 32-bit integers
+        free(c);
+        cism_set_error("grid too large: %lld nodes, %lld elements (limit 2^30 each)", g.nn, g.ne);
+        return CISM_ERR_ARG;
+    }
     c->device = device;
     c->maxiters = 200;
     c->tolerance = 1.0e-8;
@@ -637,7 +643,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             L2 = 0.0;
             break;
         }
-        CISM_TRY(residual_vector_device(c, c->t, nullptr));
+        // the mixed path takes the first residual of its refinement from this pass
+        const bool share_r = o->precision == 1;
+        if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
+        CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));
         CISM_TRY(sync_scalars(c));
         L2 = sqrt(c->sc_host->sums[0]);
         const double rhs = sqrt(c->sc_host->sums[1]);
@@ -653,7 +662,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         if (o->precision == 1) {
             int n64 = 0, n32 = 0;
             st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
-                              &n64, &n32);
+                              &n64, &n32, 1);
             S.fp64_matvecs += n64;
             S.fp32_matvecs += n32;
             S.refinement_steps += n64;

@@ -3,12 +3,17 @@
 // (glissade_velo_higher.F90).
 //
 // Assembly is split in two atomic-free kernels:
-//  1. k_element: one workgroup = 32 hexahedral elements x 8 quadrature points.
-//     Each lane evaluates one quadrature point (Jacobian, inverse, basis
-//     derivatives, Glen-law viscosity) into LDS; then all lanes form the 136
-//     distinct entries of the element's 8x8 blocks (Kuu, Kvv symmetric; Kvu =
-//     Kuv^T, both exactly, since every product is formed from the same factors)
-//     and store them element-fastest to a scratch array.
+//  1. k_element: one workgroup = 32 hexahedral elements x 8 lanes.  Lane p
+//     evaluates quadrature point p (Jacobian, inverse, basis derivatives,
+//     Glen-law viscosity) into LDS; then lane a sums over the points the rows of
+//     node a of the five point-weighted products Pxx = sum_p f x_a x_b, Pyy,
+//     Pzz, Pxy = sum_p f x_a y_b and Pyx = sum_p f y_a x_b, reading the other
+//     nodes' derivatives as broadcasts, and forms from them its row of Kuu and
+//     Kvv (the upper part: b >= a) and of Kuv.  Kuu = 4 Pxx + Pyy + Pzz,
+//     Kvv = Pxx + 4 Pyy + Pzz and Kuv = 2 Pxy + Pyx are the reference's sums
+//     over points of f (4 x_a x_b + y_a y_b + z_a z_b) etc. regrouped, so the
+//     entries agree with it to rounding.  Kvu = Kuv^T is not stored.  The 136
+//     entries go element-fastest to a scratch array.
 //  2. k_gather: one lane per node sums, slot by slot, the entries of the (up to
 //     8) elements that share the node, in the reference's element loop order
 //     (j, i, k ascending), so every coefficient is the reference's sum in the
@@ -40,20 +45,18 @@ __host__ __device__ inline long long keidx(int ent, long long e) {
     return (((e >> 6) * CISM_KE_ENTRIES + ent) << 6) + (e & 63);
 }
 
-// Per-element LDS records are padded to an odd number of doubles so that the
-// 32 elements a wave reads side by side fall in 32 different bank pairs.
-#define QP_STRIDE (8 * 8 + 1)
+// LDS record of quadrature point p of an element: the basis derivatives x[8],
+// y[8], z[8] of its nodes, then the weight f = efvs det / vol0 and one pad, so
+// each record and each node pair in it is 16-byte aligned.  An element's eight
+// records are padded to 210 doubles: the eight elements a wave reads side by
+// side then start in eight disjoint groups of four banks.
+#define QP_REC 26
+#define EL_REC (8 * QP_REC + 2)
 struct QPLds {
-    double dx[EL_PER_WG][QP_STRIDE], dy[EL_PER_WG][QP_STRIDE], dz[EL_PER_WG][QP_STRIDE];  // [el][p*8+n]
-    double f[EL_PER_WG][9], ev[EL_PER_WG][9];
+    alignas(16) double rec[EL_PER_WG][EL_REC];
+    double ev[EL_PER_WG][9];
     double nodes[EL_PER_WG][8 * 5 + 1];                                                  // x y z u v
 };
-
-// packed upper-triangle entry t (0..35) -> (a, b), a <= b
-__constant__ unsigned char c_tri_a[36] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2,
-                                          2, 2, 2, 3, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 6, 6, 7};
-__constant__ unsigned char c_tri_b[36] = {0, 1, 2, 3, 4, 5, 6, 7, 1, 2, 3, 4, 5, 6, 7, 2, 3, 4,
-                                          5, 6, 7, 3, 4, 5, 6, 7, 4, 5, 6, 7, 5, 6, 7, 6, 7, 7};
 
 // One quadrature point p of a hexahedral element whose 8 nodes are given as
 // (x, y, z, u, v) records: basis-function derivatives (get_basis_function_
@@ -102,10 +105,11 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
     double Ji[3][3];
     singular = !(fabs(det) > 0.0);
     if (!singular) {
+        const double rdet = 1.0 / det;
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-            for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = cof[cc][rr] / det;
+            for (int rr = 0; rr < 3; ++rr) Ji[rr][cc] = cof[cc][rr] * rdet;
     } else {
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc)
@@ -147,6 +151,7 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
     }
 }
 
+template <int APPROX>
 __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__restrict__ sigma,
                                                  const int32_t *__restrict__ ac,
                                                  const double *__restrict__ xV,
@@ -155,32 +160,39 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
                                                  const double *__restrict__ susrf,
                                                  const double *__restrict__ sthck,
                                                  const double *__restrict__ flwafact,
-                                                 int whichapprox, int whichefvs,
-                                                 double efvs_constant, double *__restrict__ Ke,
-                                                 double *__restrict__ efvs,
+                                                 int whichefvs, double efvs_constant,
+                                                 double *__restrict__ Ke, double *__restrict__ efvs,
                                                  DevScal *__restrict__ sc) {
     __shared__ QPLds L;
     __shared__ int valid[EL_PER_WG];
     const int el = threadIdx.x >> 3, sub = threadIdx.x & 7;
-    const long long e = (long long)blockIdx.x * EL_PER_WG + el;
+    // element and node indices fit in 32 bits (the context rejects larger grids)
+    const int e = blockIdx.x * EL_PER_WG + el;
     const int nzm = g.nz - 1;
     int k = 0, ic = 0, jc = 0;
-    bool ok = e < g.ne;
+    bool ok = e < (int)g.ne;
     if (ok) {
-        k = (int)(e % nzm) + 1;                 // 1-based layer
-        const long long cell = e / nzm;
-        ic = (int)(cell % g.nx) + 1;            // 1-based cell
-        jc = (int)(cell / g.nx) + 1;
+        const int cell = e / nzm;
+        k = e - cell * nzm + 1;                 // 1-based layer
+        jc = cell / g.nx;
+        ic = cell - jc * g.nx + 1;              // 1-based cell
+        jc += 1;
         ok = ic >= CISM_NHALO + 1 && ic <= g.nx - CISM_NHALO + 1 && jc >= CISM_NHALO + 1 &&
-             jc <= g.ny - CISM_NHALO + 1 && ac[(ic - 1) + (long long)g.nx * (jc - 1)] != 0;
+             jc <= g.ny - CISM_NHALO + 1 && ac[cell] != 0;
     }
     if (sub == 0) valid[el] = ok;
+    // a group with no active element only clears its efvs
+    if (!__syncthreads_or(ok)) {
+        if (threadIdx.x < EL_PER_WG && blockIdx.x * EL_PER_WG + (int)threadIdx.x < (int)g.ne)
+            efvs[blockIdx.x * EL_PER_WG + threadIdx.x] = 0.0;
+        return;
+    }
     // phase 0: lane (el, n) gathers node n (velo_higher:5100-5127)
     if (ok) {
         const int n = sub;
         const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
-        const long long v = (in - 1) + (long long)g.sx * (jn - 1);
-        const long long qn = (kn - 1) + (long long)g.nz * v;
+        const int v = (in - 1) + g.sx * (jn - 1);
+        const int qn = (kn - 1) + g.nz * v;
         L.nodes[el][n * 5 + 0] = xV[v];
         L.nodes[el][n * 5 + 1] = yV[v];
         L.nodes[el][n * 5 + 2] = susrf[v] - sigma[kn - 1] * sthck[v];
@@ -191,70 +203,73 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
     // phase 1: lane (el, p) evaluates quadrature point p (velo_higher:6360-6579, 7199-7378)
     if (ok) {
         const int p = sub;
+        double *r = &L.rec[el][p * QP_REC];
         double det, ev, st[6];
         bool singular;
-        eval_qp(p, L.nodes[el], whichapprox, whichefvs, efvs_constant, flwafact[e], &L.dx[el][p * 8],
-                &L.dy[el][p * 8], &L.dz[el][p * 8], det, ev, st, singular);
+        eval_qp(p, L.nodes[el], APPROX, whichefvs, efvs_constant, flwafact[e], r, r + 8, r + 16, det, ev,
+                st, singular);
         if (singular) atomicOr(&sc->singular_flag, 1);
         L.ev[el][p] = ev;
-        L.f[el][p] = ev * 1.0 * det / CISM_VOL0;
+        r[24] = ev * 1.0 * det / CISM_VOL0;
     }
     __syncthreads();
     // element efvs = mean over quadrature points (velo_higher:5167-5172)
     if (threadIdx.x < EL_PER_WG) {
-        const long long ee = (long long)blockIdx.x * EL_PER_WG + threadIdx.x;
-        if (ee < g.ne) {
+        const int ee = blockIdx.x * EL_PER_WG + threadIdx.x;
+        if (ee < (int)g.ne) {
             double s = 0.0;
             if (valid[threadIdx.x])
                 for (int p = 0; p < 8; ++p) s = s + L.ev[threadIdx.x][p];
             efvs[ee] = valid[threadIdx.x] ? s / 8 : 0.0;
         }
     }
-    // phase 2: entries, element-fastest (compute_element_matrix, velo_higher:7960-7982).
-    // Task tt < 36: the pair (a, b), a <= b, of the upper triangles of Kuu and Kvv, which
-    // share their six derivative reads; task 36 + tt: Kuv(a, b) and Kuv(b, a), which share
-    // four.  Every entry keeps its own sum over the points in point order.
-    for (int t = threadIdx.x; t < 72 * EL_PER_WG; t += blockDim.x) {
-        const int task = t / EL_PER_WG, le = t % EL_PER_WG;
-        const long long ee = (long long)blockIdx.x * EL_PER_WG + le;
-        if (ee >= g.ne || !valid[le]) continue;
-        const int tt = task < 36 ? task : task - 36;
-        const int a = c_tri_a[tt], bb = c_tri_b[tt];
-        double acc0 = 0.0, acc1 = 0.0;
-        if (task < 36) {
-            for (int p = 0; p < 8; ++p) {
-                const double f = L.f[le][p];
-                const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8], *Z = &L.dz[le][p * 8];
-                const double xa = X[a], xb = X[bb], ya = Y[a], yb = Y[bb], za = Z[a], zb = Z[bb];
-                double tu, tv;
-                if (whichapprox == 0) {
-                    tu = za * zb;
-                    tv = tu;
-                } else if (whichapprox == 1) {
-                    tu = 4.0 * xa * xb + ya * yb;
-                    tv = 4.0 * ya * yb + xa * xb;
-                } else {
-                    tu = 4.0 * xa * xb + ya * yb + za * zb;
-                    tv = 4.0 * ya * yb + xa * xb + za * zb;
-                }
-                acc0 = acc0 + f * tu;
-                acc1 = acc1 + f * tv;
+    if (!ok) return;
+    // phase 2: lane (el, a) sums row a of the point-weighted products over the
+    // points, in point order (compute_element_matrix, velo_higher:7960-7982)
+    const int a = sub;
+    double Pxx[8], Pyy[8], Pzz[8], Pxy[8], Pyx[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) Pxx[b] = Pyy[b] = Pzz[b] = Pxy[b] = Pyx[b] = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 8; ++p) {
+        const double *r = &L.rec[el][p * QP_REC];
+        const double f = r[24];
+        const double fx = f * r[a], fy = f * r[8 + a], fz = f * r[16 + a];
+#pragma unroll
+        for (int b = 0; b < 8; b += 2) {
+            if (APPROX != 0) {
+                const double2 X = *reinterpret_cast<const double2 *>(r + b);
+                const double2 Y = *reinterpret_cast<const double2 *>(r + 8 + b);
+                Pxx[b] = fma(fx, X.x, Pxx[b]);  Pxx[b + 1] = fma(fx, X.y, Pxx[b + 1]);
+                Pyy[b] = fma(fy, Y.x, Pyy[b]);  Pyy[b + 1] = fma(fy, Y.y, Pyy[b + 1]);
+                Pxy[b] = fma(fx, Y.x, Pxy[b]);  Pxy[b + 1] = fma(fx, Y.y, Pxy[b + 1]);
+                Pyx[b] = fma(fy, X.x, Pyx[b]);  Pyx[b + 1] = fma(fy, X.y, Pyx[b + 1]);
             }
-            Ke[keidx(tt, ee)] = acc0;
-            Ke[keidx(36 + tt, ee)] = acc1;
-        } else {
-            // Kuv(nr, nc) = sum f (2 X[nr] Y[nc] + Y[nr] X[nc]); nr = a, nc = bb and swapped
-            if (whichapprox != 0)
-                for (int p = 0; p < 8; ++p) {
-                    const double f = L.f[le][p];
-                    const double *X = &L.dx[le][p * 8], *Y = &L.dy[le][p * 8];
-                    const double xa = X[a], xb = X[bb], ya = Y[a], yb = Y[bb];
-                    acc0 = acc0 + f * (2.0 * xa * yb + ya * xb);
-                    acc1 = acc1 + f * (2.0 * xb * ya + yb * xa);
-                }
-            Ke[keidx(72 + a + 8 * bb, ee)] = acc0;
-            if (a != bb) Ke[keidx(72 + bb + 8 * a, ee)] = acc1;
+            if (APPROX != 1) {
+                const double2 Z = *reinterpret_cast<const double2 *>(r + 16 + b);
+                Pzz[b] = fma(fz, Z.x, Pzz[b]);  Pzz[b + 1] = fma(fz, Z.y, Pzz[b + 1]);
+            }
         }
+    }
+    double *kcol = Ke + keidx(0, e);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (b >= a) {
+            double ku, kv;
+            if (APPROX == 0) {
+                ku = kv = Pzz[b];
+            } else if (APPROX == 1) {
+                ku = 4.0 * Pxx[b] + Pyy[b];
+                kv = Pxx[b] + 4.0 * Pyy[b];
+            } else {
+                ku = 4.0 * Pxx[b] + Pyy[b] + Pzz[b];
+                kv = Pxx[b] + 4.0 * Pyy[b] + Pzz[b];
+            }
+            const int t = tri_index(a, b);
+            kcol[t << 6] = ku;
+            kcol[(36 + t) << 6] = kv;
+        }
+        kcol[(72 + a + 8 * b) << 6] = APPROX == 0 ? 0.0 : 2.0 * Pxy[b] + Pyx[b];
     }
 }
 
@@ -444,46 +459,70 @@ int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric
     return CISM_OK;
 }
 
-// value of K_block(nr, nc) of element e from the packed scratch
-__device__ inline double ke_val(const double *__restrict__ Ke, int blk, int nr, int nc,
-                                long long e) {
-    int ent;
-    if (blk == 0) ent = tri_index(nr, nc);
-    else if (blk == 3) ent = 36 + tri_index(nr, nc);
-    else if (blk == 1) ent = 72 + nr + 8 * nc;   // Kuv(nr, nc)
-    else ent = 72 + nc + 8 * nr;                 // Kvu(nr, nc) = Kuv(nc, nr)
-    return Ke[keidx(ent, e)];
+// A scratch column of zeros (the stride of keidx): the gather reads it in place of
+// an element that is not there, so every one of its loads is unconditional and
+// they can all be in flight together.  Adding +0.0 leaves every sum unchanged.
+static __device__ double ke_zero[CISM_KE_ENTRIES * 64];
+
+// entry ent of the element whose scratch column starts at base (base = Ke + keidx(0, e))
+__device__ __forceinline__ double ke_at(const double *__restrict__ base, int ent) {
+    return base[ent << 6];
+}
+
+// value of K_block(nr, nc) of an element from its scratch column
+__device__ __forceinline__ double ke_val(const double *__restrict__ base, int blk, int nr, int nc) {
+    if (blk == 0) return ke_at(base, tri_index(nr, nc));
+    if (blk == 3) return ke_at(base, 36 + tri_index(nr, nc));
+    if (blk == 1) return ke_at(base, 72 + nr + 8 * nc);   // Kuv(nr, nc)
+    return ke_at(base, 72 + nc + 8 * nr);                 // Kvu(nr, nc) = Kuv(nc, nr)
 }
 
 // One lane per node (k, i, j): A(m, node) = sum over elements in (j, i, k) order
 // (element_to_global_matrix_3d, velo_higher:7990-8054), then the local basal
-// friction term (basal_sliding_bc LOCAL, velo_higher:8216-8232).
+// friction term (basal_sliding_bc LOCAL, velo_higher:8216-8232).  A node with no
+// active element around it gets zero rows.  Node and element indices fit in 32
+// bits (the context rejects larger grids).
 __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__restrict__ ac,
                                                 const int32_t *__restrict__ av,
                                                 const double *__restrict__ Ke,
                                                 const double *__restrict__ beta, int add_basal,
                                                 double basal_scale, double *__restrict__ A) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nn) return;
-    const int k = (int)(q % g.nz) + 1;                // 1-based
-    const long long col = q / g.nz;
-    const int i = (int)(col % g.sx) + 1, j = (int)(col / g.sx) + 1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int)g.nn) return;
+    const int k = q % g.nz + 1;                // 1-based
+    const int col = q / g.nz;
+    const int i = col % g.sx + 1, j = col / g.sx + 1;
+    const int nzm = g.nz - 1;
     // candidate elements: cell (i+dI, j+dJ), layer k-1+dK
-    long long eid[2][2][2];
-    bool ev[2][2][2];
+    const double *base[2][2][2];
+    bool any = false;
 #pragma unroll
     for (int dJ = 0; dJ < 2; ++dJ)
 #pragma unroll
-        for (int dI = 0; dI < 2; ++dI)
+        for (int dI = 0; dI < 2; ++dI) {
+            const int icl = i + dI, jcl = j + dJ;
+            const bool okc = icl >= CISM_NHALO + 1 && icl <= g.nx - CISM_NHALO + 1 &&
+                             jcl >= CISM_NHALO + 1 && jcl <= g.ny - CISM_NHALO + 1;
+            const int cell = (icl - 1) + g.nx * (jcl - 1);
+            const bool act = okc && ac[cell] != 0;
 #pragma unroll
             for (int dK = 0; dK < 2; ++dK) {
-                const int icl = i + dI, jcl = j + dJ, ke = k - 1 + dK;
-                const bool okc = icl >= CISM_NHALO + 1 && icl <= g.nx - CISM_NHALO + 1 &&
-                                 jcl >= CISM_NHALO + 1 && jcl <= g.ny - CISM_NHALO + 1 && ke >= 1 &&
-                                 ke <= g.nz - 1;
-                ev[dJ][dI][dK] = okc && ac[(icl - 1) + (long long)g.nx * (jcl - 1)] != 0;
-                eid[dJ][dI][dK] = okc ? (ke - 1) + (long long)(g.nz - 1) * ((icl - 1) + (long long)g.nx * (jcl - 1)) : 0;
+                const int ke = k - 1 + dK;
+                const bool ok = act && ke >= 1 && ke <= nzm;
+                any = any || ok;
+                base[dJ][dI][dK] = ok ? Ke + keidx(0, (ke - 1) + (long long)nzm * cell) : ke_zero;
             }
+        }
+    const bool basal = add_basal && k == g.nz && av[col];
+    if (!any && !basal) {
+        const double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < 27; ++m) {
+            *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = z;
+            *reinterpret_cast<double2 *>(A + aidx(2, m, q)) = z;
+        }
+        return;
+    }
     // slot by slot, the four blocks together: each slot's coefficients leave as two
     // 16-byte stores (the tile keeps a node's four blocks of one slot adjacent)
 #pragma unroll
@@ -501,11 +540,10 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                         const int ix = 1 - dI, jy = 1 - dJ, kl = 1 - dK;
                         const int cx = ix + iA, cy = jy + jA, cl = kl + kA;
                         if (cx < 0 || cx > 1 || cy < 0 || cy > 1 || cl < 0 || cl > 1) continue;
-                        if (!ev[dJ][dI][dK]) continue;
-                        acc[blk] = acc[blk] + ke_val(Ke, blk, node_id(ix, jy, kl), node_id(cx, cy, cl),
-                                                     eid[dJ][dI][dK]);
+                        acc[blk] = acc[blk] + ke_val(base[dJ][dI][dK], blk, node_id(ix, jy, kl),
+                                                     node_id(cx, cy, cl));
                     }
-            if (add_basal && m == CISM_DIAG && (blk == 0 || blk == 3) && k == g.nz && av[col])
+            if (basal && m == CISM_DIAG && (blk == 0 || blk == 3))
                 acc[blk] = acc[blk] + basal_scale * beta[col];
         }
         *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = make_double2(acc[0], acc[1]);
@@ -518,9 +556,9 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
     CISM_TRY(ensure_ke(c));
     const GridDev &g = c->g;
     const int nb = (int)((g.ne + EL_PER_WG - 1) / EL_PER_WG);
-    hipLaunchKernelGGL(k_element, dim3(nb), dim3(256), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV,
-                       c->x, c->susrf, c->sthck, c->flwafact, whichapprox, whichefvs, efvs_constant,
-                       c->Ke, c->efvs, c->sc);
+    auto kern = whichapprox == 0 ? k_element<0> : whichapprox == 1 ? k_element<1> : k_element<2>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV, c->x,
+                       c->susrf, c->sthck, c->flwafact, whichefvs, efvs_constant, c->Ke, c->efvs, c->sc);
     HIP_TRY(hipGetLastError());
     const double basal_scale = dx * dy / CISM_VOL0;
     hipLaunchKernelGGL(k_gather, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, c->ac, c->av,
@@ -761,47 +799,89 @@ int count_nonzeros_device(cism_ctx *c) {
 }
 
 // compute_residual_vector_3d (velo_higher:8768-8927): r = A x - b on owned
-// active rows (0 elsewhere); sums[0] = |r|^2, sums[1] = |b|^2 over the same rows
+// active rows (0 elsewhere); sums[0] = |r|^2, sums[1] = |b|^2 over the same rows.
+// With rs (the first residual of the iterative refinement, which needs the same
+// product): rs = b - A x on the whole grid, the negation of r on owned active
+// rows, b on owned inactive rows, and on halo rows the value of their periodic
+// image (0 without one); sums[2] = |rs|^2 and sums[3] = |b|^2 over owned rows.
+template <bool RS>
 __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__restrict__ av,
                                                 const double *__restrict__ A, const double *__restrict__ x,
                                                 const double *__restrict__ b, double *__restrict__ r,
-                                                double *__restrict__ partials) {
-    double acc[2] = {0.0, 0.0};
+                                                double *__restrict__ rs, double *__restrict__ partials) {
+    constexpr int NV = RS ? 4 : 2;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long cst = g.nz, rst = (long long)g.nz * g.sx;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
-        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        const bool own = is_owned(g, i, j);
+        long long qs = q;     // row whose product this lane forms
+        bool none = false;    // a halo row without an image
+        if (RS && !own) {
+            const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
+            if (ii < 0 || jj < 0) none = true;
+            else { i = ii; j = jj; qs = k + (long long)g.nz * (i + (long long)g.sx * j); }
+        } else if (!own) {
+            none = true;
+        }
         double ru = 0.0, rv = 0.0;
-        if (is_owned(g, i, j) && av[col]) {
+        bool act = false;
+        if (!none && av[qs / g.nz]) {
+            act = true;
             double su = 0.0, sv = 0.0;
             for (int m = 0; m < 27; ++m) {
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
                 if (k + kA < 0 || k + kA >= g.nz) continue;
-                const long long qn = q + kA + iA * cst + jA * rst;
+                const long long qn = qs + kA + iA * cst + jA * rst;
                 const double xu = x[2 * qn], xv = x[2 * qn + 1];
-                su = su + A[aidx(0, m, q)] * xu + A[aidx(1, m, q)] * xv;
-                sv = sv + A[aidx(2, m, q)] * xu + A[aidx(3, m, q)] * xv;
+                su = su + A[aidx(0, m, qs)] * xu + A[aidx(1, m, qs)] * xv;
+                sv = sv + A[aidx(2, m, qs)] * xu + A[aidx(3, m, qs)] * xv;
             }
-            ru = su - b[2 * q];
-            rv = sv - b[2 * q + 1];
-            acc[0] += ru * ru + rv * rv;
-            acc[1] += b[2 * q] * b[2 * q] + b[2 * q + 1] * b[2 * q + 1];
+            ru = su - b[2 * qs];
+            rv = sv - b[2 * qs + 1];
         }
-        r[2 * q] = ru;
-        r[2 * q + 1] = rv;
+        if (own) {
+            r[2 * q] = ru;
+            r[2 * q + 1] = rv;
+            const double bu = b[2 * q], bv = b[2 * q + 1];
+            if (act) {
+                acc[0] += ru * ru + rv * rv;
+                acc[1] += bu * bu + bv * bv;
+            }
+            if (RS) {
+                const double su = act ? -ru : bu, sv = act ? -rv : bv;
+                rs[2 * q] = su;
+                rs[2 * q + 1] = sv;
+                acc[NV - 2] += su * su + sv * sv;
+                acc[NV - 1] += bu * bu + bv * bv;
+            }
+        } else {
+            r[2 * q] = 0.0;
+            r[2 * q + 1] = 0.0;
+            if (RS) {
+                rs[2 * q] = none ? 0.0 : act ? -ru : b[2 * qs];
+                rs[2 * q + 1] = none ? 0.0 : act ? -rv : b[2 * qs + 1];
+            }
+        }
     }
-    block_reduce_store<2>(acc, partials);
+    block_reduce_store<NV>(acc, partials);
 }
 
-int residual_vector_device(cism_ctx *c, double *ru, double *rv) {
-    (void)rv;
+int residual_vector_device(cism_ctx *c, double *ru, double *rs) {
     const int nb = red_blocks_for(c->g.nn);
-    hipLaunchKernelGGL(k_resvec, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->x, c->b, ru,
-                       c->partials);
+    if (rs)
+        hipLaunchKernelGGL(k_resvec<true>, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->x, c->b,
+                           ru, rs, c->partials);
+    else
+        hipLaunchKernelGGL(k_resvec<false>, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->x,
+                           c->b, ru, nullptr, c->partials);
     HIP_TRY(hipGetLastError());
-    return launch_reduce(c, nb, 2, RED_SUMS);
+    return launch_reduce(c, nb, rs ? 4 : 2, RED_SUMS);
 }
 
 // compute_residual_velocity_3d (velo_higher:9111-9234) -> sums[0] (and the

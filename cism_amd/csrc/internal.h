@@ -283,15 +283,20 @@ int launch_cg_matvec(cism_ctx *c, int fp32);
 int rel_err_device(cism_ctx *c, double *rel);
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol,
                  double *err, int *niters);
+// r_ready: c->r already holds b - A x with sums[2] = |r|^2 and (on the host too)
+// sums[3] = |b|^2, as residual_vector_device leaves them with rs = c->r
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
-                 double *err, int *niters, int *n64, int *n32);
+                 double *err, int *niters, int *n64, int *n32, int r_ready = 0);
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);
 int dirichlet_device(cism_ctx *c);
 int symmetry_device(cism_ctx *c, float *As);
 int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
-int residual_vector_device(cism_ctx *c, double *ru, double *rv);  // sums[0]=|r|^2, [1]=|b|^2
+// r = A x - b on owned active rows: sums[0] = |r|^2, sums[1] = |b|^2 over them.  With
+// rs: also rs = b - A x, halo rows from their images; sums[2] = |rs|^2, sums[3] = |b|^2
+// over owned rows
+int residual_vector_device(cism_ctx *c, double *ru, double *rs);
 int residual_velocity_device(cism_ctx *c, int whichresid);       // sums[0]
 int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy);
 int load_lateral_device(cism_ctx *c, const int32_t *ocean, const int32_t *cfront);

@@ -793,35 +793,49 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_refine(long long n2, const float
     x[q] = x[q] + sc->sums[2] * (double)e[q];
 }
 
-__global__ void k_save_scale(DevScal *sc) { sc->sums[2] = sqrt(sc->sums[0]); }
+__global__ void k_save_scale(DevScal *sc, int from) { sc->sums[2] = sqrt(sc->sums[from]); }
 
 // Iterative refinement: r = b - A64 x (FP64), solve A32 e = r/|r| with FP32
 // Chron-Gear to relative tolerance inner_rtol, x += |r| e, until the FP64
 // relative residual is below tol.  The FP32 matrix must be current: the symmetry pass
-// of the Picard loop writes it (symmetry_device with c->A32).
+// of the Picard loop writes it (symmetry_device with c->A32).  With r_ready the
+// first residual, its norm and |b| come from the Picard residual pass.
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol, double *err_out,
-                 int *niters_out, int *n64, int *n32) {
+                 int *niters_out, int *n64, int *n32, int r_ready) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn);
     const long long n2 = 2 * g.npad;
     CISM_TRY(ensure_a32(c));
-    hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->b, c->b, c->partials);
-    CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
-    CISM_TRY(sync_scalars(c));
-    const double bnorm = sqrt(c->sc_host->sums[0]);
+    double bnorm;
+    if (r_ready) {
+        bnorm = sqrt(c->sc_host->sums[3]);
+    } else {
+        hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->b, c->b,
+                           c->partials);
+        CISM_TRY(launch_reduce(c, nb, 1, RED_SUMS));
+        CISM_TRY(sync_scalars(c));
+        bnorm = sqrt(c->sc_host->sums[0]);
+    }
     int total = 0, outer = 0;
     double err = 0.0;
     CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32,
                        c->cscale_on ? c->cscale : nullptr};
     while (true) {
-        CISM_TRY(launch_halo_vec(c, c->x));
-        hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream,
-                           g, MatFull<double>{c->A}, c->av, c->x, c->b, c->r, 1, c->partials);
-        CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
-        hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc);
-        CISM_TRY(sync_scalars(c));
+        double rr;
+        if (r_ready && outer == 0) {
+            rr = c->sc_host->sums[2];
+            hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 2);
+        } else {
+            CISM_TRY(launch_halo_vec(c, c->x));
+            hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK),
+                               0, c->stream, g, MatFull<double>{c->A}, c->av, c->x, c->b, c->r, 1, c->partials);
+            CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
+            hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 0);
+            CISM_TRY(sync_scalars(c));
+            rr = c->sc_host->sums[0];
+        }
         ++*n64;
-        err = bnorm > 0 ? sqrt(c->sc_host->sums[0]) / bnorm : 0.0;
+        err = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
         if (err < tol || total >= maxiters * 4 || outer >= 50) break;
         hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->r, c->sc,
                            c->b32, c->x32);
