@@ -1,12 +1,14 @@
 """Parity of the HIP library (through its C ABI) with the CPU oracle.
 
 Bars: integer/index results bit-exact; floating point within the stated
-tolerances.  The HIP kernels are compiled with -ffp-contract=off and follow the
-oracle's operation order, so assembly, matvec and the Dirichlet update agree to
-the last bits except where pow() rounding differs (glibc vs device libm), which
-bounds the assembled coefficients at ~1e-15 relative.  Solver iterates differ
-only by the order of the global sums (tree vs sequential), which stays far
-below the 1e-8 relative L2 velocity bar.
+tolerances.  The HIP kernels are compiled with -ffp-contract=off.  The matvec and
+the Dirichlet update follow the oracle's operation order and agree to the last
+bits.  The element matrices are the oracle's sums over quadrature points
+regrouped (Kuu = 4 Pxx + Pyy + Pzz with Pxx = sum_p f x_a x_b, formed with fused
+multiply-adds) and the Jacobian inverse multiplies by 1/det; with pow() rounding
+(glibc vs device libm) that keeps the assembled coefficients within ~1e-15
+relative of the oracle's.  Solver iterates differ only by the order of the global
+sums (tree vs sequential), which stays far below the 1e-8 relative L2 velocity bar.
 """
 import numpy as np
 import pytest
