@@ -375,6 +375,55 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
     }
 }
 
+// One column and component of the SIA preconditioner: the Thomas substitutions
+// on s (level k at s[2k]) with the factors precomputed by k_sia_setup.
+template <typename T, int NZMAX>
+__device__ __forceinline__ void sia_column(int nz, long long ncol, const T *__restrict__ F, long long col,
+                                           int comp, T *s) {
+    const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
+    const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
+    const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
+    if constexpr (NZMAX <= 12) {
+        // every factor load is issued before the substitutions, which would otherwise
+        // wait on one global load per level
+        T fs[NZMAX], fb[NZMAX], fg[NZMAX];
+#pragma unroll
+        for (int k = 0; k < NZMAX; ++k)
+            if (k < nz) {
+                fs[k] = sb[k * ncol];
+                fb[k] = be[k * ncol];
+                fg[k] = ga[k * ncol];
+            }
+        T prev = s[0] / fb[0];
+        s[0] = prev;
+#pragma unroll
+        for (int k = 1; k < NZMAX; ++k)
+            if (k < nz) {
+                prev = (s[2 * k] - fs[k] * prev) / fb[k];
+                s[2 * k] = prev;
+            }
+#pragma unroll
+        for (int k = NZMAX - 2; k >= 0; --k)
+            if (k <= nz - 2) {
+                prev = s[2 * k] - fg[k + 1] * prev;
+                s[2 * k] = prev;
+            }
+    } else {
+        T prev = s[0] / be[0];
+        s[0] = prev;
+        for (int k = 1; k < nz; ++k) {
+            prev = (s[2 * k] - sb[k * ncol] * prev) / be[k * ncol];
+            s[2 * k] = prev;
+        }
+        for (int k = nz - 2; k >= 0; --k) {
+            prev = s[2 * k] - ga[(k + 1) * ncol] * prev;
+            s[2 * k] = prev;
+        }
+    }
+}
+
+// z = M^-1 r, M the SIA (column tridiagonal) preconditioner: one block per 64
+// columns, one lane per column and component; r staged in LDS for coalescing
 template <typename T, int NZMAX>
 __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restrict__ F,
                                                    const int32_t *__restrict__ av,
@@ -390,49 +439,7 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
     __syncthreads();
     const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
     const long long col = col0 + lc;
-    if (lc < ncols && av[col]) {
-        T *s = sr + 2 * lc * nz + comp;   // level k at s[2k]
-        const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
-        const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
-        const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
-        if constexpr (NZMAX <= 12) {
-            // every factor load is issued before the substitutions, which would otherwise
-            // wait on one global load per level
-            T fs[NZMAX], fb[NZMAX], fg[NZMAX];
-#pragma unroll
-            for (int k = 0; k < NZMAX; ++k)
-                if (k < nz) {
-                    fs[k] = sb[k * ncol];
-                    fb[k] = be[k * ncol];
-                    fg[k] = ga[k * ncol];
-                }
-            T prev = s[0] / fb[0];
-            s[0] = prev;
-#pragma unroll
-            for (int k = 1; k < NZMAX; ++k)
-                if (k < nz) {
-                    prev = (s[2 * k] - fs[k] * prev) / fb[k];
-                    s[2 * k] = prev;
-                }
-#pragma unroll
-            for (int k = NZMAX - 2; k >= 0; --k)
-                if (k <= nz - 2) {
-                    prev = s[2 * k] - fg[k + 1] * prev;
-                    s[2 * k] = prev;
-                }
-        } else {
-            T prev = s[0] / be[0];
-            s[0] = prev;
-            for (int k = 1; k < nz; ++k) {
-                prev = (s[2 * k] - sb[k * ncol] * prev) / be[k * ncol];
-                s[2 * k] = prev;
-            }
-            for (int k = nz - 2; k >= 0; --k) {
-                prev = s[2 * k] - ga[(k + 1) * ncol] * prev;
-                s[2 * k] = prev;
-            }
-        }
-    }
+    if (lc < ncols && av[col]) sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
     __syncthreads();
     for (int t = threadIdx.x; t < n2; t += blockDim.x)
         if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
@@ -555,31 +562,30 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long n2, const T *__
     r[t] = r[t] - alpha * qv[t];
 }
 
-// Chron-Gear (pcg:1517-1536) with the halo update of Az (pcg:1494-1497) folded
-// in: halo entries read Az at their periodic image (or 0, outflow).
-template <typename T>
+// Chron-Gear (pcg:1517-1536) of node q with the halo update of Az (pcg:1494-1497)
+// folded in: halo entries read Az at their periodic image (or 0, outflow).
+// Returns false for a skipped node.
 //
 // skip_av (diagonal or SIA preconditioner only): owned nodes of inactive columns are
 // left alone.  There z = 0 (zeroed at the start, never written by those
 // preconditioners), A z = 0 and d = q = 0, so their update would rewrite d = q = 0
 // and leave x and r unchanged: skipping it changes no value and saves the traffic.
-__global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__restrict__ z,
-                                                          const T *__restrict__ Az,
-                                                          T *__restrict__ d, T *__restrict__ qv,
-                                                          T *__restrict__ x, T *__restrict__ r,
-                                                          const DevScal *__restrict__ sc, int first,
-                                                          const int32_t *__restrict__ skip_av) {
+template <typename T>
+__device__ __forceinline__ bool cg_update_node(const GridDev &g, long long q, const T *__restrict__ z,
+                                               const T *__restrict__ Az, T *__restrict__ d,
+                                               T *__restrict__ qv, T *__restrict__ x, T *__restrict__ r,
+                                               T alpha, T beta, int first,
+                                               const int32_t *__restrict__ skip_av,
+                                               typename Vec2<T>::t &rr) {
     // one lane per node: both components through one 8- or 16-byte access per vector
     using V2 = typename Vec2<T>::t;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.nn) return;
     V2 az;
     {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         const bool own = is_owned(g, i, j);
-        if (own && skip_av && !skip_av[col]) return;
+        if (own && skip_av && !skip_av[col]) return false;
         if (own) {
             az = reinterpret_cast<const V2 *>(Az)[q];
         } else {
@@ -592,7 +598,6 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
             }
         }
     }
-    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
     const V2 zz = reinterpret_cast<const V2 *>(z)[q];
     V2 dd, qq;
     if (first) {   // first pass: d = z, q = A d (already halo-mapped), pcg:1351-1405
@@ -607,14 +612,66 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
     }
     reinterpret_cast<V2 *>(d)[q] = dd;
     reinterpret_cast<V2 *>(qv)[q] = qq;
-    V2 xx = reinterpret_cast<const V2 *>(x)[q], rr = reinterpret_cast<const V2 *>(r)[q];
+    V2 xx = reinterpret_cast<const V2 *>(x)[q];
+    rr = reinterpret_cast<const V2 *>(r)[q];
     xx.x = xx.x + alpha * dd.x;
     xx.y = xx.y + alpha * dd.y;
     rr.x = rr.x - alpha * qq.x;
     rr.y = rr.y - alpha * qq.y;
     reinterpret_cast<V2 *>(x)[q] = xx;
     reinterpret_cast<V2 *>(r)[q] = rr;
+    return true;
 }
+
+template <typename T>
+__global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__restrict__ z,
+                                                          const T *__restrict__ Az,
+                                                          T *__restrict__ d, T *__restrict__ qv,
+                                                          T *__restrict__ x, T *__restrict__ r,
+                                                          const DevScal *__restrict__ sc, int first,
+                                                          const int32_t *__restrict__ skip_av) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    typename Vec2<T>::t rr;
+    cg_update_node<T>(g, q, z, Az, d, qv, x, r, (T)sc->alpha, (T)sc->beta, first, skip_av, rr);
+}
+
+// The Chron-Gear update followed by the SIA preconditioner of the next iteration
+// (z = M^-1 r, pcg:1462-1470), for one block of 64 columns: the block updates its
+// nodes, keeps the new r in LDS, solves its columns there and writes z.  The
+// update of a node reads only its own z, so z is overwritten in place.  Same
+// values as k_cg_update then k_sia_apply; one pass over r fewer.
+template <typename T, int NZMAX>
+__global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T *__restrict__ Az,
+                                                       T *__restrict__ d, T *__restrict__ qv,
+                                                       T *__restrict__ x, T *__restrict__ r,
+                                                       const DevScal *__restrict__ sc, int first,
+                                                       const int32_t *__restrict__ av,
+                                                       const T *__restrict__ F) {
+    const int nz = g.nz;
+    const long long ncol = (long long)g.sx * g.sy;
+    const long long col0 = (long long)blockIdx.x * 64;
+    __shared__ T sr[2 * 64 * NZMAX];
+    const int ncols = (int)(col0 + 64 <= ncol ? 64 : ncol - col0);
+    const long long Q0 = col0 * nz;
+    const int nn = ncols * nz;
+    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+        typename Vec2<T>::t rr;
+        if (cg_update_node<T>(g, Q0 + t, z, Az, d, qv, x, r, alpha, beta, first, av, rr)) {
+            sr[2 * t] = rr.x;
+            sr[2 * t + 1] = rr.y;
+        }
+    }
+    __syncthreads();
+    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+    const long long col = col0 + lc;
+    if (lc < ncols && av[col]) sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * nn; t += blockDim.x)
+        if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
+}
+
 
 // ---------------------------------------------------------------------------
 // PCG drivers
@@ -650,6 +707,29 @@ struct CGVecs {
     const float *cs;   // optional per-column preconditioner scale (2*ncol) or null
 };
 
+template <typename T>
+static int launch_cg_update(cism_ctx *c, int precond, CGVecs<T> &v, int first) {
+    const GridDev &g = c->g;
+    if (precond == 2) {
+        // fused with the SIA preconditioner; the caller skips its own precond call
+        const dim3 grid((unsigned)(((long long)g.sx * g.sy + 63) / 64));
+        if (g.nz <= 12)
+            hipLaunchKernelGGL((k_cg_update_sia<T, 12>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+        else if (g.nz <= 24)
+            hipLaunchKernelGGL((k_cg_update_sia<T, 24>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+        else
+            hipLaunchKernelGGL((k_cg_update_sia<T, 48>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+    } else {
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g, v.z,
+                           v.Az, v.d, v.q, v.x, v.r, c->sc, first, precond != 0 ? c->av : nullptr);
+    }
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
 // The reference iterations on a generic precision.  check = 0: every fifth
 // iteration the true residual b - A x decides convergence against tol (the
 // reference rule, pcg:478-537, 1542-1597).  check = 1 (inner solves of the
@@ -657,7 +737,7 @@ struct CGVecs {
 // with no extra matvec, against tol * |r0|.
 template <typename T, typename M>
 static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precond, int maxiters,
-                  double tol, int check, double *err_out, int *niters_out) {
+                  double tol, int check, double *err_out, int *niters_out, int x_zero = 0) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
     const int nbv = M::xcd ? xcd_blocks_for(g.nn) : nbm;   // matvec grid (partials per block)
@@ -718,23 +798,25 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             }
         }
     } else {
-        // pcg:1258-1405: halo x; r = b - A x; halo r; z = M^-1 r; d = z; q = A d
-        CISM_TRY((launch_halo_t<T, false>(c, v.x)));
-        hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
-                           v.x, v.b, v.r, 1, c->partials);
+        // pcg:1258-1405: halo x; r = b - A x; halo r; z = M^-1 r; d = z; q = A d.  With
+        // x_zero the caller has set x = 0 and r = b (halo entries included).
+        if (!x_zero) {
+            CISM_TRY((launch_halo_t<T, false>(c, v.x)));
+            hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A, c->av,
+                               v.x, v.b, v.r, 1, c->partials);
+        }
         CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
         hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream, g, A,
                            c->av, v.z, v.Az, v.r, c->partials);
         CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_FIRST));
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
-                           v.d, v.q, v.x, v.r, c->sc, 1, precond != 0 ? c->av : nullptr);
+        // with the SIA preconditioner the update also forms the next z
+        CISM_TRY(launch_cg_update<T>(c, precond, v, 1));
         for (int n = 2; n <= maxiters && !done; ++n) {
-            CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
+            if (precond != 2) CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
             hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
             CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_ITER));
-            hipLaunchKernelGGL(k_cg_update<T>, dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, v.z, v.Az,
-                               v.d, v.q, v.x, v.r, c->sc, 0, precond != 0 ? c->av : nullptr);
+            CISM_TRY(launch_cg_update<T>(c, precond, v, 0));
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
     }
@@ -773,14 +855,18 @@ int ensure_a32(cism_ctx *c) {
     return CISM_OK;
 }
 
-// r32 = r64 / s (all entries), x32 = 0
+// b32 = r64 / s (all entries), and the inner solve's start: x32 = 0, so its first
+// residual is b32 itself (halo entries included: r64 carries its halo images)
 __global__ __launch_bounds__(CISM_BLOCK) void k_rhs32(long long n2, const double *__restrict__ r,
                                                       const DevScal *__restrict__ sc,
-                                                      float *__restrict__ r32, float *__restrict__ x32) {
+                                                      float *__restrict__ b32, float *__restrict__ x32,
+                                                      float *__restrict__ r32) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n2) return;
     const double s = sc->sums[2];
-    r32[q] = (float)(s > 0.0 ? r[q] / s : 0.0);
+    const float v = (float)(s > 0.0 ? r[q] / s : 0.0);
+    b32[q] = v;
+    r32[q] = v;
     x32[q] = 0.0f;
 }
 
@@ -838,11 +924,11 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         err = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
         if (err < tol || total >= maxiters * 4 || outer >= 50) break;
         hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->r, c->sc,
-                           c->b32, c->x32);
+                           c->b32, c->x32, c->r32);
         double e_in = 0.0;
         int it = 0;
         // the inner reductions use sums[0..1]; the scale |r| stays in sums[2]
-        CISM_TRY(run_cg<float>(c, MatSym32{c->A32}, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it));
+        CISM_TRY(run_cg<float>(c, MatSym32{c->A32}, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it, 1));
         hipLaunchKernelGGL(k_refine, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x32, c->sc,
                            c->x);
         HIP_TRY(hipGetLastError());
