@@ -97,6 +97,7 @@ struct cism_ctx {
     double *Ke;
     // reductions
     double *partials;
+    double *red_groups;   // the first pass of a two-pass reduction (RED_GROUPS x CISM_MAX_NV)
     DevScal *sc;
     DevScal *sc_host;   // pinned mirror
     // staging for layout conversion (27*nn doubles, lazily)

@@ -12,50 +12,10 @@
 #include <stdio.h>
 
 // ---------------------------------------------------------------------------
-// final reduction + CG scalar updates (single block, fixed order)
+// final reductions + CG scalar updates (fixed order)
 // ---------------------------------------------------------------------------
-// 1024 lanes, each with 8 independent accumulators over a fixed set of partials (lane
-// t, chain u takes blocks t + 1024 (8 r + u)), combined in a fixed order: the partials
-// of a 70,000-block grid are read with 8 loads in flight per lane instead of one
-// dependent chain, and the result does not depend on timing.
-#define RED_THREADS 1024
-__global__ __launch_bounds__(RED_THREADS) void k_reduce(const double *__restrict__ partials,
-                                                        int nb, int nv, int mode,
-                                                        DevScal *__restrict__ sc) {
-    __shared__ double red[RED_THREADS];
-    const bool mx = mode == RED_MAX;
-    for (int c = 0; c < nv; ++c) {
-        double a[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = 0.0;
-        int b = threadIdx.x;
-        for (; b + 7 * RED_THREADS < nb; b += 8 * RED_THREADS) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = partials[(size_t)(b + u * RED_THREADS) * nv + c];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) a[u] = mx ? fmax(a[u], v[u]) : a[u] + v[u];
-        }
-        for (int u = 0; b < nb; b += RED_THREADS, ++u) {   // fewer than 8 strides remain
-            const double v = partials[(size_t)b * nv + c];
-            a[u] = mx ? fmax(a[u], v) : a[u] + v;
-        }
-        double s = a[0];
-#pragma unroll
-        for (int u = 1; u < 8; ++u) s = mx ? fmax(s, a[u]) : s + a[u];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) {
-                if (mode == RED_MAX) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
-                else red[threadIdx.x] += red[threadIdx.x + off];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) sc->sums[c] = red[0];
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
+// The scalar step that follows a reduction (mode), on sc->sums.
+__device__ inline void scalar_update(int mode, DevScal *__restrict__ sc) {
     const double s0 = sc->sums[0], s1 = sc->sums[1];
     switch (mode) {
     case RED_BB:
@@ -105,9 +65,117 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double *__restrict
     }
 }
 
+// k_reduce, for any width: one block of 1024 lanes, each with 8 independent
+// accumulators over a fixed set of partials (lane t, chain u takes blocks
+// t + 1024 (8 r + u)), combined in a fixed order: the partials of a 70,000-block
+// grid are read with 8 loads in flight per lane instead of one dependent chain, and
+// the result does not depend on timing.
+#define RED_THREADS 1024
+__global__ __launch_bounds__(RED_THREADS) void k_reduce(const double *__restrict__ partials,
+                                                        int nb, int nv, int mode,
+                                                        DevScal *__restrict__ sc) {
+    __shared__ double red[RED_THREADS];
+    const bool mx = mode == RED_MAX;
+    for (int c = 0; c < nv; ++c) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = 0.0;
+        int b = threadIdx.x;
+        for (; b + 7 * RED_THREADS < nb; b += 8 * RED_THREADS) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[(size_t)(b + u * RED_THREADS) * nv + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = mx ? fmax(a[u], v[u]) : a[u] + v[u];
+        }
+        for (int u = 0; b < nb; b += RED_THREADS, ++u) {   // fewer than 8 strides remain
+            const double v = partials[(size_t)b * nv + c];
+            a[u] = mx ? fmax(a[u], v) : a[u] + v;
+        }
+        double s = a[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = mx ? fmax(s, a[u]) : s + a[u];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) {
+                if (mode == RED_MAX) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+                else red[threadIdx.x] += red[threadIdx.x + off];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) sc->sums[c] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    scalar_update(mode, sc);
+}
+
+// Two-pass reduction for the common widths: RED_GROUPS workgroups each combine a
+// contiguous range of the partials (lane-strided, then a fixed shuffle tree), then
+// one wave combines the group results in group order and applies the scalar step.
+// Deterministic like k_reduce, and the first pass is spread over the chip.
+#define RED_GROUPS 64
+template <int NV>
+__global__ __launch_bounds__(CISM_BLOCK) void k_reduce_groups(const double *__restrict__ partials, int nb,
+                                                              int mode, double *__restrict__ groups) {
+    const bool mx = mode == RED_MAX;
+    const int chunk = (nb + RED_GROUPS - 1) / RED_GROUPS;
+    const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+    double a[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            const double v = partials[(size_t)b * NV + c];
+            a[c] = mx ? fmax(a[c], v) : a[c] + v;
+        }
+    if (mx) block_max_store<NV>(a, groups);
+    else block_reduce_store<NV>(a, groups);
+}
+
+// out = null: the sums go to sc->sums and the scalar step (mode) follows; else
+// they go to out
+template <int NV>
+__global__ void k_reduce_final(const double *__restrict__ groups, int mode, DevScal *__restrict__ sc,
+                               double *__restrict__ out) {
+    const bool mx = mode == RED_MAX;
+    if (threadIdx.x < NV) {
+        double s = groups[threadIdx.x];
+        for (int g = 1; g < RED_GROUPS; ++g) {
+            const double v = groups[g * NV + threadIdx.x];
+            s = mx ? fmax(s, v) : s + v;
+        }
+        if (out) out[threadIdx.x] = s;
+        else sc->sums[threadIdx.x] = s;
+    }
+    if (out) return;
+    __syncthreads();
+    if (threadIdx.x == 0) scalar_update(mode, sc);
+}
+
+template <int NV>
+static void launch_reduce2(cism_ctx *c, int nb, int mode, double *out = nullptr) {
+    hipLaunchKernelGGL(k_reduce_groups<NV>, dim3(RED_GROUPS), dim3(CISM_BLOCK), 0, c->stream, c->partials, nb,
+                       mode, c->red_groups);
+    hipLaunchKernelGGL(k_reduce_final<NV>, dim3(1), dim3(64), 0, c->stream, c->red_groups, mode, c->sc, out);
+}
+
+// sums of NV values per block of the partials into out[0..NV), nothing else
+template <int NV>
+static int launch_reduce_to(cism_ctx *c, int nb, double *out) {
+    launch_reduce2<NV>(c, nb, RED_SUMS, out);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
 int launch_reduce(cism_ctx *c, int nb, int nv, int mode) {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RED_THREADS), 0, c->stream, c->partials, nb, nv,
-                       mode, c->sc);
+    if (nv == 1) launch_reduce2<1>(c, nb, mode);
+    else if (nv == 2) launch_reduce2<2>(c, nb, mode);
+    else if (nv == 4) launch_reduce2<4>(c, nb, mode);
+    else
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RED_THREADS), 0, c->stream, c->partials, nb, nv, mode, c->sc);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
@@ -376,39 +444,59 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
 }
 
 // One column and component of the SIA preconditioner: the Thomas substitutions
-// on s (level k at s[2k]) with the factors precomputed by k_sia_setup.
+// on s (level k at s[2k]) with the factors precomputed by k_sia_setup.  For
+// nz <= 12 the factors are first loaded into registers (sia_factors), so that a
+// caller can issue those loads early; the substitutions would otherwise wait on
+// one global load per level.
 template <typename T, int NZMAX>
-__device__ __forceinline__ void sia_column(int nz, long long ncol, const T *__restrict__ F, long long col,
-                                           int comp, T *s) {
+struct SiaFactors {
+    T fs[NZMAX], fb[NZMAX], fg[NZMAX];
+};
+
+template <typename T, int NZMAX>
+__device__ __forceinline__ void sia_factors(int nz, long long ncol, const T *__restrict__ F, long long col,
+                                            int comp, SiaFactors<T, NZMAX> &f) {
     const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
     const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
     const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
+#pragma unroll
+    for (int k = 0; k < NZMAX; ++k)
+        if (k < nz) {
+            f.fs[k] = sb[k * ncol];
+            f.fb[k] = be[k * ncol];
+            f.fg[k] = ga[k * ncol];
+        }
+}
+
+template <typename T, int NZMAX>
+__device__ __forceinline__ void sia_solve(int nz, const SiaFactors<T, NZMAX> &f, T *s) {
+    T prev = s[0] / f.fb[0];
+    s[0] = prev;
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k)
+        if (k < nz) {
+            prev = (s[2 * k] - f.fs[k] * prev) / f.fb[k];
+            s[2 * k] = prev;
+        }
+#pragma unroll
+    for (int k = NZMAX - 2; k >= 0; --k)
+        if (k <= nz - 2) {
+            prev = s[2 * k] - f.fg[k + 1] * prev;
+            s[2 * k] = prev;
+        }
+}
+
+template <typename T, int NZMAX>
+__device__ __forceinline__ void sia_column(int nz, long long ncol, const T *__restrict__ F, long long col,
+                                           int comp, T *s) {
     if constexpr (NZMAX <= 12) {
-        // every factor load is issued before the substitutions, which would otherwise
-        // wait on one global load per level
-        T fs[NZMAX], fb[NZMAX], fg[NZMAX];
-#pragma unroll
-        for (int k = 0; k < NZMAX; ++k)
-            if (k < nz) {
-                fs[k] = sb[k * ncol];
-                fb[k] = be[k * ncol];
-                fg[k] = ga[k * ncol];
-            }
-        T prev = s[0] / fb[0];
-        s[0] = prev;
-#pragma unroll
-        for (int k = 1; k < NZMAX; ++k)
-            if (k < nz) {
-                prev = (s[2 * k] - fs[k] * prev) / fb[k];
-                s[2 * k] = prev;
-            }
-#pragma unroll
-        for (int k = NZMAX - 2; k >= 0; --k)
-            if (k <= nz - 2) {
-                prev = s[2 * k] - fg[k + 1] * prev;
-                s[2 * k] = prev;
-            }
+        SiaFactors<T, NZMAX> f;
+        sia_factors<T, NZMAX>(nz, ncol, F, col, comp, f);
+        sia_solve<T, NZMAX>(nz, f, s);
     } else {
+        const T *sb = F + (long long)(0 * 2 + comp) * nz * ncol + col;
+        const T *be = F + (long long)(1 * 2 + comp) * nz * ncol + col;
+        const T *ga = F + (long long)(2 * 2 + comp) * nz * ncol + col;
         T prev = s[0] / be[0];
         s[0] = prev;
         for (int k = 1; k < nz; ++k) {
@@ -564,14 +652,14 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long n2, const T *__
 
 // Chron-Gear (pcg:1517-1536) of node q with the halo update of Az (pcg:1494-1497)
 // folded in: halo entries read Az at their periodic image (or 0, outflow).
-// Returns false for a skipped node.
+// Returns 0 for a skipped node, 1 for an updated halo node, 2 for an updated owned one.
 //
 // skip_av (diagonal or SIA preconditioner only): owned nodes of inactive columns are
 // left alone.  There z = 0 (zeroed at the start, never written by those
 // preconditioners), A z = 0 and d = q = 0, so their update would rewrite d = q = 0
 // and leave x and r unchanged: skipping it changes no value and saves the traffic.
 template <typename T>
-__device__ __forceinline__ bool cg_update_node(const GridDev &g, long long q, const T *__restrict__ z,
+__device__ __forceinline__ int cg_update_node(const GridDev &g, long long q, const T *__restrict__ z,
                                                const T *__restrict__ Az, T *__restrict__ d,
                                                T *__restrict__ qv, T *__restrict__ x, T *__restrict__ r,
                                                T alpha, T beta, int first,
@@ -580,12 +668,13 @@ __device__ __forceinline__ bool cg_update_node(const GridDev &g, long long q, co
     // one lane per node: both components through one 8- or 16-byte access per vector
     using V2 = typename Vec2<T>::t;
     V2 az;
+    bool own;
     {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        const bool own = is_owned(g, i, j);
-        if (own && skip_av && !skip_av[col]) return false;
+        own = is_owned(g, i, j);
+        if (own && skip_av && !skip_av[col]) return 0;
         if (own) {
             az = reinterpret_cast<const V2 *>(Az)[q];
         } else {
@@ -620,7 +709,7 @@ __device__ __forceinline__ bool cg_update_node(const GridDev &g, long long q, co
     rr.y = rr.y - alpha * qq.y;
     reinterpret_cast<V2 *>(x)[q] = xx;
     reinterpret_cast<V2 *>(r)[q] = rr;
-    return true;
+    return own ? 2 : 1;
 }
 
 template <typename T>
@@ -640,14 +729,18 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
 // (z = M^-1 r, pcg:1462-1470), for one block of 64 columns: the block updates its
 // nodes, keeps the new r in LDS, solves its columns there and writes z.  The
 // update of a node reads only its own z, so z is overwritten in place.  Same
-// values as k_cg_update then k_sia_apply; one pass over r fewer.
+// values as k_cg_update then k_sia_apply; one pass over r fewer.  The column
+// factors are loaded before the update so their latency overlaps it.  With
+// want_rr the block also leaves (r, r) over its owned nodes in partials (the
+// convergence check of the inner solves, pcg:1540-1560).
 template <typename T, int NZMAX>
 __global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T *__restrict__ Az,
                                                        T *__restrict__ d, T *__restrict__ qv,
                                                        T *__restrict__ x, T *__restrict__ r,
                                                        const DevScal *__restrict__ sc, int first,
                                                        const int32_t *__restrict__ av,
-                                                       const T *__restrict__ F) {
+                                                       const T *__restrict__ F, int want_rr,
+                                                       double *__restrict__ partials) {
     const int nz = g.nz;
     const long long ncol = (long long)g.sx * g.sy;
     const long long col0 = (long long)blockIdx.x * 64;
@@ -655,23 +748,33 @@ __global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T 
     const int ncols = (int)(col0 + 64 <= ncol ? 64 : ncol - col0);
     const long long Q0 = col0 * nz;
     const int nn = ncols * nz;
+    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+    const long long col = col0 + lc;
+    const bool solve = lc < ncols && av[col];
+    SiaFactors<T, (NZMAX <= 12 ? NZMAX : 1)> f;
+    if constexpr (NZMAX <= 12)
+        if (solve) sia_factors<T, NZMAX>(nz, ncol, F, col, comp, f);
     const T alpha = (T)sc->alpha, beta = (T)sc->beta;
+    double acc[1] = {0.0};
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         typename Vec2<T>::t rr;
-        if (cg_update_node<T>(g, Q0 + t, z, Az, d, qv, x, r, alpha, beta, first, av, rr)) {
+        const int st = cg_update_node<T>(g, Q0 + t, z, Az, d, qv, x, r, alpha, beta, first, av, rr);
+        if (st) {
             sr[2 * t] = rr.x;
             sr[2 * t + 1] = rr.y;
+            if (st == 2) acc[0] += (double)rr.x * rr.x + (double)rr.y * rr.y;
         }
     }
     __syncthreads();
-    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
-    const long long col = col0 + lc;
-    if (lc < ncols && av[col]) sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
+    if (solve) {
+        if constexpr (NZMAX <= 12) sia_solve<T, NZMAX>(nz, f, sr + 2 * lc * nz + comp);
+        else sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < 2 * nn; t += blockDim.x)
         if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
+    if (want_rr) block_reduce_store<1>(acc, partials);
 }
-
 
 // ---------------------------------------------------------------------------
 // PCG drivers
@@ -707,21 +810,24 @@ struct CGVecs {
     const float *cs;   // optional per-column preconditioner scale (2*ncol) or null
 };
 
+// blocks of the fused update (one per 64 columns)
+static inline int sia_blocks(const GridDev &g) { return (int)(((long long)g.sx * g.sy + 63) / 64); }
+
 template <typename T>
-static int launch_cg_update(cism_ctx *c, int precond, CGVecs<T> &v, int first) {
+static int launch_cg_update(cism_ctx *c, int precond, CGVecs<T> &v, int first, int want_rr = 0) {
     const GridDev &g = c->g;
     if (precond == 2) {
         // fused with the SIA preconditioner; the caller skips its own precond call
-        const dim3 grid((unsigned)(((long long)g.sx * g.sy + 63) / 64));
+        const dim3 grid((unsigned)sia_blocks(g));
         if (g.nz <= 12)
             hipLaunchKernelGGL((k_cg_update_sia<T, 12>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
         else if (g.nz <= 24)
             hipLaunchKernelGGL((k_cg_update_sia<T, 24>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
         else
             hipLaunchKernelGGL((k_cg_update_sia<T, 48>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
     } else {
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g, v.z,
                            v.Az, v.d, v.q, v.x, v.r, c->sc, first, precond != 0 ? c->av : nullptr);
@@ -758,6 +864,9 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);
             CISM_TRY(launch_reduce(c, nbm, 1, RED_ERR));
+        } else if (chrongear && precond == 2) {
+            // (r, r) came with the fused update of this iteration
+            CISM_TRY(launch_reduce(c, sia_blocks(g), 1, RED_ERR));
         } else {
             hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.r, v.r,
                                c->partials);
@@ -816,7 +925,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
             CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_ITER));
-            CISM_TRY(launch_cg_update<T>(c, precond, v, 0));
+            CISM_TRY(launch_cg_update<T>(c, precond, v, 0, check == 1 && n % 5 == 0));
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
     }
@@ -998,13 +1107,25 @@ int column_scale_device(cism_ctx *c, double *out) {
 // The fixed point is the Picard fixed point; only the path to it changes.
 // Buffers (each 2*npad doubles): dX[0..m), dF[0..m), xprev, fprev.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CISM_BLOCK) void k_aa_form(long long n2, const double *__restrict__ x,
+// Only owned nodes of active columns take part.  Elsewhere neither the linear
+// solves nor these updates change x (halo entries are refreshed from their images
+// afterwards), so f, dX and dF are zero there and the history is not kept.
+__device__ inline bool aa_entry_active(const GridDev &g, const int32_t *__restrict__ av, long long t) {
+    const long long q = t >> 1;
+    if (q >= g.nn) return false;
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    return is_owned(g, i, j) && av[col];
+}
+
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_form(GridDev g, const int32_t *__restrict__ av,
+                                                        const double *__restrict__ x,
                                                         const double *__restrict__ xn,
                                                         double *__restrict__ dX, double *__restrict__ dF,
                                                         double *__restrict__ xprev,
                                                         double *__restrict__ fprev, int add) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n2) return;
+    if (!aa_entry_active(g, av, t)) return;
     const double f = x[t] - xn[t];
     if (add) {
         dF[t] = f - fprev[t];
@@ -1017,8 +1138,9 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_aa_form(long long n2, const doub
 // Gram matrix of the k columns of dF (upper triangle, row-major), then dF_i . f,
 // then f . f; owned nodes only.  NV = 4*5/2 + 4 + 1 = 15 slots, unused ones 0.
 #define AA_NV 15
-__global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, int k, const double *__restrict__ dF,
-                                                        long long stride, const double *__restrict__ f,
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, const int32_t *__restrict__ av, int k,
+                                                        const double *__restrict__ dF, long long stride,
+                                                        const double *__restrict__ f,
                                                         double *__restrict__ partials) {
     double acc[AA_NV];
 #pragma unroll
@@ -1027,7 +1149,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, int k, const 
     if (q < g.nn) {
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        if (is_owned(g, i, j)) {
+        if (is_owned(g, i, j) && av[col]) {
             for (int h = 0; h < 2; ++h) {
                 const long long t = 2 * q + h;
                 double v[AA_MAX];
@@ -1044,33 +1166,13 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, int k, const 
     block_reduce_store<AA_NV>(acc, partials);
 }
 
-// fixed-order reduction of the per-block partials: thread t sums blocks t, t+256, ...;
-// then a tree over the 256 threads per value
-__global__ __launch_bounds__(256) void k_aa_reduce(const double *__restrict__ partials, int nb,
-                                                   double *__restrict__ out) {
-    __shared__ double red[256];
-    double acc[AA_NV];
-    for (int c = 0; c < AA_NV; ++c) acc[c] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256)
-        for (int c = 0; c < AA_NV; ++c) acc[c] += partials[(long long)b * AA_NV + c];
-    for (int c = 0; c < AA_NV; ++c) {
-        red[threadIdx.x] = acc[c];
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[c] = red[0];
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(CISM_BLOCK) void k_aa_update(long long n2, int k, const double *__restrict__ dX,
+__global__ __launch_bounds__(CISM_BLOCK) void k_aa_update(GridDev g, const int32_t *__restrict__ av, int k,
+                                                          const double *__restrict__ dX,
                                                           const double *__restrict__ dF, long long stride,
                                                           double g0, double g1, double g2, double g3,
                                                           double *__restrict__ x) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n2) return;
+    if (!aa_entry_active(g, av, t)) return;
     const double gm[AA_MAX] = {g0, g1, g2, g3};
     double v = x[t];
     for (int a = 0; a < k; ++a) v -= gm[a] * (dX[a * stride + t] + dF[a * stride + t]);
@@ -1143,16 +1245,16 @@ int anderson_step(cism_ctx *c, int m, double *fnorm) {
            *fprev = xprev + n2;
     const int add = c->aa_count > 0;
     const int slot = c->aa_head;
-    hipLaunchKernelGGL(k_aa_form, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x, c->usav,
-                       dX + (size_t)slot * n2, dF + (size_t)slot * n2, xprev, fprev, add);
+    hipLaunchKernelGGL(k_aa_form, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, g, c->av, c->x,
+                       c->usav, dX + (size_t)slot * n2, dF + (size_t)slot * n2, xprev, fprev, add);
     const int k = add ? (c->aa_count < m ? c->aa_count : m) : 0;
     if (add) c->aa_head = (c->aa_head + 1) % m;
     c->aa_count++;
     // dots: Gram of the k stored columns, their products with f (= fprev now), |f|^2
     const int nb = nblocks_for(g.nn);
-    hipLaunchKernelGGL(k_aa_dots, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, k, dF, n2, fprev,
+    hipLaunchKernelGGL(k_aa_dots, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->av, k, dF, n2, fprev,
                        c->partials);
-    hipLaunchKernelGGL(k_aa_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->aa_sums);
+    CISM_TRY(launch_reduce_to<AA_NV>(c, nb, c->aa_sums));
     HIP_TRY(hipGetLastError());
     double hs[AA_NV];
     HIP_TRY(hipMemcpyAsync(hs, c->aa_sums, sizeof hs, hipMemcpyDeviceToHost, c->stream));
@@ -1177,8 +1279,8 @@ int anderson_step(cism_ctx *c, int m, double *fnorm) {
         for (int b = a; b < AA_MAX; ++b, ++s) Gu[s] = hs[s];
     for (int a = 0; a < AA_MAX; ++a) h[a] = hs[10 + a];
     if (!small_lsq(k, Gu, h, gam)) return CISM_OK;   // degenerate history: plain step
-    hipLaunchKernelGGL(k_aa_update, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, k, dX, dF, n2,
-                       gam[0], gam[1], gam[2], gam[3], c->x);
+    hipLaunchKernelGGL(k_aa_update, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, g, c->av, k, dX, dF,
+                       n2, gam[0], gam[1], gam[2], gam[3], c->x);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
