@@ -659,20 +659,18 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_std_xr(long long n2, const T *__
 // preconditioners), A z = 0 and d = q = 0, so their update would rewrite d = q = 0
 // and leave x and r unchanged: skipping it changes no value and saves the traffic.
 template <typename T>
-__device__ __forceinline__ int cg_update_node(const GridDev &g, long long q, const T *__restrict__ z,
-                                               const T *__restrict__ Az, T *__restrict__ d,
-                                               T *__restrict__ qv, T *__restrict__ x, T *__restrict__ r,
-                                               T alpha, T beta, int first,
+__device__ __forceinline__ int cg_update_node(const GridDev &g, long long q, int k, long long col, int i,
+                                               int j, const T *__restrict__ z, const T *__restrict__ Az,
+                                               T *__restrict__ d, T *__restrict__ qv, T *__restrict__ x,
+                                               T *__restrict__ r, T alpha, T beta, int first,
                                                const int32_t *__restrict__ skip_av,
                                                typename Vec2<T>::t &rr) {
-    // one lane per node: both components through one 8- or 16-byte access per vector
+    // one lane per node (k, i, j) of column col: both components through one 8- or
+    // 16-byte access per vector
     using V2 = typename Vec2<T>::t;
     V2 az;
     bool own;
     {
-        const int k = (int)(q % g.nz);
-        const long long col = q / g.nz;
-        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
         own = is_owned(g, i, j);
         if (own && skip_av && !skip_av[col]) return 0;
         if (own) {
@@ -721,8 +719,11 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
                                                           const int32_t *__restrict__ skip_av) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.nn) return;
+    const int k = (int)(q % g.nz);
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
     typename Vec2<T>::t rr;
-    cg_update_node<T>(g, q, z, Az, d, qv, x, r, (T)sc->alpha, (T)sc->beta, first, skip_av, rr);
+    cg_update_node<T>(g, q, k, col, i, j, z, Az, d, qv, x, r, (T)sc->alpha, (T)sc->beta, first, skip_av, rr);
 }
 
 // The Chron-Gear update followed by the SIA preconditioner of the next iteration
@@ -756,9 +757,16 @@ __global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T 
         if (solve) sia_factors<T, NZMAX>(nz, ncol, F, col, comp, f);
     const T alpha = (T)sc->alpha, beta = (T)sc->beta;
     double acc[1] = {0.0};
+    // node t of the block: level t % nz of local column t / nz (32-bit: the context
+    // limits grids to 2^30 nodes)
+    const int c0 = (int)col0;
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+        const int lcol = t / nz;
+        const int cn = c0 + lcol;
+        const int jn = cn / g.sx;
         typename Vec2<T>::t rr;
-        const int st = cg_update_node<T>(g, Q0 + t, z, Az, d, qv, x, r, alpha, beta, first, av, rr);
+        const int st = cg_update_node<T>(g, Q0 + t, t - lcol * nz, cn, cn - jn * g.sx, jn, z, Az, d, qv, x, r,
+                                         alpha, beta, first, av, rr);
         if (st) {
             sr[2 * t] = rr.x;
             sr[2 * t + 1] = rr.y;
@@ -1142,25 +1150,34 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_aa_dots(GridDev g, const int32_t
                                                         const double *__restrict__ dF, long long stride,
                                                         const double *__restrict__ f,
                                                         double *__restrict__ partials) {
+    // grid-stride over nodes, both components of a column entry through one 16-byte load
     double acc[AA_NV];
 #pragma unroll
     for (int c = 0; c < AA_NV; ++c) acc[c] = 0.0;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < g.nn) {
+    const long long gs = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += gs) {
         const long long col = q / g.nz;
         const int i = (int)(col % g.sx), j = (int)(col / g.sx);
-        if (is_owned(g, i, j) && av[col]) {
-            for (int h = 0; h < 2; ++h) {
-                const long long t = 2 * q + h;
-                double v[AA_MAX];
-                for (int a = 0; a < AA_MAX; ++a) v[a] = a < k ? dF[a * stride + t] : 0.0;
-                const double fv = f[t];
-                int s = 0;
-                for (int a = 0; a < AA_MAX; ++a)
-                    for (int b = a; b < AA_MAX; ++b) acc[s++] += v[a] * v[b];
-                for (int a = 0; a < AA_MAX; ++a) acc[s++] += v[a] * fv;
-                acc[s] += fv * fv;
-            }
+        if (!is_owned(g, i, j) || !av[col]) continue;
+        double2 v2[AA_MAX];
+#pragma unroll
+        for (int a = 0; a < AA_MAX; ++a)
+            v2[a] = a < k ? *reinterpret_cast<const double2 *>(dF + a * stride + 2 * q) : make_double2(0.0, 0.0);
+        const double2 f2 = *reinterpret_cast<const double2 *>(f + 2 * q);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double v[AA_MAX];
+#pragma unroll
+            for (int a = 0; a < AA_MAX; ++a) v[a] = h ? v2[a].y : v2[a].x;
+            const double fv = h ? f2.y : f2.x;
+            int s = 0;
+#pragma unroll
+            for (int a = 0; a < AA_MAX; ++a)
+#pragma unroll
+                for (int b = a; b < AA_MAX; ++b) acc[s++] += v[a] * v[b];
+#pragma unroll
+            for (int a = 0; a < AA_MAX; ++a) acc[s++] += v[a] * fv;
+            acc[s] += fv * fv;
         }
     }
     block_reduce_store<AA_NV>(acc, partials);
@@ -1251,7 +1268,7 @@ int anderson_step(cism_ctx *c, int m, double *fnorm) {
     if (add) c->aa_head = (c->aa_head + 1) % m;
     c->aa_count++;
     // dots: Gram of the k stored columns, their products with f (= fprev now), |f|^2
-    const int nb = nblocks_for(g.nn);
+    const int nb = red_blocks_for(g.nn);
     hipLaunchKernelGGL(k_aa_dots, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->av, k, dF, n2, fprev,
                        c->partials);
     CISM_TRY(launch_reduce_to<AA_NV>(c, nb, c->aa_sums));
