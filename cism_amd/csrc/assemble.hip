@@ -83,13 +83,14 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
         b[n] = sy * (ax * az / 8.0);
         cz[n] = sz * (ax * ay / 8.0);
     }
+    // sums of products with fused multiply-adds (one rounding per term)
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         const double X = nodes[n * 5 + 0], Y = nodes[n * 5 + 1], Z = nodes[n * 5 + 2];
-        J[0][0] = J[0][0] + a[n] * X;  J[0][1] = J[0][1] + a[n] * Y;  J[0][2] = J[0][2] + a[n] * Z;
-        J[1][0] = J[1][0] + b[n] * X;  J[1][1] = J[1][1] + b[n] * Y;  J[1][2] = J[1][2] + b[n] * Z;
-        J[2][0] = J[2][0] + cz[n] * X; J[2][1] = J[2][1] + cz[n] * Y; J[2][2] = J[2][2] + cz[n] * Z;
+        J[0][0] = fma(a[n], X, J[0][0]);  J[0][1] = fma(a[n], Y, J[0][1]);  J[0][2] = fma(a[n], Z, J[0][2]);
+        J[1][0] = fma(b[n], X, J[1][0]);  J[1][1] = fma(b[n], Y, J[1][1]);  J[1][2] = fma(b[n], Z, J[1][2]);
+        J[2][0] = fma(cz[n], X, J[2][0]); J[2][1] = fma(cz[n], Y, J[2][1]); J[2][2] = fma(cz[n], Z, J[2][2]);
     }
     double cof[3][3];
     cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -119,17 +120,17 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
     double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-        const double ddx = Ji[0][0] * a[n] + Ji[0][1] * b[n] + Ji[0][2] * cz[n];
-        const double ddy = Ji[1][0] * a[n] + Ji[1][1] * b[n] + Ji[1][2] * cz[n];
-        const double ddz = Ji[2][0] * a[n] + Ji[2][1] * b[n] + Ji[2][2] * cz[n];
+        const double ddx = fma(Ji[0][2], cz[n], fma(Ji[0][1], b[n], Ji[0][0] * a[n]));
+        const double ddy = fma(Ji[1][2], cz[n], fma(Ji[1][1], b[n], Ji[1][0] * a[n]));
+        const double ddz = fma(Ji[2][2], cz[n], fma(Ji[2][1], b[n], Ji[2][0] * a[n]));
         if (dxo) {
             dxo[n] = ddx;
             dyo[n] = ddy;
             dzo[n] = ddz;
         }
         const double u = nodes[n * 5 + 3], v = nodes[n * 5 + 4];
-        ux = ux + ddx * u; uy = uy + ddy * u; uz = uz + ddz * u;
-        vx = vx + ddx * v; vy = vy + ddy * v; vz = vz + ddz * v;
+        ux = fma(ddx, u, ux); uy = fma(ddy, u, uy); uz = fma(ddz, u, uz);
+        vx = fma(ddx, v, vx); vy = fma(ddy, v, vy); vz = fma(ddz, v, vz);
     }
     strain[0] = ux; strain[1] = uy; strain[2] = uz; strain[3] = vx; strain[4] = vy; strain[5] = vz;
     if (whichefvs == 0) {
@@ -480,10 +481,16 @@ __device__ __forceinline__ double ke_val(const double *__restrict__ base, int bl
 // One lane per node (k, i, j): A(m, node) = sum over elements in (j, i, k) order
 // (element_to_global_matrix_3d, velo_higher:7990-8054), then the local basal
 // friction term (basal_sliding_bc LOCAL, velo_higher:8216-8232).  A node with no
-// active element around it gets zero rows.  Node and element indices fit in 32
-// bits (the context rejects larger grids).
+// active element around it gets zero rows.  An owned node with an inactive vertex
+// and no active cell around it, now and at the previous assembly (av_zero, ac_zero,
+// when given), is not rewritten: the previous assembly left its rows zero, and
+// since then the Dirichlet update and the basal term touch active vertices only and
+// the halo update non-owned nodes.  Node and element indices fit in 32 bits (the
+// context rejects larger grids).
 __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__restrict__ ac,
                                                 const int32_t *__restrict__ av,
+                                                const int32_t *__restrict__ av_zero,
+                                                const int32_t *__restrict__ ac_zero,
                                                 const double *__restrict__ Ke,
                                                 const double *__restrict__ beta, int add_basal,
                                                 double basal_scale, double *__restrict__ A) {
@@ -495,7 +502,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
     const int nzm = g.nz - 1;
     // candidate elements: cell (i+dI, j+dJ), layer k-1+dK
     const double *base[2][2][2];
-    bool any = false;
+    bool any = false, was = false;
 #pragma unroll
     for (int dJ = 0; dJ < 2; ++dJ)
 #pragma unroll
@@ -505,6 +512,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                              jcl >= CISM_NHALO + 1 && jcl <= g.ny - CISM_NHALO + 1;
             const int cell = (icl - 1) + g.nx * (jcl - 1);
             const bool act = okc && ac[cell] != 0;
+            if (ac_zero) was = was || (okc && ac_zero[cell] != 0);
 #pragma unroll
             for (int dK = 0; dK < 2; ++dK) {
                 const int ke = k - 1 + dK;
@@ -515,6 +523,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
         }
     const bool basal = add_basal && k == g.nz && av[col];
     if (!any && !basal) {
+        if (av_zero && !was && !av[col] && !av_zero[col] && is_owned(g, i - 1, j - 1)) return;
         const double2 z = make_double2(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < 27; ++m) {
@@ -562,8 +571,14 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
     HIP_TRY(hipGetLastError());
     const double basal_scale = dx * dy / CISM_VOL0;
     hipLaunchKernelGGL(k_gather, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, c->ac, c->av,
-                       c->Ke, c->beta, add_basal, basal_scale, c->A);
+                       c->a_zero_ok ? c->av_zero : nullptr, c->a_zero_ok ? c->ac_zero : nullptr, c->Ke, c->beta,
+                       add_basal, basal_scale, c->A);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->av_zero, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy, hipMemcpyDeviceToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->ac_zero, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,
+                           c->stream));
+    c->a_zero_ok = 1;
     return CISM_OK;
 }
 

@@ -95,6 +95,12 @@ struct cism_ctx {
     double *xref;
     // element scratch (lazily)
     double *Ke;
+    // the active-vertex and active-cell masks of the last assembly (av_zero, ac_zero),
+    // valid when a_zero_ok: an owned row with no active cell around it and an inactive
+    // vertex, then and now, is still all zero, and the next assembly need not rewrite
+    // it.  Any other writer of A clears a_zero_ok.
+    int32_t *av_zero, *ac_zero;
+    int a_zero_ok;
     // reductions
     double *partials;
     double *red_groups;   // the first pass of a two-pass reduction (RED_GROUPS x CISM_MAX_NV)

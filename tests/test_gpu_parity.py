@@ -77,6 +77,35 @@ def test_assembly(gpu, prob):
         assert np.array_equal(a == 0.0, b == 0.0)
 
 
+def _assemble_into(ctx, p, u, v):
+    Ag = O.mat_zeros(p)
+    efg = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
+    ctx.assemble_stiffness_matrix_3d(p.sigma, p.active_cell, p.xVertex, p.yVertex, u, v, p.stagusrf,
+                                     p.stagthck, p.flwafact, p.whichapprox, efg, p.whichefvs,
+                                     p.efvs_constant, *Ag)
+    return Ag
+
+
+def test_assembly_after_another_ice_extent(gpu):
+    """A context that assembled one ice extent assembles another exactly as a fresh context
+    does: rows it leaves out as already zero are zero, rows that were active are cleared"""
+    big = configs.grf_sheet(ewn=36, nsn=42, nz=6, seed=3)
+    small = configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7)
+    assert (small.active_cell != 0).sum() < (big.active_cell != 0).sum()
+    rng = np.random.default_rng(5)
+    u = np.asfortranarray(30.0 * rng.standard_normal(small.uvel.shape))
+    v = np.asfortranarray(30.0 * rng.standard_normal(small.vvel.shape))
+    with gpu.Context(small.nx, small.ny, small.nz, small.periodic) as ctx:
+        fresh_small = _assemble_into(ctx, small, u, v)
+    with gpu.Context(small.nx, small.ny, small.nz, small.periodic) as ctx:
+        fresh_big = _assemble_into(ctx, big, u, v)
+    with gpu.Context(small.nx, small.ny, small.nz, small.periodic) as ctx:
+        seq = [_assemble_into(ctx, p, u, v) for p in (big, small, small, big, small)]
+    for got, want in zip(seq, (fresh_big, fresh_small, fresh_small, fresh_big, fresh_small)):
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+
+
 def test_dirichlet_basal_symmetry_counts(gpu, prob):
     p = prob
     rng = np.random.default_rng(11)
