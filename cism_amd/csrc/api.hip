@@ -229,7 +229,9 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     // per-block partial sums: up to CISM_MAX_NV values per block (the widest reduction)
     if ((st = dalloc(&c->partials, nparts * CISM_MAX_NV))) return fail(st);
     if ((st = dalloc(&c->red_groups, (size_t)64 * CISM_MAX_NV))) return fail(st);
-    if ((st = dalloc(&c->av_zero, ncol)) || (st = dalloc(&c->ac_zero, (size_t)nx * ny))) return fail(st);
+    if ((st = dalloc(&c->av_zero, ncol)) || (st = dalloc(&c->ac_zero, (size_t)nx * ny)) ||
+        (st = dalloc(&c->as_av_prev, ncol)) || (st = dalloc(&c->as_ac_prev, (size_t)nx * ny)))
+        return fail(st);
     if ((st = dalloc(&c->sc, 1))) return fail(st);
     if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
         return fail(CISM_ERR_HIP);
@@ -244,7 +246,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
-                    c->dsdx, c->dsdy, c->Ke, c->av_zero, c->ac_zero, c->xref, c->partials, c->red_groups, c->sc, c->stage,
+                    c->dsdx, c->dsdy, c->Ke, c->av_zero, c->ac_zero, c->as_av_prev, c->as_ac_prev, c->xref, c->partials, c->red_groups, c->sc, c->stage,
                     c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
                     c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg};
     for (void *p : ptrs)

@@ -691,15 +691,32 @@ int dirichlet_device(cism_ctx *c) {
 // writes its node's FP32 symmetric storage (slots CISM_DIAG .. 26, sidx) from the
 // repaired values, which saves the mixed-precision path a separate conversion pass.
 // ---------------------------------------------------------------------------
+//
+// skip (ac, ac_prev, av_prev; when A is the assembly's and As was written by such a
+// pass before): an owned node with an inactive vertex and no active cell around it,
+// now and at that pass, is left out.  Its rows of A are zero (no element touches
+// it, and the Dirichlet and basal updates act on active vertices), so are its
+// couplings in the neighbours' rows, and its rows of As are zero from that pass.
 __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__restrict__ av,
                                                     double *__restrict__ A, float *__restrict__ As,
+                                                    const int32_t *__restrict__ ac,
+                                                    const int32_t *__restrict__ ac_prev,
+                                                    const int32_t *__restrict__ av_prev,
                                                     double *__restrict__ partials) {
     double acc[2] = {0.0, 0.0};
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < g.nn) {
-        const int k = (int)(q % g.nz);
-        const long long col = q / g.nz;
-        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    bool skip = q >= g.nn;
+    int k = 0, i = 0, j = 0;
+    long long col = 0;
+    if (!skip) {
+        k = (int)(q % g.nz);
+        col = q / g.nz;
+        i = (int)(col % g.sx);
+        j = (int)(col / g.sx);
+        skip = ac && !av[col] && !av_prev[col] && is_owned(g, i, j) && !touches_active_cell(g, ac, i, j) &&
+               !touches_active_cell(g, ac_prev, i, j);
+    }
+    if (!skip) {
         const bool vis_q = is_owned(g, i, j) && av[col];
         const double dgu = A[aidx(0, CISM_DIAG, q)], dgv = A[aidx(3, CISM_DIAG, q)];
         double warn = 0.0, nnz = 0.0;
@@ -780,9 +797,23 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
 // symmetry repair; sums[0] = warnings, sums[1] = nonzeros of owned active rows.  As: the
 // FP32 symmetric storage to write as well, or null.
 int symmetry_device(cism_ctx *c, float *As) {
-    const int nb = nblocks_for(c->g.nn);
-    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, As, c->partials);
+    const GridDev &g = c->g;
+    const int nb = nblocks_for(g.nn);
+    // rows outside the ice may be left out only when A is the assembly's (a_zero_ok)
+    // and As was last written from such an A
+    const bool skip = As && c->a_zero_ok && c->as_ok;
+    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, g, c->av, c->A, As,
+                       skip ? c->ac : nullptr, c->as_ac_prev, c->as_av_prev, c->partials);
     HIP_TRY(hipGetLastError());
+    if (As) {
+        c->as_ok = c->a_zero_ok;
+        if (c->as_ok) {
+            HIP_TRY(hipMemcpyAsync(c->as_av_prev, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy,
+                                   hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(c->as_ac_prev, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
     CISM_TRY(launch_reduce(c, nb, 2, RED_SUMS));
     return CISM_OK;
 }

@@ -101,6 +101,10 @@ struct cism_ctx {
     // it.  Any other writer of A clears a_zero_ok.
     int32_t *av_zero, *ac_zero;
     int a_zero_ok;
+    // the same record for the FP32 symmetric matrix: the masks of the last symmetry
+    // pass that wrote it from an assembled A (as_av_prev, as_ac_prev), valid when as_ok
+    int32_t *as_av_prev, *as_ac_prev;
+    int as_ok;
     // reductions
     double *partials;
     double *red_groups;   // the first pass of a two-pass reduction (RED_GROUPS x CISM_MAX_NV)
@@ -158,6 +162,21 @@ __device__ inline int halo_image(int i, int s, int periodic) {
 
 __device__ inline bool is_owned(const GridDev &g, int i, int j) {
     return i >= g.ilo && i <= g.ihi && j >= g.jlo && j <= g.jhi;
+}
+
+// whether staggered vertex (i, j) (0-based) touches an active cell of ac among the
+// cells the assembly visits (cells nhalo+1 .. n-nhalo+1, 1-based)
+__device__ inline bool touches_active_cell(const GridDev &g, const int32_t *__restrict__ ac, int i, int j) {
+    bool any = false;
+#pragma unroll
+    for (int dJ = 0; dJ < 2; ++dJ)
+#pragma unroll
+        for (int dI = 0; dI < 2; ++dI) {
+            const int ic = i + dI, jc = j + dJ;
+            if (ic >= CISM_NHALO && ic <= g.nx - CISM_NHALO && jc >= CISM_NHALO && jc <= g.ny - CISM_NHALO)
+                any = any || ac[ic + (long long)g.nx * jc] != 0;
+        }
+    return any;
 }
 
 // ---------------------------------------------------------------------------

@@ -106,6 +106,27 @@ def test_assembly_after_another_ice_extent(gpu):
             assert np.array_equal(a, b)
 
 
+def test_mixed_solve_after_another_ice_extent(gpu):
+    """Mixed-precision solves on one context across two ice extents equal those of fresh
+    contexts: the FP32 matrix rows that the symmetry pass leaves out are zero"""
+    big = configs.grf_sheet(ewn=36, nsn=42, nz=6, seed=3)
+    small = configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7)
+    kw = dict(precision=1, adaptive_linear=1, inner_rtol=1e-2, maxiter_nonlinear=4)
+
+    def solve(ctx, p):
+        lu, lv = ctx.loads(p)
+        r = ctx.velo_higher_solve_3d(p, lu, lv, **kw)
+        return np.concatenate([r["uvel"].ravel(), r["vvel"].ravel()])
+
+    fresh = {}
+    for name, p in (("big", big), ("small", small)):
+        with gpu.Context(small.nx, small.ny, small.nz, small.periodic) as ctx:
+            fresh[name] = solve(ctx, p)
+    with gpu.Context(small.nx, small.ny, small.nz, small.periodic) as ctx:
+        for name, p in (("big", big), ("small", small), ("big", big), ("small", small)):
+            assert np.array_equal(solve(ctx, p), fresh[name]), name
+
+
 def test_dirichlet_basal_symmetry_counts(gpu, prob):
     p = prob
     rng = np.random.default_rng(11)
