@@ -109,6 +109,12 @@ def main(argv=None):
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-fast", type=int, default=0, metavar="N",
+                    help="for kernel profiles: run only the learned-guess fast solve N times, "
+                         "for a fixed number of Picard iterations (--picard), without the "
+                         "reference solve and the other legs")
+    ap.add_argument("--picard", type=int, default=12,
+                    help="Picard iterations of --profile-fast (the headline solve takes 12)")
     ap.add_argument("--baseline-steps", type=int, default=1)
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -156,6 +162,22 @@ def main(argv=None):
         msc = ctx.time_kernel(3, a.steps)
         print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
                           "fp32_matvec_ms": ms32, "copy_1GiB_ms": msc, "launches": a.steps}))
+        ctx.close()
+        return 0
+
+    if a.profile_fast:
+        weights = a.weights or (SG.default_weights(family) if family else None)
+        u0 = v0 = None
+        if weights is not None and os.path.exists(weights):
+            pred = SG.Predictor(p, weights)
+            u0, v0 = pred.predict(p)
+            pred.close()
+        for s in range(a.profile_fast):
+            r = ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
+                                         inner_rtol=a.inner_rtol, maxiter_nonlinear=a.picard,
+                                         anderson_m=a.anderson)
+            log(f"fast solve {s}: {r['seconds_device']:.3f} s device, {r['picard_iters']} Picard, "
+                f"{r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 matvecs")
         ctx.close()
         return 0
 
