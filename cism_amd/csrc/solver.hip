@@ -735,7 +735,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
 // want_rr the block also leaves (r, r) over its owned nodes in partials (the
 // convergence check of the inner solves, pcg:1540-1560).
 template <typename T, int NZMAX>
-__global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T *__restrict__ Az,
+__global__ __launch_bounds__(256) void k_cg_update_sia(GridDev g, T *z, const T *__restrict__ Az,
                                                        T *__restrict__ d, T *__restrict__ qv,
                                                        T *__restrict__ x, T *__restrict__ r,
                                                        const DevScal *__restrict__ sc, int first,
@@ -749,9 +749,10 @@ __global__ __launch_bounds__(128) void k_cg_update_sia(GridDev g, T *z, const T 
     const int ncols = (int)(col0 + 64 <= ncol ? 64 : ncol - col0);
     const long long Q0 = col0 * nz;
     const int nn = ncols * nz;
-    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+    // the update runs on all 256 lanes; the column solves on the first 128 (column, component)
+    const int comp = (threadIdx.x >> 6) & 1, lc = threadIdx.x & 63;
     const long long col = col0 + lc;
-    const bool solve = lc < ncols && av[col];
+    const bool solve = threadIdx.x < 128 && lc < ncols && av[col];
     SiaFactors<T, (NZMAX <= 12 ? NZMAX : 1)> f;
     if constexpr (NZMAX <= 12)
         if (solve) sia_factors<T, NZMAX>(nz, ncol, F, col, comp, f);
@@ -828,13 +829,13 @@ static int launch_cg_update(cism_ctx *c, int precond, CGVecs<T> &v, int first, i
         // fused with the SIA preconditioner; the caller skips its own precond call
         const dim3 grid((unsigned)sia_blocks(g));
         if (g.nz <= 12)
-            hipLaunchKernelGGL((k_cg_update_sia<T, 12>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+            hipLaunchKernelGGL((k_cg_update_sia<T, 12>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
                                v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
         else if (g.nz <= 24)
-            hipLaunchKernelGGL((k_cg_update_sia<T, 24>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+            hipLaunchKernelGGL((k_cg_update_sia<T, 24>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
                                v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
         else
-            hipLaunchKernelGGL((k_cg_update_sia<T, 48>), grid, dim3(128), 0, c->stream, g, v.z, v.Az, v.d,
+            hipLaunchKernelGGL((k_cg_update_sia<T, 48>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
                                v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
     } else {
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g, v.z,
