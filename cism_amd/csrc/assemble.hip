@@ -880,13 +880,16 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
         if (!none && av[qs / g.nz]) {
             act = true;
             double su = 0.0, sv = 0.0;
+#pragma unroll
             for (int m = 0; m < 27; ++m) {
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
                 if (k + kA < 0 || k + kA >= g.nz) continue;
                 const long long qn = qs + kA + iA * cst + jA * rst;
-                const double xu = x[2 * qn], xv = x[2 * qn + 1];
-                su = su + A[aidx(0, m, qs)] * xu + A[aidx(1, m, qs)] * xv;
-                sv = sv + A[aidx(2, m, qs)] * xu + A[aidx(3, m, qs)] * xv;
+                const double2 xx = *reinterpret_cast<const double2 *>(x + 2 * qn);
+                double uu, uv, vu, vv;
+                load_blocks(A, qs, m, uu, uv, vu, vv);
+                su = su + uu * xx.x + uv * xx.y;
+                sv = sv + vu * xx.x + vv * xx.y;
             }
             ru = su - b[2 * qs];
             rv = sv - b[2 * qs + 1];

@@ -3,16 +3,19 @@
 // Device data layout:
 //   node q (0-based) = k + nz*(i + sx*j), sx = nx-1, sy = ny-1, k fastest --
 //   the reference's vector order (nz, nx-1, ny-1).
-//   Matrix: tiles of 64 nodes (one wavefront); inside a tile, slot-major, then
-//   lane, then block b (0 Auu, 1 Auv, 2 Avu, 3 Avv):
-//       A[aidx(b, m, q)],  aidx = ((q/64)*27 + m)*256 + (q%64)*4 + b
-//   so one wave's coefficients are one contiguous 27*64*4-value region (every
-//   load instruction covers 1 KB (FP32, 4 blocks) or 2 x 1 KB (FP64, 2 blocks)),
-//   and a lane gets all four blocks of a slot with one 16-byte (FP32) or two
-//   16-byte (FP64) loads.  npad (nz*sx*sy rounded up to 256) keeps tiles whole.
+//   Matrix (FP64): tiles of 64 nodes (one wavefront); inside a tile, slot-major,
+//   then the block pair (Auu, Auv) or (Avu, Avv), then lane, then the block in the
+//   pair (b = 0 Auu, 1 Auv, 2 Avu, 3 Avv):
+//       A[aidx(b, m, q)],  aidx = ((q/64)*27 + m)*256 + (b/2)*128 + (q%64)*2 + b%2
+//   so one wave's coefficients are one contiguous 27*64*4-value region, a lane
+//   gets the four blocks of a slot with two 16-byte loads, and each of those load
+//   instructions reads 1 KB contiguous over the wave.  The FP32 symmetric copy
+//   (sidx) keeps a lane's four blocks together: one 16-byte load, 1 KB per wave.
+//   npad (nz*sx*sy rounded up to 256) keeps tiles whole.
 //   Vectors interleave u and v: x[2*q + c], c = 0 (u), 1 (v).
-//   Element scratch: Ke[entry*nepad + e], e = (k-1) + (nz-1)*((i-1) + nx*(j-1)),
-//   the reference's efvs layout (nz-1, nx, ny).
+//   Element scratch: keidx(entry, e), tiles of 64 elements, entry-major inside a
+//   tile; e = (k-1) + (nz-1)*((i-1) + nx*(j-1)), the reference's efvs layout
+//   (nz-1, nx, ny).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -121,9 +124,10 @@ __host__ __device__ inline int slot_of(int iA, int jA, int kA) {
     return (iA + 1) + 3 * (jA + 1) + 9 * (kA + 1);
 }
 
-// position of coefficient (block b, slot m) of node q in the tiled matrix layout
+// position of coefficient (block b, slot m) of node q in the tiled matrix layout;
+// blocks b and b ^ 1 (b = 0, 2) are adjacent
 __host__ __device__ inline long long aidx(int b, int m, long long q) {
-    return (((q >> 6) * 27 + m) << 8) + ((q & 63) << 2) + b;
+    return (((q >> 6) * 27 + m) << 8) + ((long long)(b >> 1) << 7) + ((q & 63) << 1) + (b & 1);
 }
 
 // the four blocks (uu, uv, vu, vv) of slot m of node q: one 16-byte load (FP32) or two (FP64)
