@@ -137,6 +137,17 @@ static int up(cism_ctx *c, T *dst, const T *src, size_t n) {
     HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
     return CISM_OK;
 }
+
+// uploads the Dirichlet masks (n entries each) and records whether any node is
+// prescribed (mask value 1); with none, the Dirichlet update changes nothing and is skipped
+static int upload_dirichlet_masks(cism_ctx *c, const int32_t *um, const int32_t *vm, size_t n) {
+    CISM_TRY(up(c, c->umask, um, n));
+    CISM_TRY(up(c, c->vmask, vm, n));
+    int any = 0;
+    for (size_t t = 0; t < n && !any; ++t) any = um[t] == 1 || vm[t] == 1;
+    c->dirichlet_any = any;
+    return CISM_OK;
+}
 template <typename T>
 static int down(cism_ctx *c, T *dst, const T *src, size_t n) {
     HIP_TRY(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
@@ -428,8 +439,7 @@ extern "C" int cism_dirichlet_boundary_conditions_3d(cism_ctx *c, const int32_t 
                                                      double *Avv, double *bu, double *bv) {
     CTX_BEGIN(c);
     CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
-    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)c->g.nn));
-    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)c->g.nn));
+    CISM_TRY(upload_dirichlet_masks(c, umask_dirichlet, vmask_dirichlet, (size_t)c->g.nn));
     CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     CISM_TRY(upload_vec(c, c->b, bu, bv));
     CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
@@ -762,8 +772,7 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     CISM_TRY(upload_geometry(c, sigma, active_cell, xVertex, yVertex, stagusrf, stagthck, flwafact));
     CISM_TRY(up(c, c->av, active_vertex, ncol));
     CISM_TRY(up(c, c->beta, beta_internal, ncol));
-    CISM_TRY(up(c, c->umask, umask_dirichlet, (size_t)g.nn));
-    CISM_TRY(up(c, c->vmask, vmask_dirichlet, (size_t)g.nn));
+    CISM_TRY(upload_dirichlet_masks(c, umask_dirichlet, vmask_dirichlet, (size_t)g.nn));
     CISM_TRY(upload_vec(c, c->load, loadu, loadv));
     CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
@@ -824,8 +833,7 @@ extern "C" int cism_velo_higher_solve_2d(cism_ctx *c, const cism_solve_opts *o, 
     CISM_TRY(up(c, c->sthck, stagthck, ncol));
     CISM_TRY(up(c, c->av, active_vertex, ncol));
     CISM_TRY(up(c, c->beta, beta_internal, ncol));
-    CISM_TRY(up(c, c->umask, um2.data(), ncol));
-    CISM_TRY(up(c, c->vmask, vm2.data(), ncol));
+    CISM_TRY(upload_dirichlet_masks(c, um2.data(), vm2.data(), ncol));
     CISM_TRY(upload_vec(c, c->load, lu2.data(), lv2.data()));
     CISM_TRY(upload_vec(c, c->x, u2.data(), v2.data()));
     CISM_TRY(ssa_prepare_device(c));

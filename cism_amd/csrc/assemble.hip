@@ -662,6 +662,7 @@ __global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__r
 }
 
 int dirichlet_device(cism_ctx *c) {
+    if (!c->dirichlet_any) return CISM_OK;   // no prescribed node: nothing to replace or eliminate
     hipLaunchKernelGGL(k_dirichlet, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream, c->g, c->av,
                        c->umask, c->vmask, c->x, c->A, c->b);
     HIP_TRY(hipGetLastError());

@@ -92,6 +92,7 @@ struct cism_ctx {
     double aa_fnorm_prev;
     // masks and geometry
     int32_t *av, *ac, *umask, *vmask;
+    int dirichlet_any;   // 0: the Dirichlet masks as uploaded prescribe no node
     double *xV, *yV, *susrf, *sthck, *beta, *sigma, *wsig, *flwafact, *efvs;
     double *dsdx, *dsdy;
     // FP64 reference velocity (lazily) for error reports

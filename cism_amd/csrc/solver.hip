@@ -414,7 +414,7 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_copy(long long n, const T *__res
 // An optional per-column scale s (cs[comp*ncol + col], s > 0) turns the
 // preconditioner into z = s M^-1 r by storing sb/s and beta/s (gamma is a ratio
 // and unchanged): still symmetric positive definite, and free in the apply.
-template <typename T, typename M>
+template <typename T, typename M, int NZMAX>
 __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
                                                           const int32_t *__restrict__ av,
                                                           const float *__restrict__ cs,
@@ -429,18 +429,30 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
     const long long qb = col * nz;
     auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + col; };
     const T inv = cs ? (T)(1.0f / cs[t]) : (T)1;
-    T beta = A.entry(g, blk, CISM_DIAG, qb);
+    // the column's three diagonals, all loaded before the first store (F could alias
+    // the matrix as far as the compiler knows, and each level would wait on its loads)
+    T dg[NZMAX], up[NZMAX], lo[NZMAX];
+#pragma unroll
+    for (int k = 0; k < NZMAX; ++k)
+        if (k < nz) {
+            dg[k] = A.entry(g, blk, CISM_DIAG, qb + k);
+            up[k] = k + 1 < nz ? A.entry(g, blk, CISM_DIAG + 9, qb + k) : (T)0;
+            lo[k] = k > 0 ? A.entry(g, blk, CISM_DIAG - 9, qb + k) : (T)0;
+        }
+    T beta = dg[0];
     F[Fi(0, 0)] = (T)0;
     F[Fi(1, 0)] = cs ? beta * inv : beta;
     F[Fi(2, 0)] = (T)0;
-    for (int k = 1; k < nz; ++k) {
-        const T gam = A.entry(g, blk, CISM_DIAG + 9, qb + k - 1) / beta;   // super(k-1) / beta(k-1)
-        const T sb = A.entry(g, blk, CISM_DIAG - 9, qb + k);
-        beta = A.entry(g, blk, CISM_DIAG, qb + k) - sb * gam;
-        F[Fi(0, k)] = cs ? sb * inv : sb;
-        F[Fi(1, k)] = cs ? beta * inv : beta;
-        F[Fi(2, k)] = gam;
-    }
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k)
+        if (k < nz) {
+            const T gam = up[k - 1] / beta;   // super(k-1) / beta(k-1)
+            const T sb = lo[k];
+            beta = dg[k] - sb * gam;
+            F[Fi(0, k)] = cs ? sb * inv : sb;
+            F[Fi(1, k)] = cs ? beta * inv : beta;
+            F[Fi(2, k)] = gam;
+        }
 }
 
 // One column and component of the SIA preconditioner: the Thomas substitutions
@@ -536,8 +548,17 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
 template <typename T, typename M>
 static int launch_sia_setup(cism_ctx *c, const M A, const float *cs, T *F) {
     const long long ncol = (long long)c->g.sx * c->g.sy;
-    hipLaunchKernelGGL((k_sia_setup<T, M>), dim3(nblocks_for(2 * ncol)), dim3(CISM_BLOCK), 0, c->stream, c->g,
-                       A, c->av, cs, F);
+    const dim3 grid(nblocks_for(2 * ncol));
+    if (c->g.nz <= 12)
+        hipLaunchKernelGGL((k_sia_setup<T, M, 12>), grid, dim3(CISM_BLOCK), 0, c->stream, c->g, A, c->av, cs, F);
+    else if (c->g.nz <= 24)
+        hipLaunchKernelGGL((k_sia_setup<T, M, 24>), grid, dim3(CISM_BLOCK), 0, c->stream, c->g, A, c->av, cs, F);
+    else if (c->g.nz <= 48)
+        hipLaunchKernelGGL((k_sia_setup<T, M, 48>), grid, dim3(CISM_BLOCK), 0, c->stream, c->g, A, c->av, cs, F);
+    else {
+        cism_set_error("SIA preconditioner supports nz <= 48 (nz = %d)", c->g.nz);
+        return CISM_ERR_UNSUPPORTED;
+    }
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }

@@ -162,6 +162,7 @@ extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, doub
     CISM_TRY(ensure_ke(c));
     double *d_thck = c->Ke, *d_usrf = c->Ke + ncell, *d_topg = c->Ke + 2 * ncell, *d_flwa = c->Ke + 3 * ncell;
     int32_t *bits = c->umask;   // overwritten by the next solve's Dirichlet masks
+    c->dirichlet_any = 1;   // the masks are scratch now: no longer known empty
     HIP_TRY(hipMemcpyAsync(d_thck, thck, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(d_usrf, usrf, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(d_topg, topg, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
@@ -179,6 +180,7 @@ extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, doub
     HIP_TRY(hipGetLastError());
     int32_t *masks[4] = {ice_mask, floating_mask, ocean_mask, land_mask};
     int32_t *mbuf = c->vmask;   // scratch for one mask at a time
+    c->dirichlet_any = 1;   // the masks are scratch now: no longer known empty
     for (int b = 0; b < 4; ++b) {
         if (!masks[b]) continue;
         hipLaunchKernelGGL(k_bit, dim3(nblocks_for(ncell)), dim3(256), 0, c->stream, ncell, bits, b, mbuf);
@@ -370,6 +372,7 @@ extern "C" int cism_glissade_grounded_fraction(cism_ctx *c, const int32_t *ice_m
     const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
     // masks staged in the node-mask buffers (nn >= ncell ints each) and the load vector
     int32_t *d_ice = c->umask, *d_fl = c->vmask, *d_land = c->umask + ncell;
+    c->dirichlet_any = 1;   // the masks are scratch now: no longer known empty
     if (3 * ncell > 2 * g.nn) {
         cism_set_error("grounded_fraction: grid too small for the staging buffers");
         return CISM_ERR_ARG;
