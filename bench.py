@@ -102,9 +102,9 @@ def main(argv=None):
     ap.add_argument("--guess", default="learned", choices=["learned", "none"])
     ap.add_argument("--weights", default=None,
                     help="network weights (default: the workload family's file, if trained)")
-    ap.add_argument("--anderson", type=int, default=4,
+    ap.add_argument("--anderson", type=int, default=3,
                     help="Anderson acceleration depth of the fast path's Picard loop (0 off)")
-    ap.add_argument("--inner-rtol", type=float, default=1e-2,
+    ap.add_argument("--inner-rtol", type=float, default=5e-2,
                     help="relative residual of each FP32 inner solve of the refinement")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
@@ -113,8 +113,8 @@ def main(argv=None):
                     help="for kernel profiles: run only the learned-guess fast solve N times, "
                          "for a fixed number of Picard iterations (--picard), without the "
                          "reference solve and the other legs")
-    ap.add_argument("--picard", type=int, default=12,
-                    help="Picard iterations of --profile-fast (the headline solve takes 12)")
+    ap.add_argument("--picard", type=int, default=13,
+                    help="Picard iterations of --profile-fast (the headline solve takes 13)")
     ap.add_argument("--baseline-steps", type=int, default=1)
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -141,9 +141,10 @@ def main(argv=None):
     cap = 500
 
     def fast(u0=None, v0=None):
-        # inner FP32 solves to 1e-2 relative residual per refinement step: chosen on the
-        # held-out ice sheet of seed 7 (1e-3: 1.20 s, 1e-2: 0.87 s, 3e-2: 0.86 s), not on
-        # this workload
+        # inner FP32 solves to 5e-2 relative residual per refinement step and Anderson depth
+        # 3: chosen on the held-out ice sheets of seeds 7 and 9, not on this workload.  Mean
+        # device seconds over the two: 1e-2/AA4 0.703, 1e-2/AA3 0.695, 2e-2/AA4 0.701,
+        # 3e-2/AA4 0.698, 5e-2/AA4 0.695, 5e-2/AA3 0.650 (best on each sheet)
         return ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, precision=1, adaptive_linear=1,
                                         stop_rel_err=STOP_REL_ERR, inner_rtol=a.inner_rtol,
                                         maxiter_nonlinear=cap, anderson_m=a.anderson)

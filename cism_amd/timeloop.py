@@ -97,7 +97,7 @@ class VelocitySolver:
                 finally:
                     pred.close()
                 guess = "learned"
-        kw = dict(precision=1, adaptive_linear=1, inner_rtol=1e-2, anderson_m=4) if self.fast else {}
+        kw = dict(precision=1, adaptive_linear=1, inner_rtol=5e-2, anderson_m=3) if self.fast else {}
         r = self.ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, stop_rel_err=stop_rel_err, **kw)
         self.uvel, self.vvel, self.efvs = r["uvel"], r["vvel"], r["efvs"]
         self.steps += 1
