@@ -860,9 +860,10 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
     const long long cst = g.nz, rst = (long long)g.nz * g.sx;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+    // one lane per node
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.nn) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
         int i = (int)(col % g.sx), j = (int)(col / g.sx);
@@ -923,7 +924,7 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
 }
 
 int residual_vector_device(cism_ctx *c, double *ru, double *rs) {
-    const int nb = red_blocks_for(c->g.nn);
+    const int nb = nblocks_for(c->g.nn);
     if (rs)
         hipLaunchKernelGGL(k_resvec<true>, dim3(nb), dim3(256), 0, c->stream, c->g, c->av, c->A, c->x, c->b,
                            ru, rs, c->partials);
