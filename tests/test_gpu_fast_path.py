@@ -50,6 +50,28 @@ def test_anderson_reaches_target(setup, m):
     assert 0 <= r["final_rel_err"] <= 1e-6
 
 
+@pytest.mark.parametrize("m", [0, 3])
+def test_bench_settings_on_a_sheet_with_inactive_columns(m):
+    """the bench's fast-path settings (inner FP32 tolerance 5e-2, Anderson depth 3) on an ice
+    sheet whose margin leaves most columns inactive: the Anderson passes cover active columns
+    only, and the solve still reaches 1e-6 against the FP64 reference"""
+    p = configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7)
+    assert (p.active_vertex == 0).sum() > 0
+    with G.Context.for_problem(p) as ctx:
+        lu, lv = ctx.loads(p)
+        ref = reference_solution(ctx, p, lu, lv)
+        ctx.set_reference_velocity(ref["uvel"], ref["vvel"])
+        r = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, stop_rel_err=1e-6,
+                                     inner_rtol=5e-2, maxiter_nonlinear=300, anderson_m=m)
+    print(f"[fast] sheet, anderson {m}: {r['picard_iters']} Picard, {r['fp32_matvecs']} FP32 "
+          f"matvecs, err {r['final_rel_err']:.2e}")
+    assert 0 <= r["final_rel_err"] <= 1e-6
+    # the solve leaves the velocity of inactive owned columns as it found it (zero)
+    inact = np.zeros((p.nx - 1, p.ny - 1), dtype=bool)
+    inact[2:-2, 2:-2] = p.active_vertex[2:-2, 2:-2] == 0
+    assert np.all(r["uvel"][:, inact] == 0.0) and np.all(r["vvel"][:, inact] == 0.0)
+
+
 def test_constant_precond_scale_is_invisible(setup):
     p, ctx, lu, lv, ref = setup
     base = fast(ctx, p, lu, lv)
