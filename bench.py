@@ -113,8 +113,8 @@ def main(argv=None):
                     help="for kernel profiles: run only the learned-guess fast solve N times, "
                          "for a fixed number of Picard iterations (--picard), without the "
                          "reference solve and the other legs")
-    ap.add_argument("--picard", type=int, default=13,
-                    help="Picard iterations of --profile-fast (the headline solve takes 13)")
+    ap.add_argument("--picard", type=int, default=14,
+                    help="Picard iterations of --profile-fast (the headline solve takes 14)")
     ap.add_argument("--baseline-steps", type=int, default=1)
     a = ap.parse_args(argv)
     if a.gpus != 1:
