@@ -3,7 +3,7 @@
 // (glissade_velo_higher.F90).
 //
 // Assembly is split in two atomic-free kernels:
-//  1. k_element: one workgroup = 32 hexahedral elements x 8 lanes.  Lane p
+//  1. k_element: one workgroup = 16 hexahedral elements x 8 lanes.  Lane p
 //     evaluates quadrature point p (Jacobian, inverse, basis derivatives,
 //     Glen-law viscosity) into LDS; then lane a sums over the points the rows of
 //     node a of the five point-weighted products Pxx = sum_p f x_a x_b, Pyy,
@@ -37,7 +37,9 @@ __host__ __device__ constexpr int tri_index(int a, int b) {
     return a <= b ? (a * (15 - a)) / 2 + b : (b * (15 - b)) / 2 + a;
 }
 
-#define EL_PER_WG 32
+#define EL_PER_WG 32     // elements per workgroup of the stress kernel
+#define EL_ELEM 16       // elements per workgroup of k_element (8 lanes each): measured
+                         // best of 8, 16, 24 and 32 at the 1200 x 1400 x 11 size
 
 // element scratch: tiles of 64 elements, entry-major inside a tile, so a wave's
 // reads and writes of one entry are one contiguous 512-byte run
@@ -53,9 +55,9 @@ __host__ __device__ inline long long keidx(int ent, long long e) {
 #define QP_REC 26
 #define EL_REC (8 * QP_REC + 2)
 struct QPLds {
-    alignas(16) double rec[EL_PER_WG][EL_REC];
-    double ev[EL_PER_WG][9];
-    double nodes[EL_PER_WG][8 * 5 + 1];                                                  // x y z u v
+    alignas(16) double rec[EL_ELEM][EL_REC];
+    double ev[EL_ELEM][9];
+    double nodes[EL_ELEM][8 * 5 + 1];                                                  // x y z u v
 };
 
 // One quadrature point p of a hexahedral element whose 8 nodes are given as
@@ -153,7 +155,7 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
 }
 
 template <int APPROX>
-__global__ __launch_bounds__(256) void k_element(GridDev g, const double *__restrict__ sigma,
+__global__ __launch_bounds__(8 * EL_ELEM) void k_element(GridDev g, const double *__restrict__ sigma,
                                                  const int32_t *__restrict__ ac,
                                                  const double *__restrict__ xV,
                                                  const double *__restrict__ yV,
@@ -165,10 +167,10 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
                                                  double *__restrict__ Ke, double *__restrict__ efvs,
                                                  DevScal *__restrict__ sc) {
     __shared__ QPLds L;
-    __shared__ int valid[EL_PER_WG];
+    __shared__ int valid[EL_ELEM];
     const int el = threadIdx.x >> 3, sub = threadIdx.x & 7;
     // element and node indices fit in 32 bits (the context rejects larger grids)
-    const int e = blockIdx.x * EL_PER_WG + el;
+    const int e = blockIdx.x * EL_ELEM + el;
     const int nzm = g.nz - 1;
     int k = 0, ic = 0, jc = 0;
     bool ok = e < (int)g.ne;
@@ -184,8 +186,8 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
     if (sub == 0) valid[el] = ok;
     // a group with no active element only clears its efvs
     if (!__syncthreads_or(ok)) {
-        if (threadIdx.x < EL_PER_WG && blockIdx.x * EL_PER_WG + (int)threadIdx.x < (int)g.ne)
-            efvs[blockIdx.x * EL_PER_WG + threadIdx.x] = 0.0;
+        if (threadIdx.x < EL_ELEM && blockIdx.x * EL_ELEM + (int)threadIdx.x < (int)g.ne)
+            efvs[blockIdx.x * EL_ELEM + threadIdx.x] = 0.0;
         return;
     }
     // phase 0: lane (el, n) gathers node n (velo_higher:5100-5127)
@@ -215,8 +217,8 @@ __global__ __launch_bounds__(256) void k_element(GridDev g, const double *__rest
     }
     __syncthreads();
     // element efvs = mean over quadrature points (velo_higher:5167-5172)
-    if (threadIdx.x < EL_PER_WG) {
-        const int ee = blockIdx.x * EL_PER_WG + threadIdx.x;
+    if (threadIdx.x < EL_ELEM) {
+        const int ee = blockIdx.x * EL_ELEM + threadIdx.x;
         if (ee < (int)g.ne) {
             double s = 0.0;
             if (valid[threadIdx.x])
@@ -564,9 +566,9 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
                     int add_basal, double dx, double dy) {
     CISM_TRY(ensure_ke(c));
     const GridDev &g = c->g;
-    const int nb = (int)((g.ne + EL_PER_WG - 1) / EL_PER_WG);
+    const int nb = (int)((g.ne + EL_ELEM - 1) / EL_ELEM);
     auto kern = whichapprox == 0 ? k_element<0> : whichapprox == 1 ? k_element<1> : k_element<2>;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV, c->x,
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(8 * EL_ELEM), 0, c->stream, g, c->sigma, c->ac, c->xV, c->yV, c->x,
                        c->susrf, c->sthck, c->flwafact, whichefvs, efvs_constant, c->Ke, c->efvs, c->sc);
     HIP_TRY(hipGetLastError());
     const double basal_scale = dx * dy / CISM_VOL0;
