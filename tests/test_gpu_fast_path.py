@@ -1,5 +1,5 @@
 """The mixed-precision fast path's options on the GPU: every variant must reach the same
-1e-6 error against the FP64 reference (the plain FP64 path is the judge), and the
+1e-6 error against the FP64 reference (the plain FP64 path is the arbiter), and the
 per-column preconditioner scale must act as a scale (a constant one leaves Chron-Gear's
 iterates unchanged up to FP32 rounding)."""
 import numpy as np

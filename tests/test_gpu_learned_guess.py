@@ -1,6 +1,6 @@
 """What the trained network gains: the same solves from a zero initial guess and from the
 network's prediction, on a held-out problem, each run until its relative L2 velocity error
-against the FP64 reference is <= 1e-6 (the plain FP64 path stays the judge of the error)."""
+against the FP64 reference is <= 1e-6 (the plain FP64 path stays the arbiter of the error)."""
 import os
 import time
 
