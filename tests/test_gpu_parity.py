@@ -33,6 +33,8 @@ def problems():
         "sheet": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7),
         # a floating shelf fringe: lateral ice-ocean load at floating fronts, f_ground = 0
         "shelf": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7, shelf=0.02),
+        # more than 12 levels: the column kernels' 24-level variants
+        "deep": configs.grf_sheet(ewn=16, nsn=14, nz=15, dx=2000.0, corr=8000.0, seed=5),
     }
 
 
@@ -43,7 +45,7 @@ def gpu():
     return G
 
 
-@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet", "shelf"])
+@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet", "shelf", "deep"])
 def prob(request):
     p = problems()[request.param]
     print(f"\n[case] {p.name} ewn={p.ewn} nsn={p.nsn} nz={p.nz}", flush=True)
@@ -288,6 +290,21 @@ def test_picard_fp64(gpu, prob):
     print(f"  relative L2 velocity difference {e:.3e}")
     assert e < TOL_PICARD
     assert rel_max(rg["efvs"], ro["efvs"]) < 1e-6
+
+
+def test_picard_mixed_against_fp64(gpu, prob):
+    """the mixed-precision path (FP32 symmetric matrix, fused Chron-Gear update with the
+    column preconditioner, inexact Picard, Anderson) reaches 1e-6 of the FP64 solution"""
+    p = prob
+    lu, lv = O.loads(p)
+    with gpu.Context.for_problem(p) as ctx:
+        r64 = ctx.velo_higher_solve_3d(p, lu, lv)
+        ctx.set_reference_velocity(r64["uvel"], r64["vvel"])
+        rm = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, inner_rtol=5e-2,
+                                      anderson_m=3, stop_rel_err=1e-6, maxiter_nonlinear=300)
+    print(f"  mixed: picard={rm['picard_iters']} fp32={rm['fp32_matvecs']} "
+          f"err {rm['final_rel_err']:.2e}")
+    assert 0 <= rm["final_rel_err"] <= 1e-6
 
 
 def test_symmetry_repair_millions_of_pairs(gpu):
