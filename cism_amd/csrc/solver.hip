@@ -9,6 +9,7 @@
 #include "internal.h"
 
 #include <math.h>
+#include <type_traits>
 #include <stdio.h>
 
 // ---------------------------------------------------------------------------
@@ -455,6 +456,48 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_sia_setup(GridDev g, const M A,
         }
 }
 
+// k_sia_setup for the FP32 symmetric matrix: the same factors, from the diagonal
+// slot and the slot above (whose entry at level k-1 is the coupling below level k)
+// of 64 columns at a time, staged through LDS so that every global load is a
+// contiguous 1 KB run over the block instead of one scattered word per level.
+template <int NZMAX>
+__global__ __launch_bounds__(128) void k_sia_setup_sym32(GridDev g, const float *__restrict__ As,
+                                                         const int32_t *__restrict__ av,
+                                                         const float *__restrict__ cs,
+                                                         float *__restrict__ F) {
+    const int nz = g.nz;
+    const long long ncol = (long long)g.sx * g.sy;
+    const long long col0 = (long long)blockIdx.x * 64;
+    __shared__ float4 sd[64 * NZMAX], su[64 * NZMAX];
+    const int ncols = (int)(col0 + 64 <= ncol ? 64 : ncol - col0);
+    const long long Q0 = col0 * nz;
+    const int nn = ncols * nz;
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+        sd[t] = *reinterpret_cast<const float4 *>(As + sidx(0, 0, Q0 + t));
+        su[t] = *reinterpret_cast<const float4 *>(As + sidx(0, 9, Q0 + t));   // slot DIAG + 9
+    }
+    __syncthreads();
+    const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+    const long long col = col0 + lc;
+    if (lc >= ncols || !av[col]) return;
+    auto pick = [&](const float4 &v) { return comp == 0 ? v.x : v.w; };   // Auu or Avv
+    auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + col; };
+    const float inv = cs ? 1.0f / cs[comp * ncol + col] : 1.0f;
+    const float4 *d = sd + lc * nz, *u = su + lc * nz;
+    float beta = pick(d[0]);
+    F[Fi(0, 0)] = 0.0f;
+    F[Fi(1, 0)] = cs ? beta * inv : beta;
+    F[Fi(2, 0)] = 0.0f;
+    for (int k = 1; k < nz; ++k) {
+        const float gam = pick(u[k - 1]) / beta;   // super(k-1) / beta(k-1)
+        const float sb = pick(u[k - 1]);            // sub(k) = super(k-1), the symmetric pair
+        beta = pick(d[k]) - sb * gam;
+        F[Fi(0, k)] = cs ? sb * inv : sb;
+        F[Fi(1, k)] = cs ? beta * inv : beta;
+        F[Fi(2, k)] = gam;
+    }
+}
+
 // One column and component of the SIA preconditioner: the Thomas substitutions
 // on s (level k at s[2k]) with the factors precomputed by k_sia_setup.  For
 // nz <= 12 the factors are first loaded into registers (sia_factors), so that a
@@ -548,6 +591,21 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
 template <typename T, typename M>
 static int launch_sia_setup(cism_ctx *c, const M A, const float *cs, T *F) {
     const long long ncol = (long long)c->g.sx * c->g.sy;
+    if constexpr (std::is_same<M, MatSym32>::value) {
+        const dim3 grid((unsigned)((ncol + 63) / 64));
+        if (c->g.nz <= 12)
+            hipLaunchKernelGGL((k_sia_setup_sym32<12>), grid, dim3(128), 0, c->stream, c->g, A.A, c->av, cs, F);
+        else if (c->g.nz <= 24)
+            hipLaunchKernelGGL((k_sia_setup_sym32<24>), grid, dim3(128), 0, c->stream, c->g, A.A, c->av, cs, F);
+        else if (c->g.nz <= 48)
+            hipLaunchKernelGGL((k_sia_setup_sym32<48>), grid, dim3(128), 0, c->stream, c->g, A.A, c->av, cs, F);
+        else {
+            cism_set_error("SIA preconditioner supports nz <= 48 (nz = %d)", c->g.nz);
+            return CISM_ERR_UNSUPPORTED;
+        }
+        HIP_TRY(hipGetLastError());
+        return CISM_OK;
+    }
     const dim3 grid(nblocks_for(2 * ncol));
     if (c->g.nz <= 12)
         hipLaunchKernelGGL((k_sia_setup<T, M, 12>), grid, dim3(CISM_BLOCK), 0, c->stream, c->g, A, c->av, cs, F);
