@@ -363,20 +363,39 @@ def gaussian_random_field(n1, n2, dx, corr, rng):
 def grf_sheet(ewn=512, nsn=512, nz=11, dx=1000.0, seed=1, mean=1500.0, sd=400.0, corr=30000.0,
               beta_median=1.0e3, beta_sigma=1.0, A=1.0e-16, thklim=1.0,
               whichsparse=HO_SPARSE_PCG_CHRONGEAR, whichprecond=HO_PRECOND_SIA, **kw) -> Problem:
-    """Periodic ice sheet: GRF thickness on a flat bed, lognormal beta at vertices (babc 5)."""
+    """Ice sheet of GRF thickness on a flat bed, lognormal beta at vertices (babc 5).
+
+    periodic=False gives the outflow form of the same sheet: the halo cells are zero, as
+    parallel_halo leaves them on an outflow domain (parallel_slap.F90:1794-1860), so the ice
+    ends at the domain edge, the owned vertices include the west and south boundary rows
+    (staggered_ilo = staggered_lhalo, parallel_slap.F90:661-673) and the staggered halo
+    is zeroed beyond the boundary (parallel_slap.F90:2856-2863).  The thickness is
+    thinned over the last edge_cells cells toward the edge."""
+    periodic = kw.pop("periodic", True)
+    edge_cells = kw.pop("edge_cells", 5.0)
     rng = np.random.default_rng(seed)
     H = settle(mean + sd * gaussian_random_field(ewn, nsn, dx, corr, rng))
+    if not periodic:
+        # thinned toward the domain edge (to about 1 % in the boundary cells), so the ice
+        # that ends there is a thin margin, not a cliff
+        di = np.minimum(np.arange(ewn), np.arange(ewn)[::-1])[:, None]
+        dj = np.minimum(np.arange(nsn), np.arange(nsn)[::-1])[None, :]
+        H = settle(H * np.minimum(1.0, (np.minimum(di, dj) + 0.5) / edge_cells) ** 2)
     H = np.maximum(H, 10.0)
-    thck = pad_periodic(H)
+    thck = pad_periodic(H) if periodic else np.asfortranarray(np.pad(H, NHALO))
     topg = fzeros(thck.shape)
     usrf = np.asfortranarray(topg + thck)
     bo = settle(np.exp(np.log(beta_median) + beta_sigma * rng.standard_normal((ewn, nsn))))
     bv = np.zeros((ewn + 3, nsn + 3))
     bv[2:ewn + 2, 2:nsn + 2] = bo          # owned vertices 3..nx-2 (1-based)
-    bv = stag_halo(np.asfortranarray(bv), True)
-    return _finish("grf_sheet", ewn, nsn, nz, dx, dx, thck, usrf, topg, thklim, A,
-                   HO_BABC_BETA_EXTERNAL, beta_vertex=bv, whichsparse=whichsparse,
-                   whichprecond=whichprecond, settled=True, **kw)
+    if not periodic:                       # and the west/south boundary rows 2 (1-based)
+        bv[1, 1:nsn + 2] = bv[2, 1:nsn + 2]
+        bv[1:ewn + 2, 1] = bv[1:ewn + 2, 2]
+    bv = stag_halo(np.asfortranarray(bv), periodic)
+    return _finish("grf_sheet" if periodic else "grf_sheet_outflow", ewn, nsn, nz, dx, dx, thck,
+                   usrf, topg, thklim, A, HO_BABC_BETA_EXTERNAL, beta_vertex=bv,
+                   whichsparse=whichsparse, whichprecond=whichprecond, settled=True,
+                   periodic=periodic, **kw)
 
 
 def slab(ewn=40, nsn=40, nz=11, dx=1000.0, H=1000.0, theta_deg=0.5, efvs=2336041.42829,

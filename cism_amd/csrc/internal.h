@@ -291,6 +291,8 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode);
 int launch_matvec64(cism_ctx *c, const double *A, const double *x, double *y);
 int launch_halo_vec(cism_ctx *c, double *v);      // 2-component vector
 int column_scale_device(cism_ctx *c, double *out);  // out: 2*ncol doubles
+// 2 J(x) = x.(A x - b) - b.x of the context's x on owned active rows, t = A x - b
+int guess_energy_device(cism_ctx *c, const double *t, double *twoJ);
 // SSA (one-level context, ssa.hip)
 int ensure_ssa(cism_ctx *c, int nz3);
 int ssa_prepare_device(cism_ctx *c);
@@ -316,8 +318,10 @@ int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double t
                  double *err, int *niters);
 // r_ready: c->r already holds b - A x with sums[2] = |r|^2 and (on the host too)
 // sums[3] = |b|^2, as residual_vector_device leaves them with rs = c->r
+// stalled (optional): set to 1 when the refinement is given up (refine_policy.h)
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
-                 double *err, int *niters, int *n64, int *n32, int r_ready = 0);
+                 double *err, int *niters, int *n64, int *n32, int r_ready = 0,
+                 int *stalled = nullptr);
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);

@@ -7,6 +7,7 @@
 // values evolve exactly as in the reference.  Halo updates
 // (staggered_parallel_halo) are device-local copies from the periodic image.
 #include "internal.h"
+#include "refine_policy.h"
 
 #include <math.h>
 #include <type_traits>
@@ -1082,9 +1083,11 @@ __global__ void k_save_scale(DevScal *sc, int from) { sc->sums[2] = sqrt(sc->sum
 // Chron-Gear to relative tolerance inner_rtol, x += |r| e, until the FP64
 // relative residual is below tol.  The FP32 matrix must be current: the symmetry pass
 // of the Picard loop writes it (symmetry_device with c->A32).  With r_ready the
-// first residual, its norm and |b| come from the Picard residual pass.
+// first residual, its norm and |b| come from the Picard residual pass.  *stalled is set
+// when the refinement is given up (refine_policy.h): the caller then falls back to the
+// FP64 solver; an FP32 breakdown (alpha = NaN) is such a stall, not an error.
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol, double *err_out,
-                 int *niters_out, int *n64, int *n32, int r_ready) {
+                 int *niters_out, int *n64, int *n32, int r_ready, int *stalled) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn);
     const long long n2 = 2 * g.npad;
@@ -1103,6 +1106,9 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
     double err = 0.0;
     CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32,
                        c->cscale_on ? c->cscale : nullptr};
+    RefineMonitor mon;
+    refine_monitor_init(&mon);
+    if (stalled) *stalled = 0;
     while (true) {
         double rr;
         if (r_ready && outer == 0) {
@@ -1119,13 +1125,26 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         }
         ++*n64;
         err = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
-        if (err < tol || total >= maxiters * 4 || outer >= 50) break;
+        const int verdict = refine_decide(&mon, err, tol, outer, total, maxiters);
+        if (verdict == REFINE_CONVERGED) break;
+        if (verdict == REFINE_STALLED) {
+            if (stalled) *stalled = 1;
+            break;
+        }
         hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->r, c->sc,
                            c->b32, c->x32, c->r32);
         double e_in = 0.0;
         int it = 0;
         // the inner reductions use sums[0..1]; the scale |r| stays in sums[2]
-        CISM_TRY(run_cg<float>(c, MatSym32{c->A32}, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it, 1));
+        const int st_in = run_cg<float>(c, MatSym32{c->A32}, v, 1, precond, maxiters, inner_rtol, 1, &e_in, &it, 1);
+        if (st_in == CISM_ERR_NAN && stalled) {
+            // the FP32 iteration broke down; x is untouched by this outer step
+            *stalled = 1;
+            *n32 += it;
+            total += it;
+            break;
+        }
+        CISM_TRY(st_in);
         hipLaunchKernelGGL(k_refine, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x32, c->sc,
                            c->x);
         HIP_TRY(hipGetLastError());
@@ -1173,6 +1192,40 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_colscale(GridDev g, const double
         if (den > 0.0 && num > 0.0) s = fmin(16.0, fmax(1.0 / 16.0, num / den));
     }
     out[t] = s;
+}
+
+// Energy of an initial guess in the first Picard system A x = b (A symmetric positive
+// definite after the Dirichlet elimination): 2 J(x) = x.A x - 2 b.x = x.(A x - b) - b.x,
+// with t = A x - b from the residual pass.  J(0) = 0 and 2 (J(x) - J(x*)) is the squared
+// A-norm error, so J(x) < 0 exactly when x is closer to that system's solution than the
+// zero guess is, in the A-norm.  Owned active rows; sums[0] = x.t, sums[1] = b.x.
+__global__ __launch_bounds__(CISM_BLOCK) void k_energy(GridDev g, const int32_t *__restrict__ av,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ t,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nn; q += stride) {
+        const long long col = q / g.nz;
+        const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+        if (is_owned(g, i, j) && av[col]) {
+            acc[0] += x[2 * q] * t[2 * q] + x[2 * q + 1] * t[2 * q + 1];
+            acc[1] += b[2 * q] * x[2 * q] + b[2 * q + 1] * x[2 * q + 1];
+        }
+    }
+    block_reduce_store<2>(acc, partials);
+}
+
+int guess_energy_device(cism_ctx *c, const double *t, double *twoJ) {
+    const GridDev &g = c->g;
+    const int nb = red_blocks_for(g.nn);
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, c->av, c->x, t, c->b,
+                       c->partials);
+    CISM_TRY(launch_reduce(c, nb, 2, RED_SUMS));
+    CISM_TRY(sync_scalars(c));
+    *twoJ = c->sc_host->sums[0] - c->sc_host->sums[1];
+    return CISM_OK;
 }
 
 int column_scale_device(cism_ctx *c, double *out) {

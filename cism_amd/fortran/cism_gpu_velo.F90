@@ -45,11 +45,11 @@ module cism_gpu_velo
      real(c_double) :: seconds_total
      integer(c_int) :: status, refinement_steps
      real(c_double) :: final_rel_err, seconds_device
-     integer(c_int) :: reserved(2)
+     integer(c_int) :: converged, fallbacks, guess_rejected, reserved
   end type cism_solve_stats
 
   type(c_ptr), save :: ctx = c_null_ptr
-  integer, save :: ctx_nx = -1, ctx_ny = -1, ctx_nz = -1
+  integer, save :: ctx_nx = -1, ctx_ny = -1, ctx_nz = -1, ctx_iper = -1
   logical, save :: ctx_periodic = .true.
 
   interface
@@ -154,16 +154,18 @@ contains
     logical, intent(in), optional :: periodic
     integer(c_int) :: st, iper
     if (present(periodic)) ctx_periodic = periodic
+    iper = merge(1, 0, ctx_periodic)
     if (c_associated(ctx)) then
-       if (nx == ctx_nx .and. ny == ctx_ny .and. nz == ctx_nz) return
+       ! the context's owned range and halo rules depend on the boundary type too
+       if (nx == ctx_nx .and. ny == ctx_ny .and. nz == ctx_nz .and. iper == ctx_iper) return
        call cism_gpu_velo_finalize()
     endif
-    iper = merge(1, 0, ctx_periodic)
     st = c_ctx_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), 2_c_int, iper, 0_c_int, ctx)
     call check(st, 'cism_gpu_velo_init')
     ctx_nx = nx
     ctx_ny = ny
     ctx_nz = nz
+    ctx_iper = iper
   end subroutine cism_gpu_velo_init
 
   subroutine cism_gpu_velo_finalize()
@@ -171,6 +173,7 @@ contains
     if (c_associated(ctx)) st = c_ctx_destroy(ctx)
     ctx = c_null_ptr
     ctx_nx = -1
+    ctx_iper = -1
   end subroutine cism_gpu_velo_finalize
 
   !> maxiters and tolerance of the linear solvers (module constants of the reference)

@@ -193,13 +193,20 @@ def encode(vel, uscale: float, target: str = "linear"):
     raise ValueError(f"unknown target encoding {target!r}")
 
 
+# largest |output| decoded (network units): a diverging output is capped at about
+# 2e6 m/yr (log1p, uscale 100) or 1e6 m/yr (linear, uscale 1000) rather than reaching inf
+OUT_CLIP = {"log1p": 10.0, "linear": 1000.0}
+
+
 def decode(out, uscale: float, target: str = "linear"):
-    """network units -> velocity (m/yr), the inverse of encode"""
+    """network units -> velocity (m/yr), the inverse of encode (outputs clipped to
+    +-OUT_CLIP[target] first)"""
+    if target not in OUT_CLIP:
+        raise ValueError(f"unknown target encoding {target!r}")
+    out = np.clip(out, -OUT_CLIP[target], OUT_CLIP[target])
     if target == "log1p":
         return np.sign(out) * uscale * np.expm1(np.abs(out))
-    if target == "linear":
-        return uscale * out
-    raise ValueError(f"unknown target encoding {target!r}")
+    return uscale * out
 
 
 def targets(p: configs.Problem, uvel, vvel, uscale: float, su=None, sv=None, target: str = "linear"):
@@ -270,6 +277,11 @@ class Predictor:
         if not p.periodic:
             raise ValueError("the surrogate runs on periodic grids (circular padding)")
         params, self.meta = load_weights(weights)
+        nz_w = int(self.meta.get("nz", NZ_MAX))
+        if p.nz != nz_w:
+            # the output channels hold nz_w levels of u, then nz_w of v: another nz would
+            # read the wrong channels
+            raise ValueError(f"the weights predict {nz_w} levels, the grid has nz = {p.nz}")
         self.uscale = float(self.meta["uscale"])
         self.target = self.meta.get("target", "linear")
         self.net = Surrogate(p.nsn, p.ewn, 1, device)

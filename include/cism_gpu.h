@@ -205,7 +205,15 @@ typedef struct {
     int    refinement_steps;   /* FP64 residual evaluations of the mixed-precision solves    */
     double final_rel_err;      /* |x - x_ref| / |x_ref| at exit (-1 without a reference)     */
     double seconds_device;     /* the Picard loop alone: inputs resident, outputs not copied  */
-    int    reserved[2];
+    int    converged;          /* converged_soln of the reference (velo_higher:3731-3745):
+                                  the loop ended before maxiter_nonlinear (or on stop_rel_err) */
+    int    fallbacks;          /* precision 1: linear solves redone by the FP64 solver after the
+                                  mixed-precision refinement stalled (the rest of the solve then
+                                  stays on FP64)                                            */
+    int    guess_rejected;     /* precision 1: 1 if the initial guess was no closer than zero
+                                  to the solution of the first Picard system in its energy
+                                  norm (x.A x / 2 - b.x > 0) and was replaced by zero         */
+    int    reserved;
 } cism_solve_stats;
 
 void cism_solve_opts_default(cism_solve_opts *o);

@@ -852,6 +852,16 @@ static void pc_apply(const orc_grid *g, int precond, const int32_t *active_verte
     }
 }
 
+void orc_precond_apply(const orc_grid *g, const int32_t *active_vertex, int precond,
+                       const double *Auu, const double *Avv, const double *ru, const double *rv,
+                       double *zu, double *zv)
+{
+    orc_pc pc;
+    pc_setup(g, precond, Auu, Avv, &pc);
+    pc_apply(g, precond, active_vertex, &pc, ru, rv, zu, zv);
+    pc_free(&pc);
+}
+
 /* pcg_solver_standard_3d (pcg:84-549) */
 int orc_pcg_standard_3d(const orc_grid *g, const int32_t *av,
                         const double *Auu, const double *Auv,
@@ -1053,6 +1063,7 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
         }
         orc_residual_vector_3d(g, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
                                resid_u, resid_v, &L2, &L2rel);
+        if (o->L2_history && counter <= o->L2_history_cap) o->L2_history[counter - 1] = L2;
         double err; int niters;
         if (o->whichsparse == 3)
             status = orc_pcg_chrongear_3d(g, active_vertex, Auu, Auv, Avu, Avv, bu, bv, uvel, vvel,
@@ -1069,7 +1080,9 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
         if (o->whichresid == 3) { crit = L2; target = L2_target; }
         else if (o->whichresid == 4) { crit = L2rel; target = L2_target_rel; }
         else { crit = resid_velo; target = resid_target; }
+        if (o->target_override > 0) target = o->target_override;
     }
+    o->converged = counter < o->maxiter_nonlinear;
     o->picard_iters = counter;
     o->final_L2_norm = L2;
     o->final_resid_velo = resid_velo;
@@ -1556,6 +1569,7 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
         }
         orc_residual_vector_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2, resid_u, resid_v,
                                &L2, &L2rel);
+        if (o->L2_history && counter <= o->L2_history_cap) o->L2_history[counter - 1] = L2;
         double err; int niters;
         if (o->whichsparse == 3)
             status = orc_pcg_chrongear_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
@@ -1573,7 +1587,9 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
         if (o->whichresid == 3) { crit = L2; target = L2_target; }
         else if (o->whichresid == 4) { crit = L2rel; target = L2_target_rel; }
         else { crit = resid_velo; target = resid_target; }
+        if (o->target_override > 0) target = o->target_override;
     }
+    o->converged = counter < o->maxiter_nonlinear;
     for (int j = 1; j <= ny - 1; ++j)
         for (int i = 1; i <= nx - 1; ++i) {
             const size_t c = (i - 1) + (size_t)(nx - 1) * (j - 1);

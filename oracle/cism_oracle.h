@@ -187,7 +187,24 @@ typedef struct {
      * beta_internal is not read */
     const orc_basal_physics *bp;
     const double *beta_external, *effecpress_stag, *f_ground;
+    /* optional: > 0 replaces the Picard target of whichresid (1e-4 for the L2 norm) */
+    double target_override;
+    /* optional output: the residual norm L2_norm measured before the linear solve of
+     * each Picard iteration (velo_higher:3310-3320), for the first L2_history_cap ones */
+    double *L2_history;
+    int L2_history_cap;
+    /* output: converged_soln of the reference, counter < maxiter_nonlinear
+     * (velo_higher:3731-3745) */
+    int converged;
 } orc_picard_opts;
+
+/* z = M^-1 r for precond 0 (identity), 1 (diagonal) or 2 (column tridiagonal) with the
+ * preconditioner of setup_preconditioner_3d (pcg:2150-2265) built from Auu, Avv and applied
+ * as the PCG loops do (pcg:300-365, easy_sia_solver pcg:2759-2848); z is left unchanged
+ * in inactive columns under precond 2, as easy_sia_solver leaves it. */
+void orc_precond_apply(const orc_grid *g, const int32_t *active_vertex, int precond,
+                       const double *Auu, const double *Avv, const double *ru, const double *rv,
+                       double *zu, double *zv);
 
 int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
                      const double *sigma, const int32_t *active_cell,

@@ -59,7 +59,9 @@ class PicardOpts(C.Structure):
                [(n, C.c_int) for n in ("picard_iters", "total_linear_iters", "status")] + \
                [(n, C.c_double) for n in ("final_L2_norm", "final_resid_velo")] + \
                [("bp", C.POINTER(BasalPhysics))] + \
-               [(n, C.POINTER(C.c_double)) for n in ("beta_external", "effecpress_stag", "f_ground")]
+               [(n, C.POINTER(C.c_double)) for n in ("beta_external", "effecpress_stag", "f_ground")] + \
+               [("target_override", C.c_double), ("L2_history", C.POINTER(C.c_double)),
+                ("L2_history_cap", C.c_int), ("converged", C.c_int)]
 
 
 def _set_basal(o, basal, keep):
@@ -120,6 +122,7 @@ def _declare(L):
     L.orc_pcg_chrongear_3d.argtypes = pcg
     L.orc_picard_solve.argtypes = [G, C.POINTER(PicardOpts), D, I, I, D, D, D, D, D, D, I, I,
                                    D, D, D, D, D, D, D, D, D]
+    L.orc_precond_apply.argtypes = [G, I, C.c_int, D, D, D, D, D, D]
 
 
 def dp(a):
@@ -305,15 +308,35 @@ def pcg(p, A, bu, bv, xu, xv, precond, chrongear=True, maxiters=200, tolerance=1
     return st, err.value, nit.value
 
 
+def precond_apply(p, A, precond, ru, rv, zu=None, zv=None, active_vertex=None):
+    """z = M^-1 r with the PCG preconditioner built from A (0 none, 1 diagonal, 2 column
+    tridiagonal); zu, zv start as given (zeros by default)."""
+    g = grid_of(p)
+    av = p.active_vertex if active_vertex is None else active_vertex
+    zu = vec_zeros(p) if zu is None else zu
+    zv = vec_zeros(p) if zv is None else zv
+    lib().orc_precond_apply(C.byref(g), ip(av), precond, dp(A[0]), dp(A[3]), dp(ru), dp(rv),
+                            dp(zu), dp(zv))
+    return zu, zv
+
+
 def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
-           whichsparse=None, whichprecond=None, whichresid=None, basal=None):
-    """Runs the oracle Picard loop from p.uvel/p.vvel (copied). Returns a result dict.
+           whichsparse=None, whichprecond=None, whichresid=None, basal=None, target=None,
+           uvel=None, vvel=None):
+    """Runs the oracle Picard loop from p.uvel/p.vvel (copied; or from uvel/vvel). Returns a
+    result dict with the pre-solve residual norm of every Picard iteration (L2_history).
     basal: (BasalPhysics, beta_external, effecpress_stag, f_ground) to recompute beta with
-    calcbeta every Picard iteration, or None for the fixed p.beta_internal."""
+    calcbeta every Picard iteration, or None for the fixed p.beta_internal.  target > 0
+    replaces the Picard target (1e-4 for the L2 norm)."""
     g = grid_of(p)
     o = PicardOpts()
     keep = []
     _set_basal(o, basal, keep)
+    cap = p.maxiter_nonlinear if maxiter is None else maxiter
+    hist = np.zeros(max(cap, 1))
+    o.L2_history = dp(hist)
+    o.L2_history_cap = cap
+    o.target_override = 0.0 if target is None else float(target)
     o.whichbabc = p.whichbabc
     o.whichefvs = p.whichefvs
     o.whichresid = p.whichresid if whichresid is None else whichresid
@@ -325,7 +348,8 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
     o.efvs_constant = p.efvs_constant
     o.linear_tolerance = linear_tolerance
     o.dx, o.dy = p.dx, p.dy
-    u, v = p.uvel.copy(order="F"), p.vvel.copy(order="F")
+    u = (p.uvel if uvel is None else uvel).copy(order="F")
+    v = (p.vvel if vvel is None else vvel).copy(order="F")
     efvs = np.zeros((p.nz - 1, p.nx, p.ny), order="F")
     bu, bv, ru, rv = vec_zeros(p), vec_zeros(p), vec_zeros(p), vec_zeros(p)
     st = lib().orc_picard_solve(C.byref(g), C.byref(o), dp(p.sigma), ip(p.active_cell),
@@ -336,7 +360,8 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
                                 dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv))
     return dict(status=st, uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
                 picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
-                L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo)
+                L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo,
+                L2_history=hist[:min(o.picard_iters, cap)].copy(), converged=bool(o.converged))
 
 
 def grounded_fraction(ice, floating, land, whichground=0):

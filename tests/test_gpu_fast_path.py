@@ -99,3 +99,38 @@ def test_column_scale_target_shape(setup):
         o = s[2:-2, 2:-2]
         assert np.all(o[av] >= 1 / 16) and np.all(o[av] <= 16)
         assert np.all(o[~av] == 1.0)
+
+
+def test_stalled_refinement_falls_back_to_fp64(setup):
+    """an FP64 linear tolerance beyond the FP32 operator's reach (1e-20): the refinement
+    stalls at its rounding floor, that Picard iteration is redone by the FP64 solver (which
+    then serves the rest of the solve), and the solve still reaches 1e-6 of the FP64
+    reference"""
+    p, ctx, lu, lv, ref = setup
+    ctx.set_linear_solver(200, 1e-20)
+    try:
+        r = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=0, inner_rtol=5e-2,
+                                     stop_rel_err=1e-6, maxiter_nonlinear=300)
+    finally:
+        ctx.set_linear_solver(200, 1e-8)
+    print(f"[fast] unreachable tolerance: {r['picard_iters']} Picard, {r['fallbacks']} fallback(s), "
+          f"{r['fp32_matvecs']} FP32 matvecs, err {r['final_rel_err']:.2e}")
+    assert r["fallbacks"] == 1
+    assert r["converged"] and 0 <= r["final_rel_err"] <= 1e-6
+
+
+def test_worse_than_zero_guess_is_rejected(setup):
+    """an initial guess whose Picard residual exceeds the zero guess's is dropped: the solve
+    is then the zero-guess solve, bit for bit"""
+    p, ctx, lu, lv, ref = setup
+    rng = np.random.default_rng(0)
+    bu = np.asfortranarray(1e5 * rng.standard_normal(p.uvel.shape))
+    bv = np.asfortranarray(1e5 * rng.standard_normal(p.vvel.shape))
+    bad = fast(ctx, p, lu, lv, uvel=bu, vvel=bv, anderson_m=3)
+    zero = fast(ctx, p, lu, lv, anderson_m=3)
+    print(f"[fast] bad guess: rejected {bad['guess_rejected']}, {bad['picard_iters']} Picard, "
+          f"err {bad['final_rel_err']:.2e}")
+    assert bad["guess_rejected"] == 1 and zero["guess_rejected"] == 0
+    assert 0 <= bad["final_rel_err"] <= 1e-6
+    assert bad["picard_iters"] == zero["picard_iters"]
+    assert np.array_equal(bad["uvel"], zero["uvel"]) and np.array_equal(bad["vvel"], zero["vvel"])

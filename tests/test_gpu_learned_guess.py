@@ -68,6 +68,7 @@ def test_learned_guess_reduces_work(setup, precision):
               f"{r['seconds']:.3f} s, err {r['final_rel_err']:.2e}")
     for r in runs.values():
         assert r["final_rel_err"] <= 1e-6
+        assert r["guess_rejected"] == 0 and r["fallbacks"] == 0
         assert rel_l2(r["uvel"], r["vvel"], ref["uvel"], ref["vvel"]) <= 1.01e-6
     z, l = runs["zero"], runs["learned"]
     assert l["picard_iters"] < z["picard_iters"]

@@ -35,7 +35,27 @@ def problems():
         "shelf": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7, shelf=0.02),
         # more than 12 levels: the column kernels' 24-level variants
         "deep": configs.grf_sheet(ewn=16, nsn=14, nz=15, dx=2000.0, corr=8000.0, seed=5),
+        # an outflow domain whose ice reaches the domain edge: the owned range takes in the
+        # west and south boundary rows, the staggered halo is zeroed beyond the boundary
+        # (parallel_slap.F90:661-673, 2856-2863), and the edge rows' neighbours are zero
+        "outflow": configs.grf_sheet(ewn=24, nsn=20, nz=7, dx=2000.0, corr=8000.0, seed=3,
+                                     periodic=False),
     }
+
+
+# the oracle's Picard fixed point, converged past the reference's stopping rule: linear
+# solves to 1e-10, Picard until the pre-solve residual is below 1e-6 (the reference stops
+# at 1e-4); within 1e-8 of tighter runs on every case here
+_TIGHT = {}
+
+
+def oracle_fixed_point(p):
+    if p.name + str(p.ewn) not in _TIGHT:
+        lu, lv = O.loads(p)
+        r = O.picard(p, lu, lv, maxiter=300, linear_maxiters=1000, linear_tolerance=1e-10, target=1e-6)
+        assert r["status"] == 0 and r["converged"]
+        _TIGHT[p.name + str(p.ewn)] = r
+    return _TIGHT[p.name + str(p.ewn)]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +65,7 @@ def gpu():
     return G
 
 
-@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet", "shelf", "deep"])
+@pytest.fixture(scope="module", params=["dome", "ismip", "grf", "sheet", "shelf", "deep", "outflow"])
 def prob(request):
     p = problems()[request.param]
     print(f"\n[case] {p.name} ewn={p.ewn} nsn={p.nsn} nz={p.nz}", flush=True)
@@ -292,19 +312,46 @@ def test_picard_fp64(gpu, prob):
     assert rel_max(rg["efvs"], ro["efvs"]) < 1e-6
 
 
-def test_picard_mixed_against_fp64(gpu, prob):
+def test_picard_mixed_against_oracle(gpu, prob):
     """the mixed-precision path (FP32 symmetric matrix, fused Chron-Gear update with the
-    column preconditioner, inexact Picard, Anderson) reaches 1e-6 of the FP64 solution"""
+    column preconditioner, inexact Picard, Anderson) reaches 1e-6 of the oracle's FP64
+    Picard fixed point"""
     p = prob
     lu, lv = O.loads(p)
+    ro = oracle_fixed_point(p)
     with gpu.Context.for_problem(p) as ctx:
-        r64 = ctx.velo_higher_solve_3d(p, lu, lv)
-        ctx.set_reference_velocity(r64["uvel"], r64["vvel"])
+        ctx.set_reference_velocity(ro["uvel"], ro["vvel"])
         rm = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, inner_rtol=5e-2,
                                       anderson_m=3, stop_rel_err=1e-6, maxiter_nonlinear=300)
     print(f"  mixed: picard={rm['picard_iters']} fp32={rm['fp32_matvecs']} "
-          f"err {rm['final_rel_err']:.2e}")
+          f"err {rm['final_rel_err']:.2e} fallbacks {rm['fallbacks']}")
+    assert rm["converged"] and rm["fallbacks"] == 0
     assert 0 <= rm["final_rel_err"] <= 1e-6
+    xo = np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()])
+    xg = np.concatenate([rm["uvel"].ravel(), rm["vvel"].ravel()])
+    assert rel_l2(xg, xo) <= 1e-6
+
+
+def test_picard_mixed_under_the_reference_stopping_rule(gpu, prob):
+    """the fast path as deployed: no known answer to stop on, the reference's Picard test
+    (pre-solve L2 < 1e-4) ends the loop.  It converges within maxiter_nonlinear and lands
+    as close to the oracle's fixed point as the plain FP64 path does under the same rule
+    (the oracle's own run at the reference settings), within a factor of 10 and 1e-6"""
+    p = prob
+    lu, lv = O.loads(p)
+    ro = oracle_fixed_point(p)
+    rd = O.picard(p, lu, lv)                 # the reference algorithm and stopping rule
+    xo = np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()])
+    e_ref = rel_l2(np.concatenate([rd["uvel"].ravel(), rd["vvel"].ravel()]), xo)
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.set_reference_velocity(ro["uvel"], ro["vvel"])
+        rm = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, inner_rtol=5e-2,
+                                      anderson_m=3)
+    print(f"  deployed fast path: picard={rm['picard_iters']} (reference rule, oracle "
+          f"{rd['picard_iters']}) err {rm['final_rel_err']:.2e} (oracle at its rule {e_ref:.2e})")
+    assert rm["converged"] and rm["picard_iters"] < p.maxiter_nonlinear
+    assert rm["final_L2_norm"] < 1e-4
+    assert 0 <= rm["final_rel_err"] <= max(10 * e_ref, 1e-6)
 
 
 def test_symmetry_repair_millions_of_pairs(gpu):
