@@ -293,6 +293,9 @@ def _finish(name, ewn, nsn, nz, dx, dy, thck, usrf, topg, thklim, flwa_val, whic
                 vmask_dirichlet=vm, uvel=fzeros((nz, nx - 1, ny - 1)),
                 vvel=fzeros((nz, nx - 1, ny - 1)), whichbabc=whichbabc, ice_mask=ice,
                 floating_mask=floating, ocean_mask=ocean, land_mask=land, **kw)
+    if beta_vertex is not None:
+        # calcbeta's beta_external (which_ho_babc 5): beta_internal before the f_ground weight
+        p.extra["beta_external"] = np.asfortranarray(beta_vertex, dtype=np.float64)
     return p
 
 

@@ -254,12 +254,13 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     if (!c) return CISM_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    learned_guess_free(c);
     void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->av_zero, c->ac_zero, c->as_av_prev, c->as_ac_prev, c->xref, c->partials, c->red_groups, c->sc, c->stage,
                     c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
-                    c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg};
+                    c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg, c->planes};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -605,9 +606,11 @@ static double now_s() {
 
 // The Picard loop on the device (velo_higher:2206-3727), inputs already resident: the
 // 3D Blatter-Pattyn system, or with dim2 the SSA system of a one-level context.
+// t_dev0 > 0: the time the inputs became resident (seconds_device counts from there),
+// else from this call, after the stream has drained
 static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double t0, double *uvel,
                          double *vvel, double *efvs, double *bu, double *bv, double *resid_u,
-                         double *resid_v, cism_solve_stats *stats) {
+                         double *resid_v, cism_solve_stats *stats, double t_dev0 = 0.0) {
     const GridDev &g = c->g;
     if (o->anderson_m < 0 || o->anderson_m > AA_MAX) {
         cism_set_error("anderson_m must be 0..%d", AA_MAX);
@@ -615,7 +618,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
     }
     anderson_reset(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const double t_dev0 = now_s();   // inputs resident from here on
+    if (!(t_dev0 > 0.0)) t_dev0 = now_s();   // inputs resident from here on
     double crit = 1.0e10, target = 1.0e-12;
     double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
     int counter = 0, lin_total = 0, st = CISM_OK;
@@ -829,9 +832,107 @@ extern "C" int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, 
     CISM_TRY(up(c, c->beta, beta_internal, ncol));
     CISM_TRY(upload_dirichlet_masks(c, umask_dirichlet, vmask_dirichlet, (size_t)g.nn));
     CISM_TRY(upload_vec(c, c->load, loadu, loadv));
-    CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    if (o->initial_guess == 0) {
+        CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    } else if (o->initial_guess != 2) {
+        cism_set_error("initial_guess must be 0 (uvel, vvel) or 2 (the context's velocity) here");
+        return CISM_ERR_ARG;
+    }
     HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
     return picard_device(c, o, 0, t0, uvel, vvel, efvs, bu, bv, resid_u, resid_v, stats);
+}
+
+// the Dirichlet masks of glissade_velo_higher_solve (velo_higher:1269-1324) without
+// kinbcmask or no-penetration boundaries: the bed level under no-slip, else none
+__global__ __launch_bounds__(256) void k_noslip_masks(GridDev g, int noslip, int32_t *__restrict__ um,
+                                                      int32_t *__restrict__ vm) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int m = noslip && (int)(q % g.nz) == g.nz - 1;
+    um[q] = m;
+    vm[q] = m;
+}
+
+// glissade_velo_higher_solve (velo_higher:654-3986, 3D branch) from the ice geometry:
+// the inputs, grounded fraction, beta, loads and Dirichlet masks on the device, then the
+// Picard loop; only the cell fields go in and the velocity (and efvs) come out.
+extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opts *o,
+                                               const cism_velo_setup *su, const double *sigma,
+                                               const double *thck, const double *usrf, const double *topg,
+                                               const double *flwa, const cism_basal_physics *bp,
+                                               const double *beta_external, const double *effecpress_stag,
+                                               double *uvel, double *vvel, double *efvs,
+                                               cism_solve_stats *stats) {
+    CTX_BEGIN(c);
+    if (!o || !su || !sigma || !uvel || !vvel) {
+        cism_set_error("glissade_velo_higher_solve: null options, setup, sigma or velocity");
+        return CISM_ERR_ARG;
+    }
+    CISM_TRY(check_options(o->whichapprox, o->whichefvs));
+    if (o->whichsparse != 2 && o->whichsparse != 3) {
+        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const GridDev &g = c->g;
+    if (g.nz < 2) {
+        cism_set_error("glissade_velo_higher_solve needs a 3D context (nz >= 2)");
+        return CISM_ERR_ARG;
+    }
+    const bool noslip = o->whichbabc == 6;
+    if (!noslip && !bp) {
+        cism_set_error("glissade_velo_higher_solve: basal physics needed unless which_ho_babc = 6 (no-slip)");
+        return CISM_ERR_ARG;
+    }
+    if (su->initial_guess < 0 || su->initial_guess > 2 || (su->initial_guess == 1 && !c->guess)) {
+        cism_set_error("glissade_velo_higher_solve: initial_guess 0, 1 (after cism_learned_guess_load) or 2");
+        return CISM_ERR_ARG;
+    }
+    if (su->whichassemble_taud != 0 && su->whichassemble_taud != 1) {
+        cism_set_error("glissade_velo_higher_solve: whichassemble_taud must be 0 or 1");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    const double t0 = now_s();
+    // sigma and stagwbndsigma (glide_setup.F90:355-363)
+    std::vector<double> w(g.nz + 1);
+    w[0] = 0.0;
+    for (int k = 1; k < g.nz; ++k) w[k] = (sigma[k - 1] + sigma[k]) / 2.0;
+    w[g.nz] = 1.0;
+    // host arrays in: sigma, the cell fields, the basal fields (and the initial velocity)
+    CISM_TRY(up(c, c->sigma, sigma, (size_t)g.nz));
+    CISM_TRY(up(c, c->wsig, w.data(), (size_t)g.nz + 1));
+    CISM_TRY(velo_inputs_upload(c, thck, usrf, topg, flwa));
+    if (bp) CISM_TRY(basal_fields_upload(c, bp, beta_external, effecpress_stag));
+    if (su->initial_guess == 0) CISM_TRY(upload_vec(c, c->x, uvel, vvel));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double t_dev0 = now_s();   // inputs resident from here on
+    CISM_TRY(velo_inputs_compute(c, su->dx, su->dy, su->thklim, su->eus, su->max_slope));
+    CISM_TRY(mask_planes_device(c));
+    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
+    if (bp) {
+        CISM_TRY(basal_setup_device(c, c->planes, su->whichground, bp));
+    } else {
+        c->bp_on = 0;
+        HIP_TRY(hipMemsetAsync(c->beta, 0, sizeof(double) * ncol, c->stream));
+    }
+    // gravity and lateral ice-ocean loads (velo_higher:2033-2111)
+    HIP_TRY(hipMemsetAsync(c->load, 0, sizeof(double) * 2 * g.npad, c->stream));
+    CISM_TRY(load_gravity_device(c, su->whichassemble_taud, su->dx, su->dy));
+    if (su->lateral_bc) CISM_TRY(load_lateral_device(c, c->planes + 2 * ncell, c->planes + 4 * ncell));
+    hipLaunchKernelGGL(k_noslip_masks, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, noslip ? 1 : 0,
+                       c->umask, c->vmask);
+    HIP_TRY(hipGetLastError());
+    c->dirichlet_any = noslip ? 1 : 0;
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    if (su->initial_guess == 1) {
+        // beta at zero velocity for the network's input (exact for laws that do not
+        // depend on the velocity); the Picard loop recomputes it from its iterates
+        HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
+        if (c->bp_on) CISM_TRY(calcbeta_device(c));
+        CISM_TRY(learned_guess_device(c));
+    }
+    cism_solve_opts oo = *o;
+    oo.initial_guess = 2;
+    return picard_device(c, &oo, 0, t0, uvel, vvel, efvs, nullptr, nullptr, nullptr, nullptr, stats, t_dev0);
 }
 
 // The SSA branch of glissade_velo_higher_solve (solve_2d, velo_higher:2206-3767) on a

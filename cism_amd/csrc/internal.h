@@ -57,6 +57,8 @@ struct DevScal {
     int beta_bad;    // calcbeta met a negative or NaN beta
 };
 
+struct cism_guess;   // the learned initial guess of a context (surrogate.hip)
+
 struct cism_ctx {
     GridDev g;
     int device;
@@ -116,6 +118,11 @@ struct cism_ctx {
     DevScal *sc_host;   // pinned mirror
     // staging for layout conversion (27*nn doubles, lazily)
     double *stage;
+    // learned initial guess (cism_learned_guess_load), or null
+    cism_guess *guess;
+    // cell masks of cism_glissade_velo_higher_solve as int32 planes (ice, floating,
+    // ocean, land, calving front = 0), 5 * nx * ny, lazily
+    int32_t *planes;
 };
 
 // ---------------------------------------------------------------------------
@@ -335,6 +342,23 @@ int residual_vector_device(cism_ctx *c, double *ru, double *rs);
 int residual_velocity_device(cism_ctx *c, int whichresid);       // sums[0]
 int load_gravity_device(cism_ctx *c, int whichassemble_taud, double dx, double dy);
 int load_lateral_device(cism_ctx *c, const int32_t *ocean, const int32_t *cfront);
+
+// the learned guess into c->x from the context's av, sthck, dsdx, dsdy, beta, flwafact
+int learned_guess_device(cism_ctx *c);
+void learned_guess_free(cism_ctx *c);
+// the solve's inputs from cell fields (host arrays) into the context; mask bits in c->umask
+int velo_inputs_device(cism_ctx *c, double dx, double dy, double thklim, double eus, double max_slope,
+                       const double *thck, const double *usrf, const double *topg, const double *flwa);
+int velo_inputs_upload(cism_ctx *c, const double *thck, const double *usrf, const double *topg,
+                       const double *flwa);
+int velo_inputs_compute(cism_ctx *c, double dx, double dy, double thklim, double eus, double max_slope);
+// the mask bits left in c->umask by velo_inputs_device as int32 planes in c->planes
+int mask_planes_device(cism_ctx *c);
+// grounded fraction and basal physics of the Picard loop from the context's mask planes
+int basal_setup_device(cism_ctx *c, const int32_t *planes, int whichground, const cism_basal_physics *bp);
+// the upload half of basal_setup_device (beta_external and N to the device); checks bp
+int basal_fields_upload(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
+                        const double *effecpress_stag);
 
 int ensure_stage(cism_ctx *c);
 int ensure_ke(cism_ctx *c);

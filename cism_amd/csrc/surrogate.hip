@@ -21,6 +21,7 @@
 #include "internal.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -822,4 +823,235 @@ extern "C" int cism_surrogate_set_channel_weights(cism_surrogate *s, const float
     HIP_TRY(hipMemcpyAsync(s->chw, w24, sizeof(float) * SG_OUT, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The learned initial guess on a solver context (cism_learned_guess*): features
+// from the context's resident fields, the network, and the decoded velocity written
+// into the context's x, all on the device.
+//
+// Image layout: pixel (y, x) is owned vertex (i, j) = (x + ilo, y + jlo).  A periodic
+// grid's owned vertices are the whole image (their count must be a multiple of 8 in
+// each direction, the network's periodic padding then matches the grid's).  An
+// outflow grid's owned vertices sit in the corner of a canvas at least 8 pixels wider
+// and taller, rounded up to a multiple of 8, whose other pixels carry zero features
+// (no ice) and separate the domain's edges across the periodic wrap.
+// ---------------------------------------------------------------------------
+#define SG_V_CH 11             // output channels: u at levels 0..10, v at 11..21
+#define SG_NZ_MAX 11
+
+struct cism_guess {
+    cism_surrogate *net;
+    int W, H, Wown, Hown, i0, j0;
+    int nz, encoding;          // encoding 0: v = uscale o; 1: v = sign(o) uscale expm1(|o|)
+    double uscale;
+};
+
+// features of one pixel (stagthck / 1000, 100 ds/dx, 100 ds/dy, (ln max(beta, 1e-3) - 7) / 2,
+// log10 of the column-mean flow factor + 16; zero at inactive vertices and off the grid),
+// the column-mean A of each of the vertex's four cells from flwafact = A^(-1/3) / 2
+__global__ __launch_bounds__(256) void k_guess_features(GridDev g, int W, int H, int Wown, int Hown, int i0,
+                                                        int j0, const int32_t *__restrict__ av,
+                                                        const double *__restrict__ sthck,
+                                                        const double *__restrict__ dsdx,
+                                                        const double *__restrict__ dsdy,
+                                                        const double *__restrict__ beta,
+                                                        const double *__restrict__ ff, float *__restrict__ in) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)W * H) return;
+    const int y = (int)(p / W), x = (int)(p % W);
+    float f[SG_IN_RAW] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (x < Wown && y < Hown) {
+        const int i = x + i0, j = y + j0;
+        const long long v = i + (long long)g.sx * j;
+        if (av[v]) {
+            double vA = 0.0;
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    const long long cell = (i + di) + (long long)g.nx * (j + dj);
+                    double colA = 0.0;
+                    for (int k = 0; k < g.nz - 1; ++k) {
+                        const double t = ff[k + (long long)(g.nz - 1) * cell];
+                        colA += t > 0.0 ? 1.0 / (8.0 * t * t * t) : 0.0;
+                    }
+                    vA += colA / (g.nz - 1);
+                }
+            vA *= 0.25;
+            f[0] = (float)(sthck[v] / 1000.0);
+            f[1] = (float)(dsdx[v] * 100.0);
+            f[2] = (float)(dsdy[v] * 100.0);
+            f[3] = (float)((log(fmax(beta[v], 1e-3)) - 7.0) / 2.0);
+            f[4] = (float)(log10(fmax(vA, 1e-30)) + 16.0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < SG_IN; ++c) in[p * SG_IN + c] = c < SG_IN_RAW ? f[c] : 0.f;
+}
+
+// largest |output| decoded (network units), as the host decoder clips
+__device__ inline double guess_decode(float o, int encoding, double uscale) {
+    const double lim = encoding == 1 ? 10.0 : 1000.0;
+    const double t = fmin(lim, fmax(-lim, (double)o));
+    if (encoding == 1) return (t < 0 ? -uscale : uscale) * expm1(fabs(t));
+    return uscale * t;
+}
+
+// x = the decoded velocity at owned active vertices, 0 elsewhere (halo images follow)
+__global__ __launch_bounds__(256) void k_guess_decode(GridDev g, int W, int Wown, int Hown, int i0, int j0,
+                                                      int encoding, double uscale,
+                                                      const int32_t *__restrict__ av,
+                                                      const float *__restrict__ out, double *__restrict__ x) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.nn) return;
+    const int k = (int)(q % g.nz);
+    const long long col = q / g.nz;
+    const int i = (int)(col % g.sx), j = (int)(col / g.sx);
+    const int px = i - i0, py = j - j0;
+    double u = 0.0, v = 0.0;
+    if (px >= 0 && px < Wown && py >= 0 && py < Hown && av[col]) {
+        const long long p = (long long)py * W + px;
+        u = guess_decode(out[p * SG_OUT_PAD + k], encoding, uscale);
+        v = guess_decode(out[p * SG_OUT_PAD + SG_V_CH + k], encoding, uscale);
+    }
+    x[2 * q] = u;
+    x[2 * q + 1] = v;
+}
+
+void learned_guess_free(cism_ctx *c) {
+    if (!c || !c->guess) return;
+    cism_surrogate_destroy(c->guess->net);
+    delete c->guess;
+    c->guess = nullptr;
+}
+
+int learned_guess_device(cism_ctx *c) {
+    cism_guess *gs = c->guess;
+    if (!gs) {
+        cism_set_error("learned guess: no network loaded (cism_learned_guess_load)");
+        return CISM_ERR_ARG;
+    }
+    const GridDev &g = c->g;
+    cism_surrogate *s = gs->net;
+    const long long P = (long long)gs->W * gs->H;
+    hipLaunchKernelGGL(k_guess_features, dim3(nblocks_for(P)), dim3(256), 0, c->stream, g, gs->W, gs->H, gs->Wown,
+                       gs->Hown, gs->i0, gs->j0, c->av, c->sthck, c->dsdx, c->dsdy, c->beta, c->flwafact,
+                       s->A[A_IN].y);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    CISM_TRY(forward(s));
+    HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipEventElapsedTime(&s->last_forward_ms, s->ev0, s->ev1));
+    hipLaunchKernelGGL(k_guess_decode, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, gs->W, gs->Wown,
+                       gs->Hown, gs->i0, gs->j0, gs->encoding, gs->uscale, c->av, s->A[A_OUT].y, c->x);
+    HIP_TRY(hipGetLastError());
+    return launch_halo_vec(c, c->x);
+}
+
+extern "C" int cism_learned_guess_load(cism_ctx *c, const float *params, long long nparams, int nz, int encoding,
+                                       double uscale) {
+    if (!c || !params || (encoding != 0 && encoding != 1) || !(uscale > 0)) {
+        cism_set_error("learned_guess_load: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const GridDev &g = c->g;
+    if (nz != g.nz || nz > SG_NZ_MAX) {
+        // the output channels hold nz levels of u then of v from channel 11: another nz
+        // would read the wrong channels
+        cism_set_error("learned_guess_load: the weights predict %d levels, the grid has nz = %d (at most %d)", nz,
+                       g.nz, SG_NZ_MAX);
+        return CISM_ERR_ARG;
+    }
+    const int Wown = g.ihi - g.ilo + 1, Hown = g.jhi - g.jlo + 1;
+    int W = Wown, H = Hown;
+    if (g.periodic) {
+        if (Wown % 8 || Hown % 8) {
+            cism_set_error("learned_guess_load: a periodic grid needs ewn and nsn divisible by 8 (got %d x %d)",
+                           Wown, Hown);
+            return CISM_ERR_UNSUPPORTED;
+        }
+    } else {
+        W = (Wown + 8 + 7) / 8 * 8;
+        H = (Hown + 8 + 7) / 8 * 8;
+    }
+    learned_guess_free(c);
+    cism_surrogate *s = nullptr;
+    CISM_TRY(cism_surrogate_create(H, W, 1, c->device, &s));
+    if (nparams != s->nparams) {
+        cism_surrogate_destroy(s);
+        cism_set_error("learned_guess_load: %lld parameters given, the network has %lld", nparams, s->nparams);
+        return CISM_ERR_ARG;
+    }
+    const int st = cism_surrogate_set_params(s, params);
+    if (st) {
+        cism_surrogate_destroy(s);
+        return st;
+    }
+    cism_guess *gs = new cism_guess();
+    gs->net = s;
+    gs->W = W; gs->H = H; gs->Wown = Wown; gs->Hown = Hown; gs->i0 = g.ilo; gs->j0 = g.jlo;
+    gs->nz = nz; gs->encoding = encoding; gs->uscale = uscale;
+    c->guess = gs;
+    return CISM_OK;
+}
+
+// weight file: "CISMSG01", int32 nz, int32 encoding, float64 uscale, int64 nparams, then
+// nparams float32 (little endian)
+extern "C" int cism_learned_guess_load_file(cism_ctx *c, const char *path) {
+    if (!c || !path) {
+        cism_set_error("learned_guess_load_file: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        cism_set_error("learned_guess_load_file: cannot open %s", path);
+        return CISM_ERR_ARG;
+    }
+    char magic[8];
+    int32_t nz = 0, enc = 0;
+    double uscale = 0.0;
+    long long n = 0;
+    int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "CISMSG01", 8) == 0 && fread(&nz, 4, 1, f) == 1 &&
+             fread(&enc, 4, 1, f) == 1 && fread(&uscale, 8, 1, f) == 1 && fread(&n, 8, 1, f) == 1 && n > 0 &&
+             n < (1LL << 28);
+    std::vector<float> w;
+    if (ok) {
+        w.resize((size_t)n);
+        ok = fread(w.data(), sizeof(float), (size_t)n, f) == (size_t)n;
+    }
+    fclose(f);
+    if (!ok) {
+        cism_set_error("learned_guess_load_file: %s is not a weight file", path);
+        return CISM_ERR_ARG;
+    }
+    return cism_learned_guess_load(c, w.data(), n, nz, enc, uscale);
+}
+
+extern "C" int cism_learned_guess(cism_ctx *c, const double *stagthck, const double *dusrf_dx,
+                                  const double *dusrf_dy, const double *beta_internal, const double *flwafact,
+                                  const int32_t *active_vertex, double *uvel, double *vvel) {
+    if (!c) {
+        cism_set_error("learned_guess: null context");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    struct { double *dst; const double *src; size_t n; } ins[] = {
+        {c->sthck, stagthck, ncol}, {c->dsdx, dusrf_dx, ncol}, {c->dsdy, dusrf_dy, ncol},
+        {c->beta, beta_internal, ncol}, {c->flwafact, flwafact, (size_t)g.ne}};
+    for (auto &t : ins)
+        if (t.src) HIP_TRY(hipMemcpyAsync(t.dst, t.src, sizeof(double) * t.n, hipMemcpyHostToDevice, c->stream));
+    if (active_vertex)
+        HIP_TRY(hipMemcpyAsync(c->av, active_vertex, sizeof(int32_t) * ncol, hipMemcpyHostToDevice, c->stream));
+    CISM_TRY(learned_guess_device(c));
+    if (uvel && vvel) CISM_TRY(download_vec(c, c->x, uvel, vvel));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CISM_OK;
+}
+
+extern "C" double cism_learned_guess_last_ms(const cism_ctx *c) {
+    return c && c->guess ? (double)c->guess->net->last_forward_ms : 0.0;
 }

@@ -139,34 +139,50 @@ __global__ void k_bit(long long n, const int32_t *__restrict__ bits, int b, int3
     if (t < n) out[t] = (bits[t] >> b) & 1;
 }
 
-extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, double thklim, double eus,
-                                         double max_slope, const double *thck, const double *usrf,
-                                         const double *topg, const double *flwa, int32_t *ice_mask,
-                                         int32_t *floating_mask, int32_t *ocean_mask, int32_t *land_mask,
-                                         int32_t *active_cell, int32_t *active_vertex, double *xVertex,
-                                         double *yVertex, double *stagusrf, double *stagthck,
-                                         double *dusrf_dx, double *dusrf_dy, double *flwafact) {
-    if (!c || !thck || !usrf || !topg || !flwa || !(dx > 0) || !(dy > 0) || !(max_slope > 0)) {
-        cism_set_error("glissade_velo_inputs: invalid arguments");
+// The inputs of the solve from the cell fields thck, usrf, topg (nx, ny) and flwa
+// (nz-1, nx, ny), all host arrays: results in the context (sthck, susrf, dsdx, dsdy, ac,
+// xV, yV, av, flwafact); the cell masks as bits (k_masks) in c->umask, until the next
+// writer of the Dirichlet masks.
+int velo_inputs_device(cism_ctx *c, double dx, double dy, double thklim, double eus, double max_slope,
+                       const double *thck, const double *usrf, const double *topg, const double *flwa) {
+    CISM_TRY(velo_inputs_upload(c, thck, usrf, topg, flwa));
+    return velo_inputs_compute(c, dx, dy, thklim, eus, max_slope);
+}
+
+// the cell fields into device staging: the element scratch (allocated by the first
+// assembly or here) holds thck, usrf, topg and flwa until the next assembly
+int velo_inputs_upload(cism_ctx *c, const double *thck, const double *usrf, const double *topg,
+                       const double *flwa) {
+    const GridDev &g = c->g;
+    if (!thck || !usrf || !topg || !flwa) {
+        cism_set_error("glissade_velo_inputs: null cell field");
         return CISM_ERR_ARG;
     }
-    HIP_TRY(hipSetDevice(c->device));
-    const GridDev &g = c->g;
     if (g.nz < 2) {
         cism_set_error("glissade_velo_inputs needs a 3D context (nz >= 2)");
         return CISM_ERR_ARG;
     }
-    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
-    // device staging: cell fields in the element scratch (allocated by the first assembly
-    // or here), bits in the node-mask buffer of u (nn >= ncell ints)
+    const long long ncell = (long long)g.nx * g.ny;
     CISM_TRY(ensure_ke(c));
+    HIP_TRY(hipMemcpyAsync(c->Ke, thck, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Ke + ncell, usrf, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Ke + 2 * ncell, topg, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->Ke + 3 * ncell, flwa, sizeof(double) * g.ne, hipMemcpyHostToDevice, c->stream));
+    return CISM_OK;
+}
+
+// the inputs from the staged cell fields; the mask bits go to the node-mask buffer of u
+// (nn >= ncell ints), overwritten by the next solve's Dirichlet masks
+int velo_inputs_compute(cism_ctx *c, double dx, double dy, double thklim, double eus, double max_slope) {
+    if (!(dx > 0) || !(dy > 0) || !(max_slope > 0)) {
+        cism_set_error("glissade_velo_inputs: invalid arguments");
+        return CISM_ERR_ARG;
+    }
+    const GridDev &g = c->g;
+    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
     double *d_thck = c->Ke, *d_usrf = c->Ke + ncell, *d_topg = c->Ke + 2 * ncell, *d_flwa = c->Ke + 3 * ncell;
-    int32_t *bits = c->umask;   // overwritten by the next solve's Dirichlet masks
+    int32_t *bits = c->umask;
     c->dirichlet_any = 1;   // the masks are scratch now: no longer known empty
-    HIP_TRY(hipMemcpyAsync(d_thck, thck, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(d_usrf, usrf, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(d_topg, topg, sizeof(double) * ncell, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(d_flwa, flwa, sizeof(double) * g.ne, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_masks, dim3(nblocks_for(ncell)), dim3(256), 0, c->stream, g, d_thck, d_topg, eus,
                        thklim, bits);
     hipLaunchKernelGGL(k_stagger, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, d_thck, d_usrf, bits,
@@ -178,6 +194,25 @@ extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, doub
                        c->xV, c->yV, c->av);
     hipLaunchKernelGGL(k_flwafact, dim3(nblocks_for(g.ne)), dim3(256), 0, c->stream, g, d_flwa, c->flwafact);
     HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+extern "C" int cism_glissade_velo_inputs(cism_ctx *c, double dx, double dy, double thklim, double eus,
+                                         double max_slope, const double *thck, const double *usrf,
+                                         const double *topg, const double *flwa, int32_t *ice_mask,
+                                         int32_t *floating_mask, int32_t *ocean_mask, int32_t *land_mask,
+                                         int32_t *active_cell, int32_t *active_vertex, double *xVertex,
+                                         double *yVertex, double *stagusrf, double *stagthck,
+                                         double *dusrf_dx, double *dusrf_dy, double *flwafact) {
+    if (!c) {
+        cism_set_error("glissade_velo_inputs: null context");
+        return CISM_ERR_ARG;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    CISM_TRY(velo_inputs_device(c, dx, dy, thklim, eus, max_slope, thck, usrf, topg, flwa));
+    const GridDev &g = c->g;
+    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
+    const int32_t *bits = c->umask;
     int32_t *masks[4] = {ice_mask, floating_mask, ocean_mask, land_mask};
     int32_t *mbuf = c->vmask;   // scratch for one mask at a time
     c->dirichlet_any = 1;   // the masks are scratch now: no longer known empty
@@ -452,6 +487,71 @@ extern "C" int cism_set_basal_physics(cism_ctx *c, const cism_basal_physics *bp,
     }
     CISM_TRY(upload_basal_fields(c, beta_external, effecpress_stag, f_ground, c->bp_bext, c->bp_N, c->bp_fg));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->bp = *bp;
+    c->bp_on = 1;
+    return CISM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// cism_glissade_velo_higher_solve's basal set-up from the resident geometry
+// ---------------------------------------------------------------------------
+// the mask bits of k_masks as int32 planes: ice, floating, ocean, land, and a zero
+// calving-front plane (no subgrid calving front)
+__global__ __launch_bounds__(256) void k_mask_planes(long long ncell, const int32_t *__restrict__ bits,
+                                                     int32_t *__restrict__ planes) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    const int b = bits[t];
+    planes[t] = b & 1;
+    planes[ncell + t] = (b >> 1) & 1;
+    planes[2 * ncell + t] = (b >> 2) & 1;
+    planes[3 * ncell + t] = (b >> 3) & 1;
+    planes[4 * ncell + t] = 0;
+}
+
+int mask_planes_device(cism_ctx *c) {
+    const long long ncell = (long long)c->g.nx * c->g.ny;
+    if (!c->planes) HIP_TRY(hipMalloc(&c->planes, sizeof(int32_t) * 5 * ncell));
+    hipLaunchKernelGGL(k_mask_planes, dim3(nblocks_for(ncell)), dim3(256), 0, c->stream, ncell, c->umask,
+                       c->planes);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
+int basal_fields_upload(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
+                        const double *effecpress_stag) {
+    const long long ncol = (long long)c->g.sx * c->g.sy;
+    // the same checks as calcbeta's; f_ground comes from the device here
+    double fg_dummy = 0.0;
+    CISM_TRY(check_basal_fields(c, bp, beta_external, effecpress_stag, &fg_dummy));
+    if (!c->bp_fg) {
+        HIP_TRY(hipMalloc(&c->bp_fg, sizeof(double) * ncol));
+        HIP_TRY(hipMalloc(&c->bp_bext, sizeof(double) * ncol));
+        HIP_TRY(hipMalloc(&c->bp_N, sizeof(double) * ncol));
+    }
+    if (beta_external)
+        HIP_TRY(hipMemcpyAsync(c->bp_bext, beta_external, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    else
+        HIP_TRY(hipMemsetAsync(c->bp_bext, 0, sizeof(double) * ncol, c->stream));
+    if (effecpress_stag)
+        HIP_TRY(hipMemcpyAsync(c->bp_N, effecpress_stag, sizeof(double) * ncol, hipMemcpyHostToDevice, c->stream));
+    else
+        HIP_TRY(hipMemsetAsync(c->bp_N, 0, sizeof(double) * ncol, c->stream));
+    return CISM_OK;
+}
+
+// after basal_fields_upload: f_ground from the mask planes; bp becomes the Picard loop's
+int basal_setup_device(cism_ctx *c, const int32_t *planes, int whichground, const cism_basal_physics *bp) {
+    const GridDev &g = c->g;
+    const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
+    if (whichground != 0 && whichground != 2) {
+        cism_set_error("which_ho_ground %d not supported (0 or 2; the reference stops on the GLP option 1)",
+                       whichground);
+        return CISM_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_grounded_fraction, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, planes,
+                       planes + ncell, planes + 3 * ncell, whichground, c->bp_fg);
+    HIP_TRY(hipGetLastError());
     c->bp = *bp;
     c->bp_on = 1;
     return CISM_OK;

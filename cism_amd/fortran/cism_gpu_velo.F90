@@ -12,7 +12,15 @@
 !   glissade_velo_higher_solve_3d_gpu  the Picard loop of glissade_velo_higher_solve
 !                                      (3D branch), device-resident, with explicit
 !                                      arrays instead of the model derived type
+!   glissade_velo_higher_solve_gpu     glissade_velo_higher_solve from the ice geometry
+!                                      (thck, usrf, topg, flwa): masks, staggering, surface
+!                                      gradient, beta, loads and the Picard loop on the
+!                                      device, optionally from the learned initial guess
 !   cism_gpu_velo_init / _finalize     context lifetime (glissade_velo_higher_init)
+!   cism_gpu_velo_set_learned_guess    loads the learned-guess network's weight file
+! The two solves take the fast path's options as optional arguments: precision
+! (1: FP32 matrix with FP64 refinement), adaptive_linear, inner_rtol, anderson_m, and
+! report converged (the reference's converged_soln) and fallbacks.
 !
 ! Each call converts LOGICAL masks to int32 (LOGICAL's kind and .true. value
 ! are compiler specific) and calls libcism_gpu.so through bind(C).  A non-zero
@@ -28,6 +36,7 @@ module cism_gpu_velo
 
   public :: pcg_solver_standard_3d, pcg_solver_chrongear_3d, matvec_multiply_structured_3d
   public :: assemble_stiffness_matrix_3d, glissade_velo_higher_solve_3d_gpu
+  public :: glissade_velo_higher_solve_gpu, cism_gpu_velo_set_learned_guess
   public :: cism_gpu_velo_init, cism_gpu_velo_finalize, cism_gpu_set_linear_solver
 
   type, bind(C) :: cism_solve_opts
@@ -36,7 +45,7 @@ module cism_gpu_velo
      real(c_double) :: efvs_constant, dx, dy, target_override, inner_rtol
      integer(c_int) :: adaptive_linear
      real(c_double) :: adaptive_gamma, stop_rel_err
-     integer(c_int) :: anderson_m, reserved(4)
+     integer(c_int) :: anderson_m, initial_guess, reserved(3)
   end type cism_solve_opts
 
   type, bind(C) :: cism_solve_stats
@@ -47,6 +56,17 @@ module cism_gpu_velo
      real(c_double) :: final_rel_err, seconds_device
      integer(c_int) :: converged, fallbacks, guess_rejected, reserved
   end type cism_solve_stats
+
+  type, bind(C) :: cism_basal_physics
+     integer(c_int) :: whichbabc, reserved
+     real(c_double) :: ho_beta_const, ho_beta_large, powerlaw_c, powerlaw_m, coulomb_c
+     real(c_double) :: beta_grounded_min, beta_powerlaw_umax
+  end type cism_basal_physics
+
+  type, bind(C) :: cism_velo_setup
+     real(c_double) :: dx, dy, thklim, eus, max_slope
+     integer(c_int) :: whichground, whichassemble_taud, lateral_bc, initial_guess, reserved(4)
+  end type cism_velo_setup
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: ctx_nx = -1, ctx_ny = -1, ctx_nz = -1, ctx_iper = -1
@@ -143,7 +163,28 @@ module cism_gpu_velo
        real(c_double), intent(out) :: efvs(*), bu(*), bv(*), ru(*), rv(*)
        type(cism_solve_stats), intent(out) :: stats
      end function c_solve
+
+     integer(c_int) function c_geometry_solve(h, o, su, sigma, thck, usrf, topg, flwa, bp, bext, &
+          effecpress, uvel, vvel, efvs, stats) bind(C, name='cism_glissade_velo_higher_solve')
+       import :: c_int, c_ptr, c_double, cism_solve_opts, cism_velo_setup, cism_solve_stats
+       type(c_ptr), value :: h
+       type(cism_solve_opts) :: o
+       type(cism_velo_setup) :: su
+       real(c_double), intent(in) :: sigma(*), thck(*), usrf(*), topg(*), flwa(*)
+       type(c_ptr), value :: bp, bext, effecpress
+       real(c_double), intent(inout) :: uvel(*), vvel(*)
+       real(c_double), intent(out) :: efvs(*)
+       type(cism_solve_stats), intent(out) :: stats
+     end function c_geometry_solve
+
+     integer(c_int) function c_guess_load_file(h, path) bind(C, name='cism_learned_guess_load_file')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+     end function c_guess_load_file
   end interface
+
+  logical, save :: guess_loaded = .false.
 
 contains
 
@@ -174,6 +215,7 @@ contains
     ctx = c_null_ptr
     ctx_nx = -1
     ctx_iper = -1
+    guess_loaded = .false.
   end subroutine cism_gpu_velo_finalize
 
   !> maxiters and tolerance of the linear solvers (module constants of the reference)
@@ -379,7 +421,9 @@ contains
                                                whichsparse, whichprecond, whichapprox,     &
                                                maxiter_nonlinear, efvs_constant,           &
                                                uvel, vvel, efvs, bu, bv, resid_u, resid_v, &
-                                               picard_iters, linear_iters)
+                                               picard_iters, linear_iters, precision,      &
+                                               adaptive_linear, inner_rtol, anderson_m,    &
+                                               converged, fallbacks)
     integer, intent(in) :: nx, ny, nz
     real(dp), intent(in) :: dx, dy
     real(dp), dimension(nz), intent(in) :: sigma
@@ -396,6 +440,10 @@ contains
     real(dp), dimension(nz-1,nx,ny), intent(out) :: efvs
     real(dp), dimension(nz,nx-1,ny-1), intent(out) :: bu, bv, resid_u, resid_v
     integer, intent(out) :: picard_iters, linear_iters
+    integer, intent(in), optional :: precision, adaptive_linear, anderson_m
+    real(dp), intent(in), optional :: inner_rtol
+    logical, intent(out), optional :: converged
+    integer, intent(out), optional :: fallbacks
     type(cism_solve_opts) :: o
     type(cism_solve_stats) :: stats
     integer(c_int32_t), allocatable :: ac(:,:), av(:,:), um(:,:,:), vm(:,:,:)
@@ -406,6 +454,7 @@ contains
     o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
     o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
     o%dx = dx; o%dy = dy
+    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
     ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
     av = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     um = int(umask_dirichlet, c_int32_t)
@@ -416,6 +465,101 @@ contains
                'glissade_velo_higher_solve_3d_gpu')
     picard_iters = stats%picard_iters
     linear_iters = stats%linear_iters
+    if (present(converged)) converged = stats%converged /= 0
+    if (present(fallbacks)) fallbacks = stats%fallbacks
   end subroutine glissade_velo_higher_solve_3d_gpu
+
+  !> the fast path's options (absent: the reference path, precision 0)
+  subroutine fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
+    type(cism_solve_opts), intent(inout) :: o
+    integer, intent(in), optional :: precision, adaptive_linear, anderson_m
+    real(dp), intent(in), optional :: inner_rtol
+    if (present(precision)) o%precision = precision
+    if (present(adaptive_linear)) o%adaptive_linear = adaptive_linear
+    if (present(inner_rtol)) o%inner_rtol = inner_rtol
+    if (present(anderson_m)) o%anderson_m = anderson_m
+  end subroutine fast_path_options
+
+  !> Loads the learned initial guess's weight file (cism_learned_guess_load_file) into the
+  !> context of this grid.
+  subroutine cism_gpu_velo_set_learned_guess(nx, ny, nz, path, periodic)
+    integer, intent(in) :: nx, ny, nz
+    character(*), intent(in) :: path
+    logical, intent(in), optional :: periodic
+    call cism_gpu_velo_init(nx, ny, nz, periodic)
+    call check(c_guess_load_file(ctx, trim(path)//c_null_char), 'cism_gpu_velo_set_learned_guess')
+    guess_loaded = .true.
+  end subroutine cism_gpu_velo_set_learned_guess
+
+  !> glissade_velo_higher_solve (glissade_velo_higher.F90:654-3986, 3D branch) from the ice
+  !> geometry: thck, usrf, topg (nx, ny) and flwa (nz-1, nx, ny) in, uvel, vvel (the
+  !> initial guess unless learned_guess) and efvs out.  Basal traction by calcbeta from
+  !> beta_external (which_ho_babc 5) or the other laws of cism_basal_physics in every
+  !> Picard iteration; which_ho_babc 6 is no-slip.  Lateral ice-ocean pressure at marine
+  !> margins; local gravity load (whichassemble_taud 1) unless given.
+  subroutine glissade_velo_higher_solve_gpu(nx, ny, nz, dx, dy, sigma, thck, usrf, topg, flwa,    &
+                                            thklim, eus, whichbabc, beta_external, whichefvs,   &
+                                            whichresid, whichsparse, whichprecond, whichapprox, &
+                                            maxiter_nonlinear, efvs_constant, uvel, vvel, efvs, &
+                                            picard_iters, linear_iters, whichassemble_taud,     &
+                                            precision, adaptive_linear, inner_rtol, anderson_m, &
+                                            learned_guess, converged, fallbacks, periodic)
+    integer, intent(in) :: nx, ny, nz
+    real(dp), intent(in) :: dx, dy, thklim, eus, efvs_constant
+    real(dp), dimension(nz), intent(in) :: sigma
+    real(dp), dimension(nx,ny), intent(in) :: thck, usrf, topg
+    real(dp), dimension(nz-1,nx,ny), intent(in) :: flwa
+    real(dp), dimension(nx-1,ny-1), intent(in), target :: beta_external
+    integer, intent(in) :: whichbabc, whichefvs, whichresid, whichsparse, whichprecond
+    integer, intent(in) :: whichapprox, maxiter_nonlinear
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: uvel, vvel
+    real(dp), dimension(nz-1,nx,ny), intent(out) :: efvs
+    integer, intent(out) :: picard_iters, linear_iters
+    integer, intent(in), optional :: whichassemble_taud, precision, adaptive_linear, anderson_m
+    real(dp), intent(in), optional :: inner_rtol
+    logical, intent(in), optional :: learned_guess, periodic
+    logical, intent(out), optional :: converged
+    integer, intent(out), optional :: fallbacks
+    type(cism_solve_opts) :: o
+    type(cism_solve_stats) :: stats
+    type(cism_velo_setup) :: su
+    type(cism_basal_physics), target :: bp
+    type(c_ptr) :: pbp, pbext
+    call cism_gpu_velo_init(nx, ny, nz, periodic)
+    call c_solve_opts_default(o)
+    o%whichbabc = whichbabc; o%whichefvs = whichefvs; o%whichresid = whichresid
+    o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
+    o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
+    o%dx = dx; o%dy = dy
+    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
+    su%dx = dx; su%dy = dy; su%thklim = thklim; su%eus = eus; su%max_slope = 1.0_dp
+    su%whichground = 0; su%lateral_bc = 1; su%initial_guess = 0; su%reserved = 0
+    su%whichassemble_taud = 1
+    if (present(whichassemble_taud)) su%whichassemble_taud = whichassemble_taud
+    if (present(learned_guess)) then
+       if (learned_guess) then
+          if (.not. guess_loaded) then
+             write(*,'(a)') 'FATAL (glissade_velo_higher_solve_gpu): learned_guess without '// &
+                  'cism_gpu_velo_set_learned_guess'
+             error stop 1
+          endif
+          su%initial_guess = 1
+       endif
+    endif
+    ! calcbeta's parameters at the reference defaults (glide_types.F90:1308-1370)
+    bp%whichbabc = whichbabc; bp%reserved = 0
+    bp%ho_beta_const = 1000.0_dp; bp%ho_beta_large = 1.0e10_dp; bp%powerlaw_c = 1.0e4_dp
+    bp%powerlaw_m = 3.0_dp; bp%coulomb_c = 0.42_dp; bp%beta_grounded_min = 1.0_dp
+    bp%beta_powerlaw_umax = 0.0_dp
+    pbp = c_null_ptr
+    if (whichbabc /= 6) pbp = c_loc(bp)
+    pbext = c_loc(beta_external)
+    call check(c_geometry_solve(ctx, o, su, sigma, thck, usrf, topg, flwa, pbp, pbext, c_null_ptr, &
+                                uvel, vvel, efvs, stats), 'glissade_velo_higher_solve_gpu')
+    picard_iters = stats%picard_iters
+    linear_iters = stats%linear_iters
+    if (present(converged)) converged = stats%converged /= 0
+    if (present(fallbacks)) fallbacks = stats%fallbacks
+  end subroutine glissade_velo_higher_solve_gpu
 
 end module cism_gpu_velo

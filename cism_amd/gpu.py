@@ -38,7 +38,8 @@ EXPORTED = [
     "cism_surrogate_create", "cism_surrogate_destroy", "cism_surrogate_num_params",
     "cism_surrogate_get_params", "cism_surrogate_set_params", "cism_surrogate_set_channel_weights",
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
-    "cism_surrogate_last_forward_ms",
+    "cism_surrogate_last_forward_ms", "cism_glissade_velo_higher_solve", "cism_learned_guess_load",
+    "cism_learned_guess_load_file", "cism_learned_guess", "cism_learned_guess_last_ms",
 ]
 
 
@@ -55,7 +56,15 @@ class SolveOpts(C.Structure):
                 ("efvs_constant", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
-                ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("reserved", C.c_int * 4)]
+                ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("initial_guess", C.c_int),
+                ("reserved", C.c_int * 3)]
+
+
+class VeloSetup(C.Structure):
+    """cism_velo_setup: what cism_glissade_velo_higher_solve needs besides the geometry"""
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("thklim", C.c_double), ("eus", C.c_double),
+                ("max_slope", C.c_double), ("whichground", C.c_int), ("whichassemble_taud", C.c_int),
+                ("lateral_bc", C.c_int), ("initial_guess", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class BasalPhysics(C.Structure):
@@ -145,6 +154,15 @@ def lib():
         L.cism_glissade_grounded_fraction.argtypes = [VP, I32, I32, I32, C.c_int, D]
         L.cism_calcbeta.argtypes = [VP, C.POINTER(BasalPhysics), D, D, D, D, D, D]
         L.cism_set_basal_physics.argtypes = [VP, C.POINTER(BasalPhysics), D, D, D]
+        L.cism_glissade_velo_higher_solve.argtypes = [VP, C.POINTER(SolveOpts), C.POINTER(VeloSetup), D, D, D,
+                                                      D, D, C.POINTER(BasalPhysics), D, D, D, D, D,
+                                                      C.POINTER(SolveStats)]
+        L.cism_learned_guess_load.argtypes = [VP, C.POINTER(C.c_float), C.c_longlong, C.c_int, C.c_int,
+                                              C.c_double]
+        L.cism_learned_guess_load_file.argtypes = [VP, C.c_char_p]
+        L.cism_learned_guess.argtypes = [VP, D, D, D, D, D, I32, D, D]
+        L.cism_learned_guess_last_ms.argtypes = [VP]
+        L.cism_learned_guess_last_ms.restype = C.c_double
         _lib = L
     return _lib
 
@@ -430,11 +448,9 @@ class Context:
         _check(lib().cism_staggered_parallel_halo(self.h, lead, dp(a)))
 
     # --- device-resident Picard solve ------------------------------------------
-    def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
-                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
-                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
-                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0,
-                             anderson_m=0):
+    def _opts(self, p, precision=0, maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
+              whichresid=None, target_override=0.0, inner_rtol=1e-3, adaptive_linear=0,
+              adaptive_gamma=0.05, stop_rel_err=0.0, anderson_m=0, initial_guess=0):
         o = SolveOpts()
         lib().cism_solve_opts_default(C.byref(o))
         o.whichbabc = p.whichbabc
@@ -453,6 +469,84 @@ class Context:
         o.adaptive_gamma = adaptive_gamma
         o.stop_rel_err = stop_rel_err
         o.anderson_m = anderson_m
+        o.initial_guess = initial_guess
+        return o
+
+    def glissade_velo_higher_solve(self, p, bp=None, beta_external=None, effecpress_stag=None,
+                                   initial_guess=0, uvel=None, vvel=None, lateral_bc=None,
+                                   whichground=0, max_slope=1.0, eus=0.0, efvs=True, **kw):
+        """glissade_velo_higher_solve from the geometry of p (thck, usrf, topg, flwa) with every
+        input computed on the device; bp (BasalPhysics) is required unless p is no-slip.
+        initial_guess: 0 uvel/vvel (zero by default), 1 the learned guess, 2 the context's
+        last velocity.  Options of the Picard loop as in velo_higher_solve_3d."""
+        o = self._opts(p, **kw)
+        su = VeloSetup(dx=p.dx, dy=p.dy, thklim=p.thklim, eus=eus, max_slope=max_slope,
+                       whichground=whichground, whichassemble_taud=p.whichassemble_taud,
+                       lateral_bc=int(p.ocean_mask.any()) if lateral_bc is None else int(lateral_bc),
+                       initial_guess=initial_guess)
+        u = (p.uvel if uvel is None else uvel).copy(order="F")
+        v = (p.vvel if vvel is None else vvel).copy(order="F")
+        ef = np.zeros((p.nz - 1, p.nx, p.ny), order="F") if efvs else None
+        f = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)   # noqa: E731
+        bext, N = f(beta_external), f(effecpress_stag)
+        st = SolveStats()
+        _check(lib().cism_glissade_velo_higher_solve(
+            self.h, C.byref(o), C.byref(su), dp(p.sigma), dp(f(p.thck)), dp(f(p.usrf)), dp(f(p.topg)),
+            dp(f(p.flwa)), C.byref(bp) if bp is not None else None, dp(bext), dp(N), dp(u), dp(v),
+            dp(ef), C.byref(st)))
+        return dict(uvel=u, vvel=v, efvs=ef, **st.as_dict())
+
+    def learned_guess_load(self, params, nz, encoding, uscale):
+        w = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().cism_learned_guess_load(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), w.size, nz,
+                                             encoding, uscale))
+
+    def learned_guess_load_file(self, path):
+        _check(lib().cism_learned_guess_load_file(self.h, os.fsencode(path)))
+
+    def learned_guess(self, p=None, download=True):
+        """the learned guess into the context's velocity from p's fields (None: the
+        context's resident fields); returns (uvel, vvel) when download"""
+        shape = (self.nz, self.nx - 1, self.ny - 1)
+        u = np.zeros(shape, order="F") if download else None
+        v = np.zeros(shape, order="F") if download else None
+        if p is None:
+            args = [None] * 5 + [None]
+        else:
+            args = [dp(p.stagthck), dp(p.dusrf_dx), dp(p.dusrf_dy), dp(p.beta_internal), dp(p.flwafact),
+                    ip(p.active_vertex)]
+        _check(lib().cism_learned_guess(self.h, *args, dp(u), dp(v)))
+        return u, v
+
+    def learned_guess_last_ms(self):
+        return float(lib().cism_learned_guess_last_ms(self.h))
+
+    def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
+                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
+                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
+                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0,
+                             anderson_m=0, initial_guess=0):
+        """initial_guess 2: start from the context's velocity (the last solve's or the learned
+        guess), uvel/vvel then only receive the solution"""
+        o = SolveOpts()
+        lib().cism_solve_opts_default(C.byref(o))
+        o.whichbabc = p.whichbabc
+        o.whichefvs = p.whichefvs
+        o.whichresid = p.whichresid if whichresid is None else whichresid
+        o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+        o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+        o.whichapprox = p.whichapprox
+        o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter_nonlinear is None else maxiter_nonlinear
+        o.precision = precision
+        o.efvs_constant = p.efvs_constant
+        o.dx, o.dy = p.dx, p.dy
+        o.target_override = target_override
+        o.inner_rtol = inner_rtol
+        o.adaptive_linear = adaptive_linear
+        o.adaptive_gamma = adaptive_gamma
+        o.stop_rel_err = stop_rel_err
+        o.anderson_m = anderson_m
+        o.initial_guess = initial_guess
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
         shape = (p.nz, p.nx - 1, p.ny - 1)

@@ -266,6 +266,20 @@ def load_weights(path: str):
         return z["params"].astype(np.float32), json.loads(str(z["meta"]))
 
 
+def export_weights_bin(npz_path: str, bin_path: str):
+    """Writes a weight file for cism_learned_guess_load_file (e.g. for a Fortran host):
+    b"CISMSG01", int32 nz, int32 encoding (0 linear, 1 log1p), float64 uscale, int64
+    nparams, then the float32 parameters, little endian."""
+    params, meta = load_weights(npz_path)
+    enc = {"linear": 0, "log1p": 1}[meta.get("target", "linear")]
+    with open(bin_path, "wb") as f:
+        f.write(b"CISMSG01")
+        f.write(np.array([int(meta.get("nz", NZ_MAX)), enc], dtype="<i4").tobytes())
+        f.write(np.array([float(meta["uscale"])], dtype="<f8").tobytes())
+        f.write(np.array([params.size], dtype="<i8").tobytes())
+        f.write(np.asarray(params, dtype="<f4").tobytes())
+
+
 def default_weights(family: str) -> str:
     return os.path.join(WEIGHTS_DIR, f"surrogate_{family}.npz")
 

@@ -188,7 +188,11 @@ typedef struct {
                                   x_ref set by cism_set_reference_velocity                   */
     int    anderson_m;         /* precision 1: Anderson acceleration of the Picard map with
                                   this many history columns (0 off, at most 4)              */
-    int    reserved[4];
+    int    initial_guess;      /* cism_velo_higher_solve_3d: 0 start from uvel, vvel; 2 from
+                                  the context's velocity (the last solve's, or the guess of
+                                  cism_learned_guess), uvel and vvel then only receive the
+                                  solution                                                  */
+    int    reserved[3];
 } cism_solve_opts;
 
 typedef struct {
@@ -275,6 +279,37 @@ typedef struct {
     double ho_beta_const, ho_beta_large, powerlaw_c, powerlaw_m, coulomb_c;
     double beta_grounded_min, beta_powerlaw_umax;
 } cism_basal_physics;
+
+/* glissade_velo_higher_solve (glissade_velo_higher.F90:654-3986, the 3D branch) from the
+ * ice geometry, every step on the device: the inputs as cism_glissade_velo_inputs computes
+ * them (masks, staggered thickness and surface, surface gradient, vertex geometry,
+ * flwafact), the grounded fraction and calcbeta in every Picard iteration (bp; NULL only
+ * with opts->whichbabc = 6, no-slip), the gravity load and the lateral ice-ocean load,
+ * the Dirichlet masks (the bed level under no-slip; kinbcmask and no-penetration
+ * boundaries are not supported), the initial guess and the Picard loop.  Host transfers:
+ * thck, usrf, topg (nx, ny) and flwa (nz-1, nx, ny) in, uvel, vvel out, and efvs when
+ * non-NULL.  The call leaves bp set as the context's basal physics (as
+ * cism_set_basal_physics does). */
+typedef struct {
+    double dx, dy;             /* m                                                          */
+    double thklim;             /* ice thickness above which a cell is active (m)             */
+    double eus;                /* sea level (m)                                              */
+    double max_slope;          /* surface-slope limit (the reference's default is 1)         */
+    int    whichground;        /* which_ho_ground: 0 no GLP, 2 all grounded                  */
+    int    whichassemble_taud; /* 0 standard finite-element quadrature, 1 local             */
+    int    lateral_bc;         /* 1: lateral ice-ocean pressure at marine margins            */
+    int    initial_guess;      /* 0 uvel, vvel as given; 1 the learned guess (needs
+                                  cism_learned_guess_load); 2 the context's last velocity    */
+    int    reserved[4];
+} cism_velo_setup;
+
+int  cism_glissade_velo_higher_solve(cism_ctx *ctx, const cism_solve_opts *opts,
+                                     const cism_velo_setup *setup, const double *sigma,
+                                     const double *thck, const double *usrf, const double *topg,
+                                     const double *flwa, const cism_basal_physics *bp,
+                                     const double *beta_external, const double *effecpress_stag,
+                                     double *uvel, double *vvel, double *efvs,
+                                     cism_solve_stats *stats);
 
 /* ---------------------------------------------------------------------------
  * The solve's inputs computed on the device from the ice geometry (the set-up of
@@ -400,6 +435,31 @@ int  cism_surrogate_loss_grad(cism_surrogate *s, const float *in, const float *t
 /* loss_grad followed by one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) */
 int  cism_surrogate_train_step(cism_surrogate *s, const float *in, const float *target,
                                const float *mask, int nch, float lr, double *loss);
+
+/* The learned initial guess of a solver context, on the device (no counterpart in the
+ * reference).  cism_learned_guess_load builds the network for the context's grid and
+ * sets its parameters; nz must equal the context's (the output channels hold nz levels
+ * of u from channel 0 and of v from channel 11); encoding 0 decodes v = uscale o, 1
+ * decodes v = sign(o) uscale (exp|o| - 1), outputs clipped to |o| <= 1000 (0) or 10 (1).
+ * A periodic grid needs ewn and nsn divisible by 8; an outflow grid is embedded in a
+ * zero-feature canvas.  cism_learned_guess_load_file reads a weight file: the 8 bytes
+ * "CISMSG01", int32 nz, int32 encoding, float64 uscale, int64 nparams, then nparams
+ * float32 (little endian).  cism_learned_guess computes the features (stagthck / 1000,
+ * 100 ds/dx, 100 ds/dy, (ln max(beta, 1e-3) - 7) / 2, log10 of the column-mean flow
+ * factor + 16, zero at inactive vertices), runs the network and writes the decoded
+ * velocity, halos filled, into the context's velocity, where a solve with initial_guess
+ * 2 starts from it.  Its inputs are host arrays (vertex fields (nx-1, ny-1), flwafact
+ * (nz-1, nx, ny)); a NULL input keeps the context's current field (the one the last
+ * solve or cism_glissade_velo_inputs left).  uvel, vvel (optional) receive the guess. */
+int  cism_learned_guess_load(cism_ctx *ctx, const float *params, long long nparams, int nz,
+                             int encoding, double uscale);
+int  cism_learned_guess_load_file(cism_ctx *ctx, const char *path);
+int  cism_learned_guess(cism_ctx *ctx, const double *stagthck, const double *dusrf_dx,
+                        const double *dusrf_dy, const double *beta_internal,
+                        const double *flwafact, const int32_t *active_vertex,
+                        double *uvel, double *vvel);
+/* device time (ms) of the network's forward pass in the last guess */
+double cism_learned_guess_last_ms(const cism_ctx *ctx);
 
 #ifdef __cplusplus
 }

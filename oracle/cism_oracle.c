@@ -29,6 +29,17 @@
  */
 #include "cism_oracle.h"
 
+/* ORC_OMP (liborc_omp.so only): the hot loops run on all host cores with OpenMP, for the
+ * all-core CPU baseline.  That build sums in another order and is never the checker; the
+ * serial build (liborc.so) compiles the pragmas away and keeps the reference's order. */
+#ifdef ORC_OMP
+#define ORC_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define ORC_PAR_SUM_S _Pragma("omp parallel for schedule(static) reduction(+:s)")
+#else
+#define ORC_PAR_FOR
+#define ORC_PAR_SUM_S
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -294,6 +305,39 @@ int orc_assemble_3d(const orc_grid *g, const double *sigma, const int32_t *activ
     memset(Avu, 0, na * sizeof(double));
     memset(Avv, 0, na * sizeof(double));
     memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+#ifdef ORC_OMP
+    /* cell rows of one parity scatter to disjoint vertex rows (j-1, j) */
+    int status = ORC_OK;
+    for (int color = 0; color < 2; ++color) {
+        ORC_PAR_FOR
+        for (int j = nh + 1 + color; j <= ny - nh + 1; j += 2)
+            for (int i = nh + 1; i <= nx - nh + 1; ++i) {
+                if (!C2(active_cell, i, j)) continue;
+                double Kuu[64], Kuv[64], Kvu[64], Kvv[64];
+                for (int k = 1; k <= nz - 1; ++k) {
+                    double ev;
+                    int st = orc_element_matrix(g, i, j, k, sigma, xV, yV, uvel, vvel, stagusrf,
+                                                stagthck, flwafact, whichapprox, whichefvs,
+                                                efvs_constant, Kuu, Kuv, Kvu, Kvv, &ev);
+                    if (st) { status = st; continue; }
+                    F3(efvs, k, i, j) = ev;
+                    for (int nr = 0; nr < 8; ++nr) {
+                        int ir = i - 1 + NODE_IX[nr], jr = j - 1 + NODE_JY[nr], kr = k + NODE_KL[nr];
+                        for (int nc = 0; nc < 8; ++nc) {
+                            int m = IDX_A(NODE_IX[nc] - NODE_IX[nr], NODE_JY[nc] - NODE_JY[nr],
+                                          NODE_KL[nc] - NODE_KL[nr]);
+                            int e = nr + 8 * nc;
+                            A4(Auu, m, kr, ir, jr) += Kuu[e];
+                            A4(Auv, m, kr, ir, jr) += Kuv[e];
+                            A4(Avu, m, kr, ir, jr) += Kvu[e];
+                            A4(Avv, m, kr, ir, jr) += Kvv[e];
+                        }
+                    }
+                }
+            }
+    }
+    return status;
+#endif
     double Kuu[64], Kuv[64], Kvu[64], Kvv[64];
     for (int j = nh + 1; j <= ny - nh + 1; ++j)
         for (int i = nh + 1; i <= nx - nh + 1; ++i) {
@@ -754,6 +798,7 @@ void orc_matvec_3d(const orc_grid *g, const int32_t *active_vertex,
     size_t nv = (size_t)nz * (nx - 1) * (ny - 1);
     memset(yu, 0, nv * sizeof(double));
     memset(yv, 0, nv * sizeof(double));
+    ORC_PAR_FOR
     for (int j = g->jlo; j <= g->jhi; ++j)
         for (int i = g->ilo; i <= g->ihi; ++i) {
             if (!S2(active_vertex, i, j)) continue;
@@ -768,6 +813,7 @@ static double gsum2(const orc_grid *g, const double *w1, const double *w2)
 {
     const int nx = g->nx, nz = g->nz;
     double s = 0.0;
+    ORC_PAR_SUM_S
     for (int j = g->jlo; j <= g->jhi; ++j)
         for (int i = g->ilo; i <= g->ihi; ++i)
             for (int k = 1; k <= nz; ++k) s = s + V3(w1, k, i, j) + V3(w2, k, i, j);
@@ -818,9 +864,10 @@ static void sia_solve(const orc_grid *g, const int32_t *active_vertex, const dou
                       const double *r, double *z)
 {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    double sb[64], d[64], sp[64], rhs[64], soln[64];
+    ORC_PAR_FOR
     for (int j = 1; j <= ny - 1; ++j)
         for (int i = 1; i <= nx - 1; ++i) {
+            double sb[64], d[64], sp[64], rhs[64], soln[64];
             if (!S2(active_vertex, i, j)) continue;
             size_t base = (size_t)nz * ((i - 1) + (size_t)(nx - 1) * (j - 1));
             for (int k = 0; k < nz; ++k) {
@@ -941,16 +988,20 @@ int orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *av,
     pc_setup(g, precond, Auu, Avv, &pc);
     int status = ORC_OK, niters = maxiters;
     double err = 0.0;
+    ORC_PAR_FOR
     for (size_t q = 0; q < nv; ++q) { wu[q] = bu[q] * bu[q]; wv[q] = bv[q] * bv[q]; }
     double L2_rhs = sqrt(gsum2(g, wu, wv));
     orc_halo(g, nz, xu); orc_halo(g, nz, xv);
     orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
+    ORC_PAR_FOR
     for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
     orc_halo(g, nz, ru); orc_halo(g, nz, rv);
     pc_apply(g, precond, av, &pc, ru, rv, zu, zv);
+    ORC_PAR_FOR
     for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * zu[q]; wv[q] = rv[q] * zv[q]; }
     memcpy(du, zu, nv * 8); memcpy(dv, zv, nv * 8);
     orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, du, dv, qu, qv);
+    ORC_PAR_FOR
     for (size_t q = 0; q < nv; ++q) { w2u[q] = du[q] * qu[q]; w2v[q] = dv[q] * qv[q]; }
     double rho_old = gsum2(g, wu, wv);
     double sigma = gsum2(g, w2u, w2v);
@@ -958,14 +1009,16 @@ int orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *av,
     double alpha = rho_old / sigma;
     if (alpha != alpha) status = ORC_ERR_NAN;
     if (!status) {
-        for (size_t q = 0; q < nv; ++q) {
+        ORC_PAR_FOR
+    for (size_t q = 0; q < nv; ++q) {
             xu[q] = xu[q] + alpha * du[q]; xv[q] = xv[q] + alpha * dv[q];
             ru[q] = ru[q] - alpha * qu[q]; rv[q] = rv[q] - alpha * qv[q];
         }
         for (int n = 2; n <= maxiters; ++n) {
             pc_apply(g, precond, av, &pc, ru, rv, zu, zv);
             orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, zu, zv, Azu, Azv);
-            for (size_t q = 0; q < nv; ++q) {
+            ORC_PAR_FOR
+    for (size_t q = 0; q < nv; ++q) {
                 wu[q] = ru[q] * zu[q]; wv[q] = rv[q] * zv[q];
                 w2u[q] = Azu[q] * zu[q]; w2v[q] = Azv[q] * zv[q];
             }
@@ -976,7 +1029,8 @@ int orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *av,
             alpha = rho / sigma;
             rho_old = rho;
             if (alpha != alpha) { status = ORC_ERR_NAN; break; }
-            for (size_t q = 0; q < nv; ++q) {
+            ORC_PAR_FOR
+    for (size_t q = 0; q < nv; ++q) {
                 du[q] = zu[q] + beta * du[q]; dv[q] = zv[q] + beta * dv[q];
                 qu[q] = Azu[q] + beta * qu[q]; qv[q] = Azv[q] + beta * qv[q];
                 xu[q] = xu[q] + alpha * du[q]; xv[q] = xv[q] + alpha * dv[q];
@@ -984,8 +1038,10 @@ int orc_pcg_chrongear_3d(const orc_grid *g, const int32_t *av,
             }
             if (n % 5 == 0) {
                 orc_matvec_3d(g, av, Auu, Auv, Avu, Avv, xu, xv, zu, zv);
-                for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
-                for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * ru[q]; wv[q] = rv[q] * rv[q]; }
+                ORC_PAR_FOR
+    for (size_t q = 0; q < nv; ++q) { ru[q] = bu[q] - zu[q]; rv[q] = bv[q] - zv[q]; }
+                ORC_PAR_FOR
+    for (size_t q = 0; q < nv; ++q) { wu[q] = ru[q] * ru[q]; wv[q] = rv[q] * rv[q]; }
                 err = sqrt(gsum2(g, wu, wv)) / L2_rhs;
                 if (err < tolerance) { niters = n; break; }
                 orc_halo(g, nz, ru); orc_halo(g, nz, rv);

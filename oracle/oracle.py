@@ -14,22 +14,34 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
+# the same restatement with its hot loops on all host cores (OpenMP), for the all-core
+# CPU baseline only: it sums in another order and is never the checker
+LIB_OMP = os.path.join(HERE, "liborc_omp.so")
 
-_lib = None
+_libs = {}
+_all_cores = False
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def set_all_cores(on: bool):
+    """True: the functions below run the OpenMP build (timing only); False: the serial
+    oracle."""
+    global _all_cores
+    _all_cores = bool(on)
+
+
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+    path = LIB_OMP if _all_cores else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB)
-        _declare(_lib)
-    return _lib
+        L = C.CDLL(path)
+        _declare(L)
+        _libs[path] = L
+    return _libs[path]
 
 
 class Grid(C.Structure):

@@ -18,6 +18,10 @@ program fortran_driver
   real(dp), allocatable :: sigma(:), xVx(:,:), yVx(:,:), susrf(:,:), sthck(:,:), ff(:,:,:), efvs(:,:,:)
   real(dp), allocatable :: beta(:,:), lu(:,:,:), lv(:,:,:), ru(:,:,:), rv(:,:,:)
   real(dp) :: err, scal(4)
+  real(dp), allocatable :: thk(:,:), srf(:,:), bed(:,:), flw(:,:,:)
+  character(len=512) :: wfile
+  logical :: conv
+  integer :: nfb
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -80,6 +84,24 @@ program fortran_driver
           wapprox, 100, scal(3), xu, xv, efvs, bu, bv, ru, rv, n1, n2)
      scal = [real(n1, dp), real(n2, dp), 0.0_dp, 0.0_dp]
      write(11) xu, xv, scal
+  case (6)   ! solve from the geometry, fast path, optionally from the learned guess
+     ! hdr: precond, whichapprox, whichefvs, learned (0/1); third argument: weight file
+     allocate(sigma(nz), thk(nx,ny), srf(nx,ny), bed(nx,ny), flw(nz-1,nx,ny), beta(nx-1,ny-1))
+     allocate(efvs(nz-1,nx,ny))
+     read(10) sigma, thk, srf, bed, flw, beta, scal
+     ! scal: dx, thklim, efvs_constant, precision
+     if (hdr(8) /= 0) then
+        call get_command_argument(3, wfile)
+        call cism_gpu_set_learned_guess_checked(nx, ny, nz, trim(wfile))
+     endif
+     xu = 0.0_dp
+     xv = 0.0_dp
+     call glissade_velo_higher_solve_gpu(nx, ny, nz, scal(1), scal(1), sigma, thk, srf, bed, flw,  &
+          scal(2), 0.0_dp, 5, beta, wefvs, 3, 3, precond, wapprox, 100, scal(3), xu, xv, efvs,    &
+          n1, n2, precision=nint(scal(4)), adaptive_linear=1, inner_rtol=5.0e-2_dp, anderson_m=3, &
+          learned_guess=hdr(8) /= 0, converged=conv, fallbacks=nfb)
+     scal = [real(n1, dp), real(n2, dp), merge(1.0_dp, 0.0_dp, conv), real(nfb, dp)]
+     write(11) xu, xv, scal
   case default
      write(*,*) 'unknown mode', mode
      error stop 2
@@ -87,4 +109,12 @@ program fortran_driver
   close(10)
   close(11)
   call cism_gpu_velo_finalize()
+
+contains
+
+  subroutine cism_gpu_set_learned_guess_checked(nx, ny, nz, path)
+    integer, intent(in) :: nx, ny, nz
+    character(*), intent(in) :: path
+    call cism_gpu_velo_set_learned_guess(nx, ny, nz, path)
+  end subroutine cism_gpu_set_learned_guess_checked
 end program fortran_driver

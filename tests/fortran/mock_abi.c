@@ -145,3 +145,46 @@ int cism_velo_higher_solve_3d(cism_ctx *c, const cism_solve_opts *o, const doubl
     st->linear_iters = 100 * o->whichsparse + 10 * o->whichefvs + o->whichapprox;
     return CISM_OK;
 }
+
+static int guess_loaded = 0;
+
+/* reads the 8-byte magic of the weight file; records that a guess was loaded */
+int cism_learned_guess_load_file(cism_ctx *c, const char *path) {
+    (void)c;
+    FILE *f = fopen(path, "rb");
+    char magic[9] = "";
+    if (!f) return CISM_ERR_ARG;
+    size_t n = fread(magic, 1, 8, f);
+    fclose(f);
+    if (n != 8 || memcmp(magic, "CISMSG01", 8) != 0) return CISM_ERR_ARG;
+    guess_loaded = 1;
+    return CISM_OK;
+}
+
+/* uvel = thck(i, j) + flwa(1, i, j) + precision + 10 initial_guess; vvel = usrf(i, j) + topg(i, j)
+ * + beta_external(i, j) (0 without bp) + anderson_m; efvs = sigma(nz) + inner_rtol;
+ * stats: picard = whichbabc + 100 adaptive_linear, linear = lateral_bc + 10 whichassemble_taud,
+ * converged = 1, fallbacks = 100 thklim (vertex (i, j) of a node is its cell (i, j)) */
+int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opts *o, const cism_velo_setup *su,
+                                    const double *sigma, const double *thck, const double *usrf,
+                                    const double *topg, const double *flwa, const cism_basal_physics *bp,
+                                    const double *bext, const double *N, double *uvel, double *vvel,
+                                    double *efvs, cism_solve_stats *st) {
+    (void)N;
+    if (su->initial_guess == 1 && !guess_loaded) return CISM_ERR_ARG;
+    const long ne = (long)(c->nz - 1) * c->nx * c->ny;
+    for (long q = 0; q < nodes(c); ++q) {
+        const long col = q / c->nz;
+        const long i = col % (c->nx - 1), j = col / (c->nx - 1);
+        const long cell = i + (long)c->nx * j;
+        uvel[q] = thck[cell] + flwa[(long)(c->nz - 1) * cell] + o->precision + 10.0 * su->initial_guess;
+        vvel[q] = usrf[cell] + topg[cell] + (bp ? bext[col] : 0.0) + o->anderson_m;
+    }
+    for (long e = 0; e < ne; ++e) efvs[e] = sigma[c->nz - 1] + o->inner_rtol;
+    memset(st, 0, sizeof *st);
+    st->picard_iters = o->whichbabc + 100 * o->adaptive_linear;
+    st->linear_iters = su->lateral_bc + 10 * su->whichassemble_taud;
+    st->converged = 1;
+    st->fallbacks = (int)(100.0 * su->thklim);
+    return CISM_OK;
+}

@@ -119,3 +119,32 @@ def test_picard_marshalling(driver):
     np.testing.assert_array_equal(v, lv + vm + 10.0 * av[None] + dx / 1000.0)
     assert sc[0] == 100 + 5          # maxiter_nonlinear + whichbabc
     assert sc[1] == 300 + 20 + 2     # 100 whichsparse + 10 whichefvs + whichapprox
+
+
+def test_geometry_solve_marshalling(driver, tmp_path):
+    """glissade_velo_higher_solve_gpu: the cell fields, beta_external and the fast path's
+    optional arguments reach cism_glissade_velo_higher_solve; the weight file is loaded
+    first (mode 6 of the driver: precision from scal(4), adaptive 1, inner_rtol 5e-2,
+    Anderson 3, learned guess from header word 8)"""
+    sigma = np.linspace(0, 1, NZ)
+    thk, srf, bed = rnd((NX, NY)), rnd((NX, NY)), rnd((NX, NY))
+    flw, bext = rnd(C), rnd((NX - 1, NY - 1))
+    wfile = tmp_path / "w.bin"
+    wfile.write_bytes(b"CISMSG01" + bytes(24))
+    fin, fout = driver / "in6.bin", driver / "out6.bin"
+    with open(fin, "wb") as f:
+        f.write(np.asarray([6, NX, NY, NZ, 2, 2, 2, 1], dtype=np.int32).tobytes())
+        for a in (sigma, thk, srf, bed, flw, bext, np.array([1000.0, 0.25, 5.0, 1.0])):
+            f.write(np.asfortranarray(a).tobytes(order="F"))
+    r = subprocess.run([str(driver / "drv"), str(fin), str(fout), str(wfile)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = np.fromfile(fout, dtype=np.float64)
+    n = int(np.prod(V))
+    u, v, sc = data[:n].reshape(V, order="F"), data[n:2 * n].reshape(V, order="F"), data[2 * n:]
+    cell = (slice(0, NX - 1), slice(0, NY - 1))
+    np.testing.assert_array_equal(u, np.broadcast_to((thk[cell] + flw[0][cell])[None] + 1.0 + 10.0, V))
+    np.testing.assert_array_equal(v, np.broadcast_to((srf[cell] + bed[cell] + bext)[None] + 3.0, V))
+    # picard = whichbabc 5 + 100 adaptive; linear = lateral 1 + 10 local taud; converged;
+    # fallbacks = 100 thklim
+    assert list(sc) == [105.0, 11.0, 1.0, 25.0]

@@ -97,3 +97,49 @@ def test_fortran_picard(prob, tmp_path):
                np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()]))
     print(f"  Fortran-driven Picard: {int(sc[0])} iterations, rel L2 diff {e:.2e}")
     assert e < 1e-8
+
+
+def test_fortran_geometry_solve_with_the_learned_guess(tmp_path):
+    """glissade_velo_higher_solve_gpu from a Fortran program: geometry in, the fast path
+    (precision 1, inexact Picard, Anderson) from the learned guess read from a weight file,
+    under the reference's stopping rule.  Same result as the same call from Python, and fewer
+    Picard iterations than from zero."""
+    from cism_amd import gpu as G, surrogate as SG
+    w = SG.default_weights("grf")
+    if not os.path.exists(w):
+        pytest.skip(f"no trained weights at {w}")
+    wbin = tmp_path / "grf.bin"
+    SG.export_weights_bin(w, str(wbin))
+    p = configs.grf_sheet(ewn=256, nsn=256, nz=11, dx=1000.0, seed=3100)
+    vshape = (p.nz, p.nx - 1, p.ny - 1)
+    bext = p.extra["beta_external"]
+    runs = {}
+    for learned in (0, 1):
+        fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(np.asarray([6, p.nx, p.ny, p.nz, p.whichprecond, p.whichapprox, p.whichefvs, learned],
+                               dtype=np.int32).tobytes())
+            for a in (p.sigma, p.thck, p.usrf, p.topg, p.flwa, bext,
+                      np.array([p.dx, p.thklim, p.efvs_constant, 1.0])):
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        print(f"[fortran] mode 6 learned={learned} grid {p.nx}x{p.ny}x{p.nz}", flush=True)
+        r = subprocess.run([DRIVER, str(fin), str(fout), str(wbin)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        data = np.fromfile(fout, dtype=np.float64)
+        n = int(np.prod(vshape))
+        runs[learned] = (data[:n].reshape(vshape, order="F"), data[n:2 * n].reshape(vshape, order="F"),
+                         data[2 * n:])
+    for learned, (u, v, sc) in runs.items():
+        print(f"  learned={learned}: {int(sc[0])} Picard, converged {int(sc[2])}, fallbacks {int(sc[3])}")
+        assert sc[2] == 1.0 and sc[3] == 0.0
+    assert runs[1][2][0] < runs[0][2][0]
+    params, meta = SG.load_weights(w)
+    with G.Context.for_problem(p) as ctx:
+        ctx.learned_guess_load(params, meta["nz"], 0, meta["uscale"])
+        rp = ctx.glissade_velo_higher_solve(p, bp=G.BasalPhysics(configs.HO_BABC_BETA_EXTERNAL),
+                                            beta_external=bext, initial_guess=1, precision=1,
+                                            adaptive_linear=1, inner_rtol=5e-2, anderson_m=3)
+    u, v, sc = runs[1]
+    assert rp["picard_iters"] == int(sc[0])
+    assert np.array_equal(rp["uvel"], u) and np.array_equal(rp["vvel"], v)

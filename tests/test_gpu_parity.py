@@ -50,12 +50,14 @@ _TIGHT = {}
 
 
 def oracle_fixed_point(p):
-    if p.name + str(p.ewn) not in _TIGHT:
+    import hashlib
+    key = hashlib.sha256(p.thck.tobytes() + p.beta_internal.tobytes() + bytes([p.nz])).hexdigest()
+    if key not in _TIGHT:
         lu, lv = O.loads(p)
         r = O.picard(p, lu, lv, maxiter=300, linear_maxiters=1000, linear_tolerance=1e-10, target=1e-6)
         assert r["status"] == 0 and r["converged"]
-        _TIGHT[p.name + str(p.ewn)] = r
-    return _TIGHT[p.name + str(p.ewn)]
+        _TIGHT[key] = r
+    return _TIGHT[key]
 
 
 @pytest.fixture(scope="module")

@@ -160,3 +160,30 @@ def test_velocity_residual_closed_forms():
     assert O.residual_velocity(p, 1, u, v, us, vs) == 0.25
     nown = p.ewn * p.nsn * (nz - 1) - 1          # owned non-bed nodes with a nonzero speed
     assert O.residual_velocity(p, 2, u, v, us, vs) == pytest.approx((0.25 + c5) / nown, rel=1e-15)
+
+
+def test_all_core_build_computes_the_same_system():
+    """the OpenMP build (the all-core CPU baseline) assembles the same matrix up to the order
+    of the scatter's sums and its Chron-Gear iterates agree to rounding"""
+    from helpers import rel_max
+    p = configs.grf_sheet(ewn=20, nsn=18, nz=6, dx=2000.0, corr=8000.0, seed=3)
+    rng = np.random.default_rng(2)
+    u = np.asfortranarray(20 * rng.standard_normal(p.uvel.shape))
+    v = np.asfortranarray(20 * rng.standard_normal(p.uvel.shape))
+    st, A1, e1 = O.assemble(p, u, v)
+    O.set_all_cores(True)
+    try:
+        st2, A2, e2 = O.assemble(p, u, v)
+        assert st2 == 0 and np.array_equal(e1, e2)
+        for a, b in zip(A2, A1):
+            assert rel_max(a, b) < 1e-14
+        A, bu, bv, _, _, _ = assembled_system(p, seed=5)
+        x2 = [np.zeros(p.uvel.shape, order="F") for _ in range(2)]
+        st2, err2, n2 = O.pcg(p, A, bu, bv, *x2, 2, chrongear=True, maxiters=30)
+    finally:
+        O.set_all_cores(False)
+    x1 = [np.zeros(p.uvel.shape, order="F") for _ in range(2)]
+    st1, err1, n1 = O.pcg(p, A, bu, bv, *x1, 2, chrongear=True, maxiters=30)
+    assert st1 == st2 == 0 and n1 == n2
+    d = np.linalg.norm(np.concatenate([(a - b).ravel() for a, b in zip(x2, x1)]))
+    assert d <= 1e-11 * np.linalg.norm(np.concatenate([a.ravel() for a in x1]))
