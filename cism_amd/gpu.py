@@ -40,6 +40,7 @@ EXPORTED = [
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
     "cism_surrogate_last_forward_ms", "cism_glissade_velo_higher_solve", "cism_learned_guess_load",
     "cism_learned_guess_load_file", "cism_learned_guess", "cism_learned_guess_last_ms",
+    "cism_compute_residual_vector_3d_noalloc",
 ]
 
 

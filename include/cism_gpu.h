@@ -133,6 +133,23 @@ int  cism_compute_residual_vector_3d(cism_ctx *ctx, const int32_t *active_vertex
                                      double *resid_u, double *resid_v,
                                      double *L2_norm, double *L2_norm_relative);
 
+/* No counterpart in the reference: compute_residual_vector_3d (the entry point above) on
+ * caller-owned DEVICE arrays, allocating nothing -- for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that must not call hipMalloc).  The
+ * arrays are device pointers in the layouts above; norms (device, 2 doubles) receives
+ * L2_norm and L2_norm_relative.  Enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream) and not synchronized.  The only scratch is a fixed per-GPU array of 2 x 1024
+ * partial sums (16 KiB) inside the library; grids above 2^30 nodes are rejected
+ * (CISM_ERR_ARG).  Calls that overlap on one GPU (other streams or threads) race on that
+ * scratch.  periodic as in cism_ctx_create; device = HIP device ordinal. */
+int  cism_compute_residual_vector_3d_noalloc(int nx, int ny, int nz, int periodic, int device,
+                                             void *stream, const int32_t *active_vertex,
+                                             const double *Auu, const double *Auv,
+                                             const double *Avu, const double *Avv,
+                                             const double *bu, const double *bv,
+                                             const double *uvel, const double *vvel,
+                                             double *resid_u, double *resid_v, double *norms);
+
 /* compute_residual_velocity_3d (glissade_velo_higher.F90:9111-9234) */
 int  cism_compute_residual_velocity_3d(cism_ctx *ctx, int whichresid,
                                        const double *uvel, const double *vvel,

@@ -33,6 +33,23 @@ def test_library_exports_every_declared_symbol():
     assert set(gpu.EXPORTED) == set(names)
 
 
+def test_noalloc_entry_point_rejects_bad_sizes_before_any_gpu_call():
+    """the no-allocation residual entry point checks its grid against its fixed scratch and
+    limits before it touches the GPU: these calls return CISM_ERR_ARG on a host without one"""
+    lib_path = os.path.join(ROOT, "cism_amd", "libcism_gpu.so")
+    if not os.path.exists(lib_path):
+        pytest.fail("libcism_gpu.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(lib_path)
+    f = lib.cism_compute_residual_vector_3d_noalloc
+    f.argtypes = [ctypes.c_int] * 5 + [ctypes.c_void_p] * 13
+    lib.cism_last_error.restype = ctypes.c_char_p
+    dummy = [ctypes.c_void_p(8)] * 12
+    assert f(40000, 40000, 11, 1, 0, None, *dummy) == 5      # 1.8e10 nodes > 2^30
+    assert b"maximum" in lib.cism_last_error()
+    assert f(5, 40, 11, 1, 0, None, *dummy) == 5             # nx < 7
+    assert f(40, 40, 11, 1, 0, None, *([None] * 12)) == 5   # null arrays
+
+
 def test_fortran_shim_exports():
     path = os.path.join(ROOT, "cism_amd", "libcism_fortran.so")
     if not os.path.exists(path):
