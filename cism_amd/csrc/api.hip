@@ -714,9 +714,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             lin_tol = fmin(1.0e-2, fmax(lin_tol, gam * L2rel));
         }
         if (prec == 1) {
-            int n64 = 0, n32 = 0, stalled = 0;
+            int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
             st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
-                              &n64, &n32, 1, &stalled);
+                              &n64, &n32, 1, &stalled, &unconv);
+            S.unconverged_linear += unconv;
             S.fp64_matvecs += n64;
             S.fp32_matvecs += n32;
             S.refinement_steps += n64;
@@ -735,6 +736,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         } else {
             st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err,
                               &niters);
+            S.unconverged_linear += !st && !(err < lin_tol);
         }
         if (st) return st;
         t_lin += now_s() - tl;

@@ -325,10 +325,11 @@ int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double t
                  double *err, int *niters);
 // r_ready: c->r already holds b - A x with sums[2] = |r|^2 and (on the host too)
 // sums[3] = |b|^2, as residual_vector_device leaves them with rs = c->r
-// stalled (optional): set to 1 when the refinement is given up (refine_policy.h)
+// stalled (optional): set to 1 when the refinement is given up (refine_policy.h);
+// unconverged (optional): set to 1 when its budget ran out before the tolerance
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
                  double *err, int *niters, int *n64, int *n32, int r_ready = 0,
-                 int *stalled = nullptr);
+                 int *stalled = nullptr, int *unconverged = nullptr);
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);

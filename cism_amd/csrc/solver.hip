@@ -1086,8 +1086,10 @@ __global__ void k_save_scale(DevScal *sc, int from) { sc->sums[2] = sqrt(sc->sum
 // first residual, its norm and |b| come from the Picard residual pass.  *stalled is set
 // when the refinement is given up (refine_policy.h): the caller then falls back to the
 // FP64 solver; an FP32 breakdown (alpha = NaN) is such a stall, not an error.
+// *unconverged is set when the budget ran out first (an inexact solve, as the
+// reference's PCG ends at maxiters).
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol, double *err_out,
-                 int *niters_out, int *n64, int *n32, int r_ready, int *stalled) {
+                 int *niters_out, int *n64, int *n32, int r_ready, int *stalled, int *unconverged) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn);
     const long long n2 = 2 * g.npad;
@@ -1108,7 +1110,9 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
                        c->cscale_on ? c->cscale : nullptr};
     RefineMonitor mon;
     refine_monitor_init(&mon);
+    int inner_conv = 1;
     if (stalled) *stalled = 0;
+    if (unconverged) *unconverged = 0;
     while (true) {
         double rr;
         if (r_ready && outer == 0) {
@@ -1125,10 +1129,14 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         }
         ++*n64;
         err = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
-        const int verdict = refine_decide(&mon, err, tol, outer, total, maxiters);
+        const int verdict = refine_decide(&mon, err, tol, outer, total, maxiters, inner_conv);
         if (verdict == REFINE_CONVERGED) break;
         if (verdict == REFINE_STALLED) {
             if (stalled) *stalled = 1;
+            break;
+        }
+        if (verdict == REFINE_EXHAUSTED) {
+            if (unconverged) *unconverged = 1;
             break;
         }
         hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->r, c->sc,
@@ -1145,6 +1153,7 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
             break;
         }
         CISM_TRY(st_in);
+        inner_conv = it < maxiters;
         hipLaunchKernelGGL(k_refine, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->x32, c->sc,
                            c->x);
         HIP_TRY(hipGetLastError());

@@ -54,7 +54,7 @@ module cism_gpu_velo
      real(c_double) :: seconds_total
      integer(c_int) :: status, refinement_steps
      real(c_double) :: final_rel_err, seconds_device
-     integer(c_int) :: converged, fallbacks, guess_rejected, reserved
+     integer(c_int) :: converged, fallbacks, guess_rejected, unconverged_linear
   end type cism_solve_stats
 
   type, bind(C) :: cism_basal_physics

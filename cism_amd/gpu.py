@@ -90,7 +90,7 @@ class SolveStats(C.Structure):
                 ("status", C.c_int), ("refinement_steps", C.c_int),
                 ("final_rel_err", C.c_double), ("seconds_device", C.c_double),
                 ("converged", C.c_int), ("fallbacks", C.c_int), ("guess_rejected", C.c_int),
-                ("reserved", C.c_int)]
+                ("unconverged_linear", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

@@ -234,7 +234,8 @@ typedef struct {
     int    guess_rejected;     /* precision 1: 1 if the initial guess was no closer than zero
                                   to the solution of the first Picard system in its energy
                                   norm (x.A x / 2 - b.x > 0) and was replaced by zero         */
-    int    reserved;
+    int    unconverged_linear; /* linear solves that ended on their iteration budget above
+                                  their tolerance (the reference's niters = maxiters case)   */
 } cism_solve_stats;
 
 void cism_solve_opts_default(cism_solve_opts *o);

@@ -11,18 +11,19 @@ HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
                    "refine_policy.h")
 SRC = r"""
 #include "%s"
-extern "C" int run(const double *errs, int n, double tol, int inner_step, int maxiters, int *steps) {
+extern "C" int run(const double *errs, const int *conv, int n, double tol, int inner_step, int maxiters,
+                   int *steps) {
     RefineMonitor m;
     refine_monitor_init(&m);
     for (int t = 0; t < n; ++t) {
-        const int v = refine_decide(&m, errs[t], tol, t, t * inner_step, maxiters);
+        const int v = refine_decide(&m, errs[t], tol, t, t * inner_step, maxiters, conv[t]);
         if (v != REFINE_CONTINUE) { *steps = t; return v; }
     }
     *steps = n;
     return REFINE_CONTINUE;
 }
 """
-CONTINUE, CONVERGED, STALLED = 0, 1, 2
+CONTINUE, CONVERGED, STALLED, EXHAUSTED = 0, 1, 2, 3
 
 
 @pytest.fixture(scope="module")
@@ -32,12 +33,16 @@ def policy(tmp_path_factory):
     src.write_text(SRC % HDR)
     subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", str(src), "-o", str(so)], check=True)
     L = C.CDLL(str(so))
-    L.run.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.run.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int, C.c_double, C.c_int, C.c_int,
+                      C.POINTER(C.c_int)]
 
-    def run(errs, tol=1e-8, inner_step=10, maxiters=200):
+    def run(errs, tol=1e-8, inner_step=10, maxiters=200, conv=None):
+        """conv[t]: whether the inner solve before residual t reached its tolerance"""
+        conv = [1] * len(errs) if conv is None else conv
         a = (C.c_double * len(errs))(*errs)
+        c = (C.c_int * len(errs))(*conv)
         steps = C.c_int()
-        v = L.run(a, len(errs), tol, inner_step, maxiters, C.byref(steps))
+        v = L.run(a, c, len(errs), tol, inner_step, maxiters, C.byref(steps))
         return v, steps.value
     return run
 
@@ -76,10 +81,20 @@ def test_non_finite_residual_is_a_stall(policy):
     assert policy([1e-2, float("inf")]) == (STALLED, 1)
 
 
-def test_budgets(policy):
+def test_budgets_end_unconverged(policy):
     # 50 outer steps
     v, n = policy([0.75 ** t for t in range(80)], tol=1e-30, inner_step=1)
-    assert (v, n) == (STALLED, 50)
+    assert (v, n) == (EXHAUSTED, 50)
     # 4 x maxiters inner iterations: reached at outer step 8 with 100 per step, maxiters 200
     v, n = policy([0.5 ** t for t in range(20)], tol=1e-30, inner_step=100, maxiters=200)
-    assert (v, n) == (STALLED, 8)
+    assert (v, n) == (EXHAUSTED, 8)
+
+
+def test_slow_inner_solves_are_not_a_precision_floor(policy):
+    # inner solves ending on their iteration count with little progress: the window
+    # restarts after each of them, and the refinement runs to its budget instead
+    errs = [1.0, 0.95, 0.9, 0.86, 0.82, 0.78, 0.74, 0.7, 0.66]
+    conv = [1] + [0] * (len(errs) - 1)
+    assert policy(errs, tol=1e-8, inner_step=200, maxiters=200, conv=conv) == (EXHAUSTED, 4)
+    # the same residuals after converged inner solves are a stall
+    assert policy(errs, tol=1e-8, inner_step=1, maxiters=200) == (STALLED, 3)
