@@ -72,7 +72,8 @@ def build_lib(force=True, jobs=4):
         if p.wait() != 0:
             raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -414,7 +414,8 @@ extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
                                         int whichassemble_beta, double *Auu, double *Avv) {
     CTX_BEGIN(c);
     if (whichassemble_beta != 1) {
-        cism_set_error("only the local basal assembly (which_ho_assemble_beta = 1) is implemented");
+        cism_set_error("cism_basal_sliding_bc_3d is the local assembly (which_ho_assemble_beta = 1); "
+                       "cism_basal_sliding_bc takes the standard one");
         return CISM_ERR_UNSUPPORTED;
     }
     const size_t ncol = (size_t)c->g.sx * c->g.sy;
@@ -427,6 +428,34 @@ extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
     hipLaunchKernelGGL(k_basal_local, dim3(nblocks_for((long long)ncol)), dim3(256), 0, c->stream,
                        c->g, c->av, c->beta, dx * dy / CISM_VOL0, c->A);
     HIP_TRY(hipGetLastError());
+    CISM_TRY(download_block(c, 0, Auu));
+    CISM_TRY(download_block(c, 3, Avv));
+    return CISM_OK;
+}
+
+// basal_sliding_bc (velo_higher:8125-8381) with the reference's arguments, both assemblies,
+// on the basal layer of the full 3D blocks (standard: velo_higher:8236-8363)
+extern "C" int cism_basal_sliding_bc(cism_ctx *c, double dx, double dy, const int32_t *active_cell,
+                                     const int32_t *active_vertex, const double *beta,
+                                     const double *xVertex, const double *yVertex,
+                                     int whichassemble_beta, double *Auu, double *Avv) {
+    CTX_BEGIN(c);
+    if (whichassemble_beta == 1) return cism_basal_sliding_bc_3d(c, dx, dy, active_vertex, beta, 1, Auu, Avv);
+    if (whichassemble_beta != 0) {
+        cism_set_error("whichassemble_beta must be 0 (standard) or 1 (local)");
+        return CISM_ERR_ARG;
+    }
+    const GridDev &g = c->g;
+    const size_t ncol = (size_t)g.sx * g.sy;
+    CISM_TRY(up(c, c->ac, active_cell, (size_t)g.nx * g.ny));
+    CISM_TRY(up(c, c->beta, beta, ncol));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(ensure_stage(c));
+    CISM_TRY(upload_block(c, 0, Auu));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    CISM_TRY(upload_block(c, 3, Avv));
+    CISM_TRY(basal_standard_device(c));
     CISM_TRY(download_block(c, 0, Auu));
     CISM_TRY(download_block(c, 3, Avv));
     return CISM_OK;
@@ -598,6 +627,7 @@ extern "C" void cism_solve_opts_default(cism_solve_opts *o) {
     o->inner_rtol = 1.0e-3;
     o->adaptive_gamma = 0.05;
     o->anderson_m = 0;
+    o->whichassemble_beta = 1;
 }
 
 static double now_s() {
@@ -612,6 +642,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                          double *vvel, double *efvs, double *bu, double *bv, double *resid_u,
                          double *resid_v, cism_solve_stats *stats, double t_dev0 = 0.0) {
     const GridDev &g = c->g;
+    if (o->whichassemble_beta != 0 && o->whichassemble_beta != 1) {
+        cism_set_error("whichassemble_beta must be 0 (standard) or 1 (local)");
+        return CISM_ERR_ARG;
+    }
     if (o->anderson_m < 0 || o->anderson_m > AA_MAX) {
         cism_set_error("anderson_m must be 0..%d", AA_MAX);
         return CISM_ERR_ARG;
@@ -631,26 +665,42 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
     // whose Picard residual exceeds the zero guess's (|b|) and starts from zero instead
     int prec = o->precision;
     bool guess_checked = prec != 1;
+    TimerRange r_loop("glissade_vhs_nonlinear_loop");
     while (crit >= target && counter < o->maxiter_nonlinear) {
         counter++;
         HIP_TRY(hipMemcpyAsync(c->usav, c->x, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                c->stream));
         if (c->bp_on) CISM_TRY(calcbeta_device(c));   // velo_higher:2502-2517
         double ta = now_s();
-        if (dim2)
-            CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, o->whichbabc != 6, o->dx, o->dy));
-        else
-            CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
-                                     o->whichbabc != 6, o->dx, o->dy));
-        HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
-                               c->stream));
-        CISM_TRY(dirichlet_device(c));
-        CISM_TRY(launch_halo_mat(c, c->A));
-        CISM_TRY(launch_halo_vec(c, c->b));
-        // the mixed-precision path gets its FP32 symmetric matrix from the same pass
-        if (prec == 1) CISM_TRY(ensure_a32(c));
-        CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
-        CISM_TRY(sync_scalars(c));
+        {
+            // the ranges end at the synchronization below (sync_scalars)
+            TimerRange r_asm(dim2 ? "glissade_assemble_2d" : "glissade_assemble_3d");
+            // basal friction (velo_higher:3039-3052): the local assembly inside the gather, the
+            // standard one after it
+            const bool basal = o->whichbabc != 6, standard = o->whichassemble_beta == 0;
+            if (dim2)
+                CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx, o->dy));
+            else
+                CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                         basal && !standard, o->dx, o->dy));
+            if (basal && standard) CISM_TRY(basal_standard_device(c));
+            HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            {
+                TimerRange r_d(dim2 ? "glissade_dirichlet_2d" : "glissade_dirichlet_3d");
+                CISM_TRY(dirichlet_device(c));
+            }
+            {
+                TimerRange r_h("glissade_halo_Axxs");
+                CISM_TRY(launch_halo_mat(c, c->A));
+                CISM_TRY(launch_halo_vec(c, c->b));
+            }
+            // the mixed-precision path gets its FP32 symmetric matrix from the same pass
+            if (prec == 1) CISM_TRY(ensure_a32(c));
+            TimerRange r_s("glissade_chk_symmetry");
+            CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
+            CISM_TRY(sync_scalars(c));
+        }
         if (c->sc_host->singular_flag) {
             cism_set_error("Jacobian matrix is singular");
             return CISM_ERR_SINGULAR;
@@ -668,9 +718,12 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         }
         // the mixed path takes the first residual of its refinement from this pass
         const bool share_r = prec == 1;
-        if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
-        CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));
-        CISM_TRY(sync_scalars(c));
+        {
+            TimerRange r_r("glissade_resid_vec");
+            if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
+            CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));
+            CISM_TRY(sync_scalars(c));
+        }
         L2 = sqrt(c->sc_host->sums[0]);
         const double rhs = sqrt(c->sc_host->sums[1]);
         L2rel = rhs > 0.0 ? L2 / rhs : 0.0;
@@ -713,6 +766,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             const double gam = o->adaptive_gamma > 0 ? o->adaptive_gamma : 0.05;
             lin_tol = fmin(1.0e-2, fmax(lin_tol, gam * L2rel));
         }
+        roctxRangePushA("glissade_pcg_slv_struct");   // popped once the linear solve has ended
         if (prec == 1) {
             int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
             st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
@@ -738,6 +792,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                               &niters);
             S.unconverged_linear += !st && !(err < lin_tol);
         }
+        roctxRangePop();
         if (st) return st;
         t_lin += now_s() - tl;
         lin_total += niters;
@@ -907,6 +962,7 @@ extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opt
     if (su->initial_guess == 0) CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double t_dev0 = now_s();   // inputs resident from here on
+    TimerRange r_init("glissade_vhs_init");
     CISM_TRY(velo_inputs_compute(c, su->dx, su->dy, su->thklim, su->eus, su->max_slope));
     CISM_TRY(mask_planes_device(c));
     const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
@@ -918,8 +974,14 @@ extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opt
     }
     // gravity and lateral ice-ocean loads (velo_higher:2033-2111)
     HIP_TRY(hipMemsetAsync(c->load, 0, sizeof(double) * 2 * g.npad, c->stream));
-    CISM_TRY(load_gravity_device(c, su->whichassemble_taud, su->dx, su->dy));
-    if (su->lateral_bc) CISM_TRY(load_lateral_device(c, c->planes + 2 * ncell, c->planes + 4 * ncell));
+    {
+        TimerRange r_g("glissade_load_vector_gravity");
+        CISM_TRY(load_gravity_device(c, su->whichassemble_taud, su->dx, su->dy));
+    }
+    if (su->lateral_bc) {
+        TimerRange r_l("glissade_load_vector_lateral_bc");
+        CISM_TRY(load_lateral_device(c, c->planes + 2 * ncell, c->planes + 4 * ncell));
+    }
     hipLaunchKernelGGL(k_noslip_masks, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, noslip ? 1 : 0,
                        c->umask, c->vmask);
     HIP_TRY(hipGetLastError());

@@ -562,6 +562,100 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
     }
 }
 
+// basal_sliding_bc with HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363) in gather
+// form, after the assembly: one lane per basal-layer node and horizontal slot (iA, jA, 0)
+// adds, onto the assembled Auu and Avv coefficients, the terms
+//   K = beta_qp w_p detJ / vol0 phi_r(p) phi_c(p)
+// of every active cell that holds the node (corner r) and its neighbour (corner c): cells
+// in the reference's loop order (j, i ascending), quadrature points in order, one term at
+// a time as the reference accumulates them.  detJ comes from the 2D Jacobian of the cell's
+// corners (get_basis_function_derivatives_2d, velo_higher:6583-6751), beta_qp is the
+// bilinear interpolation of the corners' beta; Kvv = Kuu.
+__global__ __launch_bounds__(256) void k_basal_standard(GridDev g, const int32_t *__restrict__ ac,
+                                                        const double *__restrict__ xV,
+                                                        const double *__restrict__ yV,
+                                                        const double *__restrict__ beta,
+                                                        double *__restrict__ A) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncol = (long long)g.sx * g.sy;
+    if (t >= 9 * ncol) return;
+    const int s = (int)(t / ncol);
+    const long long col = t - s * ncol;
+    const int iA = s % 3 - 1, jA = s / 3 - 1;
+    const int ii = (int)(col % g.sx) + 1, jj = (int)(col / g.sx) + 1;   // 1-based vertex
+    // corners SW, SE, NE, NW of a cell relative to its NE vertex (ishift(3, n), jshift(3, n))
+    const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+    const double r3 = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r3, r3, r3, -r3}, yq[4] = {-r3, -r3, r3, r3};
+    const long long q = (long long)(g.nz - 1) + (long long)g.nz * col;   // basal-layer node
+    const int m = slot_of(iA, jA, 0);
+    double au = 0.0, av = 0.0;
+    bool any = false;
+    // the cells with this vertex as a corner, in (j, i) order; the vertex is their NE, NW,
+    // SE and SW corner respectively
+#pragma unroll 1
+    for (int cc = 0; cc < 4; ++cc) {
+        const int ci = ii + (cc & 1), cj = jj + (cc >> 1);
+        const int nr = cc == 0 ? 2 : cc == 1 ? 3 : cc == 2 ? 1 : 0;
+        if (ci < CISM_NHALO + 1 || ci > g.nx - CISM_NHALO + 1 || cj < CISM_NHALO + 1 ||
+            cj > g.ny - CISM_NHALO + 1 || !ac[(ci - 1) + (long long)g.nx * (cj - 1)])
+            continue;
+        int nc = -1;   // the neighbour (ii + iA, jj + jA) as a corner of this cell
+        for (int n = 0; n < 4; ++n)
+            if (ci + di[n] == ii + iA && cj + dj[n] == jj + jA) nc = n;
+        if (nc < 0) continue;
+        double x[4], y[4], b[4];
+        for (int n = 0; n < 4; ++n) {
+            const long long v = (ci + di[n] - 1) + (long long)g.sx * (cj + dj[n] - 1);
+            x[n] = xV[v];
+            y[n] = yV[v];
+            b[n] = beta[v];
+        }
+        if (!any) {
+            au = A[aidx(0, m, q)];
+            av = A[aidx(3, m, q)];
+            any = true;
+        }
+        for (int p = 0; p < 4; ++p) {
+            double phi[4], dxr[4], dyr[4];
+            phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+            phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+            phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+            phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+            dxr[0] = -(1.0 - yq[p]) / 4.0; dxr[1] = (1.0 - yq[p]) / 4.0;
+            dxr[2] = (1.0 + yq[p]) / 4.0;  dxr[3] = -(1.0 + yq[p]) / 4.0;
+            dyr[0] = -(1.0 - xq[p]) / 4.0; dyr[1] = -(1.0 + xq[p]) / 4.0;
+            dyr[2] = (1.0 + xq[p]) / 4.0;  dyr[3] = (1.0 - xq[p]) / 4.0;
+            double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+            for (int n = 0; n < 4; ++n) {
+                J11 = J11 + dxr[n] * x[n];
+                J12 = J12 + dxr[n] * y[n];
+                J21 = J21 + dyr[n] * x[n];
+                J22 = J22 + dyr[n] * y[n];
+            }
+            const double detJ = J11 * J22 - J12 * J21;
+            double bq = 0.0;
+            for (int n = 0; n < 4; ++n) bq = bq + phi[n] * b[n];
+            const double K = 0.0 + bq * 1.0 * detJ / CISM_VOL0 * phi[nr] * phi[nc];
+            au = au + K;
+            av = av + K;
+        }
+    }
+    if (any) {
+        A[aidx(0, m, q)] = au;
+        A[aidx(3, m, q)] = av;
+    }
+}
+
+int basal_standard_device(cism_ctx *c) {
+    const GridDev &g = c->g;
+    const long long n = 9LL * g.sx * g.sy;
+    hipLaunchKernelGGL(k_basal_standard, dim3(nblocks_for(n)), dim3(256), 0, c->stream, g, c->ac, c->xV,
+                       c->yV, c->beta, c->A);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy) {
     CISM_TRY(ensure_ke(c));

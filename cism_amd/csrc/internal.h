@@ -19,6 +19,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -123,6 +124,16 @@ struct cism_ctx {
     // cell masks of cism_glissade_velo_higher_solve as int32 planes (ice, floating,
     // ocean, land, calving front = 0), 5 * nx * ny, lazily
     int32_t *planes;
+};
+
+// A named host-side range for rocprofv3 --marker-trace, with the reference's timer names
+// (t_startf/t_stopf regions of glissade_velo_higher.F90 and glissade_velo_higher_pcg.F90).
+// A range that ends at a synchronization covers the device work it enqueued.
+struct TimerRange {
+    explicit TimerRange(const char *name) { roctxRangePushA(name); }
+    ~TimerRange() { roctxRangePop(); }
+    TimerRange(const TimerRange &) = delete;
+    TimerRange &operator=(const TimerRange &) = delete;
 };
 
 // ---------------------------------------------------------------------------
@@ -334,6 +345,9 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);
 int dirichlet_device(cism_ctx *c);
+// basal friction by the standard finite-element assembly onto the assembled c->A (c->ac,
+// xV, yV, beta); the local assembly is part of the gather (assemble_device's add_basal)
+int basal_standard_device(cism_ctx *c);
 int symmetry_device(cism_ctx *c, float *As);
 int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
 // r = A x - b on owned active rows: sums[0] = |r|^2, sums[1] = |b|^2 over them.  With

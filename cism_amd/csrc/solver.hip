@@ -940,7 +940,10 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     // an inner solve (check = 1) whose residual becomes exactly zero has converged; the
     // reference recurrences would divide 0 by 0 there
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc, check);
-    if (precond == 2) CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
+    if (precond == 2) {
+        TimerRange r_p("pcg_precond_init");
+        CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
+    }
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
                        v.d, v.z, v.q);
     // (b, b) -> L2_rhs
@@ -949,6 +952,8 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     int niters = maxiters;
     double err = 0.0;
     auto check_conv = [&](int n, bool &done) -> int {
+        // the true residual every fifth iteration, up to the synchronization that reads it
+        TimerRange r_c("pcg_matmult_resid");
         if (check == 0) {
             hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);

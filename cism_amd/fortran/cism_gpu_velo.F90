@@ -45,7 +45,7 @@ module cism_gpu_velo
      real(c_double) :: efvs_constant, dx, dy, target_override, inner_rtol
      integer(c_int) :: adaptive_linear
      real(c_double) :: adaptive_gamma, stop_rel_err
-     integer(c_int) :: anderson_m, initial_guess, reserved(3)
+     integer(c_int) :: anderson_m, initial_guess, whichassemble_beta, reserved(2)
   end type cism_solve_opts
 
   type, bind(C) :: cism_solve_stats

@@ -40,7 +40,7 @@ EXPORTED = [
     "cism_surrogate_forward", "cism_surrogate_loss_grad", "cism_surrogate_train_step",
     "cism_surrogate_last_forward_ms", "cism_glissade_velo_higher_solve", "cism_learned_guess_load",
     "cism_learned_guess_load_file", "cism_learned_guess", "cism_learned_guess_last_ms",
-    "cism_compute_residual_vector_3d_noalloc",
+    "cism_compute_residual_vector_3d_noalloc", "cism_basal_sliding_bc",
 ]
 
 
@@ -58,7 +58,7 @@ class SolveOpts(C.Structure):
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
                 ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("initial_guess", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("whichassemble_beta", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class VeloSetup(C.Structure):
@@ -124,6 +124,7 @@ def lib():
         L.cism_assemble_stiffness_matrix_3d.argtypes = [VP, D, I32, D, D, D, D, D, D, D, C.c_int, D,
                                                         C.c_int, C.c_double, D, D, D, D]
         L.cism_basal_sliding_bc_3d.argtypes = [VP, C.c_double, C.c_double, I32, D, C.c_int, D, D]
+        L.cism_basal_sliding_bc.argtypes = [VP, C.c_double, C.c_double, I32, I32, D, D, D, C.c_int, D, D]
         L.cism_dirichlet_boundary_conditions_3d.argtypes = [VP, I32, I32, I32, D, D, D, D, D, D,
                                                             D, D]
         L.cism_check_symmetry_assembled_matrix_3d.argtypes = [VP, I32, D, D, D, D, I64]
@@ -268,6 +269,12 @@ class Context:
     def basal_sliding_bc_3d(self, dx, dy, active_vertex, beta, whichassemble_beta, Auu, Avv):
         _check(lib().cism_basal_sliding_bc_3d(self.h, dx, dy, ip(active_vertex), dp(beta),
                                               whichassemble_beta, dp(Auu), dp(Avv)))
+
+    def basal_sliding_bc(self, dx, dy, active_cell, active_vertex, beta, xVertex, yVertex,
+                         whichassemble_beta, Auu, Avv):
+        """basal_sliding_bc with the reference's arguments (0 standard, 1 local assembly)"""
+        _check(lib().cism_basal_sliding_bc(self.h, dx, dy, ip(active_cell), ip(active_vertex), dp(beta),
+                                           dp(xVertex), dp(yVertex), whichassemble_beta, dp(Auu), dp(Avv)))
 
     def dirichlet_boundary_conditions_3d(self, active_vertex, umask, vmask, uvel, vvel,
                                          Auu, Auv, Avu, Avv, bu, bv):
@@ -470,6 +477,7 @@ class Context:
         o.adaptive_gamma = adaptive_gamma
         o.stop_rel_err = stop_rel_err
         o.anderson_m = anderson_m
+        o.whichassemble_beta = p.whichassemble_beta
         o.initial_guess = initial_guess
         return o
 
@@ -547,6 +555,7 @@ class Context:
         o.adaptive_gamma = adaptive_gamma
         o.stop_rel_err = stop_rel_err
         o.anderson_m = anderson_m
+        o.whichassemble_beta = p.whichassemble_beta
         o.initial_guess = initial_guess
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
@@ -590,6 +599,7 @@ class Context:
         o.adaptive_linear = adaptive_linear
         o.adaptive_gamma = adaptive_gamma
         o.anderson_m = anderson_m
+        o.whichassemble_beta = p.whichassemble_beta
         o.stop_rel_err = stop_rel_err
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")

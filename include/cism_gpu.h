@@ -105,6 +105,15 @@ int  cism_basal_sliding_bc_3d(cism_ctx *ctx, double dx, double dy,
                               const int32_t *active_vertex, const double *beta,
                               int whichassemble_beta, double *Auu, double *Avv);
 
+/* basal_sliding_bc (glissade_velo_higher.F90:8125-8381) with the reference's arguments:
+ * whichassemble_beta 1 local (as cism_basal_sliding_bc_3d), 0 standard finite-element
+ * assembly (:8236-8363): beta phi_r phi_c integrated by 2x2 Gauss points over each active
+ * cell into the horizontal slots of the basal layer of the full 3D blocks. */
+int  cism_basal_sliding_bc(cism_ctx *ctx, double dx, double dy, const int32_t *active_cell,
+                           const int32_t *active_vertex, const double *beta,
+                           const double *xVertex, const double *yVertex,
+                           int whichassemble_beta, double *Auu, double *Avv);
+
 /* dirichlet_boundary_conditions_3d (glissade_velo_higher.F90:8385-8578) */
 int  cism_dirichlet_boundary_conditions_3d(cism_ctx *ctx, const int32_t *active_vertex,
                                            const int32_t *umask_dirichlet,
@@ -209,7 +218,9 @@ typedef struct {
                                   the context's velocity (the last solve's, or the guess of
                                   cism_learned_guess), uvel and vvel then only receive the
                                   solution                                                  */
-    int    reserved[3];
+    int    whichassemble_beta; /* which_ho_assemble_beta: 0 standard finite-element basal
+                                  friction, 1 local (the default)                            */
+    int    reserved[2];
 } cism_solve_opts;
 
 typedef struct {

@@ -381,6 +381,62 @@ void orc_basal_sliding_local(const orc_grid *g, double dx, double dy,
             }
 }
 
+/* basal_sliding_bc, HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363), on the basal layer:
+ * for each active cell (j, i ascending) and each quadrature point p, the 2D Jacobian of the
+ * cell's corners (get_basis_function_derivatives_2d, velo_higher:6583-6751), beta at p
+ * interpolated from the corners, and Kuu(nr, nc) = beta_qp w_p detJ / vol0 phi_nr phi_nc
+ * added into slot indxA(iA, jA, 0) of node nr (Kvv = Kuu). */
+void orc_basal_sliding_standard(const orc_grid *g, const int32_t *active_cell,
+                                const double *xV, const double *yV, const double *beta,
+                                double *Auu, double *Avv)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, h = g->nhalo;
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r}, wq[4] = {1.0, 1.0, 1.0, 1.0};
+    /* corners SW, SE, NE, NW relative to the cell's NE vertex (ishift(3, n), jshift(3, n)) */
+    const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+    for (int j = h + 1; j <= ny - h + 1; ++j)
+        for (int i = h + 1; i <= nx - h + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            double x[4], y[4], b[4];
+            for (int n = 0; n < 4; ++n) {
+                x[n] = S2(xV, i + di[n], j + dj[n]);
+                y[n] = S2(yV, i + di[n], j + dj[n]);
+                b[n] = S2(beta, i + di[n], j + dj[n]);
+            }
+            for (int p = 0; p < 4; ++p) {
+                double phi[4], dxr[4], dyr[4];
+                phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+                phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+                phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+                phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+                dxr[0] = -(1.0 - yq[p]) / 4.0; dxr[1] = (1.0 - yq[p]) / 4.0;
+                dxr[2] = (1.0 + yq[p]) / 4.0;  dxr[3] = -(1.0 + yq[p]) / 4.0;
+                dyr[0] = -(1.0 - xq[p]) / 4.0; dyr[1] = -(1.0 + xq[p]) / 4.0;
+                dyr[2] = (1.0 + xq[p]) / 4.0;  dyr[3] = (1.0 - xq[p]) / 4.0;
+                double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+                for (int n = 0; n < 4; ++n) {
+                    J11 = J11 + dxr[n] * x[n];
+                    J12 = J12 + dxr[n] * y[n];
+                    J21 = J21 + dyr[n] * x[n];
+                    J22 = J22 + dyr[n] * y[n];
+                }
+                const double detJ = J11 * J22 - J12 * J21;
+                double beta_qp = 0.0;
+                for (int n = 0; n < 4; ++n) beta_qp = beta_qp + phi[n] * b[n];
+                for (int nr = 0; nr < 4; ++nr) {
+                    const int ii = i + di[nr], jj = j + dj[nr];
+                    for (int nc = 0; nc < 4; ++nc) {
+                        const int m = IDX_A(di[nc] - di[nr], dj[nc] - dj[nr], 0);
+                        const double K = 0.0 + beta_qp * wq[p] * detJ / VOL0 * phi[nr] * phi[nc];
+                        A4(Auu, m, nz, ii, jj) = A4(Auu, m, nz, ii, jj) + K;
+                        A4(Avv, m, nz, ii, jj) = A4(Avv, m, nz, ii, jj) + K;
+                    }
+                }
+            }
+        }
+}
+
 /* dirichlet_boundary_conditions_3d (velo_higher:8385-8578) */
 void orc_dirichlet_3d(const orc_grid *g, const int32_t *active_vertex,
                       const int32_t *umask, const int32_t *vmask,
@@ -1101,8 +1157,12 @@ int orc_picard_solve(const orc_grid *g, orc_picard_opts *o,
                                  flwafact, o->whichapprox, efvs, o->whichefvs, o->efvs_constant,
                                  Auu, Auv, Avu, Avv);
         if (status) break;
-        if (o->whichbabc != 6)   /* HO_BABC_NO_SLIP */
-            orc_basal_sliding_local(g, o->dx, o->dy, active_vertex, beta_internal, Auu, Avv);
+        if (o->whichbabc != 6) {   /* HO_BABC_NO_SLIP */
+            if (o->assemble_beta_standard)
+                orc_basal_sliding_standard(g, active_cell, xV, yV, beta_internal, Auu, Avv);
+            else
+                orc_basal_sliding_local(g, o->dx, o->dy, active_vertex, beta_internal, Auu, Avv);
+        }
         memcpy(bu, loadu, nv * 8); memcpy(bv, loadv, nv * 8);
         orc_dirichlet_3d(g, active_vertex, umask, vmask, uvel, vvel, Auu, Auv, Avu, Avv, bu, bv);
         orc_halo(g, 27 * nz, Auu); orc_halo(g, 27 * nz, Auv);

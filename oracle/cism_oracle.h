@@ -71,6 +71,12 @@ int  orc_assemble_3d(const orc_grid *g, const double *sigma, const int32_t *acti
                      double *efvs, int whichefvs, double efvs_constant,
                      double *Auu, double *Auv, double *Avu, double *Avv);
 
+/* basal_sliding_bc, HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363): the 2x2-point
+ * finite-element integral of beta phi_r phi_c over each active cell, into the kA = 0 slots
+ * of the basal layer k = nz */
+void orc_basal_sliding_standard(const orc_grid *g, const int32_t *active_cell,
+                                const double *xV, const double *yV, const double *beta,
+                                double *Auu, double *Avv);
 void orc_basal_sliding_local(const orc_grid *g, double dx, double dy,
                              const int32_t *active_vertex, const double *beta,
                              double *Auu, double *Avv);
@@ -196,6 +202,9 @@ typedef struct {
     /* output: converged_soln of the reference, counter < maxiter_nonlinear
      * (velo_higher:3731-3745) */
     int converged;
+    /* 1: basal friction by the standard finite-element assembly (which_ho_assemble_beta =
+     * HO_ASSEMBLE_BETA_STANDARD); 0: the local assembly (the default, HO_ASSEMBLE_BETA_LOCAL) */
+    int assemble_beta_standard;
 } orc_picard_opts;
 
 /* z = M^-1 r for precond 0 (identity), 1 (diagonal) or 2 (column tridiagonal) with the

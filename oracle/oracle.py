@@ -73,7 +73,8 @@ class PicardOpts(C.Structure):
                [("bp", C.POINTER(BasalPhysics))] + \
                [(n, C.POINTER(C.c_double)) for n in ("beta_external", "effecpress_stag", "f_ground")] + \
                [("target_override", C.c_double), ("L2_history", C.POINTER(C.c_double)),
-                ("L2_history_cap", C.c_int), ("converged", C.c_int)]
+                ("L2_history_cap", C.c_int), ("converged", C.c_int),
+                ("assemble_beta_standard", C.c_int)]
 
 
 def _set_basal(o, basal, keep):
@@ -104,6 +105,7 @@ def _declare(L):
     L.orc_assemble_3d.argtypes = [G, D, I, D, D, D, D, D, D, D, C.c_int, D, C.c_int,
                                   C.c_double, D, D, D, D]
     L.orc_basal_sliding_local.argtypes = [G, C.c_double, C.c_double, I, D, D, D]
+    L.orc_basal_sliding_standard.argtypes = [G, I, D, D, D, D, D]
     L.orc_dirichlet_3d.argtypes = [G, I, I, I, D, D, D, D, D, D, D, D]
     L.orc_halo.argtypes = [G, C.c_int, D]
     L.orc_check_symmetry_3d.argtypes = [G, I, D, D, D, D, D]
@@ -221,6 +223,14 @@ def basal_local(p, A):
                                   dp(A[0]), dp(A[3]))
 
 
+def basal_standard(p, A, beta=None):
+    """basal friction by the standard finite-element assembly, added into A[0], A[3]"""
+    g = grid_of(p)
+    b = p.beta_internal if beta is None else beta
+    lib().orc_basal_sliding_standard(C.byref(g), ip(p.active_cell), dp(p.xVertex), dp(p.yVertex), dp(b),
+                                     dp(A[0]), dp(A[3]))
+
+
 def dirichlet(p, A, bu, bv, uvel=None, vvel=None):
     uvel = p.uvel if uvel is None else uvel
     vvel = p.vvel if vvel is None else vvel
@@ -334,7 +344,7 @@ def precond_apply(p, A, precond, ru, rv, zu=None, zv=None, active_vertex=None):
 
 def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
            whichsparse=None, whichprecond=None, whichresid=None, basal=None, target=None,
-           uvel=None, vvel=None):
+           uvel=None, vvel=None, assemble_beta=None):
     """Runs the oracle Picard loop from p.uvel/p.vvel (copied; or from uvel/vvel). Returns a
     result dict with the pre-solve residual norm of every Picard iteration (L2_history).
     basal: (BasalPhysics, beta_external, effecpress_stag, f_ground) to recompute beta with
@@ -349,6 +359,8 @@ def picard(p, loadu, loadv, maxiter=None, linear_maxiters=200, linear_tolerance=
     o.L2_history = dp(hist)
     o.L2_history_cap = cap
     o.target_override = 0.0 if target is None else float(target)
+    wab = p.whichassemble_beta if assemble_beta is None else assemble_beta
+    o.assemble_beta_standard = 1 if wab == 0 else 0
     o.whichbabc = p.whichbabc
     o.whichefvs = p.whichefvs
     o.whichresid = p.whichresid if whichresid is None else whichresid

@@ -191,6 +191,44 @@ def test_dirichlet_basal_symmetry_counts(gpu, prob):
         assert ctx.count_nonzeros_3d(*Ag, p.active_vertex) == O.count_nonzeros(p, Ao)
 
 
+def test_basal_standard(gpu, prob):
+    """the standard finite-element basal friction (which_ho_assemble_beta = 0), bit for bit:
+    the same terms added in the reference's order onto the same assembled coefficients"""
+    p = prob
+    rng = np.random.default_rng(13)
+    u = np.asfortranarray(20.0 * rng.standard_normal(p.uvel.shape))
+    v = np.asfortranarray(20.0 * rng.standard_normal(p.vvel.shape))
+    st, A, _ = O.assemble(p, u, v)
+    beta = np.asfortranarray(np.abs(p.beta_internal) + 50.0 * rng.random(p.beta_internal.shape))
+    Ao = [a.copy(order="F") for a in A]
+    O.basal_standard(p, Ao, beta=beta)
+    Ag = [a.copy(order="F") for a in A]
+    with gpu.Context.for_problem(p) as ctx:
+        ctx.basal_sliding_bc(p.dx, p.dy, p.active_cell, p.active_vertex, beta, p.xVertex, p.yVertex, 0,
+                             Ag[0], Ag[3])
+    assert np.array_equal(Ag[0], Ao[0]) and np.array_equal(Ag[3], Ao[3])
+    assert not np.array_equal(Ag[0], A[0])
+
+
+@pytest.mark.parametrize("name", ["grf", "sheet", "outflow"])
+def test_picard_fp64_standard_basal(gpu, name):
+    """the Picard loop with the standard basal assembly against the oracle's"""
+    p = problems()[name]
+    p.whichassemble_beta = 0
+    lu, lv = O.loads(p)
+    ro = O.picard(p, lu, lv)
+    assert ro["status"] == 0
+    with gpu.Context.for_problem(p) as ctx:
+        rg = ctx.velo_higher_solve_3d(p, lu, lv)
+    xo = np.concatenate([ro["uvel"].ravel(), ro["vvel"].ravel()])
+    xg = np.concatenate([rg["uvel"].ravel(), rg["vvel"].ravel()])
+    e = rel_l2(xg, xo)
+    print(f"  standard basal: oracle picard={ro['picard_iters']} gpu picard={rg['picard_iters']} "
+          f"diff {e:.2e}")
+    assert rg["picard_iters"] == ro["picard_iters"]
+    assert e < TOL_PICARD
+
+
 def test_matvec_and_residual(gpu, prob):
     p = prob
     A, bu, bv, u, v, _ = assembled_system(p, seed=5)

@@ -187,3 +187,82 @@ def test_all_core_build_computes_the_same_system():
     assert st1 == st2 == 0 and n1 == n2
     d = np.linalg.norm(np.concatenate([(a - b).ravel() for a, b in zip(x2, x1)]))
     assert d <= 1e-11 * np.linalg.norm(np.concatenate([a.ravel() for a in x1]))
+
+
+def _basal_standard_bruteforce(p, beta):
+    """the standard basal friction entries of the basal layer, integrated independently: a
+    3 x 3 Gauss rule per cell (exact for the bilinear beta times two bilinear basis
+    functions, as the reference's 2 x 2 rule is), rectangular cells of dx x dy; returns
+    {(ii, jj, iA, jA): value} (0-based vertices)"""
+    g, w = np.polynomial.legendre.leggauss(3)
+    out = {}
+    corners = [(-1, -1), (0, -1), (0, 0), (-1, 0)]          # SW, SE, NE, NW of cell (i, j)
+    sgn = [(-1, -1), (1, -1), (1, 1), (-1, 1)]
+    for j in range(2, p.ny - 1):                              # cells nhalo+1 .. n-nhalo+1, 0-based
+        for i in range(2, p.nx - 1):
+            if not p.active_cell[i, j]:
+                continue
+            b = [beta[i + di, j + dj] for di, dj in corners]
+            for nr in range(4):
+                for nc in range(4):
+                    s = 0.0
+                    for a, wa in zip(g, w):
+                        for c, wc in zip(g, w):
+                            phi = [(1 + sx * a) * (1 + sy * c) / 4 for sx, sy in sgn]
+                            bq = sum(phi[n] * b[n] for n in range(4))
+                            s += wa * wc * bq * phi[nr] * phi[nc]
+                    s *= p.dx * p.dy / 4 / 1.0e9                # detJ = dx dy / 4
+                    key = (i + corners[nr][0], j + corners[nr][1],
+                           corners[nc][0] - corners[nr][0], corners[nc][1] - corners[nr][1])
+                    out[key] = out.get(key, 0.0) + s
+    return out
+
+
+def test_basal_standard_is_the_exact_bilinear_integral():
+    """HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363) against an independent 3 x 3 Gauss
+    integration of beta phi_r phi_c on each cell, on a grid with variable beta; with
+    constant beta, the rows of an interior vertex are the bilinear mass matrix: 4/9, 1/9
+    (edge neighbours), 1/36 (corner neighbours) of beta dx dy / vol0, summing to the local
+    assembly's single diagonal entry"""
+    p = configs.grf_sheet(ewn=10, nsn=9, nz=5, dx=2000.0, corr=8000.0, seed=3, periodic=False)
+    A = O.mat_zeros(p)
+    O.basal_standard(p, A)
+    want = _basal_standard_bruteforce(p, p.beta_internal)
+    nz = p.nz
+    scale = max(abs(v) for v in want.values())
+    for (ii, jj, iA, jA), val in want.items():
+        m = (iA + 1) + 3 * (jA + 1) + 9
+        assert abs(A[0][m, nz - 1, ii, jj] - val) <= 1e-13 * scale
+        assert A[3][m, nz - 1, ii, jj] == A[0][m, nz - 1, ii, jj]
+    touched = np.zeros(A[0].shape, bool)
+    for (ii, jj, iA, jA) in want:
+        touched[(iA + 1) + 3 * (jA + 1) + 9, nz - 1, ii, jj] = True
+    assert not A[0][~touched].any() and not A[1].any() and not A[2].any()
+    # constant beta: the mass matrix at an interior vertex
+    q = configs.slab(ewn=8, nsn=8, nz=5)
+    B = O.mat_zeros(q)
+    bc = np.full(q.beta_internal.shape, 250.0, order="F")
+    O.basal_standard(q, B, beta=bc)
+    f = 250.0 * q.dx * q.dy / 1.0e9
+    row = B[0][:, q.nz - 1, 5, 5]
+    assert row[13] == pytest.approx(4 / 9 * f, rel=1e-14)
+    for m in (12, 14, 10, 16):
+        assert row[m] == pytest.approx(1 / 9 * f, rel=1e-14)
+    for m in (9, 11, 15, 17):
+        assert row[m] == pytest.approx(1 / 36 * f, rel=1e-14)
+    assert row.sum() == pytest.approx(f, rel=1e-14)
+
+
+def test_picard_with_standard_basal_friction_converges_to_the_slab_balance():
+    """the whole Picard loop with the standard basal assembly on the sliding slab: a uniform
+    slab has u = rho_i g H tan(theta) / beta at the bed (basal balance), whichever
+    assembly, since the mass matrix's rows sum to the local diagonal"""
+    p = configs.sliding_slab(ewn=12, nsn=10, nz=7, H=1000.0, theta_deg=0.5, beta=500.0)
+    lu, lv = O.loads(p)
+    rs = O.picard(p, lu, lv, assemble_beta=0)
+    rl = O.picard(p, lu, lv, assemble_beta=1)
+    assert rs["status"] == 0 and rl["status"] == 0
+    ub = rs["uvel"][p.nz - 1, 2:-2, 2:-2]
+    assert np.allclose(ub, rl["uvel"][p.nz - 1, 2:-2, 2:-2], rtol=1e-8)
+    want = 910.0 * 9.81 * 1000.0 * np.tan(np.radians(0.5)) / 500.0
+    assert np.allclose(ub, want, rtol=1e-3)
