@@ -13,6 +13,14 @@
 
 static thread_local char g_err[512] = "";
 
+bool roctx_sync_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("CISM_ROCTX_SYNC");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 void cism_set_error(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -673,31 +681,33 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         if (c->bp_on) CISM_TRY(calcbeta_device(c));   // velo_higher:2502-2517
         double ta = now_s();
         {
-            // the ranges end at the synchronization below (sync_scalars)
-            TimerRange r_asm(dim2 ? "glissade_assemble_2d" : "glissade_assemble_3d");
             // basal friction (velo_higher:3039-3052): the local assembly inside the gather, the
             // standard one after it
             const bool basal = o->whichbabc != 6, standard = o->whichassemble_beta == 0;
-            if (dim2)
-                CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx, o->dy));
-            else
-                CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
-                                         basal && !standard, o->dx, o->dy));
-            if (basal && standard) CISM_TRY(basal_standard_device(c));
+            {
+                TimerRange r_asm(dim2 ? "glissade_assemble_2d" : "glissade_assemble_3d", c->stream, true);
+                if (dim2)
+                    CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx,
+                                               o->dy));
+                else
+                    CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                             basal && !standard, o->dx, o->dy));
+                if (basal && standard) CISM_TRY(basal_standard_device(c));
+            }
             HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                    c->stream));
             {
-                TimerRange r_d(dim2 ? "glissade_dirichlet_2d" : "glissade_dirichlet_3d");
+                TimerRange r_d(dim2 ? "glissade_dirichlet_2d" : "glissade_dirichlet_3d", c->stream, true);
                 CISM_TRY(dirichlet_device(c));
             }
             {
-                TimerRange r_h("glissade_halo_Axxs");
+                TimerRange r_h("glissade_halo_Axxs", c->stream, true);
                 CISM_TRY(launch_halo_mat(c, c->A));
                 CISM_TRY(launch_halo_vec(c, c->b));
             }
             // the mixed-precision path gets its FP32 symmetric matrix from the same pass
             if (prec == 1) CISM_TRY(ensure_a32(c));
-            TimerRange r_s("glissade_chk_symmetry");
+            TimerRange r_s("glissade_chk_symmetry");   // ends at the synchronization below
             CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
             CISM_TRY(sync_scalars(c));
         }
@@ -962,7 +972,7 @@ extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opt
     if (su->initial_guess == 0) CISM_TRY(upload_vec(c, c->x, uvel, vvel));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double t_dev0 = now_s();   // inputs resident from here on
-    TimerRange r_init("glissade_vhs_init");
+    roctxRangePushA("glissade_vhs_init");   // popped once the loop's inputs are set
     CISM_TRY(velo_inputs_compute(c, su->dx, su->dy, su->thklim, su->eus, su->max_slope));
     CISM_TRY(mask_planes_device(c));
     const long long ncell = (long long)g.nx * g.ny, ncol = (long long)g.sx * g.sy;
@@ -975,11 +985,11 @@ extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opt
     // gravity and lateral ice-ocean loads (velo_higher:2033-2111)
     HIP_TRY(hipMemsetAsync(c->load, 0, sizeof(double) * 2 * g.npad, c->stream));
     {
-        TimerRange r_g("glissade_load_vector_gravity");
+        TimerRange r_g("glissade_load_vector_gravity", c->stream, true);
         CISM_TRY(load_gravity_device(c, su->whichassemble_taud, su->dx, su->dy));
     }
     if (su->lateral_bc) {
-        TimerRange r_l("glissade_load_vector_lateral_bc");
+        TimerRange r_l("glissade_load_vector_lateral_bc", c->stream, true);
         CISM_TRY(load_lateral_device(c, c->planes + 2 * ncell, c->planes + 4 * ncell));
     }
     hipLaunchKernelGGL(k_noslip_masks, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, noslip ? 1 : 0,
@@ -994,6 +1004,8 @@ extern "C" int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opt
         if (c->bp_on) CISM_TRY(calcbeta_device(c));
         CISM_TRY(learned_guess_device(c));
     }
+    if (roctx_sync_enabled()) HIP_TRY(hipStreamSynchronize(c->stream));
+    roctxRangePop();
     cism_solve_opts oo = *o;
     oo.initial_guess = 2;
     return picard_device(c, &oo, 0, t0, uvel, vvel, efvs, nullptr, nullptr, nullptr, nullptr, stats, t_dev0);

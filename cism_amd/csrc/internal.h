@@ -128,12 +128,24 @@ struct cism_ctx {
 
 // A named host-side range for rocprofv3 --marker-trace, with the reference's timer names
 // (t_startf/t_stopf regions of glissade_velo_higher.F90 and glissade_velo_higher_pcg.F90).
-// A range that ends at a synchronization covers the device work it enqueued.
+// The device work a range enqueues runs asynchronously; with the environment variable
+// CISM_ROCTX_SYNC=1 a range given a stream waits for it before it closes, so that each
+// range covers its own device work (a few microseconds per range).
+bool roctx_sync_enabled();
 struct TimerRange {
-    explicit TimerRange(const char *name) { roctxRangePushA(name); }
-    ~TimerRange() { roctxRangePop(); }
+    explicit TimerRange(const char *name, hipStream_t s = nullptr, bool sync = false) : s_(s), sync_(sync) {
+        roctxRangePushA(name);
+    }
+    ~TimerRange() {
+        if (sync_ && roctx_sync_enabled()) (void)hipStreamSynchronize(s_);
+        roctxRangePop();
+    }
     TimerRange(const TimerRange &) = delete;
     TimerRange &operator=(const TimerRange &) = delete;
+
+  private:
+    hipStream_t s_;
+    bool sync_;
 };
 
 // ---------------------------------------------------------------------------
