@@ -7,13 +7,14 @@ the previous step's velocity.  `VelocitySolver` does the same on one GPU:
 
   - one `gpu.Context` for the whole run (no per-step allocation or upload of the
     solver's buffers);
-  - per step, the solve's inputs from the geometry on the device
-    (cism_glissade_velo_inputs: masks, staggered thickness and surface, surface
-    gradient, vertex geometry, flow factor), the grounded fraction
-    (cism_glissade_grounded_fraction), beta from the external field (calcbeta,
-    which_ho_babc = 5) and the gravity and lateral loads;
-  - the Picard loop starts from the previous step's velocity; on the first step from
-    the learned initial guess when weights are given, else from zero;
+  - per step, one call of cism_glissade_velo_higher_solve with the new thickness,
+    surface and bed: the masks, staggered thickness and surface, surface gradient,
+    vertex geometry, flow factor, grounded fraction, beta (calcbeta,
+    which_ho_babc = 5) and the gravity and lateral loads are computed on the device,
+    so the geometry's derived fields never round-trip through the host;
+  - the Picard loop starts from the previous step's velocity, which stays resident
+    on the device; on the first step from the learned initial guess (computed on the
+    device) when weights are given, else from zero;
   - `save_restart` / `load_restart` keep uvel, vvel and efvs (the restart velocity
     fields of glide_vars.def) in an .npz file.
 
@@ -23,6 +24,7 @@ thickness, surface and bed of each step.
 from __future__ import annotations
 
 import os
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -47,6 +49,7 @@ class VelocitySolver:
         self.ctx = G.Context(self.nx, self.ny, nz, periodic, device)
         self.weights = weights
         self.uvel = self.vvel = self.efvs = None
+        self._resident = False   # the context holds the last step's velocity
         self.steps = 0
 
     def close(self):
@@ -58,48 +61,49 @@ class VelocitySolver:
     def __exit__(self, *a):
         self.close()
 
-    def problem(self, thck, usrf, topg):
-        """The step's Problem, every input computed on the device from the geometry."""
-        c = self.ctx
+    def _geometry(self, thck, usrf, topg):
+        """The step's inputs to cism_glissade_velo_higher_solve: the geometry, the flow
+        factor and the solver options; the masks, staggered fields, grounded fraction,
+        beta and loads are computed from them on the device."""
         f = lambda a: np.asfortranarray(a, dtype=np.float64)   # noqa: E731
-        o = c.glissade_velo_inputs(self.dx, self.dx, self.thklim, self.eus, f(thck), f(usrf), f(topg),
-                                   self.flwa, max_slope=self.max_slope)
-        fg = c.grounded_fraction(o["ice_mask"], o["floating_mask"], o["land_mask"], 0)
-        s2 = (self.nx - 1, self.ny - 1)
-        beta = c.calcbeta(G.BasalPhysics(configs.HO_BABC_BETA_EXTERNAL), np.zeros(s2, order="F"),
-                          np.zeros(s2, order="F"), self.beta_external, None, fg)
-        z3 = lambda: np.zeros((self.nz, self.nx - 1, self.ny - 1), order="F")   # noqa: E731
-        mask = lambda: np.zeros((self.nz, self.nx - 1, self.ny - 1), dtype=np.int32, order="F")  # noqa: E731
-        return configs.Problem(
-            name="time_step", ewn=self.ewn, nsn=self.nsn, nz=self.nz, dx=self.dx, dy=self.dx,
-            periodic=self.periodic, sigma=self.sigma, stagwbndsigma=configs.stagwbnd(self.sigma),
-            thck=f(thck), usrf=f(usrf), topg=f(topg), thklim=self.thklim,
-            active_cell=o["active_cell"], active_vertex=o["active_vertex"], xVertex=o["xVertex"],
-            yVertex=o["yVertex"], stagusrf=o["stagusrf"], stagthck=o["stagthck"],
-            dusrf_dx=o["dusrf_dx"], dusrf_dy=o["dusrf_dy"], flwa=self.flwa, flwafact=o["flwafact"],
-            beta_internal=beta, umask_dirichlet=mask(), vmask_dirichlet=mask(), uvel=z3(), vvel=z3(),
-            whichbabc=configs.HO_BABC_BETA_EXTERNAL, ice_mask=o["ice_mask"],
-            floating_mask=o["floating_mask"], ocean_mask=o["ocean_mask"], land_mask=o["land_mask"])
+        z3 = np.zeros((self.nz, self.nx - 1, self.ny - 1), order="F")
+        return SimpleNamespace(
+            nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dx, thklim=self.thklim,
+            sigma=self.sigma, thck=f(thck), usrf=f(usrf), topg=f(topg), flwa=self.flwa,
+            uvel=z3, vvel=z3, whichbabc=configs.HO_BABC_BETA_EXTERNAL,
+            whichefvs=configs.HO_EFVS_NONLINEAR, whichapprox=configs.HO_APPROX_BP,
+            whichresid=configs.HO_RESID_L2NORM, whichsparse=configs.HO_SPARSE_PCG_CHRONGEAR,
+            whichprecond=configs.HO_PRECOND_SIA, whichassemble_taud=1, whichassemble_beta=1,
+            efvs_constant=2336041.42829, maxiter_nonlinear=100)
+
+    def _load_weights(self):
+        if self.weights is None or not os.path.exists(self.weights):
+            return False
+        from . import surrogate as SG
+        params, meta = SG.load_weights(self.weights)
+        self.ctx.learned_guess_load(params, meta["nz"], 1 if meta.get("target") == "log1p" else 0,
+                                    meta["uscale"])
+        return True
 
     def step(self, thck, usrf, topg, stop_rel_err=0.0):
-        """Solves the velocity of the new geometry; returns the solve's result dict."""
-        p = self.problem(thck, usrf, topg)
-        lu, lv = self.ctx.loads(p)
-        u0, v0 = self.uvel, self.vvel
-        guess = "previous step"
-        if u0 is None:
-            guess = "zero"
-            if self.weights is not None and os.path.exists(self.weights):
-                from . import surrogate as SG
-                pred = SG.Predictor(p, self.weights)
-                try:
-                    u0, v0 = pred.predict(p)
-                finally:
-                    pred.close()
-                guess = "learned"
+        """Solves the velocity of the new geometry; returns the solve's result dict.  The
+        velocity stays on the device between steps: the next step starts from it without
+        a transfer."""
+        g = self._geometry(thck, usrf, topg)
+        u0 = v0 = None
+        if self.uvel is None:
+            guess, mode = ("learned", 1) if self._load_weights() else ("zero", 0)
+        elif self._resident:
+            guess, mode = "previous step", 2
+        else:                              # from a restart file
+            guess, mode, u0, v0 = "previous step", 0, self.uvel, self.vvel
         kw = dict(precision=1, adaptive_linear=1, inner_rtol=5e-2, anderson_m=3) if self.fast else {}
-        r = self.ctx.velo_higher_solve_3d(p, lu, lv, uvel=u0, vvel=v0, stop_rel_err=stop_rel_err, **kw)
+        r = self.ctx.glissade_velo_higher_solve(
+            g, bp=G.BasalPhysics(configs.HO_BABC_BETA_EXTERNAL), beta_external=self.beta_external,
+            initial_guess=mode, uvel=u0, vvel=v0, lateral_bc=1, max_slope=self.max_slope, eus=self.eus,
+            stop_rel_err=stop_rel_err, **kw)
         self.uvel, self.vvel, self.efvs = r["uvel"], r["vvel"], r["efvs"]
+        self._resident = True
         self.steps += 1
         r["initial_guess"] = guess
         return r
@@ -113,3 +117,4 @@ class VelocitySolver:
             self.vvel = np.asfortranarray(z["vvel"])
             self.efvs = np.asfortranarray(z["efvs"])
             self.steps = int(z["steps"])
+        self._resident = False

@@ -36,6 +36,20 @@ def make_problem(family: str, ewn: int, nsn: int, seed: int):
     raise SystemExit(f"unknown family {family}")
 
 
+def solve_guessed(ctx, p, params, meta):
+    """The fast path from the geometry with a learned initial guess (an earlier network):
+    the same Picard fixed point under the reference's stopping rule, reached in fewer
+    iterations than from zero."""
+    if not getattr(ctx, "_guess_ready", False):
+        ctx.learned_guess_load(params, int(meta.get("nz", 11)), 1 if meta.get("target") == "log1p" else 0,
+                               float(meta["uscale"]))
+        ctx._guess_ready = True
+    bp = G.BasalPhysics(p.whichbabc)
+    return ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=p.extra["beta_external"], initial_guess=1,
+                                          precision=1, adaptive_linear=1, inner_rtol=5e-2, anderson_m=3,
+                                          efvs=False)
+
+
 def solve_plain(ctx, p, log, chunk=10, precision=0):
     """The plain FP64 path at the reference's default tolerances (precision 0), or the
     mixed-precision path with inexact linear solves and Anderson acceleration to the same
@@ -124,6 +138,10 @@ def main(argv=None):
                     help="0: training solves on the plain FP64 path; 1: on the mixed-precision path")
     ap.add_argument("--scales", type=int, default=0,
                     help="also learn the per-column preconditioner scales (channels 22, 23)")
+    ap.add_argument("--checkpoint", type=int, default=0, help="save the weights every this many epochs")
+    ap.add_argument("--data-guess", default=None,
+                    help="weights of an earlier network: the training solves start from its guess "
+                         "(fast path from the geometry, the reference's stopping rule)")
     a = ap.parse_args(argv)
     t_start = time.perf_counter()
     nch = 24 if a.scales else 22
@@ -133,34 +151,48 @@ def main(argv=None):
         raise SystemExit("--crop must be a multiple of 8 no larger than the grid")
     th, tw = (c, c) if c else (nsn, ewn)
 
+    guess_w = SG.load_weights(a.data_guess) if a.data_guess else None
+
     def gen(seed0, want, tag):
         """solves seeds seed0, seed0+1, ... and keeps the first `want` whose Picard loop
-        converged (a solve that reaches the iteration cap is not a solution to learn)"""
-        X, T, M, used = [], [], [], []
+        converged (a solve that reaches the iteration cap is not a solution to learn);
+        the samples go into arrays allocated once, at the first sample's shapes"""
+        X = T = M = None
+        used = []
         ctx = None
         s = seed0
-        while len(X) < want and s < seed0 + want * a.candidates:
+        while len(used) < want and s < seed0 + want * a.candidates:
             p = make_problem(a.family, ewn, nsn, s)
             if ctx is None:
                 ctx = G.Context.for_problem(p)
             t0 = time.perf_counter()
-            r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True),
-                            precision=a.data_precision)
+            if guess_w is not None:
+                r = solve_guessed(ctx, p, *guess_w)
+            else:
+                r = solve_plain(ctx, p, lambda m: print(f"[train] {tag} seed {s}: {m}", flush=True),
+                                precision=a.data_precision)
             umax = float(np.abs(r["uvel"]).max())
             ok = r["picard_iters"] < p.maxiter_nonlinear
-            print(f"[train] {tag} {len(X) + ok}/{want} seed {s}: {r['picard_iters']} Picard, "
+            print(f"[train] {tag} {len(used) + ok}/{want} seed {s}: {r['picard_iters']} Picard, "
                   f"{time.perf_counter() - t0:.2f} s, max|u| {umax:.0f} m/yr"
                   + ("" if ok else " -- not converged, skipped"), flush=True)
             if ok:
                 x = SG.features(p)
                 su, sv = ctx.column_scale_target() if a.scales else (None, None)
                 t, m = SG.targets(p, r["uvel"], r["vvel"], a.uscale, su, sv, target=a.target)
-                X.append(x), T.append(t), M.append(m), used.append(s)
+                if X is None:
+                    X = np.empty((want,) + x.shape, x.dtype)
+                    T = np.empty((want,) + t.shape, t.dtype)
+                    M = np.empty((want,) + m.shape, m.dtype)
+                n = len(used)
+                X[n], T[n], M[n] = x, t, m
+                used.append(s)
             s += 1
         ctx.close()
-        if not X:
+        if not used:
             raise SystemExit(f"no converged {tag} problem among the candidate seeds")
-        return np.stack(X), np.stack(T), np.stack(M), used
+        n = len(used)
+        return X[:n], T[:n], M[:n], used
 
     Xtr, Ttr, Mtr, seeds_tr = gen(a.seed0, a.samples, "data")
     Xva, Tva, Mva, seeds_va = gen(a.val_seed0, a.val, "val")
@@ -180,6 +212,11 @@ def main(argv=None):
         errs = [rel_err(dec(evalnet.forward(Xva[i:i + 1])[0]), dec(Tva[i]), Mva[i], 22)
                 for i in range(len(Xva))]
         return float(np.mean(errs))
+
+    def meta_base():
+        return {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11,
+                "uscale": a.uscale, "samples": len(seeds_tr), "seeds": seeds_tr, "val_seeds": seeds_va,
+                "target": a.target, "batch": a.batch, "lr": a.lr}
 
     rng = np.random.default_rng(0)
     n = len(Xtr)
@@ -206,8 +243,11 @@ def main(argv=None):
         hist.append((ep, float(np.mean(losses)), ve))
         print(f"[train] epoch {ep + 1}/{a.epochs}: loss {np.mean(losses):.4f}, val rel L2 {ve:.4f}, "
               f"lr {lr:.2e}, {time.perf_counter() - t0:.1f} s", flush=True)
-    meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": a.samples,
+        if a.checkpoint and (ep + 1) % a.checkpoint == 0:
+            SG.save_weights(a.out, net.get_params(), dict(meta_base(), epochs_done=ep + 1, history=hist))
+    meta = {"scales": bool(a.scales), "family": a.family, "ewn": ewn, "nsn": nsn, "crop": c, "nz": 11, "uscale": a.uscale, "samples": len(seeds_tr),
             "seeds": seeds_tr, "val_seeds": seeds_va, "data_precision": a.data_precision,
+            "data_guess": a.data_guess, "init": a.init,
             "target": a.target,
             "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "val_rel_l2": hist[-1][2],
             "history": hist, "seconds": time.perf_counter() - t_start}

@@ -59,3 +59,34 @@ def test_warm_start_and_restart(tmp_path):
         rr = vr.step(H2, s2, p.topg)
     assert rr["picard_iters"] == rw["picard_iters"]
     assert np.array_equal(rr["uvel"], rw["uvel"]) and np.array_equal(rr["vvel"], rw["vvel"])
+
+
+def test_fast_loop_from_the_learned_guess():
+    """fast=True: the first step from the device-computed learned guess, the second warm;
+    both reach the reference's Picard tolerance and agree with the FP64 solve of the
+    same geometry to it"""
+    import os
+    from cism_amd import surrogate as SG
+    w = SG.default_weights("grf")
+    if not os.path.exists(w):
+        pytest.skip("no GRF weights")
+    p = configs.grf_sheet(ewn=32, nsn=32, nz=11, dx=1000.0, seed=3107)
+    bext = np.asfortranarray(p.extra["beta_external"])
+    x = np.linspace(0.0, 2 * np.pi, p.nx)[:, None]
+    H2 = np.asfortranarray(p.thck * (1.0 + 0.02 * np.sin(x) ** 2))
+    s2 = np.asfortranarray(p.topg + H2)
+    with VelocitySolver(p.ewn, p.nsn, p.nz, p.dx, bext, thklim=p.thklim, fast=True, weights=w) as vs:
+        r1 = vs.step(p.thck, p.usrf, p.topg)
+        r2 = vs.step(H2, s2, p.topg)
+    with VelocitySolver(p.ewn, p.nsn, p.nz, p.dx, bext, thklim=p.thklim, fast=False) as ref:
+        c1 = ref.step(p.thck, p.usrf, p.topg)
+        c2 = ref.step(H2, s2, p.topg)
+    print(f"  fast: learned {r1['picard_iters']} Picard, warm {r2['picard_iters']}; "
+          f"FP64: {c1['picard_iters']}, warm {c2['picard_iters']}")
+    assert r1["initial_guess"] == "learned" and r1["guess_rejected"] == 0
+    assert r2["initial_guess"] == "previous step"
+    for r, c in ((r1, c1), (r2, c2)):
+        assert r["converged"] and c["converged"] and r["fallbacks"] == 0
+        xr = np.concatenate([r["uvel"].ravel(), r["vvel"].ravel()])
+        xc = np.concatenate([c["uvel"].ravel(), c["vvel"].ravel()])
+        assert rel_l2(xr, xc) < 1e-4
