@@ -268,7 +268,9 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->av_zero, c->ac_zero, c->as_av_prev, c->as_ac_prev, c->xref, c->partials, c->red_groups, c->sc, c->stage,
                     c->cscale, c->aa, c->aa_sums, c->ssa_Ke, c->ssa_ff2, c->ssa_efvs2, c->ssa_sigma,
-                    c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg, c->planes};
+                    c->ssa_flw3, c->bp_bext, c->bp_N, c->bp_fg, c->planes, c->dep_flwa, c->dep_thck,
+                    c->dep_efvs3, c->dep_eqp, c->dep_omk, c->dep_om, c->dep_beff, c->dep_som, c->dep_btr,
+                    c->dep_u3, c->dep_v3, c->dep_cs, c->dep_ub, c->dep_ice, c->dep_ipl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -658,6 +660,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         cism_set_error("anderson_m must be 0..%d", AA_MAX);
         return CISM_ERR_ARG;
     }
+    // the depth-integrated 2D solvers L1L2 (3) and DIVA (4); with a no-slip bed the SSA and
+    // L1L2 velocities are zero and the loop ends at once (velo_higher:3234-3245)
+    const bool depth = dim2 && (o->whichapprox == 3 || o->whichapprox == 4);
+    const bool zero_2d = dim2 && o->whichbabc == 6 && o->whichapprox != 4;
     anderson_reset(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!(t_dev0 > 0.0)) t_dev0 = now_s();   // inputs resident from here on
@@ -678,21 +684,32 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         counter++;
         HIP_TRY(hipMemcpyAsync(c->usav, c->x, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                c->stream));
-        if (c->bp_on) CISM_TRY(calcbeta_device(c));   // velo_higher:2502-2517
+        if (c->bp_on) {   // velo_higher:2237-2243, 2502-2517: DIVA's basal velocity is the 3D bed level
+            if (depth && o->whichapprox == 4)
+                CISM_TRY(calcbeta_from_device(c, c->dep_u3 + (c->dep_nz3 - 1), c->dep_v3 + (c->dep_nz3 - 1),
+                                              c->dep_nz3));
+            else
+                CISM_TRY(calcbeta_device(c));
+        }
         double ta = now_s();
         {
             // basal friction (velo_higher:3039-3052): the local assembly inside the gather, the
-            // standard one after it
-            const bool basal = o->whichbabc != 6, standard = o->whichassemble_beta == 0;
+            // standard one after it; the 2D branch applies it for every which_ho_babc
+            // (:2832-2852), DIVA with beta_eff
+            const bool basal = dim2 || o->whichbabc != 6, standard = o->whichassemble_beta == 0;
             {
                 TimerRange r_asm(dim2 ? "glissade_assemble_2d" : "glissade_assemble_3d", c->stream, true);
-                if (dim2)
+                if (depth)
+                    CISM_TRY(assemble_depth_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                                   basal && !standard, o->whichbabc == 6, o->dx, o->dy));
+                else if (dim2)
                     CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx,
                                                o->dy));
                 else
                     CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
                                              basal && !standard, o->dx, o->dy));
-                if (basal && standard) CISM_TRY(basal_standard_device(c));
+                if (basal && standard)
+                    CISM_TRY(basal_standard_device(c, depth ? depth_basal_beta(c, o->whichapprox) : nullptr));
             }
             HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
                                    c->stream));
@@ -726,86 +743,93 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             L2 = 0.0;
             break;
         }
-        // the mixed path takes the first residual of its refinement from this pass
-        const bool share_r = prec == 1;
-        {
-            TimerRange r_r("glissade_resid_vec");
-            if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
-            CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));
-            CISM_TRY(sync_scalars(c));
-        }
-        L2 = sqrt(c->sc_host->sums[0]);
-        const double rhs = sqrt(c->sc_host->sums[1]);
-        L2rel = rhs > 0.0 ? L2 / rhs : 0.0;
-        if (!guess_checked) {
-            guess_checked = true;
-            // the guess is kept if it is closer than the zero guess to the solution of this
-            // first Picard system in that system's energy norm or as close, i.e. J(x) =
-            // x.A x / 2 - b.x <= J(0) = 0 (the zero guess itself passes); otherwise (a NaN
-            // included) it is dropped and this Picard iteration starts over from zero.  (The residual norm is no such test: a guess
-            // within 10 % of the solution can have a residual far above |b|, the zero
-            // guess's, because the operator amplifies its rough error components.)
-            double twoJ = 0.0;
-            CISM_TRY(guess_energy_device(c, c->t, &twoJ));
-            if (!(twoJ <= 0.0)) {
-                HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
-                S.guess_rejected = 1;
-                counter--;
-                t_asm += now_s() - ta;
-                continue;
-            }
-        }
-        double tl = now_s(), err = 0.0;
-        int niters = 0;
-        // the reference path solves every linear system to the fixed tolerance
-        double lin_tol = c->tolerance;
-        if (o->precision == 1 || o->adaptive_linear) {
-            // The Picard test on the absolute L2 norm (target 1e-4, velo_higher:2007) ends
-            // the loop only once |A(x) x - b| < 1e-4, and near the fixed point that norm is
-            // the last linear solve's residual.  Relative to |b| (1e4 .. 1e6 on ice sheets)
-            // the target is 1e-8 .. 1e-10, tighter than the linear tolerance: the fast path
-            // solves to half the target, which its FP64 refinement reaches, so that its
-            // loop can end by the reference's own test.
-            if (o->whichresid == 3 && rhs > 0.0) {
-                const double t_abs = o->target_override > 0 ? o->target_override : 1.0e-4;
-                lin_tol = fmin(lin_tol, 0.5 * t_abs / rhs);
-            }
-        }
-        if (o->adaptive_linear) {
-            // inexact Picard: no tighter than gamma times the current relative residual
-            const double gam = o->adaptive_gamma > 0 ? o->adaptive_gamma : 0.05;
-            lin_tol = fmin(1.0e-2, fmax(lin_tol, gam * L2rel));
-        }
-        roctxRangePushA("glissade_pcg_slv_struct");   // popped once the linear solve has ended
-        if (prec == 1) {
-            int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
-            st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
-                              &n64, &n32, 1, &stalled, &unconv);
-            S.unconverged_linear += unconv;
-            S.fp64_matvecs += n64;
-            S.fp32_matvecs += n32;
-            S.refinement_steps += n64;
-            if (!st && stalled) {
-                // the FP64 fallback: this Picard iteration's system again from its start
-                // velocity, with the reference's solver; the rest of the solve stays on it
-                S.fallbacks++;
-                prec = 0;
-                CISM_TRY(anderson_reset(c));
-                HIP_TRY(hipMemcpyAsync(c->x, c->usav, sizeof(double) * 2 * g.npad,
-                                       hipMemcpyDeviceToDevice, c->stream));
-                int n2 = 0;
-                st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err, &n2);
-                niters += n2;
-            }
+        if (zero_2d) {
+            HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
+            HIP_TRY(hipMemsetAsync(c->t, 0, sizeof(double) * 2 * g.npad, c->stream));
+            L2 = 0.0;
+            L2rel = 0.0;
         } else {
-            st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err,
-                              &niters);
-            S.unconverged_linear += !st && !(err < lin_tol);
+            // the mixed path takes the first residual of its refinement from this pass
+            const bool share_r = prec == 1;
+            {
+                TimerRange r_r("glissade_resid_vec");
+                if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
+                CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));
+                CISM_TRY(sync_scalars(c));
+            }
+            L2 = sqrt(c->sc_host->sums[0]);
+            const double rhs = sqrt(c->sc_host->sums[1]);
+            L2rel = rhs > 0.0 ? L2 / rhs : 0.0;
+            if (!guess_checked) {
+                guess_checked = true;
+                // the guess is kept if it is closer than the zero guess to the solution of this
+                // first Picard system in that system's energy norm or as close, i.e. J(x) =
+                // x.A x / 2 - b.x <= J(0) = 0 (the zero guess itself passes); otherwise (a NaN
+                // included) it is dropped and this Picard iteration starts over from zero.  (The residual norm is no such test: a guess
+                // within 10 % of the solution can have a residual far above |b|, the zero
+                // guess's, because the operator amplifies its rough error components.)
+                double twoJ = 0.0;
+                CISM_TRY(guess_energy_device(c, c->t, &twoJ));
+                if (!(twoJ <= 0.0)) {
+                    HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
+                    S.guess_rejected = 1;
+                    counter--;
+                    t_asm += now_s() - ta;
+                    continue;
+                }
+            }
+            double tl = now_s(), err = 0.0;
+            int niters = 0;
+            // the reference path solves every linear system to the fixed tolerance
+            double lin_tol = c->tolerance;
+            if (o->precision == 1 || o->adaptive_linear) {
+                // The Picard test on the absolute L2 norm (target 1e-4, velo_higher:2007) ends
+                // the loop only once |A(x) x - b| < 1e-4, and near the fixed point that norm is
+                // the last linear solve's residual.  Relative to |b| (1e4 .. 1e6 on ice sheets)
+                // the target is 1e-8 .. 1e-10, tighter than the linear tolerance: the fast path
+                // solves to half the target, which its FP64 refinement reaches, so that its
+                // loop can end by the reference's own test.
+                if (o->whichresid == 3 && rhs > 0.0) {
+                    const double t_abs = o->target_override > 0 ? o->target_override : 1.0e-4;
+                    lin_tol = fmin(lin_tol, 0.5 * t_abs / rhs);
+                }
+            }
+            if (o->adaptive_linear) {
+                // inexact Picard: no tighter than gamma times the current relative residual
+                const double gam = o->adaptive_gamma > 0 ? o->adaptive_gamma : 0.05;
+                lin_tol = fmin(1.0e-2, fmax(lin_tol, gam * L2rel));
+            }
+            roctxRangePushA("glissade_pcg_slv_struct");   // popped once the linear solve has ended
+            if (prec == 1) {
+                int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
+                st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
+                                  &n64, &n32, 1, &stalled, &unconv);
+                S.unconverged_linear += unconv;
+                S.fp64_matvecs += n64;
+                S.fp32_matvecs += n32;
+                S.refinement_steps += n64;
+                if (!st && stalled) {
+                    // the FP64 fallback: this Picard iteration's system again from its start
+                    // velocity, with the reference's solver; the rest of the solve stays on it
+                    S.fallbacks++;
+                    prec = 0;
+                    CISM_TRY(anderson_reset(c));
+                    HIP_TRY(hipMemcpyAsync(c->x, c->usav, sizeof(double) * 2 * g.npad,
+                                           hipMemcpyDeviceToDevice, c->stream));
+                    int n2 = 0;
+                    st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err, &n2);
+                    niters += n2;
+                }
+            } else {
+                st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err,
+                                  &niters);
+                S.unconverged_linear += !st && !(err < lin_tol);
+            }
+            roctxRangePop();
+            if (st) return st;
+            t_lin += now_s() - tl;
+            lin_total += niters;
         }
-        roctxRangePop();
-        if (st) return st;
-        t_lin += now_s() - tl;
-        lin_total += niters;
         CISM_TRY(launch_halo_vec(c, c->x));
         if (prec == 1 && o->anderson_m > 0) {
             double fn = 0.0;
@@ -815,6 +839,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         CISM_TRY(residual_velocity_device(c, o->whichresid));
         CISM_TRY(sync_scalars(c));
         resid_velo = c->sc_host->sums[0];
+        if (depth && o->whichapprox == 4) CISM_TRY(diva_post_solve_device(c));
         if (o->whichresid == 2) resid_velo = c->sc_host->sums[1] > 0 ? resid_velo / c->sc_host->sums[1] : 0.0;
         if (o->whichresid == 3) { crit = L2; target = 1.0e-4; }
         else if (o->whichresid == 4) { crit = L2rel; target = 1.0e-7; }
@@ -838,6 +863,14 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
         HIP_TRY(hipMemsetAsync(c->b, 0, sizeof(double) * 2 * g.npad, c->stream));
         HIP_TRY(hipMemsetAsync(c->t, 0, sizeof(double) * 2 * g.npad, c->stream));
+    }
+    if (depth && o->whichapprox == 3) {
+        CISM_TRY(l1l2_velocity_device(c, o->dx, o->dy));
+        CISM_TRY(sync_scalars(c));
+        if (c->sc_host->singular_flag) {
+            cism_set_error("Jacobian matrix is singular");
+            return CISM_ERR_SINGULAR;
+        }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     S.seconds_device = now_s() - t_dev0;
@@ -1076,6 +1109,106 @@ extern "C" int cism_velo_higher_solve_2d(cism_ctx *c, const cism_solve_opts *o, 
             uvel[k + (size_t)nz3 * col] = u2[col];
             vvel[k + (size_t)nz3 * col] = v2[col];
         }
+    return CISM_OK;
+}
+
+// The L1L2 and DIVA branches of glissade_velo_higher_solve (solve_2d with which_ho_approx
+// 3 or 4, velo_higher:1117-1173, 2206-3830) on a one-level context; see cism_gpu.h
+extern "C" int cism_velo_higher_solve_depth(
+    cism_ctx *c, const cism_solve_opts *o, int nz3, const double *sigma, const int32_t *ice_mask,
+    const int32_t *ice_plus_land_mask, const int32_t *active_cell, const int32_t *active_vertex,
+    const double *xVertex, const double *yVertex, const double *thck, const double *stagthck,
+    const double *dusrf_dx, const double *dusrf_dy, const double *flwa, const double *flwafact,
+    const double *beta_internal, const int32_t *umask_dirichlet, const int32_t *vmask_dirichlet,
+    const double *loadu, const double *loadv, double *uvel, double *vvel, double *uvel_2d, double *vvel_2d,
+    double *btractx, double *btracty, double *efvs, double *bu, double *bv, double *resid_u, double *resid_v,
+    cism_solve_stats *stats) {
+    CTX_BEGIN(c);
+    const GridDev &g = c->g;
+    if (!o || g.nz != 1 || nz3 < 2 || nz3 > 64) {
+        cism_set_error("velo_higher_solve_depth: needs a one-level context (nz = 1) and 2 <= nz3 <= 64");
+        return CISM_ERR_ARG;
+    }
+    const int approx = o->whichapprox;
+    if ((approx != 3 && approx != 4) || o->whichefvs < 0 || o->whichefvs > 2) {
+        cism_set_error("velo_higher_solve_depth runs L1L2 (whichapprox = 3) or DIVA (4)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    if (o->whichsparse != 2 && o->whichsparse != 3) {
+        cism_set_error("whichsparse must be 2 or 3 (native PCG)");
+        return CISM_ERR_UNSUPPORTED;
+    }
+    if (approx == 4 && o->anderson_m != 0) {
+        cism_set_error("DIVA carries its viscosity and basal traction across Picard iterations: anderson_m must be 0");
+        return CISM_ERR_ARG;
+    }
+    if (!sigma || !active_cell || !active_vertex || !xVertex || !yVertex || !thck || !stagthck || !flwafact ||
+        !beta_internal || !umask_dirichlet || !vmask_dirichlet || !loadu || !loadv || !uvel || !vvel ||
+        (approx == 3 && (!ice_mask || !dusrf_dx || !dusrf_dy || !flwa)) ||
+        (approx == 4 && (!ice_plus_land_mask || !uvel_2d || !vvel_2d || !efvs))) {
+        cism_set_error("velo_higher_solve_depth: a required array is null");
+        return CISM_ERR_ARG;
+    }
+    const double t0 = now_s();
+    const size_t ncol = (size_t)g.sx * g.sy, ncell = (size_t)g.nx * g.ny, L = (size_t)nz3 - 1;
+    // host marshalling: the 2D start velocity, bed-level masks and column-summed loads
+    std::vector<double> u2(ncol), v2(ncol), lu2(ncol, 0.0), lv2(ncol, 0.0);
+    std::vector<int32_t> um2(ncol), vm2(ncol);
+    for (size_t col = 0; col < ncol; ++col) {
+        const size_t b = (size_t)(nz3 - 1) + (size_t)nz3 * col;
+        u2[col] = approx == 4 ? uvel_2d[col] : uvel[b];
+        v2[col] = approx == 4 ? vvel_2d[col] : vvel[b];
+        um2[col] = umask_dirichlet[b];
+        vm2[col] = vmask_dirichlet[b];
+        for (int k = 0; k < nz3; ++k) {
+            lu2[col] = lu2[col] + loadu[k + (size_t)nz3 * col];
+            lv2[col] = lv2[col] + loadv[k + (size_t)nz3 * col];
+        }
+    }
+    CISM_TRY(ensure_depth(c, nz3));
+    CISM_TRY(up(c, c->ssa_sigma, sigma, (size_t)nz3));
+    CISM_TRY(up(c, c->ssa_flw3, flwafact, L * ncell));
+    if (flwa) CISM_TRY(up(c, c->dep_flwa, flwa, L * ncell));
+    CISM_TRY(up(c, c->dep_thck, thck, ncell));
+    if (ice_mask) CISM_TRY(up(c, c->dep_ice, ice_mask, ncell));
+    if (ice_plus_land_mask) CISM_TRY(up(c, c->dep_ipl, ice_plus_land_mask, ncell));
+    CISM_TRY(up(c, c->ac, active_cell, ncell));
+    CISM_TRY(up(c, c->xV, xVertex, ncol));
+    CISM_TRY(up(c, c->yV, yVertex, ncol));
+    CISM_TRY(up(c, c->sthck, stagthck, ncol));
+    if (dusrf_dx) CISM_TRY(up(c, c->dsdx, dusrf_dx, ncol));
+    if (dusrf_dy) CISM_TRY(up(c, c->dsdy, dusrf_dy, ncol));
+    CISM_TRY(up(c, c->av, active_vertex, ncol));
+    CISM_TRY(up(c, c->beta, beta_internal, ncol));
+    CISM_TRY(up(c, c->dep_u3, uvel, (size_t)nz3 * ncol));
+    CISM_TRY(up(c, c->dep_v3, vvel, (size_t)nz3 * ncol));
+    if (approx == 4) {
+        CISM_TRY(up(c, c->dep_efvs3, efvs, L * ncell));
+        if (btractx && btracty) {
+            CISM_TRY(up(c, c->dep_btr, btractx, ncol));
+            CISM_TRY(up(c, c->dep_btr + ncol, btracty, ncol));
+        } else {
+            HIP_TRY(hipMemsetAsync(c->dep_btr, 0, sizeof(double) * 2 * ncol, c->stream));
+        }
+    }
+    CISM_TRY(upload_dirichlet_masks(c, um2.data(), vm2.data(), ncol));
+    CISM_TRY(upload_vec(c, c->load, lu2.data(), lv2.data()));
+    CISM_TRY(upload_vec(c, c->x, u2.data(), v2.data()));
+    CISM_TRY(depth_prepare_device(c, approx));
+    HIP_TRY(hipMemsetAsync(c->sc, 0, sizeof(DevScal), c->stream));
+    CISM_TRY(picard_device(c, o, 1, t0, u2.data(), v2.data(), nullptr, bu, bv, resid_u, resid_v, stats));
+    if (uvel_2d && vvel_2d) {
+        memcpy(uvel_2d, u2.data(), sizeof(double) * ncol);
+        memcpy(vvel_2d, v2.data(), sizeof(double) * ncol);
+    }
+    CISM_TRY(down(c, uvel, c->dep_u3, (size_t)nz3 * ncol));
+    CISM_TRY(down(c, vvel, c->dep_v3, (size_t)nz3 * ncol));
+    if (efvs) CISM_TRY(down(c, efvs, c->dep_efvs3, L * ncell));
+    if (approx == 4 && btractx && btracty) {
+        CISM_TRY(down(c, btractx, c->dep_btr, ncol));
+        CISM_TRY(down(c, btracty, c->dep_btr + ncol, ncol));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return CISM_OK;
 }
 

@@ -647,11 +647,11 @@ __global__ __launch_bounds__(256) void k_basal_standard(GridDev g, const int32_t
     }
 }
 
-int basal_standard_device(cism_ctx *c) {
+int basal_standard_device(cism_ctx *c, const double *beta) {
     const GridDev &g = c->g;
     const long long n = 9LL * g.sx * g.sy;
     hipLaunchKernelGGL(k_basal_standard, dim3(nblocks_for(n)), dim3(256), 0, c->stream, g, c->ac, c->xV,
-                       c->yV, c->beta, c->A);
+                       c->yV, beta ? beta : c->beta, c->A);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }

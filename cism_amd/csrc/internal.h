@@ -84,6 +84,15 @@ struct cism_ctx {
     // sigma and the 3D flow factor of the caller's nz3 levels
     double *ssa_Ke, *ssa_ff2, *ssa_efvs2, *ssa_sigma, *ssa_flw3;
     int ssa_nz3;
+    // L1L2 / DIVA on a one-level context (ssa.hip): flow factor A and cell thickness, the
+    // layer viscosities (nz3-1 per cell; DIVA also per quadrature point), DIVA's column
+    // integrals, stag_omega, beta_eff and basal traction, the 3D velocity (nz3 per
+    // vertex), L1L2's cell strain rates, a basal-velocity stage, the ice and
+    // ice-plus-land cell masks
+    double *dep_flwa, *dep_thck, *dep_efvs3, *dep_eqp, *dep_omk, *dep_om, *dep_beff, *dep_som;
+    double *dep_btr, *dep_u3, *dep_v3, *dep_cs, *dep_ub;
+    int32_t *dep_ice, *dep_ipl;
+    int dep_nz3;
     // basal physics of the Picard loop (cism_set_basal_physics): with bp_on, beta is
     // recomputed from the bed velocity at the start of every Picard iteration
     cism_basal_physics bp;
@@ -328,6 +337,16 @@ int ensure_ssa(cism_ctx *c, int nz3);
 int ssa_prepare_device(cism_ctx *c);
 int assemble2d_device(cism_ctx *c, int whichefvs, double efvs_constant, int add_basal, double dx, double dy);
 int ssa_efvs_layers_device(cism_ctx *c, double *efvs3);
+// L1L2 (approx 3) and DIVA (approx 4) on a one-level context (ssa.hip)
+int ensure_depth(cism_ctx *c, int nz3);
+int depth_prepare_device(cism_ctx *c, int approx);
+int assemble_depth_device(cism_ctx *c, int approx, int whichefvs, double efvs_constant, int add_basal,
+                          int noslip, double dx, double dy);
+const double *depth_basal_beta(cism_ctx *c, int approx);   // beta_eff (DIVA) or beta
+int diva_post_solve_device(cism_ctx *c);
+int l1l2_velocity_device(cism_ctx *c, double dx, double dy);
+// calcbeta from a basal velocity (u at ub[stride*col], v at vb[stride*col])
+int calcbeta_from_device(cism_ctx *c, const double *ub, const double *vb, long long stride);
 int calcbeta_device(cism_ctx *c);
 // post-solve diagnostics from the context's geometry and velocity c->x
 int internal_stress_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
@@ -359,7 +378,7 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
 int dirichlet_device(cism_ctx *c);
 // basal friction by the standard finite-element assembly onto the assembled c->A (c->ac,
 // xV, yV, beta); the local assembly is part of the gather (assemble_device's add_basal)
-int basal_standard_device(cism_ctx *c);
+int basal_standard_device(cism_ctx *c, const double *beta = nullptr);   // null: c->beta
 int symmetry_device(cism_ctx *c, float *As);
 int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
 // r = A x - b on owned active rows: sums[0] = |r|^2, sums[1] = |b|^2 over them.  With

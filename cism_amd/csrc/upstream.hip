@@ -349,6 +349,10 @@ int calcbeta_device(cism_ctx *c) {
                            &c->sc->beta_bad);
 }
 
+int calcbeta_from_device(cism_ctx *c, const double *ub, const double *vb, long long stride) {
+    return launch_calcbeta(c, c->bp, ub, vb, stride, c->bp_bext, c->bp_N, c->bp_fg, c->beta, &c->sc->beta_bad);
+}
+
 static int check_basal_fields(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
                               const double *effecpress_stag, const double *f_ground) {
     const GridDev &g = c->g;

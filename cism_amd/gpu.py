@@ -29,7 +29,7 @@ EXPORTED = [
     "cism_dirichlet_boundary_conditions_3d", "cism_check_symmetry_assembled_matrix_3d",
     "cism_count_nonzeros_3d", "cism_compute_residual_vector_3d", "cism_compute_residual_velocity_3d",
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
-    "cism_velo_higher_solve_3d", "cism_velo_higher_solve_2d", "cism_set_reference_velocity", "cism_time_kernel",
+    "cism_velo_higher_solve_3d", "cism_velo_higher_solve_2d", "cism_velo_higher_solve_depth", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_load_vector_lateral_bc",
     "cism_set_precond_scale", "cism_column_scale_target",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
@@ -143,6 +143,9 @@ def lib():
         L.cism_velo_higher_solve_2d.argtypes = [VP, C.POINTER(SolveOpts), C.c_int, D, I32, I32, D, D,
                                                 D, D, D, D, I32, I32, D, D, D, D, D, D, D, D, D,
                                                 C.POINTER(SolveStats)]
+        L.cism_velo_higher_solve_depth.argtypes = [VP, C.POINTER(SolveOpts), C.c_int, D, I32, I32, I32, I32,
+                                                   D, D, D, D, D, D, D, D, D, I32, I32, D, D, D, D, D, D,
+                                                   D, D, D, D, D, D, D, C.POINTER(SolveStats)]
         L.cism_set_reference_velocity.argtypes = [VP, D, D]
         L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
         L.cism_set_precond_scale.argtypes = [VP, D, D]
@@ -570,6 +573,35 @@ class Context:
             dp(loadv), dp(u), dp(v), dp(efvs), dp(bu), dp(bv), dp(ru), dp(rv), C.byref(st)))
         return dict(uvel=u, vvel=v, efvs=efvs, bu=bu, bv=bv, resid_u=ru, resid_v=rv,
                     **st.as_dict())
+
+    def velo_higher_solve_depth(self, p, loadu, loadv, approx, uvel=None, vvel=None, uvel_2d=None,
+                                vvel_2d=None, btractx=None, btracty=None, efvs=None, **kw):
+        """The L1L2 (approx 3) or DIVA (approx 4) branch on a one-level context: 3D inputs of
+        p in; the 3D velocity, the 2D solution (uvel_2d/vvel_2d), DIVA's basal traction and
+        the layer viscosities out.  DIVA starts from uvel_2d/vvel_2d, btractx/btracty and
+        efvs (zero by default).  Options of the Picard loop as in velo_higher_solve_3d."""
+        if self.nz != 1:
+            raise ValueError("velo_higher_solve_depth needs a one-level context (Context.for_problem_2d)")
+        o = self._opts(p, **kw)
+        o.whichapprox = approx
+        f = lambda a: np.asfortranarray(a, dtype=np.float64)   # noqa: E731
+        s2 = (p.nx - 1, p.ny - 1)
+        z2 = lambda a: np.zeros(s2, order="F") if a is None else f(a).copy(order="F")   # noqa: E731
+        u = (p.uvel if uvel is None else uvel).copy(order="F")
+        v = (p.vvel if vvel is None else vvel).copy(order="F")
+        u2, v2, bx, by = z2(uvel_2d), z2(vvel_2d), z2(btractx), z2(btracty)
+        ef = np.zeros((p.nz - 1, p.nx, p.ny), order="F") if efvs is None else f(efvs).copy(order="F")
+        ipl = np.asfortranarray((p.ice_mask != 0) | (p.land_mask != 0), dtype=np.int32)
+        bu, bv, ru, rv = (np.zeros(s2, order="F") for _ in range(4))
+        st = SolveStats()
+        _check(lib().cism_velo_higher_solve_depth(
+            self.h, C.byref(o), p.nz, dp(p.sigma), ip(p.ice_mask), ip(ipl), ip(p.active_cell),
+            ip(p.active_vertex), dp(p.xVertex), dp(p.yVertex), dp(f(p.thck)), dp(p.stagthck), dp(p.dusrf_dx),
+            dp(p.dusrf_dy), dp(f(p.flwa)), dp(p.flwafact), dp(p.beta_internal), ip(p.umask_dirichlet),
+            ip(p.vmask_dirichlet), dp(loadu), dp(loadv), dp(u), dp(v), dp(u2), dp(v2), dp(bx), dp(by), dp(ef),
+            dp(bu), dp(bv), dp(ru), dp(rv), C.byref(st)))
+        return dict(uvel=u, vvel=v, uvel_2d=u2, vvel_2d=v2, btractx=bx, btracty=by, efvs=ef, bu=bu, bv=bv,
+                    resid_u=ru, resid_v=rv, **st.as_dict())
 
     def velo_higher_solve_2d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
                              maxiter_nonlinear=None, whichsparse=None, whichprecond=None,

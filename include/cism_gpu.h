@@ -297,6 +297,37 @@ int  cism_velo_higher_solve_2d(cism_ctx *ctx, const cism_solve_opts *opts, int n
                                double *bu, double *bv, double *resid_u, double *resid_v,
                                cism_solve_stats *stats);
 
+/* The L1L2 and DIVA branches of glissade_velo_higher_solve (solve_2d with which_ho_approx
+ * = 3 or 4; glissade_velo_higher.F90:1117-1173, 2206-3830, assemble_stiffness_matrix_2d
+ * :5208-5595, compute_integrals_diva :5601-5726, compute_3d_velocity_diva :5730-5830,
+ * compute_3d_velocity_L1L2 :5834-6356, compute_effective_viscosity_L1L2/_diva
+ * :7382-7864) on a one-level context (cism_ctx_create with nz = 1), the arrays of the 3D
+ * solve with nz3 levels:
+ *   ice_mask, ice_plus_land_mask (nx, ny): L1L2's gradient and staggering mask; DIVA's
+ *     staggering mask of omega (active ice or land);
+ *   thck (nx, ny), dusrf_dx/dusrf_dy (nx-1, ny-1), flwa and flwafact (nz3-1, nx, ny);
+ *   uvel/vvel (nz3, nx-1, ny-1) in/out: L1L2 starts from their bed level; the 3D velocity
+ *     comes out (L1L2 integrated from the bed after the loop, DIVA after every solve);
+ *   uvel_2d/vvel_2d (nx-1, ny-1): DIVA's depth-mean velocity in; the 2D solution out;
+ *   btractx/btracty (nx-1, ny-1): DIVA's basal traction in/out (null: zero in, not out);
+ *   efvs (nz3-1, nx, ny): DIVA's layer viscosities in (their quadrature-point start);
+ *     every layer's viscosity of the last assembly out;
+ *   bu, bv, resid_u, resid_v one-level (nx-1, ny-1).
+ * The 2D load is the column sum of loadu/loadv and the Dirichlet masks are the bed
+ * level's, as in cism_velo_higher_solve_2d.  DIVA requires opts->anderson_m = 0. */
+int  cism_velo_higher_solve_depth(cism_ctx *ctx, const cism_solve_opts *opts, int nz3,
+                                  const double *sigma, const int32_t *ice_mask,
+                                  const int32_t *ice_plus_land_mask, const int32_t *active_cell,
+                                  const int32_t *active_vertex, const double *xVertex,
+                                  const double *yVertex, const double *thck, const double *stagthck,
+                                  const double *dusrf_dx, const double *dusrf_dy, const double *flwa,
+                                  const double *flwafact, const double *beta_internal,
+                                  const int32_t *umask_dirichlet, const int32_t *vmask_dirichlet,
+                                  const double *loadu, const double *loadv, double *uvel, double *vvel,
+                                  double *uvel_2d, double *vvel_2d, double *btractx, double *btracty,
+                                  double *efvs, double *bu, double *bv, double *resid_u, double *resid_v,
+                                  cism_solve_stats *stats);
+
 /* The basal_physics fields calcbeta reads (glide_types.F90:1308-1370; reference
  * defaults ho_beta_const 1000, ho_beta_large 1e10, powerlaw_c 1e4, powerlaw_m 3,
  * coulomb_c 0.42, beta_grounded_min 1, beta_powerlaw_umax 0). whichbabc is

@@ -1668,7 +1668,10 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
         status = orc_assemble_2d_ssa(&g2, nz, sigma, active_cell, xV, yV, u2, v2, stagthck, flwafact,
                                      o->whichefvs, o->efvs_constant, efvs, Auu, Auv, Avu, Avv);
         if (status) break;
-        if (o->whichbabc != 6)
+        /* the 2D branch applies the basal friction for every which_ho_babc (:2832-2852) */
+        if (o->assemble_beta_standard)
+            orc_basal_sliding_standard(&g2, active_cell, xV, yV, beta_internal, Auu, Avv);
+        else
             orc_basal_sliding_local(&g2, o->dx, o->dy, active_vertex, beta_internal, Auu, Avv);
         memcpy(bu, l2u, nv * 8); memcpy(bv, l2v, nv * 8);
         orc_dirichlet_3d(&g2, active_vertex, um2, vm2, u2, v2, Auu, Auv, Avu, Avv, bu, bv);
@@ -1683,20 +1686,27 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
             L2 = 0.0;
             break;
         }
-        orc_residual_vector_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2, resid_u, resid_v,
-                               &L2, &L2rel);
-        if (o->L2_history && counter <= o->L2_history_cap) o->L2_history[counter - 1] = L2;
-        double err; int niters;
-        if (o->whichsparse == 3)
-            status = orc_pcg_chrongear_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
-                                          o->whichprecond, o->linear_maxiters, o->linear_tolerance,
-                                          &err, &niters);
-        else
-            status = orc_pcg_standard_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
-                                         o->whichprecond, o->linear_maxiters, o->linear_tolerance,
-                                         &err, &niters);
-        if (status) break;
-        o->total_linear_iters += niters;
+        if (o->whichbabc == 6) {   /* no-slip: the SSA velocity is zero (:3234-3245) */
+            memset(u2, 0, nv * 8); memset(v2, 0, nv * 8);
+            memset(resid_u, 0, nv * 8); memset(resid_v, 0, nv * 8);
+            L2 = 0.0;
+            L2rel = 0.0;
+        } else {
+            orc_residual_vector_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2, resid_u, resid_v,
+                                   &L2, &L2rel);
+            if (o->L2_history && counter <= o->L2_history_cap) o->L2_history[counter - 1] = L2;
+            double err; int niters;
+            if (o->whichsparse == 3)
+                status = orc_pcg_chrongear_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                              o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                              &err, &niters);
+            else
+                status = orc_pcg_standard_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                             o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                             &err, &niters);
+            if (status) break;
+            o->total_linear_iters += niters;
+        }
         orc_halo(&g2, 1, u2); orc_halo(&g2, 1, v2);
         /* compute_residual_velocity_2d: the 2D field has no separate bed level */
         resid_velo = orc_residual_velocity_3d(&g2, o->whichresid == 1 ? 0 : o->whichresid, u2, v2, us, vs);
@@ -1720,5 +1730,621 @@ int orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
     o->status = status;
     free(Auu); free(Auv); free(Avu); free(Avv); free(u2); free(v2); free(us); free(vs);
     free(l2u); free(l2v); free(um2); free(vm2); free(beta_work);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Depth-integrated 2D solvers: L1L2 (which_ho_approx = 3) and DIVA (= 4)   */
+/* (velo_higher:1117-1173, 2703-2830, 3655-3700, 3773-3830, 5208-6356,      */
+/* 7382-7864; glissade_grid_operators.F90:60-410).                          */
+/* ------------------------------------------------------------------------ */
+
+/* glissade_stagger (glissade_grid_operators.F90:60-148): margin 0 averages the four
+ * cells around each vertex, margin 1 only those with mask = 1 (zero where none has) */
+void orc_stagger(int nx, int ny, const double *var, double *stagvar, const int32_t *mask, int margin)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            double s = 0.0;
+            if (margin == 0) {
+                s = (C2(var, i, j + 1) + C2(var, i + 1, j + 1) + C2(var, i, j) + C2(var, i + 1, j)) / 4.0;
+            } else {
+                const double sv = (double)C2(mask, i, j + 1) * C2(var, i, j + 1) +
+                                  (double)C2(mask, i + 1, j + 1) * C2(var, i + 1, j + 1) +
+                                  (double)C2(mask, i, j) * C2(var, i, j) +
+                                  (double)C2(mask, i + 1, j) * C2(var, i + 1, j);
+                const double sm = (double)(C2(mask, i, j + 1) + C2(mask, i + 1, j + 1) + C2(mask, i, j) +
+                                           C2(mask, i + 1, j));
+                if (sm > 0.0) s = sv / sm;
+            }
+            S2(stagvar, i, j) = s;
+        }
+}
+
+/* glissade_gradient (glissade_grid_operators.F90:235-410): the vertex gradient is the
+ * mean of the two adjacent edge differences; with margin 1 an edge counts only when
+ * both its cells have mask = 1 */
+void orc_gradient(int nx, int ny, double dx, double dy, const double *f, double *dfdx, double *dfdy,
+                  const int32_t *mask, int margin)
+{
+#define EX(i, j) (margin == 0 || (C2(mask, i, j) == 1 && C2(mask, (i) + 1, j) == 1))
+#define EY(i, j) (margin == 0 || (C2(mask, i, j) == 1 && C2(mask, i, (j) + 1) == 1))
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const double xn = (C2(f, i + 1, j + 1) - C2(f, i, j + 1)) / dx;
+            const double xs = (C2(f, i + 1, j) - C2(f, i, j)) / dx;
+            double gx = 0.0, gy = 0.0;
+            if (EX(i, j) && EX(i, j + 1)) gx = (xn + xs) / 2.0;
+            else if (EX(i, j)) gx = xs;
+            else if (EX(i, j + 1)) gx = xn;
+            const double ye = (C2(f, i + 1, j + 1) - C2(f, i + 1, j)) / dy;
+            const double yw = (C2(f, i, j + 1) - C2(f, i, j)) / dy;
+            if (EY(i, j) && EY(i + 1, j)) gy = (ye + yw) / 2.0;
+            else if (EY(i, j)) gy = yw;
+            else if (EY(i + 1, j)) gy = ye;
+            S2(dfdx, i, j) = gx;
+            S2(dfdy, i, j) = gy;
+        }
+#undef EX
+#undef EY
+}
+
+/* get_basis_function_derivatives_2d (velo_higher:6583-6751) for reference-element
+ * derivatives dxr, dyr; returns nonzero for a singular Jacobian */
+static int basis_2d(const double *x, const double *y, const double *dxr, const double *dyr,
+                    double *dx, double *dy, double *detJ)
+{
+    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+    for (int n = 0; n < 4; ++n) {
+        J11 = J11 + dxr[n] * x[n];
+        J12 = J12 + dxr[n] * y[n];
+        J21 = J21 + dyr[n] * x[n];
+        J22 = J22 + dyr[n] * y[n];
+    }
+    const double det = J11 * J22 - J12 * J21;
+    *detJ = det;
+    if (!(fabs(det) > 0.0)) return 1;
+    const double i11 = J22 / det, i12 = -J12 / det, i21 = -J21 / det, i22 = J11 / det;
+    for (int n = 0; n < 4; ++n) {
+        dx[n] = i11 * dxr[n] + i12 * dyr[n];
+        dy[n] = i21 * dxr[n] + i22 * dyr[n];
+    }
+    return 0;
+}
+
+/* reference-element tables of the 2D quadrature point p (0..3) or, for p = 4, the cell
+ * center (velo_higher:560-600) */
+static void tables_2d(int p, double *phi, double *dxr, double *dyr)
+{
+    const double r = 1.0 / sqrt(3.0);
+    const double xq = p == 4 ? 0.0 : ((p == 0 || p == 3) ? -r : r);
+    const double yq = p == 4 ? 0.0 : (p < 2 ? -r : r);
+    dxr[0] = -(1.0 - yq) / 4.0; dxr[1] = (1.0 - yq) / 4.0;
+    dxr[2] = (1.0 + yq) / 4.0;  dxr[3] = -(1.0 + yq) / 4.0;
+    dyr[0] = -(1.0 - xq) / 4.0; dyr[1] = -(1.0 + xq) / 4.0;
+    dyr[2] = (1.0 + xq) / 4.0;  dyr[3] = (1.0 - xq) / 4.0;
+    phi[0] = (1.0 - xq) * (1.0 - yq) / 4.0;
+    phi[1] = (1.0 + xq) * (1.0 - yq) / 4.0;
+    phi[2] = (1.0 + xq) * (1.0 + yq) / 4.0;
+    phi[3] = (1.0 - xq) * (1.0 + yq) / 4.0;
+}
+
+/* compute_effective_viscosity_L1L2 (velo_higher:7382-7618): the viscosity of every
+ * layer at one quadrature point.  Nonlinear case: tau_parallel is the real root of
+ * x^3 + a x + b = 0 (a = tau_perp^2, b = -effstrain/A, Cardano's form with the
+ * first-order branch for small a), efvs = 1 / (2 A (tau_parallel^2 + tau_perp^2)). */
+void orc_efvs_l1l2(int whichefvs, double efvs_constant, int nz3, const double *sigma,
+                   const double *phi, const double *dphi_dx, const double *dphi_dy,
+                   const double *u, const double *v, const double *h, const double *dsdx,
+                   const double *dsdy, const double *flwa, const double *flwafact, double *efvs)
+{
+    const double p_effstr = (1.0 - (double)GN) / (double)GN, emin = 1.0e-8;
+    if (whichefvs == 0) {
+        for (int k = 0; k < nz3 - 1; ++k) efvs[k] = efvs_constant;
+        return;
+    }
+    if (whichefvs == 1) {
+        const double effstrain = VEL_SCALE / LEN_SCALE * SCYR;
+        for (int k = 0; k < nz3 - 1; ++k) efvs[k] = flwafact[k] * pow(effstrain, p_effstr);
+        return;
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0, sx = 0, sy = 0, th = 0;
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dphi_dx[n] * u[n];
+        uy = uy + dphi_dy[n] * u[n];
+        vx = vx + dphi_dx[n] * v[n];
+        vy = vy + dphi_dy[n] * v[n];
+        sx = sx + phi[n] * dsdx[n];
+        sy = sy + phi[n] * dsdy[n];
+        th = th + phi[n] * h[n];
+    }
+    const double sq = emin * emin + ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
+    const double effstrain = sqrt(sq);
+    for (int k = 0; k < nz3 - 1; ++k) {
+        const double depth = th * sigma[k + 1];
+        const double grads = sqrt(sx * sx + sy * sy);
+        const double tau_perp = RHOI * GRAV * depth * grads;
+        const double a = tau_perp * tau_perp;
+        const double b = -effstrain / flwa[k];
+        const double c = sqrt(b * b / 4.0 + a * a * a / 27.0);
+        const double rootA = pow(-b / 2.0 + c, 1.0 / 3.0);
+        double rootB;
+        if (a * a * a / 27.0 > 1.0e-6 * (b * b / 4.0))
+            rootB = -pow(b / 2.0 + c, 1.0 / 3.0);
+        else
+            rootB = -a / (3.0 * pow(fabs(b), 1.0 / 3.0));
+        const double tp = rootA + rootB;
+        efvs[k] = 1.0 / (2.0 * flwa[k] * (tp * tp + tau_perp * tau_perp));
+    }
+}
+
+/* compute_effective_viscosity_diva (velo_higher:7622-7864) with cubic = .false.: the
+ * vertical shear of each layer from the previous efvs (efvs in/out; a zero efvs starts
+ * from the minimum strain rate's) */
+void orc_efvs_diva(int whichefvs, double efvs_constant, int nz3, const double *stagsigma,
+                   const double *phi, const double *dphi_dx, const double *dphi_dy,
+                   const double *u, const double *v, const double *bx, const double *by,
+                   const double *flwafact, double *efvs)
+{
+    const double p_effstr = (1.0 - (double)GN) / (double)GN, p2_effstr = p_effstr / 2, emin = 1.0e-8;
+    if (whichefvs == 0) {
+        for (int k = 0; k < nz3 - 1; ++k) efvs[k] = efvs_constant;
+        return;
+    }
+    if (whichefvs == 1) {
+        const double effstrain = VEL_SCALE / LEN_SCALE * SCYR;
+        for (int k = 0; k < nz3 - 1; ++k) efvs[k] = flwafact[k] * pow(effstrain, p_effstr);
+        return;
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0, tx = 0, ty = 0;
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dphi_dx[n] * u[n];
+        uy = uy + dphi_dy[n] * u[n];
+        vx = vx + dphi_dx[n] * v[n];
+        vy = vy + dphi_dy[n] * v[n];
+        tx = tx + phi[n] * bx[n];
+        ty = ty + phi[n] * by[n];
+    }
+    for (int k = 0; k < nz3 - 1; ++k) {
+        if (efvs[k] == 0.0) efvs[k] = flwafact[k] * pow(emin, p_effstr);
+        const double uz = tx * stagsigma[k] / efvs[k];
+        const double vz = ty * stagsigma[k] / efvs[k];
+        const double sq = emin * emin + ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)) +
+                          0.25 * (uz * uz + vz * vz);
+        efvs[k] = flwafact[k] * pow(sq, p2_effstr);
+    }
+}
+
+/* compute_integrals_diva (velo_higher:5601-5726): omega_k(k) (nz3 levels) is the
+ * quadrature-point mean of the integral from the surface of depth/efvs over the layers
+ * below level k; omega the trapezoid integral of omega_k over sigma.  efvs_qp is
+ * (nz3-1, 4). */
+void orc_integrals_diva(int nz3, const double *sigma, double thck, const double *efvs_qp,
+                        double *omega_k, double *omega)
+{
+    double okp[64][4];
+    for (int p = 0; p < 4; ++p) {
+        okp[nz3 - 1][p] = 0.0;
+        for (int k = nz3 - 2; k >= 0; --k) {
+            const double depth = 0.5 * (sigma[k] + sigma[k + 1]);
+            const double dz = (sigma[k + 1] - sigma[k]) * thck;
+            okp[k][p] = okp[k + 1][p] + depth / efvs_qp[k + (size_t)(nz3 - 1) * p] * dz;
+        }
+    }
+    for (int k = 0; k < nz3; ++k) omega_k[k] = (okp[k][0] + okp[k][1] + okp[k][2] + okp[k][3]) / 4;
+    double om = 0.0;
+    for (int k = 0; k < nz3 - 1; ++k) {
+        const double layer_avg = 0.5 * (omega_k[k] + omega_k[k + 1]);
+        om = om + layer_avg * (sigma[k + 1] - sigma[k]);
+    }
+    *omega = om;
+}
+
+/* assemble_stiffness_matrix_2d, L1L2 and DIVA branches (velo_higher:5208-5595).
+ * g: the one-level grid.  Per active cell and quadrature point the layer viscosities,
+ * their sigma-weighted column mean, and the element matrix of thickness x that mean
+ * (compute_element_matrix's Blatter-Pattyn form with dphi_dz = 0); efvs (nz3-1, nx, ny)
+ * is the quadrature-point mean.  DIVA: efvs_qp3 (nz3-1, 4, nx, ny) in/out, omega_k
+ * (nz3, nx, ny) and omega (nx, ny) out for active cells. */
+int orc_assemble_2d_depth(const orc_grid *g, int whichapprox, int nz3, const double *sigma,
+                          const int32_t *active_cell, const double *xV, const double *yV,
+                          const double *uvel, const double *vvel, const double *stagthck,
+                          const double *dusrf_dx, const double *dusrf_dy, const double *thck,
+                          const double *flwa, const double *flwafact, const double *btractx,
+                          const double *btracty, int whichefvs, double efvs_constant, double *efvs,
+                          double *efvs_qp3, double *omega_k, double *omega,
+                          double *Auu, double *Auv, double *Avu, double *Avv)
+{
+    const int nx = g->nx, ny = g->ny, h = g->nhalo;
+    const int nz = 1;
+    const int L = nz3 - 1;
+    const size_t na = 27 * (size_t)(nx - 1) * (ny - 1);
+    memset(Auu, 0, na * 8); memset(Auv, 0, na * 8); memset(Avu, 0, na * 8); memset(Avv, 0, na * 8);
+    memset(efvs, 0, (size_t)L * nx * ny * 8);
+    const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+    double stagsigma[64];
+    for (int k = 0; k < L; ++k) stagsigma[k] = (sigma[k] + sigma[k + 1]) / 2.0;
+    int status = ORC_OK;
+    for (int j = h + 1; j <= ny - h + 1 && !status; ++j)
+        for (int i = h + 1; i <= nx - h + 1 && !status; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            double x[4], y[4], u[4], v[4], hn[4], sx[4] = {0}, sy[4] = {0}, bx[4] = {0}, by[4] = {0};
+            for (int n = 0; n < 4; ++n) {
+                const int iv = i + di[n], jv = j + dj[n];
+                x[n] = S2(xV, iv, jv);
+                y[n] = S2(yV, iv, jv);
+                u[n] = S2(uvel, iv, jv);
+                v[n] = S2(vvel, iv, jv);
+                hn[n] = S2(stagthck, iv, jv);
+                if (whichapprox == 3) { sx[n] = S2(dusrf_dx, iv, jv); sy[n] = S2(dusrf_dy, iv, jv); }
+                if (whichapprox == 4) { bx[n] = S2(btractx, iv, jv); by[n] = S2(btracty, iv, jv); }
+            }
+            const size_t cell = (i - 1) + (size_t)nx * (j - 1);
+            const double *fl = flwa + (size_t)L * cell, *ff = flwafact + (size_t)L * cell;
+            double eqp[64 * 4];
+            if (whichapprox == 4) memcpy(eqp, efvs_qp3 + (size_t)L * 4 * cell, sizeof(double) * L * 4);
+            double Kuu[16] = {0}, Kuv[16] = {0}, Kvu[16] = {0}, Kvv[16] = {0};
+            for (int p = 0; p < 4; ++p) {
+                double phi[4], dxr[4], dyr[4], dx[4], dy[4], det;
+                tables_2d(p, phi, dxr, dyr);
+                if (basis_2d(x, y, dxr, dyr, dx, dy, &det)) { status = ORC_ERR_SINGULAR; break; }
+                double *e = eqp + (size_t)L * p;
+                if (whichapprox == 3)
+                    orc_efvs_l1l2(whichefvs, efvs_constant, nz3, sigma, phi, dx, dy, u, v, hn, sx, sy, fl, ff, e);
+                else
+                    orc_efvs_diva(whichefvs, efvs_constant, nz3, stagsigma, phi, dx, dy, u, v, bx, by, ff, e);
+                double avg = 0.0;
+                for (int k = 0; k < L; ++k) avg = avg + e[k] * (sigma[k + 1] - sigma[k]);
+                double hq = 0.0;
+                for (int n = 0; n < 4; ++n) hq = hq + phi[n] * hn[n];
+                const double f = hq * avg * 1.0 * det / VOL0;
+                for (int nc = 0; nc < 4; ++nc)
+                    for (int nr = 0; nr < 4; ++nr) {
+                        const int q = nr + 4 * nc;
+                        Kuu[q] = Kuu[q] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc] + 0.0 * 0.0);
+                        Kuv[q] = Kuv[q] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                        Kvu[q] = Kvu[q] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                        Kvv[q] = Kvv[q] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc] + 0.0 * 0.0);
+                    }
+            }
+            if (status) break;
+            if (whichapprox == 4) {
+                memcpy(efvs_qp3 + (size_t)L * 4 * cell, eqp, sizeof(double) * L * 4);
+                orc_integrals_diva(nz3, sigma, C2(thck, i, j), eqp, omega_k + (size_t)nz3 * cell,
+                                   &C2(omega, i, j));
+            }
+            for (int p = 0; p < 4; ++p)
+                for (int k = 0; k < L; ++k) efvs[k + (size_t)L * cell] = efvs[k + (size_t)L * cell] + eqp[k + (size_t)L * p];
+            for (int k = 0; k < L; ++k) efvs[k + (size_t)L * cell] = efvs[k + (size_t)L * cell] / 4;
+            for (int a = 0; a < 4 && !status; ++a)
+                for (int b = 0; b < 4; ++b)
+                    if (fabs(Kuu[a + 4 * b] - Kuu[b + 4 * a]) > 1.0e-10 ||
+                        fabs(Kvv[a + 4 * b] - Kvv[b + 4 * a]) > 1.0e-10 ||
+                        fabs(Kuv[a + 4 * b] - Kvu[b + 4 * a]) > 1.0e-10) {
+                        status = ORC_ERR_ASYMMETRIC;
+                        break;
+                    }
+            for (int nr = 0; nr < 4; ++nr) {
+                const int iv = i + di[nr], jv = j + dj[nr];
+                for (int nc = 0; nc < 4; ++nc) {
+                    const int m = IDX_A(di[nc] - di[nr], dj[nc] - dj[nr], 0);
+                    const int q = nr + 4 * nc;
+                    A4(Auu, m, 1, iv, jv) = A4(Auu, m, 1, iv, jv) + Kuu[q];
+                    A4(Auv, m, 1, iv, jv) = A4(Auv, m, 1, iv, jv) + Kuv[q];
+                    A4(Avu, m, 1, iv, jv) = A4(Avu, m, 1, iv, jv) + Kvu[q];
+                    A4(Avv, m, 1, iv, jv) = A4(Avv, m, 1, iv, jv) + Kvv[q];
+                }
+            }
+        }
+    return status;
+}
+
+/* compute_3d_velocity_L1L2 (velo_higher:5834-6356), method 1 (edge_velocity = .false.):
+ * g: the 3D grid (nz levels).  On input uvel/vvel hold the basal (2D) velocity at level
+ * nz; the levels above are integrated from the bed with the vertical shear stresses
+ * tau_xz, tau_yz (the SIA stress plus centered differences of the depth-integrated
+ * membrane stresses) and tau_eff^2 = tau_parallel^2 + tau_xz^2 + tau_yz^2 on owned
+ * active vertices; every other vertex keeps zero above the bed.  umask2/vmask2 are the
+ * bed level's Dirichlet masks ((nx-1) x (ny-1)); no halo update (the caller's). */
+int orc_velocity_3d_l1l2(const orc_grid *g, const double *sigma, double dx, double dy,
+                         const int32_t *ice_mask, const int32_t *active_cell,
+                         const int32_t *active_vertex, const int32_t *umask2, const int32_t *vmask2,
+                         const double *xV, const double *yV, const double *thck, const double *stagthck,
+                         const double *dusrf_dx, const double *dusrf_dy, const double *flwa,
+                         const double *efvs, double *uvel, double *vvel)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, h = g->nhalo;
+    const size_t nc = (size_t)nx * ny, nv = (size_t)(nx - 1) * (ny - 1);
+    double *ux = calloc(nc, 8), *uy = calloc(nc, 8), *vx = calloc(nc, 8), *vy = calloc(nc, 8);
+    double *tpar = calloc((nz - 1) * nc, 8), *eint = calloc((nz - 1) * nc, 8);
+    double *txz = calloc((nz - 1) * nv, 8), *tyz = calloc((nz - 1) * nv, 8);
+    double *w1 = malloc(nc * 8), *w2 = malloc(nc * 8), *w3 = malloc(nc * 8);
+    double *g1x = malloc(nv * 8), *g1y = malloc(nv * 8), *g2x = malloc(nv * 8), *g2y = malloc(nv * 8);
+    double *g3x = malloc(nv * 8), *g3y = malloc(nv * 8), *st = malloc(nv * 8), *sf = malloc(nv * 8);
+    double *wk = malloc(nc * 8);
+    const int di[4] = {-1, 0, 0, -1}, dj[4] = {-1, -1, 0, 0};
+    int status = ORC_OK;
+    for (int j = h + 1; j <= ny - h + 1 && !status; ++j)
+        for (int i = h + 1; i <= nx - h + 1; ++i) {
+            if (!C2(active_cell, i, j)) continue;
+            double x[4], y[4], u[4], v[4], phi[4], dxr[4], dyr[4], ddx[4], ddy[4], det;
+            for (int n = 0; n < 4; ++n) {
+                x[n] = S2(xV, i + di[n], j + dj[n]);
+                y[n] = S2(yV, i + di[n], j + dj[n]);
+                u[n] = V3(uvel, nz, i + di[n], j + dj[n]);
+                v[n] = V3(vvel, nz, i + di[n], j + dj[n]);
+            }
+            tables_2d(4, phi, dxr, dyr);
+            if (basis_2d(x, y, dxr, dyr, ddx, ddy, &det)) { status = ORC_ERR_SINGULAR; break; }
+            for (int n = 0; n < 4; ++n) {
+                C2(ux, i, j) = C2(ux, i, j) + ddx[n] * u[n];
+                C2(uy, i, j) = C2(uy, i, j) + ddy[n] * u[n];
+                C2(vx, i, j) = C2(vx, i, j) + ddx[n] * v[n];
+                C2(vy, i, j) = C2(vy, i, j) + ddy[n] * v[n];
+            }
+            const double a = C2(ux, i, j), b = C2(vy, i, j), t = C2(vx, i, j) + C2(uy, i, j);
+            const double eps = sqrt(a * a + b * b + a * b + 0.25 * (t * t));
+            for (int k = 1; k <= nz - 1; ++k) F3(tpar, k, i, j) = 2.0 * F3(efvs, k, i, j) * eps;
+            F3(eint, 1, i, j) = F3(efvs, 1, i, j) * (sigma[1] - sigma[0]) * C2(thck, i, j);
+            for (int k = 2; k <= nz - 1; ++k)
+                F3(eint, k, i, j) = F3(eint, k - 1, i, j) + F3(efvs, k, i, j) * (sigma[k] - sigma[k - 1]) * C2(thck, i, j);
+        }
+    if (!status) {
+        for (int k = 1; k <= nz - 1; ++k) {
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    const double e = F3(eint, k, i, j);
+                    C2(w1, i, j) = e * (2.0 * C2(ux, i, j) + C2(vy, i, j));
+                    C2(w2, i, j) = e * (C2(uy, i, j) + C2(vx, i, j));
+                    C2(w3, i, j) = e * (2.0 * C2(vy, i, j) + C2(ux, i, j));
+                }
+            orc_gradient(nx, ny, dx, dy, w1, g1x, g1y, ice_mask, 1);
+            orc_gradient(nx, ny, dx, dy, w2, g2x, g2y, ice_mask, 1);
+            orc_gradient(nx, ny, dx, dy, w3, g3x, g3y, ice_mask, 1);
+            for (int j = g->jlo; j <= g->jhi; ++j)
+                for (int i = g->ilo; i <= g->ihi; ++i) {
+                    if (!S2(active_vertex, i, j)) continue;
+                    const double depth = 0.5 * (sigma[k - 1] + sigma[k]) * S2(stagthck, i, j);
+                    const size_t q = (k - 1) + (size_t)(nz - 1) * ((i - 1) + (size_t)(nx - 1) * (j - 1));
+                    txz[q] = -(RHOI * GRAV * depth * S2(dusrf_dx, i, j)) + 2.0 * S2(g1x, i, j) + S2(g2y, i, j);
+                    tyz[q] = -(RHOI * GRAV * depth * S2(dusrf_dy, i, j)) + S2(g2x, i, j) + 2.0 * S2(g3y, i, j);
+                }
+        }
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i)
+                for (int k = 1; k <= nz - 1; ++k) { V3(uvel, k, i, j) = 0.0; V3(vvel, k, i, j) = 0.0; }
+        for (int k = nz - 1; k >= 1; --k) {
+            for (size_t c = 0; c < nc; ++c) wk[c] = tpar[(k - 1) + (size_t)(nz - 1) * c];
+            orc_stagger(nx, ny, wk, st, ice_mask, 1);
+            for (size_t c = 0; c < nv; ++c) st[c] = st[c] * st[c];
+            for (size_t c = 0; c < nc; ++c) wk[c] = flwa[(k - 1) + (size_t)(nz - 1) * c];
+            orc_stagger(nx, ny, wk, sf, ice_mask, 1);
+            for (int j = g->jlo; j <= g->jhi; ++j)
+                for (int i = g->ilo; i <= g->ihi; ++i) {
+                    if (!S2(active_vertex, i, j)) continue;
+                    const size_t q = (k - 1) + (size_t)(nz - 1) * ((i - 1) + (size_t)(nx - 1) * (j - 1));
+                    const double tes = S2(st, i, j) + txz[q] * txz[q] + tyz[q] * tyz[q];
+                    /* tau_eff_sq**((gn-1)/2) with gn = 3 is tau_eff_sq itself */
+                    const double fact = 2.0 * S2(sf, i, j) * tes * (sigma[k] - sigma[k - 1]) * S2(stagthck, i, j);
+                    V3(uvel, k, i, j) = S2(umask2, i, j) == 1 ? V3(uvel, nz, i, j) : V3(uvel, k + 1, i, j) + fact * txz[q];
+                    V3(vvel, k, i, j) = S2(vmask2, i, j) == 1 ? V3(vvel, nz, i, j) : V3(vvel, k + 1, i, j) + fact * tyz[q];
+                }
+        }
+    }
+    free(ux); free(uy); free(vx); free(vy); free(tpar); free(eint); free(txz); free(tyz);
+    free(w1); free(w2); free(w3); free(g1x); free(g1y); free(g2x); free(g2y); free(g3x); free(g3y);
+    free(st); free(sf); free(wk);
+    return status;
+}
+
+/* compute_3d_velocity_diva (velo_higher:5730-5830), diva_level_index = 0: on active
+ * vertices the bed velocity u_b = u_mean - tau_b stag_omega and u(k) = u_b + tau_b
+ * stag_omega_k(k) (stag_omega_k is (nz, nx-1, ny-1)) */
+void orc_velocity_3d_diva(const orc_grid *g, const int32_t *active_vertex, const double *stag_omega_k,
+                          const double *stag_omega, const double *btx, const double *bty,
+                          const double *u2, const double *v2, double *uvel, double *vvel)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!S2(active_vertex, i, j)) continue;
+            V3(uvel, nz, i, j) = S2(u2, i, j) - S2(btx, i, j) * S2(stag_omega, i, j);
+            V3(vvel, nz, i, j) = S2(v2, i, j) - S2(bty, i, j) * S2(stag_omega, i, j);
+            for (int k = 1; k <= nz - 1; ++k) {
+                V3(uvel, k, i, j) = V3(uvel, nz, i, j) + S2(btx, i, j) * V3(stag_omega_k, k, i, j);
+                V3(vvel, k, i, j) = V3(vvel, nz, i, j) + S2(bty, i, j) * V3(stag_omega_k, k, i, j);
+            }
+        }
+}
+
+/* The Picard loop of glissade_velo_higher_solve for which_ho_approx = 3 (L1L2) and 4
+ * (DIVA) (velo_higher:1117-1173, 1274-1324, 2206-3767 with solve_2d).  g: the 3D grid.
+ *   L1L2: the 2D velocity starts from the bed level of uvel/vvel; beta_internal is the
+ *         basal friction; after the loop the 3D velocity is integrated from the bed
+ *         (compute_3d_velocity_L1L2) and halo-updated.  With a no-slip bed the velocity
+ *         is zero and the loop ends at once (velo_higher:3234-3245).
+ *   DIVA: the 2D velocity is the depth-mean velocity, from df->uvel_2d/vvel_2d; the
+ *         layer viscosities of each quadrature point start from efvs (in/out); after
+ *         each assembly omega (halo-updated) is staggered over df->ice_plus_land_mask and
+ *         the basal friction is beta_eff = beta / (1 + beta omega) (1/omega under
+ *         no-slip); after each solve the basal traction beta_eff u and the 3D velocity
+ *         (compute_3d_velocity_diva) follow.
+ * The 2D load is the column sum of loadu/loadv; the Dirichlet masks are the bed level's.
+ * Outputs: uvel/vvel (3D), df->uvel_2d/vvel_2d, df->btractx/btracty (DIVA), efvs (the
+ * layer viscosities of the last assembly), bu/bv/resid_u/resid_v one-level. */
+int orc_picard_solve_depth(const orc_grid *g, orc_picard_opts *o, const orc_depth_fields *df,
+                           const double *sigma, const int32_t *active_cell,
+                           const int32_t *active_vertex, const double *xV, const double *yV,
+                           const double *stagthck, const double *flwafact, const double *beta_internal,
+                           const int32_t *umask, const int32_t *vmask,
+                           const double *loadu, const double *loadv,
+                           double *uvel, double *vvel, double *efvs,
+                           double *bu, double *bv, double *resid_u, double *resid_v)
+{
+    const int nx = g->nx, ny = g->ny, nz = g->nz, L = nz - 1;
+    const int approx = o->whichapprox;
+    if ((approx != 3 && approx != 4) || nz < 2 || nz > 64) return ORC_ERR_ARG;
+    orc_grid g2;
+    orc_grid_init(&g2, nx, ny, 1, g->periodic);
+    const size_t nv = (size_t)(nx - 1) * (ny - 1), nc = (size_t)nx * ny, na = 27 * nv;
+    double *Auu = malloc(na * 8), *Auv = malloc(na * 8), *Avu = malloc(na * 8), *Avv = malloc(na * 8);
+    double *u2 = malloc(nv * 8), *v2 = malloc(nv * 8), *us = malloc(nv * 8), *vs = malloc(nv * 8);
+    double *l2u = calloc(nv, 8), *l2v = calloc(nv, 8);
+    int32_t *um2 = malloc(nv * 4), *vm2 = malloc(nv * 4);
+    double *eqp = NULL, *omk = NULL, *om = NULL, *som = NULL, *somk = NULL, *beff = NULL, *wk = NULL;
+    double *s1 = NULL;
+    if (approx == 4) {
+        s1 = malloc(nv * 8);
+        eqp = malloc((size_t)L * 4 * nc * 8);
+        omk = calloc((size_t)nz * nc, 8);
+        om = calloc(nc, 8);
+        som = calloc(nv, 8);
+        somk = calloc((size_t)nz * nv, 8);
+        beff = calloc(nv, 8);
+        wk = malloc(nc * 8);
+        for (size_t c = 0; c < nc; ++c)
+            for (int p = 0; p < 4; ++p)
+                memcpy(eqp + (size_t)L * (p + 4 * c), efvs + (size_t)L * c, sizeof(double) * L);
+    }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const size_t c = (i - 1) + (size_t)(nx - 1) * (j - 1);
+            u2[c] = approx == 4 ? df->uvel_2d[c] : V3(uvel, nz, i, j);
+            v2[c] = approx == 4 ? df->vvel_2d[c] : V3(vvel, nz, i, j);
+            um2[c] = V3(umask, nz, i, j);
+            vm2[c] = V3(vmask, nz, i, j);
+            for (int k = 1; k <= nz; ++k) {
+                l2u[c] = l2u[c] + V3(loadu, k, i, j);
+                l2v[c] = l2v[c] + V3(loadv, k, i, j);
+            }
+        }
+    /* velocity of inactive owned vertices is zero (velo_higher:1960-1972) */
+    for (int j = g->jlo; j <= g->jhi; ++j)
+        for (int i = g->ilo; i <= g->ihi; ++i)
+            if (!S2(active_vertex, i, j)) {
+                S2(u2, i, j) = 0.0;
+                S2(v2, i, j) = 0.0;
+                for (int k = 1; k <= nz; ++k) { V3(uvel, k, i, j) = 0.0; V3(vvel, k, i, j) = 0.0; }
+            }
+    const double L2_target = 1.0e-4, L2_target_rel = 1.0e-7, resid_target = 1.0e-4;
+    double crit = 1.0e10, target = 1.0e-12;
+    double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
+    int counter = 0, status = ORC_OK;
+    double *beta_work = o->bp ? malloc(nv * 8) : NULL;
+    double *ub = o->bp ? malloc(nv * 8) : NULL, *vb = o->bp ? malloc(nv * 8) : NULL;
+    const int noslip = o->whichbabc == 6;
+    o->total_linear_iters = 0;
+    while (crit >= target && counter < o->maxiter_nonlinear) {
+        counter++;
+        memcpy(us, u2, nv * 8); memcpy(vs, v2, nv * 8);
+        const double *beta = beta_internal;
+        if (o->bp) {   /* the basal velocity: the 2D one (L1L2) or the bed level (DIVA), :2237-2243 */
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i) {
+                    S2(ub, i, j) = approx == 4 ? V3(uvel, nz, i, j) : S2(u2, i, j);
+                    S2(vb, i, j) = approx == 4 ? V3(vvel, nz, i, j) : S2(v2, i, j);
+                }
+            status = orc_calcbeta(g, o->bp, ub, vb, o->beta_external, o->effecpress_stag, o->f_ground,
+                                  beta_work);
+            if (status) break;
+            beta = beta_work;
+        }
+        status = orc_assemble_2d_depth(&g2, approx, nz, sigma, active_cell, xV, yV, u2, v2, stagthck,
+                                       df->dusrf_dx, df->dusrf_dy, df->thck, df->flwa, flwafact,
+                                       df->btractx, df->btracty, o->whichefvs, o->efvs_constant, efvs,
+                                       eqp, omk, om, Auu, Auv, Avu, Avv);
+        if (status) break;
+        if (approx == 4) {
+            orc_cell_halo(g, om);
+            orc_stagger(nx, ny, om, som, df->ice_plus_land_mask, 1);
+            for (size_t c = 0; c < nv; ++c) {
+                if (noslip) beff[c] = som[c] > 0.0 ? 1.0 / som[c] : 0.0;
+                else beff[c] = beta[c] / (1.0 + beta[c] * som[c]);
+            }
+            beta = beff;
+        }
+        /* the 2D branch applies the basal friction for every which_ho_babc (:2832-2852) */
+        if (o->assemble_beta_standard)
+            orc_basal_sliding_standard(&g2, active_cell, xV, yV, beta, Auu, Avv);
+        else
+            orc_basal_sliding_local(&g2, o->dx, o->dy, active_vertex, beta, Auu, Avv);
+        memcpy(bu, l2u, nv * 8); memcpy(bv, l2v, nv * 8);
+        orc_dirichlet_3d(&g2, active_vertex, um2, vm2, u2, v2, Auu, Auv, Avu, Avv, bu, bv);
+        orc_halo(&g2, 27, Auu); orc_halo(&g2, 27, Auv); orc_halo(&g2, 27, Avu); orc_halo(&g2, 27, Avv);
+        orc_halo(&g2, 1, bu); orc_halo(&g2, 1, bv);
+        orc_check_symmetry_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, NULL);
+        long nnz = orc_count_nonzeros_3d(&g2, active_vertex, Auu, Auv, Avu, Avv);
+        if (nnz == 0) {
+            memset(resid_u, 0, nv * 8); memset(resid_v, 0, nv * 8);
+            memset(bu, 0, nv * 8); memset(bv, 0, nv * 8);
+            memset(u2, 0, nv * 8); memset(v2, 0, nv * 8);
+            L2 = 0.0;
+            break;
+        }
+        if (approx == 3 && noslip) {
+            memset(u2, 0, nv * 8); memset(v2, 0, nv * 8);
+            memset(resid_u, 0, nv * 8); memset(resid_v, 0, nv * 8);
+            L2 = 0.0;
+            L2rel = 0.0;
+        } else {
+            orc_residual_vector_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2, resid_u, resid_v,
+                                   &L2, &L2rel);
+            if (o->L2_history && counter <= o->L2_history_cap) o->L2_history[counter - 1] = L2;
+            double err; int niters;
+            if (o->whichsparse == 3)
+                status = orc_pcg_chrongear_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                              o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                              &err, &niters);
+            else
+                status = orc_pcg_standard_3d(&g2, active_vertex, Auu, Auv, Avu, Avv, bu, bv, u2, v2,
+                                             o->whichprecond, o->linear_maxiters, o->linear_tolerance,
+                                             &err, &niters);
+            if (status) break;
+            o->total_linear_iters += niters;
+        }
+        orc_halo(&g2, 1, u2); orc_halo(&g2, 1, v2);
+        resid_velo = orc_residual_velocity_3d(&g2, o->whichresid == 1 ? 0 : o->whichresid, u2, v2, us, vs);
+        if (approx == 4) {
+            /* basal traction (Goldberg 2011 eq. 38-39) and the 3D velocity (velo_higher:3661-3700) */
+            for (size_t c = 0; c < nv; ++c) {
+                df->btractx[c] = beff[c] * u2[c];
+                df->btracty[c] = beff[c] * v2[c];
+            }
+            for (int k = 1; k <= nz; ++k) {
+                for (size_t c = 0; c < nc; ++c) wk[c] = omk[(k - 1) + (size_t)nz * c];
+                orc_stagger(nx, ny, wk, s1, df->ice_plus_land_mask, 1);
+                for (size_t c = 0; c < nv; ++c) somk[(k - 1) + (size_t)nz * c] = s1[c];
+            }
+            orc_velocity_3d_diva(g, active_vertex, somk, som, df->btractx, df->btracty, u2, v2, uvel, vvel);
+            orc_halo(g, nz, uvel);
+            orc_halo(g, nz, vvel);
+        }
+        if (o->whichresid == 3) { crit = L2; target = L2_target; }
+        else if (o->whichresid == 4) { crit = L2rel; target = L2_target_rel; }
+        else { crit = resid_velo; target = resid_target; }
+        if (o->target_override > 0) target = o->target_override;
+    }
+    o->converged = counter < o->maxiter_nonlinear;
+    memcpy(df->uvel_2d, u2, nv * 8);
+    memcpy(df->vvel_2d, v2, nv * 8);
+    if (approx == 3 && !status) {
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i) {
+                V3(uvel, nz, i, j) = S2(u2, i, j);
+                V3(vvel, nz, i, j) = S2(v2, i, j);
+            }
+        status = orc_velocity_3d_l1l2(g, sigma, o->dx, o->dy, df->ice_mask, active_cell, active_vertex, um2,
+                                      vm2, xV, yV, df->thck, stagthck, df->dusrf_dx, df->dusrf_dy, df->flwa,
+                                      efvs, uvel, vvel);
+        orc_halo(g, nz, uvel);
+        orc_halo(g, nz, vvel);
+    }
+    o->picard_iters = counter;
+    o->final_L2_norm = L2;
+    o->final_resid_velo = resid_velo;
+    o->status = status;
+    free(Auu); free(Auv); free(Avu); free(Avv); free(u2); free(v2); free(us); free(vs);
+    free(l2u); free(l2v); free(um2); free(vm2); free(beta_work); free(ub); free(vb);
+    free(eqp); free(omk); free(om); free(som); free(somk); free(beff); free(wk); free(s1);
     return status;
 }

@@ -246,6 +246,59 @@ int  orc_picard_solve_2d(const orc_grid *g, orc_picard_opts *o,
                          double *uvel, double *vvel, double *efvs,
                          double *bu, double *bv, double *resid_u, double *resid_v);
 
+/* Depth-integrated 2D solvers, L1L2 (which_ho_approx = 3) and DIVA (= 4)
+ * (velo_higher:5208-6356, 7382-7864; glissade_grid_operators.F90:60-410) */
+void orc_stagger(int nx, int ny, const double *var, double *stagvar, const int32_t *mask, int margin);
+void orc_gradient(int nx, int ny, double dx, double dy, const double *f, double *dfdx, double *dfdy,
+                  const int32_t *mask, int margin);
+void orc_efvs_l1l2(int whichefvs, double efvs_constant, int nz3, const double *sigma,
+                   const double *phi, const double *dphi_dx, const double *dphi_dy,
+                   const double *u, const double *v, const double *h, const double *dsdx,
+                   const double *dsdy, const double *flwa, const double *flwafact, double *efvs);
+void orc_efvs_diva(int whichefvs, double efvs_constant, int nz3, const double *stagsigma,
+                   const double *phi, const double *dphi_dx, const double *dphi_dy,
+                   const double *u, const double *v, const double *bx, const double *by,
+                   const double *flwafact, double *efvs);
+void orc_integrals_diva(int nz3, const double *sigma, double thck, const double *efvs_qp,
+                        double *omega_k, double *omega);
+int  orc_assemble_2d_depth(const orc_grid *g, int whichapprox, int nz3, const double *sigma,
+                           const int32_t *active_cell, const double *xV, const double *yV,
+                           const double *uvel, const double *vvel, const double *stagthck,
+                           const double *dusrf_dx, const double *dusrf_dy, const double *thck,
+                           const double *flwa, const double *flwafact, const double *btractx,
+                           const double *btracty, int whichefvs, double efvs_constant, double *efvs,
+                           double *efvs_qp3, double *omega_k, double *omega,
+                           double *Auu, double *Auv, double *Avu, double *Avv);
+int  orc_velocity_3d_l1l2(const orc_grid *g, const double *sigma, double dx, double dy,
+                          const int32_t *ice_mask, const int32_t *active_cell,
+                          const int32_t *active_vertex, const int32_t *umask2, const int32_t *vmask2,
+                          const double *xV, const double *yV, const double *thck, const double *stagthck,
+                          const double *dusrf_dx, const double *dusrf_dy, const double *flwa,
+                          const double *efvs, double *uvel, double *vvel);
+void orc_velocity_3d_diva(const orc_grid *g, const int32_t *active_vertex, const double *stag_omega_k,
+                          const double *stag_omega, const double *btx, const double *bty,
+                          const double *u2, const double *v2, double *uvel, double *vvel);
+
+/* the fields the L1L2 and DIVA loops read beyond the SSA one's */
+typedef struct {
+    const double *thck;                 /* (nx, ny) m                                  */
+    const double *dusrf_dx, *dusrf_dy;  /* (nx-1, ny-1) L1L2                           */
+    const double *flwa;                 /* (nz-1, nx, ny) Pa^-3 yr^-1, L1L2             */
+    const int32_t *ice_mask;            /* (nx, ny) L1L2 gradients and staggering       */
+    const int32_t *ice_plus_land_mask;  /* (nx, ny) DIVA staggering of omega            */
+    double *uvel_2d, *vvel_2d;          /* (nx-1, ny-1): DIVA in (mean velocity); out   */
+    double *btractx, *btracty;          /* (nx-1, ny-1): DIVA in/out basal traction     */
+} orc_depth_fields;
+
+int  orc_picard_solve_depth(const orc_grid *g, orc_picard_opts *o, const orc_depth_fields *df,
+                            const double *sigma, const int32_t *active_cell,
+                            const int32_t *active_vertex, const double *xV, const double *yV,
+                            const double *stagthck, const double *flwafact, const double *beta_internal,
+                            const int32_t *umask, const int32_t *vmask,
+                            const double *loadu, const double *loadv,
+                            double *uvel, double *vvel, double *efvs,
+                            double *bu, double *bv, double *resid_u, double *resid_v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,14 @@ class PicardOpts(C.Structure):
                 ("assemble_beta_standard", C.c_int)]
 
 
+class DepthFields(C.Structure):
+    _fields_ = [("thck", C.POINTER(C.c_double)), ("dusrf_dx", C.POINTER(C.c_double)),
+                ("dusrf_dy", C.POINTER(C.c_double)), ("flwa", C.POINTER(C.c_double)),
+                ("ice_mask", C.POINTER(C.c_int32)), ("ice_plus_land_mask", C.POINTER(C.c_int32)),
+                ("uvel_2d", C.POINTER(C.c_double)), ("vvel_2d", C.POINTER(C.c_double)),
+                ("btractx", C.POINTER(C.c_double)), ("btracty", C.POINTER(C.c_double))]
+
+
 def _set_basal(o, basal, keep):
     """basal = (BasalPhysics, beta_external, effecpress_stag, f_ground) or None"""
     if basal is None:
@@ -137,6 +145,16 @@ def _declare(L):
     L.orc_picard_solve.argtypes = [G, C.POINTER(PicardOpts), D, I, I, D, D, D, D, D, D, I, I,
                                    D, D, D, D, D, D, D, D, D]
     L.orc_precond_apply.argtypes = [G, I, C.c_int, D, D, D, D, D, D]
+    L.orc_stagger.argtypes = [C.c_int, C.c_int, D, D, I, C.c_int]
+    L.orc_gradient.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, D, D, D, I, C.c_int]
+    L.orc_efvs_l1l2.argtypes = [C.c_int, C.c_double, C.c_int, D, D, D, D, D, D, D, D, D, D, D, D]
+    L.orc_efvs_diva.argtypes = [C.c_int, C.c_double, C.c_int, D, D, D, D, D, D, D, D, D, D]
+    L.orc_integrals_diva.argtypes = [C.c_int, D, C.c_double, D, D, D]
+    L.orc_velocity_3d_l1l2.argtypes = [G, D, C.c_double, C.c_double, I, I, I, I, I, D, D, D, D, D, D,
+                                       D, D, D, D]
+    L.orc_velocity_3d_diva.argtypes = [G, I, D, D, D, D, D, D, D, D]
+    L.orc_picard_solve_depth.argtypes = [G, C.POINTER(PicardOpts), C.POINTER(DepthFields), D, I, I, D,
+                                         D, D, D, D, I, I, D, D, D, D, D, D, D, D, D]
 
 
 def dp(a):
@@ -501,3 +519,131 @@ def basal_traction(p, uvel, vvel, beta=None):
     by = np.zeros((p.nx - 1, p.ny - 1), order="F")
     lib().orc_basal_traction(C.byref(g), dp(beta), dp(uvel), dp(vvel), dp(bx), dp(by))
     return bx, by
+
+
+# --- depth-integrated 2D solvers: L1L2 and DIVA (velo_higher:5208-6356, 7382-7864) ----
+def stagger(var, mask=None, margin=0):
+    """glissade_stagger of a cell field (nx, ny) to the vertices (nx-1, ny-1)"""
+    var = np.asfortranarray(var, dtype=np.float64)
+    nx, ny = var.shape
+    m = np.zeros((nx, ny), np.int32, order="F") if mask is None else np.asfortranarray(mask, dtype=np.int32)
+    out = np.zeros((nx - 1, ny - 1), order="F")
+    lib().orc_stagger(nx, ny, dp(var), dp(out), ip(m), margin)
+    return out
+
+
+def gradient(f, dx, dy, mask=None, margin=0):
+    """glissade_gradient of a cell field: (df_dx, df_dy) at the vertices"""
+    f = np.asfortranarray(f, dtype=np.float64)
+    nx, ny = f.shape
+    m = np.zeros((nx, ny), np.int32, order="F") if mask is None else np.asfortranarray(mask, dtype=np.int32)
+    gx, gy = np.zeros((nx - 1, ny - 1), order="F"), np.zeros((nx - 1, ny - 1), order="F")
+    lib().orc_gradient(nx, ny, dx, dy, dp(f), dp(gx), dp(gy), ip(m), margin)
+    return gx, gy
+
+
+def _v(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def efvs_l1l2(whichefvs, efvs_constant, sigma, phi, dphi_dx, dphi_dy, u, v, h, dsdx, dsdy, flwa, flwafact):
+    """compute_effective_viscosity_L1L2 at one quadrature point: the nz-1 layer values"""
+    sigma = _v(sigma)
+    out = np.zeros(len(sigma) - 1)
+    a = [_v(x) for x in (phi, dphi_dx, dphi_dy, u, v, h, dsdx, dsdy, flwa, flwafact)]
+    lib().orc_efvs_l1l2(whichefvs, efvs_constant, len(sigma), dp(sigma), *[dp(x) for x in a], dp(out))
+    return out
+
+
+def efvs_diva(whichefvs, efvs_constant, stagsigma, phi, dphi_dx, dphi_dy, u, v, bx, by, flwafact, efvs_old):
+    """compute_effective_viscosity_diva (cubic = .false.) at one quadrature point"""
+    stagsigma = _v(stagsigma)
+    out = _v(efvs_old).copy()
+    a = [_v(x) for x in (phi, dphi_dx, dphi_dy, u, v, bx, by, flwafact)]
+    lib().orc_efvs_diva(whichefvs, efvs_constant, len(stagsigma) + 1, dp(stagsigma), *[dp(x) for x in a],
+                        dp(out))
+    return out
+
+
+def integrals_diva(sigma, thck, efvs_qp):
+    """compute_integrals_diva: (omega_k (nz), omega); efvs_qp is (nz-1, 4)"""
+    sigma = _v(sigma)
+    e = np.asfortranarray(efvs_qp, dtype=np.float64)
+    ok = np.zeros(len(sigma))
+    om = np.zeros(1)
+    lib().orc_integrals_diva(len(sigma), dp(sigma), float(thck), dp(e), dp(ok), dp(om))
+    return ok, float(om[0])
+
+
+def velocity_3d_diva(p, stag_omega_k, stag_omega, btx, bty, u2, v2, uvel=None, vvel=None):
+    g = grid_of(p)
+    u = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F") if uvel is None else uvel.copy(order="F")
+    v = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F") if vvel is None else vvel.copy(order="F")
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)   # noqa: E731
+    lib().orc_velocity_3d_diva(C.byref(g), ip(p.active_vertex), dp(f(stag_omega_k)), dp(f(stag_omega)),
+                               dp(f(btx)), dp(f(bty)), dp(f(u2)), dp(f(v2)), dp(u), dp(v))
+    return u, v
+
+
+def velocity_3d_l1l2(p, ubas, vbas, efvs, umask2=None, vmask2=None):
+    """compute_3d_velocity_L1L2 from the basal velocity (nx-1, ny-1) and the layer
+    viscosities (nz-1, nx, ny); no halo update; (status, uvel, vvel)"""
+    g = grid_of(p)
+    u = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
+    v = np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F")
+    u[-1], v[-1] = ubas, vbas
+    z = np.zeros((p.nx - 1, p.ny - 1), np.int32, order="F")
+    um = z if umask2 is None else np.asfortranarray(umask2, dtype=np.int32)
+    vm = z if vmask2 is None else np.asfortranarray(vmask2, dtype=np.int32)
+    st = lib().orc_velocity_3d_l1l2(C.byref(g), dp(p.sigma), p.dx, p.dy, ip(p.ice_mask), ip(p.active_cell),
+                                    ip(p.active_vertex), ip(um), ip(vm), dp(p.xVertex), dp(p.yVertex),
+                                    dp(p.thck), dp(p.stagthck), dp(p.dusrf_dx), dp(p.dusrf_dy), dp(p.flwa),
+                                    dp(np.asfortranarray(efvs)), dp(u), dp(v))
+    return st, u, v
+
+
+def ice_plus_land(p):
+    return np.asfortranarray((p.ice_mask != 0) | (p.land_mask != 0), dtype=np.int32)
+
+
+def picard_depth(p, loadu, loadv, approx, maxiter=None, linear_maxiters=200, linear_tolerance=1e-8,
+                 whichsparse=None, whichprecond=None, whichresid=None, basal=None, uvel_2d=None,
+                 vvel_2d=None, btractx=None, btracty=None, efvs=None, target=0.0, assemble_beta=1):
+    """The L1L2 (approx 3) or DIVA (approx 4) branch of the Picard loop from p.uvel/p.vvel
+    (L1L2: their bed level; DIVA: uvel_2d/vvel_2d, btractx/btracty and efvs, zero by
+    default)"""
+    g = grid_of(p)
+    o = PicardOpts()
+    keep = []
+    _set_basal(o, basal, keep)
+    o.whichbabc = p.whichbabc
+    o.whichefvs = p.whichefvs
+    o.whichresid = p.whichresid if whichresid is None else whichresid
+    o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
+    o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+    o.whichapprox = approx
+    o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter is None else maxiter
+    o.linear_maxiters = linear_maxiters
+    o.efvs_constant = p.efvs_constant
+    o.linear_tolerance = linear_tolerance
+    o.dx, o.dy = p.dx, p.dy
+    o.target_override = target
+    o.assemble_beta_standard = 1 if assemble_beta == 0 else 0
+    s2 = (p.nx - 1, p.ny - 1)
+    z2 = lambda a: np.zeros(s2, order="F") if a is None else np.asfortranarray(a, dtype=np.float64).copy(order="F")  # noqa: E731
+    u2, v2, bx, by = z2(uvel_2d), z2(vvel_2d), z2(btractx), z2(btracty)
+    ipl = ice_plus_land(p)
+    df = DepthFields(thck=dp(p.thck), dusrf_dx=dp(p.dusrf_dx), dusrf_dy=dp(p.dusrf_dy), flwa=dp(p.flwa),
+                     ice_mask=ip(p.ice_mask), ice_plus_land_mask=ip(ipl), uvel_2d=dp(u2), vvel_2d=dp(v2),
+                     btractx=dp(bx), btracty=dp(by))
+    u, v = p.uvel.copy(order="F"), p.vvel.copy(order="F")
+    ef = np.zeros((p.nz - 1, p.nx, p.ny), order="F") if efvs is None else efvs.copy(order="F")
+    bu, bv, ru, rv = (np.zeros(s2, order="F") for _ in range(4))
+    st = lib().orc_picard_solve_depth(C.byref(g), C.byref(o), C.byref(df), dp(p.sigma), ip(p.active_cell),
+                                      ip(p.active_vertex), dp(p.xVertex), dp(p.yVertex), dp(p.stagthck),
+                                      dp(p.flwafact), dp(p.beta_internal), ip(p.umask_dirichlet),
+                                      ip(p.vmask_dirichlet), dp(loadu), dp(loadv), dp(u), dp(v), dp(ef),
+                                      dp(bu), dp(bv), dp(ru), dp(rv))
+    return dict(status=st, uvel=u, vvel=v, uvel_2d=u2, vvel_2d=v2, btractx=bx, btracty=by, efvs=ef, bu=bu,
+                bv=bv, resid_u=ru, resid_v=rv, picard_iters=o.picard_iters, linear_iters=o.total_linear_iters,
+                L2_norm=o.final_L2_norm, resid_velo=o.final_resid_velo, converged=o.converged)
