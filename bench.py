@@ -62,8 +62,9 @@ def workload(name):
                 "GRF sheet 512x512x11, dx 1 km, lognormal beta, Chron-Gear + SIA precond", "grf")
     if name == "c5":
         return (configs.ice_sheet(ewn=1200, nsn=1400, nz=11, dx=1000.0, seed=5),
-                "ice sheet 1200x1400x11, dx 1 km, multiscale thickness, bed relief, floating "
-                "fringe with lateral ocean pressure, lognormal beta, Chron-Gear + SIA precond", "sheet")
+                "ice sheet 1200x1400x11, dx 1 km, grounded (13 floating cells; margins at most "
+                "300 m thick, lateral ocean pressure where they meet the sea), cold ice A = 1e-17, "
+                "multiscale thickness, bed relief, lognormal beta, Chron-Gear + SIA precond", "sheet")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -174,9 +175,11 @@ def main(argv=None):
         ctx.velo_higher_solve_3d(p, lu, lv, precision=1, maxiter_nonlinear=1)
         ctx.set_linear_solver(200, 1e-8)
         ms32 = ctx.time_kernel(1, a.steps)
+        msu = ctx.time_kernel(4, a.steps)
         msc = ctx.time_kernel(3, a.steps)
         print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
-                          "fp32_matvec_ms": ms32, "copy_1GiB_ms": msc, "launches": a.steps}))
+                          "fp32_matvec_ms": ms32, "fp32_update_ms": msu, "copy_1GiB_ms": msc,
+                          "launches": a.steps}))
         ctx.close()
         return 0
 
@@ -280,6 +283,8 @@ def main(argv=None):
         log("cpu baseline (oracle, all cores, bounded sample)")
         cpu_all = cpu_baseline(p, lu, lv, base, all_cores=True)
 
+    depth = depth_solvers_report(log, with_cpu=not a.no_cpu_baseline)
+
     line = {
         "metric": "s/velocity solve to 1e-6 err: learned+FP32-PCG vs FP64 PCG; matvec HBM GB/s",
         "value": round(fast_s, 4),
@@ -338,10 +343,41 @@ def main(argv=None):
                      "algorithmic_bytes_per_launch": bytes32},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
+        "solvers_2d": depth,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
+
+
+def depth_solvers_report(log, with_cpu=True):
+    """The 2D solvers (SSA, L1L2, DIVA) on the GRF sheet: their plain FP64 Picard loops on the
+    GPU at 512 x 512 (one-level context), and at 128 x 128 beside the CPU oracle's loop on the
+    same problem.  Device seconds from resident inputs, as the headline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from cism_amd import configs
+    from cism_amd import gpu as G
+    out = {}
+    for n in (512, 128):
+        p = configs.grf_sheet(ewn=n, nsn=n, nz=11, dx=1000.0, seed=1)
+        lu, lv = O.loads(p)
+        with G.Context.for_problem_2d(p) as ctx:
+            for name, approx in (("ssa", 1), ("l1l2", 3), ("diva", 4)):
+                if approx == 1:
+                    r = ctx.velo_higher_solve_2d(p, lu, lv)
+                else:
+                    r = ctx.velo_higher_solve_depth(p, lu, lv, approx)
+                e = {"gpu_seconds": round(r["seconds_device"], 4), "picard_iters": r["picard_iters"],
+                     "pcg_iters": r["linear_iters"], "converged": bool(r["converged"])}
+                if n == 128 and with_cpu:
+                    t0 = time.perf_counter()
+                    ro = O.picard_2d(p, lu, lv) if approx == 1 else O.picard_depth(p, lu, lv, approx)
+                    e["cpu_oracle_seconds_1_thread"] = round(time.perf_counter() - t0, 3)
+                    e["cpu_picard_iters"] = ro["picard_iters"]
+                out[f"{name}_{n}x{n}"] = e
+                log(f"2D {name} {n}x{n}: {e}")
+    return out
 
 
 def inputs_hash(p):

@@ -1391,8 +1391,8 @@ __global__ __launch_bounds__(256) void k_copy_calib(long long n, const double2 *
 
 extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per_launch) {
     CTX_BEGIN(c);
-    if (reps < 1 || which < 0 || which > 3) return CISM_ERR_ARG;
-    if (which == 1 && !c->A32) {
+    if (reps < 1 || which < 0 || which > 4) return CISM_ERR_ARG;
+    if ((which == 1 || which == 4) && !c->A32) {
         cism_set_error("no FP32 matrix yet: run a mixed-precision solve first");
         return CISM_ERR_ARG;
     }
@@ -1416,6 +1416,7 @@ extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per
             HIP_TRY(hipGetLastError());
             return CISM_OK;
         }
+        if (which == 4) return launch_cg_update32(c);
         return launch_cg_matvec(c, which);
     };
     hipEvent_t e0, e1;

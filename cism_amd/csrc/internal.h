@@ -362,6 +362,7 @@ int anderson_reset(cism_ctx *c);
 int anderson_step(cism_ctx *c, int m, double *fnorm);
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
 int launch_cg_matvec(cism_ctx *c, int fp32);
+int launch_cg_update32(cism_ctx *c);   // one FP32 fused update (timing)
 int rel_err_device(cism_ctx *c, double *rel);
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol,
                  double *err, int *niters);

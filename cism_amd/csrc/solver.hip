@@ -1467,6 +1467,13 @@ int launch_cg_matvec(cism_ctx *c, int fp32) {
     return CISM_OK;
 }
 
+// one launch of the FP32 fused Chron-Gear update with the column preconditioner, as the
+// inner solves issue it (vectors, factors and scalars of the last mixed solve)
+int launch_cg_update32(cism_ctx *c) {
+    CGVecs<float> v = {c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F32, nullptr};
+    return launch_cg_update<float>(c, 2, v, 0, 1);
+}
+
 // sums[0] = |x - xref|^2, sums[1] = |xref|^2 over owned nodes
 __global__ __launch_bounds__(CISM_BLOCK) void k_err_sums(GridDev g, const double *__restrict__ x,
                                                          const double *__restrict__ xr,

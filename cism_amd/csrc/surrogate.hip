@@ -65,14 +65,20 @@ __device__ inline long long shifted_pixel(long long p, int tap, int ks, int H, i
     return b * HW + (long long)y * W + x;
 }
 
+// K values staged in LDS per step of the forward convolution: 4 MFMA k-steps between
+// two barriers; a thread stages 16 consecutive k of one weight row and of one pixel
+// (Ci % 16 == 0, so the 16 lie in one tap and their 3x3 shift is computed once)
+#define CONV_CK 64
+
 __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__restrict__ in,
                                                    const short *__restrict__ Wb,
                                                    const float *__restrict__ bias,
                                                    float *__restrict__ out) {
-    __shared__ short As[64][16 + 4];
-    __shared__ short Bs[64][16 + 4];
+    __shared__ short As[64][CONV_CK + 8];   // weights [co][k]
+    __shared__ short Bs[64][CONV_CK + 8];   // inputs [pixel][k]
     const int K = d.ks * d.ks * d.Ci;
-    const long long P = (long long)d.B * d.H * d.W;
+    const int HW = d.H * d.W;
+    const long long P = (long long)d.B * HW;
     const int co0 = blockIdx.y * 64;
     const long long p0 = (long long)blockIdx.x * 64;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -80,52 +86,79 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__re
     f32x4 acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    const int lr = t >> 2, lk = (t & 3) * 4;      // loader: row / pixel lr, 4 k values from lk
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        // A: weights rows co0+lr, k0+lk..+3
-        {
-            const int co = co0 + lr;
-            for (int j = 0; j < 4; ++j) As[lr][lk + j] = co < d.Co ? Wb[(long long)co * K + k0 + lk + j] : (short)0;
+    const int lr = t >> 2, lk = (t & 3) * 16;   // loader: row / pixel lr, 16 k values from lk
+    const int co = co0 + lr;
+    const long long p = p0 + lr;
+    const bool pok = p < P, cok = co < d.Co;
+    // the loader pixel's image coordinates, once
+    long long pb = 0;
+    int py = 0, px = 0;
+    if (pok) {
+        pb = p / HW;
+        const int r = (int)(p - pb * HW);
+        py = r / d.W;
+        px = r - py * d.W;
+    }
+    for (int k0 = 0; k0 < K; k0 += CONV_CK) {
+        const int k = k0 + lk;
+        uint4 a0 = {0u, 0u, 0u, 0u}, a1 = {0u, 0u, 0u, 0u};
+        if (cok && k < K) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(Wb + (long long)co * K + k);
+            a0 = src[0];
+            a1 = src[1];
         }
-        // B: pixel p0+lr, channels of tap k0/Ci
-        {
-            const long long p = p0 + lr;
-            const int tap = k0 / d.Ci, ci = k0 % d.Ci + lk;
-            if (p < P) {
-                const long long src = shifted_pixel(p, tap, d.ks, d.H, d.W);
-                const float4 v = *reinterpret_cast<const float4 *>(in + src * d.Ci + ci);
-                Bs[lr][lk + 0] = f2bf(v.x);
-                Bs[lr][lk + 1] = f2bf(v.y);
-                Bs[lr][lk + 2] = f2bf(v.z);
-                Bs[lr][lk + 3] = f2bf(v.w);
-            } else {
-                for (int j = 0; j < 4; ++j) Bs[lr][lk + j] = 0;
+        *reinterpret_cast<uint4 *>(&As[lr][lk]) = a0;
+        *reinterpret_cast<uint4 *>(&As[lr][lk + 8]) = a1;
+        if (pok && k < K) {
+            const int tap = k / d.Ci, ci = k - tap * d.Ci;
+            long long src = p;
+            if (d.ks == 3) {
+                int y = py + tap / 3 - 1, x = px + tap % 3 - 1;
+                y = y < 0 ? y + d.H : (y >= d.H ? y - d.H : y);
+                x = x < 0 ? x + d.W : (x >= d.W ? x - d.W : x);
+                src = pb * HW + (long long)y * d.W + x;
             }
+            const float4 *v = reinterpret_cast<const float4 *>(in + src * d.Ci + ci);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 f = v[q];
+                Bs[lr][lk + 4 * q + 0] = f2bf(f.x);
+                Bs[lr][lk + 4 * q + 1] = f2bf(f.y);
+                Bs[lr][lk + 4 * q + 2] = f2bf(f.z);
+                Bs[lr][lk + 4 * q + 3] = f2bf(f.w);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) Bs[lr][lk + j] = 0;
         }
         __syncthreads();
-        bf16x4 af[2], bfm[2];
-        for (int a = 0; a < 2; ++a) {
-            const int row = wc + a * 16 + (lane & 15);
-            for (int j = 0; j < 4; ++j) af[a][j] = As[row][4 * (lane >> 4) + j];
+#pragma unroll
+        for (int kk = 0; kk < CONV_CK; kk += 16) {
+            if (k0 + kk >= K) break;   // the same for the whole block
+            bf16x4 af[2], bfm[2];
+            for (int a = 0; a < 2; ++a) {
+                const int row = wc + a * 16 + (lane & 15);
+                af[a] = *reinterpret_cast<const bf16x4 *>(&As[row][kk + 4 * (lane >> 4)]);
+            }
+            for (int b = 0; b < 2; ++b) {
+                const int col = wp + b * 16 + (lane & 15);
+                bfm[b] = *reinterpret_cast<const bf16x4 *>(&Bs[col][kk + 4 * (lane >> 4)]);
+            }
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
         }
-        for (int b = 0; b < 2; ++b) {
-            const int col = wp + b * 16 + (lane & 15);
-            for (int j = 0; j < 4; ++j) bfm[b][j] = Bs[col][4 * (lane >> 4) + j];
-        }
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int i = 0; i < 4; ++i) {
-                const int co = co0 + wc + a * 16 + 4 * (lane >> 4) + i;
-                const long long p = p0 + wp + b * 16 + (lane & 15);
-                if (co < d.Co && p < P) {
-                    float v = acc[a][b][i] + (bias ? bias[co] : 0.f);
+                const int oc = co0 + wc + a * 16 + 4 * (lane >> 4) + i;
+                const long long op = p0 + wp + b * 16 + (lane & 15);
+                if (oc < d.Co && op < P) {
+                    float v = acc[a][b][i] + (bias ? bias[oc] : 0.f);
                     if (d.relu) v = v > 0.f ? v : 0.f;
-                    out[p * d.Co + co] = v;
+                    out[op * d.Co + oc] = v;
                 }
             }
 }
@@ -595,6 +628,10 @@ static int prep_weights(cism_surrogate *s) {
 
 static int conv_launch(cism_surrogate *s, const ConvDims &d, const float *in, const short *Wb,
                        const float *bias, float *out) {
+    if (d.Ci % 16 != 0 || (d.ks != 1 && d.ks != 3)) {
+        cism_set_error("surrogate convolution: Ci %% 16 must be 0 and the kernel 1x1 or 3x3");
+        return CISM_ERR_ARG;
+    }
     const long long P = (long long)d.B * d.H * d.W;
     dim3 grid((unsigned)((P + 63) / 64), (unsigned)((d.Co + 63) / 64));
     hipLaunchKernelGGL(k_conv_mfma, grid, dim3(256), 0, s->stream, d, in, Wb, bias, out);

@@ -275,7 +275,9 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
  *           2 assembly (element kernel + node gather) of the current velocity
  *           3 plain copy of 1 GiB (k_copy_calib: 1 GiB read + 1 GiB written), the
- *             calibration of hardware byte counters */
+ *             calibration of hardware byte counters
+ *           4 FP32 fused Chron-Gear update with the column preconditioner (needs a
+ *             mixed solve first; it changes that solve's leftover vectors) */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
 /* The SSA branch of glissade_velo_higher_solve (solve_2d with which_ho_approx = 1,

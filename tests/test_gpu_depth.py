@@ -1,8 +1,9 @@
 """The depth-integrated 2D solvers on the GPU, L1L2 (which_ho_approx 3) and DIVA (4),
 against the oracle's Picard loops (glissade_velo_higher.F90:1117-1173, 2206-3830,
 5208-6356, 7382-7864): same Picard count, 2D and 3D velocity within the FP64 parity bar
-(1e-8 relative), the layer viscosities and DIVA's basal traction; the sliding slab
-against the analytic Glen profile; a no-slip DIVA column (beta_eff = 1/omega)."""
+(1e-8 relative), the layer viscosities and DIVA's basal traction, on periodic and outflow
+grids; the sliding slab against the analytic Glen profile; a no-slip DIVA column
+(beta_eff = 1/omega)."""
 import math
 
 import numpy as np
@@ -26,10 +27,12 @@ def problems():
         "grf": configs.grf_sheet(ewn=24, nsn=20, nz=7, seed=4),
         "sheet": configs.ice_sheet(ewn=36, nsn=42, nz=6, dx=33333.0, seed=7),
         "ismip_noslip": ism,
+        "grf_outflow": configs.grf_sheet(ewn=24, nsn=20, nz=7, seed=4, periodic=False),
     }
 
 
-CASES = [("slab", 3), ("slab", 4), ("grf", 3), ("grf", 4), ("sheet", 3), ("sheet", 4), ("ismip_noslip", 4)]
+CASES = [("slab", 3), ("slab", 4), ("grf", 3), ("grf", 4), ("sheet", 3), ("sheet", 4), ("ismip_noslip", 4),
+         ("grf_outflow", 3), ("grf_outflow", 4)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[f"{n}-{'l1l2' if a == 3 else 'diva'}" for n, a in CASES])
