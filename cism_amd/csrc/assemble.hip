@@ -42,7 +42,9 @@ __host__ __device__ constexpr int tri_index(int a, int b) {
                          // best of 8, 16, 24 and 32 at the 1200 x 1400 x 11 size
 
 // element scratch: tiles of 64 elements, entry-major inside a tile, so a wave's
-// reads and writes of one entry are one contiguous 512-byte run
+// reads and writes of one entry are one contiguous 512-byte run.  k_element writes it
+// and k_gather writes the matrix with non-temporal stores: neither is read again before
+// the whole pass has run, and the gather's scratch reads then keep the L2
 __host__ __device__ inline long long keidx(int ent, long long e) {
     return (((e >> 6) * CISM_KE_ENTRIES + ent) << 6) + (e & 63);
 }
@@ -269,10 +271,10 @@ __global__ __launch_bounds__(8 * EL_ELEM) void k_element(GridDev g, const double
                 kv = Pxx[b] + 4.0 * Pyy[b] + Pzz[b];
             }
             const int t = tri_index(a, b);
-            kcol[t << 6] = ku;
-            kcol[(36 + t) << 6] = kv;
+            __builtin_nontemporal_store(ku, &kcol[t << 6]);
+            __builtin_nontemporal_store(kv, &kcol[(36 + t) << 6]);
         }
-        kcol[(72 + a + 8 * b) << 6] = APPROX == 0 ? 0.0 : 2.0 * Pxy[b] + Pyx[b];
+        __builtin_nontemporal_store(APPROX == 0 ? 0.0 : 2.0 * Pxy[b] + Pyx[b], &kcol[(72 + a + 8 * b) << 6]);
     }
 }
 
@@ -557,8 +559,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
             if (basal && m == CISM_DIAG && (blk == 0 || blk == 3))
                 acc[blk] = acc[blk] + basal_scale * beta[col];
         }
-        *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = make_double2(acc[0], acc[1]);
-        *reinterpret_cast<double2 *>(A + aidx(2, m, q)) = make_double2(acc[2], acc[3]);
+        store_blocks_nt(A, q, m, acc[0], acc[1], acc[2], acc[3]);
     }
 }
 
@@ -926,7 +927,7 @@ __global__ __launch_bounds__(256) void k_nnz(GridDev g, const int32_t *__restric
         int n = 0;
         for (int m = 0; m < 27; ++m) {
             double a0, a1, a2, a3;
-            load_blocks(A, q, m, a0, a1, a2, a3);
+            load_blocks_nt(A, q, m, a0, a1, a2, a3);
             n += (a0 != 0.0) + (a1 != 0.0) + (a2 != 0.0) + (a3 != 0.0);
         }
         acc[0] = n;
@@ -985,7 +986,7 @@ __global__ __launch_bounds__(256) void k_resvec(GridDev g, const int32_t *__rest
                 const long long qn = qs + kA + iA * cst + jA * rst;
                 const double2 xx = *reinterpret_cast<const double2 *>(x + 2 * qn);
                 double uu, uv, vu, vv;
-                load_blocks(A, qs, m, uu, uv, vu, vv);
+                load_blocks_nt(A, qs, m, uu, uv, vu, vv);
                 su = su + uu * xx.x + uv * xx.y;
                 sv = sv + vu * xx.x + vv * xx.y;
             }

@@ -191,6 +191,23 @@ __device__ inline void load_blocks(const double *__restrict__ A, long long q, in
     const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
     uu = a.x; uv = a.y; vu = b.x; vv = b.y;
 }
+// The same with non-temporal loads, for kernels that stream the FP64 matrix once (matvec,
+// residuals, nonzero count): the matrix does not displace the vectors from L2, and the
+// FP64 matvec runs 2-3 % faster.  The symmetry pass reads each pair twice and keeps the
+// cached form.
+typedef double dvec2_nt __attribute__((ext_vector_type(2)));
+__device__ inline void load_blocks_nt(const double *__restrict__ A, long long q, int m, double &uu,
+                                      double &uv, double &vu, double &vv) {
+    const dvec2_nt a = __builtin_nontemporal_load(reinterpret_cast<const dvec2_nt *>(A + aidx(0, m, q)));
+    const dvec2_nt b = __builtin_nontemporal_load(reinterpret_cast<const dvec2_nt *>(A + aidx(2, m, q)));
+    uu = a.x; uv = a.y; vu = b.x; vv = b.y;
+}
+__device__ inline void store_blocks_nt(double *__restrict__ A, long long q, int m, double uu, double uv, double vu,
+                                       double vv) {
+    const dvec2_nt a = {uu, uv}, b = {vu, vv};
+    __builtin_nontemporal_store(a, reinterpret_cast<dvec2_nt *>(A + aidx(0, m, q)));
+    __builtin_nontemporal_store(b, reinterpret_cast<dvec2_nt *>(A + aidx(2, m, q)));
+}
 
 // periodic / outflow image of a staggered index (parallel_slap.F90:2807-2925):
 // returns the owned index whose value the halo entry takes, or -1 for zero

@@ -202,7 +202,8 @@ struct MatFull {   // all 27 slots stored (aidx)
     const T *A;
     __device__ inline void blocks(long long q, int m, long long qn, T &uu, T &uv, T &vu, T &vv) const {
         (void)qn;
-        load_blocks(A, q, m, uu, uv, vu, vv);
+        if constexpr (std::is_same<T, double>::value) load_blocks_nt(A, q, m, uu, uv, vu, vv);
+        else load_blocks(A, q, m, uu, uv, vu, vv);
     }
     __device__ inline T entry(const GridDev &g, int b, int m, long long q) const {
         (void)g;

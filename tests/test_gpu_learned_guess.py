@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 WEIGHTS = SG.default_weights("grf")
 N = 256
-SEED = 3100          # outside the training (1000..) and validation (3000..3007) seeds
+SEED = 3100          # outside the training (1000..2999) and validation (3000..3007) seeds
 
 
 def rel_l2(u, v, ru, rv):
