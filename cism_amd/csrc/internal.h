@@ -143,6 +143,9 @@ struct cism_ctx {
 bool roctx_sync_enabled();
 struct TimerRange {
     explicit TimerRange(const char *name, hipStream_t s = nullptr, bool sync = false) : s_(s), sync_(sync) {
+        // with CISM_ROCTX_SYNC=1 a synchronized range first drains the work queued before
+        // it, so that it spans its own device work only
+        if (sync_ && roctx_sync_enabled()) (void)hipStreamSynchronize(s_);
         roctxRangePushA(name);
     }
     ~TimerRange() {

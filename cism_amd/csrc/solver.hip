@@ -942,7 +942,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     // reference recurrences would divide 0 by 0 there
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc, check);
     if (precond == 2) {
-        TimerRange r_p("pcg_precond_init");
+        TimerRange r_p("pcg_precond_init", c->stream, true);
         CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
     }
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
@@ -954,7 +954,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     double err = 0.0;
     auto check_conv = [&](int n, bool &done) -> int {
         // the true residual every fifth iteration, up to the synchronization that reads it
-        TimerRange r_c("pcg_matmult_resid");
+        TimerRange r_c("pcg_matmult_resid", c->stream, true);
         if (check == 0) {
             hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);

@@ -16,6 +16,8 @@
 !                                      (thck, usrf, topg, flwa): masks, staggering, surface
 !                                      gradient, beta, loads and the Picard loop on the
 !                                      device, optionally from the learned initial guess
+!   glissade_velo_higher_solve_2d_gpu  the 2D branches of glissade_velo_higher_solve
+!                                      (solve_2d: SSA, L1L2, DIVA) on a one-level context
 !   cism_gpu_velo_init / _finalize     context lifetime (glissade_velo_higher_init)
 !   cism_gpu_velo_set_learned_guess    loads the learned-guess network's weight file
 ! The two solves take the fast path's options as optional arguments: precision
@@ -37,6 +39,7 @@ module cism_gpu_velo
   public :: pcg_solver_standard_3d, pcg_solver_chrongear_3d, matvec_multiply_structured_3d
   public :: assemble_stiffness_matrix_3d, glissade_velo_higher_solve_3d_gpu
   public :: glissade_velo_higher_solve_gpu, cism_gpu_velo_set_learned_guess
+  public :: glissade_velo_higher_solve_2d_gpu
   public :: cism_gpu_velo_init, cism_gpu_velo_finalize, cism_gpu_set_linear_solver
 
   type, bind(C) :: cism_solve_opts
@@ -70,6 +73,9 @@ module cism_gpu_velo
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: ctx_nx = -1, ctx_ny = -1, ctx_nz = -1, ctx_iper = -1
+  ! the one-level context of the 2D solvers
+  type(c_ptr), save :: ctx2 = c_null_ptr
+  integer, save :: ctx2_nx = -1, ctx2_ny = -1, ctx2_iper = -1
   logical, save :: ctx_periodic = .true.
 
   interface
@@ -177,6 +183,36 @@ module cism_gpu_velo
        type(cism_solve_stats), intent(out) :: stats
      end function c_geometry_solve
 
+     integer(c_int) function c_solve_2d(h, o, nz3, sigma, ac, av, xV, yV, susrf, sthck, flwafact, beta, &
+          umask, vmask, loadu, loadv, uvel, vvel, efvs, bu, bv, ru, rv, stats) &
+          bind(C, name='cism_velo_higher_solve_2d')
+       import :: c_int, c_int32_t, c_ptr, c_double, cism_solve_opts, cism_solve_stats
+       type(c_ptr), value :: h
+       type(cism_solve_opts) :: o
+       integer(c_int), value :: nz3
+       real(c_double), intent(in) :: sigma(*), xV(*), yV(*), susrf(*), sthck(*), flwafact(*), beta(*)
+       integer(c_int32_t), intent(in) :: ac(*), av(*), umask(*), vmask(*)
+       real(c_double), intent(in) :: loadu(*), loadv(*)
+       real(c_double), intent(inout) :: uvel(*), vvel(*)
+       real(c_double), intent(out) :: efvs(*), bu(*), bv(*), ru(*), rv(*)
+       type(cism_solve_stats), intent(out) :: stats
+     end function c_solve_2d
+
+     integer(c_int) function c_solve_depth(h, o, nz3, sigma, ice, ipl, ac, av, xV, yV, thck, sthck, &
+          dsdx, dsdy, flwa, flwafact, beta, umask, vmask, loadu, loadv, uvel, vvel, u2, v2, btx, bty, &
+          efvs, bu, bv, ru, rv, stats) bind(C, name='cism_velo_higher_solve_depth')
+       import :: c_int, c_int32_t, c_ptr, c_double, cism_solve_opts, cism_solve_stats
+       type(c_ptr), value :: h
+       type(cism_solve_opts) :: o
+       integer(c_int), value :: nz3
+       real(c_double), intent(in) :: sigma(*), xV(*), yV(*), thck(*), sthck(*), dsdx(*), dsdy(*)
+       real(c_double), intent(in) :: flwa(*), flwafact(*), beta(*), loadu(*), loadv(*)
+       integer(c_int32_t), intent(in) :: ice(*), ipl(*), ac(*), av(*), umask(*), vmask(*)
+       real(c_double), intent(inout) :: uvel(*), vvel(*), u2(*), v2(*), btx(*), bty(*), efvs(*)
+       real(c_double), intent(out) :: bu(*), bv(*), ru(*), rv(*)
+       type(cism_solve_stats), intent(out) :: stats
+     end function c_solve_depth
+
      integer(c_int) function c_guess_load_file(h, path) bind(C, name='cism_learned_guess_load_file')
        import :: c_int, c_ptr, c_char
        type(c_ptr), value :: h
@@ -216,7 +252,28 @@ contains
     ctx_nx = -1
     ctx_iper = -1
     guess_loaded = .false.
+    if (c_associated(ctx2)) st = c_ctx_destroy(ctx2)
+    ctx2 = c_null_ptr
+    ctx2_nx = -1
+    ctx2_iper = -1
   end subroutine cism_gpu_velo_finalize
+
+  !> the one-level context of the 2D solvers, for an nx x ny grid
+  subroutine cism_gpu_velo_init_2d(nx, ny)
+    integer, intent(in) :: nx, ny
+    integer(c_int) :: st, iper
+    iper = merge(1, 0, ctx_periodic)
+    if (c_associated(ctx2)) then
+       if (nx == ctx2_nx .and. ny == ctx2_ny .and. iper == ctx2_iper) return
+       st = c_ctx_destroy(ctx2)
+       ctx2 = c_null_ptr
+    endif
+    st = c_ctx_create(int(nx, c_int), int(ny, c_int), 1_c_int, 2_c_int, iper, 0_c_int, ctx2)
+    call check(st, 'cism_gpu_velo_init_2d')
+    ctx2_nx = nx
+    ctx2_ny = ny
+    ctx2_iper = iper
+  end subroutine cism_gpu_velo_init_2d
 
   !> maxiters and tolerance of the linear solvers (module constants of the reference)
   subroutine cism_gpu_set_linear_solver(nx, ny, nz, maxiters, tolerance)
@@ -468,6 +525,80 @@ contains
     if (present(converged)) converged = stats%converged /= 0
     if (present(fallbacks)) fallbacks = stats%fallbacks
   end subroutine glissade_velo_higher_solve_3d_gpu
+
+  !> The 2D branches of glissade_velo_higher_solve (solve_2d, glissade_velo_higher.F90:
+  !> 1117-1173, 2206-3830): which_ho_approx 1 (SSA), 3 (L1L2) or 4 (DIVA), with the 3D
+  !> solve's arrays (nz levels).  The 2D load is the column sum of loadu/loadv and the
+  !> Dirichlet masks are the bed level's.  uvel_2d/vvel_2d: DIVA's depth-mean velocity in;
+  !> the 2D solution out (every approximation).  btractx/btracty (DIVA's basal traction) and
+  !> efvs are in/out: DIVA starts its layer viscosities from efvs.  uvel/vvel: the 3D
+  !> velocity out (SSA: the 2D solution on every level; L1L2: the bed level in, integrated
+  !> upward; DIVA: reconstructed after every solve).  ice_mask and ice_plus_land_mask are
+  !> the reference's integer masks (L1L2's gradients; DIVA's staggering of omega).
+  subroutine glissade_velo_higher_solve_2d_gpu(nx, ny, nz, dx, dy, sigma, ice_mask,             &
+                                               ice_plus_land_mask, active_cell, active_vertex,  &
+                                               xVertex, yVertex, thck, stagusrf, stagthck,      &
+                                               dusrf_dx, dusrf_dy, flwa, flwafact,              &
+                                               beta_internal, umask_dirichlet, vmask_dirichlet, &
+                                               loadu, loadv, whichapprox, whichbabc, whichefvs, &
+                                               whichresid, whichsparse, whichprecond,           &
+                                               maxiter_nonlinear, efvs_constant, uvel, vvel,    &
+                                               uvel_2d, vvel_2d, btractx, btracty, efvs,        &
+                                               picard_iters, linear_iters, converged)
+    integer, intent(in) :: nx, ny, nz
+    real(dp), intent(in) :: dx, dy
+    real(dp), dimension(nz), intent(in) :: sigma
+    integer, dimension(nx,ny), intent(in) :: ice_mask, ice_plus_land_mask
+    logical, dimension(nx,ny), intent(in) :: active_cell
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(nx-1,ny-1), intent(in) :: xVertex, yVertex, stagusrf, stagthck
+    real(dp), dimension(nx-1,ny-1), intent(in) :: dusrf_dx, dusrf_dy, beta_internal
+    real(dp), dimension(nx,ny), intent(in) :: thck
+    real(dp), dimension(nz-1,nx,ny), intent(in) :: flwa, flwafact
+    integer, dimension(nz,nx-1,ny-1), intent(in) :: umask_dirichlet, vmask_dirichlet
+    real(dp), dimension(nz,nx-1,ny-1), intent(in) :: loadu, loadv
+    integer, intent(in) :: whichapprox, whichbabc, whichefvs, whichresid, whichsparse, whichprecond
+    integer, intent(in) :: maxiter_nonlinear
+    real(dp), intent(in) :: efvs_constant
+    real(dp), dimension(nz,nx-1,ny-1), intent(inout) :: uvel, vvel
+    real(dp), dimension(nx-1,ny-1), intent(inout) :: uvel_2d, vvel_2d, btractx, btracty
+    real(dp), dimension(nz-1,nx,ny), intent(inout) :: efvs
+    integer, intent(out) :: picard_iters, linear_iters
+    logical, intent(out), optional :: converged
+    type(cism_solve_opts) :: o
+    type(cism_solve_stats) :: stats
+    integer(c_int32_t), allocatable :: ac(:,:), av(:,:), um(:,:,:), vm(:,:,:), ice(:,:), ipl(:,:)
+    real(dp), allocatable :: bu(:,:), bv(:,:), ru(:,:), rv(:,:)
+    call cism_gpu_velo_init_2d(nx, ny)
+    allocate(ac(nx,ny), av(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1), ice(nx,ny), ipl(nx,ny))
+    allocate(bu(nx-1,ny-1), bv(nx-1,ny-1), ru(nx-1,ny-1), rv(nx-1,ny-1))
+    call c_solve_opts_default(o)
+    o%whichbabc = whichbabc; o%whichefvs = whichefvs; o%whichresid = whichresid
+    o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
+    o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
+    o%dx = dx; o%dy = dy
+    ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
+    av = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
+    um = int(umask_dirichlet, c_int32_t)
+    vm = int(vmask_dirichlet, c_int32_t)
+    ice = int(ice_mask, c_int32_t)
+    ipl = int(ice_plus_land_mask, c_int32_t)
+    if (whichapprox == 1) then
+       call check(c_solve_2d(ctx2, o, int(nz, c_int), sigma, ac, av, xVertex, yVertex, stagusrf, stagthck, &
+                             flwafact, beta_internal, um, vm, loadu, loadv, uvel, vvel, efvs, bu, bv,   &
+                             ru, rv, stats), 'glissade_velo_higher_solve_2d_gpu')
+       uvel_2d = uvel(nz,:,:)
+       vvel_2d = vvel(nz,:,:)
+    else
+       call check(c_solve_depth(ctx2, o, int(nz, c_int), sigma, ice, ipl, ac, av, xVertex, yVertex, thck, &
+                                stagthck, dusrf_dx, dusrf_dy, flwa, flwafact, beta_internal, um, vm,    &
+                                loadu, loadv, uvel, vvel, uvel_2d, vvel_2d, btractx, btracty, efvs,     &
+                                bu, bv, ru, rv, stats), 'glissade_velo_higher_solve_2d_gpu')
+    endif
+    picard_iters = stats%picard_iters
+    linear_iters = stats%linear_iters
+    if (present(converged)) converged = stats%converged /= 0
+  end subroutine glissade_velo_higher_solve_2d_gpu
 
   !> the fast path's options (absent: the reference path, precision 0)
   subroutine fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)

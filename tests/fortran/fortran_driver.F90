@@ -22,6 +22,8 @@ program fortran_driver
   character(len=512) :: wfile
   logical :: conv
   integer :: nfb
+  integer, allocatable :: ice(:,:), ipl(:,:)
+  real(dp), allocatable :: dsdx(:,:), dsdy(:,:), u2(:,:), v2(:,:), btx(:,:), bty(:,:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -102,6 +104,23 @@ program fortran_driver
           learned_guess=hdr(8) /= 0, converged=conv, fallbacks=nfb)
      scal = [real(n1, dp), real(n2, dp), merge(1.0_dp, 0.0_dp, conv), real(nfb, dp)]
      write(11) xu, xv, scal
+  case (7)   ! the 2D solvers (SSA, L1L2, DIVA) on the module's one-level context
+     ! hdr: precond, whichapprox, whichefvs, whichsparse; scal: dx, dy, efvs_constant, whichbabc
+     allocate(sigma(nz), ice(nx,ny), ipl(nx,ny), iac(nx,ny), ac(nx,ny), xVx(nx-1,ny-1), yVx(nx-1,ny-1))
+     allocate(thk(nx,ny), susrf(nx-1,ny-1), sthck(nx-1,ny-1), dsdx(nx-1,ny-1), dsdy(nx-1,ny-1))
+     allocate(flw(nz-1,nx,ny), ff(nz-1,nx,ny), beta(nx-1,ny-1), um(nz,nx-1,ny-1), vm(nz,nx-1,ny-1))
+     allocate(lu(nz,nx-1,ny-1), lv(nz,nx-1,ny-1), u2(nx-1,ny-1), v2(nx-1,ny-1), btx(nx-1,ny-1))
+     allocate(bty(nx-1,ny-1), efvs(nz-1,nx,ny))
+     read(10) sigma, ice, ipl, iac, iav, xVx, yVx, thk, susrf, sthck, dsdx, dsdy, flw, ff, beta, &
+          um, vm, lu, lv, xu, xv, u2, v2, btx, bty, efvs, scal
+     ac = iac /= 0
+     av = iav /= 0
+     call glissade_velo_higher_solve_2d_gpu(nx, ny, nz, scal(1), scal(2), sigma, ice, ipl, ac, av,   &
+          xVx, yVx, thk, susrf, sthck, dsdx, dsdy, flw, ff, beta, um, vm, lu, lv, wapprox,          &
+          nint(scal(4)), wefvs, 3, hdr(8), precond, 100, scal(3), xu, xv, u2, v2, btx, bty, efvs,   &
+          n1, n2, converged=conv)
+     scal = [real(n1, dp), real(n2, dp), merge(1.0_dp, 0.0_dp, conv), 0.0_dp]
+     write(11) xu, xv, u2, v2, btx, bty, efvs, scal
   case default
      write(*,*) 'unknown mode', mode
      error stop 2

@@ -188,3 +188,67 @@ int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opts *o, const
     st->fallbacks = (int)(100.0 * su->thklim);
     return CISM_OK;
 }
+
+/* 2D solvers on a one-level context (c->nz == 1), arrays of nz3 levels.
+ * SSA: uvel = loadu + umask + beta(col) + nz3; vvel = loadv + vmask + 10 active_vertex(col)
+ * + whichapprox; efvs = flwafact + ac; stats: picard = maxiter_nonlinear + whichbabc,
+ * linear = 100 whichsparse + 10 whichefvs + whichprecond, converged = 1 */
+int cism_velo_higher_solve_2d(cism_ctx *c, const cism_solve_opts *o, int nz3, const double *sigma,
+                              const int32_t *ac, const int32_t *av, const double *xV, const double *yV,
+                              const double *susrf, const double *sthck, const double *flwafact,
+                              const double *beta, const int32_t *umask, const int32_t *vmask,
+                              const double *loadu, const double *loadv, double *uvel, double *vvel,
+                              double *efvs, double *bu, double *bv, double *ru, double *rv,
+                              cism_solve_stats *st) {
+    (void)sigma, (void)xV, (void)yV, (void)susrf, (void)sthck;
+    if (c->nz != 1) return CISM_ERR_ARG;
+    const long ncol = (long)(c->nx - 1) * (c->ny - 1), ne = (long)(nz3 - 1) * c->nx * c->ny;
+    for (long q = 0; q < nz3 * ncol; ++q) {
+        const long col = q / nz3;
+        uvel[q] = loadu[q] + umask[q] + beta[col] + nz3;
+        vvel[q] = loadv[q] + vmask[q] + 10.0 * av[col] + o->whichapprox;
+    }
+    for (long col = 0; col < ncol; ++col) bu[col] = bv[col] = ru[col] = rv[col] = 0.0;
+    for (long e = 0; e < ne; ++e) efvs[e] = flwafact[e] + ac[e / (nz3 - 1)];
+    memset(st, 0, sizeof *st);
+    st->picard_iters = o->maxiter_nonlinear + o->whichbabc;
+    st->linear_iters = 100 * o->whichsparse + 10 * o->whichefvs + o->whichprecond;
+    st->converged = 1;
+    return CISM_OK;
+}
+
+/* L1L2 / DIVA: (cell of vertex column col = (i, j) is cell (i, j))
+ * uvel = loadu + umask + thck(cell) + flwa(1, cell) + ice(cell) + 10 ipl(cell);
+ * vvel = loadv + vmask + dusrf_dx(col) + 2 dusrf_dy(col) + beta(col) + 100 active_vertex(col);
+ * uvel_2d += btractx + xVertex + stagthck; vvel_2d += btracty + yVertex + sigma(nz3);
+ * btractx *= 2; btracty *= 3; efvs += flwafact + ac; stats: picard = maxiter_nonlinear +
+ * whichbabc + 1000 whichapprox, linear = 100 whichsparse + 10 whichefvs + whichprecond */
+int cism_velo_higher_solve_depth(cism_ctx *c, const cism_solve_opts *o, int nz3, const double *sigma,
+                                 const int32_t *ice, const int32_t *ipl, const int32_t *ac, const int32_t *av,
+                                 const double *xV, const double *yV, const double *thck, const double *sthck,
+                                 const double *dsdx, const double *dsdy, const double *flwa,
+                                 const double *flwafact, const double *beta, const int32_t *umask,
+                                 const int32_t *vmask, const double *loadu, const double *loadv, double *uvel,
+                                 double *vvel, double *u2, double *v2, double *btx, double *bty, double *efvs,
+                                 double *bu, double *bv, double *ru, double *rv, cism_solve_stats *st) {
+    if (c->nz != 1) return CISM_ERR_ARG;
+    const long sx = c->nx - 1, ncol = sx * (c->ny - 1), L = nz3 - 1, ne = L * c->nx * c->ny;
+    for (long q = 0; q < nz3 * ncol; ++q) {
+        const long col = q / nz3, i = col % sx, j = col / sx, cell = i + (long)c->nx * j;
+        uvel[q] = loadu[q] + umask[q] + thck[cell] + flwa[L * cell] + ice[cell] + 10.0 * ipl[cell];
+        vvel[q] = loadv[q] + vmask[q] + dsdx[col] + 2.0 * dsdy[col] + beta[col] + 100.0 * av[col];
+    }
+    for (long col = 0; col < ncol; ++col) {
+        u2[col] = u2[col] + btx[col] + xV[col] + sthck[col];
+        v2[col] = v2[col] + bty[col] + yV[col] + sigma[nz3 - 1];
+        btx[col] = 2.0 * btx[col];
+        bty[col] = 3.0 * bty[col];
+        bu[col] = bv[col] = ru[col] = rv[col] = 0.0;
+    }
+    for (long e = 0; e < ne; ++e) efvs[e] = efvs[e] + flwafact[e] + ac[e / L];
+    memset(st, 0, sizeof *st);
+    st->picard_iters = o->maxiter_nonlinear + o->whichbabc + 1000 * o->whichapprox;
+    st->linear_iters = 100 * o->whichsparse + 10 * o->whichefvs + o->whichprecond;
+    st->converged = 1;
+    return CISM_OK;
+}

@@ -148,3 +148,48 @@ def test_geometry_solve_marshalling(driver, tmp_path):
     # picard = whichbabc 5 + 100 adaptive; linear = lateral 1 + 10 local taud; converged;
     # fallbacks = 100 thklim
     assert list(sc) == [105.0, 11.0, 1.0, 25.0]
+
+
+@pytest.mark.parametrize("approx", [1, 3, 4])
+def test_2d_solvers_marshalling(driver, approx):
+    """glissade_velo_higher_solve_2d_gpu: SSA goes to cism_velo_higher_solve_2d, L1L2 and
+    DIVA to cism_velo_higher_solve_depth, on the module's one-level context; every array
+    reaches its argument in Fortran layout and the in/out ones come back"""
+    sigma = np.linspace(0, 1, NZ)
+    ice = (rng.random((NX, NY)) > 0.3).astype(np.int32)
+    ipl = (rng.random((NX, NY)) > 0.3).astype(np.int32)
+    ac = (rng.random((NX, NY)) > 0.5).astype(np.int32)
+    av = (rng.random((NX - 1, NY - 1)) > 0.5).astype(np.int32)
+    S = (NX - 1, NY - 1)
+    xV, yV, su, st, dsx, dsy, beta = (rnd(S) for _ in range(7))
+    thk = rnd((NX, NY))
+    flw, ff = rnd(C), rnd(C)
+    um = (rng.random(V) > 0.7).astype(np.int32)
+    vm = (rng.random(V) > 0.7).astype(np.int32)
+    lu, lv, u0, v0 = rnd(V), rnd(V), rnd(V), rnd(V)
+    u2, v2, btx, bty = rnd(S), rnd(S), rnd(S), rnd(S)
+    ef0 = rnd(C)
+    dx, dy, ec, babc = 2000.0, 2000.0, 7.0, 5.0
+    outs = call(driver, [7, NX, NY, NZ, 2, approx, 2, 3],
+                [sigma, ice, ipl, ac, av, xV, yV, thk, su, st, dsx, dsy, flw, ff, beta, um, vm, lu, lv,
+                 u0, v0, u2, v2, btx, bty, ef0, np.array([dx, dy, ec, babc])],
+                [V, V, S, S, S, S, C, (4,)])
+    u, v, ou2, ov2, obx, oby, oef, sc = outs
+    cell = (slice(0, NX - 1), slice(0, NY - 1))
+    if approx == 1:
+        np.testing.assert_array_equal(u, lu + um + beta[None] + NZ)
+        np.testing.assert_array_equal(v, lv + vm + 10.0 * av[None] + approx)
+        np.testing.assert_array_equal(ou2, u[-1])          # the 2D solution out
+        np.testing.assert_array_equal(oef, ff + ac[None])
+        np.testing.assert_array_equal(obx, btx)
+        assert list(sc) == [100 + 5, 300 + 20 + 2, 1.0, 0.0]
+    else:
+        close = lambda a, b: np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)   # noqa: E731
+        close(u, lu + um + (thk[cell] + flw[0][cell] + ice[cell] + 10.0 * ipl[cell])[None])
+        close(v, lv + vm + (dsx + 2.0 * dsy + beta + 100.0 * av)[None])
+        close(ou2, u2 + btx + xV + st)
+        close(ov2, v2 + bty + yV + sigma[-1])
+        np.testing.assert_array_equal(obx, 2.0 * btx)
+        np.testing.assert_array_equal(oby, 3.0 * bty)
+        close(oef, ef0 + ff + ac[None])
+        assert list(sc) == [100 + 5 + 1000 * approx, 300 + 20 + 2, 1.0, 0.0]

@@ -143,3 +143,32 @@ def test_fortran_geometry_solve_with_the_learned_guess(tmp_path):
     u, v, sc = runs[1]
     assert rp["picard_iters"] == int(sc[0])
     assert np.array_equal(rp["uvel"], u) and np.array_equal(rp["vvel"], v)
+
+
+@pytest.mark.parametrize("approx", [1, 3, 4])
+def test_fortran_2d_solvers(tmp_path, approx):
+    """glissade_velo_higher_solve_2d_gpu (SSA, L1L2, DIVA) from Fortran against the
+    oracle's loops on a GRF sheet: same Picard count, 2D and 3D velocity within 1e-8"""
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    p = configs.grf_sheet(ewn=24, nsn=20, nz=7, seed=4)
+    lu, lv = O.loads(p)
+    vshape, s2, c3 = (p.nz, p.nx - 1, p.ny - 1), (p.nx - 1, p.ny - 1), (p.nz - 1, p.nx, p.ny)
+    z2 = np.zeros(s2, order="F")
+    ipl = O.ice_plus_land(p)
+    gu, gv, g2u, g2v, gbx, gby, gef, sc = run_driver(
+        tmp_path, [7, p.nx, p.ny, p.nz, p.whichprecond, approx, p.whichefvs, p.whichsparse],
+        [p.sigma, p.ice_mask.astype(np.int32), ipl, p.active_cell.astype(np.int32),
+         p.active_vertex.astype(np.int32), p.xVertex, p.yVertex, p.thck, p.stagusrf, p.stagthck,
+         p.dusrf_dx, p.dusrf_dy, p.flwa, p.flwafact, p.beta_internal, p.umask_dirichlet,
+         p.vmask_dirichlet, lu, lv, p.uvel, p.vvel, z2, z2, z2, z2, np.zeros(c3, order="F"),
+         np.array([p.dx, p.dy, p.efvs_constant, float(p.whichbabc)])],
+        [vshape, vshape, s2, s2, s2, s2, c3, (4,)])
+    ro = O.picard_2d(p, lu, lv) if approx == 1 else O.picard_depth(p, lu, lv, approx)
+    print(f"  Fortran 2D approx {approx}: {int(sc[0])} Picard (oracle {ro['picard_iters']})")
+    assert int(sc[0]) == ro["picard_iters"] and sc[2] == 1.0
+    x3 = lambda u, v: np.concatenate([u.ravel(), v.ravel()])   # noqa: E731
+    assert rel_l2(x3(gu, gv), x3(ro["uvel"], ro["vvel"])) < 1e-8
+    if approx != 1:
+        assert rel_l2(x3(g2u, g2v), x3(ro["uvel_2d"], ro["vvel_2d"])) < 1e-8
+        assert rel_l2(gef, ro["efvs"]) < 1e-8
