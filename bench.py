@@ -134,6 +134,8 @@ def main(argv=None):
                     help="Anderson acceleration depth of the fast path's Picard loop (0 off)")
     ap.add_argument("--inner-rtol", type=float, default=5e-2,
                     help="relative residual of each FP32 inner solve of the refinement")
+    ap.add_argument("--coarse", type=int, default=1, choices=[0, 1],
+                    help="aggregation coarse correction in the FP32 inner solves' preconditioner")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -196,7 +198,7 @@ def main(argv=None):
             r = ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=p.extra.get("beta_external"),
                                                initial_guess=guess, precision=1, adaptive_linear=1,
                                                inner_rtol=a.inner_rtol, maxiter_nonlinear=a.picard,
-                                               anderson_m=a.anderson, efvs=False)
+                                               anderson_m=a.anderson, coarse=a.coarse, efvs=False)
             log(f"fast solve {s}: {r['seconds_device']:.3f} s device, {r['picard_iters']} Picard, "
                 f"{r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 matvecs")
         ctx.close()
@@ -221,8 +223,8 @@ def main(argv=None):
         in, every input of the Picard loop is computed on the device, the velocity comes out.
         seconds = device time from resident inputs to the solution on the device (inputs,
         loads, learned guess, Picard loop); seconds_wall adds the host transfers."""
-        kw = dict(precision=1, adaptive_linear=1, inner_rtol=a.inner_rtol, anderson_m=a.anderson) \
-            if precision == 1 else dict(precision=0)
+        kw = dict(precision=1, adaptive_linear=1, inner_rtol=a.inner_rtol, anderson_m=a.anderson,
+                  coarse=a.coarse) if precision == 1 else dict(precision=0)
         t0 = time.perf_counter()
         r = ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=bext, initial_guess=guess,
                                            stop_rel_err=stop, maxiter_nonlinear=cap, efvs=False, **kw)

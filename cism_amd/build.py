@@ -29,7 +29,7 @@ FLIB = os.path.join(PKG, "libcism_fortran.so")
 OBJ = os.path.join(PKG, "_build")
 
 SOURCES = ["api.hip", "solver.hip", "assemble.hip", "surrogate.hip", "upstream.hip", "ssa.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "coarse.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FC", "/opt/rocm/bin/amdflang")

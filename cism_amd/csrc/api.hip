@@ -263,6 +263,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     learned_guess_free(c);
+    coarse_free(c);
     void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
@@ -802,7 +803,8 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             roctxRangePushA("glissade_pcg_slv_struct");   // popped once the linear solve has ended
             if (prec == 1) {
                 int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
-                st = solver_mixed(c, o->whichprecond, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
+                const int pc = o->whichprecond == 2 && o->coarse ? 3 : o->whichprecond;
+                st = solver_mixed(c, pc, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
                                   &n64, &n32, 1, &stalled, &unconv);
                 S.unconverged_linear += unconv;
                 S.fp64_matvecs += n64;
@@ -1387,6 +1389,73 @@ __global__ __launch_bounds__(256) void k_copy_calib(long long n, const double2 *
                                                     double2 *__restrict__ b) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n / 2) b[t] = a[t];
+}
+
+// ---------------------------------------------------------------------------
+// coarse-correction diagnostics (coarse.hip)
+// ---------------------------------------------------------------------------
+extern "C" int cism_coarse_build(cism_ctx *c, const int32_t *active_vertex, const int32_t *umask_dirichlet,
+                                 const int32_t *vmask_dirichlet, const double *Auu, const double *Auv,
+                                 const double *Avu, const double *Avv, int *nlevels) {
+    CTX_BEGIN(c);
+    CISM_TRY(up(c, c->av, active_vertex, (size_t)c->g.sx * c->g.sy));
+    CISM_TRY(upload_dirichlet_masks(c, umask_dirichlet, vmask_dirichlet, (size_t)c->g.nn));
+    CISM_TRY(upload_matrix(c, Auu, Auv, Avu, Avv));
+    c->a_zero_ok = 0;   // A is not the assembly's
+    c->as_ok = 0;
+    CISM_TRY(ensure_a32(c));
+    CISM_TRY(symmetry_device(c, c->A32));
+    CISM_TRY(coarse_setup(c, c->A32));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (nlevels) *nlevels = c->coarse->nlev;
+    return CISM_OK;
+}
+
+extern "C" int cism_coarse_level(cism_ctx *c, int level, int *ax, int *ay, double *blocks) {
+    CTX_BEGIN(c);
+    if (!c->coarse || !c->coarse->ok) {
+        cism_set_error("no coarse hierarchy: build one (cism_coarse_build or a solve with coarse = 1)");
+        return CISM_ERR_ARG;
+    }
+    if (level < 1 || level > c->coarse->nlev) {
+        cism_set_error("coarse level %d out of 1..%d", level, c->coarse->nlev);
+        return CISM_ERR_ARG;
+    }
+    const CoarseLevel &L = c->coarse->lev[level - 1];
+    if (ax) *ax = L.ax;
+    if (ay) *ay = L.ay;
+    if (blocks) {
+        HIP_TRY(hipMemcpyAsync(blocks, L.A64, sizeof(double) * 36 * (size_t)L.ax * L.ay, hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return CISM_OK;
+}
+
+__global__ void k_d2f(long long n, const double *__restrict__ a, float *__restrict__ b) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) b[q] = (float)a[q];
+}
+__global__ void k_f2d(long long n, const float *__restrict__ a, double *__restrict__ b) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) b[q] = (double)a[q];
+}
+
+extern "C" int cism_coarse_correction(cism_ctx *c, const double *ru, const double *rv, double *zu, double *zv) {
+    CTX_BEGIN(c);
+    if (!c->coarse || !c->coarse->ok || !c->A32) {
+        cism_set_error("no coarse hierarchy: build one (cism_coarse_build or a solve with coarse = 1)");
+        return CISM_ERR_ARG;
+    }
+    const long long n2 = 2 * c->g.nn;
+    CISM_TRY(upload_vec(c, c->t, ru, rv));
+    hipLaunchKernelGGL(k_d2f, dim3(nblocks_for(n2)), dim3(256), 0, c->stream, n2, c->t, c->r32);
+    HIP_TRY(hipMemsetAsync(c->z32, 0, sizeof(float) * n2, c->stream));
+    CISM_TRY(coarse_colsum(c, c->r32));
+    CISM_TRY(coarse_apply(c, c->z32));
+    hipLaunchKernelGGL(k_f2d, dim3(nblocks_for(n2)), dim3(256), 0, c->stream, n2, c->z32, c->t);
+    HIP_TRY(hipGetLastError());
+    return download_vec(c, c->t, zu, zv);
 }
 
 extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per_launch) {

@@ -907,6 +907,7 @@ int symmetry_device(cism_ctx *c, float *As) {
                        skip ? c->ac : nullptr, c->as_ac_prev, c->as_av_prev, c->partials);
     HIP_TRY(hipGetLastError());
     if (As) {
+        coarse_invalidate(c);
         c->as_ok = c->a_zero_ok;
         if (c->as_ok) {
             HIP_TRY(hipMemcpyAsync(c->as_av_prev, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy,

@@ -60,6 +60,27 @@ struct DevScal {
 
 struct cism_guess;   // the learned initial guess of a context (surrogate.hip)
 
+// aggregation coarse correction of the mixed-precision inner solves (coarse.hip): per
+// level an ax x ay grid of aggregates with its 9-point 2 x 2-block stencil in FP64 (the
+// Galerkin sums) and FP32 (the V-cycle), the l1-Jacobi diagonal, and the V-cycle's
+// residual, smoothed and corrected vectors (2 per aggregate each)
+#define CG_MAXLEV 12
+struct CoarseLevel {
+    int ax, ay;
+    double *A64;
+    float *A, *dl1, *r, *e, *eo;
+};
+struct CoarseHier {
+    int nlev;       // levels; the last one is solved with the dense inverse Ainv
+    int ok;         // built for the current FP32 matrix
+    CoarseLevel lev[CG_MAXLEV];
+    int32_t *cidx;  // level-1 aggregate of every column (-1: none)
+    float *colsum;  // (u, v) column sums of the residual, 2 per column
+    float *colF;    // level-1 column blocks (9 column directions and the diagonal), 40 per column
+    double *S;      // level-1 Galerkin partial sums, 40 per aggregate
+    float *Ainv;    // dense inverse of the coarsest level
+};
+
 struct cism_ctx {
     GridDev g;
     int device;
@@ -130,6 +151,8 @@ struct cism_ctx {
     double *stage;
     // learned initial guess (cism_learned_guess_load), or null
     cism_guess *guess;
+    // coarse correction of the mixed-precision inner solves (lazily), or null
+    CoarseHier *coarse;
     // cell masks of cism_glissade_velo_higher_solve as int32 planes (ice, floating,
     // ocean, land, calving front = 0), 5 * nx * ny, lazily
     int32_t *planes;
@@ -426,6 +449,17 @@ int basal_setup_device(cism_ctx *c, const int32_t *planes, int whichground, cons
 // the upload half of basal_setup_device (beta_external and N to the device); checks bp
 int basal_fields_upload(cism_ctx *c, const cism_basal_physics *bp, const double *beta_external,
                         const double *effecpress_stag);
+
+// the aggregation coarse correction (coarse.hip).  coarse_setup builds the hierarchy from
+// the FP32 symmetric matrix unless it is current; coarse_invalidate marks it stale (the
+// symmetry pass that rewrites A32 calls it); coarse_colsum forms the column sums of r that
+// coarse_apply restricts (the fused FP32 update writes them itself); coarse_apply adds
+// P V(P^T r) to z.
+int coarse_setup(cism_ctx *c, const float *As);
+void coarse_invalidate(cism_ctx *c);
+int coarse_colsum(cism_ctx *c, const float *r);
+int coarse_apply(cism_ctx *c, float *z);
+void coarse_free(cism_ctx *c);
 
 int ensure_stage(cism_ctx *c);
 int ensure_ke(cism_ctx *c);

@@ -623,6 +623,11 @@ static int launch_sia_setup(cism_ctx *c, const M A, const float *cs, T *F) {
     return CISM_OK;
 }
 
+// precond 3 (FP32 symmetric matrix only): the SIA preconditioner plus the aggregation
+// coarse correction (coarse.hip), whose V-cycle and prolongation follow the SIA apply or
+// the fused update
+#define PRECOND_SIA_COARSE 3
+
 template <typename T, typename M>
 static int launch_precond(cism_ctx *c, int precond, const M A, const T *F, const T *r, T *z) {
     const GridDev &g = c->g;
@@ -645,6 +650,12 @@ static int launch_precond(cism_ctx *c, int precond, const M A, const T *F, const
             cism_set_error("SIA preconditioner supports nz <= 48 (nz = %d)", g.nz);
             return CISM_ERR_UNSUPPORTED;
         }
+        HIP_TRY(hipGetLastError());
+        if constexpr (std::is_same<M, MatSym32>::value && std::is_same<T, float>::value)
+            if (precond == PRECOND_SIA_COARSE) {
+                CISM_TRY(coarse_colsum(c, r));
+                CISM_TRY(coarse_apply(c, z));
+            }
     }
     HIP_TRY(hipGetLastError());
     return CISM_OK;
@@ -814,7 +825,9 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_cg_update(GridDev g, const T *__
 // values as k_cg_update then k_sia_apply; one pass over r fewer.  The column
 // factors are loaded before the update so their latency overlaps it.  With
 // want_rr the block also leaves (r, r) over its owned nodes in partials (the
-// convergence check of the inner solves, pcg:1540-1560).
+// convergence check of the inner solves, pcg:1540-1560).  With colsum the block also
+// leaves the (u, v) sums of the new r over the levels of its owned active columns, the
+// first stage of the coarse correction's restriction (coarse.hip).
 template <typename T, int NZMAX>
 __global__ __launch_bounds__(256) void k_cg_update_sia(GridDev g, T *z, const T *__restrict__ Az,
                                                        T *__restrict__ d, T *__restrict__ qv,
@@ -822,7 +835,9 @@ __global__ __launch_bounds__(256) void k_cg_update_sia(GridDev g, T *z, const T 
                                                        const DevScal *__restrict__ sc, int first,
                                                        const int32_t *__restrict__ av,
                                                        const T *__restrict__ F, int want_rr,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials, T *__restrict__ colsum,
+                                                       const int32_t *__restrict__ um,
+                                                       const int32_t *__restrict__ vm) {
     const int nz = g.nz;
     const long long ncol = (long long)g.sx * g.sy;
     const long long col0 = (long long)blockIdx.x * 64;
@@ -857,6 +872,17 @@ __global__ __launch_bounds__(256) void k_cg_update_sia(GridDev g, T *z, const T 
     }
     __syncthreads();
     if (solve) {
+        if (colsum) {
+            const int jc = (int)(col / g.sx), ic = (int)(col - (long long)jc * g.sx);
+            if (is_owned(g, ic, jc)) {
+                // prescribed (Dirichlet) components stay out of the coarse space
+                const int32_t *dm = comp == 0 ? um : vm;
+                T cs = (T)0;
+                for (int k = 0; k < nz; ++k)
+                    if (!dm || dm[k + (long long)nz * col] != 1) cs += sr[2 * (lc * nz + k) + comp];
+                colsum[2 * col + comp] = cs;
+            }
+        }
         if constexpr (NZMAX <= 12) sia_solve<T, NZMAX>(nz, f, sr + 2 * lc * nz + comp);
         else sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
     }
@@ -906,18 +932,27 @@ static inline int sia_blocks(const GridDev &g) { return (int)(((long long)g.sx *
 template <typename T>
 static int launch_cg_update(cism_ctx *c, int precond, CGVecs<T> &v, int first, int want_rr = 0) {
     const GridDev &g = c->g;
-    if (precond == 2) {
+    if (precond >= 2) {
         // fused with the SIA preconditioner; the caller skips its own precond call
         const dim3 grid((unsigned)sia_blocks(g));
+        T *cs = nullptr;
+        if constexpr (std::is_same<T, float>::value)
+            if (precond == PRECOND_SIA_COARSE) cs = c->coarse->colsum;
+        const int32_t *um = cs && c->dirichlet_any ? c->umask : nullptr;
+        const int32_t *vm = cs && c->dirichlet_any ? c->vmask : nullptr;
         if (g.nz <= 12)
             hipLaunchKernelGGL((k_cg_update_sia<T, 12>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials, cs, um, vm);
         else if (g.nz <= 24)
             hipLaunchKernelGGL((k_cg_update_sia<T, 24>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials, cs, um, vm);
         else
             hipLaunchKernelGGL((k_cg_update_sia<T, 48>), grid, dim3(256), 0, c->stream, g, v.z, v.Az, v.d,
-                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials);
+                               v.q, v.x, v.r, c->sc, first, c->av, v.F, want_rr, c->partials, cs, um, vm);
+        HIP_TRY(hipGetLastError());
+        if constexpr (std::is_same<T, float>::value)
+            if (precond == PRECOND_SIA_COARSE) CISM_TRY(coarse_apply(c, v.z));
+        return CISM_OK;
     } else {
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0, c->stream, g, v.z,
                            v.Az, v.d, v.q, v.x, v.r, c->sc, first, precond != 0 ? c->av : nullptr);
@@ -941,9 +976,15 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     // an inner solve (check = 1) whose residual becomes exactly zero has converged; the
     // reference recurrences would divide 0 by 0 there
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, c->stream, c->sc, check);
-    if (precond == 2) {
+    if (precond == PRECOND_SIA_COARSE && !std::is_same<M, MatSym32>::value) {
+        cism_set_error("the coarse correction is a preconditioner of the FP32 inner solves only");
+        return CISM_ERR_ARG;
+    }
+    if (precond >= 2) {
         TimerRange r_p("pcg_precond_init", c->stream, true);
         CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
+        if constexpr (std::is_same<M, MatSym32>::value)
+            if (precond == PRECOND_SIA_COARSE) CISM_TRY(coarse_setup(c, A.A));
     }
     hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
                        v.d, v.z, v.q);
@@ -959,7 +1000,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             hipLaunchKernelGGL((k_residual<T, M>), dim3(nbm), dim3(CISM_BLOCK), 0, c->stream, g, A,
                                c->av, v.x, v.b, v.r, chrongear, c->partials);
             CISM_TRY(launch_reduce(c, nbm, 1, RED_ERR));
-        } else if (chrongear && precond == 2) {
+        } else if (chrongear && precond >= 2) {
             // (r, r) came with the fused update of this iteration
             CISM_TRY(launch_reduce(c, sia_blocks(g), 1, RED_ERR));
         } else {
@@ -1016,7 +1057,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
         // with the SIA preconditioner the update also forms the next z
         CISM_TRY(launch_cg_update<T>(c, precond, v, 1));
         for (int n = 2; n <= maxiters && !done; ++n) {
-            if (precond != 2) CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
+            if (precond < 2) CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
             hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
             CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_ITER));

@@ -30,6 +30,7 @@ EXPORTED = [
     "cism_count_nonzeros_3d", "cism_compute_residual_vector_3d", "cism_compute_residual_velocity_3d",
     "cism_load_vector_gravity", "cism_staggered_parallel_halo", "cism_solve_opts_default",
     "cism_velo_higher_solve_3d", "cism_velo_higher_solve_2d", "cism_velo_higher_solve_depth", "cism_set_reference_velocity", "cism_time_kernel",
+    "cism_coarse_build", "cism_coarse_level", "cism_coarse_correction",
     "cism_load_vector_lateral_bc",
     "cism_set_precond_scale", "cism_column_scale_target",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
@@ -58,7 +59,7 @@ class SolveOpts(C.Structure):
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
                 ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("initial_guess", C.c_int),
-                ("whichassemble_beta", C.c_int), ("reserved", C.c_int * 2)]
+                ("whichassemble_beta", C.c_int), ("coarse", C.c_int), ("reserved", C.c_int)]
 
 
 class VeloSetup(C.Structure):
@@ -148,6 +149,9 @@ def lib():
                                                    D, D, D, D, D, D, D, C.POINTER(SolveStats)]
         L.cism_set_reference_velocity.argtypes = [VP, D, D]
         L.cism_time_kernel.argtypes = [VP, C.c_int, C.c_int, D]
+        L.cism_coarse_build.argtypes = [VP, I32, I32, I32, D, D, D, D, C.POINTER(C.c_int)]
+        L.cism_coarse_level.argtypes = [VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), D]
+        L.cism_coarse_correction.argtypes = [VP, D, D, D, D]
         L.cism_set_precond_scale.argtypes = [VP, D, D]
         L.cism_compute_internal_stress.argtypes = [VP, D, I32, D, D, D, D, D, D, C.c_int, C.c_double,
                                                    C.c_int, D, D, D, D, D, D, D, D]
@@ -434,6 +438,31 @@ class Context:
         _check(lib().cism_time_kernel(self.h, which, reps, C.byref(ms)))
         return ms.value
 
+    # --- coarse-correction diagnostics -----------------------------------------
+    def coarse_build(self, active_vertex, umask, vmask, Auu, Auv, Avu, Avv):
+        """symmetry pass (FP32 matrix) and coarse hierarchy from a matrix in the reference's
+        layout; returns the level count"""
+        n = C.c_int()
+        _check(lib().cism_coarse_build(self.h, ip(active_vertex), ip(umask), ip(vmask), dp(Auu), dp(Auv),
+                                       dp(Avu), dp(Avv), C.byref(n)))
+        return n.value
+
+    def coarse_level(self, level):
+        """(ax, ay, blocks): the FP64 Galerkin stencil of a level, blocks[aj, ai, d, b] with
+        d = (di+1) + 3 (dj+1) and b = uu, uv, vu, vv"""
+        ax, ay = C.c_int(), C.c_int()
+        _check(lib().cism_coarse_level(self.h, level, C.byref(ax), C.byref(ay), None))
+        blk = np.zeros(36 * ax.value * ay.value)
+        _check(lib().cism_coarse_level(self.h, level, C.byref(ax), C.byref(ay), dp(blk)))
+        return ax.value, ay.value, blk.reshape(ay.value, ax.value, 9, 4)
+
+    def coarse_correction(self, ru, rv):
+        """z = P V(P^T r) of the current hierarchy for r in the reference's layout"""
+        ru, rv = np.asfortranarray(ru, dtype=np.float64), np.asfortranarray(rv, dtype=np.float64)
+        zu, zv = np.zeros_like(ru, order="F"), np.zeros_like(rv, order="F")
+        _check(lib().cism_coarse_correction(self.h, dp(ru), dp(rv), dp(zu), dp(zv)))
+        return zu, zv
+
     def load_vector_lateral_bc(self, sigma, land_mask, ocean_mask, calving_front_mask, active_cell,
                                xVertex, yVertex, stagusrf, stagthck, loadu, loadv):
         _check(lib().cism_load_vector_lateral_bc(
@@ -461,7 +490,9 @@ class Context:
     # --- device-resident Picard solve ------------------------------------------
     def _opts(self, p, precision=0, maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
               whichresid=None, target_override=0.0, inner_rtol=1e-3, adaptive_linear=0,
-              adaptive_gamma=0.05, stop_rel_err=0.0, anderson_m=0, initial_guess=0):
+              adaptive_gamma=0.05, stop_rel_err=0.0, anderson_m=0, initial_guess=0, coarse=0):
+        """the Picard loop's options (cism_solve_opts) from problem p; coarse=1 adds the
+        aggregation coarse correction to the SIA preconditioner of the FP32 inner solves"""
         o = SolveOpts()
         lib().cism_solve_opts_default(C.byref(o))
         o.whichbabc = p.whichbabc
@@ -482,6 +513,7 @@ class Context:
         o.anderson_m = anderson_m
         o.whichassemble_beta = p.whichassemble_beta
         o.initial_guess = initial_guess
+        o.coarse = coarse
         return o
 
     def glissade_velo_higher_solve(self, p, bp=None, beta_external=None, effecpress_stag=None,
@@ -533,33 +565,11 @@ class Context:
     def learned_guess_last_ms(self):
         return float(lib().cism_learned_guess_last_ms(self.h))
 
-    def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
-                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
-                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
-                             adaptive_linear=0, adaptive_gamma=0.05, stop_rel_err=0.0,
-                             anderson_m=0, initial_guess=0):
-        """initial_guess 2: start from the context's velocity (the last solve's or the learned
-        guess), uvel/vvel then only receive the solution"""
-        o = SolveOpts()
-        lib().cism_solve_opts_default(C.byref(o))
-        o.whichbabc = p.whichbabc
-        o.whichefvs = p.whichefvs
-        o.whichresid = p.whichresid if whichresid is None else whichresid
-        o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
-        o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
-        o.whichapprox = p.whichapprox
-        o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter_nonlinear is None else maxiter_nonlinear
-        o.precision = precision
-        o.efvs_constant = p.efvs_constant
-        o.dx, o.dy = p.dx, p.dy
-        o.target_override = target_override
-        o.inner_rtol = inner_rtol
-        o.adaptive_linear = adaptive_linear
-        o.adaptive_gamma = adaptive_gamma
-        o.stop_rel_err = stop_rel_err
-        o.anderson_m = anderson_m
-        o.whichassemble_beta = p.whichassemble_beta
-        o.initial_guess = initial_guess
+    def velo_higher_solve_3d(self, p, loadu, loadv, uvel=None, vvel=None, **kw):
+        """the device Picard loop from prepared inputs; options as in _opts.  initial_guess
+        2: start from the context's velocity (the last solve's or the learned guess),
+        uvel/vvel then only receive the solution"""
+        o = self._opts(p, **kw)
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
         shape = (p.nz, p.nx - 1, p.ny - 1)
@@ -603,36 +613,15 @@ class Context:
         return dict(uvel=u, vvel=v, uvel_2d=u2, vvel_2d=v2, btractx=bx, btracty=by, efvs=ef, bu=bu, bv=bv,
                     resid_u=ru, resid_v=rv, **st.as_dict())
 
-    def velo_higher_solve_2d(self, p, loadu, loadv, uvel=None, vvel=None, precision=0,
-                             maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
-                             whichresid=None, target_override=0.0, inner_rtol=1e-3,
-                             adaptive_linear=0, adaptive_gamma=0.05, anderson_m=0,
-                             stop_rel_err=0.0):
+    def velo_higher_solve_2d(self, p, loadu, loadv, uvel=None, vvel=None, **kw):
         """The SSA branch (whichapprox 1) on a one-level context: 3D inputs of p in, the
         2D solution broadcast to every level of uvel/vvel out; bu/bv/resid are 2D.
         stop_rel_err > 0 stops at that relative error against the reference velocity
         (set_reference_velocity with (1, nx-1, ny-1) arrays)."""
         if self.nz != 1:
             raise ValueError("velo_higher_solve_2d needs a one-level context (Context.for_problem_2d)")
-        o = SolveOpts()
-        lib().cism_solve_opts_default(C.byref(o))
-        o.whichbabc = p.whichbabc
-        o.whichefvs = p.whichefvs
-        o.whichresid = p.whichresid if whichresid is None else whichresid
-        o.whichsparse = p.whichsparse if whichsparse is None else whichsparse
-        o.whichprecond = p.whichprecond if whichprecond is None else whichprecond
+        o = self._opts(p, **kw)
         o.whichapprox = 1
-        o.maxiter_nonlinear = p.maxiter_nonlinear if maxiter_nonlinear is None else maxiter_nonlinear
-        o.precision = precision
-        o.efvs_constant = p.efvs_constant
-        o.dx, o.dy = p.dx, p.dy
-        o.target_override = target_override
-        o.inner_rtol = inner_rtol
-        o.adaptive_linear = adaptive_linear
-        o.adaptive_gamma = adaptive_gamma
-        o.anderson_m = anderson_m
-        o.whichassemble_beta = p.whichassemble_beta
-        o.stop_rel_err = stop_rel_err
         u = (p.uvel if uvel is None else uvel).copy(order="F")
         v = (p.vvel if vvel is None else vvel).copy(order="F")
         s2 = (p.nx - 1, p.ny - 1)

@@ -220,7 +220,12 @@ typedef struct {
                                   solution                                                  */
     int    whichassemble_beta; /* which_ho_assemble_beta: 0 standard finite-element basal
                                   friction, 1 local (the default)                            */
-    int    reserved[2];
+    int    coarse;             /* precision 1 with whichprecond 2: 1 adds an aggregation coarse
+                                  correction (depth-uniform aggregates of 4 x 4 columns, one
+                                  l1-Jacobi V-cycle) to the SIA preconditioner of the FP32
+                                  inner solves; 0 (default) the SIA preconditioner alone.  The
+                                  FP64 solver keeps the reference's preconditioners.        */
+    int    reserved;
 } cism_solve_opts;
 
 typedef struct {
@@ -279,6 +284,24 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *           4 FP32 fused Chron-Gear update with the column preconditioner (needs a
  *             mixed solve first; it changes that solve's leftover vectors) */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
+
+/* Diagnostics of the aggregation coarse correction of the mixed-precision inner solves
+ * (cism_solve_opts.coarse; no reference counterpart, for tests and tools).
+ * cism_coarse_build: uploads the active-vertex and Dirichlet masks and a matrix in the
+ * reference's layout (as cism_check_symmetry_assembled_matrix_3d takes it), runs the
+ * symmetry pass that writes the FP32 matrix and builds the hierarchy from it; *nlevels
+ * receives its level count.  A later solve replaces both.
+ * cism_coarse_level: the aggregate grid of level (1-based) and, when blocks is not null,
+ * its FP64 Galerkin stencil, 36 values per aggregate a = ai + ax*aj: for direction
+ * d = (di+1) + 3*(dj+1) the blocks uu, uv, vu, vv of the coupling to aggregate
+ * (ai+di, aj+dj), wrapped on periodic grids.
+ * cism_coarse_correction: z = P V(P^T r), the term the coarse correction adds to the SIA
+ * preconditioner's output, for r in the reference's vector layout (FP32 on the device). */
+int  cism_coarse_build(cism_ctx *ctx, const int32_t *active_vertex, const int32_t *umask_dirichlet,
+                       const int32_t *vmask_dirichlet, const double *Auu, const double *Auv,
+                       const double *Avu, const double *Avv, int *nlevels);
+int  cism_coarse_level(cism_ctx *ctx, int level, int *ax, int *ay, double *blocks);
+int  cism_coarse_correction(cism_ctx *ctx, const double *ru, const double *rv, double *zu, double *zv);
 
 /* The SSA branch of glissade_velo_higher_solve (solve_2d with which_ho_approx = 1,
  * glissade_velo_higher.F90:2206-3767, assemble_stiffness_matrix_2d :5208-5595, and
