@@ -130,12 +130,15 @@ def main(argv=None):
     ap.add_argument("--guess", default="learned", choices=["learned", "none"])
     ap.add_argument("--weights", default=None,
                     help="network weights (default: the workload family's file, if trained)")
-    ap.add_argument("--anderson", type=int, default=3,
+    ap.add_argument("--anderson", type=int, default=4,
                     help="Anderson acceleration depth of the fast path's Picard loop (0 off)")
-    ap.add_argument("--inner-rtol", type=float, default=5e-2,
+    ap.add_argument("--inner-rtol", type=float, default=1e-2,
                     help="relative residual of each FP32 inner solve of the refinement")
     ap.add_argument("--coarse", type=int, default=1, choices=[0, 1],
                     help="aggregation coarse correction in the FP32 inner solves' preconditioner")
+    ap.add_argument("--skip-final", type=int, default=1, choices=[0, 1],
+                    help="end a linear solve on its FP32 inner solve's estimate when that meets half "
+                         "the tolerance (cism_solve_opts.skip_final_residual)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only the FP32 matvec and the copy calibration kernel (for counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -198,7 +201,8 @@ def main(argv=None):
             r = ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=p.extra.get("beta_external"),
                                                initial_guess=guess, precision=1, adaptive_linear=1,
                                                inner_rtol=a.inner_rtol, maxiter_nonlinear=a.picard,
-                                               anderson_m=a.anderson, coarse=a.coarse, efvs=False)
+                                               anderson_m=a.anderson, coarse=a.coarse,
+                                               skip_final_residual=a.skip_final, efvs=False)
             log(f"fast solve {s}: {r['seconds_device']:.3f} s device, {r['picard_iters']} Picard, "
                 f"{r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 matvecs")
         ctx.close()
@@ -224,7 +228,7 @@ def main(argv=None):
         seconds = device time from resident inputs to the solution on the device (inputs,
         loads, learned guess, Picard loop); seconds_wall adds the host transfers."""
         kw = dict(precision=1, adaptive_linear=1, inner_rtol=a.inner_rtol, anderson_m=a.anderson,
-                  coarse=a.coarse) if precision == 1 else dict(precision=0)
+                  coarse=a.coarse, skip_final_residual=a.skip_final) if precision == 1 else dict(precision=0)
         t0 = time.perf_counter()
         r = ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=bext, initial_guess=guess,
                                            stop_rel_err=stop, maxiter_nonlinear=cap, efvs=False, **kw)
@@ -307,6 +311,7 @@ def main(argv=None):
                    "entry_point": "cism_glissade_velo_higher_solve (geometry in, velocity out)",
                    "parallelism": "single GPU, whole domain resident"},
         "fast_path": {"seconds": round(fast_s, 4), "anderson_m": a.anderson, "inner_rtol": a.inner_rtol,
+                      "coarse_correction": bool(a.coarse), "skip_final_residual": bool(a.skip_final),
                       "picard_iters": stats["picard_iters"],
                       "fp32_matvecs": stats["fp32_matvecs"], "fp64_matvecs": stats["fp64_matvecs"],
                       "rel_err": stats["final_rel_err"], "learned_guess": use_guess,

@@ -805,7 +805,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 int n64 = 0, n32 = 0, stalled = 0, unconv = 0;
                 const int pc = o->whichprecond == 2 && o->coarse ? 3 : o->whichprecond;
                 st = solver_mixed(c, pc, c->maxiters, lin_tol, o->inner_rtol, &err, &niters,
-                                  &n64, &n32, 1, &stalled, &unconv);
+                                  &n64, &n32, 1, &stalled, &unconv, o->skip_final_residual);
                 S.unconverged_linear += unconv;
                 S.fp64_matvecs += n64;
                 S.fp32_matvecs += n32;

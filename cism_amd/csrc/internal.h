@@ -413,9 +413,11 @@ int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double t
 // sums[3] = |b|^2, as residual_vector_device leaves them with rs = c->r
 // stalled (optional): set to 1 when the refinement is given up (refine_policy.h);
 // unconverged (optional): set to 1 when its budget ran out before the tolerance
+// skip_final: end after the first refinement step without the confirming FP64 residual
+// when the inner solve's own estimate meets half the tolerance (cism_solve_opts)
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
                  double *err, int *niters, int *n64, int *n32, int r_ready = 0,
-                 int *stalled = nullptr, int *unconverged = nullptr);
+                 int *stalled = nullptr, int *unconverged = nullptr, int skip_final = 0);
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy);

@@ -48,7 +48,7 @@ module cism_gpu_velo
      real(c_double) :: efvs_constant, dx, dy, target_override, inner_rtol
      integer(c_int) :: adaptive_linear
      real(c_double) :: adaptive_gamma, stop_rel_err
-     integer(c_int) :: anderson_m, initial_guess, whichassemble_beta, coarse, reserved
+     integer(c_int) :: anderson_m, initial_guess, whichassemble_beta, coarse, skip_final_residual
   end type cism_solve_opts
 
   type, bind(C) :: cism_solve_stats

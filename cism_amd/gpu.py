@@ -59,7 +59,7 @@ class SolveOpts(C.Structure):
                 ("target_override", C.c_double), ("inner_rtol", C.c_double),
                 ("adaptive_linear", C.c_int), ("adaptive_gamma", C.c_double),
                 ("stop_rel_err", C.c_double), ("anderson_m", C.c_int), ("initial_guess", C.c_int),
-                ("whichassemble_beta", C.c_int), ("coarse", C.c_int), ("reserved", C.c_int)]
+                ("whichassemble_beta", C.c_int), ("coarse", C.c_int), ("skip_final_residual", C.c_int)]
 
 
 class VeloSetup(C.Structure):
@@ -490,9 +490,11 @@ class Context:
     # --- device-resident Picard solve ------------------------------------------
     def _opts(self, p, precision=0, maxiter_nonlinear=None, whichsparse=None, whichprecond=None,
               whichresid=None, target_override=0.0, inner_rtol=1e-3, adaptive_linear=0,
-              adaptive_gamma=0.05, stop_rel_err=0.0, anderson_m=0, initial_guess=0, coarse=0):
+              adaptive_gamma=0.05, stop_rel_err=0.0, anderson_m=0, initial_guess=0, coarse=0,
+              skip_final_residual=0):
         """the Picard loop's options (cism_solve_opts) from problem p; coarse=1 adds the
-        aggregation coarse correction to the SIA preconditioner of the FP32 inner solves"""
+        aggregation coarse correction to the SIA preconditioner of the FP32 inner solves;
+        skip_final_residual=1 lets a linear solve end on its inner solve's estimate"""
         o = SolveOpts()
         lib().cism_solve_opts_default(C.byref(o))
         o.whichbabc = p.whichbabc
@@ -514,6 +516,7 @@ class Context:
         o.whichassemble_beta = p.whichassemble_beta
         o.initial_guess = initial_guess
         o.coarse = coarse
+        o.skip_final_residual = skip_final_residual
         return o
 
     def glissade_velo_higher_solve(self, p, bp=None, beta_external=None, effecpress_stag=None,

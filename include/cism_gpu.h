@@ -225,7 +225,13 @@ typedef struct {
                                   l1-Jacobi V-cycle) to the SIA preconditioner of the FP32
                                   inner solves; 0 (default) the SIA preconditioner alone.  The
                                   FP64 solver keeps the reference's preconditioners.        */
-    int    reserved;
+    int    skip_final_residual; /* precision 1: 1 ends a linear solve after its first
+                                  refinement step, without the FP64 residual that would
+                                  confirm it, when the FP32 inner solve's own relative
+                                  residual e (>= 1e-4, above FP32's rounding floor) times
+                                  the step's FP64 residual is below half the tolerance; the
+                                  next Picard iteration's residual checks the result.  0
+                                  (default): every step is confirmed in FP64.             */
 } cism_solve_opts;
 
 typedef struct {

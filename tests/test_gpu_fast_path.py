@@ -134,3 +134,18 @@ def test_worse_than_zero_guess_is_rejected(setup):
     assert 0 <= bad["final_rel_err"] <= 1e-6
     assert bad["picard_iters"] == zero["picard_iters"]
     assert np.array_equal(bad["uvel"], zero["uvel"]) and np.array_equal(bad["vvel"], zero["vvel"])
+
+
+def test_skipping_the_final_residual_keeps_the_iterates(setup):
+    """skip_final_residual only leaves out the FP64 residual that would confirm a linear
+    solve its inner estimate already meets: the solve reaches the same 1e-6 with fewer
+    FP64 matvecs"""
+    p, ctx, lu, lv, ref = setup
+    kw = dict(anderson_m=4, coarse=1)
+    r0 = fast(ctx, p, lu, lv, skip_final_residual=0, **kw)
+    r1 = fast(ctx, p, lu, lv, skip_final_residual=1, **kw)
+    print(f"[fast] confirmed: {r0['picard_iters']} Picard, {r0['fp64_matvecs']} FP64; skipped: "
+          f"{r1['picard_iters']} Picard, {r1['fp64_matvecs']} FP64, err {r1['final_rel_err']:.2e}")
+    for r in (r0, r1):
+        assert r["converged"] and 0 <= r["final_rel_err"] <= 1e-6 and r["fallbacks"] == 0
+    assert r1["fp64_matvecs"] < r0["fp64_matvecs"]
