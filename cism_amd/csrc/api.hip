@@ -693,11 +693,18 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 CISM_TRY(calcbeta_device(c));
         }
         double ta = now_s();
+        bool fuse32 = false;
         {
             // basal friction (velo_higher:3039-3052): the local assembly inside the gather, the
             // standard one after it; the 2D branch applies it for every which_ho_babc
             // (:2832-2852), DIVA with beta_eff
             const bool basal = dim2 || o->whichbabc != 6, standard = o->whichassemble_beta == 0;
+            // The 3D gather sums a coefficient and its transpose over the same elements in the
+            // same order from symmetric element matrices, so its matrix is symmetric as it
+            // stands; when no Dirichlet row or separate basal pass changes it afterwards, the
+            // mixed path takes the FP32 matrix and the nonzero count from the gather and
+            // skips the symmetry pass (which would find nothing to repair)
+            fuse32 = prec == 1 && !dim2 && !c->dirichlet_any && !(basal && standard);
             {
                 TimerRange r_asm(dim2 ? "glissade_assemble_2d" : "glissade_assemble_3d", c->stream, true);
                 if (depth)
@@ -706,9 +713,11 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 else if (dim2)
                     CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx,
                                                o->dy));
-                else
+                else {
+                    if (fuse32) CISM_TRY(ensure_a32(c));
                     CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
-                                             basal && !standard, o->dx, o->dy));
+                                             basal && !standard, o->dx, o->dy, fuse32 ? c->A32 : nullptr));
+                }
                 if (basal && standard)
                     CISM_TRY(basal_standard_device(c, depth ? depth_basal_beta(c, o->whichapprox) : nullptr));
             }
@@ -723,10 +732,14 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 CISM_TRY(launch_halo_mat(c, c->A));
                 CISM_TRY(launch_halo_vec(c, c->b));
             }
-            // the mixed-precision path gets its FP32 symmetric matrix from the same pass
-            if (prec == 1) CISM_TRY(ensure_a32(c));
             TimerRange r_s("glissade_chk_symmetry");   // ends at the synchronization below
-            CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
+            if (fuse32) {
+                CISM_TRY(launch_a32_halo(c));
+            } else {
+                // the mixed-precision path gets its FP32 symmetric matrix from the same pass
+                if (prec == 1) CISM_TRY(ensure_a32(c));
+                CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
+            }
             CISM_TRY(sync_scalars(c));
         }
         if (c->sc_host->singular_flag) {

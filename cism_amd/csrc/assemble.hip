@@ -500,9 +500,16 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                                                 const int32_t *__restrict__ ac_zero,
                                                 const double *__restrict__ Ke,
                                                 const double *__restrict__ beta, int add_basal,
-                                                double basal_scale, double *__restrict__ A) {
+                                                double basal_scale, double *__restrict__ A,
+                                                float *__restrict__ As, const int32_t *__restrict__ as_av,
+                                                const int32_t *__restrict__ as_ac,
+                                                double *__restrict__ partials) {
+    // As (the mixed-precision path without Dirichlet rows or a separate basal pass): the
+    // lane also writes its node's FP32 symmetric storage (slots CISM_DIAG .. 26) and counts
+    // the nonzeros of owned active rows into partials (every lane reaches the reduction)
+    double red[2] = {0.0, 0.0};
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int)g.nn) return;
+    if (q < (int)g.nn) {
     const int k = q % g.nz + 1;                // 1-based
     const int col = q / g.nz;
     const int i = col % g.sx + 1, j = col / g.sx + 1;
@@ -530,15 +537,24 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
         }
     const bool basal = add_basal && k == g.nz && av[col];
     if (!any && !basal) {
-        if (av_zero && !was && !av[col] && !av_zero[col] && is_owned(g, i - 1, j - 1)) return;
-        const double2 z = make_double2(0.0, 0.0);
+        const bool owned = is_owned(g, i - 1, j - 1);
+        if (!(av_zero && !was && !av[col] && !av_zero[col] && owned)) {
+            const double2 z = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int m = 0; m < 27; ++m) {
-            *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = z;
-            *reinterpret_cast<double2 *>(A + aidx(2, m, q)) = z;
+            for (int m = 0; m < 27; ++m) {
+                *reinterpret_cast<double2 *>(A + aidx(0, m, q)) = z;
+                *reinterpret_cast<double2 *>(A + aidx(2, m, q)) = z;
+            }
         }
-        return;
-    }
+        // the FP32 row is zero already when the node had no active cell at its last write
+        if (As && !(as_av && !av[col] && !as_av[col] && owned && !touches_active_cell(g, as_ac, i - 1, j - 1))) {
+#pragma unroll
+            for (int s = 0; s < CISM_SYM_SLOTS; ++s)
+                *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    } else {
+    const bool count = As && is_owned(g, i - 1, j - 1) && av[col];
+    double nnz = 0.0;
     // slot by slot, the four blocks together: each slot's coefficients leave as two
     // 16-byte stores (the tile keeps a node's four blocks of one slot adjacent)
 #pragma unroll
@@ -563,7 +579,15 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                 acc[blk] = acc[blk] + basal_scale * beta[col];
         }
         store_blocks_nt(A, q, m, acc[0], acc[1], acc[2], acc[3]);
+        if (As && m >= CISM_DIAG)
+            *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) =
+                make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+        if (count) nnz += (double)((acc[0] != 0.0) + (acc[1] != 0.0) + (acc[2] != 0.0) + (acc[3] != 0.0));
     }
+    red[1] = nnz;
+    }
+    }
+    if (partials) block_reduce_store<2>(red, partials);
 }
 
 // basal_sliding_bc with HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363) in gather
@@ -661,7 +685,7 @@ int basal_standard_device(cism_ctx *c, const double *beta) {
 }
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
-                    int add_basal, double dx, double dy) {
+                    int add_basal, double dx, double dy, float *As) {
     CISM_TRY(ensure_ke(c));
     const GridDev &g = c->g;
     const int nb = (int)((g.ne + EL_ELEM - 1) / EL_ELEM);
@@ -670,10 +694,23 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
                        c->susrf, c->sthck, c->flwafact, whichefvs, efvs_constant, c->Ke, c->efvs, c->sc);
     HIP_TRY(hipGetLastError());
     const double basal_scale = dx * dy / CISM_VOL0;
-    hipLaunchKernelGGL(k_gather, dim3(nblocks_for(g.nn)), dim3(256), 0, c->stream, g, c->ac, c->av,
+    const int nbg = nblocks_for(g.nn);
+    hipLaunchKernelGGL(k_gather, dim3(nbg), dim3(256), 0, c->stream, g, c->ac, c->av,
                        c->a_zero_ok ? c->av_zero : nullptr, c->a_zero_ok ? c->ac_zero : nullptr, c->Ke, c->beta,
-                       add_basal, basal_scale, c->A);
+                       add_basal, basal_scale, c->A, As, As && c->as_ok ? c->as_av_prev : nullptr,
+                       As && c->as_ok ? c->as_ac_prev : nullptr, As ? c->partials : nullptr);
     HIP_TRY(hipGetLastError());
+    if (As) {
+        // the FP32 matrix of this assembly (its halo rows follow the FP64 halo update,
+        // launch_a32_halo); sums[1] = nonzeros of owned active rows, as the symmetry pass
+        coarse_invalidate(c);
+        CISM_TRY(launch_reduce(c, nbg, 2, RED_SUMS));
+        HIP_TRY(hipMemcpyAsync(c->as_av_prev, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy,
+                               hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->as_ac_prev, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny,
+                               hipMemcpyDeviceToDevice, c->stream));
+        c->as_ok = 1;
+    }
     HIP_TRY(hipMemcpyAsync(c->av_zero, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy, hipMemcpyDeviceToDevice,
                            c->stream));
     HIP_TRY(hipMemcpyAsync(c->ac_zero, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,

@@ -419,8 +419,12 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
                  double *err, int *niters, int *n64, int *n32, int r_ready = 0,
                  int *stalled = nullptr, int *unconverged = nullptr, int skip_final = 0);
 
+// As (optional): also write the FP32 symmetric matrix and count the nonzeros (sums[1]) in
+// the gather, for a matrix that no later pass changes except the halo update (no Dirichlet
+// rows, basal friction local); launch_a32_halo then copies the FP32 halo rows
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
-                    int add_basal, double dx, double dy);
+                    int add_basal, double dx, double dy, float *As = nullptr);
+int launch_a32_halo(cism_ctx *c);
 int dirichlet_device(cism_ctx *c);
 // basal friction by the standard finite-element assembly onto the assembled c->A (c->ac,
 // xV, yV, beta); the local assembly is part of the gather (assemble_device's add_basal)

@@ -712,6 +712,32 @@ static int launch_halo_t(cism_ctx *c, T *v) {
     return CISM_OK;
 }
 
+// the FP32 symmetric rows (slots CISM_DIAG .. 26) of the halo nodes from the FP64 matrix
+// after its halo update, for an FP32 matrix written by the gather
+__global__ __launch_bounds__(CISM_BLOCK) void k_a32_halo(GridDev g, const double *__restrict__ A,
+                                                         float *__restrict__ As) {
+    const long long nh = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nh * CISM_SYM_SLOTS) return;
+    const int s = (int)(t / nh);
+    const long long hv = t % nh;
+    const int k = (int)(hv % g.nz);
+    int i, j;
+    if (!halo_vertex_of(g, hv / g.nz, i, j)) return;
+    const long long q = k + (long long)g.nz * (i + (long long)g.sx * j);
+    double a0, a1, a2, a3;
+    load_blocks(A, q, CISM_DIAG + s, a0, a1, a2, a3);
+    *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
+}
+
+int launch_a32_halo(cism_ctx *c) {
+    const GridDev &g = c->g;
+    const long long n = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz * CISM_SYM_SLOTS;
+    hipLaunchKernelGGL(k_a32_halo, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, g, c->A, c->A32);
+    HIP_TRY(hipGetLastError());
+    return CISM_OK;
+}
+
 int launch_halo_vec(cism_ctx *c, double *v) { return launch_halo_t<double, false>(c, v); }
 int launch_halo_mat(cism_ctx *c, double *A) { return launch_halo_t<double, true>(c, A); }
 
