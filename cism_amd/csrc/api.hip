@@ -714,7 +714,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                     CISM_TRY(assemble2d_device(c, o->whichefvs, o->efvs_constant, basal && !standard, o->dx,
                                                o->dy));
                 else {
-                    if (fuse32) CISM_TRY(ensure_a32(c));
+                    if (fuse32) {
+                        CISM_TRY(ensure_a32(c));
+                        CISM_TRY(launch_halo_vec(c, c->x));   // the gather's residual reads x's halo
+                    }
                     CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
                                              basal && !standard, o->dx, o->dy, fuse32 ? c->A32 : nullptr));
                 }
@@ -735,6 +738,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             TimerRange r_s("glissade_chk_symmetry");   // ends at the synchronization below
             if (fuse32) {
                 CISM_TRY(launch_a32_halo(c));
+                CISM_TRY(launch_halo_vec(c, c->r));
             } else {
                 // the mixed-precision path gets its FP32 symmetric matrix from the same pass
                 if (prec == 1) CISM_TRY(ensure_a32(c));
@@ -750,7 +754,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             cism_set_error("Invalid beta value in calcbeta (negative or NaN)");
             return CISM_ERR_NAN;
         }
-        const double nnz = c->sc_host->sums[1];
+        const double nnz = c->sc_host->sums[fuse32 ? 4 : 1];
         t_asm += now_s() - ta;
         if (nnz == 0.0) {
             zero_exit = true;
@@ -765,7 +769,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         } else {
             // the mixed path takes the first residual of its refinement from this pass
             const bool share_r = prec == 1;
-            {
+            if (!fuse32) {   // the fused gather has formed it already
                 TimerRange r_r("glissade_resid_vec");
                 if (share_r) CISM_TRY(launch_halo_vec(c, c->x));
                 CISM_TRY(residual_vector_device(c, c->t, share_r ? c->r : nullptr));

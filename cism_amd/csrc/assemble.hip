@@ -503,11 +503,18 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                                                 double basal_scale, double *__restrict__ A,
                                                 float *__restrict__ As, const int32_t *__restrict__ as_av,
                                                 const int32_t *__restrict__ as_ac,
+                                                const double *__restrict__ x, const double *__restrict__ bl,
+                                                double *__restrict__ tv, double *__restrict__ rsv,
                                                 double *__restrict__ partials) {
     // As (the mixed-precision path without Dirichlet rows or a separate basal pass): the
-    // lane also writes its node's FP32 symmetric storage (slots CISM_DIAG .. 26) and counts
-    // the nonzeros of owned active rows into partials (every lane reaches the reduction)
-    double red[2] = {0.0, 0.0};
+    // lane also writes its node's FP32 symmetric storage (slots CISM_DIAG .. 26) and, as
+    // k_resvec<true> with b = bl (the load vector, no Dirichlet row changes it), the
+    // residual of its row from the coefficients it has just formed: tv = A x - b on owned
+    // active rows (0 elsewhere), rsv = b - A x on owned rows (the halo rows follow from their
+    // images by a vector halo update); partials per block: |tv|^2 and |b|^2 over owned
+    // active rows, |rsv|^2 and |b|^2 over owned rows, the nonzeros of owned active rows
+    // (every lane reaches the reduction).  x must carry its halo.
+    double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < (int)g.nn) {
     const int k = q % g.nz + 1;                // 1-based
@@ -552,9 +559,19 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
             for (int s = 0; s < CISM_SYM_SLOTS; ++s)
                 *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        if (As) {   // no active row here (an owned active vertex touches an active cell)
+            reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
+            if (owned) {
+                const double2 bb = reinterpret_cast<const double2 *>(bl)[q];
+                reinterpret_cast<double2 *>(rsv)[q] = bb;
+                red[2] = bb.x * bb.x + bb.y * bb.y;
+                red[3] = bb.x * bb.x + bb.y * bb.y;
+            }
+        }
     } else {
-    const bool count = As && is_owned(g, i - 1, j - 1) && av[col];
-    double nnz = 0.0;
+    const bool owned = is_owned(g, i - 1, j - 1);
+    const bool count = As && owned && av[col];
+    double nnz = 0.0, su = 0.0, sv = 0.0;
     // slot by slot, the four blocks together: each slot's coefficients leave as two
     // 16-byte stores (the tile keeps a node's four blocks of one slot adjacent)
 #pragma unroll
@@ -582,12 +599,40 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
         if (As && m >= CISM_DIAG)
             *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) =
                 make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
-        if (count) nnz += (double)((acc[0] != 0.0) + (acc[1] != 0.0) + (acc[2] != 0.0) + (acc[3] != 0.0));
+        if (count) {
+            nnz += (double)((acc[0] != 0.0) + (acc[1] != 0.0) + (acc[2] != 0.0) + (acc[3] != 0.0));
+            // the row product in k_resvec's order (slots in indxA order, the same operations)
+            if (k - 1 + kA >= 0 && k - 1 + kA < g.nz) {
+                const long long qn = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+                const double2 xx = reinterpret_cast<const double2 *>(x)[qn];
+                su = su + acc[0] * xx.x + acc[1] * xx.y;
+                sv = sv + acc[2] * xx.x + acc[3] * xx.y;
+            }
+        }
     }
-    red[1] = nnz;
+    if (As) {
+        const double2 bb = reinterpret_cast<const double2 *>(bl)[q];
+        if (count) {
+            const double ru = su - bb.x, rv = sv - bb.y;
+            reinterpret_cast<double2 *>(tv)[q] = make_double2(ru, rv);
+            reinterpret_cast<double2 *>(rsv)[q] = make_double2(-ru, -rv);
+            red[0] = ru * ru + rv * rv;
+            red[1] = bb.x * bb.x + bb.y * bb.y;
+            red[2] = ru * ru + rv * rv;
+            red[3] = bb.x * bb.x + bb.y * bb.y;
+        } else {
+            reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
+            if (owned) {
+                reinterpret_cast<double2 *>(rsv)[q] = bb;
+                red[2] = bb.x * bb.x + bb.y * bb.y;
+                red[3] = bb.x * bb.x + bb.y * bb.y;
+            }
+        }
+    }
+    red[4] = nnz;
     }
     }
-    if (partials) block_reduce_store<2>(red, partials);
+    if (partials) block_reduce_store<5>(red, partials);
 }
 
 // basal_sliding_bc with HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363) in gather
@@ -698,13 +743,16 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
     hipLaunchKernelGGL(k_gather, dim3(nbg), dim3(256), 0, c->stream, g, c->ac, c->av,
                        c->a_zero_ok ? c->av_zero : nullptr, c->a_zero_ok ? c->ac_zero : nullptr, c->Ke, c->beta,
                        add_basal, basal_scale, c->A, As, As && c->as_ok ? c->as_av_prev : nullptr,
-                       As && c->as_ok ? c->as_ac_prev : nullptr, As ? c->partials : nullptr);
+                       As && c->as_ok ? c->as_ac_prev : nullptr, c->x, c->load, As ? c->t : nullptr,
+                       As ? c->r : nullptr, As ? c->partials : nullptr);
     HIP_TRY(hipGetLastError());
     if (As) {
         // the FP32 matrix of this assembly (its halo rows follow the FP64 halo update,
-        // launch_a32_halo); sums[1] = nonzeros of owned active rows, as the symmetry pass
+        // launch_a32_halo); the residual of residual_vector_device(c, c->t, c->r) in c->t,
+        // c->r (halo rows: launch_halo_vec(c->r)) and sums[0..3]; sums[4] = nonzeros of owned
+        // active rows
         coarse_invalidate(c);
-        CISM_TRY(launch_reduce(c, nbg, 2, RED_SUMS));
+        CISM_TRY(launch_reduce(c, nbg, 5, RED_SUMS));
         HIP_TRY(hipMemcpyAsync(c->as_av_prev, c->av, sizeof(int32_t) * (size_t)g.sx * g.sy,
                                hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(hipMemcpyAsync(c->as_ac_prev, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny,

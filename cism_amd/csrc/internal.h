@@ -419,9 +419,11 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
                  double *err, int *niters, int *n64, int *n32, int r_ready = 0,
                  int *stalled = nullptr, int *unconverged = nullptr, int skip_final = 0);
 
-// As (optional): also write the FP32 symmetric matrix and count the nonzeros (sums[1]) in
-// the gather, for a matrix that no later pass changes except the halo update (no Dirichlet
-// rows, basal friction local); launch_a32_halo then copies the FP32 halo rows
+// As (optional): also write the FP32 symmetric matrix, the Picard residual of the context's
+// x (c->t = A x - b, c->r = b - A x with b the load vector, sums[0..3] as
+// residual_vector_device) and the nonzero count (sums[4]) in the gather, for a matrix that
+// no later pass changes except the halo update (no Dirichlet rows, basal friction local);
+// x must carry its halo; launch_a32_halo and launch_halo_vec(c->r) complete the halo rows
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy, float *As = nullptr);
 int launch_a32_halo(cism_ctx *c);
