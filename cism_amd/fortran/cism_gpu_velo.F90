@@ -21,7 +21,8 @@
 !   cism_gpu_velo_init / _finalize     context lifetime (glissade_velo_higher_init)
 !   cism_gpu_velo_set_learned_guess    loads the learned-guess network's weight file
 ! The two solves take the fast path's options as optional arguments: precision
-! (1: FP32 matrix with FP64 refinement), adaptive_linear, inner_rtol, anderson_m, and
+! (1: FP32 matrix with FP64 refinement), adaptive_linear, inner_rtol, anderson_m, coarse
+! (the aggregation coarse correction of the FP32 inner solves), skip_final_residual, and
 ! report converged (the reference's converged_soln) and fallbacks.
 !
 ! Each call converts LOGICAL masks to int32 (LOGICAL's kind and .true. value
@@ -480,7 +481,8 @@ contains
                                                uvel, vvel, efvs, bu, bv, resid_u, resid_v, &
                                                picard_iters, linear_iters, precision,      &
                                                adaptive_linear, inner_rtol, anderson_m,    &
-                                               converged, fallbacks)
+                                               converged, fallbacks, coarse,               &
+                                               skip_final_residual)
     integer, intent(in) :: nx, ny, nz
     real(dp), intent(in) :: dx, dy
     real(dp), dimension(nz), intent(in) :: sigma
@@ -501,6 +503,7 @@ contains
     real(dp), intent(in), optional :: inner_rtol
     logical, intent(out), optional :: converged
     integer, intent(out), optional :: fallbacks
+    integer, intent(in), optional :: coarse, skip_final_residual
     type(cism_solve_opts) :: o
     type(cism_solve_stats) :: stats
     integer(c_int32_t), allocatable :: ac(:,:), av(:,:), um(:,:,:), vm(:,:,:)
@@ -511,7 +514,8 @@ contains
     o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
     o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
     o%dx = dx; o%dy = dy
-    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
+    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m, coarse, &
+                           skip_final_residual)
     ac = merge(1_c_int32_t, 0_c_int32_t, active_cell)
     av = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     um = int(umask_dirichlet, c_int32_t)
@@ -601,14 +605,18 @@ contains
   end subroutine glissade_velo_higher_solve_2d_gpu
 
   !> the fast path's options (absent: the reference path, precision 0)
-  subroutine fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
+  subroutine fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m, coarse, &
+                               skip_final_residual)
     type(cism_solve_opts), intent(inout) :: o
-    integer, intent(in), optional :: precision, adaptive_linear, anderson_m
+    integer, intent(in), optional :: precision, adaptive_linear, anderson_m, coarse
+    integer, intent(in), optional :: skip_final_residual
     real(dp), intent(in), optional :: inner_rtol
     if (present(precision)) o%precision = precision
     if (present(adaptive_linear)) o%adaptive_linear = adaptive_linear
     if (present(inner_rtol)) o%inner_rtol = inner_rtol
     if (present(anderson_m)) o%anderson_m = anderson_m
+    if (present(coarse)) o%coarse = coarse
+    if (present(skip_final_residual)) o%skip_final_residual = skip_final_residual
   end subroutine fast_path_options
 
   !> Loads the learned initial guess's weight file (cism_learned_guess_load_file) into the
@@ -634,7 +642,8 @@ contains
                                             maxiter_nonlinear, efvs_constant, uvel, vvel, efvs, &
                                             picard_iters, linear_iters, whichassemble_taud,     &
                                             precision, adaptive_linear, inner_rtol, anderson_m, &
-                                            learned_guess, converged, fallbacks, periodic)
+                                            learned_guess, converged, fallbacks, periodic,   &
+                                            coarse, skip_final_residual)
     integer, intent(in) :: nx, ny, nz
     real(dp), intent(in) :: dx, dy, thklim, eus, efvs_constant
     real(dp), dimension(nz), intent(in) :: sigma
@@ -651,6 +660,7 @@ contains
     logical, intent(in), optional :: learned_guess, periodic
     logical, intent(out), optional :: converged
     integer, intent(out), optional :: fallbacks
+    integer, intent(in), optional :: coarse, skip_final_residual
     type(cism_solve_opts) :: o
     type(cism_solve_stats) :: stats
     type(cism_velo_setup) :: su
@@ -662,7 +672,8 @@ contains
     o%whichsparse = whichsparse; o%whichprecond = whichprecond; o%whichapprox = whichapprox
     o%maxiter_nonlinear = maxiter_nonlinear; o%efvs_constant = efvs_constant
     o%dx = dx; o%dy = dy
-    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m)
+    call fast_path_options(o, precision, adaptive_linear, inner_rtol, anderson_m, coarse, &
+                           skip_final_residual)
     su%dx = dx; su%dy = dy; su%thklim = thklim; su%eus = eus; su%max_slope = 1.0_dp
     su%whichground = 0; su%lateral_bc = 1; su%initial_guess = 0; su%reserved = 0
     su%whichassemble_taud = 1

@@ -101,7 +101,7 @@ program fortran_driver
      call glissade_velo_higher_solve_gpu(nx, ny, nz, scal(1), scal(1), sigma, thk, srf, bed, flw,  &
           scal(2), 0.0_dp, 5, beta, wefvs, 3, 3, precond, wapprox, 100, scal(3), xu, xv, efvs,    &
           n1, n2, precision=nint(scal(4)), adaptive_linear=1, inner_rtol=5.0e-2_dp, anderson_m=3, &
-          learned_guess=hdr(8) /= 0, converged=conv, fallbacks=nfb)
+          learned_guess=hdr(8) /= 0, converged=conv, fallbacks=nfb, coarse=1, skip_final_residual=1)
      scal = [real(n1, dp), real(n2, dp), merge(1.0_dp, 0.0_dp, conv), real(nfb, dp)]
      write(11) xu, xv, scal
   case (7)   ! the 2D solvers (SSA, L1L2, DIVA) on the module's one-level context

@@ -185,7 +185,7 @@ int cism_glissade_velo_higher_solve(cism_ctx *c, const cism_solve_opts *o, const
     st->picard_iters = o->whichbabc + 100 * o->adaptive_linear;
     st->linear_iters = su->lateral_bc + 10 * su->whichassemble_taud;
     st->converged = 1;
-    st->fallbacks = (int)(100.0 * su->thklim);
+    st->fallbacks = (int)(100.0 * su->thklim) + 1000 * o->coarse + 2000 * o->skip_final_residual;
     return CISM_OK;
 }
 

@@ -125,7 +125,7 @@ def test_geometry_solve_marshalling(driver, tmp_path):
     """glissade_velo_higher_solve_gpu: the cell fields, beta_external and the fast path's
     optional arguments reach cism_glissade_velo_higher_solve; the weight file is loaded
     first (mode 6 of the driver: precision from scal(4), adaptive 1, inner_rtol 5e-2,
-    Anderson 3, learned guess from header word 8)"""
+    Anderson 3, coarse 1, skip_final_residual 1, learned guess from header word 8)"""
     sigma = np.linspace(0, 1, NZ)
     thk, srf, bed = rnd((NX, NY)), rnd((NX, NY)), rnd((NX, NY))
     flw, bext = rnd(C), rnd((NX - 1, NY - 1))
@@ -146,8 +146,8 @@ def test_geometry_solve_marshalling(driver, tmp_path):
     np.testing.assert_array_equal(u, np.broadcast_to((thk[cell] + flw[0][cell])[None] + 1.0 + 10.0, V))
     np.testing.assert_array_equal(v, np.broadcast_to((srf[cell] + bed[cell] + bext)[None] + 3.0, V))
     # picard = whichbabc 5 + 100 adaptive; linear = lateral 1 + 10 local taud; converged;
-    # fallbacks = 100 thklim
-    assert list(sc) == [105.0, 11.0, 1.0, 25.0]
+    # fallbacks = 100 thklim + 1000 coarse + 2000 skip_final_residual
+    assert list(sc) == [105.0, 11.0, 1.0, 3025.0]
 
 
 @pytest.mark.parametrize("approx", [1, 3, 4])

@@ -139,7 +139,8 @@ def test_fortran_geometry_solve_with_the_learned_guess(tmp_path):
         ctx.learned_guess_load(params, meta["nz"], 0, meta["uscale"])
         rp = ctx.glissade_velo_higher_solve(p, bp=G.BasalPhysics(configs.HO_BABC_BETA_EXTERNAL),
                                             beta_external=bext, initial_guess=1, precision=1,
-                                            adaptive_linear=1, inner_rtol=5e-2, anderson_m=3)
+                                            adaptive_linear=1, inner_rtol=5e-2, anderson_m=3,
+                                            coarse=1, skip_final_residual=1)
     u, v, sc = runs[1]
     assert rp["picard_iters"] == int(sc[0])
     assert np.array_equal(rp["uvel"], u) and np.array_equal(rp["vvel"], v)
