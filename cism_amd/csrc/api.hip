@@ -82,6 +82,7 @@ int ensure_ke(cism_ctx *c) {
 
 static int upload_block(cism_ctx *c, int b, const double *h) {
     c->a_zero_ok = 0;   // the caller's matrix may hold anything in inactive rows
+    c->a64_valid = 1;
     HIP_TRY(hipMemcpyAsync(c->stage, h, sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
                            c->stream));
     hipLaunchKernelGGL(k_fortran_to_tiled, dim3((unsigned)((c->g.nn + 63) / 64)), dim3(256), 0,
@@ -254,6 +255,7 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     if ((st = dalloc(&c->sc, 1))) return fail(st);
     if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
         return fail(CISM_ERR_HIP);
+    c->a64_valid = 1;
     *out = c;
     return CISM_OK;
 }
@@ -661,6 +663,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
         cism_set_error("anderson_m must be 0..%d", AA_MAX);
         return CISM_ERR_ARG;
     }
+    if (o->skip_final_residual < 0 || o->skip_final_residual > 2) {
+        cism_set_error("skip_final_residual must be 0, 1 or 2");
+        return CISM_ERR_ARG;
+    }
     // the depth-integrated 2D solvers L1L2 (3) and DIVA (4); with a no-slip bed the SSA and
     // L1L2 velocities are zero and the loop ends at once (velo_higher:3234-3245)
     const bool depth = dim2 && (o->whichapprox == 3 || o->whichapprox == 4);
@@ -718,8 +724,12 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                         CISM_TRY(ensure_a32(c));
                         CISM_TRY(launch_halo_vec(c, c->x));   // the gather's residual reads x's halo
                     }
+                    // with skip_final_residual the FP64 matrix is read at most once per
+                    // Picard iteration, by a refinement residual that the element matrices
+                    // give as well (residual_from_elements): its rows are not written
                     CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
-                                             basal && !standard, o->dx, o->dy, fuse32 ? c->A32 : nullptr));
+                                             basal && !standard, o->dx, o->dy, fuse32 ? c->A32 : nullptr,
+                                             fuse32 ? o->skip_final_residual == 0 : 1));
                 }
                 if (basal && standard)
                     CISM_TRY(basal_standard_device(c, depth ? depth_basal_beta(c, o->whichapprox) : nullptr));
@@ -732,7 +742,7 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             }
             {
                 TimerRange r_h("glissade_halo_Axxs", c->stream, true);
-                CISM_TRY(launch_halo_mat(c, c->A));
+                if (c->a64_valid) CISM_TRY(launch_halo_mat(c, c->A));
                 CISM_TRY(launch_halo_vec(c, c->b));
             }
             TimerRange r_s("glissade_chk_symmetry");   // ends at the synchronization below
@@ -835,6 +845,14 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                     CISM_TRY(anderson_reset(c));
                     HIP_TRY(hipMemcpyAsync(c->x, c->usav, sizeof(double) * 2 * g.npad,
                                            hipMemcpyDeviceToDevice, c->stream));
+                    if (!c->a64_valid) {
+                        // the mixed path kept no FP64 rows: this Picard iteration's matrix
+                        // again (the same velocity), as the reference path forms it
+                        CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
+                                                 c->asm_add_basal, o->dx, o->dy));
+                        CISM_TRY(launch_halo_mat(c, c->A));
+                        CISM_TRY(symmetry_device(c, nullptr));
+                    }
                     int n2 = 0;
                     st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err, &n2);
                     niters += n2;
@@ -1480,6 +1498,10 @@ extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per
     if (reps < 1 || which < 0 || which > 4) return CISM_ERR_ARG;
     if ((which == 1 || which == 4) && !c->A32) {
         cism_set_error("no FP32 matrix yet: run a mixed-precision solve first");
+        return CISM_ERR_ARG;
+    }
+    if (which == 0 && !c->a64_valid) {
+        cism_set_error("no FP64 matrix: the last mixed-precision solve (skip_final_residual) kept none");
         return CISM_ERR_ARG;
     }
     // which 3: 1 GiB read + 1 GiB write copy, buffers owned by this call

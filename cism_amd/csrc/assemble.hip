@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                                                 const int32_t *__restrict__ ac_zero,
                                                 const double *__restrict__ Ke,
                                                 const double *__restrict__ beta, int add_basal,
-                                                double basal_scale, double *__restrict__ A,
+                                                double basal_scale, double *__restrict__ A, int store64,
                                                 float *__restrict__ As, const int32_t *__restrict__ as_av,
                                                 const int32_t *__restrict__ as_ac,
                                                 const double *__restrict__ x, const double *__restrict__ bl,
@@ -513,8 +513,12 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
     // active rows (0 elsewhere), rsv = b - A x on owned rows (the halo rows follow from their
     // images by a vector halo update); partials per block: |tv|^2 and |b|^2 over owned
     // active rows, |rsv|^2 and |b|^2 over owned rows, the nonzeros of owned active rows
-    // (every lane reaches the reduction).  x must carry its halo.
+    // (every lane reaches the reduction).  x must carry its halo.  store64 = 0: the FP64
+    // matrix is not written (the mixed path that confirms no linear solve in FP64 reads it
+    // at most once per Picard iteration: its residual comes from a pass of this kernel with
+    // only rsv, from the element matrices).
     double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool res = rsv != nullptr;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < (int)g.nn) {
     const int k = q % g.nz + 1;                // 1-based
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
     const bool basal = add_basal && k == g.nz && av[col];
     if (!any && !basal) {
         const bool owned = is_owned(g, i - 1, j - 1);
-        if (!(av_zero && !was && !av[col] && !av_zero[col] && owned)) {
+        if (store64 && !(av_zero && !was && !av[col] && !av_zero[col] && owned)) {
             const double2 z = make_double2(0.0, 0.0);
 #pragma unroll
             for (int m = 0; m < 27; ++m) {
@@ -559,8 +563,8 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
             for (int s = 0; s < CISM_SYM_SLOTS; ++s)
                 *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if (As) {   // no active row here (an owned active vertex touches an active cell)
-            reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
+        if (res) {   // no active row here (an owned active vertex touches an active cell)
+            if (tv) reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
             if (owned) {
                 const double2 bb = reinterpret_cast<const double2 *>(bl)[q];
                 reinterpret_cast<double2 *>(rsv)[q] = bb;
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
         }
     } else {
     const bool owned = is_owned(g, i - 1, j - 1);
-    const bool count = As && owned && av[col];
+    const bool count = res && owned && av[col];
     double nnz = 0.0, su = 0.0, sv = 0.0;
     // slot by slot, the four blocks together: each slot's coefficients leave as two
     // 16-byte stores (the tile keeps a node's four blocks of one slot adjacent)
@@ -595,7 +599,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
             if (basal && m == CISM_DIAG && (blk == 0 || blk == 3))
                 acc[blk] = acc[blk] + basal_scale * beta[col];
         }
-        store_blocks_nt(A, q, m, acc[0], acc[1], acc[2], acc[3]);
+        if (store64) store_blocks_nt(A, q, m, acc[0], acc[1], acc[2], acc[3]);
         if (As && m >= CISM_DIAG)
             *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) =
                 make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
@@ -610,18 +614,18 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
             }
         }
     }
-    if (As) {
+    if (res) {
         const double2 bb = reinterpret_cast<const double2 *>(bl)[q];
         if (count) {
             const double ru = su - bb.x, rv = sv - bb.y;
-            reinterpret_cast<double2 *>(tv)[q] = make_double2(ru, rv);
+            if (tv) reinterpret_cast<double2 *>(tv)[q] = make_double2(ru, rv);
             reinterpret_cast<double2 *>(rsv)[q] = make_double2(-ru, -rv);
             red[0] = ru * ru + rv * rv;
             red[1] = bb.x * bb.x + bb.y * bb.y;
             red[2] = ru * ru + rv * rv;
             red[3] = bb.x * bb.x + bb.y * bb.y;
         } else {
-            reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
+            if (tv) reinterpret_cast<double2 *>(tv)[q] = make_double2(0.0, 0.0);
             if (owned) {
                 reinterpret_cast<double2 *>(rsv)[q] = bb;
                 red[2] = bb.x * bb.x + bb.y * bb.y;
@@ -632,7 +636,16 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
     red[4] = nnz;
     }
     }
-    if (partials) block_reduce_store<5>(red, partials);
+    if (partials) {
+        if (res && !tv) {
+            // the residual-only pass: |r|^2 alone, in k_residual's one-value record, so that
+            // its reduction adds in the same order
+            double r1[1] = {red[2]};
+            block_reduce_store<1>(r1, partials);
+        } else {
+            block_reduce_store<5>(red, partials);
+        }
+    }
 }
 
 // basal_sliding_bc with HO_ASSEMBLE_BETA_STANDARD (velo_higher:8236-8363) in gather
@@ -730,7 +743,7 @@ int basal_standard_device(cism_ctx *c, const double *beta) {
 }
 
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
-                    int add_basal, double dx, double dy, float *As) {
+                    int add_basal, double dx, double dy, float *As, int store64) {
     CISM_TRY(ensure_ke(c));
     const GridDev &g = c->g;
     const int nb = (int)((g.ne + EL_ELEM - 1) / EL_ELEM);
@@ -740,9 +753,10 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
     HIP_TRY(hipGetLastError());
     const double basal_scale = dx * dy / CISM_VOL0;
     const int nbg = nblocks_for(g.nn);
+    if (!As) store64 = 1;
     hipLaunchKernelGGL(k_gather, dim3(nbg), dim3(256), 0, c->stream, g, c->ac, c->av,
                        c->a_zero_ok ? c->av_zero : nullptr, c->a_zero_ok ? c->ac_zero : nullptr, c->Ke, c->beta,
-                       add_basal, basal_scale, c->A, As, As && c->as_ok ? c->as_av_prev : nullptr,
+                       add_basal, basal_scale, c->A, store64, As, As && c->as_ok ? c->as_av_prev : nullptr,
                        As && c->as_ok ? c->as_ac_prev : nullptr, c->x, c->load, As ? c->t : nullptr,
                        As ? c->r : nullptr, As ? c->partials : nullptr);
     HIP_TRY(hipGetLastError());
@@ -763,7 +777,11 @@ int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_con
                            c->stream));
     HIP_TRY(hipMemcpyAsync(c->ac_zero, c->ac, sizeof(int32_t) * (size_t)g.nx * g.ny, hipMemcpyDeviceToDevice,
                            c->stream));
-    c->a_zero_ok = 1;
+    // without the FP64 rows (store64 = 0) neither the matrix nor its zero rows are current
+    c->a_zero_ok = store64;
+    c->a64_valid = store64;
+    c->asm_add_basal = add_basal;
+    c->asm_basal_scale = basal_scale;
     return CISM_OK;
 }
 
@@ -1003,6 +1021,23 @@ int symmetry_device(cism_ctx *c, float *As) {
     }
     CISM_TRY(launch_reduce(c, nb, 2, RED_SUMS));
     return CISM_OK;
+}
+
+// r = b - A x (c->r; halo rows from their images) of the last assembly's matrix, formed
+// from its element matrices by a pass of the gather that stores nothing else: the FP64
+// residual of the mixed path after an assembly without the FP64 rows, the same values as
+// k_residual's from the stored matrix (and the same norm, reduced in the same order).
+// sums[0] = |r|^2 over owned nodes.  x must carry its halo.
+int residual_from_elements(cism_ctx *c) {
+    const GridDev &g = c->g;
+    const int nbg = nblocks_for(g.nn);
+    hipLaunchKernelGGL(k_gather, dim3(nbg), dim3(256), 0, c->stream, g, c->ac, c->av, (const int32_t *)nullptr,
+                       (const int32_t *)nullptr, c->Ke, c->beta, c->asm_add_basal, c->asm_basal_scale, c->A, 0,
+                       (float *)nullptr, (const int32_t *)nullptr, (const int32_t *)nullptr, c->x, c->b,
+                       (double *)nullptr, c->r, c->partials);
+    HIP_TRY(hipGetLastError());
+    CISM_TRY(launch_reduce(c, nbg, 1, RED_SUMS));
+    return launch_halo_vec(c, c->r);
 }
 
 // count_nonzeros_3d (velo_higher:9355-9410) -> sums[0]

@@ -138,6 +138,11 @@ struct cism_ctx {
     // it.  Any other writer of A clears a_zero_ok.
     int32_t *av_zero, *ac_zero;
     int a_zero_ok;
+    // c->A holds the last assembly's matrix (0 after a mixed-path assembly that did not
+    // write it); the basal terms of that assembly, for residual_from_elements
+    int a64_valid;
+    int asm_add_basal;
+    double asm_basal_scale;
     // the same record for the FP32 symmetric matrix: the masks of the last symmetry
     // pass that wrote it from an assembled A (as_av_prev, as_ac_prev), valid when as_ok
     int32_t *as_av_prev, *as_ac_prev;
@@ -413,7 +418,7 @@ int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double t
 // sums[3] = |b|^2, as residual_vector_device leaves them with rs = c->r
 // stalled (optional): set to 1 when the refinement is given up (refine_policy.h);
 // unconverged (optional): set to 1 when its budget ran out before the tolerance
-// skip_final: end after the first refinement step without the confirming FP64 residual
+// skip_final = 1: end after the first refinement step without the confirming FP64 residual
 // when the inner solve's own estimate meets half the tolerance (cism_solve_opts)
 int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inner_rtol,
                  double *err, int *niters, int *n64, int *n32, int r_ready = 0,
@@ -424,9 +429,12 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
 // residual_vector_device) and the nonzero count (sums[4]) in the gather, for a matrix that
 // no later pass changes except the halo update (no Dirichlet rows, basal friction local);
 // x must carry its halo; launch_a32_halo and launch_halo_vec(c->r) complete the halo rows
+// store64 = 0 (with As only): the FP64 rows are not written (c->a64_valid = 0);
+// residual_from_elements then gives the FP64 residual of that matrix
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
-                    int add_basal, double dx, double dy, float *As = nullptr);
+                    int add_basal, double dx, double dy, float *As = nullptr, int store64 = 1);
 int launch_a32_halo(cism_ctx *c);
+int residual_from_elements(cism_ctx *c);   // c->r = b - A x, sums[0] = |r|^2 (owned)
 int dirichlet_device(cism_ctx *c);
 // basal friction by the standard finite-element assembly onto the assembled c->A (c->ac,
 // xV, yV, beta); the local assembly is part of the gather (assemble_device's add_basal)

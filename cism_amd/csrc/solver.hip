@@ -712,10 +712,10 @@ static int launch_halo_t(cism_ctx *c, T *v) {
     return CISM_OK;
 }
 
-// the FP32 symmetric rows (slots CISM_DIAG .. 26) of the halo nodes from the FP64 matrix
-// after its halo update, for an FP32 matrix written by the gather
-__global__ __launch_bounds__(CISM_BLOCK) void k_a32_halo(GridDev g, const double *__restrict__ A,
-                                                         float *__restrict__ As) {
+// the FP32 symmetric rows (slots CISM_DIAG .. 26) of the halo nodes: their images' rows
+// (zero without an image), as the FP64 halo update makes them, for an FP32 matrix written
+// by the gather
+__global__ __launch_bounds__(CISM_BLOCK) void k_a32_halo(GridDev g, float *__restrict__ As) {
     const long long nh = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nh * CISM_SYM_SLOTS) return;
@@ -724,16 +724,17 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_a32_halo(GridDev g, const double
     const int k = (int)(hv % g.nz);
     int i, j;
     if (!halo_vertex_of(g, hv / g.nz, i, j)) return;
+    const int ii = halo_image(i, g.sx, g.periodic), jj = halo_image(j, g.sy, g.periodic);
     const long long q = k + (long long)g.nz * (i + (long long)g.sx * j);
-    double a0, a1, a2, a3;
-    load_blocks(A, q, CISM_DIAG + s, a0, a1, a2, a3);
-    *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ii >= 0 && jj >= 0) v = *reinterpret_cast<const float4 *>(As + sidx(0, s, k + (long long)g.nz * (ii + (long long)g.sx * jj)));
+    *reinterpret_cast<float4 *>(As + sidx(0, s, q)) = v;
 }
 
 int launch_a32_halo(cism_ctx *c) {
     const GridDev &g = c->g;
     const long long n = (3LL * g.sx + 3LL * (g.sy - 3)) * g.nz * CISM_SYM_SLOTS;
-    hipLaunchKernelGGL(k_a32_halo, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, g, c->A, c->A32);
+    hipLaunchKernelGGL(k_a32_halo, dim3(nblocks_for(n)), dim3(CISM_BLOCK), 0, c->stream, g, c->A32);
     HIP_TRY(hipGetLastError());
     return CISM_OK;
 }
@@ -1194,12 +1195,21 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
             hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 2);
         } else {
             CISM_TRY(launch_halo_vec(c, c->x));
-            hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK),
-                               0, c->stream, g, MatFull<double>{c->A}, c->av, c->x, c->b, c->r, 1, c->partials);
-            CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
-            hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 0);
-            CISM_TRY(sync_scalars(c));
-            rr = c->sc_host->sums[0];
+            if (c->a64_valid) {
+                hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)),
+                                   dim3(CISM_BLOCK), 0, c->stream, g, MatFull<double>{c->A}, c->av, c->x, c->b,
+                                   c->r, 1, c->partials);
+                CISM_TRY(launch_reduce(c, nblocks_for(g.nn), 1, RED_SUMS));
+                hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 0);
+                CISM_TRY(sync_scalars(c));
+                rr = c->sc_host->sums[0];
+            } else {
+                // the assembly kept no FP64 rows: the same residual from its element matrices
+                CISM_TRY(residual_from_elements(c));
+                hipLaunchKernelGGL(k_save_scale, dim3(1), dim3(1), 0, c->stream, c->sc, 0);
+                CISM_TRY(sync_scalars(c));
+                rr = c->sc_host->sums[0];
+            }
         }
         ++*n64;
         err = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
@@ -1238,7 +1248,7 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
         // relative to |r|), so the FP64 relative residual is about err * e_in; when that is
         // below half the tolerance and e_in is far above FP32's rounding floor, the FP64
         // residual that would confirm it is left to the next Picard iteration's residual
-        if (skip_final && outer == 1 && inner_conv && e_in >= 1.0e-4 && err * e_in <= 0.5 * tol) {
+        if (skip_final == 1 && outer == 1 && inner_conv && e_in >= 1.0e-4 && err * e_in <= 0.5 * tol) {
             err = err * e_in;
             break;
         }
@@ -1321,6 +1331,10 @@ int guess_energy_device(cism_ctx *c, const double *t, double *twoJ) {
 
 int column_scale_device(cism_ctx *c, double *out) {
     const GridDev &g = c->g;
+    if (!c->a64_valid) {
+        cism_set_error("no FP64 matrix: the last mixed-precision solve (skip_final_residual) kept none");
+        return CISM_ERR_ARG;
+    }
     const long long ncol = (long long)g.sx * g.sy;
     CISM_TRY(launch_halo_vec(c, c->x));
     hipLaunchKernelGGL((k_residual<double, MatFull<double>>), dim3(nblocks_for(g.nn)), dim3(CISM_BLOCK), 0,

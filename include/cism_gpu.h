@@ -230,8 +230,16 @@ typedef struct {
                                   confirm it, when the FP32 inner solve's own relative
                                   residual e (>= 1e-4, above FP32's rounding floor) times
                                   the step's FP64 residual is below half the tolerance; the
-                                  next Picard iteration's residual checks the result.  0
-                                  (default): every step is confirmed in FP64.             */
+                                  next Picard iteration's residual checks the result.  Where
+                                  the gather forms the FP32 matrix itself (no Dirichlet row,
+                                  local basal friction) the FP64 matrix is then not stored:
+                                  the few FP64 residuals left come from the element
+                                  matrices, and cism_column_scale_target and the FP64
+                                  matvec timing need a new assembly.  2: no FP64 matrix
+                                  either, but every step confirmed, its FP64 residual from
+                                  the element matrices (the same values as from the stored
+                                  matrix).  0 (default): every step is confirmed in FP64
+                                  with the stored matrix.                                  */
 } cism_solve_opts;
 
 typedef struct {
@@ -282,7 +290,9 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
 /* Kernel timing on the context's stream with HIP events, for roofline reports
  * (no reference counterpart).  Each times `reps` launches on the matrix and
  * vectors left by the last solve and returns the mean duration in ms.
- *   which = 0 FP64 matvec (with the fused Chron-Gear dot products)
+ *   which = 0 FP64 matvec (with the fused Chron-Gear dot products; not after a mixed
+ *             solve with skip_final_residual, which keeps no FP64 matrix: assemble with
+ *             which = 2 first)
  *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
  *           2 assembly (element kernel + node gather) of the current velocity
  *           3 plain copy of 1 GiB (k_copy_calib: 1 GiB read + 1 GiB written), the

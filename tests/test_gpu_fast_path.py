@@ -149,3 +149,25 @@ def test_skipping_the_final_residual_keeps_the_iterates(setup):
     for r in (r0, r1):
         assert r["converged"] and 0 <= r["final_rel_err"] <= 1e-6 and r["fallbacks"] == 0
     assert r1["fp64_matvecs"] < r0["fp64_matvecs"]
+
+
+def test_fp64_residual_from_element_matrices(setup):
+    """With skip_final_residual 1 or 2 the mixed path stores no FP64 matrix; a refinement
+    step's FP64 residual then comes from the element matrices (the gather's own sums, in
+    the stored matrix's operation order).  Mode 2 confirms every step as mode 0 does, so
+    the two take the same steps and must agree bit for bit."""
+    p, ctx, lu, lv, ref = setup
+    kw = dict(inner_rtol=0.3, coarse=1)
+    r0 = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, stop_rel_err=1e-6,
+                                  maxiter_nonlinear=300, skip_final_residual=0, **kw)
+    r1 = ctx.velo_higher_solve_3d(p, lu, lv, precision=1, adaptive_linear=1, stop_rel_err=1e-6,
+                                  maxiter_nonlinear=300, skip_final_residual=2, **kw)
+    print(f"[fast] stored FP64 matrix: {r0['picard_iters']} Picard, {r0['fp64_matvecs']} FP64; "
+          f"element residuals: {r1['picard_iters']} Picard, {r1['fp64_matvecs']} FP64")
+    assert r0["fp64_matvecs"] > r0["picard_iters"]   # refinement residuals did run
+    for key in ("picard_iters", "fp32_matvecs", "fp64_matvecs"):
+        assert r1[key] == r0[key]
+    assert np.array_equal(r1["uvel"], r0["uvel"]) and np.array_equal(r1["vvel"], r0["vvel"])
+    # the FP64 matrix is gone: the kernels that need it say so
+    with pytest.raises(G.CismError):
+        ctx.column_scale_target()
