@@ -1278,6 +1278,7 @@ extern "C" int cism_set_precond_scale(cism_ctx *c, const double *scale_u, const 
     CTX_BEGIN(c);
     if (!scale_u || !scale_v) {
         c->cscale_on = 0;
+        coarse_invalidate(c);   // the coarse setup's SIA factors carry the scale
         return CISM_OK;
     }
     const long long ncol = (long long)c->g.sx * c->g.sy;
@@ -1297,6 +1298,7 @@ extern "C" int cism_set_precond_scale(cism_ctx *c, const double *scale_u, const 
         return CISM_ERR_ARG;
     }
     c->cscale_on = 1;
+    coarse_invalidate(c);
     return CISM_OK;
 }
 
@@ -1440,7 +1442,7 @@ extern "C" int cism_coarse_build(cism_ctx *c, const int32_t *active_vertex, cons
     c->as_ok = 0;
     CISM_TRY(ensure_a32(c));
     CISM_TRY(symmetry_device(c, c->A32));
-    CISM_TRY(coarse_setup(c, c->A32));
+    CISM_TRY(coarse_setup(c, c->A32, c->F32, c->cscale_on ? c->cscale : nullptr));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (nlevels) *nlevels = c->coarse->nlev;
     return CISM_OK;

@@ -99,11 +99,16 @@ __device__ inline double cg_sum16(double x) {
 // entries do not cancel, so FP32 keeps them to its rounding; the cancellation of the
 // Galerkin sums happens in the FP64 passes after this one.
 #define CG_NZMAX 48
+//
+// Sf (optional): the same block also forms the SIA preconditioner's factors of its columns
+// (k_sia_setup_sym32's operations: the diagonal slot kept from the first round, the slot
+// above at round 9), scaled by cs when given, so that the coarse setup replaces that pass.
 template <int NZMAX>
 __global__ __launch_bounds__(256) void k_cg_colblocks(GridDev g, const float *__restrict__ As,
                                                       const int32_t *__restrict__ av,
                                                       const int32_t *__restrict__ um,
-                                                      const int32_t *__restrict__ vm, float *__restrict__ F) {
+                                                      const int32_t *__restrict__ vm, float *__restrict__ F,
+                                                      const float *__restrict__ cs, float *__restrict__ Sf) {
     const int nz = g.nz;
     const long long ncol = (long long)g.sx * g.sy;
     const long long col0 = (long long)blockIdx.x * 64;
@@ -111,6 +116,7 @@ __global__ __launch_bounds__(256) void k_cg_colblocks(GridDev g, const float *__
     const long long Q0 = col0 * nz;
     const int nn = ncols * nz;
     __shared__ float4 st[64 * NZMAX];
+    __shared__ float4 sd[64 * NZMAX];   // the diagonal slot, for the SIA factors
     const int c = threadIdx.x;
     const bool mine = c < ncols && av[col0 + c];
     float4 acc[10];
@@ -120,9 +126,35 @@ __global__ __launch_bounds__(256) void k_cg_colblocks(GridDev g, const float *__
     for (int s = 0; s < CISM_SYM_SLOTS; ++s) {
         const int m = CISM_DIAG + s, kA = m / 9 - 1, d = s == 0 ? 9 : m % 9;
         __syncthreads();
-        for (int t = threadIdx.x; t < nn; t += blockDim.x)
-            st[t] = *reinterpret_cast<const float4 *>(As + sidx(0, s, Q0 + t));
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            const float4 a = *reinterpret_cast<const float4 *>(As + sidx(0, s, Q0 + t));
+            st[t] = a;
+            if (s == 0 && Sf) sd[t] = a;
+        }
         __syncthreads();
+        if (s == 9 && Sf) {
+            // SIA factors (k_sia_setup_sym32): lane (comp, column) of the first 128
+            const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
+            const long long colc = col0 + lc;
+            if (threadIdx.x < 128 && lc < ncols && av[colc]) {
+                auto pick = [&](const float4 &v) { return comp == 0 ? v.x : v.w; };
+                auto Fi = [&](int f, int k) { return ((long long)(f * 2 + comp) * nz + k) * ncol + colc; };
+                const float inv = cs ? 1.0f / cs[comp * ncol + colc] : 1.0f;
+                const float4 *d = sd + lc * nz, *u = st + lc * nz;
+                float beta = pick(d[0]);
+                Sf[Fi(0, 0)] = 0.0f;
+                Sf[Fi(1, 0)] = cs ? beta * inv : beta;
+                Sf[Fi(2, 0)] = 0.0f;
+                for (int k = 1; k < nz; ++k) {
+                    const float gam = pick(u[k - 1]) / beta;
+                    const float sb = pick(u[k - 1]);
+                    beta = pick(d[k]) - sb * gam;
+                    Sf[Fi(0, k)] = cs ? sb * inv : sb;
+                    Sf[Fi(1, k)] = cs ? beta * inv : beta;
+                    Sf[Fi(2, k)] = gam;
+                }
+            }
+        }
         if (mine) {
             for (int k = 0; k + kA < nz; ++k) {
                 float4 B = st[c * nz + k];
@@ -566,7 +598,7 @@ void coarse_invalidate(cism_ctx *c) {
     if (c->coarse) c->coarse->ok = 0;
 }
 
-int coarse_setup(cism_ctx *c, const float *As) {
+int coarse_setup(cism_ctx *c, const float *As, float *Sf, const float *cs) {
     if (c->g.nz > CG_NZMAX) {
         cism_set_error("coarse correction supports nz <= %d (nz = %d)", CG_NZMAX, c->g.nz);
         return CISM_ERR_UNSUPPORTED;
@@ -583,11 +615,12 @@ int coarse_setup(cism_ctx *c, const float *As) {
     const int32_t *um = c->dirichlet_any ? c->umask : nullptr, *vm = c->dirichlet_any ? c->vmask : nullptr;
     const dim3 gcol((unsigned)((ncol + 63) / 64));
     if (g.nz <= 12)
-        hipLaunchKernelGGL(k_cg_colblocks<12>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF);
+        hipLaunchKernelGGL(k_cg_colblocks<12>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF, cs, Sf);
     else if (g.nz <= 24)
-        hipLaunchKernelGGL(k_cg_colblocks<24>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF);
+        hipLaunchKernelGGL(k_cg_colblocks<24>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF, cs, Sf);
     else
-        hipLaunchKernelGGL(k_cg_colblocks<CG_NZMAX>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF);
+        hipLaunchKernelGGL(k_cg_colblocks<CG_NZMAX>, gcol, dim3(256), 0, c->stream, g, As, c->av, um, vm, h->colF, cs,
+                           Sf);
     hipLaunchKernelGGL(k_cg_gal1, dim3(nblocks_for(16 * n1)), dim3(CISM_BLOCK), 0, c->stream, g, h->colF, c->av,
                        h->cidx, lv[0].ax, lv[0].ay, h->S);
     hipLaunchKernelGGL(k_cg_combine1, dim3(nblocks_for(n1)), dim3(CISM_BLOCK), 0, c->stream, lv[0].ax, lv[0].ay,

@@ -471,7 +471,9 @@ int basal_fields_upload(cism_ctx *c, const cism_basal_physics *bp, const double 
 // symmetry pass that rewrites A32 calls it); coarse_colsum forms the column sums of r that
 // coarse_apply restricts (the fused FP32 update writes them itself); coarse_apply adds
 // P V(P^T r) to z.
-int coarse_setup(cism_ctx *c, const float *As);
+// Sf, cs: the SIA preconditioner's FP32 factors are formed in the same pass (scaled by cs
+// when not null); they stay current with the hierarchy
+int coarse_setup(cism_ctx *c, const float *As, float *Sf, const float *cs);
 void coarse_invalidate(cism_ctx *c);
 int coarse_colsum(cism_ctx *c, const float *r);
 int coarse_apply(cism_ctx *c, float *z);

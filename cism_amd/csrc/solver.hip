@@ -176,6 +176,7 @@ int launch_reduce(cism_ctx *c, int nb, int nv, int mode) {
     if (nv == 1) launch_reduce2<1>(c, nb, mode);
     else if (nv == 2) launch_reduce2<2>(c, nb, mode);
     else if (nv == 4) launch_reduce2<4>(c, nb, mode);
+    else if (nv == 5) launch_reduce2<5>(c, nb, mode);
     else
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RED_THREADS), 0, c->stream, c->partials, nb, nv, mode, c->sc);
     HIP_TRY(hipGetLastError());
@@ -569,10 +570,16 @@ __device__ __forceinline__ void sia_column(int nz, long long ncol, const T *__re
 
 // z = M^-1 r, M the SIA (column tridiagonal) preconditioner: one block per 64
 // columns, one lane per column and component; r staged in LDS for coalescing
+// colsum (optional): the (u, v) sums of r over the levels of owned active columns, as the
+// fused update leaves them (prescribed components left out with um/vm), for the coarse
+// correction's restriction
 template <typename T, int NZMAX>
 __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restrict__ F,
                                                    const int32_t *__restrict__ av,
-                                                   const T *__restrict__ r, T *__restrict__ z) {
+                                                   const T *__restrict__ r, T *__restrict__ z,
+                                                   T *__restrict__ colsum = nullptr,
+                                                   const int32_t *__restrict__ um = nullptr,
+                                                   const int32_t *__restrict__ vm = nullptr) {
     const int nz = g.nz;
     const long long ncol = (long long)g.sx * g.sy;
     const long long col0 = (long long)blockIdx.x * 64;
@@ -584,7 +591,19 @@ __global__ __launch_bounds__(128) void k_sia_apply(GridDev g, const T *__restric
     __syncthreads();
     const int comp = threadIdx.x >> 6, lc = threadIdx.x & 63;
     const long long col = col0 + lc;
-    if (lc < ncols && av[col]) sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
+    if (lc < ncols && av[col]) {
+        if (colsum) {
+            const int jc = (int)(col / g.sx), ic = (int)(col - (long long)jc * g.sx);
+            if (is_owned(g, ic, jc)) {
+                const int32_t *dm = comp == 0 ? um : vm;
+                T cs = (T)0;
+                for (int k = 0; k < nz; ++k)
+                    if (!dm || dm[k + (long long)nz * col] != 1) cs += sr[2 * (lc * nz + k) + comp];
+                colsum[2 * col + comp] = cs;
+            }
+        }
+        sia_column<T, NZMAX>(nz, ncol, F, col, comp, sr + 2 * lc * nz + comp);
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < n2; t += blockDim.x)
         if (av[col0 + t / (2 * nz)]) z[2 * Q0 + t] = sr[t];
@@ -640,22 +659,24 @@ static int launch_precond(cism_ctx *c, int precond, const M A, const T *F, const
     } else {
         const long long ncol = (long long)g.sx * g.sy;
         const dim3 grid((unsigned)((ncol + 63) / 64));
+        T *cs = nullptr;
+        if constexpr (std::is_same<M, MatSym32>::value && std::is_same<T, float>::value)
+            if (precond == PRECOND_SIA_COARSE) cs = c->coarse->colsum;
+        const int32_t *um = cs && c->dirichlet_any ? c->umask : nullptr;
+        const int32_t *vm = cs && c->dirichlet_any ? c->vmask : nullptr;
         if (g.nz <= 12)
-            hipLaunchKernelGGL((k_sia_apply<T, 12>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
+            hipLaunchKernelGGL((k_sia_apply<T, 12>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z, cs, um, vm);
         else if (g.nz <= 24)
-            hipLaunchKernelGGL((k_sia_apply<T, 24>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
+            hipLaunchKernelGGL((k_sia_apply<T, 24>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z, cs, um, vm);
         else if (g.nz <= 48)
-            hipLaunchKernelGGL((k_sia_apply<T, 48>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z);
+            hipLaunchKernelGGL((k_sia_apply<T, 48>), grid, dim3(128), 0, c->stream, g, F, c->av, r, z, cs, um, vm);
         else {
             cism_set_error("SIA preconditioner supports nz <= 48 (nz = %d)", g.nz);
             return CISM_ERR_UNSUPPORTED;
         }
         HIP_TRY(hipGetLastError());
         if constexpr (std::is_same<M, MatSym32>::value && std::is_same<T, float>::value)
-            if (precond == PRECOND_SIA_COARSE) {
-                CISM_TRY(coarse_colsum(c, r));
-                CISM_TRY(coarse_apply(c, z));
-            }
+            if (precond == PRECOND_SIA_COARSE) CISM_TRY(coarse_apply(c, z));   // column sums: k_sia_apply
     }
     HIP_TRY(hipGetLastError());
     return CISM_OK;
@@ -1009,15 +1030,22 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     }
     if (precond >= 2) {
         TimerRange r_p("pcg_precond_init", c->stream, true);
-        CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
-        if constexpr (std::is_same<M, MatSym32>::value)
-            if (precond == PRECOND_SIA_COARSE) CISM_TRY(coarse_setup(c, A.A));
+        if constexpr (std::is_same<M, MatSym32>::value && std::is_same<T, float>::value) {
+            // with the coarse correction its setup also forms the SIA factors (in the pass
+            // that stages the matrix's slots per column anyway), once per FP32 matrix
+            if (precond == PRECOND_SIA_COARSE) CISM_TRY(coarse_setup(c, A.A, v.F, v.cs));
+            else CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
+        } else {
+            CISM_TRY((launch_sia_setup<T, M>(c, A, v.cs, v.F)));
+        }
     }
-    hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
-                       v.d, v.z, v.q);
-    // (b, b) -> L2_rhs
-    hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.b, v.b, c->partials);
-    CISM_TRY(launch_reduce(c, nb, 1, RED_BB));
+    if (!x_zero) {   // with x_zero the caller has zeroed z and formed L2_rhs (k_rhs32)
+        hipLaunchKernelGGL(k_zero2<T>, dim3(nblocks_for(2 * np)), dim3(CISM_BLOCK), 0, c->stream, 2 * np,
+                           v.d, v.z, v.q);
+        // (b, b) -> L2_rhs
+        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(CISM_BLOCK), 0, c->stream, g, v.b, v.b, c->partials);
+        CISM_TRY(launch_reduce(c, nb, 1, RED_BB));
+    }
     int niters = maxiters;
     double err = 0.0;
     auto check_conv = [&](int n, bool &done) -> int {
@@ -1129,17 +1157,31 @@ int ensure_a32(cism_ctx *c) {
 
 // b32 = r64 / s (all entries), and the inner solve's start: x32 = 0, so its first
 // residual is b32 itself (halo entries included: r64 carries its halo images)
-__global__ __launch_bounds__(CISM_BLOCK) void k_rhs32(long long n2, const double *__restrict__ r,
+// Also z32 = 0 (the preconditioner writes z on active columns only, and the matvec reads
+// it around them; d and q are written before they are read) and, per block, (b32, b32)
+// over owned nodes: the inner solve's |b| (RED_BB), so that run_cg with x_zero needs
+// neither its zeroing pass nor its dot product.
+__global__ __launch_bounds__(CISM_BLOCK) void k_rhs32(GridDev g, long long n2, const double *__restrict__ r,
                                                       const DevScal *__restrict__ sc,
                                                       float *__restrict__ b32, float *__restrict__ x32,
-                                                      float *__restrict__ r32) {
+                                                      float *__restrict__ r32, float *__restrict__ z32,
+                                                      double *__restrict__ partials) {
+    double acc[1] = {0.0};
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n2) return;
-    const double s = sc->sums[2];
-    const float v = (float)(s > 0.0 ? r[q] / s : 0.0);
-    b32[q] = v;
-    r32[q] = v;
-    x32[q] = 0.0f;
+    if (q < n2) {
+        const double s = sc->sums[2];
+        const float v = (float)(s > 0.0 ? r[q] / s : 0.0);
+        b32[q] = v;
+        r32[q] = v;
+        x32[q] = 0.0f;
+        z32[q] = 0.0f;
+        const long long node = q >> 1;
+        if (node < g.nn) {
+            const long long col = node / g.nz;
+            if (is_owned(g, (int)(col % g.sx), (int)(col / g.sx))) acc[0] = (double)v * v;
+        }
+    }
+    block_reduce_store<1>(acc, partials);
 }
 
 // x += s * e (all entries)
@@ -1223,8 +1265,9 @@ int solver_mixed(cism_ctx *c, int precond, int maxiters, double tol, double inne
             if (unconverged) *unconverged = 1;
             break;
         }
-        hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, n2, c->r, c->sc,
-                           c->b32, c->x32, c->r32);
+        hipLaunchKernelGGL(k_rhs32, dim3(nblocks_for(n2)), dim3(CISM_BLOCK), 0, c->stream, g, n2, c->r, c->sc,
+                           c->b32, c->x32, c->r32, c->z32, c->partials);
+        CISM_TRY(launch_reduce(c, nblocks_for(n2), 1, RED_BB));
         double e_in = 0.0;
         int it = 0;
         // the inner reductions use sums[0..1]; the scale |r| stays in sums[2]
