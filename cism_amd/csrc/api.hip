@@ -676,6 +676,8 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
     if (!(t_dev0 > 0.0)) t_dev0 = now_s();   // inputs resident from here on
     double crit = 1.0e10, target = 1.0e-12;
     double L2 = 1.0e20, L2rel = 1.0e20, resid_velo = 1.0;
+    const bool rv_after = o->whichresid == 3 || o->whichresid == 4;
+    bool b_loaded = false;   // c->b holds the load vector with its halo (3D, no prescribed node)
     int counter = 0, lin_total = 0, st = CISM_OK;
     double t_asm = 0, t_lin = 0;
     bool zero_exit = false, stop_hit = false;
@@ -734,8 +736,12 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 if (basal && standard)
                     CISM_TRY(basal_standard_device(c, depth ? depth_basal_beta(c, o->whichapprox) : nullptr));
             }
-            HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
-                                   c->stream));
+            // b = load, as the Dirichlet update leaves it; without prescribed nodes in the 3D
+            // loop nothing changes b after the first Picard iteration's copy and its halo
+            const bool b_fresh = dim2 || c->dirichlet_any || !b_loaded;
+            if (b_fresh)
+                HIP_TRY(hipMemcpyAsync(c->b, c->load, sizeof(double) * 2 * g.npad, hipMemcpyDeviceToDevice,
+                                       c->stream));
             {
                 TimerRange r_d(dim2 ? "glissade_dirichlet_2d" : "glissade_dirichlet_3d", c->stream, true);
                 CISM_TRY(dirichlet_device(c));
@@ -743,7 +749,8 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             {
                 TimerRange r_h("glissade_halo_Axxs", c->stream, true);
                 if (c->a64_valid) CISM_TRY(launch_halo_mat(c, c->A));
-                CISM_TRY(launch_halo_vec(c, c->b));
+                if (b_fresh) CISM_TRY(launch_halo_vec(c, c->b));
+                b_loaded = true;
             }
             TimerRange r_s("glissade_chk_symmetry");   // ends at the synchronization below
             if (fuse32) {
@@ -873,9 +880,13 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
             CISM_TRY(anderson_step(c, o->anderson_m, &fn));
             CISM_TRY(launch_halo_vec(c, c->x));
         }
-        CISM_TRY(residual_velocity_device(c, o->whichresid));
-        CISM_TRY(sync_scalars(c));
-        resid_velo = c->sc_host->sums[0];
+        // the loop's test is on L2 (whichresid 3, 4) or on the velocity residual; with the
+        // former, the velocity residual is a report only and is formed once, after the loop
+        if (!rv_after) {
+            CISM_TRY(residual_velocity_device(c, o->whichresid));
+            CISM_TRY(sync_scalars(c));
+            resid_velo = c->sc_host->sums[0];
+        }
         if (depth && o->whichapprox == 4) CISM_TRY(diva_post_solve_device(c));
         if (o->whichresid == 2) resid_velo = c->sc_host->sums[1] > 0 ? resid_velo / c->sc_host->sums[1] : 0.0;
         if (o->whichresid == 3) { crit = L2; target = 1.0e-4; }
@@ -896,6 +907,12 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
     // maxiter_nonlinear; a solve stopped on its error target, or with nothing to solve,
     // has converged too
     S.converged = counter < o->maxiter_nonlinear || stop_hit || zero_exit;
+    if (rv_after && counter > 0 && !zero_exit) {
+        // the last iteration's x and its start usav are still in place
+        CISM_TRY(residual_velocity_device(c, o->whichresid));
+        CISM_TRY(sync_scalars(c));
+        resid_velo = c->sc_host->sums[0];
+    }
     if (zero_exit) {
         HIP_TRY(hipMemsetAsync(c->x, 0, sizeof(double) * 2 * g.npad, c->stream));
         HIP_TRY(hipMemsetAsync(c->b, 0, sizeof(double) * 2 * g.npad, c->stream));
