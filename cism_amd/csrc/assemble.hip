@@ -600,9 +600,11 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                 acc[blk] = acc[blk] + basal_scale * beta[col];
         }
         if (store64) store_blocks_nt(A, q, m, acc[0], acc[1], acc[2], acc[3]);
-        if (As && m >= CISM_DIAG)
-            *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) =
-                make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+        if (As && m >= CISM_DIAG) {
+            typedef float fvec4_nt __attribute__((ext_vector_type(4)));
+            const fvec4_nt v = {(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<fvec4_nt *>(As + sidx(0, m - CISM_DIAG, q)));
+        }
         if (count) {
             nnz += (double)((acc[0] != 0.0) + (acc[1] != 0.0) + (acc[2] != 0.0) + (acc[3] != 0.0));
             // the row product in k_resvec's order (slots in indxA order, the same operations)
