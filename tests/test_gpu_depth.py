@@ -95,10 +95,12 @@ def test_depth_inputs_are_checked():
             ctx3.velo_higher_solve_depth(p, lu, lv, 3)
 
 
+@pytest.mark.parametrize("coarse", [0, 1])
 @pytest.mark.parametrize("approx", [3, 4])
-def test_depth_mixed_precision_reaches_the_fp64_solution(approx):
+def test_depth_mixed_precision_reaches_the_fp64_solution(approx, coarse):
     """the fast path's options on the 2D solvers: FP32 Chron-Gear in FP64 refinement with
-    inexact Picard (and Anderson for L1L2) to 1e-6 of the plain FP64 2D solution"""
+    inexact Picard (and Anderson for L1L2), with and without the aggregation coarse
+    correction, to 1e-6 of the plain FP64 2D solution"""
     p = configs.grf_sheet(ewn=24, nsn=20, nz=7, seed=4)
     lu, lv = O.loads(p)
     with G.Context.for_problem_2d(p) as ctx:
@@ -107,7 +109,7 @@ def test_depth_mixed_precision_reaches_the_fp64_solution(approx):
         ctx.set_reference_velocity(np.asfortranarray(ref["uvel_2d"][None]), np.asfortranarray(ref["vvel_2d"][None]))
         r = ctx.velo_higher_solve_depth(p, lu, lv, approx, precision=1, adaptive_linear=1, inner_rtol=5e-2,
                                         anderson_m=3 if approx == 3 else 0, stop_rel_err=1e-6,
-                                        maxiter_nonlinear=300)
-    print(f"  approx {approx}: FP64 {ref['picard_iters']} Picard; mixed {r['picard_iters']} Picard, "
+                                        maxiter_nonlinear=300, coarse=coarse)
+    print(f"  approx {approx} coarse {coarse}: FP64 {ref['picard_iters']} Picard; mixed {r['picard_iters']} Picard, "
           f"{r['fp32_matvecs']} FP32 + {r['fp64_matvecs']} FP64 matvecs, err {r['final_rel_err']:.2e}")
     assert r["converged"] and 0 <= r["final_rel_err"] <= 1e-6 and r["fallbacks"] == 0
