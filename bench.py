@@ -182,9 +182,10 @@ def main(argv=None):
         ms32 = ctx.time_kernel(1, a.steps)
         msu = ctx.time_kernel(4, a.steps)
         msc = ctx.time_kernel(3, a.steps)
+        ms_asm = ctx.time_kernel(2, max(1, a.steps // 10))
         print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
                           "fp32_matvec_ms": ms32, "fp32_update_ms": msu, "copy_1GiB_ms": msc,
-                          "launches": a.steps}))
+                          "assembly_ms": ms_asm, "launches": a.steps}))
         ctx.close()
         return 0
 

@@ -23,10 +23,16 @@
 #include <math.h>
 
 // node n of a hexahedron: horizontal offsets (ix, jy) in {0,1} from the SW
-// vertex, layer kl = 0 upper (level k) / 1 lower (level k+1)   (velo_higher:453-486)
-__constant__ int c_node_ix[8] = {0, 1, 1, 0, 0, 1, 1, 0};
-__constant__ int c_node_jy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
-__constant__ int c_node_kl[8] = {1, 1, 1, 1, 0, 0, 0, 0};
+// vertex, layer kl = 0 upper (level k) / 1 lower (level k+1)   (velo_higher:453-486):
+// ix = {0,1,1,0,0,1,1,0}, jy = {0,0,1,1,0,0,1,1}, kl = {1,1,1,1,0,0,0,0}.  Written as
+// functions of n so that in unrolled loops the basis-function signs fold at compile time.
+__host__ __device__ constexpr int c_node_ix(int n) { return ((n + 1) >> 1) & 1; }
+__host__ __device__ constexpr int c_node_jy(int n) { return (n >> 1) & 1; }
+__host__ __device__ constexpr int c_node_kl(int n) { return n < 4 ? 1 : 0; }
+static_assert(c_node_ix(0) == 0 && c_node_ix(1) == 1 && c_node_ix(2) == 1 && c_node_ix(3) == 0 &&
+              c_node_ix(4) == 0 && c_node_ix(5) == 1 && c_node_ix(6) == 1 && c_node_ix(7) == 0, "ix");
+static_assert(c_node_jy(0) == 0 && c_node_jy(1) == 0 && c_node_jy(2) == 1 && c_node_jy(3) == 1 &&
+              c_node_jy(4) == 0 && c_node_jy(5) == 0 && c_node_jy(6) == 1 && c_node_jy(7) == 1, "jy");
 
 __host__ __device__ constexpr int node_id(int ix, int jy, int kl) {
     return kl == 1 ? (jy == 0 ? (ix == 0 ? 0 : 1) : (ix == 0 ? 3 : 2))
@@ -52,18 +58,25 @@ __host__ __device__ inline long long keidx(int ent, long long e) {
     return (((e >> 6) * CISM_KE_ENTRIES + ent) << 6) + (e & 63);
 }
 
-// LDS record of quadrature point p of an element: the basis derivatives x[8],
-// y[8], z[8] of its nodes, then the weight f = efvs det / vol0 and one pad, so
-// each record and each node pair in it is 16-byte aligned.  An element's eight
-// records are padded to 210 doubles: the eight elements a wave reads side by
-// side then start in eight disjoint groups of four banks.
-#define QP_REC 26
+// LDS record of quadrature point p of an element: the basis derivatives x[8], y[8],
+// z[8] of its nodes (24 doubles, so each record and each node pair in it is 16-byte
+// aligned); the weight f = efvs det / vol0 and the point's efvs sit in arrays of their own.
+// An element's eight records are padded to 194 doubles: the eight elements a wave reads
+// side by side then start in eight disjoint groups of four banks.  The nodes' (x, y, z, u,
+// v) share the records' memory (a barrier separates their last read from the first
+// record store), which keeps the workgroup under 26.7 KB: six workgroups (three waves per
+// SIMD) fit a CU.
+#define QP_REC 24
 #define EL_REC (8 * QP_REC + 2)
 struct QPLds {
-    alignas(16) double rec[EL_ELEM][EL_REC];
-    double ev[EL_ELEM][9];
-    double nodes[EL_ELEM][8 * 5 + 1];                                                  // x y z u v
+    union {
+        alignas(16) double rec[EL_ELEM][EL_REC];
+        double nodes[EL_ELEM][8 * 5 + 1];                                              // x y z u v
+    };
+    double f[EL_ELEM][8];
+    double ev[EL_ELEM][8];
 };
+static_assert(sizeof(QPLds) + EL_ELEM * sizeof(int) <= 163840 / 6, "six element workgroups per CU");
 
 // One quadrature point p of a hexahedral element whose 8 nodes are given as
 // (x, y, z, u, v) records: basis-function derivatives (get_basis_function_
@@ -81,8 +94,8 @@ __device__ __forceinline__ void eval_qp(int p, const double *__restrict__ nodes,
     double a[8], b[8], cz[8];
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-        const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
-        const double sz = c_node_kl[n] ? -1.0 : 1.0;
+        const double sx = c_node_ix(n) ? 1.0 : -1.0, sy = c_node_jy(n) ? 1.0 : -1.0;
+        const double sz = c_node_kl(n) ? -1.0 : 1.0;
         const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
         const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
         const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;
@@ -198,7 +211,7 @@ __global__ __launch_bounds__(8 * EL_ELEM) void k_element(GridDev g, const double
     // phase 0: lane (el, n) gathers node n (velo_higher:5100-5127)
     if (ok) {
         const int n = sub;
-        const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
+        const int in = ic - 1 + c_node_ix(n), jn = jc - 1 + c_node_jy(n), kn = k + c_node_kl(n);
         const int v = (in - 1) + g.sx * (jn - 1);
         const int qn = (kn - 1) + g.nz * v;
         L.nodes[el][n * 5 + 0] = xV[v];
@@ -209,16 +222,29 @@ __global__ __launch_bounds__(8 * EL_ELEM) void k_element(GridDev g, const double
     }
     __syncthreads();
     // phase 1: lane (el, p) evaluates quadrature point p (velo_higher:6360-6579, 7199-7378)
+    double dx[8], dy[8], dz[8], evp = 0.0, fw = 0.0;
+    if (ok) {
+        const int p = sub;
+        double det, ev, st[6];
+        bool singular;
+        eval_qp(p, L.nodes[el], APPROX, whichefvs, efvs_constant, flwafact[e], dx, dy, dz, det, ev, st,
+                singular);
+        if (singular) atomicOr(&sc->singular_flag, 1);
+        evp = ev;
+        fw = ev * 1.0 * det / CISM_VOL0;
+    }
+    __syncthreads();   // every lane has read the nodes: the records may overwrite them
     if (ok) {
         const int p = sub;
         double *r = &L.rec[el][p * QP_REC];
-        double det, ev, st[6];
-        bool singular;
-        eval_qp(p, L.nodes[el], APPROX, whichefvs, efvs_constant, flwafact[e], r, r + 8, r + 16, det, ev,
-                st, singular);
-        if (singular) atomicOr(&sc->singular_flag, 1);
-        L.ev[el][p] = ev;
-        r[24] = ev * 1.0 * det / CISM_VOL0;
+#pragma unroll
+        for (int n = 0; n < 8; n += 2) {
+            *reinterpret_cast<double2 *>(r + n) = make_double2(dx[n], dx[n + 1]);
+            *reinterpret_cast<double2 *>(r + 8 + n) = make_double2(dy[n], dy[n + 1]);
+            *reinterpret_cast<double2 *>(r + 16 + n) = make_double2(dz[n], dz[n + 1]);
+        }
+        L.f[el][p] = fw;
+        L.ev[el][p] = evp;
     }
     __syncthreads();
     // element efvs = mean over quadrature points (velo_higher:5167-5172)
@@ -241,7 +267,7 @@ __global__ __launch_bounds__(8 * EL_ELEM) void k_element(GridDev g, const double
 #pragma unroll 1
     for (int p = 0; p < 8; ++p) {
         const double *r = &L.rec[el][p * QP_REC];
-        const double f = r[24];
+        const double f = L.f[el][p];
         const double fx = f * r[a], fy = f * r[8 + a], fz = f * r[16 + a];
 #pragma unroll
         for (int b = 0; b < 8; b += 2) {
@@ -314,7 +340,7 @@ __global__ __launch_bounds__(256) void k_stress(GridDev g, const double *__restr
     }
     if (ok) {
         const int n = sub;
-        const int in = ic - 1 + c_node_ix[n], jn = jc - 1 + c_node_jy[n], kn = k + c_node_kl[n];
+        const int in = ic - 1 + c_node_ix(n), jn = jc - 1 + c_node_jy(n), kn = k + c_node_kl(n);
         const long long v = (in - 1) + (long long)g.sx * (jn - 1);
         const long long qn = (kn - 1) + (long long)g.nz * v;
         nodes[el][n * 5 + 0] = xV[v];
@@ -1240,7 +1266,7 @@ __global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *
                 const int me = node_id(1 - dI, 1 - dJ, 1 - dK);
                 double X[8], Y[8], Z[8], SX[8], SY[8];
                 for (int n = 0; n < 8; ++n) {
-                    const int in = icl - 1 + c_node_ix[n], jn = jcl - 1 + c_node_jy[n], kn = ke + c_node_kl[n];
+                    const int in = icl - 1 + c_node_ix(n), jn = jcl - 1 + c_node_jy(n), kn = ke + c_node_kl(n);
                     const long long v = (in - 1) + (long long)g.sx * (jn - 1);
                     X[n] = xV[v]; Y[n] = yV[v];
                     Z[n] = susrf[v] - sigma[kn - 1] * sthck[v];
@@ -1252,8 +1278,8 @@ __global__ __launch_bounds__(256) void k_load_standard(GridDev g, const double *
                     const double zq = p < 4 ? -r : r;
                     double a[8], b[8], cz[8], ph[8];
                     for (int n = 0; n < 8; ++n) {
-                        const double sx = c_node_ix[n] ? 1.0 : -1.0, sy = c_node_jy[n] ? 1.0 : -1.0;
-                        const double sz = c_node_kl[n] ? -1.0 : 1.0;
+                        const double sx = c_node_ix(n) ? 1.0 : -1.0, sy = c_node_jy(n) ? 1.0 : -1.0;
+                        const double sz = c_node_kl(n) ? -1.0 : 1.0;
                         const double ax = sx > 0 ? 1.0 + xq : 1.0 - xq;
                         const double ay = sy > 0 ? 1.0 + yq : 1.0 - yq;
                         const double az = sz > 0 ? 1.0 + zq : 1.0 - zq;

@@ -5,7 +5,7 @@
 // backward kernels and Adam.
 //
 // Precision: FP32 master weights and activations; every convolution runs as an
-// implicit GEMM on MFMA (v_mfma_f32_16x16x16_bf16) with BF16 operands and FP32
+// implicit GEMM on gfx950's MFMA (v_mfma_f32_16x16x32_bf16) with BF16 operands and FP32
 // accumulation.  Activations are NHWC (channels fastest); 3x3 convolutions use
 // periodic (circular) padding, matching the periodic grids of the solver.
 //
@@ -27,7 +27,7 @@
 
 #include <vector>
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline short f2bf(float f) {
@@ -65,17 +65,20 @@ __device__ inline long long shifted_pixel(long long p, int tap, int ks, int H, i
     return b * HW + (long long)y * W + x;
 }
 
-// K values staged in LDS per step of the forward convolution: 4 MFMA k-steps between
-// two barriers; a thread stages 16 consecutive k of one weight row and of one pixel
-// (Ci % 16 == 0, so the 16 lie in one tap and their 3x3 shift is computed once)
+// K values staged in LDS per step of the forward convolution: 2 MFMA k-steps of 32
+// between two barriers; a thread stages 16 consecutive k of one weight row and of one
+// pixel (Ci % 16 == 0, so the 16 lie in one tap and their 3x3 shift is computed once).
+// A K that is not a multiple of 32 (3x3 over 16 channels: 144) ends in staged zeros.
+// Lane l of an MFMA holds row (l & 15) of A and column (l & 15) of B with 8 consecutive
+// k from 8 (l >> 4); its 4 results are rows 4 (l >> 4) + i of column (l & 15).
 #define CONV_CK 64
 
 __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__restrict__ in,
                                                    const short *__restrict__ Wb,
                                                    const float *__restrict__ bias,
                                                    float *__restrict__ out) {
-    __shared__ short As[64][CONV_CK + 8];   // weights [co][k]
-    __shared__ short Bs[64][CONV_CK + 8];   // inputs [pixel][k]
+    alignas(16) __shared__ short As[64][CONV_CK + 8];   // weights [co][k]
+    alignas(16) __shared__ short Bs[64][CONV_CK + 8];   // inputs [pixel][k]
     const int K = d.ks * d.ks * d.Ci;
     const int HW = d.H * d.W;
     const long long P = (long long)d.B * HW;
@@ -133,20 +136,20 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__re
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < CONV_CK; kk += 16) {
+        for (int kk = 0; kk < CONV_CK; kk += 32) {
             if (k0 + kk >= K) break;   // the same for the whole block
-            bf16x4 af[2], bfm[2];
+            bf16x8 af[2], bfm[2];
             for (int a = 0; a < 2; ++a) {
                 const int row = wc + a * 16 + (lane & 15);
-                af[a] = *reinterpret_cast<const bf16x4 *>(&As[row][kk + 4 * (lane >> 4)]);
+                af[a] = *reinterpret_cast<const bf16x8 *>(&As[row][kk + 8 * (lane >> 4)]);
             }
             for (int b = 0; b < 2; ++b) {
                 const int col = wp + b * 16 + (lane & 15);
-                bfm[b] = *reinterpret_cast<const bf16x4 *>(&Bs[col][kk + 4 * (lane >> 4)]);
+                bfm[b] = *reinterpret_cast<const bf16x8 *>(&Bs[col][kk + 8 * (lane >> 4)]);
             }
             for (int a = 0; a < 2; ++a)
                 for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -168,8 +171,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__re
 __global__ __launch_bounds__(256) void k_wgrad_mfma(ConvDims d, const float *__restrict__ dY,
                                                     const float *__restrict__ X,
                                                     float *__restrict__ part, long long pchunk) {
-    __shared__ short As[64][16 + 4];   // [co][pix]
-    __shared__ short Bs[64][16 + 4];   // [k][pix]
+    alignas(16) __shared__ short As[64][32 + 8];   // [co][pix]
+    alignas(16) __shared__ short Bs[64][32 + 8];   // [k][pix]
     const int K = d.ks * d.ks * d.Ci;
     const long long P = (long long)d.B * d.H * d.W;
     const int co0 = blockIdx.y * 64, kq0 = blockIdx.x * 64;
@@ -180,44 +183,49 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(ConvDims d, const float *__r
     f32x4 acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    const int lp = t >> 4, lc = (t & 15) * 4;     // loader: pixel lp (0..15), 4 values from lc
-    for (long long q0 = pbeg; q0 < pend; q0 += 16) {
-        const long long p = q0 + lp;
-        const bool okp = p < pend;
-        {   // A: dY[p][co0+lc..+3] -> As[co][pix]
-            float4 v = {0.f, 0.f, 0.f, 0.f};
-            if (okp && co0 + lc < d.Co) v = *reinterpret_cast<const float4 *>(dY + p * d.Co + co0 + lc);
-            As[lc + 0][lp] = f2bf(v.x);
-            As[lc + 1][lp] = f2bf(v.y);
-            As[lc + 2][lp] = f2bf(v.z);
-            As[lc + 3][lp] = f2bf(v.w);
-        }
-        {   // B: X(p, kq0+lc..+3) -> Bs[k][pix]
-            float4 v = {0.f, 0.f, 0.f, 0.f};
-            const int k = kq0 + lc;
-            if (okp && k < K) {
-                const int tap = k / d.Ci, ci = k % d.Ci;
-                const long long src = shifted_pixel(p, tap, d.ks, d.H, d.W);
-                v = *reinterpret_cast<const float4 *>(X + src * d.Ci + ci);
+    const int lp = t >> 4, lc = (t & 15) * 4;     // loader: pixels lp and lp + 16, 4 values from lc
+    // the B loader's tap and channel are fixed for the block
+    const int kl = kq0 + lc;
+    const int ltap = kl < K ? kl / d.Ci : 0, lci = kl < K ? kl - ltap * d.Ci : 0;
+    for (long long q0 = pbeg; q0 < pend; q0 += 32) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int pl = lp + 16 * h;
+            const long long p = q0 + pl;
+            const bool okp = p < pend;
+            {   // A: dY[p][co0+lc..+3] -> As[co][pix]
+                float4 v = {0.f, 0.f, 0.f, 0.f};
+                if (okp && co0 + lc < d.Co) v = *reinterpret_cast<const float4 *>(dY + p * d.Co + co0 + lc);
+                As[lc + 0][pl] = f2bf(v.x);
+                As[lc + 1][pl] = f2bf(v.y);
+                As[lc + 2][pl] = f2bf(v.z);
+                As[lc + 3][pl] = f2bf(v.w);
             }
-            Bs[lc + 0][lp] = f2bf(v.x);
-            Bs[lc + 1][lp] = f2bf(v.y);
-            Bs[lc + 2][lp] = f2bf(v.z);
-            Bs[lc + 3][lp] = f2bf(v.w);
+            {   // B: X(p, kq0+lc..+3) -> Bs[k][pix]
+                float4 v = {0.f, 0.f, 0.f, 0.f};
+                if (okp && kl < K) {
+                    const long long src = shifted_pixel(p, ltap, d.ks, d.H, d.W);
+                    v = *reinterpret_cast<const float4 *>(X + src * d.Ci + lci);
+                }
+                Bs[lc + 0][pl] = f2bf(v.x);
+                Bs[lc + 1][pl] = f2bf(v.y);
+                Bs[lc + 2][pl] = f2bf(v.z);
+                Bs[lc + 3][pl] = f2bf(v.w);
+            }
         }
         __syncthreads();
-        bf16x4 af[2], bfm[2];
+        bf16x8 af[2], bfm[2];
         for (int a = 0; a < 2; ++a) {
             const int row = wc + a * 16 + (lane & 15);
-            for (int j = 0; j < 4; ++j) af[a][j] = As[row][4 * (lane >> 4) + j];
+            af[a] = *reinterpret_cast<const bf16x8 *>(&As[row][8 * (lane >> 4)]);
         }
         for (int b = 0; b < 2; ++b) {
             const int col = wk + b * 16 + (lane & 15);
-            for (int j = 0; j < 4; ++j) bfm[b][j] = Bs[col][4 * (lane >> 4) + j];
+            bfm[b] = *reinterpret_cast<const bf16x8 *>(&Bs[col][8 * (lane >> 4)]);
         }
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfm[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfm[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
     float *pz = part + (long long)blockIdx.z * d.Co * K;
@@ -448,6 +456,7 @@ struct cism_surrogate {
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     float last_forward_ms;   // device time of the last forward pass (kernels only)
+    int wprep_ok;            // the BF16 copies Wb, Wt match w (cleared whenever w changes)
 };
 
 // activation indices
@@ -568,6 +577,7 @@ extern "C" int cism_surrogate_create(int H, int W, int batch, int device, cism_s
     HIP_TRY(hipMemcpyAsync(s->w, h.data(), sizeof(float) * s->nparams, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->step = 0;
+    s->wprep_ok = 0;
     *out = s;
     return CISM_OK;
 }
@@ -608,6 +618,7 @@ extern "C" int cism_surrogate_get_params(cism_surrogate *s, float *host) {
 }
 
 extern "C" int cism_surrogate_set_params(cism_surrogate *s, const float *host) {
+    s->wprep_ok = 0;
     HIP_TRY(hipMemcpyAsync(s->w, host, sizeof(float) * s->nparams, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemsetAsync(s->m, 0, sizeof(float) * s->nparams, s->stream));
     HIP_TRY(hipMemsetAsync(s->v, 0, sizeof(float) * s->nparams, s->stream));
@@ -616,13 +627,17 @@ extern "C" int cism_surrogate_set_params(cism_surrogate *s, const float *host) {
     return CISM_OK;
 }
 
+// BF16 copies of the weights, rebuilt only after the weights changed (creation,
+// cism_surrogate_set_params, a training step)
 static int prep_weights(cism_surrogate *s) {
+    if (s->wprep_ok) return CISM_OK;
     for (auto &c : s->L) {
         const long long n = (long long)c.Co * c.ks * c.ks * c.Ci;
         hipLaunchKernelGGL(k_prep_weights, dim3(nblocks_for(n)), dim3(256), 0, s->stream, c.Co, c.Ci, c.ks,
                            s->w + c.woff, c.Wb, c.Wt);
     }
     HIP_TRY(hipGetLastError());
+    s->wprep_ok = 1;
     return CISM_OK;
 }
 
@@ -851,6 +866,7 @@ extern "C" int cism_surrogate_train_step(cism_surrogate *s, const float *in, con
     const float c1 = 1.f - powf(b1, (float)s->step), c2 = 1.f - powf(b2, (float)s->step);
     hipLaunchKernelGGL(k_adam, dim3(nblocks_for(s->nparams)), dim3(256), 0, s->stream, s->nparams, s->w, s->g,
                        s->m, s->v, lr, b1, b2, 1e-8f, c1, c2);
+    s->wprep_ok = 0;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     return CISM_OK;
