@@ -183,9 +183,13 @@ def main(argv=None):
         msu = ctx.time_kernel(4, a.steps)
         msc = ctx.time_kernel(3, a.steps)
         ms_asm = ctx.time_kernel(2, max(1, a.steps // 10))
+        ms64f = ctx.time_kernel(5, a.steps)
+        ms64 = ctx.time_kernel(0, a.steps)
+        ms64o = ctx.time_kernel(6, a.steps)
         print(json.dumps({"workload": a.workload, "owned_active_nodes": nodes,
                           "fp32_matvec_ms": ms32, "fp32_update_ms": msu, "copy_1GiB_ms": msc,
-                          "assembly_ms": ms_asm, "launches": a.steps}))
+                          "assembly_ms": ms_asm, "fp64_matvec_ms": ms64, "fp64_matvec_full_ms": ms64f,
+                          "fp64_matvec_sym_other_order_ms": ms64o, "launches": a.steps}))
         ctx.close()
         return 0
 
@@ -271,7 +275,8 @@ def main(argv=None):
     solve(1 if use_guess else 0)   # leaves the FP32 matrix of a fast solve
     ms32 = ctx.time_kernel(1, 50)
     ms_asm = ctx.time_kernel(2, 5)   # (also leaves an FP64 matrix: the fast solve stores none)
-    ms64 = ctx.time_kernel(0, 50)
+    ms64f = ctx.time_kernel(5, 50)   # FP64 matvec on the full storage
+    ms64 = ctx.time_kernel(0, 50)    # as the FP64 solver runs it: on the symmetric copy
     # algorithmic bytes of one matvec: 108 coefficients + 2 x + 2 y per owned node (448 B
     # in FP32); the FP32 matrix is kept in symmetric storage (14 of 27 slots, 248 B per
     # node with x, y, r), which lowers the bytes moved but not this count
@@ -339,6 +344,8 @@ def main(argv=None):
                            "seconds": round(ref["seconds"], 1)},
         "matvec": {"fp32_ms": round(ms32, 4), "fp32_GBps": round(gbs32, 1),
                    "fp64_ms": round(ms64, 4), "fp64_GBps": round(gbs64, 1),
+                   "fp64_full_storage_ms": round(ms64f, 4),
+                   "fp64_full_storage_GBps": round(bytes64 / (ms64f * 1e-3) / 1e9, 1),
                    "bytes_per_node_fp32": 448, "bytes_per_node_fp64": 896,
                    "fp32_stored_bytes_per_node": SYM_SLOTS * 16 + 24,
                    "fp32_stored_GBps": round(nodes * (SYM_SLOTS * 16 + 24) / (ms32 * 1e-3) / 1e9, 1)},

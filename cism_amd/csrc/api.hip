@@ -83,6 +83,7 @@ int ensure_ke(cism_ctx *c) {
 static int upload_block(cism_ctx *c, int b, const double *h) {
     c->a_zero_ok = 0;   // the caller's matrix may hold anything in inactive rows
     c->a64_valid = 1;
+    c->a64s_valid = 0;
     HIP_TRY(hipMemcpyAsync(c->stage, h, sizeof(double) * 27 * c->g.nn, hipMemcpyHostToDevice,
                            c->stream));
     hipLaunchKernelGGL(k_fortran_to_tiled, dim3((unsigned)((c->g.nn + 63) / 64)), dim3(256), 0,
@@ -256,6 +257,7 @@ extern "C" int cism_ctx_create(int nx, int ny, int nz, int nhalo, int periodic, 
     if (hipHostMalloc(&c->sc_host, sizeof(DevScal), hipHostMallocDefault) != hipSuccess)
         return fail(CISM_ERR_HIP);
     c->a64_valid = 1;
+    c->sym64_on = 1;
     *out = c;
     return CISM_OK;
 }
@@ -266,7 +268,7 @@ extern "C" int cism_ctx_destroy(cism_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     learned_guess_free(c);
     coarse_free(c);
-    void *ptrs[] = {c->A, c->A32, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
+    void *ptrs[] = {c->A, c->A32, c->A64s, c->b, c->x, c->r, c->d, c->z, c->q, c->Az, c->t, c->load, c->usav, c->vstage,
                     c->x32, c->b32, c->r32, c->d32, c->z32, c->q32, c->Az32, c->F64, c->F32, c->av, c->ac, c->umask, c->vmask,
                     c->xV, c->yV, c->susrf, c->sthck, c->beta, c->sigma, c->wsig, c->flwafact, c->efvs,
                     c->dsdx, c->dsdy, c->Ke, c->av_zero, c->ac_zero, c->as_av_prev, c->as_ac_prev, c->xref, c->partials, c->red_groups, c->sc, c->stage,
@@ -438,6 +440,7 @@ extern "C" int cism_basal_sliding_bc_3d(cism_ctx *c, double dx, double dy,
     CISM_TRY(upload_block(c, 0, Auu));
     HIP_TRY(hipStreamSynchronize(c->stream));
     CISM_TRY(upload_block(c, 3, Avv));
+    c->a64s_valid = 0;
     hipLaunchKernelGGL(k_basal_local, dim3(nblocks_for((long long)ncol)), dim3(256), 0, c->stream,
                        c->g, c->av, c->beta, dx * dy / CISM_VOL0, c->A);
     HIP_TRY(hipGetLastError());
@@ -757,11 +760,14 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                 CISM_TRY(launch_a32_halo(c));
                 CISM_TRY(launch_halo_vec(c, c->r));
             } else {
-                // the mixed-precision path gets its FP32 symmetric matrix from the same pass
+                // the mixed-precision path gets its FP32 symmetric matrix from the same pass,
+                // the FP64 one its FP64 symmetric copy (used when no pair stays unrepaired)
                 if (prec == 1) CISM_TRY(ensure_a32(c));
-                CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr));
+                double *as64 = prec == 0 && c->sym64_on && ensure_a64s(c) == CISM_OK ? c->A64s : nullptr;
+                CISM_TRY(symmetry_device(c, prec == 1 ? c->A32 : nullptr, as64));
             }
             CISM_TRY(sync_scalars(c));
+            sym64_commit(c);
         }
         if (c->sc_host->singular_flag) {
             cism_set_error("Jacobian matrix is singular");
@@ -858,7 +864,10 @@ static int picard_device(cism_ctx *c, const cism_solve_opts *o, int dim2, double
                         CISM_TRY(assemble_device(c, o->whichapprox, o->whichefvs, o->efvs_constant,
                                                  c->asm_add_basal, o->dx, o->dy));
                         CISM_TRY(launch_halo_mat(c, c->A));
-                        CISM_TRY(symmetry_device(c, nullptr));
+                        CISM_TRY(symmetry_device(
+                            c, nullptr, c->sym64_on && ensure_a64s(c) == CISM_OK ? c->A64s : nullptr));
+                        CISM_TRY(sync_scalars(c));
+                        sym64_commit(c);
                     }
                     int n2 = 0;
                     st = solver_pcg64(c, o->whichsparse == 3, o->whichprecond, c->maxiters, lin_tol, &err, &n2);
@@ -1291,6 +1300,17 @@ __global__ __launch_bounds__(256) void k_scale_to_f32(long long n, const double 
     out[t] = (float)v;
 }
 
+extern "C" int cism_set_fp64_symmetric(cism_ctx *c, int enable) {
+    CTX_BEGIN(c);
+    if (enable != 0 && enable != 1) {
+        cism_set_error("cism_set_fp64_symmetric: enable must be 0 or 1");
+        return CISM_ERR_ARG;
+    }
+    c->sym64_on = enable;
+    if (!enable) c->a64s_valid = 0;
+    return CISM_OK;
+}
+
 extern "C" int cism_set_precond_scale(cism_ctx *c, const double *scale_u, const double *scale_v) {
     CTX_BEGIN(c);
     if (!scale_u || !scale_v) {
@@ -1514,14 +1534,27 @@ extern "C" int cism_coarse_correction(cism_ctx *c, const double *ru, const doubl
 
 extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per_launch) {
     CTX_BEGIN(c);
-    if (reps < 1 || which < 0 || which > 4) return CISM_ERR_ARG;
+    if (reps < 1 || which < 0 || which > 6) return CISM_ERR_ARG;
     if ((which == 1 || which == 4) && !c->A32) {
         cism_set_error("no FP32 matrix yet: run a mixed-precision solve first");
         return CISM_ERR_ARG;
     }
-    if (which == 0 && !c->a64_valid) {
+    const bool fp64_matvec = which == 0 || which == 5 || which == 6;
+    if (fp64_matvec && !c->a64_valid) {
         cism_set_error("no FP64 matrix: the last mixed-precision solve (skip_final_residual) kept none");
         return CISM_ERR_ARG;
+    }
+    if ((which == 0 || which == 6) && !c->a64s_valid) {
+        // the FP64 solver's symmetric copy, as its Picard iteration forms it: a symmetry pass
+        if (ensure_a64s(c) == CISM_OK) {
+            CISM_TRY(symmetry_device(c, nullptr, c->A64s));
+            CISM_TRY(sync_scalars(c));
+            sym64_commit(c);
+        }
+        if (which == 6 && !c->a64s_valid) {
+            cism_set_error("the FP64 matrix has no exactly symmetric copy (allocation or unrepaired pairs)");
+            return CISM_ERR_ARG;
+        }
     }
     // which 3: 1 GiB read + 1 GiB write copy, buffers owned by this call
     const long long ncopy = 1LL << 27;

@@ -764,6 +764,7 @@ __global__ __launch_bounds__(256) void k_basal_standard(GridDev g, const int32_t
 int basal_standard_device(cism_ctx *c, const double *beta) {
     const GridDev &g = c->g;
     const long long n = 9LL * g.sx * g.sy;
+    c->a64s_valid = 0;
     hipLaunchKernelGGL(k_basal_standard, dim3(nblocks_for(n)), dim3(256), 0, c->stream, g, c->ac, c->xV,
                        c->yV, beta ? beta : c->beta, c->A);
     HIP_TRY(hipGetLastError());
@@ -773,6 +774,7 @@ int basal_standard_device(cism_ctx *c, const double *beta) {
 int assemble_device(cism_ctx *c, int whichapprox, int whichefvs, double efvs_constant,
                     int add_basal, double dx, double dy, float *As, int store64) {
     CISM_TRY(ensure_ke(c));
+    c->a64s_valid = 0;
     const GridDev &g = c->g;
     const int nb = (int)((g.ne + EL_ELEM - 1) / EL_ELEM);
     auto kern = whichapprox == 0 ? k_element<0> : whichapprox == 1 ? k_element<1> : k_element<2>;
@@ -894,6 +896,7 @@ __global__ __launch_bounds__(256) void k_dirichlet(GridDev g, const int32_t *__r
 
 int dirichlet_device(cism_ctx *c) {
     if (!c->dirichlet_any) return CISM_OK;   // no prescribed node: nothing to replace or eliminate
+    c->a64s_valid = 0;
     hipLaunchKernelGGL(k_dirichlet, dim3(nblocks_for(c->g.nn)), dim3(256), 0, c->stream, c->g, c->av,
                        c->umask, c->vmask, c->x, c->A, c->b);
     HIP_TRY(hipGetLastError());
@@ -931,6 +934,7 @@ int dirichlet_device(cism_ctx *c) {
 // couplings in the neighbours' rows, and its rows of As are zero from that pass.
 __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__restrict__ av,
                                                     double *__restrict__ A, float *__restrict__ As,
+                                                    double *__restrict__ As64,
                                                     const int32_t *__restrict__ ac,
                                                     const int32_t *__restrict__ ac_prev,
                                                     const int32_t *__restrict__ av_prev,
@@ -955,13 +959,22 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
         auto count4 = [&](long long e0, long long e2) {   // four blocks: (b0, b1) at e0, (b2, b3) at e2
             return (double)((A[e0] != 0.0) + (A[e0 + 1] != 0.0) + (A[e2] != 0.0) + (A[e2 + 1] != 0.0));
         };
+        // slot m of row q is final once its pairs are repaired (only this lane writes a
+        // row's slots from the diagonal on): its FP32 and FP64 symmetric copies
         auto store32 = [&](int m) {
-            if (!As) return;
+            if (!As && !As64) return;
             const double2 a = *reinterpret_cast<const double2 *>(A + aidx(0, m, q));
             const double2 b = *reinterpret_cast<const double2 *>(A + aidx(2, m, q));
-            float4 o;
-            o.x = (float)a.x; o.y = (float)a.y; o.z = (float)b.x; o.w = (float)b.y;
-            *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) = o;
+            if (As) {
+                float4 o;
+                o.x = (float)a.x; o.y = (float)a.y; o.z = (float)b.x; o.w = (float)b.y;
+                *reinterpret_cast<float4 *>(As + sidx(0, m - CISM_DIAG, q)) = o;
+            }
+            if (As64) {
+                const dvec2_nt a2 = {a.x, a.y}, b2 = {b.x, b.y};
+                __builtin_nontemporal_store(a2, reinterpret_cast<dvec2_nt *>(As64 + s64idx(0, m - CISM_DIAG, q)));
+                __builtin_nontemporal_store(b2, reinterpret_cast<dvec2_nt *>(As64 + s64idx(2, m - CISM_DIAG, q)));
+            }
         };
         // one coefficient pair (e1, e2) with values v1, v2; qualification from each side
         auto pair = [&](long long e1, long long e2, double v1, double v2, bool va, double da, bool vb,
@@ -1027,14 +1040,17 @@ __global__ __launch_bounds__(256) void k_sym_repair(GridDev g, const int32_t *__
 }
 
 // symmetry repair; sums[0] = warnings, sums[1] = nonzeros of owned active rows.  As: the
-// FP32 symmetric storage to write as well, or null.
-int symmetry_device(cism_ctx *c, float *As) {
+// FP32 symmetric storage to write as well, or null; As64 (c->A64s): the FP64 one, valid
+// after sym64_commit when the pass counted no warning.
+int symmetry_device(cism_ctx *c, float *As, double *As64) {
     const GridDev &g = c->g;
     const int nb = nblocks_for(g.nn);
     // rows outside the ice may be left out only when A is the assembly's (a_zero_ok)
-    // and As was last written from such an A
-    const bool skip = As && c->a_zero_ok && c->as_ok;
-    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, g, c->av, c->A, As,
+    // and As was last written from such an A (never with As64, which covers every row)
+    const bool skip = As && !As64 && c->a_zero_ok && c->as_ok;
+    c->a64s_valid = 0;
+    c->a64s_pending = As64 != nullptr;
+    hipLaunchKernelGGL(k_sym_repair, dim3(nb), dim3(256), 0, c->stream, g, c->av, c->A, As, As64,
                        skip ? c->ac : nullptr, c->as_ac_prev, c->as_av_prev, c->partials);
     HIP_TRY(hipGetLastError());
     if (As) {
@@ -1049,6 +1065,11 @@ int symmetry_device(cism_ctx *c, float *As) {
     }
     CISM_TRY(launch_reduce(c, nb, 2, RED_SUMS));
     return CISM_OK;
+}
+
+void sym64_commit(cism_ctx *c) {
+    c->a64s_valid = c->a64s_pending && c->sc_host->sums[0] == 0.0;
+    c->a64s_pending = 0;
 }
 
 // r = b - A x (c->r; halo rows from their images) of the last assembly's matrix, formed

@@ -90,6 +90,13 @@ struct cism_ctx {
     // matrix, FP64 and (lazily) FP32
     double *A;
     float *A32;
+    // (lazily) the FP64 matrix in symmetric storage, written by a symmetry pass
+    // (a64s_pending) and valid (a64s_valid) once that pass left no pair unrepaired: every
+    // coefficient pair of an owned active row then holds one value, and products with
+    // A64s equal those with A bit for bit.  Every writer of A clears a64s_valid.
+    double *A64s;
+    int a64s_valid, a64s_pending;
+    int sym64_on;   // cism_set_fp64_symmetric (default 1)
     // vectors (2*npad each): b x r d z q Az t(temp) load usav
     double *b, *x, *r, *d, *z, *q, *Az, *t, *load, *usav;
     double *vstage;   // 2*npad staging for (de)interleaving host vectors
@@ -210,6 +217,12 @@ __host__ __device__ inline long long aidx(int b, int m, long long q) {
 __host__ __device__ inline long long sidx(int b, int s, long long q) {
     return (((q >> 6) * CISM_SYM_SLOTS + s) << 8) + ((q & 63) << 2) + b;
 }
+// FP64 symmetric storage (A64s: the same 14 slots, for the FP64 solver once the symmetry
+// pass has left the matrix exactly symmetric) in aidx's tiled form: blocks b and b ^ 1
+// adjacent, so each 16-byte load of a wave reads 1 KB contiguous
+__host__ __device__ inline long long s64idx(int b, int s, long long q) {
+    return (((q >> 6) * CISM_SYM_SLOTS + s) << 8) + ((long long)(b >> 1) << 7) + ((q & 63) << 1) + (b & 1);
+}
 
 __device__ inline void load_blocks(const float *__restrict__ A, long long q, int m, float &uu,
                                    float &uv, float &vu, float &vv) {
@@ -274,8 +287,10 @@ __device__ inline bool touches_active_cell(const GridDev &g, const int32_t *__re
 // ---------------------------------------------------------------------------
 // deterministic block reduction (fixed lane/wave order)
 // ---------------------------------------------------------------------------
+// the block's sums to partials[blk * NV + c] (blk: the block's index, or a node-block
+// index of the kernel's own when it maps blocks to nodes in another order)
 template <int NV>
-__device__ inline void block_reduce_store(double (&v)[NV], double *partials) {
+__device__ inline void block_reduce_store(double (&v)[NV], double *partials, long long blk = -1) {
     static_assert(NV <= CISM_MAX_NV, "partials hold CISM_MAX_NV values per block");
     __shared__ double red[CISM_BLOCK / 64][NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -290,7 +305,7 @@ __device__ inline void block_reduce_store(double (&v)[NV], double *partials) {
     if (threadIdx.x < NV) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w][threadIdx.x];
-        partials[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+        partials[(size_t)(blk < 0 ? (long long)blockIdx.x : blk) * NV + threadIdx.x] = s;
     }
 }
 
@@ -409,7 +424,7 @@ int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric
 int anderson_reset(cism_ctx *c);
 int anderson_step(cism_ctx *c, int m, double *fnorm);
 int launch_halo_mat(cism_ctx *c, double *A);      // 108 planes
-int launch_cg_matvec(cism_ctx *c, int fp32);
+int launch_cg_matvec(cism_ctx *c, int which);
 int launch_cg_update32(cism_ctx *c);   // one FP32 fused update (timing)
 int rel_err_device(cism_ctx *c, double *rel);
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol,
@@ -439,7 +454,11 @@ int dirichlet_device(cism_ctx *c);
 // basal friction by the standard finite-element assembly onto the assembled c->A (c->ac,
 // xV, yV, beta); the local assembly is part of the gather (assemble_device's add_basal)
 int basal_standard_device(cism_ctx *c, const double *beta = nullptr);   // null: c->beta
-int symmetry_device(cism_ctx *c, float *As);
+int symmetry_device(cism_ctx *c, float *As, double *As64 = nullptr);
+// after the synchronization that follows symmetry_device: A64s is valid when that pass
+// wrote it and counted no unrepaired pair (sums[0] == 0)
+void sym64_commit(cism_ctx *c);
+int ensure_a64s(cism_ctx *c);   // CISM_ERR_NOMEM leaves the FP64 solver on the full storage
 int count_nonzeros_device(cism_ctx *c);          // result in sc->sums[0]
 // r = A x - b on owned active rows: sums[0] = |r|^2, sums[1] = |b|^2 over them.  With
 // rs: also rs = b - A x, halo rows from their images; sums[2] = |rs|^2, sums[3] = |b|^2

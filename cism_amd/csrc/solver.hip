@@ -200,6 +200,7 @@ template <> struct Vec2<double> { using t = double2; };
 template <typename T>
 struct MatFull {   // all 27 slots stored (aidx)
     static constexpr bool xcd = false;
+    static constexpr bool node_partials = false;
     const T *A;
     __device__ inline void blocks(long long q, int m, long long qn, T &uu, T &uv, T &vu, T &vv) const {
         (void)qn;
@@ -214,6 +215,7 @@ struct MatFull {   // all 27 slots stored (aidx)
 
 struct MatSym32 {   // FP32 symmetric storage (sidx); matvec launched XCD-aware
     static constexpr bool xcd = true;
+    static constexpr bool node_partials = false;   // partials in launch order
     const float *A;
     __device__ inline void blocks(long long q, int m, long long qn, float &uu, float &uv, float &vu,
                                   float &vv) const {
@@ -231,6 +233,37 @@ struct MatSym32 {   // FP32 symmetric storage (sidx); matvec launched XCD-aware
         const long long qn = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
         const int bt = b == 1 ? 2 : (b == 2 ? 1 : b);
         return A[sidx(bt, CISM_DIAG - m, qn)];
+    }
+};
+
+// FP64 symmetric storage (s64idx) of an exactly symmetric FP64 matrix: the same products
+// as MatFull on it, bit for bit, from 448 instead of 864 stored bytes per node.  XCD: the
+// matvec's launch order (see xcd_block); its dot partials keep the node-block order of the
+// plain launch either way (partial_block), so every sum is the one MatFull gives.
+#define CISM_SYM64_XCD true
+template <bool XCD>
+struct MatSym64 {
+    static constexpr bool xcd = XCD;
+    static constexpr bool node_partials = true;
+    const double *A;
+    __device__ inline void blocks(long long q, int m, long long qn, double &uu, double &uv, double &vu,
+                                  double &vv) const {
+        if (m >= CISM_DIAG) {
+            const double2 a = *reinterpret_cast<const double2 *>(A + s64idx(0, m - CISM_DIAG, q));
+            const double2 b = *reinterpret_cast<const double2 *>(A + s64idx(2, m - CISM_DIAG, q));
+            uu = a.x; uv = a.y; vu = b.x; vv = b.y;
+        } else {
+            const double2 a = *reinterpret_cast<const double2 *>(A + s64idx(0, CISM_DIAG - m, qn));
+            const double2 b = *reinterpret_cast<const double2 *>(A + s64idx(2, CISM_DIAG - m, qn));
+            uu = a.x; uv = b.x; vu = a.y; vv = b.y;
+        }
+    }
+    __device__ inline double entry(const GridDev &g, int b, int m, long long q) const {
+        if (m >= CISM_DIAG) return A[s64idx(b, m - CISM_DIAG, q)];
+        const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+        const long long qn = q + kA + (long long)g.nz * (iA + (long long)g.sx * jA);
+        const int bt = b == 1 ? 2 : (b == 2 ? 1 : b);
+        return A[s64idx(bt, CISM_DIAG - m, qn)];
     }
 };
 
@@ -306,7 +339,8 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const M A
     double acc[NDOT];
 #pragma unroll
     for (int c = 0; c < NDOT; ++c) acc[c] = 0.0;
-    const long long q = (M::xcd ? xcd_block() : (long long)blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long lb = M::xcd ? xcd_block() : (long long)blockIdx.x;
+    const long long q = lb * blockDim.x + threadIdx.x;
     if (q < g.nn) {
         const int k = (int)(q % g.nz);
         const long long col = q / g.nz;
@@ -326,7 +360,8 @@ __global__ __launch_bounds__(CISM_BLOCK) void k_matvec_dots(GridDev g, const M A
             }
         }
     }
-    block_reduce_store<NDOT>(acc, partials);
+    // node_partials: the partials of node block lb sit where the plain launch puts them
+    block_reduce_store<NDOT>(acc, partials, M::node_partials ? lb : -1);
 }
 
 // r = b - A x on the full grid (A x = 0 off owned active rows), s0 = (r, r) over
@@ -761,7 +796,10 @@ int launch_a32_halo(cism_ctx *c) {
 }
 
 int launch_halo_vec(cism_ctx *c, double *v) { return launch_halo_t<double, false>(c, v); }
-int launch_halo_mat(cism_ctx *c, double *A) { return launch_halo_t<double, true>(c, A); }
+int launch_halo_mat(cism_ctx *c, double *A) {
+    if (A == c->A) c->a64s_valid = 0;
+    return launch_halo_t<double, true>(c, A);
+}
 
 // ---------------------------------------------------------------------------
 // vector updates (elementwise over the 2*nn interleaved entries)
@@ -1019,7 +1057,10 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
                   double tol, int check, double *err_out, int *niters_out, int x_zero = 0) {
     const GridDev &g = c->g;
     const int nb = red_blocks_for(g.nn), nbm = nblocks_for(g.nn), nb2 = nblocks_for(2 * g.nn);
-    const int nbv = M::xcd ? xcd_blocks_for(g.nn) : nbm;   // matvec grid (partials per block)
+    const int nbv = M::xcd ? xcd_blocks_for(g.nn) : nbm;   // matvec grid
+    // partials of a matvec: one per launched block, or per node block (node_partials; the
+    // padding blocks of an XCD grid, past the nodes, are left out of the sums)
+    const int nbr = M::xcd && !M::node_partials ? nbv : nbm;
     const long long np = g.npad;
     // an inner solve (check = 1) whose residual becomes exactly zero has converged; the
     // reference recurrences would divide 0 by 0 there
@@ -1089,7 +1130,7 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
             CISM_TRY((launch_halo_t<T, false>(c, v.d)));
             hipLaunchKernelGGL((k_matvec_dots<T, 1, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream, g,
                                A, c->av, v.d, v.q, (const T *)nullptr, c->partials);
-            CISM_TRY(launch_reduce(c, nbv, 1, RED_STD_ETA2));
+            CISM_TRY(launch_reduce(c, nbr, 1, RED_STD_ETA2));
             hipLaunchKernelGGL(k_std_xr<T>, dim3(nb2), dim3(CISM_BLOCK), 0, c->stream, 2 * g.nn, v.d,
                                v.q, v.x, v.r, c->sc);
             if (n % 5 == 0) {
@@ -1108,14 +1149,14 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
         CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
         hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream, g, A,
                            c->av, v.z, v.Az, v.r, c->partials);
-        CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_FIRST));
+        CISM_TRY(launch_reduce(c, nbr, 2, RED_CG_FIRST));
         // with the SIA preconditioner the update also forms the next z
         CISM_TRY(launch_cg_update<T>(c, precond, v, 1));
         for (int n = 2; n <= maxiters && !done; ++n) {
             if (precond < 2) CISM_TRY((launch_precond<T, M>(c, precond, A, v.F, v.r, v.z)));
             hipLaunchKernelGGL((k_matvec_dots<T, 2, M>), dim3(nbv), dim3(CISM_BLOCK), 0, c->stream,
                                g, A, c->av, v.z, v.Az, v.r, c->partials);
-            CISM_TRY(launch_reduce(c, nbv, 2, RED_CG_ITER));
+            CISM_TRY(launch_reduce(c, nbr, 2, RED_CG_ITER));
             CISM_TRY(launch_cg_update<T>(c, precond, v, 0, check == 1 && n % 5 == 0));
             if (n % 5 == 0) CISM_TRY(check_conv(n, done));
         }
@@ -1130,11 +1171,26 @@ static int run_cg(cism_ctx *c, const M A, CGVecs<T> v, int chrongear, int precon
     return CISM_OK;
 }
 
-// Solves A x = b for c->x (initial guess in place) with the FP64 matrix c->A.
+// Solves A x = b for c->x (initial guess in place) with the FP64 matrix c->A, read from its
+// symmetric copy when the last symmetry pass left it exactly symmetric (the same iterates)
 int solver_pcg64(cism_ctx *c, int chrongear, int precond, int maxiters, double tol, double *err,
                  int *niters) {
     CGVecs<double> v = {c->x, c->b, c->r, c->d, c->z, c->q, c->Az, c->F64, nullptr};
+    if (c->a64s_valid && c->sym64_on)
+        return run_cg<double>(c, MatSym64<CISM_SYM64_XCD>{c->A64s}, v, chrongear, precond, maxiters, tol, 0, err,
+                              niters);
     return run_cg<double>(c, MatFull<double>{c->A}, v, chrongear, precond, maxiters, tol, 0, err, niters);
+}
+
+int ensure_a64s(cism_ctx *c) {
+    if (c->A64s) return CISM_OK;
+    hipError_t e = hipMalloc(&c->A64s, sizeof(double) * 4 * CISM_SYM_SLOTS * (size_t)c->g.npad);
+    if (e != hipSuccess) {   // not an error of the solve: the full storage serves
+        c->A64s = nullptr;
+        (void)hipGetLastError();
+        return CISM_ERR_NOMEM;
+    }
+    return CISM_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1587,14 +1643,25 @@ int anderson_step(cism_ctx *c, int m, double *fnorm) {
 // measurement helpers
 // ---------------------------------------------------------------------------
 // one launch of the solver's matvec exactly as the Chron-Gear iteration issues it
-int launch_cg_matvec(cism_ctx *c, int fp32) {
+// which: 1 the FP32 symmetric matvec; 0 the FP64 one the FP64 solver runs (symmetric
+// storage when valid); 5 the FP64 one on the full storage; 6 the FP64 symmetric one in the
+// other launch order
+int launch_cg_matvec(cism_ctx *c, int which) {
     const GridDev &g = c->g;
     const int nb = nblocks_for(g.nn);
-    if (fp32)
+    if (which == 1)
         hipLaunchKernelGGL((k_matvec_dots<float, 2, MatSym32>), dim3(xcd_blocks_for(g.nn)), dim3(CISM_BLOCK), 0,
                            c->stream, g,
                            MatSym32{c->A32}, c->av, c->z32, c->Az32, c->r32, c->partials);
-    else
+    else if ((which == 0 || which == 6) && c->a64s_valid) {
+        if ((which == 0) == CISM_SYM64_XCD)
+            hipLaunchKernelGGL((k_matvec_dots<double, 2, MatSym64<true>>), dim3(xcd_blocks_for(g.nn)),
+                               dim3(CISM_BLOCK), 0, c->stream, g, MatSym64<true>{c->A64s}, c->av, c->z, c->Az,
+                               c->r, c->partials);
+        else
+            hipLaunchKernelGGL((k_matvec_dots<double, 2, MatSym64<false>>), dim3(nb), dim3(CISM_BLOCK), 0,
+                               c->stream, g, MatSym64<false>{c->A64s}, c->av, c->z, c->Az, c->r, c->partials);
+    } else
         hipLaunchKernelGGL((k_matvec_dots<double, 2, MatFull<double>>), dim3(nb), dim3(CISM_BLOCK), 0,
                            c->stream, g, MatFull<double>{c->A}, c->av, c->z, c->Az, c->r, c->partials);
     HIP_TRY(hipGetLastError());

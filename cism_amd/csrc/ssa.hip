@@ -269,7 +269,8 @@ int assemble2d_device(cism_ctx *c, int whichefvs, double efvs_constant, int add_
     hipLaunchKernelGGL(k_element_2d, dim3(nblocks_for(ncell)), dim3(256), 0, c->stream, g, c->ac, c->xV, c->yV,
                        c->x, c->sthck, c->ssa_ff2, whichefvs, efvs_constant, c->ssa_Ke, c->ssa_efvs2, c->sc);
     HIP_TRY(hipGetLastError());
-    c->a_zero_ok = 0;   // the 3D gather's record of zero rows does not cover this matrix
+    c->a_zero_ok = 0;
+    c->a64s_valid = 0;   // the 3D gather's record of zero rows does not cover this matrix
     c->a64_valid = 1;
     hipLaunchKernelGGL(k_gather_2d, dim3(nblocks_for((long long)g.sx * g.sy)), dim3(256), 0, c->stream, g,
                        c->ac, c->av, c->ssa_Ke, c->beta, add_basal, dx * dy / CISM_VOL0, c->A);
@@ -836,6 +837,7 @@ int assemble_depth_device(cism_ctx *c, int approx, int whichefvs, double efvs_co
         HIP_TRY(hipGetLastError());
     }
     c->a_zero_ok = 0;
+    c->a64s_valid = 0;
     c->a64_valid = 1;
     hipLaunchKernelGGL(k_gather_2d, dim3(nblocks_for(ncol)), dim3(256), 0, c->stream, g, c->ac, c->av, c->ssa_Ke,
                        approx == 4 ? c->dep_beff : c->beta, add_basal, dx * dy / CISM_VOL0, c->A);

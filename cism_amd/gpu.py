@@ -32,7 +32,7 @@ EXPORTED = [
     "cism_velo_higher_solve_3d", "cism_velo_higher_solve_2d", "cism_velo_higher_solve_depth", "cism_set_reference_velocity", "cism_time_kernel",
     "cism_coarse_build", "cism_coarse_level", "cism_coarse_correction",
     "cism_load_vector_lateral_bc",
-    "cism_set_precond_scale", "cism_column_scale_target",
+    "cism_set_precond_scale", "cism_column_scale_target", "cism_set_fp64_symmetric",
     "cism_compute_internal_stress", "cism_compute_basal_friction_heatflx",
     "cism_compute_basal_traction", "cism_velo_diagnostics", "cism_glissade_velo_inputs",
     "cism_glissade_grounded_fraction", "cism_calcbeta", "cism_set_basal_physics",
@@ -153,6 +153,7 @@ def lib():
         L.cism_coarse_level.argtypes = [VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), D]
         L.cism_coarse_correction.argtypes = [VP, D, D, D, D]
         L.cism_set_precond_scale.argtypes = [VP, D, D]
+        L.cism_set_fp64_symmetric.argtypes = [VP, C.c_int]
         L.cism_compute_internal_stress.argtypes = [VP, D, I32, D, D, D, D, D, D, C.c_int, C.c_double,
                                                    C.c_int, D, D, D, D, D, D, D, D]
         L.cism_compute_basal_friction_heatflx.argtypes = [VP, I32, I32, D, D, D, D, D, C.c_int, D]
@@ -414,6 +415,11 @@ class Context:
         return dict(tau_xz=tau[0], tau_yz=tau[1], tau_xx=tau[2], tau_yy=tau[3], tau_xy=tau[4],
                     tau_eff=tau[5], bfricflx=bf, btractx=bx, btracty=by)
 
+    def set_fp64_symmetric(self, enable):
+        """1 (default): the FP64 solver reads the exactly symmetric FP64 matrix from its
+        symmetric copy (the same iterates); 0: always from the full storage."""
+        _check(lib().cism_set_fp64_symmetric(self.h, int(enable)))
+
     def set_precond_scale(self, su, sv):
         """Per-column scale of the mixed-precision SIA preconditioner (vertex fields); None
         clears it."""
@@ -432,8 +438,10 @@ class Context:
         return su, sv
 
     def time_kernel(self, which, reps=20):
-        """Mean ms per launch (HIP events on the context stream): 0 FP64 matvec,
-        1 FP32 matvec, 2 assembly, 3 1 GiB copy (counter calibration)."""
+        """Mean ms per launch (HIP events on the context stream): 0 FP64 matvec as the
+        FP64 solver runs it, 1 FP32 matvec, 2 assembly, 3 1 GiB copy (counter
+        calibration), 4 FP32 fused update, 5 FP64 matvec on the full storage, 6 FP64
+        symmetric matvec in the other launch order."""
         ms = C.c_double()
         _check(lib().cism_time_kernel(self.h, which, reps, C.byref(ms)))
         return ms.value

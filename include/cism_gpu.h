@@ -290,15 +290,21 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
 /* Kernel timing on the context's stream with HIP events, for roofline reports
  * (no reference counterpart).  Each times `reps` launches on the matrix and
  * vectors left by the last solve and returns the mean duration in ms.
- *   which = 0 FP64 matvec (with the fused Chron-Gear dot products; not after a mixed
- *             solve with skip_final_residual, which keeps no FP64 matrix: assemble with
- *             which = 2 first)
+ *   which = 0 FP64 matvec as the FP64 solver runs it (with the fused Chron-Gear dot
+ *             products): on the symmetric copy that a symmetry pass of the current
+ *             matrix leaves when no pair stays unrepaired (the pass runs first, once,
+ *             if the copy is not current), else on the full storage.  Not after a
+ *             mixed solve with skip_final_residual, which keeps no FP64 matrix:
+ *             assemble with which = 2 first
  *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
  *           2 assembly (element kernel + node gather) of the current velocity
  *           3 plain copy of 1 GiB (k_copy_calib: 1 GiB read + 1 GiB written), the
  *             calibration of hardware byte counters
  *           4 FP32 fused Chron-Gear update with the column preconditioner (needs a
- *             mixed solve first; it changes that solve's leftover vectors) */
+ *             mixed solve first; it changes that solve's leftover vectors)
+ *           5 FP64 matvec on the full storage (all 27 slots)
+ *           6 FP64 matvec on the symmetric copy in the other launch order (plain
+ *             instead of XCD-aware); an error when there is no exactly symmetric copy */
 int  cism_time_kernel(cism_ctx *ctx, int which, int reps, double *ms_per_launch);
 
 /* Diagnostics of the aggregation coarse correction of the mixed-precision inner solves
@@ -491,6 +497,15 @@ int  cism_velo_diagnostics(cism_ctx *ctx, int whichapprox, int whichefvs, double
                            int whichassemble_bfric, double *tau_xz, double *tau_yz,
                            double *tau_xx, double *tau_yy, double *tau_xy, double *tau_eff,
                            double *bfricflx, double *btractx, double *btracty);
+
+/* Storage of the FP64 matrix in the FP64 solver (no counterpart in the reference).
+ * enable = 1 (the default): each Picard iteration's symmetry pass also writes the
+ * matrix in symmetric storage (14 of 27 slots, a coupling before the diagonal read as
+ * the transpose of its neighbour's), and the FP64 PCG and its residuals read that copy
+ * whenever the pass left no coefficient pair unrepaired; the products, and so every
+ * iterate, are the same bit for bit, from about half the matrix bytes.  enable = 0:
+ * always the full storage. */
+int  cism_set_fp64_symmetric(cism_ctx *ctx, int enable);
 
 /* Per-column preconditioner scale of the mixed-precision solves (no counterpart in
  * the reference): the SIA preconditioner becomes z = s M^-1 r with one positive
