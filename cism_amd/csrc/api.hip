@@ -1544,7 +1544,7 @@ extern "C" int cism_time_kernel(cism_ctx *c, int which, int reps, double *ms_per
         cism_set_error("no FP64 matrix: the last mixed-precision solve (skip_final_residual) kept none");
         return CISM_ERR_ARG;
     }
-    if ((which == 0 || which == 6) && !c->a64s_valid) {
+    if (((which == 0 && c->sym64_on) || which == 6) && !c->a64s_valid) {
         // the FP64 solver's symmetric copy, as its Picard iteration forms it: a symmetry pass
         if (ensure_a64s(c) == CISM_OK) {
             CISM_TRY(symmetry_device(c, nullptr, c->A64s));

@@ -1653,7 +1653,7 @@ int launch_cg_matvec(cism_ctx *c, int which) {
         hipLaunchKernelGGL((k_matvec_dots<float, 2, MatSym32>), dim3(xcd_blocks_for(g.nn)), dim3(CISM_BLOCK), 0,
                            c->stream, g,
                            MatSym32{c->A32}, c->av, c->z32, c->Az32, c->r32, c->partials);
-    else if ((which == 0 || which == 6) && c->a64s_valid) {
+    else if (((which == 0 && c->sym64_on) || which == 6) && c->a64s_valid) {
         if ((which == 0) == CISM_SYM64_XCD)
             hipLaunchKernelGGL((k_matvec_dots<double, 2, MatSym64<true>>), dim3(xcd_blocks_for(g.nn)),
                                dim3(CISM_BLOCK), 0, c->stream, g, MatSym64<true>{c->A64s}, c->av, c->z, c->Az,

@@ -293,7 +293,8 @@ int  cism_set_reference_velocity(cism_ctx *ctx, const double *uref, const double
  *   which = 0 FP64 matvec as the FP64 solver runs it (with the fused Chron-Gear dot
  *             products): on the symmetric copy that a symmetry pass of the current
  *             matrix leaves when no pair stays unrepaired (the pass runs first, once,
- *             if the copy is not current), else on the full storage.  Not after a
+ *             if the copy is not current), else, or after
+ *             cism_set_fp64_symmetric(ctx, 0), on the full storage.  Not after a
  *             mixed solve with skip_final_residual, which keeps no FP64 matrix:
  *             assemble with which = 2 first
  *           1 FP32 matvec (same kernel, FP32 matrix; needs a mixed solve first)
