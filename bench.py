@@ -227,13 +227,20 @@ def main(argv=None):
     bp = G.BasalPhysics(p.whichbabc) if p.whichbabc != 6 else None
     bext = p.extra.get("beta_external")
 
+    # the model's velocity arrays, kept across solves as a Fortran caller keeps its own
+    vel_out = tuple(np.zeros((p.nz, p.nx - 1, p.ny - 1), order="F") for _ in range(2))
+
     def solve(guess, stop=STOP_REL_ERR, precision=1):
         """one solve from the geometry (cism_glissade_velo_higher_solve): the cell fields go
         in, every input of the Picard loop is computed on the device, the velocity comes out.
         seconds = device time from resident inputs to the solution on the device (inputs,
-        loads, learned guess, Picard loop); seconds_wall adds the host transfers."""
+        loads, learned guess, Picard loop); seconds_wall adds the host transfers (with a
+        learned guess the solution lands in the persistent velocity arrays; the zero guess
+        uploads p's zero velocity)."""
         kw = dict(precision=1, adaptive_linear=1, inner_rtol=a.inner_rtol, anderson_m=a.anderson,
                   coarse=a.coarse, skip_final_residual=a.skip_final) if precision == 1 else dict(precision=0)
+        if guess != 0:
+            kw["out"] = vel_out
         t0 = time.perf_counter()
         r = ctx.glissade_velo_higher_solve(p, bp=bp, beta_external=bext, initial_guess=guess,
                                            stop_rel_err=stop, maxiter_nonlinear=cap, efvs=False, **kw)

@@ -529,18 +529,32 @@ class Context:
 
     def glissade_velo_higher_solve(self, p, bp=None, beta_external=None, effecpress_stag=None,
                                    initial_guess=0, uvel=None, vvel=None, lateral_bc=None,
-                                   whichground=0, max_slope=1.0, eus=0.0, efvs=True, **kw):
+                                   whichground=0, max_slope=1.0, eus=0.0, efvs=True, out=None, **kw):
         """glissade_velo_higher_solve from the geometry of p (thck, usrf, topg, flwa) with every
         input computed on the device; bp (BasalPhysics) is required unless p is no-slip.
         initial_guess: 0 uvel/vvel (zero by default), 1 the learned guess, 2 the context's
-        last velocity.  Options of the Picard loop as in velo_higher_solve_3d."""
+        last velocity.  Options of the Picard loop as in velo_higher_solve_3d.
+        out: (uvel, vvel), F-ordered float64 arrays of shape (nz, nx-1, ny-1) that the solve
+        uses in place as its in/out velocity (read only with initial_guess 0), as a Fortran
+        caller passes its model arrays; without it uvel/vvel (default p.uvel/p.vvel) are
+        copied for initial_guess 0, and new arrays receive the solution otherwise."""
         o = self._opts(p, **kw)
         su = VeloSetup(dx=p.dx, dy=p.dy, thklim=p.thklim, eus=eus, max_slope=max_slope,
                        whichground=whichground, whichassemble_taud=p.whichassemble_taud,
                        lateral_bc=int(p.ocean_mask.any()) if lateral_bc is None else int(lateral_bc),
                        initial_guess=initial_guess)
-        u = (p.uvel if uvel is None else uvel).copy(order="F")
-        v = (p.vvel if vvel is None else vvel).copy(order="F")
+        shape = (p.nz, p.nx - 1, p.ny - 1)
+        if out is not None:
+            u, v = out
+            for a in (u, v):
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == shape
+                        and a.flags.f_contiguous and a.flags.writeable):
+                    raise ValueError(f"out: writeable F-ordered float64 arrays of shape {shape}")
+        elif initial_guess == 0:
+            u = (p.uvel if uvel is None else uvel).copy(order="F")
+            v = (p.vvel if vvel is None else vvel).copy(order="F")
+        else:   # the input velocity is not read
+            u, v = np.empty(shape, order="F"), np.empty(shape, order="F")
         ef = np.zeros((p.nz - 1, p.nx, p.ny), order="F") if efvs else None
         f = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)   # noqa: E731
         bext, N = f(beta_external), f(effecpress_stag)

@@ -124,3 +124,30 @@ def test_learned_guess_on_an_outflow_grid():
     assert rl["guess_rejected"] == 0
     for r in (rz, rl):
         assert r["converged"] and 0 <= r["final_rel_err"] <= 1e-6
+
+
+def test_persistent_velocity_arrays(grf256):
+    """out=(uvel, vvel): the solve writes the caller's arrays in place (a Fortran caller's
+    model arrays) with the same solution as into fresh arrays; a zero-guess solve reads
+    them as its start; arrays of the wrong shape are refused before any device work"""
+    p, params, meta = grf256
+    shape = (p.nz, p.nx - 1, p.ny - 1)
+    with G.Context.for_problem(p) as ctx:
+        ctx.learned_guess_load(params, meta["nz"], 1 if meta.get("target") == "log1p" else 0,
+                               meta["uscale"])
+        kw = dict(bp=_bp(p), beta_external=p.extra["beta_external"], precision=1, adaptive_linear=1,
+                  inner_rtol=1e-2, anderson_m=4, coarse=1, skip_final_residual=1, efvs=False)
+        fresh = ctx.glissade_velo_higher_solve(p, initial_guess=1, **kw)
+        U, V = np.full(shape, 7.0, order="F"), np.full(shape, -7.0, order="F")
+        kept = ctx.glissade_velo_higher_solve(p, initial_guess=1, out=(U, V), **kw)
+        assert kept["uvel"] is U and kept["vvel"] is V
+        assert np.array_equal(U, fresh["uvel"]) and np.array_equal(V, fresh["vvel"])
+        assert kept["picard_iters"] == fresh["picard_iters"]
+        # initial_guess 0 reads the arrays: starting from the solution, the first Picard
+        # residual is already below the reference's tolerance
+        again = ctx.glissade_velo_higher_solve(p, initial_guess=0, out=(U, V), **kw)
+        assert again["picard_iters"] <= 2
+        with pytest.raises(ValueError):
+            ctx.glissade_velo_higher_solve(p, initial_guess=1, out=(U[:-1], V), **kw)
+        with pytest.raises(ValueError):
+            ctx.glissade_velo_higher_solve(p, initial_guess=1, out=(np.zeros(shape), V), **kw)
