@@ -121,18 +121,20 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvDims d, const float *__re
                 x = x < 0 ? x + d.W : (x >= d.W ? x - d.W : x);
                 src = pb * HW + (long long)y * d.W + x;
             }
+            // 16 BF16 values packed two per word: two 16-byte LDS stores
             const float4 *v = reinterpret_cast<const float4 *>(in + src * d.Ci + ci);
+            unsigned w[8];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float4 f = v[q];
-                Bs[lr][lk + 4 * q + 0] = f2bf(f.x);
-                Bs[lr][lk + 4 * q + 1] = f2bf(f.y);
-                Bs[lr][lk + 4 * q + 2] = f2bf(f.z);
-                Bs[lr][lk + 4 * q + 3] = f2bf(f.w);
+                w[2 * q] = (unsigned)(unsigned short)f2bf(f.x) | ((unsigned)(unsigned short)f2bf(f.y) << 16);
+                w[2 * q + 1] = (unsigned)(unsigned short)f2bf(f.z) | ((unsigned)(unsigned short)f2bf(f.w) << 16);
             }
+            *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = make_uint4(w[0], w[1], w[2], w[3]);
+            *reinterpret_cast<uint4 *>(&Bs[lr][lk + 8]) = make_uint4(w[4], w[5], w[6], w[7]);
         } else {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) Bs[lr][lk + j] = 0;
+            *reinterpret_cast<uint4 *>(&Bs[lr][lk]) = make_uint4(0u, 0u, 0u, 0u);
+            *reinterpret_cast<uint4 *>(&Bs[lr][lk + 8]) = make_uint4(0u, 0u, 0u, 0u);
         }
         __syncthreads();
 #pragma unroll
