@@ -37,6 +37,7 @@
 
 #define CG_AGG 4
 #define CG_DENSE 32   // aggregates of the coarsest level; its dense inverse is (2 CG_DENSE)^2
+static_assert(2 * CG_DENSE <= 64, "k_cg_dense maps one lane column to each of at most 64 unknowns");
 
 __host__ __device__ inline int cg_ceil(int n, int d) { return (n + d - 1) / d; }
 
@@ -362,9 +363,14 @@ __global__ __launch_bounds__(256) void k_cg_dense(int ax, int ay, int periodic, 
         __syncthreads();
         if (t < n) M[k][t] = (t == k ? 1.0 : M[k][t]) / piv;
         __syncthreads();
-        for (int w = t; w < n * n; w += blockDim.x) {
-            const int i = w / n, j = w - i * n;
-            if (i != k) M[i][j] = (j == k ? 0.0 : M[i][j]) - colk[i] * M[k][j];
+        // lane t: column t & 63, rows t >> 6, (t >> 6) + 4, ... (n <= 64, 256 lanes)
+        {
+            const int j = t & 63;
+            if (j < n) {
+                const double mkj = M[k][j];
+                for (int i = t >> 6; i < n; i += 4)
+                    if (i != k) M[i][j] = (j == k ? 0.0 : M[i][j]) - colk[i] * mkj;
+            }
         }
         __syncthreads();
     }
