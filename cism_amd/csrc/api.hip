@@ -71,11 +71,16 @@ int ensure_stage(cism_ctx *c) {
 
 int ensure_ke(cism_ctx *c) {
     if (!c->Ke) {
-        hipError_t e = hipMalloc(&c->Ke, sizeof(double) * CISM_KE_ENTRIES * (size_t)c->g.nepad);
+        // the element tiles, then one tile of zeros that the gather reads in place of a
+        // missing element (no element kernel writes it)
+        const size_t tile = (size_t)CISM_KE_ENTRIES * 64;
+        hipError_t e = hipMalloc(&c->Ke, sizeof(double) * (CISM_KE_ENTRIES * (size_t)c->g.nepad + tile));
         if (e != hipSuccess) {
+            c->Ke = nullptr;
             cism_set_error("element scratch allocation failed: %s", hipGetErrorString(e));
             return CISM_ERR_NOMEM;
         }
+        HIP_TRY(hipMemsetAsync(c->Ke + CISM_KE_ENTRIES * (size_t)c->g.nepad, 0, sizeof(double) * tile, c->stream));
     }
     return CISM_OK;
 }

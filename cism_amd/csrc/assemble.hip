@@ -493,10 +493,12 @@ int basal_diagnostics_device(cism_ctx *c, int whichassemble_bfric, double *bfric
     return CISM_OK;
 }
 
-// A scratch column of zeros (the stride of keidx): the gather reads it in place of
-// an element that is not there, so every one of its loads is unconditional and
-// they can all be in flight together.  Adding +0.0 leaves every sum unchanged.
-static __device__ double ke_zero[CISM_KE_ENTRIES * 64];
+// The element scratch ends with one tile of zeros (ensure_ke): the gather reads its first
+// column in place of an element that is not there, so every one of its loads is
+// unconditional, and adding +0.0 leaves every sum unchanged.  Taking it from Ke itself
+// keeps every load of the gather on the restrict-qualified scratch, so the compiler may
+// move the next slot's loads above this slot's stores to A and As.
+__host__ __device__ inline long long ke_zero_offset(long long nepad) { return keidx(0, nepad); }
 
 // entry ent of the element whose scratch column starts at base (base = Ke + keidx(0, e))
 __device__ __forceinline__ double ke_at(const double *__restrict__ base, int ent) {
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(256) void k_gather(GridDev g, const int32_t *__rest
                 const int ke = k - 1 + dK;
                 const bool ok = act && ke >= 1 && ke <= nzm;
                 any = any || ok;
-                base[dJ][dI][dK] = ok ? Ke + keidx(0, (ke - 1) + (long long)nzm * cell) : ke_zero;
+                base[dJ][dI][dK] = Ke + (ok ? keidx(0, (ke - 1) + (long long)nzm * cell) : ke_zero_offset(g.nepad));
             }
         }
     const bool basal = add_basal && k == g.nz && av[col];
